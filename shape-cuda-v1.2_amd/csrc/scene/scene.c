@@ -1,0 +1,1057 @@
+/*
+ * scene.c -- build synthetic SHAPE scenes as ABI structs (see scene.h).
+ *
+ * What is generated and why it looks like the reference:
+ *  - Ellipsoid tessellation: rings of vertices at colatitude t*pi/T with
+ *    max(1, round(2T sin theta)) vertices each, zipped into triangles ring by
+ *    ring; this reproduces the vertex/facet counts of read_mod.c:2179-2380
+ *    (T = 30 gives 1148 vertices, 2292 facets), which is what the
+ *    ellipsoid-converted-to-vertex models of the reference contain.
+ *  - Icosphere: 4-way subdivision of an icosahedron, V = 10*4^k + 2.
+ *  - Vertex components hold base point a = 0, direction u (unit), radius r
+ *    along u (vertex_t, shape2.h:723), so realize_mod's VERTEX branch
+ *    x = s*(u*r + a) (realize_mod.c:475-479) yields the deformed shape.
+ *  - vertex->af lists attached facets in ascending facet order, as
+ *    setupvertices (read_mod.c:2504-2590) builds them.
+ *  - POS blocks are shared by every frame for pos_scope GLOBAL and separate
+ *    per frame for LOCAL (read_dat.c:1239-1330); km_per_pixel = width/(2n).
+ */
+#include "scene.h"
+
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+
+/* ------------------------------------------------------------------ */
+/* NR-style allocators: row pointers into one zeroed block             */
+/* ------------------------------------------------------------------ */
+double **shp_matrix(int nrl, int nrh, int ncl, int nch)
+{
+  size_t nr = (size_t)(nrh - nrl + 1), nc = (size_t)(nch - ncl + 1);
+  double **m = (double **)malloc(nr * sizeof(double *));
+  double *blk = (double *)calloc(nr * nc, sizeof(double));
+  if (!m || !blk) abort();
+  for (size_t i = 0; i < nr; i++) m[i] = blk + i * nc - ncl;
+  return m - nrl;
+}
+
+int **shp_imatrix(int nrl, int nrh, int ncl, int nch)
+{
+  size_t nr = (size_t)(nrh - nrl + 1), nc = (size_t)(nch - ncl + 1);
+  int **m = (int **)malloc(nr * sizeof(int *));
+  int *blk = (int *)calloc(nr * nc, sizeof(int));
+  if (!m || !blk) abort();
+  for (size_t i = 0; i < nr; i++) m[i] = blk + i * nc - ncl;
+  return m - nrl;
+}
+
+double *shp_vector(int nl, int nh)
+{
+  double *v = (double *)calloc((size_t)(nh - nl + 1), sizeof(double));
+  if (!v) abort();
+  return v - nl;
+}
+
+void shp_free_matrix(double **m, int nrl, int ncl)
+{
+  if (!m) return;
+  free(m[nrl] + ncl);
+  free(m + nrl);
+}
+
+void shp_free_imatrix(int **m, int nrl, int ncl)
+{
+  if (!m) return;
+  free(m[nrl] + ncl);
+  free(m + nrl);
+}
+
+void shp_free_vector(double *v, int nl)
+{
+  if (v) free(v + nl);
+}
+
+/* ------------------------------------------------------------------ */
+/* deterministic random numbers (splitmix64 + Box-Muller)              */
+/* ------------------------------------------------------------------ */
+typedef struct { uint64_t s; int have; double spare; } rng_t;
+
+static uint64_t rng_next(rng_t *r)
+{
+  uint64_t z = (r->s += 0x9E3779B97F4A7C15ULL);
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+  return z ^ (z >> 31);
+}
+static double rng_uniform(rng_t *r) { return (rng_next(r) >> 11) * (1.0 / 9007199254740992.0); }
+static double rng_range(rng_t *r, double a, double b) { return a + (b - a) * rng_uniform(r); }
+static int rng_int(rng_t *r, int a, int b) { return a + (int)floor(rng_uniform(r) * (b - a + 1)); }
+static double rng_normal(rng_t *r)
+{
+  if (r->have) { r->have = 0; return r->spare; }
+  double u1 = rng_uniform(r), u2 = rng_uniform(r);
+  if (u1 < 1e-300) u1 = 1e-300;
+  double rad = sqrt(-2.0 * log(u1));
+  r->spare = rad * sin(TWOPI * u2);
+  r->have = 1;
+  return rad * cos(TWOPI * u2);
+}
+static rng_t rng_make(uint64_t seed) { rng_t r; r.s = seed * 0x2545F4914F6CDD1DULL + 12345; r.have = 0; r.spare = 0; return r; }
+
+/* ------------------------------------------------------------------ */
+/* mesh builders                                                      */
+/* ------------------------------------------------------------------ */
+typedef struct { int nv, nf; double *u; int *fv; } mesh_t; /* u: 3*nv unit dirs, fv: 3*nf */
+
+static int round_half_away(double x) { return x < 0 ? (int)(x - 0.5) : (int)(x + 0.5); }
+
+static void mesh_push_tri(mesh_t *m, int *cap, int a, int b, int c)
+{
+  if (m->nf == *cap) { *cap = *cap ? 2 * *cap : 1024; m->fv = (int *)realloc(m->fv, sizeof(int) * 3 * (size_t)*cap); }
+  m->fv[3 * m->nf] = a; m->fv[3 * m->nf + 1] = b; m->fv[3 * m->nf + 2] = c;
+  m->nf++;
+}
+
+/* Ring-zipper tessellation of the unit sphere with T colatitude steps. */
+static mesh_t mesh_ellipsoid_tess(int T)
+{
+  mesh_t m = {0, 0, NULL, NULL};
+  int *np = (int *)malloc(sizeof(int) * (T + 1));
+  int *first = (int *)malloc(sizeof(int) * (T + 1));
+  for (int t = 0; t <= T; t++) {
+    int k = round_half_away(2 * T * sin(t * PIE / T));
+    np[t] = k < 1 ? 1 : k;
+    first[t] = m.nv;
+    m.nv += np[t];
+  }
+  m.u = (double *)malloc(sizeof(double) * 3 * m.nv);
+  for (int t = 0, v = 0; t <= T; t++) {
+    double th = t * PIE / T;
+    for (int p = 0; p < np[t]; p++, v++) {
+      double ph = p * TWOPI / np[t];
+      m.u[3 * v] = sin(th) * cos(ph);
+      m.u[3 * v + 1] = sin(th) * sin(ph);
+      m.u[3 * v + 2] = cos(th);
+    }
+  }
+  int cap = 0;
+  for (int t = 0; t < T; t++) {
+    int na = np[t], nb = np[t + 1], a = 0, b = 0;
+    /* walk both rings eastward; the step whose next vertex has the smaller
+       azimuth goes first (top ring first on ties); poles only fan. */
+    while (a < na || b < nb) {
+      double pa = (double)(a + 1) / na, pb = (double)(b + 1) / nb;
+      int step_top;
+      if (na == 1) step_top = 0;
+      else if (nb == 1) step_top = 1;
+      else if (a >= na) step_top = 0;
+      else if (b >= nb) step_top = 1;
+      else step_top = (pa <= pb);
+      int va = first[t] + (a % na), vb = first[t + 1] + (b % nb);
+      if (step_top) {
+        a++;
+        mesh_push_tri(&m, &cap, va, vb, first[t] + (a % na));
+      } else {
+        b++;
+        mesh_push_tri(&m, &cap, va, vb, first[t + 1] + (b % nb));
+      }
+      if (na == 1 && b >= nb) a = na;
+      if (nb == 1 && a >= na) b = nb;
+    }
+  }
+  free(np);
+  free(first);
+  return m;
+}
+
+/* icosphere: subdivide an icosahedron `level` times, projecting to the sphere */
+typedef struct { uint64_t key; int v; } edge_ent;
+
+static int edge_mid(edge_ent *tab, size_t tabsz, mesh_t *m, int *vcap, int a, int b)
+{
+  int lo = a < b ? a : b, hi = a < b ? b : a;
+  uint64_t key = ((uint64_t)lo << 32) | (uint64_t)hi;
+  size_t h = (size_t)((key * 0x9E3779B97F4A7C15ULL) >> 20) & (tabsz - 1);
+  while (tab[h].key != UINT64_MAX) {
+    if (tab[h].key == key) return tab[h].v;
+    h = (h + 1) & (tabsz - 1);
+  }
+  if (m->nv == *vcap) { *vcap *= 2; m->u = (double *)realloc(m->u, sizeof(double) * 3 * (size_t)*vcap); }
+  double x = m->u[3 * a] + m->u[3 * b], y = m->u[3 * a + 1] + m->u[3 * b + 1], z = m->u[3 * a + 2] + m->u[3 * b + 2];
+  double nn = sqrt(x * x + y * y + z * z);
+  m->u[3 * m->nv] = x / nn; m->u[3 * m->nv + 1] = y / nn; m->u[3 * m->nv + 2] = z / nn;
+  tab[h].key = key;
+  tab[h].v = m->nv;
+  return m->nv++;
+}
+
+static mesh_t mesh_icosphere(int level)
+{
+  const double p = (1.0 + sqrt(5.0)) / 2.0;
+  static const int F0[20][3] = {{0, 11, 5}, {0, 5, 1}, {0, 1, 7}, {0, 7, 10}, {0, 10, 11},
+                                {1, 5, 9}, {5, 11, 4}, {11, 10, 2}, {10, 7, 6}, {7, 1, 8},
+                                {3, 9, 4}, {3, 4, 2}, {3, 2, 6}, {3, 6, 8}, {3, 8, 9},
+                                {4, 9, 5}, {2, 4, 11}, {6, 2, 10}, {8, 6, 7}, {9, 8, 1}};
+  double V0[12][3] = {{-1, p, 0}, {1, p, 0}, {-1, -p, 0}, {1, -p, 0}, {0, -1, p}, {0, 1, p},
+                      {0, -1, -p}, {0, 1, -p}, {p, 0, -1}, {p, 0, 1}, {-p, 0, -1}, {-p, 0, 1}};
+  mesh_t m = {0, 0, NULL, NULL};
+  int vcap = 12;
+  m.u = (double *)malloc(sizeof(double) * 3 * vcap);
+  for (int i = 0; i < 12; i++) {
+    double nn = sqrt(V0[i][0] * V0[i][0] + V0[i][1] * V0[i][1] + V0[i][2] * V0[i][2]);
+    for (int k = 0; k < 3; k++) m.u[3 * i + k] = V0[i][k] / nn;
+  }
+  m.nv = 12;
+  int cap = 0;
+  for (int f = 0; f < 20; f++) mesh_push_tri(&m, &cap, F0[f][0], F0[f][1], F0[f][2]);
+  for (int l = 0; l < level; l++) {
+    size_t tabsz = 1;
+    while (tabsz < (size_t)m.nf * 4) tabsz <<= 1;
+    edge_ent *tab = (edge_ent *)malloc(sizeof(edge_ent) * tabsz);
+    for (size_t i = 0; i < tabsz; i++) tab[i].key = UINT64_MAX;
+    mesh_t nm = {m.nv, 0, m.u, NULL};
+    int ncap = 0;
+    for (int f = 0; f < m.nf; f++) {
+      int a = m.fv[3 * f], b = m.fv[3 * f + 1], c = m.fv[3 * f + 2];
+      int ab = edge_mid(tab, tabsz, &nm, &vcap, a, b);
+      int bc = edge_mid(tab, tabsz, &nm, &vcap, b, c);
+      int ca = edge_mid(tab, tabsz, &nm, &vcap, c, a);
+      mesh_push_tri(&nm, &ncap, a, ab, ca);
+      mesh_push_tri(&nm, &ncap, b, bc, ab);
+      mesh_push_tri(&nm, &ncap, c, ca, bc);
+      mesh_push_tri(&nm, &ncap, ab, bc, ca);
+    }
+    free(tab);
+    free(m.fv);
+    m = nm;
+  }
+  return m;
+}
+
+/* orient every facet outward (normal points away from the origin) */
+static void mesh_orient(mesh_t *m)
+{
+  for (int f = 0; f < m->nf; f++) {
+    int *v = m->fv + 3 * f;
+    const double *p0 = m->u + 3 * v[0], *p1 = m->u + 3 * v[1], *p2 = m->u + 3 * v[2];
+    double a[3], b[3], n[3], c[3];
+    for (int k = 0; k < 3; k++) { a[k] = p1[k] - p0[k]; b[k] = p2[k] - p1[k]; c[k] = p0[k] + p1[k] + p2[k]; }
+    n[0] = a[1] * b[2] - a[2] * b[1];
+    n[1] = a[2] * b[0] - a[0] * b[2];
+    n[2] = a[0] * b[1] - a[1] * b[0];
+    if (n[0] * c[0] + n[1] * c[1] + n[2] * c[2] < 0) { int t = v[1]; v[1] = v[2]; v[2] = t; }
+  }
+}
+
+/* ------------------------------------------------------------------ */
+/* scene                                                              */
+/* ------------------------------------------------------------------ */
+struct shp_scene {
+  shp_spec spec;
+  struct par_t par;
+  struct mod_t mod;
+  struct dat_t dat;
+  int nparams;
+  double *params[32];
+  int ptype[32];
+  /* allocations to release */
+  void **allocs;
+  int nalloc, capalloc;
+};
+
+static void *track(shp_scene *sc, void *p)
+{
+  if (sc->nalloc == sc->capalloc) {
+    sc->capalloc = sc->capalloc ? 2 * sc->capalloc : 256;
+    sc->allocs = (void **)realloc(sc->allocs, sizeof(void *) * sc->capalloc);
+  }
+  sc->allocs[sc->nalloc++] = p;
+  return p;
+}
+static void *tcalloc(shp_scene *sc, size_t n, size_t sz) { void *p = calloc(n ? n : 1, sz); if (!p) abort(); return track(sc, p); }
+
+/* matrix whose block and row array are both tracked */
+static double **t_matrix(shp_scene *sc, int nrl, int nrh, int ncl, int nch)
+{
+  double **m = shp_matrix(nrl, nrh, ncl, nch);
+  track(sc, m[nrl] + ncl);
+  track(sc, m + nrl);
+  return m;
+}
+static int **t_imatrix(shp_scene *sc, int nrl, int nrh, int ncl, int nch)
+{
+  int **m = shp_imatrix(nrl, nrh, ncl, nch);
+  track(sc, m[nrl] + ncl);
+  track(sc, m + nrl);
+  return m;
+}
+static double *t_vector(shp_scene *sc, int nl, int nh)
+{
+  double *v = shp_vector(nl, nh);
+  track(sc, v + nl);
+  return v;
+}
+
+void shp_spec_defaults(shp_spec *s)
+{
+  memset(s, 0, sizeof(*s));
+  s->shape_kind = SHP_SHAPE_ELLIPSOID_TESS;
+  s->theta_steps = 30;
+  s->semi_axes[0] = 1.0; s->semi_axes[1] = 0.8; s->semi_axes[2] = 0.6;
+  s->rdev_sigma = 0.02;
+  s->seed_geom = 1001;
+  s->seed_frames = 1002;
+  s->spin_period_h = 6.0;
+  s->pole_lambda_deg = 30.0;
+  s->pole_beta_deg = 60.0;
+  s->phase0_deg = 0.0;
+  s->t0 = 2458000.5;
+  s->epoch_span_days = 0.0;
+  s->pos_n = 75;
+  s->km_per_pixel = 0.015;
+  s->pos_scope = GLOBAL;
+  s->pos_smooth = 1;
+  s->Ftx = 8560.0;
+  s->codemethod = LONG_MOD;
+  s->spb = 1;
+  s->stride = 1;
+  s->dopfftlen = 64;
+  s->R_min = s->R_max = 0.2;
+  s->C_min = s->C_max = 1.5;
+  s->one_set_per_frame = 1;
+  s->n_deldop = 1;
+  s->ndel_min = s->ndel_max = 100;
+  s->ndop_min = s->ndop_max = 80;
+  s->del_per_pixel = 0.5;
+  s->dop_per_pixel = 0.5;
+  s->n_doppler = 0;
+  s->nbin_min = 100; s->nbin_max = 400;
+  s->dop_per_bin_min = 0.25; s->dop_per_bin_max = 2.0;
+  s->n_lightcurves = 0;
+  s->lc_points = 100;
+  s->lc_ncalc = 40;
+  s->phase_min_deg = 5.0; s->phase_max_deg = 60.0;
+  s->kaas_wt = 0.1;
+  s->delcor_a0 = 0.0;
+  s->delcor_a1 = 0.0;
+  s->obs_lat_min_deg = 10.0;
+  s->obs_lat_max_deg = 10.0;
+}
+
+/* Euler-angle matrix with the reference's convention (util/euler.c). */
+static void euler_mat(double m[3][3], double phi, double theta, double psi)
+{
+  double ct = cos(theta), st = sin(theta), cp = cos(phi), sp = sin(phi), cs = cos(psi), ss = sin(psi);
+  m[0][0] = cs * cp - ct * sp * ss;
+  m[1][0] = -ss * cp - ct * sp * cs;
+  m[2][0] = st * sp;
+  m[0][1] = cs * sp + ct * cp * ss;
+  m[1][1] = -ss * sp + ct * cp * cs;
+  m[2][1] = -st * cp;
+  m[0][2] = ss * st;
+  m[1][2] = cs * st;
+  m[2][2] = ct;
+}
+
+/* Frame whose z axis points along dir (ecliptic), y toward ecliptic north. */
+static void frame_from_dir(double m[3][3], const double dir[3])
+{
+  double z[3] = {dir[0], dir[1], dir[2]}, y[3], x[3];
+  double nn = sqrt(z[0] * z[0] + z[1] * z[1] + z[2] * z[2]);
+  for (int k = 0; k < 3; k++) z[k] /= nn;
+  double up[3] = {0, 0, 1};
+  double d = up[0] * z[0] + up[1] * z[1] + up[2] * z[2];
+  if (fabs(d) > 0.999) { up[0] = 1; up[2] = 0; d = z[0]; }
+  for (int k = 0; k < 3; k++) y[k] = up[k] - d * z[k];
+  nn = sqrt(y[0] * y[0] + y[1] * y[1] + y[2] * y[2]);
+  for (int k = 0; k < 3; k++) y[k] /= nn;
+  x[0] = y[1] * z[2] - y[2] * z[1];
+  x[1] = y[2] * z[0] - y[0] * z[2];
+  x[2] = y[0] * z[1] - y[1] * z[0];
+  for (int k = 0; k < 3; k++) { m[0][k] = x[k]; m[1][k] = y[k]; m[2][k] = z[k]; }
+}
+
+static void los_dir(const shp_spec *sp, double frac, double dir[3])
+{
+  double lat = (sp->obs_lat_min_deg + (sp->obs_lat_max_deg - sp->obs_lat_min_deg) * frac) * D2R;
+  double lon = (20.0 + 40.0 * frac) * D2R;
+  dir[0] = cos(lat) * cos(lon);
+  dir[1] = cos(lat) * sin(lon);
+  dir[2] = sin(lat);
+}
+
+static void alloc_pos(shp_scene *sc, struct pos_t *pos, int n, double kmpp, int bistatic)
+{
+  memset(pos, 0, sizeof(*pos));
+  pos->n = n;
+  pos->km_per_pixel = kmpp;
+  pos->b = t_matrix(sc, -n, n, -n, n);
+  pos->cose = t_matrix(sc, -n, n, -n, n);
+  pos->cosi = t_matrix(sc, -n, n, -n, n);
+  pos->z = t_matrix(sc, -n, n, -n, n);
+  pos->body = t_imatrix(sc, -n, n, -n, n);
+  pos->comp = t_imatrix(sc, -n, n, -n, n);
+  pos->f = t_imatrix(sc, -n, n, -n, n);
+  pos->cosill = t_matrix(sc, -n, n, -n, n);
+  pos->zill = t_matrix(sc, -n, n, -n, n);
+  pos->bodyill = t_imatrix(sc, -n, n, -n, n);
+  pos->compill = t_imatrix(sc, -n, n, -n, n);
+  pos->fill = t_imatrix(sc, -n, n, -n, n);
+  pos->bistatic = bistatic;
+}
+
+/* Give a frame its POS: shared block (GLOBAL) or its own (LOCAL). */
+static void frame_pos(shp_scene *sc, struct pos_t *dst, int bistatic)
+{
+  if (sc->par.pos_scope == GLOBAL) {
+    *dst = sc->dat.pos;
+    dst->bistatic = bistatic;
+  } else {
+    alloc_pos(sc, dst, sc->dat.pos.n, sc->dat.pos.km_per_pixel, bistatic);
+  }
+}
+
+static void add_param(shp_scene *sc, double *p, int type)
+{
+  if (sc->nparams < 32) { sc->params[sc->nparams] = p; sc->ptype[sc->nparams] = type; sc->nparams++; }
+}
+
+static void build_model(shp_scene *sc)
+{
+  const shp_spec *sp = &sc->spec;
+  mesh_t m = sp->shape_kind == SHP_SHAPE_ICOSPHERE ? mesh_icosphere(sp->ico_level)
+                                                   : mesh_ellipsoid_tess(sp->theta_steps);
+  mesh_orient(&m);
+  rng_t rg = rng_make(sp->seed_geom);
+
+  struct mod_t *mod = &sc->mod;
+  strcpy(mod->name, "synthetic");
+  mod->shape.ncomp = 1;
+  mod->shape.comp = (struct comp_t *)tcalloc(sc, 1, sizeof(struct comp_t));
+  struct comp_t *c = &mod->shape.comp[0];
+  c->type = VERTEX;
+  for (int k = 0; k < 3; k++) { c->off[k].state = 'c'; c->rot[k].state = 'c'; }
+  struct vertices_t *vt = &c->desc.ver;
+  for (int k = 0; k < 3; k++) { vt->scalefactor[k].state = 'f'; vt->scalefactor[k].val = 1.0; }
+  vt->nv = m.nv;
+  vt->nf = m.nf;
+  vt->v = (struct vertex_t *)tcalloc(sc, m.nv, sizeof(struct vertex_t));
+  vt->f = (struct facet_t *)tcalloc(sc, m.nf, sizeof(struct facet_t));
+  double a2[3];
+  for (int k = 0; k < 3; k++) a2[k] = 1.0 / (sp->semi_axes[k] * sp->semi_axes[k]);
+  for (int i = 0; i < m.nv; i++) {
+    struct vertex_t *v = &vt->v[i];
+    double *u = m.u + 3 * i;
+    double r = 1.0 / sqrt(a2[0] * u[0] * u[0] + a2[1] * u[1] * u[1] + a2[2] * u[2] * u[2]);
+    r *= 1.0 + sp->rdev_sigma * rng_normal(&rg);
+    for (int k = 0; k < 3; k++) { v->u[k] = u[k]; v->a[k] = 0.0; }
+    v->r.state = 'f';
+    v->r.val = r;
+    v->act = 1;
+    v->v_mirror = i;
+  }
+  for (int f = 0; f < m.nf; f++) {
+    for (int k = 0; k < 3; k++) vt->f[f].v[k] = m.fv[3 * f + k];
+    vt->f[f].act = 1;
+  }
+  /* attached facets, ascending facet order */
+  for (int f = 0; f < m.nf; f++)
+    for (int k = 0; k < 3; k++) vt->v[m.fv[3 * f + k]].naf++;
+  for (int i = 0; i < m.nv; i++) {
+    vt->v[i].af = (int *)tcalloc(sc, vt->v[i].naf, sizeof(int));
+    vt->v[i].naf = 0;
+  }
+  for (int f = 0; f < m.nf; f++)
+    for (int k = 0; k < 3; k++) {
+      struct vertex_t *v = &vt->v[m.fv[3 * f + k]];
+      v->af[v->naf++] = f;
+    }
+  c->real = *vt; /* setupreal: the VERTEX realization shares the description's arrays */
+  free(m.u);
+  free(m.fv);
+
+  /* spin */
+  struct spin_t *s = &mod->spin;
+  s->pa = 1;
+  s->t0 = sp->t0;
+  s->angle[0].val = (sp->pole_lambda_deg + 90.0) * D2R;
+  s->angle[1].val = (90.0 - sp->pole_beta_deg) * D2R;
+  s->angle[2].val = sp->phase0_deg * D2R;
+  s->omega[2].val = TWOPI / (sp->spin_period_h / 24.0);
+  for (int k = 0; k < 3; k++) {
+    s->angle[k].state = 'f'; s->omega[k].state = k == 2 ? 'f' : 'c'; s->omegadot[k].state = 'c';
+    s->inertia[k].state = 'c'; s->inertia[k].val = 1.0;
+  }
+  s->lib_amp.state = s->lib_freq.state = s->lib_phase.state = 'c';
+
+  add_param(sc, &c->desc.ver.scalefactor[0].val, SHP_SIZEPAR);
+  add_param(sc, &c->desc.ver.scalefactor[1].val, SHP_SIZEPAR);
+  add_param(sc, &c->desc.ver.scalefactor[2].val, SHP_SIZEPAR);
+  add_param(sc, &s->omega[2].val, SHP_SPINPAR);
+  add_param(sc, &s->angle[0].val, SHP_SPINPAR);
+  add_param(sc, &s->angle[1].val, SHP_SPINPAR);
+}
+
+/* epoch of the k-th of n frames */
+static double frame_epoch(const shp_spec *sp, rng_t *r, int k, int n)
+{
+  (void)k; (void)n;
+  return sp->t0 + sp->epoch_span_days * rng_uniform(r);
+}
+
+static void init_delcor(shp_scene *sc, struct delcor_t *dc, double t0)
+{
+  dc->t0 = t0;
+  dc->n = 1;
+  dc->a = (struct param_t *)tcalloc(sc, 2, sizeof(struct param_t));
+  dc->a[0].state = 'f';
+  dc->a[0].val = sc->spec.delcor_a0;
+  dc->a[1].state = 'f';
+  dc->a[1].val = sc->spec.delcor_a1;
+  dc->delcor0_save = dc->a[0].val;
+}
+
+/* Doppler width (bins) the echo needs at resolution dpp: bandwidth plus sinc^2 wings */
+static double echo_bins(const shp_spec *sp, double dpp)
+{
+  double rmax = sp->semi_axes[0] * (1.0 + 3.0 * sp->rdev_sigma);
+  double omega = TWOPI / (sp->spin_period_h / 24.0);
+  double bw = 2.0 * rmax * omega * KM2HZFACT * sp->Ftx; /* Hz */
+  return bw / dpp;
+}
+
+static void build_data(shp_scene *sc)
+{
+  const shp_spec *sp = &sc->spec;
+  struct dat_t *dat = &sc->dat;
+  rng_t rf = rng_make(sp->seed_frames);
+  int n = sp->pos_n;
+  strcpy(dat->name, "synthetic");
+  alloc_pos(sc, &dat->pos, n, sp->km_per_pixel, 1);
+
+  int nrad_sets = sp->one_set_per_frame ? sp->n_deldop + sp->n_doppler
+                                        : (sp->n_deldop > 0) + (sp->n_doppler > 0);
+  int nsets = nrad_sets + sp->n_lightcurves;
+  dat->nsets = nsets;
+  dat->set = (struct set_t *)tcalloc(sc, nsets, sizeof(struct set_t));
+
+  /* photometric laws: one radar law per radar set, one optical law per lightcurve */
+  struct photo_t *ph = &sc->mod.photo;
+  ph->nradlaws = nrad_sets > 0 ? nrad_sets : 1;
+  ph->radtype = (unsigned char *)tcalloc(sc, ph->nradlaws, 1);
+  ph->radar = (union radscat_t *)tcalloc(sc, ph->nradlaws, sizeof(union radscat_t));
+  for (int l = 0; l < ph->nradlaws; l++) {
+    ph->radtype[l] = COSINELAW_DIFF;
+    ph->radar[l].RC.R.state = 'f';
+    ph->radar[l].RC.R.val = rng_range(&rf, sp->R_min, sp->R_max);
+    ph->radar[l].RC.C.state = 'f';
+    ph->radar[l].RC.C.val = rng_range(&rf, sp->C_min, sp->C_max);
+    ph->radar[l].RC.R_save = ph->radar[l].RC.R.val;
+  }
+  ph->noptlaws = sp->n_lightcurves > 0 ? sp->n_lightcurves : 1;
+  ph->opttype = (unsigned char *)tcalloc(sc, ph->noptlaws, 1);
+  ph->optical = (union optscat_t *)tcalloc(sc, ph->noptlaws, sizeof(union optscat_t));
+  for (int l = 0; l < ph->noptlaws; l++) {
+    ph->opttype[l] = KAASALAINEN;
+    struct kaas_t *k = &ph->optical[l].kaas;
+    k->R.state = 'f'; k->R.val = 1.0; k->R_save = 1.0;
+    k->wt.state = 'c'; k->wt.val = sp->kaas_wt;
+    k->A0.state = 'c'; k->A0.val = 0.0;
+    k->D.state = 'c'; k->D.val = 0.1;
+    k->k.state = 'c'; k->k.val = 0.0;
+  }
+
+  double tmid = sp->t0 + 0.5 * sp->epoch_span_days;
+  int s = 0;
+  double dof_dd = 0, dof_dop = 0, dof_lc = 0;
+
+  /* ---- delay-Doppler ---- */
+  int nframes_left = sp->n_deldop;
+  while (nframes_left > 0) {
+    int nfr = sp->one_set_per_frame ? 1 : nframes_left;
+    nframes_left -= nfr;
+    struct set_t *set = &dat->set[s];
+    set->type = DELAY;
+    set->inputnode = s;
+    for (int k = 0; k < 3; k++) { set->angleoff[k].state = 'c'; set->omegaoff[k].state = 'c'; }
+    struct deldop_t *dd = &set->desc.deldop;
+    dd->iradlaw = s;
+    dd->Ftx = sp->Ftx;
+    init_delcor(sc, &dd->delcor, tmid);
+    dd->del_per_pixel = sp->del_per_pixel;
+    dd->dop_per_pixel = sp->dop_per_pixel;
+    dd->codemethod = sp->codemethod;
+    dd->spb = sp->spb;
+    dd->stride = sp->stride;
+    dd->dopfftlen = sp->dopfftlen;
+    dd->dopscale.state = 'c';
+    dd->dopscale.val = 1.0;
+    dd->dopscale_save = 1.0;
+    dd->nviews = 1;
+    dd->v0 = 0;
+    dd->nframes = nfr;
+    dd->frame = (struct deldopfrm_t *)tcalloc(sc, nfr, sizeof(struct deldopfrm_t));
+    double set_dof = 0;
+    for (int f = 0; f < nfr; f++) {
+      struct deldopfrm_t *fr = &dd->frame[f];
+      int ndel = rng_int(&rf, sp->ndel_min, sp->ndel_max);
+      int ndop = rng_int(&rf, sp->ndop_min, sp->ndop_max);
+      double t = frame_epoch(sp, &rf, f, nfr);
+      if (sp->dop_per_pixel <= 0.0) {
+        /* choose the Doppler resolution (0.25 Hz steps) so the echo fits */
+        double dpp = 0.25;
+        while (echo_bins(sp, dpp) + sp->dopfftlen * 0 + 14 > ndop) dpp += 0.25;
+        dd->dop_per_pixel = dpp; /* set-level value; frames of a set share it */
+      }
+      fr->ndel = ndel;
+      fr->ndop = ndop;
+      fr->t0 = t;
+      fr->nlooks = 0;
+      fr->weight = 1.0;
+      fr->sdev = 1.0;
+      fr->cal.state = 'f';
+      fr->cal.val = 1.0;
+      fr->delcom_vig = 0.5 * (ndel + 1) + rng_range(&rf, -2.0, 2.0);
+      fr->dopcom_vig = 0.5 * (ndop + 1) + rng_range(&rf, -2.0, 2.0);
+      fr->dopDC_vig = fr->dopcom_vig + rng_range(&rf, -3.0, 3.0);
+      fr->delcom = fr->delcom_vig;
+      fr->obs = t_matrix(sc, 1, ndel, 1, ndop);
+      fr->fit = t_matrix(sc, 1, ndel, 1, ndop);
+      fr->oneovervar = t_matrix(sc, 1, ndel, 1, ndop);
+      for (int i = 1; i <= ndel; i++)
+        for (int j = 1; j <= ndop; j++) fr->oneovervar[i][j] = 1.0 / (fr->sdev * fr->sdev);
+      fr->dof = fr->weight * ndel * ndop;
+      set_dof += fr->dof;
+      snprintf(fr->name, sizeof(fr->name), "dd_%03d_%03d", s, f);
+      frame_pos(sc, &fr->pos, 0);
+      fr->view = (struct deldopview_t *)tcalloc(sc, 1, sizeof(struct deldopview_t));
+      fr->view[0].t = t;
+      double dir[3];
+      los_dir(sp, sp->epoch_span_days > 0 ? (t - sp->t0) / sp->epoch_span_days : 0.0, dir);
+      frame_from_dir(fr->view[0].oe, dir);
+    }
+    dd->dof = set_dof;
+    set->dof = set_dof;
+    dof_dd += set_dof;
+    s++;
+  }
+
+  /* ---- Doppler ---- */
+  nframes_left = sp->n_doppler;
+  while (nframes_left > 0) {
+    int nfr = sp->one_set_per_frame ? 1 : nframes_left;
+    nframes_left -= nfr;
+    struct set_t *set = &dat->set[s];
+    set->type = DOPPLER;
+    set->inputnode = s;
+    for (int k = 0; k < 3; k++) { set->angleoff[k].state = 'c'; set->omegaoff[k].state = 'c'; }
+    struct doppler_t *dp = &set->desc.doppler;
+    dp->iradlaw = s;
+    dp->Ftx = sp->Ftx;
+    init_delcor(sc, &dp->delcor, tmid);
+    dp->dop_per_bin = rng_range(&rf, sp->dop_per_bin_min, sp->dop_per_bin_max);
+    dp->dopscale.state = 'c';
+    dp->dopscale.val = 1.0;
+    dp->dopscale_save = 1.0;
+    dp->nviews = 1;
+    dp->v0 = 0;
+    dp->nframes = nfr;
+    dp->frame = (struct dopfrm_t *)tcalloc(sc, nfr, sizeof(struct dopfrm_t));
+    double set_dof = 0;
+    for (int f = 0; f < nfr; f++) {
+      struct dopfrm_t *fr = &dp->frame[f];
+      int ndop = rng_int(&rf, sp->nbin_min, sp->nbin_max);
+      double t = frame_epoch(sp, &rf, f, nfr);
+      fr->ndop = ndop;
+      fr->t0 = t;
+      fr->weight = 1.0;
+      fr->sdev = 1.0;
+      fr->cal.state = 'f';
+      fr->cal.val = 1.0;
+      fr->dopcom_vig = 0.5 * (ndop + 1) + rng_range(&rf, -2.0, 2.0);
+      fr->obs = t_vector(sc, 1, ndop);
+      fr->fit = t_vector(sc, 1, ndop);
+      fr->oneovervar = t_vector(sc, 1, ndop);
+      for (int j = 1; j <= ndop; j++) fr->oneovervar[j] = 1.0;
+      fr->dof = fr->weight * ndop;
+      set_dof += fr->dof;
+      snprintf(fr->name, sizeof(fr->name), "dop_%03d_%03d", s, f);
+      frame_pos(sc, &fr->pos, 0);
+      fr->view = (struct dopview_t *)tcalloc(sc, 1, sizeof(struct dopview_t));
+      fr->view[0].t = t;
+      double dir[3];
+      los_dir(sp, sp->epoch_span_days > 0 ? (t - sp->t0) / sp->epoch_span_days : 0.0, dir);
+      frame_from_dir(fr->view[0].oe, dir);
+    }
+    dp->dof = set_dof;
+    set->dof = set_dof;
+    dof_dop += set_dof;
+    s++;
+  }
+
+  /* ---- lightcurves ---- */
+  for (int l = 0; l < sp->n_lightcurves; l++, s++) {
+    struct set_t *set = &dat->set[s];
+    set->type = LGHTCRV;
+    set->inputnode = s;
+    for (int k = 0; k < 3; k++) { set->angleoff[k].state = 'c'; set->omegaoff[k].state = 'c'; }
+    struct lghtcrv_t *lc = &set->desc.lghtcrv;
+    lc->ioptlaw = l;
+    lc->nviews = 1;
+    lc->v0 = 0;
+    lc->n = sp->lc_points;
+    lc->ncalc = sp->lc_ncalc;
+    lc->weight = 1.0;
+    lc->cal.state = 'f';
+    lc->cal.val = 1.0;
+    int nn = lc->n, nc = lc->ncalc;
+    lc->t0 = t_vector(sc, 1, nn);
+    lc->t = t_matrix(sc, 1, nn, 0, 0);
+    lc->obs = t_vector(sc, 1, nn);
+    lc->oneovervar = t_vector(sc, 1, nn);
+    lc->fit = t_vector(sc, 1, nn);
+    lc->rotphase_calc = t_vector(sc, 1, nc);
+    lc->rotphase_obs = t_vector(sc, 1, nn);
+    lc->x0 = t_vector(sc, 1, nc);
+    lc->x = t_vector(sc, 1, nc);
+    lc->y = t_vector(sc, 1, nc);
+    lc->y2 = t_vector(sc, 1, nc);
+    lc->solar_phase = t_vector(sc, 1, nc);
+    lc->solar_azimuth = t_vector(sc, 1, nc);
+    lc->rend = (struct crvrend_t *)tcalloc(sc, nc + 1, sizeof(struct crvrend_t));
+    /* one night: calc points cover [tstart, tstart+len], observations inside */
+    double len = sp->spin_period_h / 24.0 * 1.2;
+    double tstart = sp->t0 + (sp->epoch_span_days - len > 0 ? (sp->epoch_span_days - len) * rng_uniform(&rf) : 0.0);
+    double phase = rng_range(&rf, sp->phase_min_deg, sp->phase_max_deg) * D2R;
+    double dir[3], sdir[3], oe[3][3];
+    los_dir(sp, sp->epoch_span_days > 0 ? (tstart - sp->t0) / sp->epoch_span_days : 0.0, dir);
+    frame_from_dir(oe, dir);
+    /* sun direction: rotate the line of sight by the solar phase angle toward +x */
+    for (int k = 0; k < 3; k++) sdir[k] = cos(phase) * oe[2][k] + sin(phase) * oe[0][k];
+    for (int i = 1; i <= nc; i++) {
+      lc->x[i] = lc->x0[i] = tstart + len * (i - 1) / (nc - 1);
+      lc->solar_phase[i] = phase;
+      lc->solar_azimuth[i] = 0.0;
+      struct crvrend_t *rd = &lc->rend[i];
+      memcpy(rd->oe, oe, sizeof(oe));
+      frame_from_dir(rd->se, sdir);
+      frame_pos(sc, &rd->pos, 1);
+    }
+    for (int i = 1; i <= nn; i++) {
+      double t = tstart + len * (0.02 + 0.96 * (i - 0.5) / nn);
+      lc->t0[i] = t;
+      lc->t[i][0] = t;
+      lc->oneovervar[i] = 1.0;
+    }
+    lc->dof = lc->weight * nn;
+    set->dof = lc->dof;
+    dof_lc += lc->dof;
+    snprintf(lc->name, sizeof(lc->name), "lc_%03d", s);
+  }
+
+  dat->dof_deldop = dof_dd;
+  dat->dof_doppler = dof_dop;
+  dat->dof_lghtcrv = dof_lc;
+  dat->dof_poset = 0.0;
+  dat->dof = dof_dd + dof_dop + dof_lc;
+
+  /* free parameters beyond the model's: first set's radar law, first set's delay polynomial */
+  if (nrad_sets > 0) {
+    add_param(sc, &ph->radar[0].RC.R.val, SHP_PHOTOPAR);
+    add_param(sc, &ph->radar[0].RC.C.val, SHP_PHOTOPAR);
+    struct delcor_t *dc = dat->set[0].type == DELAY ? &dat->set[0].desc.deldop.delcor
+                                                    : &dat->set[0].desc.doppler.delcor;
+    add_param(sc, &dc->a[0].val, SHP_DELCORPAR);
+    add_param(sc, &dc->a[1].val, SHP_DELCORPAR);
+  }
+}
+
+static void build_par(shp_scene *sc)
+{
+  struct par_t *p = &sc->par;
+  const shp_spec *sp = &sc->spec;
+  p->action = FIT;
+  p->speckle = 0;
+  p->pos_smooth = (unsigned char)sp->pos_smooth;
+  p->pos_scope = (unsigned char)sp->pos_scope;
+  p->int_method = PERIODIC;
+  p->bad_objfactor = 2.0;
+  p->pos_pixels = 2 * sp->pos_n + 1;
+  p->pos_width = sp->km_per_pixel * 2 * sp->pos_n;
+  p->dd_gamma = 1.0;
+  p->mask_tol = p->pos_width / (p->pos_pixels - 1);
+  p->nsinc2 = 1;
+  p->sinc2width = 6;
+  p->exclude_seen = -1;
+  p->mincosine_seen = 0.0;
+  p->int_abstol = 1.0e-7;
+  p->term_prec = 1.0e-6;
+  p->npar_update = 20;
+  for (int i = 0; i < MAXSUNMAGS; i++) p->sun_appmag[i] = -26.762;
+}
+
+shp_scene *shp_scene_create(const shp_spec *spec)
+{
+  shp_scene *sc = (shp_scene *)calloc(1, sizeof(shp_scene));
+  sc->spec = *spec;
+  build_par(sc);
+  build_model(sc);
+  build_data(sc);
+  sc->par.nfpar = sc->nparams;
+  sc->par.fpntr = (double **)tcalloc(sc, sc->nparams, sizeof(double *));
+  sc->par.fpartype = (int *)tcalloc(sc, sc->nparams, sizeof(int));
+  for (int i = 0; i < sc->nparams; i++) { sc->par.fpntr[i] = sc->params[i]; sc->par.fpartype[i] = sc->ptype[i]; }
+  return sc;
+}
+
+void shp_scene_destroy(shp_scene *sc)
+{
+  if (!sc) return;
+  for (int i = sc->nalloc - 1; i >= 0; i--) free(sc->allocs[i]);
+  free(sc->allocs);
+  free(sc);
+}
+
+struct par_t *shp_scene_par(shp_scene *sc) { return &sc->par; }
+struct mod_t *shp_scene_mod(shp_scene *sc) { return &sc->mod; }
+struct dat_t *shp_scene_dat(shp_scene *sc) { return &sc->dat; }
+int shp_scene_nv(shp_scene *sc) { return sc->mod.shape.comp[0].real.nv; }
+int shp_scene_nf(shp_scene *sc) { return sc->mod.shape.comp[0].real.nf; }
+double *shp_param_ptr(shp_scene *sc, int i) { return (i >= 0 && i < sc->nparams) ? sc->params[i] : NULL; }
+int shp_param_count(shp_scene *sc) { return sc->nparams; }
+int shp_param_type(shp_scene *sc, int i) { return (i >= 0 && i < sc->nparams) ? sc->ptype[i] : 0; }
+
+void shp_scene_synthesize_obs(shp_scene *sc, double snr, double lc_rel_sigma, unsigned long long seed)
+{
+  rng_t rn = rng_make(seed);
+  struct dat_t *dat = &sc->dat;
+  for (int s = 0; s < dat->nsets; s++) {
+    struct set_t *set = &dat->set[s];
+    if (set->type == DELAY) {
+      for (int f = 0; f < set->desc.deldop.nframes; f++) {
+        struct deldopfrm_t *fr = &set->desc.deldop.frame[f];
+        double mx = 0;
+        for (int i = 1; i <= fr->ndel; i++)
+          for (int j = 1; j <= fr->ndop; j++) if (fr->fit[i][j] > mx) mx = fr->fit[i][j];
+        double A = mx > 0 ? snr * fr->sdev / mx : 1.0;
+        for (int i = 1; i <= fr->ndel; i++)
+          for (int j = 1; j <= fr->ndop; j++) fr->obs[i][j] = A * fr->fit[i][j] + fr->sdev * rng_normal(&rn);
+        fr->cal.val = A;
+      }
+    } else if (set->type == DOPPLER) {
+      for (int f = 0; f < set->desc.doppler.nframes; f++) {
+        struct dopfrm_t *fr = &set->desc.doppler.frame[f];
+        double mx = 0;
+        for (int j = 1; j <= fr->ndop; j++) if (fr->fit[j] > mx) mx = fr->fit[j];
+        double A = mx > 0 ? snr * fr->sdev / mx : 1.0;
+        for (int j = 1; j <= fr->ndop; j++) fr->obs[j] = A * fr->fit[j] + fr->sdev * rng_normal(&rn);
+        fr->cal.val = A;
+      }
+    } else if (set->type == LGHTCRV) {
+      struct lghtcrv_t *lc = &set->desc.lghtcrv;
+      double mean = 0;
+      for (int i = 1; i <= lc->n; i++) mean += lc->fit[i];
+      mean /= lc->n;
+      for (int i = 1; i <= lc->n; i++) {
+        double sig = lc_rel_sigma * mean;
+        lc->obs[i] = lc->fit[i] + sig * rng_normal(&rn);
+        lc->oneovervar[i] = 1.0 / (sig * sig);
+      }
+    }
+  }
+}
+
+/* ------------------------------------------------------------------ */
+/* accessors                                                          */
+/* ------------------------------------------------------------------ */
+int shp_set_type(struct dat_t *dat, int s) { return dat->set[s].type; }
+int shp_dat_nsets(struct dat_t *dat) { return dat->nsets; }
+
+int shp_set_nframes(struct dat_t *dat, int s)
+{
+  struct set_t *set = &dat->set[s];
+  switch (set->type) {
+  case DELAY: return set->desc.deldop.nframes;
+  case DOPPLER: return set->desc.doppler.nframes;
+  case LGHTCRV: return set->desc.lghtcrv.ncalc;
+  }
+  return 0;
+}
+
+void shp_frame_dims(struct dat_t *dat, int s, int f, int out[2])
+{
+  struct set_t *set = &dat->set[s];
+  out[0] = out[1] = 0;
+  if (set->type == DELAY) { out[0] = set->desc.deldop.frame[f].ndel; out[1] = set->desc.deldop.frame[f].ndop; }
+  else if (set->type == DOPPLER) { out[0] = 1; out[1] = set->desc.doppler.frame[f].ndop; }
+  else if (set->type == LGHTCRV) { out[0] = set->desc.lghtcrv.n; out[1] = set->desc.lghtcrv.ncalc; }
+}
+
+void shp_frame_image(struct dat_t *dat, int s, int f, int which, double *out)
+{
+  struct set_t *set = &dat->set[s];
+  if (set->type == DELAY) {
+    struct deldopfrm_t *fr = &set->desc.deldop.frame[f];
+    double **m = which == 0 ? fr->fit : which == 1 ? fr->obs : fr->oneovervar;
+    for (int i = 1; i <= fr->ndel; i++)
+      for (int j = 1; j <= fr->ndop; j++) *out++ = m[i][j];
+  } else if (set->type == DOPPLER) {
+    struct dopfrm_t *fr = &set->desc.doppler.frame[f];
+    double *m = which == 0 ? fr->fit : which == 1 ? fr->obs : fr->oneovervar;
+    for (int j = 1; j <= fr->ndop; j++) *out++ = m[j];
+  } else if (set->type == LGHTCRV) {
+    struct lghtcrv_t *lc = &set->desc.lghtcrv;
+    if (which == 3) { for (int i = 1; i <= lc->ncalc; i++) *out++ = lc->y[i]; return; }
+    double *m = which == 0 ? lc->fit : which == 1 ? lc->obs : lc->oneovervar;
+    for (int i = 1; i <= lc->n; i++) *out++ = m[i];
+  }
+}
+
+void shp_frame_set_image(struct dat_t *dat, int s, int f, int which, const double *in)
+{
+  struct set_t *set = &dat->set[s];
+  if (set->type == DELAY) {
+    struct deldopfrm_t *fr = &set->desc.deldop.frame[f];
+    double **m = which == 0 ? fr->fit : which == 1 ? fr->obs : fr->oneovervar;
+    for (int i = 1; i <= fr->ndel; i++)
+      for (int j = 1; j <= fr->ndop; j++) m[i][j] = *in++;
+  } else if (set->type == DOPPLER) {
+    struct dopfrm_t *fr = &set->desc.doppler.frame[f];
+    double *m = which == 0 ? fr->fit : which == 1 ? fr->obs : fr->oneovervar;
+    for (int j = 1; j <= fr->ndop; j++) m[j] = *in++;
+  } else if (set->type == LGHTCRV) {
+    struct lghtcrv_t *lc = &set->desc.lghtcrv;
+    double *m = which == 0 ? lc->fit : which == 1 ? lc->obs : lc->oneovervar;
+    for (int i = 1; i <= lc->n; i++) m[i] = *in++;
+  }
+}
+
+void shp_frame_scalars(struct dat_t *dat, int s, int f, double *out)
+{
+  struct set_t *set = &dat->set[s];
+  memset(out, 0, sizeof(double) * 18);
+  if (set->type == DELAY) {
+    struct deldopfrm_t *fr = &set->desc.deldop.frame[f];
+    out[0] = fr->chi2; out[1] = fr->cal.val; out[2] = fr->overflow_o2; out[3] = fr->overflow_m2;
+    out[4] = fr->badradar_logfactor; out[5] = fr->dof;
+    out[6] = fr->idellim[0]; out[7] = fr->idellim[1]; out[8] = fr->idoplim[0]; out[9] = fr->idoplim[1];
+    out[10] = fr->dellim[0]; out[11] = fr->dellim[1]; out[12] = fr->doplim[0]; out[13] = fr->doplim[1];
+    out[14] = fr->view[set->desc.deldop.v0].km2Hz; out[15] = fr->overflow_xsec;
+    out[16] = fr->overflow_dopmean; out[17] = fr->weight;
+  } else if (set->type == DOPPLER) {
+    struct dopfrm_t *fr = &set->desc.doppler.frame[f];
+    out[0] = fr->chi2; out[1] = fr->cal.val; out[2] = fr->overflow_o2; out[3] = fr->overflow_m2;
+    out[4] = fr->badradar_logfactor; out[5] = fr->dof;
+    out[8] = fr->idoplim[0]; out[9] = fr->idoplim[1];
+    out[12] = fr->doplim[0]; out[13] = fr->doplim[1];
+    out[14] = fr->view[set->desc.doppler.v0].km2Hz; out[15] = fr->overflow_xsec;
+    out[16] = fr->overflow_dopmean; out[17] = fr->weight;
+  } else if (set->type == LGHTCRV) {
+    struct lghtcrv_t *lc = &set->desc.lghtcrv;
+    out[0] = set->chi2; out[1] = lc->cal.val; out[5] = lc->dof; out[17] = lc->weight;
+  }
+}
+
+struct pos_t *shp_frame_pos(struct dat_t *dat, int s, int f)
+{
+  struct set_t *set = &dat->set[s];
+  if (set->type == DELAY) return &set->desc.deldop.frame[f].pos;
+  if (set->type == DOPPLER) return &set->desc.doppler.frame[f].pos;
+  if (set->type == LGHTCRV) return &set->desc.lghtcrv.rend[f + 1].pos;
+  return NULL;
+}
+
+void shp_pos_array(struct pos_t *pos, int which, double *out)
+{
+  int n = pos->n;
+  double **d = NULL;
+  int **iv = NULL;
+  switch (which) {
+  case 0: d = pos->b; break;
+  case 1: d = pos->cose; break;
+  case 2: d = pos->cosi; break;
+  case 3: d = pos->z; break;
+  case 4: d = pos->cosill; break;
+  case 5: d = pos->zill; break;
+  case 10: iv = pos->f; break;
+  case 11: iv = pos->body; break;
+  case 12: iv = pos->comp; break;
+  case 13: iv = pos->fill; break;
+  default: return;
+  }
+  for (int i = -n; i <= n; i++)
+    for (int j = -n; j <= n; j++) *out++ = d ? d[i][j] : (double)iv[i][j];
+}
+
+void shp_pos_limits(struct pos_t *pos, int out[8])
+{
+  out[0] = pos->xlim[0]; out[1] = pos->xlim[1]; out[2] = pos->ylim[0]; out[3] = pos->ylim[1];
+  out[4] = pos->xlim2[0]; out[5] = pos->xlim2[1]; out[6] = pos->ylim2[0]; out[7] = pos->ylim2[1];
+}
+
+static void view_ptrs(struct dat_t *dat, int s, int f, int v, double (**ae)[3], double (**oe)[3],
+                      double **spin, double **intspin, double **deloff, double **dopoff)
+{
+  struct set_t *set = &dat->set[s];
+  *deloff = *dopoff = NULL;
+  if (set->type == DELAY) {
+    struct deldopview_t *w = &set->desc.deldop.frame[f].view[v];
+    *ae = w->ae; *oe = w->oe; *spin = w->spin; *intspin = w->intspin; *deloff = &w->deloff; *dopoff = &w->dopoff;
+  } else if (set->type == DOPPLER) {
+    struct dopview_t *w = &set->desc.doppler.frame[f].view[v];
+    *ae = w->ae; *oe = w->oe; *spin = w->spin; *intspin = w->intspin; *dopoff = &w->dopoff;
+  } else {
+    struct crvrend_t *w = &set->desc.lghtcrv.rend[f + 1];
+    *ae = w->ae; *oe = w->oe; *spin = w->spin; *intspin = w->intspin;
+  }
+}
+
+void shp_view_data(struct dat_t *dat, int s, int f, int v, double *out)
+{
+  double (*ae)[3], (*oe)[3], *spin, *intspin, *deloff, *dopoff;
+  view_ptrs(dat, s, f, v, &ae, &oe, &spin, &intspin, &deloff, &dopoff);
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < 3; j++) { out[3 * i + j] = ae[i][j]; out[9 + 3 * i + j] = oe[i][j]; }
+  for (int k = 0; k < 3; k++) { out[18 + k] = spin[k]; out[21 + k] = intspin[k]; }
+  out[24] = deloff ? *deloff : 0.0;
+  out[25] = dopoff ? *dopoff : 0.0;
+  out[26] = out[27] = out[28] = 0.0;
+}
+
+void shp_set_view_data(struct dat_t *dat, int s, int f, int v, const double *in)
+{
+  double (*ae)[3], (*oe)[3], *spin, *intspin, *deloff, *dopoff;
+  view_ptrs(dat, s, f, v, &ae, &oe, &spin, &intspin, &deloff, &dopoff);
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < 3; j++) { ae[i][j] = in[3 * i + j]; oe[i][j] = in[9 + 3 * i + j]; }
+  for (int k = 0; k < 3; k++) { spin[k] = in[18 + k]; intspin[k] = in[21 + k]; }
+  if (deloff) *deloff = in[24];
+  if (dopoff) *dopoff = in[25];
+}
+
+void shp_geometry(struct mod_t *mod, double *vx, double *vn, double *fn, double *area)
+{
+  struct vertices_t *rv = &mod->shape.comp[0].real;
+  for (int i = 0; i < rv->nv; i++)
+    for (int k = 0; k < 3; k++) {
+      if (vx) vx[3 * i + k] = rv->v[i].x[k];
+      if (vn) vn[3 * i + k] = rv->v[i].n[k];
+    }
+  for (int f = 0; f < rv->nf; f++) {
+    for (int k = 0; k < 3; k++) if (fn) fn[3 * f + k] = rv->f[f].n[k];
+    if (area) area[f] = rv->f[f].area;
+  }
+}
+
+void shp_facet_seen(struct mod_t *mod, unsigned char *out)
+{
+  struct vertices_t *rv = &mod->shape.comp[0].real;
+  for (int f = 0; f < rv->nf; f++) out[f] = rv->f[f].seen;
+}
+
+void shp_par_flags(struct par_t *par, double out[6])
+{
+  out[0] = par->posbnd; out[1] = par->badradar; out[2] = par->baddiam;
+  out[3] = par->posbnd_logfactor; out[4] = par->badradar_logfactor; out[5] = par->baddiam_logfactor;
+}

@@ -1,0 +1,133 @@
+/*
+ * scene.h -- synthetic SHAPE scenes built directly as ABI structs.
+ *
+ * The reference builds par_t/mod_t/dat_t from text files (read_par.c,
+ * read_mod.c, read_dat.c).  This module builds the same structs from a
+ * numeric specification instead, so that benchmarks and parity tests can run
+ * without data files.  Allocation follows the reference's conventions
+ * (Numerical Recipes offset matrices, one POS block shared by every frame
+ * when pos_scope is GLOBAL, read_dat.c:1239-1290).
+ */
+#ifndef SHAPE_SCENE_H
+#define SHAPE_SCENE_H
+
+#include "shape_abi.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum { SHP_SHAPE_ELLIPSOID_TESS = 0, SHP_SHAPE_ICOSPHERE = 1 };
+
+/* Free-parameter kinds, numbered as mkparlist.c numbers fpartype. */
+enum { SHP_SIZEPAR = 1, SHP_SHAPEPAR = 2, SHP_PHOTOPAR = 3, SHP_SPINPAR = 4,
+       SHP_DELCORPAR = 5, SHP_DOPSCALEPAR = 6 };
+
+typedef struct shp_spec {
+  /* geometry */
+  int shape_kind;          /* SHP_SHAPE_* */
+  int theta_steps;         /* ellipsoid tessellation resolution (read_mod.c:2179) */
+  int ico_level;           /* icosphere subdivision level */
+  double semi_axes[3];     /* km */
+  double rdev_sigma;       /* vertex radial deviation: r *= 1 + sigma*N(0,1) */
+  unsigned long long seed_geom;
+  unsigned long long seed_frames;
+  /* spin state (principal-axis rotator) */
+  double spin_period_h;
+  double pole_lambda_deg, pole_beta_deg, phase0_deg;
+  double t0;               /* spin epoch (JD) */
+  double epoch_span_days;  /* frame epochs are spread over [t0, t0+span] */
+  /* plane of sky */
+  int pos_n;               /* POS is (2n+1)^2 pixels */
+  double km_per_pixel;
+  int pos_scope;           /* GLOBAL or LOCAL */
+  int pos_smooth;
+  /* radar */
+  double Ftx;              /* MHz */
+  int codemethod, spb, stride, dopfftlen;
+  double R_min, R_max, C_min, C_max;   /* cosine law per set */
+  int one_set_per_frame;   /* 1: every radar frame is its own set */
+  /* delay-Doppler frames */
+  int n_deldop;
+  int ndel_min, ndel_max, ndop_min, ndop_max;
+  double del_per_pixel;
+  double dop_per_pixel;    /* <= 0: choose per frame so the echo fits */
+  /* Doppler-only frames */
+  int n_doppler;
+  int nbin_min, nbin_max;
+  double dop_per_bin_min, dop_per_bin_max;
+  /* lightcurves */
+  int n_lightcurves;
+  int lc_points;           /* observed points per lightcurve */
+  int lc_ncalc;            /* calculated points per lightcurve */
+  double phase_min_deg, phase_max_deg;
+  double kaas_wt;          /* Lambert weight of the Kaasalainen law */
+  /* delay correction polynomial (usec, usec/day) */
+  double delcor_a0, delcor_a1;
+  /* viewing geometry drift: sub-observer latitude range (deg) */
+  double obs_lat_min_deg, obs_lat_max_deg;
+} shp_spec;
+
+typedef struct shp_scene shp_scene;
+
+void shp_spec_defaults(shp_spec *spec);
+shp_scene *shp_scene_create(const shp_spec *spec);
+void shp_scene_destroy(shp_scene *sc);
+struct par_t *shp_scene_par(shp_scene *sc);
+struct mod_t *shp_scene_mod(shp_scene *sc);
+struct dat_t *shp_scene_dat(shp_scene *sc);
+/* number of vertices / facets actually generated */
+int shp_scene_nv(shp_scene *sc);
+int shp_scene_nf(shp_scene *sc);
+
+/* Turn the current fit images into observations: obs = A*fit + N(0,sigma),
+   A chosen per frame so the brightest model pixel has signal-to-noise snr.
+   Lightcurves get obs = fit*(1 + rel_sigma*N(0,1)).  Must be called after a
+   calc_fits on the truth model. */
+void shp_scene_synthesize_obs(shp_scene *sc, double snr, double lc_rel_sigma,
+                              unsigned long long seed);
+
+/* NR-style allocation (row pointers into one zeroed block). */
+double **shp_matrix(int nrl, int nrh, int ncl, int nch);
+int **shp_imatrix(int nrl, int nrh, int ncl, int nch);
+double *shp_vector(int nl, int nh);
+void shp_free_matrix(double **m, int nrl, int ncl);
+void shp_free_imatrix(int **m, int nrl, int ncl);
+void shp_free_vector(double *v, int nl);
+
+/* ---- accessors used by tests and benchmarks ---- */
+int shp_set_type(struct dat_t *dat, int s);
+int shp_dat_nsets(struct dat_t *dat);
+int shp_set_nframes(struct dat_t *dat, int s);      /* frames, or ncalc for lightcurves */
+void shp_frame_dims(struct dat_t *dat, int s, int f, int out[2]); /* ndel, ndop (or n, ncalc) */
+/* Copy a fit/obs/oneovervar image row-major into out.  which: 0 fit, 1 obs, 2 oneovervar */
+void shp_frame_image(struct dat_t *dat, int s, int f, int which, double *out);
+void shp_frame_set_image(struct dat_t *dat, int s, int f, int which, const double *in);
+/* Per-frame scalars: chi2, cal, overflow_o2, overflow_m2, badradar_logfactor,
+   dof, idellim[2], idoplim[2], dellim[2], doplim[2], km2Hz(v0), overflow_xsec,
+   overflow_dopmean, weight -> 18 doubles */
+void shp_frame_scalars(struct dat_t *dat, int s, int f, double *out);
+struct pos_t *shp_frame_pos(struct dat_t *dat, int s, int f);
+/* Copy a POS array (2n+1)^2 row-major (x major, y minor) into out.
+   which: 0 b, 1 cose, 2 cosi, 3 z, 4 cosill, 5 zill  (doubles)
+          10 f, 11 body, 12 comp, 13 fill               (ints, written as doubles) */
+void shp_pos_array(struct pos_t *pos, int which, double *out);
+void shp_pos_limits(struct pos_t *pos, int out[8]); /* xlim, ylim, xlim2, ylim2 */
+/* view data: ae[9], oe[9], spin[3], intspin[3], deloff, dopoff -> 29 doubles */
+void shp_view_data(struct dat_t *dat, int s, int f, int v, double *out);
+void shp_set_view_data(struct dat_t *dat, int s, int f, int v, const double *in);
+/* realized geometry: x[3*nv], vertex n[3*nv], facet n[3*nf], area[nf] */
+void shp_geometry(struct mod_t *mod, double *vx, double *vn, double *fn, double *area);
+void shp_facet_seen(struct mod_t *mod, unsigned char *out);
+/* par flags: posbnd, badradar, baddiam; logfactors posbnd, badradar, baddiam */
+void shp_par_flags(struct par_t *par, double out[6]);
+double *shp_param_ptr(shp_scene *sc, int i);
+int shp_param_count(shp_scene *sc);
+int shp_param_type(shp_scene *sc, int i);
+/* Deep copy of the parts of a scene the forward model writes (fit images,
+   frame scalars, view data) from src into dst; both built from one spec. */
+
+#ifdef __cplusplus
+}
+#endif
+#endif

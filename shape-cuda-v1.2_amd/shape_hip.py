@@ -1,0 +1,329 @@
+"""ctypes bindings for the SHAPE scene builder and the HIP model-evaluation library.
+
+The libraries are C-ABI shared objects built in-tree (``lib/``):
+
+* ``libshapescene.so`` -- synthetic scenes as SHAPE ABI structs (par_t/mod_t/dat_t),
+  plus accessors that copy NR-offset arrays into numpy.
+* ``libshape_hip.so`` -- the HIP drop-ins for the reference's entry points
+  (realize_mod, realize_spin, realize_delcor, posclr, posvis, posmask, radlaw,
+  pos2deldop, pos2doppler, apply_photo, calc_fits, chi2) and the batched
+  evaluation engine.
+
+Loading the HIP library never falls back to anything: if it is missing,
+``load_hip()`` raises.
+"""
+import ctypes as C
+import math
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIBDIR = os.path.join(HERE, "lib")
+
+GLOBAL, LOCAL = 1, 2
+SHORT, LONG_ORIG, LONG_MOD = 0, 1, 2
+DELAY, DOPPLER, POS, LGHTCRV = 0, 1, 2, 3
+SHAPE_ELLIPSOID_TESS, SHAPE_ICOSPHERE = 0, 1
+
+
+class Spec(C.Structure):
+    """Mirror of shp_spec in csrc/scene/scene.h."""
+    _fields_ = [
+        ("shape_kind", C.c_int), ("theta_steps", C.c_int), ("ico_level", C.c_int),
+        ("semi_axes", C.c_double * 3), ("rdev_sigma", C.c_double),
+        ("seed_geom", C.c_ulonglong), ("seed_frames", C.c_ulonglong),
+        ("spin_period_h", C.c_double), ("pole_lambda_deg", C.c_double),
+        ("pole_beta_deg", C.c_double), ("phase0_deg", C.c_double),
+        ("t0", C.c_double), ("epoch_span_days", C.c_double),
+        ("pos_n", C.c_int), ("km_per_pixel", C.c_double), ("pos_scope", C.c_int),
+        ("pos_smooth", C.c_int),
+        ("Ftx", C.c_double), ("codemethod", C.c_int), ("spb", C.c_int), ("stride", C.c_int),
+        ("dopfftlen", C.c_int),
+        ("R_min", C.c_double), ("R_max", C.c_double), ("C_min", C.c_double), ("C_max", C.c_double),
+        ("one_set_per_frame", C.c_int),
+        ("n_deldop", C.c_int), ("ndel_min", C.c_int), ("ndel_max", C.c_int),
+        ("ndop_min", C.c_int), ("ndop_max", C.c_int),
+        ("del_per_pixel", C.c_double), ("dop_per_pixel", C.c_double),
+        ("n_doppler", C.c_int), ("nbin_min", C.c_int), ("nbin_max", C.c_int),
+        ("dop_per_bin_min", C.c_double), ("dop_per_bin_max", C.c_double),
+        ("n_lightcurves", C.c_int), ("lc_points", C.c_int), ("lc_ncalc", C.c_int),
+        ("phase_min_deg", C.c_double), ("phase_max_deg", C.c_double), ("kaas_wt", C.c_double),
+        ("delcor_a0", C.c_double), ("delcor_a1", C.c_double),
+        ("obs_lat_min_deg", C.c_double), ("obs_lat_max_deg", C.c_double),
+    ]
+
+
+_scene_lib = None
+
+
+def load_scene():
+    global _scene_lib
+    if _scene_lib is None:
+        path = os.path.join(LIBDIR, "libshapescene.so")
+        if not os.path.exists(path):
+            raise RuntimeError("libshapescene.so is not built (run __graft_entry__.build())")
+        lib = C.CDLL(path)
+        vp, ip, dp = C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_double)
+        sigs = {
+            "shp_spec_defaults": (None, [C.POINTER(Spec)]),
+            "shp_scene_create": (vp, [C.POINTER(Spec)]),
+            "shp_scene_destroy": (None, [vp]),
+            "shp_scene_par": (vp, [vp]), "shp_scene_mod": (vp, [vp]), "shp_scene_dat": (vp, [vp]),
+            "shp_scene_nv": (C.c_int, [vp]), "shp_scene_nf": (C.c_int, [vp]),
+            "shp_scene_synthesize_obs": (None, [vp, C.c_double, C.c_double, C.c_ulonglong]),
+            "shp_set_type": (C.c_int, [vp, C.c_int]),
+            "shp_dat_nsets": (C.c_int, [vp]),
+            "shp_set_nframes": (C.c_int, [vp, C.c_int]),
+            "shp_frame_dims": (None, [vp, C.c_int, C.c_int, ip]),
+            "shp_frame_image": (None, [vp, C.c_int, C.c_int, C.c_int, dp]),
+            "shp_frame_set_image": (None, [vp, C.c_int, C.c_int, C.c_int, dp]),
+            "shp_frame_scalars": (None, [vp, C.c_int, C.c_int, dp]),
+            "shp_frame_pos": (vp, [vp, C.c_int, C.c_int]),
+            "shp_pos_array": (None, [vp, C.c_int, dp]),
+            "shp_pos_limits": (None, [vp, ip]),
+            "shp_view_data": (None, [vp, C.c_int, C.c_int, C.c_int, dp]),
+            "shp_set_view_data": (None, [vp, C.c_int, C.c_int, C.c_int, dp]),
+            "shp_geometry": (None, [vp, dp, dp, dp, dp]),
+            "shp_facet_seen": (None, [vp, C.POINTER(C.c_ubyte)]),
+            "shp_par_flags": (None, [vp, dp]),
+            "shp_param_ptr": (dp, [vp, C.c_int]),
+            "shp_param_count": (C.c_int, [vp]),
+            "shp_param_type": (C.c_int, [vp, C.c_int]),
+        }
+        for name, (res, args) in sigs.items():
+            fn = getattr(lib, name)
+            fn.restype = res
+            fn.argtypes = args
+        _scene_lib = lib
+    return _scene_lib
+
+
+# ---------------------------------------------------------------------------
+# configurations (the five benchmark / parity workloads)
+# ---------------------------------------------------------------------------
+def spec_for(config, **over):
+    """Return a Spec for workload `config` (1..5); keyword overrides win.
+
+    1: triaxial ellipsoid 1.0x0.8x0.6 km, theta_steps=30 (1148 V / 2292 F), 2% radial noise,
+       one 100x80 delay-Doppler frame, POS 151^2 at 15 m.
+    2: ellipsoid tessellation theta_steps=80 (8150 V / 16296 F; 16382 F is not reachable
+       by a closed tessellation of that family), 3% noise, 64 frames rows 60-200 x cols
+       40-160, POS 255^2 at 10 m, epochs over 30 days, R~U(0.1,0.3), C~U(1,3).
+    3: icosphere level 6 (40962 V / 81920 F), 50 delay-Doppler (50-300 x 30-250),
+       40 Doppler (100-400 bins, 0.25-2 Hz), 20 lightcurves x 100 points (2000 points),
+       Kaasalainen (L-S 0.9 + Lambert 0.1), solar phase 5-60 deg.
+    4: icosphere level 7 (163842 V / 327680 F) scaled to 0.9x0.72x0.54 km, POS 1023^2 at
+       2 m, 256 frames 128-512 x 64-256.
+    5: icosphere level 5 (10242 V / 20480 F), 24 frames rows 60-250 x cols 40-160,
+       POS 255^2 at 10 m (one model; the batch driver makes 32 with per-model seeds).
+    """
+    lib = load_scene()
+    s = Spec()
+    lib.shp_spec_defaults(C.byref(s))
+    if config == 1:
+        pass
+    elif config == 2:
+        s.theta_steps = 80
+        s.rdev_sigma = 0.03
+        s.seed_geom, s.seed_frames = 2001, 2002
+        s.pos_n, s.km_per_pixel = 127, 0.010
+        s.n_deldop = 64
+        s.ndel_min, s.ndel_max, s.ndop_min, s.ndop_max = 60, 200, 40, 160
+        s.dop_per_pixel = 0.0
+        s.epoch_span_days = 30.0
+        s.R_min, s.R_max, s.C_min, s.C_max = 0.1, 0.3, 1.0, 3.0
+        s.obs_lat_min_deg, s.obs_lat_max_deg = -20.0, 40.0
+    elif config == 3:
+        s.shape_kind, s.ico_level = SHAPE_ICOSPHERE, 6
+        s.rdev_sigma = 0.03
+        s.seed_geom, s.seed_frames = 3001, 3002
+        s.pos_n, s.km_per_pixel = 127, 0.010
+        s.n_deldop = 50
+        s.ndel_min, s.ndel_max, s.ndop_min, s.ndop_max = 50, 300, 30, 250
+        s.dop_per_pixel = 0.0
+        s.n_doppler = 40
+        s.nbin_min, s.nbin_max = 100, 400
+        s.dop_per_bin_min, s.dop_per_bin_max = 0.25, 2.0
+        s.n_lightcurves, s.lc_points, s.lc_ncalc = 20, 100, 40
+        s.epoch_span_days = 30.0
+        s.R_min, s.R_max, s.C_min, s.C_max = 0.1, 0.3, 1.0, 3.0
+        s.obs_lat_min_deg, s.obs_lat_max_deg = -20.0, 40.0
+    elif config == 4:
+        s.shape_kind, s.ico_level = SHAPE_ICOSPHERE, 7
+        s.semi_axes[0], s.semi_axes[1], s.semi_axes[2] = 0.9, 0.72, 0.54
+        s.rdev_sigma = 0.03
+        s.seed_geom, s.seed_frames = 4001, 4002
+        s.pos_n, s.km_per_pixel = 511, 0.002
+        s.n_deldop = 256
+        s.ndel_min, s.ndel_max, s.ndop_min, s.ndop_max = 128, 512, 64, 256
+        s.dop_per_pixel = 0.0
+        s.epoch_span_days = 30.0
+        s.R_min, s.R_max, s.C_min, s.C_max = 0.1, 0.3, 1.0, 3.0
+        s.obs_lat_min_deg, s.obs_lat_max_deg = -20.0, 40.0
+    elif config == 5:
+        s.shape_kind, s.ico_level = SHAPE_ICOSPHERE, 5
+        s.rdev_sigma = 0.03
+        s.seed_geom, s.seed_frames = 5000, 5500
+        s.pos_n, s.km_per_pixel = 127, 0.010
+        s.n_deldop = 24
+        s.ndel_min, s.ndel_max, s.ndop_min, s.ndop_max = 60, 250, 40, 160
+        s.dop_per_pixel = 0.0
+        s.epoch_span_days = 30.0
+        s.R_min, s.R_max, s.C_min, s.C_max = 0.1, 0.3, 1.0, 3.0
+        s.obs_lat_min_deg, s.obs_lat_max_deg = -20.0, 40.0
+    else:
+        raise ValueError("config must be 1..5")
+    for k, v in over.items():
+        if k == "semi_axes":
+            for i in range(3):
+                s.semi_axes[i] = v[i]
+        else:
+            setattr(s, k, v)
+    return s
+
+
+class Scene:
+    """Owns one synthetic scene (par/mod/dat ABI structs)."""
+
+    def __init__(self, spec):
+        self.lib = load_scene()
+        self.spec = spec
+        self.h = self.lib.shp_scene_create(C.byref(spec))
+        if not self.h:
+            raise RuntimeError("scene creation failed")
+        self.par = self.lib.shp_scene_par(self.h)
+        self.mod = self.lib.shp_scene_mod(self.h)
+        self.dat = self.lib.shp_scene_dat(self.h)
+
+    def close(self):
+        if self.h:
+            self.lib.shp_scene_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ---- shape ----
+    @property
+    def nv(self):
+        return self.lib.shp_scene_nv(self.h)
+
+    @property
+    def nf(self):
+        return self.lib.shp_scene_nf(self.h)
+
+    def geometry(self):
+        nv, nf = self.nv, self.nf
+        vx, vn = np.zeros(3 * nv), np.zeros(3 * nv)
+        fn, area = np.zeros(3 * nf), np.zeros(nf)
+        dp = C.POINTER(C.c_double)
+        self.lib.shp_geometry(self.mod, vx.ctypes.data_as(dp), vn.ctypes.data_as(dp),
+                              fn.ctypes.data_as(dp), area.ctypes.data_as(dp))
+        return vx.reshape(nv, 3), vn.reshape(nv, 3), fn.reshape(nf, 3), area
+
+    def facet_seen(self):
+        out = np.zeros(self.nf, dtype=np.uint8)
+        self.lib.shp_facet_seen(self.mod, out.ctypes.data_as(C.POINTER(C.c_ubyte)))
+        return out
+
+    # ---- data sets ----
+    @property
+    def nsets(self):
+        return self.lib.shp_dat_nsets(self.dat)
+
+    def set_type(self, s):
+        return self.lib.shp_set_type(self.dat, s)
+
+    def nframes(self, s):
+        return self.lib.shp_set_nframes(self.dat, s)
+
+    def dims(self, s, f):
+        out = (C.c_int * 2)()
+        self.lib.shp_frame_dims(self.dat, s, f, out)
+        return out[0], out[1]
+
+    def image(self, s, f, which=0):
+        """which: 0 fit, 1 obs, 2 oneovervar, 3 lightcurve y (calculated points)."""
+        t = self.set_type(s)
+        a, b = self.dims(s, f)
+        if t == DELAY:
+            out = np.zeros((a, b))
+        elif t == DOPPLER:
+            out = np.zeros(b)
+        else:
+            out = np.zeros(b if which == 3 else a)
+        self.lib.shp_frame_image(self.dat, s, f, which, out.ctypes.data_as(C.POINTER(C.c_double)))
+        return out
+
+    def set_image(self, s, f, which, arr):
+        arr = np.ascontiguousarray(arr, dtype=np.float64)
+        self.lib.shp_frame_set_image(self.dat, s, f, which, arr.ctypes.data_as(C.POINTER(C.c_double)))
+
+    def scalars(self, s, f):
+        out = np.zeros(18)
+        self.lib.shp_frame_scalars(self.dat, s, f, out.ctypes.data_as(C.POINTER(C.c_double)))
+        keys = ["chi2", "cal", "overflow_o2", "overflow_m2", "badradar_logfactor", "dof",
+                "idellim0", "idellim1", "idoplim0", "idoplim1", "dellim0", "dellim1",
+                "doplim0", "doplim1", "km2Hz", "overflow_xsec", "overflow_dopmean", "weight"]
+        return dict(zip(keys, out))
+
+    def pos(self, s, f):
+        return self.lib.shp_frame_pos(self.dat, s, f)
+
+    def pos_array(self, pos, which):
+        n = self.spec.pos_n
+        out = np.zeros((2 * n + 1, 2 * n + 1))
+        self.lib.shp_pos_array(pos, which, out.ctypes.data_as(C.POINTER(C.c_double)))
+        return out
+
+    def pos_limits(self, pos):
+        out = (C.c_int * 8)()
+        self.lib.shp_pos_limits(pos, out)
+        return list(out)
+
+    def view(self, s, f, v=0):
+        out = np.zeros(29)
+        self.lib.shp_view_data(self.dat, s, f, v, out.ctypes.data_as(C.POINTER(C.c_double)))
+        return out
+
+    def set_view(self, s, f, v, arr):
+        arr = np.ascontiguousarray(arr, dtype=np.float64)
+        self.lib.shp_set_view_data(self.dat, s, f, v, arr.ctypes.data_as(C.POINTER(C.c_double)))
+
+    def par_flags(self):
+        out = np.zeros(6)
+        self.lib.shp_par_flags(self.par, out.ctypes.data_as(C.POINTER(C.c_double)))
+        return dict(zip(["posbnd", "badradar", "baddiam", "posbnd_logfactor",
+                         "badradar_logfactor", "baddiam_logfactor"], out))
+
+    # ---- free parameters ----
+    def nparams(self):
+        return self.lib.shp_param_count(self.h)
+
+    def get_param(self, i):
+        return self.lib.shp_param_ptr(self.h, i)[0]
+
+    def set_param(self, i, val):
+        self.lib.shp_param_ptr(self.h, i)[0] = val
+
+    def param_type(self, i):
+        return self.lib.shp_param_type(self.h, i)
+
+    def synthesize_obs(self, snr=20.0, lc_rel_sigma=0.01, seed=1002):
+        self.lib.shp_scene_synthesize_obs(self.h, snr, lc_rel_sigma, seed)
+
+    def all_frames(self):
+        """(s, f) pairs for every radar frame and every lightcurve set (f = -1)."""
+        out = []
+        for s in range(self.nsets):
+            t = self.set_type(s)
+            if t == LGHTCRV:
+                out.append((s, -1))
+            else:
+                for f in range(self.nframes(s)):
+                    out.append((s, f))
+        return out
