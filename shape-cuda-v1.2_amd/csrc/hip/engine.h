@@ -1,0 +1,135 @@
+// engine.h -- device-resident model-evaluation engine.
+//
+// A scene is packed once from the SHAPE ABI structs into flat device arrays:
+//   * geometry slots  -- topology (facet vertex indices, vertex->facet CSR),
+//                        realization inputs (u, a, r) and realized outputs in
+//                        FP64 (x, vertex normals, facet normals) plus FP32
+//                        float4 copies for the mixed path;
+//   * views           -- one record per (evaluation, frame, view) or
+//                        (evaluation, lightcurve point), holding the observer
+//                        matrices and, after the setup kernel, the FP64
+//                        per-view constants (oa, Doppler gradient, COM shifts);
+//   * frames          -- one record per (evaluation, radar frame) or
+//                        (evaluation, lightcurve), with offsets into the
+//                        obs / weight / fit arenas;
+//   * parameter blocks-- one flat FP64 block per evaluation holding every
+//                        fittable scalar (spin state, scattering laws, delay
+//                        polynomials, scale factors); trial values are written
+//                        into these blocks.
+// Kernels never call back to the host; one evaluation batch is a fixed
+// sequence of launches on one stream, which can be captured into a hipGraph.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <vector>
+
+#include "shape_abi.h"
+
+namespace shp {
+
+enum Method { METHOD_FP64 = 0, METHOD_MIXED = 1 };
+
+// ---- flat parameter block layout (doubles) -----------------------------
+// spin block
+constexpr int PB_ANGLE = 0, PB_OMEGA = 3, PB_OMEGADOT = 6, PB_T0 = 9, PB_LIBAMP = 10,
+              PB_LIBFREQ = 11, PB_LIBPHASE = 12, PB_PA = 13, PB_NIMP = 14, PB_TIMP = 15,
+              PB_IMP = 25, PB_SPIN_END = 55;
+// per radar law (8): type, R, C, cos_cutoff, qsR, qsC, qs_cutoff, spare
+constexpr int PB_RAD_STRIDE = 8;
+// per optical law (6): type, R, wt, A0, D, k
+constexpr int PB_OPT_STRIDE = 6;
+// per set (19): angleoff[3], omegaoff[3], dopscale, delcor t0, delcor n, a[0..9]
+constexpr int PB_SET_STRIDE = 19, PB_SET_ANGLEOFF = 0, PB_SET_OMEGAOFF = 3, PB_SET_DOPSCALE = 6,
+              PB_SET_DCT0 = 7, PB_SET_DCN = 8, PB_SET_DCA = 9;
+// per model component (9): scalefactor[3], rot[3], off[3]
+constexpr int PB_COMP_STRIDE = 9;
+
+struct GeomSlot {
+  int nv, nf;
+  int model;             // model index (topology / inputs owner)
+  int comp_pb;           // parameter-block offset of scalefactor/rot/off
+  int eval;              // evaluation whose parameter block realizes this slot (-1: model base)
+  const int *fv;         // 3*nf
+  const int *af_off;     // nv+1
+  const int *af_idx;
+  const double *u, *a;   // 3*nv each
+  double *r;             // nv (slot-private copy: vertex deviations can be fit parameters)
+  double *x, *vn;        // 3*nv
+  double *fn, *area, *fx;// 3*nf, nf, 3*nf
+  float4 *x32, *vn32;    // nv
+  float4 *fn32;          // nf
+  unsigned char *seen;   // nf
+};
+
+// view types (same numbers as set_t types)
+constexpr int VT_DELAY = 0, VT_DOPPLER = 1, VT_LGHTCRV = 3;
+
+struct ViewRec {
+  // ---- description (host filled) ----
+  int type, set, frm, v, eval, slot, frame_id, is_v0, nviews;
+  int n;                 // POS half width
+  int smooth, bistatic, host_spin;  // host_spin: ae/spin/deloff/dopoff supplied by host
+  int exclude_seen;      // 1: do not mark facets seen from this view
+  long long pos_off;     // pixel offset of this view's POS buffers
+  int pb;                // parameter block of the evaluation
+  int pb_set, pb_law;    // offsets (within the block) of the set and scattering-law params
+  int pb_dc;             // offset of the set block whose delay polynomial applies (= states)
+  int law_type;
+  double kmpp, t;
+  double oe[9], se[9], orbspin[3];
+  double ae[9], spin[3], intspin[3], deloff, dopoff;
+  double delcom_vig, dopcom_vig, dopDC_vig, dopfftlen;
+  double Ftx, del_per_pixel, dop_per_pixel;   // dop_per_pixel = dop_per_bin for Doppler frames
+  int codemethod, spb, stride, ndel, ndop;
+  int nsinc2, sinc2width;
+  double frame_dof;
+  double phase, intensityfactor;                // lightcurves
+  // ---- computed by the setup kernel (FP64) ----
+  double oa[9], sa[9], usrc[3];
+  double ax, ay, delfact, delshift, dopshift, const1, const2, km2Hz, dop_extra;
+  double xincr, yincr, dopdiff_bl, dopdiff_max;
+  int badradar_sinc; double badradar_sinc_log;
+  int idop0;
+  // ---- accumulated by kernels ----
+  int xlim[2], ylim[2], xlim2[2], ylim2[2];
+  int outbnd, outbnd2, any_overflow, nstale;
+  unsigned long long ov[4], ov2[4];       // ordered doubles: imin, -imax, jmin, -jmax (min-reduced)
+  int idellim[2], idoplim[2];
+  unsigned long long dellim[2], doplim[2]; // ordered doubles: min delPOS, -max delPOS ...
+  double ysum;                             // lightcurve brightness sum
+};
+
+struct FrameRec {
+  int type, set, frm, eval, first_view, nviews;
+  int ndel, ndop;
+  long long off;        // offset into the fit arena
+  long long obs_off;    // offset into the obs / weight arenas (shared by evaluations)
+  int cal_floats;
+  double cal_fixed, weight, dof, o2;
+  double sdev_sq, lookfact;
+  int speckle;
+  // lightcurves
+  int lc_n, lc_ncalc, lc_nviews;
+  long long lc_x_off, lc_t_off; // offsets into lightcurve epoch arenas
+  long long lc_y_off;           // offset into the per-evaluation y arena (ncalc+1 doubles)
+  long long lc_y2_off;          // offset into the y2 arena (2*(ncalc+1) doubles: y2 and scratch)
+  // ---- outputs ----
+  double chi2, cal, m2, om;
+  double overflow_o2, overflow_m2, overflow_xsec, overflow_dopmean;
+  double badradar_logfactor;
+  int badradar;
+};
+
+struct EvalRec {
+  int model, pb, first_frame, nframes;
+  double dof, dof_deldop, dof_doppler, dof_lghtcrv;
+  double bad_objfactor;
+  // outputs
+  double chi2, objective;
+  int posbnd, badradar, baddiam;
+  double posbnd_logfactor, badradar_logfactor, baddiam_logfactor;
+};
+
+}  // namespace shp

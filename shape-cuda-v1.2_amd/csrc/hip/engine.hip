@@ -1,0 +1,873 @@
+// engine.hip -- packing of SHAPE ABI structs into device arenas and the
+// launch sequence of one evaluation batch.
+#include <math.h>
+#include <string.h>
+
+#include <algorithm>
+
+#include "common.h"
+#include "engine_host.h"
+#include "mixed.h"
+
+extern "C" int shp_fail(hipError_t e) { return (int)e == 0 ? -1 : (int)e; }
+
+namespace shp {
+
+static int g_last_err = 0;
+
+Engine::~Engine() {
+  if (graph_exec) hipGraphExecDestroy(graph_exec);
+  if (graph) hipGraphDestroy(graph);
+  for (void *p : allocs) hipFree(p);
+  if (stream) hipStreamDestroy(stream);
+}
+
+int Engine::alloc(void **p, size_t bytes) {
+  if (bytes == 0) bytes = 8;
+  SHP_CHECK(hipMalloc(p, bytes));
+  allocs.push_back(*p);
+  return 0;
+}
+
+template <class T>
+static int upload(Engine *E, T **dptr, const std::vector<T> &h) {
+  void *p = nullptr;
+  int rc = 0;
+  size_t bytes = h.size() * sizeof(T);
+  if (bytes == 0) bytes = sizeof(T);
+  if (hipMalloc(&p, bytes) != hipSuccess) return -2;
+  E->allocs.push_back(p);
+  if (!h.empty() && hipMemcpy(p, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice) != hipSuccess) rc = -3;
+  *dptr = (T *)p;
+  return rc;
+}
+
+// ---------------------------------------------------------------------------
+// per-model packing state
+// ---------------------------------------------------------------------------
+struct ModelPack {
+  std::vector<double> base;  // parameter block
+  int comp_off = 0;
+  // topology / inputs (host)
+  std::vector<int> fv, af_off, af_idx;
+  std::vector<double> u, a, r;
+  int nv = 0, nf = 0;
+  // realized geometry from the host (drop-in mode)
+  std::vector<double> hx, hvn, hfn;
+  // device topology
+  int *d_fv = nullptr, *d_af_off = nullptr, *d_af_idx = nullptr;
+  double *d_u = nullptr, *d_a = nullptr;
+  // view / frame templates
+  std::vector<ViewRec> views;
+  std::vector<FrameRec> frames;
+  std::vector<int> frame_first_view;  // template view index of each frame's first view
+  long long obs_off = 0;              // into the shared obs arena
+  std::vector<double> obs, oov;
+  std::vector<double> lc_x, lc_t;
+  long long lc_x_off = 0, lc_t_off = 0;
+  double dof = 0, dof_dd = 0, dof_dop = 0, dof_lc = 0;
+};
+
+static std::vector<ModelPack> g_packs;  // scratch used during build
+
+static void set_param_target(std::map<const double *, ParamTarget> &m, const double *addr, int model, int off) {
+  m[addr] = ParamTarget{0, model, off};
+}
+
+int Engine::pack_model(int mi, struct par_t *par, struct mod_t *mod, struct dat_t *dat) {
+  ModelPack &P = g_packs[mi];
+  auto &tmap = model_params[mi];
+  if (mod->shape.ncomp != 1) return -10;  // the engine handles single-component models
+  struct comp_t *comp = &mod->shape.comp[0];
+  struct vertices_t *rv = &comp->real;
+  if (!mod->spin.pa) return -11;          // principal-axis rotators only
+  // ---- parameter block layout ----
+  int nrad = mod->photo.nradlaws, nopt = mod->photo.noptlaws, nsets = dat->nsets;
+  int off_rad = PB_SPIN_END, off_opt = off_rad + PB_RAD_STRIDE * nrad;
+  int off_set = off_opt + PB_OPT_STRIDE * nopt, off_comp = off_set + PB_SET_STRIDE * nsets;
+  int size = off_comp + PB_COMP_STRIDE;
+  P.base.assign(size, 0.0);
+  P.comp_off = off_comp;
+  double *b = P.base.data();
+  struct spin_t *sp = &mod->spin;
+  for (int j = 0; j < 3; j++) {
+    b[PB_ANGLE + j] = sp->angle[j].val; set_param_target(tmap, &sp->angle[j].val, mi, PB_ANGLE + j);
+    b[PB_OMEGA + j] = sp->omega[j].val; set_param_target(tmap, &sp->omega[j].val, mi, PB_OMEGA + j);
+    b[PB_OMEGADOT + j] = sp->omegadot[j].val; set_param_target(tmap, &sp->omegadot[j].val, mi, PB_OMEGADOT + j);
+  }
+  b[PB_T0] = sp->t0;
+  b[PB_LIBAMP] = sp->lib_amp.val; set_param_target(tmap, &sp->lib_amp.val, mi, PB_LIBAMP);
+  b[PB_LIBFREQ] = sp->lib_freq.val; set_param_target(tmap, &sp->lib_freq.val, mi, PB_LIBFREQ);
+  b[PB_LIBPHASE] = sp->lib_phase.val; set_param_target(tmap, &sp->lib_phase.val, mi, PB_LIBPHASE);
+  b[PB_PA] = 1.0;
+  b[PB_NIMP] = sp->n_impulse;
+  for (int q = 0; q < sp->n_impulse && q < MAXIMP; q++) {
+    b[PB_TIMP + q] = sp->t_impulse[q];
+    for (int j = 0; j < 3; j++) {
+      b[PB_IMP + 3 * q + j] = sp->impulse[q][j].val;
+      set_param_target(tmap, &sp->impulse[q][j].val, mi, PB_IMP + 3 * q + j);
+    }
+  }
+  for (int l = 0; l < nrad; l++) {
+    double *pl = b + off_rad + PB_RAD_STRIDE * l;
+    union radscat_t *r = &mod->photo.radar[l];
+    int t = mod->photo.radtype[l];
+    pl[0] = t;
+    switch (t) {
+    case COSINELAW_DIFF:
+      pl[1] = r->RC.R.val; pl[2] = r->RC.C.val;
+      set_param_target(tmap, &r->RC.R.val, mi, off_rad + PB_RAD_STRIDE * l + 1);
+      set_param_target(tmap, &r->RC.C.val, mi, off_rad + PB_RAD_STRIDE * l + 2);
+      break;
+    case GAUSSIANLAW: case HAGFORSLAW: case COSINELAW_QS:
+      pl[1] = r->quasispec.R.val; pl[2] = r->quasispec.C.val; pl[3] = r->quasispec.cos_cutoff;
+      set_param_target(tmap, &r->quasispec.R.val, mi, off_rad + PB_RAD_STRIDE * l + 1);
+      set_param_target(tmap, &r->quasispec.C.val, mi, off_rad + PB_RAD_STRIDE * l + 2);
+      break;
+    case GAUSSIAN_COSINE: case HAGFORS_COSINE: case COSINE_COSINE:
+      pl[1] = r->hybrid.diff.R.val; pl[2] = r->hybrid.diff.C.val;
+      pl[4] = r->hybrid.qs.R.val; pl[5] = r->hybrid.qs.C.val; pl[6] = r->hybrid.qs.cos_cutoff;
+      set_param_target(tmap, &r->hybrid.diff.R.val, mi, off_rad + PB_RAD_STRIDE * l + 1);
+      set_param_target(tmap, &r->hybrid.diff.C.val, mi, off_rad + PB_RAD_STRIDE * l + 2);
+      set_param_target(tmap, &r->hybrid.qs.R.val, mi, off_rad + PB_RAD_STRIDE * l + 4);
+      set_param_target(tmap, &r->hybrid.qs.C.val, mi, off_rad + PB_RAD_STRIDE * l + 5);
+      break;
+    default:
+      pl[0] = -1;  // not evaluated by the engine; rejected below if a set uses it
+    }
+  }
+  for (int l = 0; l < nopt; l++) {
+    double *po = b + off_opt + PB_OPT_STRIDE * l;
+    union optscat_t *o = &mod->photo.optical[l];
+    int t = mod->photo.opttype[l];
+    po[0] = t;
+    int base = off_opt + PB_OPT_STRIDE * l;
+    if (t == LAMBERTLAW || t == LOMMEL || t == GEOMETRICAL) {
+      po[1] = o->R.R.val;
+      set_param_target(tmap, &o->R.R.val, mi, base + 1);
+    } else if (t == KAASALAINEN) {
+      po[1] = o->kaas.R.val; po[2] = o->kaas.wt.val; po[3] = o->kaas.A0.val; po[4] = o->kaas.D.val; po[5] = o->kaas.k.val;
+      set_param_target(tmap, &o->kaas.R.val, mi, base + 1);
+      set_param_target(tmap, &o->kaas.wt.val, mi, base + 2);
+      set_param_target(tmap, &o->kaas.A0.val, mi, base + 3);
+      set_param_target(tmap, &o->kaas.D.val, mi, base + 4);
+      set_param_target(tmap, &o->kaas.k.val, mi, base + 5);
+    } else {
+      po[0] = -1;
+    }
+  }
+  // sets: offsets ('=' states of the delay polynomial resolved to the source set)
+  std::vector<int> dc_src(nsets, -1);
+  int last_dc = -1;
+  for (int s = 0; s < nsets; s++) {
+    struct set_t *set = &dat->set[s];
+    double *ps = b + off_set + PB_SET_STRIDE * s;
+    int base = off_set + PB_SET_STRIDE * s;
+    for (int j = 0; j < 3; j++) {
+      ps[PB_SET_ANGLEOFF + j] = set->angleoff[j].val;
+      ps[PB_SET_OMEGAOFF + j] = set->omegaoff[j].val;
+      set_param_target(tmap, &set->angleoff[j].val, mi, base + PB_SET_ANGLEOFF + j);
+      set_param_target(tmap, &set->omegaoff[j].val, mi, base + PB_SET_OMEGAOFF + j);
+    }
+    struct delcor_t *dc = nullptr;
+    struct param_t *dsc = nullptr;
+    if (set->type == DELAY) { dc = &set->desc.deldop.delcor; dsc = &set->desc.deldop.dopscale; }
+    else if (set->type == DOPPLER) { dc = &set->desc.doppler.delcor; dsc = &set->desc.doppler.dopscale; }
+    if (dsc) { ps[PB_SET_DOPSCALE] = dsc->val; set_param_target(tmap, &dsc->val, mi, base + PB_SET_DOPSCALE); }
+    if (dc) {
+      if (dc->n > 9) return -12;
+      ps[PB_SET_DCT0] = dc->t0;
+      ps[PB_SET_DCN] = dc->n;
+      for (int q = 0; q <= dc->n; q++) {
+        ps[PB_SET_DCA + q] = dc->a[q].val;
+        set_param_target(tmap, &dc->a[q].val, mi, base + PB_SET_DCA + q);
+      }
+      if (dc->a[0].state != '=') { last_dc = s; dc_src[s] = s; }
+      else dc_src[s] = last_dc >= 0 ? last_dc : s;
+    }
+  }
+  // component: scale factors ('=' tie applied at pack time), rotation, offset
+  double *pc = b + off_comp;
+  for (int j = 0; j < 3; j++) {
+    pc[j] = comp->desc.ver.scalefactor[j].val;
+    if (j > 0 && comp->desc.ver.scalefactor[j].state == '=') pc[j] = pc[j - 1];
+    set_param_target(tmap, &comp->desc.ver.scalefactor[j].val, mi, off_comp + j);
+    pc[3 + j] = comp->rot[j].val; set_param_target(tmap, &comp->rot[j].val, mi, off_comp + 3 + j);
+    pc[6 + j] = comp->off[j].val; set_param_target(tmap, &comp->off[j].val, mi, off_comp + 6 + j);
+  }
+  // ---- topology and inputs ----
+  P.nv = rv->nv;
+  P.nf = rv->nf;
+  P.fv.resize(3 * (size_t)P.nf);
+  for (int f = 0; f < P.nf; f++)
+    for (int k = 0; k < 3; k++) P.fv[3 * f + k] = rv->f[f].v[k];
+  P.af_off.resize(P.nv + 1);
+  P.u.resize(3 * (size_t)P.nv);
+  P.a.resize(3 * (size_t)P.nv);
+  P.r.resize(P.nv);
+  P.af_off[0] = 0;
+  for (int i = 0; i < P.nv; i++) {
+    struct vertex_t *v = &rv->v[i];
+    P.af_off[i + 1] = P.af_off[i] + v->naf;
+    for (int q = 0; q < v->naf; q++) P.af_idx.push_back(v->af[q]);
+    for (int k = 0; k < 3; k++) { P.u[3 * i + k] = v->u[k]; P.a[3 * i + k] = v->a[k]; }
+    int src = v->r.state == '=' ? v->v_mirror : i;
+    P.r[i] = rv->v[src].r.val;
+    tmap[&v->r.val] = ParamTarget{1, mi, i};
+  }
+  if (opt.host_geometry) {
+    P.hx.resize(3 * (size_t)P.nv); P.hvn.resize(3 * (size_t)P.nv); P.hfn.resize(3 * (size_t)P.nf);
+    for (int i = 0; i < P.nv; i++)
+      for (int k = 0; k < 3; k++) { P.hx[3 * i + k] = rv->v[i].x[k]; P.hvn[3 * i + k] = rv->v[i].n[k]; }
+    for (int f = 0; f < P.nf; f++)
+      for (int k = 0; k < 3; k++) P.hfn[3 * f + k] = rv->f[f].n[k];
+  }
+  // ---- views and frames ----
+  auto fill_view_common = [&](ViewRec &V, struct pos_t *pos) {
+    memset(&V, 0, sizeof(V));
+    V.n = pos->n;
+    V.kmpp = pos->km_per_pixel;
+    V.smooth = par->pos_smooth;
+    V.host_spin = opt.host_spin;
+    V.nsinc2 = par->nsinc2;
+    V.sinc2width = par->sinc2width;
+  };
+  auto copy9 = [](double *d, double m[3][3]) { for (int i = 0; i < 3; i++) for (int j = 0; j < 3; j++) d[3 * i + j] = m[i][j]; };
+  for (int s = 0; s < nsets; s++) {
+    struct set_t *set = &dat->set[s];
+    int pb_set = off_set + PB_SET_STRIDE * s;
+    int pb_dc = dc_src[s] >= 0 ? off_set + PB_SET_STRIDE * dc_src[s] : pb_set;
+    if (set->type == DELAY || set->type == DOPPLER) {
+      bool dd = set->type == DELAY;
+      struct deldop_t *D = &set->desc.deldop;
+      struct doppler_t *Q = &set->desc.doppler;
+      int nfr = dd ? D->nframes : Q->nframes, nv = dd ? D->nviews : Q->nviews, v0 = dd ? D->v0 : Q->v0;
+      int ilaw = dd ? D->iradlaw : Q->iradlaw;
+      if (mod->photo.radtype[ilaw] < COSINELAW_DIFF || mod->photo.radtype[ilaw] > COSINE_COSINE) return -13;
+      for (int f = 0; f < nfr; f++) {
+        FrameRec F;
+        memset(&F, 0, sizeof(F));
+        F.type = set->type;
+        F.set = s;
+        F.frm = f;
+        F.nviews = nv;
+        F.first_view = (int)P.views.size();
+        struct pos_t *pos = dd ? &D->frame[f].pos : &Q->frame[f].pos;
+        for (int k = 0; k < nv; k++) {
+          ViewRec V;
+          fill_view_common(V, pos);
+          V.type = set->type;
+          V.set = s; V.frm = f; V.v = k; V.is_v0 = (k == v0); V.nviews = nv;
+          V.exclude_seen = (s == par->exclude_seen);
+          V.pb_set = pb_set; V.pb_dc = pb_dc;
+          V.pb_law = off_rad + PB_RAD_STRIDE * ilaw;
+          V.law_type = mod->photo.radtype[ilaw];
+          V.frame_id = (int)P.frames.size();
+          if (dd) {
+            struct deldopfrm_t *fr = &D->frame[f];
+            struct deldopview_t *w = &fr->view[k];
+            V.t = w->t; copy9(V.oe, w->oe); copy9(V.ae, w->ae);
+            for (int j = 0; j < 3; j++) { V.orbspin[j] = w->orbspin[j]; V.spin[j] = w->spin[j]; V.intspin[j] = w->intspin[j]; }
+            V.deloff = w->deloff; V.dopoff = w->dopoff;
+            V.delcom_vig = fr->delcom_vig; V.dopcom_vig = fr->dopcom_vig; V.dopDC_vig = fr->dopDC_vig;
+            V.dopfftlen = D->dopfftlen; V.Ftx = D->Ftx; V.del_per_pixel = D->del_per_pixel;
+            V.dop_per_pixel = D->dop_per_pixel; V.codemethod = D->codemethod; V.spb = D->spb;
+            V.stride = D->stride; V.ndel = fr->ndel; V.ndop = fr->ndop; V.frame_dof = fr->dof;
+          } else {
+            struct dopfrm_t *fr = &Q->frame[f];
+            struct dopview_t *w = &fr->view[k];
+            V.t = w->t; copy9(V.oe, w->oe); copy9(V.ae, w->ae);
+            for (int j = 0; j < 3; j++) { V.orbspin[j] = w->orbspin[j]; V.spin[j] = w->spin[j]; V.intspin[j] = w->intspin[j]; }
+            V.deloff = 0.0; V.dopoff = w->dopoff;
+            V.dopcom_vig = fr->dopcom_vig;
+            V.Ftx = Q->Ftx; V.dop_per_pixel = Q->dop_per_bin; V.ndel = 1; V.ndop = fr->ndop;
+            V.frame_dof = fr->dof; V.spb = 1; V.stride = 1; V.codemethod = LONG_ORIG;
+          }
+          P.views.push_back(V);
+        }
+        long long N;
+        if (dd) {
+          struct deldopfrm_t *fr = &D->frame[f];
+          F.ndel = fr->ndel; F.ndop = fr->ndop;
+          N = (long long)fr->ndel * fr->ndop;
+          F.obs_off = (long long)P.obs.size();
+          if (!opt.views_only)
+            for (int i = 1; i <= fr->ndel; i++)
+              for (int j = 1; j <= fr->ndop; j++) {
+                P.obs.push_back(fr->obs[i][j]);
+                P.oov.push_back(fr->oneovervar[i][j]);
+              }
+          F.cal_floats = fr->cal.state == 'f'; F.cal_fixed = fr->cal.val; F.weight = fr->weight; F.dof = fr->dof;
+          F.sdev_sq = fr->sdev * fr->sdev; F.lookfact = fr->nlooks > 0.0 ? 1.0 / fr->nlooks : 0.0;
+          F.overflow_o2 = fr->overflow_o2; F.overflow_m2 = fr->overflow_m2;
+        } else {
+          struct dopfrm_t *fr = &Q->frame[f];
+          F.ndel = 1; F.ndop = fr->ndop;
+          N = fr->ndop;
+          F.obs_off = (long long)P.obs.size();
+          if (!opt.views_only)
+            for (int j = 1; j <= fr->ndop; j++) {
+              P.obs.push_back(fr->obs[j]);
+              P.oov.push_back(fr->oneovervar[j]);
+            }
+          F.cal_floats = fr->cal.state == 'f'; F.cal_fixed = fr->cal.val; F.weight = fr->weight; F.dof = fr->dof;
+          F.sdev_sq = fr->sdev * fr->sdev; F.lookfact = fr->nlooks > 0.0 ? 1.0 / fr->nlooks : 0.0;
+          F.overflow_o2 = fr->overflow_o2; F.overflow_m2 = fr->overflow_m2;
+        }
+        F.speckle = par->speckle;
+        F.off = N;  // temporarily: element count
+        P.frames.push_back(F);
+      }
+    } else if (set->type == LGHTCRV) {
+      struct lghtcrv_t *L = &set->desc.lghtcrv;
+      int t = mod->photo.opttype[L->ioptlaw];
+      if (!(t == LAMBERTLAW || t == LOMMEL || t == GEOMETRICAL || t == KAASALAINEN)) return -14;
+      FrameRec F;
+      memset(&F, 0, sizeof(F));
+      F.type = VT_LGHTCRV;
+      F.set = s;
+      F.nviews = 1;
+      F.first_view = (int)P.views.size();
+      F.lc_n = L->n; F.lc_ncalc = L->ncalc; F.lc_nviews = L->nviews;
+      for (int i = 1; i <= L->ncalc; i++) {
+        struct crvrend_t *rd = &L->rend[i];
+        ViewRec V;
+        fill_view_common(V, &rd->pos);
+        V.type = VT_LGHTCRV;
+        V.set = s; V.frm = i - 1; V.v = 0; V.is_v0 = 1; V.nviews = 1;
+        V.exclude_seen = (s == par->exclude_seen);
+        V.bistatic = 1;
+        V.pb_set = pb_set; V.pb_dc = pb_dc;
+        V.pb_law = off_opt + PB_OPT_STRIDE * L->ioptlaw;
+        V.law_type = t;
+        V.frame_id = (int)P.frames.size();
+        V.t = L->x[i];
+        copy9(V.oe, rd->oe); copy9(V.se, rd->se); copy9(V.ae, rd->ae);
+        for (int j = 0; j < 3; j++) { V.orbspin[j] = rd->orbspin[j]; V.spin[j] = rd->spin[j]; V.intspin[j] = rd->intspin[j]; }
+        V.phase = L->solar_phase[i];
+        V.intensityfactor = pow(rd->pos.km_per_pixel / AU, 2.0);
+        P.views.push_back(V);
+      }
+      F.obs_off = (long long)P.obs.size();
+      if (!opt.views_only)
+        for (int i = 1; i <= L->n; i++) {
+          P.obs.push_back(L->obs[i]);
+          P.oov.push_back(L->oneovervar[i]);
+        }
+      F.cal_floats = L->cal.state == 'f'; F.cal_fixed = L->cal.val; F.weight = L->weight; F.dof = L->dof;
+      F.lc_x_off = (long long)P.lc_x.size();
+      P.lc_x.push_back(0.0);
+      for (int i = 1; i <= L->ncalc; i++) P.lc_x.push_back(L->x[i]);
+      F.lc_t_off = (long long)P.lc_t.size();
+      for (int i = 1; i <= L->n; i++)
+        for (int v = 0; v < L->nviews; v++) P.lc_t.push_back(L->t[i][v]);
+      F.off = L->n;
+      P.frames.push_back(F);
+    } else {
+      return -15;  // plane-of-sky sets are outside this path
+    }
+  }
+  P.dof = dat->dof; P.dof_dd = dat->dof_deldop; P.dof_dop = dat->dof_doppler; P.dof_lc = dat->dof_lghtcrv;
+  return 0;
+}
+
+int Engine::build(struct par_t *par, struct mod_t **mods, struct dat_t **dats, int nmodels,
+                  const std::vector<EvalSpec> &evals, const RunOptions &o) {
+  opt = o;
+  nsinc2 = par->nsinc2;
+  sinc2width = par->sinc2width;
+  mask_tol = par->mask_tol;
+  mincosine_seen = par->mincosine_seen;
+  exclude_seen = par->exclude_seen;
+  if (!stream) SHP_CHECK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+  g_packs.assign(nmodels, ModelPack());
+  model_params.assign(nmodels, {});
+  int rc;
+  for (int m = 0; m < nmodels; m++)
+    if ((rc = pack_model(m, par, mods[m], dats[m])) != 0) return rc;
+  nevals = (int)evals.size();
+  pb_stride = 0;
+  for (auto &P : g_packs) pb_stride = std::max(pb_stride, (int)P.base.size());
+  pb_stride = (pb_stride + 7) & ~7;
+  // parameter blocks: evaluations first, then one base block per model
+  h_params.assign((size_t)pb_stride * (nevals + nmodels), 0.0);
+  for (int m = 0; m < nmodels; m++)
+    std::copy(g_packs[m].base.begin(), g_packs[m].base.end(), h_params.begin() + (size_t)pb_stride * (nevals + m));
+  // per-model device topology
+  for (int m = 0; m < nmodels; m++) {
+    ModelPack &P = g_packs[m];
+    if ((rc = upload(this, &P.d_fv, P.fv))) return rc;
+    if ((rc = upload(this, &P.d_af_off, P.af_off))) return rc;
+    if ((rc = upload(this, &P.d_af_idx, P.af_idx))) return rc;
+    if ((rc = upload(this, &P.d_u, P.u))) return rc;
+    if ((rc = upload(this, &P.d_a, P.a))) return rc;
+  }
+  // evaluations, their slots, views and frames
+  eval_targets.assign(nevals, {});
+  h_slots.clear();
+  std::vector<std::vector<double>> slot_r;
+  for (int m = 0; m < nmodels; m++) {  // base slots
+    GeomSlot G;
+    memset(&G, 0, sizeof(G));
+    G.model = m; G.eval = -1; G.nv = g_packs[m].nv; G.nf = g_packs[m].nf;
+    G.comp_pb = pb_stride * (nevals + m) + g_packs[m].comp_off;
+    h_slots.push_back(G);
+    slot_r.push_back(g_packs[m].r);
+  }
+  h_eval_slot.assign(nevals, 0);
+  h_views.clear();
+  h_frames.clear();
+  h_evals.clear();
+  long long fit_cursor = 0, pos_cursor = 0, y_cursor = 0, y2_cursor = 0;
+  std::vector<long long> model_obs_base(nmodels, 0), model_x_base(nmodels, 0), model_t_base(nmodels, 0);
+  long long ob = 0, xb = 0, tb = 0;
+  for (int m = 0; m < nmodels; m++) {
+    model_obs_base[m] = ob; ob += (long long)g_packs[m].obs.size();
+    model_x_base[m] = xb; xb += (long long)g_packs[m].lc_x.size();
+    model_t_base[m] = tb; tb += (long long)g_packs[m].lc_t.size();
+  }
+  for (int e = 0; e < nevals; e++) {
+    const EvalSpec &S = evals[e];
+    int m = S.model;
+    ModelPack &P = g_packs[m];
+    int pb = pb_stride * e;
+    std::copy(P.base.begin(), P.base.end(), h_params.begin() + pb);
+    bool own_slot = false;
+    std::vector<double> r;
+    for (size_t q = 0; q < S.targets.size(); q++) {
+      auto it = model_params[m].find(S.targets[q]);
+      if (it == model_params[m].end()) return -16;  // not a parameter of this model
+      ParamTarget T = it->second;
+      eval_targets[e].push_back(T);
+      if (T.kind == 0) {
+        h_params[pb + T.offset] = S.values[q];
+        if (T.offset >= P.comp_off) own_slot = true;
+      } else {
+        if (!own_slot || r.empty()) r = P.r;
+        own_slot = true;
+      }
+    }
+    if (own_slot) {
+      if (r.empty()) r = P.r;
+      for (size_t q = 0; q < S.targets.size(); q++)
+        if (eval_targets[e][q].kind == 1) r[eval_targets[e][q].offset] = S.values[q];
+      GeomSlot G;
+      memset(&G, 0, sizeof(G));
+      G.model = m; G.eval = e; G.nv = P.nv; G.nf = P.nf;
+      G.comp_pb = pb + P.comp_off;
+      h_eval_slot[e] = (int)h_slots.size();
+      h_slots.push_back(G);
+      slot_r.push_back(r);
+    } else {
+      h_eval_slot[e] = m;
+    }
+    EvalRec E;
+    memset(&E, 0, sizeof(E));
+    E.model = m; E.pb = pb;
+    E.first_frame = (int)h_frames.size();
+    E.nframes = (int)P.frames.size();
+    E.dof = P.dof; E.dof_deldop = P.dof_dd; E.dof_doppler = P.dof_dop; E.dof_lghtcrv = P.dof_lc;
+    E.bad_objfactor = par->bad_objfactor;
+    h_evals.push_back(E);
+    int view_base = (int)h_views.size(), frame_base = (int)h_frames.size();
+    for (size_t q = 0; q < P.views.size(); q++) {
+      ViewRec V = P.views[q];
+      V.eval = e;
+      V.slot = h_eval_slot[e];
+      V.pb = pb;
+      V.frame_id += frame_base;
+      V.pos_off = pos_cursor;
+      pos_cursor += (long long)(2 * V.n + 1) * (2 * V.n + 1);
+      max_pix = std::max(max_pix, (2 * V.n + 1) * (2 * V.n + 1));
+      if (V.type == VT_LGHTCRV) any_lc = true;
+      if (V.type == VT_DELAY) any_dd = true;
+      if (V.type == VT_DOPPLER) any_dop = true;
+      h_views.push_back(V);
+    }
+    for (size_t q = 0; q < P.frames.size(); q++) {
+      FrameRec F = P.frames[q];
+      long long N = F.off;
+      F.eval = e;
+      F.first_view += view_base;
+      F.off = fit_cursor;
+      fit_cursor += N;
+      F.obs_off += model_obs_base[m];
+      if (F.type == VT_LGHTCRV) {
+        F.lc_x_off += model_x_base[m];
+        F.lc_t_off += model_t_base[m];
+        F.lc_y_off = y_cursor; y_cursor += F.lc_ncalc + 1;
+        F.lc_y2_off = y2_cursor; y2_cursor += 2 * (F.lc_ncalc + 1);
+      }
+      h_frames.push_back(F);
+    }
+  }
+  nviews = (int)h_views.size();
+  nframes = (int)h_frames.size();
+  nslots = (int)h_slots.size();
+  total_pix = pos_cursor;
+  fit_len = fit_cursor;
+  // ---- device arenas ----
+  max_nf = 0;
+  for (int s = 0; s < nslots; s++) {
+    GeomSlot &G = h_slots[s];
+    ModelPack &P = g_packs[G.model];
+    max_nf = std::max(max_nf, G.nf);
+    G.fv = P.d_fv; G.af_off = P.d_af_off; G.af_idx = P.d_af_idx; G.u = P.d_u; G.a = P.d_a;
+    if ((rc = upload(this, &G.r, slot_r[s]))) return rc;
+    if ((rc = alloc((void **)&G.x, sizeof(double) * 3 * G.nv))) return rc;
+    if ((rc = alloc((void **)&G.vn, sizeof(double) * 3 * G.nv))) return rc;
+    if ((rc = alloc((void **)&G.fn, sizeof(double) * 3 * G.nf))) return rc;
+    if ((rc = alloc((void **)&G.fx, sizeof(double) * 3 * G.nf))) return rc;
+    if ((rc = alloc((void **)&G.area, sizeof(double) * G.nf))) return rc;
+    if ((rc = alloc((void **)&G.x32, sizeof(float4) * G.nv))) return rc;
+    if ((rc = alloc((void **)&G.vn32, sizeof(float4) * G.nv))) return rc;
+    if ((rc = alloc((void **)&G.fn32, sizeof(float4) * G.nf))) return rc;
+    if ((rc = alloc((void **)&G.seen, G.nf))) return rc;
+    SHP_CHECK(hipMemset(G.seen, 0, G.nf));
+    if (opt.host_geometry) {
+      SHP_CHECK(hipMemcpy(G.x, P.hx.data(), sizeof(double) * 3 * G.nv, hipMemcpyHostToDevice));
+      SHP_CHECK(hipMemcpy(G.vn, P.hvn.data(), sizeof(double) * 3 * G.nv, hipMemcpyHostToDevice));
+      SHP_CHECK(hipMemcpy(G.fn, P.hfn.data(), sizeof(double) * 3 * G.nf, hipMemcpyHostToDevice));
+      std::vector<float4> t4(std::max(G.nv, G.nf));
+      for (int i = 0; i < G.nv; i++) t4[i] = make_float4((float)P.hx[3 * i], (float)P.hx[3 * i + 1], (float)P.hx[3 * i + 2], 0.f);
+      SHP_CHECK(hipMemcpy(G.x32, t4.data(), sizeof(float4) * G.nv, hipMemcpyHostToDevice));
+      for (int i = 0; i < G.nv; i++) t4[i] = make_float4((float)P.hvn[3 * i], (float)P.hvn[3 * i + 1], (float)P.hvn[3 * i + 2], 0.f);
+      SHP_CHECK(hipMemcpy(G.vn32, t4.data(), sizeof(float4) * G.nv, hipMemcpyHostToDevice));
+      for (int i = 0; i < G.nf; i++) t4[i] = make_float4((float)P.hfn[3 * i], (float)P.hfn[3 * i + 1], (float)P.hfn[3 * i + 2], 0.f);
+      SHP_CHECK(hipMemcpy(G.fn32, t4.data(), sizeof(float4) * G.nf, hipMemcpyHostToDevice));
+    }
+  }
+  if ((rc = upload(this, &d_slots, h_slots))) return rc;
+  if ((rc = upload(this, &d_views, h_views))) return rc;
+  if ((rc = upload(this, &d_frames, h_frames))) return rc;
+  if ((rc = upload(this, &d_evals, h_evals))) return rc;
+  if ((rc = upload(this, &d_eval_slot, h_eval_slot))) return rc;
+  if ((rc = upload(this, &d_params, h_params))) return rc;
+  if (opt.views_only) { g_packs.clear(); return 0; }
+  if ((rc = alloc((void **)&d_slot_flags, sizeof(int) * nslots))) return rc;
+  if ((rc = alloc((void **)&d_slot_logf, sizeof(double) * nslots))) return rc;
+  SHP_CHECK(hipMemset(d_slot_flags, 0, sizeof(int) * nslots));
+  SHP_CHECK(hipMemset(d_slot_logf, 0, sizeof(double) * nslots));
+  std::vector<double> obs_all, oov_all, x_all, t_all;
+  for (auto &P : g_packs) {
+    obs_all.insert(obs_all.end(), P.obs.begin(), P.obs.end());
+    oov_all.insert(oov_all.end(), P.oov.begin(), P.oov.end());
+    x_all.insert(x_all.end(), P.lc_x.begin(), P.lc_x.end());
+    t_all.insert(t_all.end(), P.lc_t.begin(), P.lc_t.end());
+  }
+  obs_len = (long long)obs_all.size();
+  if ((rc = upload(this, &d_obs, obs_all))) return rc;
+  if ((rc = upload(this, &d_oov, oov_all))) return rc;
+  if ((rc = upload(this, &d_lc_x, x_all))) return rc;
+  if ((rc = upload(this, &d_lc_t, t_all))) return rc;
+  if ((rc = alloc((void **)&d_lc_y, sizeof(double) * std::max(1LL, y_cursor)))) return rc;
+  if ((rc = alloc((void **)&d_lc_y2, sizeof(double) * std::max(1LL, y2_cursor)))) return rc;
+  if ((rc = alloc((void **)&d_fit, sizeof(double) * std::max(1LL, fit_len)))) return rc;
+  SHP_CHECK(hipMemset(d_fit, 0, sizeof(double) * std::max(1LL, fit_len)));
+  if ((rc = alloc((void **)&d_overflow, sizeof(double) * 2000 * (size_t)std::max(1, nviews)))) return rc;
+  SHP_CHECK(hipMemset(d_overflow, 0, sizeof(double) * 2000 * (size_t)std::max(1, nviews)));
+  if ((rc = alloc((void **)&d_view_out, sizeof(double) * VIEW_OUT * (size_t)std::max(1, nviews)))) return rc;
+  // mixed-path frame data (FP32 obs*w and w)
+  if (opt.method == METHOD_MIXED) {
+    std::vector<float> ow(obs_len), w32(obs_len);
+    for (long long q = 0; q < obs_len; q++) { ow[q] = (float)(obs_all[q] * oov_all[q]); w32[q] = (float)oov_all[q]; }
+    if ((rc = upload(this, &d_obs32, ow))) return rc;
+    if ((rc = upload(this, &d_oov32, w32))) return rc;
+  }
+  // POS buffers
+  size_t P = (size_t)std::max(1LL, total_pix);
+  memset(&pd, 0, sizeof(pd));
+  if ((rc = alloc((void **)&pd.zkey, sizeof(unsigned long long) * P))) return rc;
+  SHP_CHECK(hipMemset(pd.zkey, 0, sizeof(unsigned long long) * P));
+  if (opt.method == METHOD_FP64 || any_lc) {
+    if ((rc = alloc((void **)&pd.fbuf, sizeof(int) * P))) return rc;
+    if ((rc = alloc((void **)&pd.z, sizeof(double) * P))) return rc;
+    if ((rc = alloc((void **)&pd.cose, sizeof(double) * P))) return rc;
+    if ((rc = alloc((void **)&pd.f, sizeof(int) * P))) return rc;
+    if ((rc = alloc((void **)&pd.stale_idx, sizeof(int) * P))) return rc;
+    hipLaunchKernelGGL(k_fill_i32, dim3(1024), dim3(256), 0, stream, pd.fbuf, (long long)P, 0x7fffffff);
+    hipLaunchKernelGGL(k_fill_i32, dim3(1024), dim3(256), 0, stream, pd.stale_idx, (long long)P, -1);
+    hipLaunchKernelGGL(k_fill_i32, dim3(1024), dim3(256), 0, stream, pd.f, (long long)P, -1);
+    SHP_CHECK(hipMemsetAsync(pd.cose, 0, sizeof(double) * P, stream));
+    if (any_lc) {
+      if ((rc = alloc((void **)&pd.cosi, sizeof(double) * P))) return rc;
+      if ((rc = alloc((void **)&pd.b, sizeof(double) * P))) return rc;
+      if ((rc = alloc((void **)&pd.zill, sizeof(double) * P))) return rc;
+      if ((rc = alloc((void **)&pd.cosill, sizeof(double) * P))) return rc;
+      if ((rc = alloc((void **)&pd.fill, sizeof(int) * P))) return rc;
+      SHP_CHECK(hipMemsetAsync(pd.cosi, 0, sizeof(double) * P, stream));
+      SHP_CHECK(hipMemsetAsync(pd.b, 0, sizeof(double) * P, stream));
+    }
+    if (opt.with_body) {
+      if ((rc = alloc((void **)&pd.body, sizeof(int) * P))) return rc;
+      if ((rc = alloc((void **)&pd.comp, sizeof(int) * P))) return rc;
+      hipLaunchKernelGGL(k_fill_i32, dim3(1024), dim3(256), 0, stream, pd.body, (long long)P, -1);
+      hipLaunchKernelGGL(k_fill_i32, dim3(1024), dim3(256), 0, stream, pd.comp, (long long)P, -1);
+      if (any_lc) {
+        if ((rc = alloc((void **)&pd.bodyill, sizeof(int) * P))) return rc;
+        if ((rc = alloc((void **)&pd.compill, sizeof(int) * P))) return rc;
+      }
+    }
+    pd.stale_cap = (int)std::min<long long>(std::max<long long>(P / 8, 4096), 4LL << 20);
+    if ((rc = alloc((void **)&pd.stale, sizeof(StaleRec) * (size_t)pd.stale_cap))) return rc;
+  }
+  if ((rc = alloc((void **)&pd.nstale, sizeof(int) * 4))) return rc;
+  pd.error = pd.nstale + 1;
+  SHP_CHECK(hipMemset(pd.nstale, 0, sizeof(int) * 4));
+  hipLaunchKernelGGL(k_frame_o2, dim3(std::max(1, nframes)), dim3(256), 0, stream, d_frames, nframes, d_obs, d_oov);
+  SHP_CHECK(hipStreamSynchronize(stream));
+  g_packs.clear();
+  return 0;
+}
+
+int Engine::set_values(int e, const std::vector<double> &values) {
+  if (e < 0 || e >= nevals || values.size() != eval_targets[e].size()) return -20;
+  for (size_t q = 0; q < values.size(); q++) {
+    const ParamTarget &T = eval_targets[e][q];
+    if (T.kind == 0) h_params[(size_t)pb_stride * e + T.offset] = values[q];
+    else {
+      int sl = h_eval_slot[e];
+      SHP_CHECK(hipMemcpyAsync(h_slots[sl].r + T.offset, &values[q], sizeof(double), hipMemcpyHostToDevice, stream));
+      SHP_CHECK(hipStreamSynchronize(stream));
+    }
+  }
+  return 0;
+}
+
+int Engine::upload_params() {
+  SHP_CHECK(hipMemcpyAsync(d_params, h_params.data(), sizeof(double) * h_params.size(), hipMemcpyHostToDevice, stream));
+  return 0;
+}
+
+static inline unsigned cdiv(long long a, long long b) { return (unsigned)((a + b - 1) / b); }
+
+int Engine::enqueue() {
+  const int T = 256;
+  dim3 gv(cdiv(nviews, 64)), bv(64);
+  hipLaunchKernelGGL(k_zero_counters, dim3(1), dim3(64), 0, stream, pd.nstale, 4);
+  hipLaunchKernelGGL(k_setup_views, gv, bv, 0, stream, d_views, nviews, d_params);
+  if (!opt.host_geometry) {
+    int max_nv = 0;
+    for (auto &G : h_slots) max_nv = std::max(max_nv, G.nv);
+    hipLaunchKernelGGL(k_realize_vertices, dim3(cdiv(max_nv, T), nslots), dim3(T), 0, stream, d_slots, nslots,
+                       d_params, d_slot_flags, d_slot_logf);
+    hipLaunchKernelGGL(k_realize_facets, dim3(cdiv(max_nf, T), nslots), dim3(T), 0, stream, d_slots, nslots);
+    hipLaunchKernelGGL(k_realize_vnormals, dim3(cdiv(max_nv, T), nslots), dim3(T), 0, stream, d_slots, nslots);
+  }
+  if (opt.method == METHOD_MIXED) return enqueue_mixed(*this);
+  dim3 gf(cdiv(max_nf, T), nviews), gp(cdiv(max_pix, T), nviews), bt(T);
+  // observer view
+  hipLaunchKernelGGL(k_raster64, gf, bt, 0, stream, d_views, d_slots, pd, 0, 1);
+  hipLaunchKernelGGL(k_raster64, gf, bt, 0, stream, d_views, d_slots, pd, 0, 2);
+  hipLaunchKernelGGL(k_resolve64, gp, bt, 0, stream, d_views, d_slots, pd, 0, 0, 1, 0, 0);
+  hipLaunchKernelGGL(k_raster64, gf, bt, 0, stream, d_views, d_slots, pd, 0, 3);
+  hipLaunchKernelGGL(k_replay64, dim3(cdiv(pd.stale_cap, T)), bt, 0, stream, d_views, d_slots, pd);
+  if (any_lc) {
+    hipLaunchKernelGGL(k_zero_counters, dim3(1), dim3(64), 0, stream, pd.nstale, 1);
+    hipLaunchKernelGGL(k_raster64, gf, bt, 0, stream, d_views, d_slots, pd, 1, 1);
+    hipLaunchKernelGGL(k_raster64, gf, bt, 0, stream, d_views, d_slots, pd, 1, 2);
+    hipLaunchKernelGGL(k_resolve64, gp, bt, 0, stream, d_views, d_slots, pd, 1, 0, 1, 0, 0);
+    hipLaunchKernelGGL(k_posmask64, gp, bt, 0, stream, d_views, pd, mask_tol);
+    hipLaunchKernelGGL(k_photo64, gp, bt, 0, stream, d_views, pd, d_params);
+  }
+  hipLaunchKernelGGL(k_mark_seen, gp, bt, 0, stream, d_views, d_slots, pd, mincosine_seen);
+  if (any_dd) hipLaunchKernelGGL(k_deldop64, gp, bt, 0, stream, d_views, d_frames, pd, d_fit, d_params);
+  if (any_dop) hipLaunchKernelGGL(k_doppler64, gp, bt, 0, stream, d_views, d_frames, pd, d_fit, d_overflow, d_params);
+  hipLaunchKernelGGL(k_finish_views, gv, bv, 0, stream, d_views, nviews, d_overflow, d_frames, d_view_out);
+  if (any_lc)
+    hipLaunchKernelGGL(k_spline, dim3(nframes), bt, 0, stream, d_frames, nframes, d_view_out, d_views, d_lc_x,
+                       d_lc_t, d_lc_y, d_lc_y2, d_fit);
+  hipLaunchKernelGGL(k_chi2_frames64, dim3(nframes), bt, 0, stream, d_frames, nframes, d_fit, d_obs, d_oov,
+                     d_view_out, opt.keep_fit ? 0 : 1, 0);
+  hipLaunchKernelGGL(k_objective, dim3(cdiv(nevals, 64)), dim3(64), 0, stream, d_evals, nevals, d_frames, d_views,
+                     d_view_out, d_slot_flags, d_slot_logf, d_eval_slot);
+  SHP_CHECK(hipGetLastError());
+  return 0;
+}
+
+int Engine::capture_graph() {
+  if (graph_exec) { hipGraphExecDestroy(graph_exec); graph_exec = nullptr; }
+  if (graph) { hipGraphDestroy(graph); graph = nullptr; }
+  SHP_CHECK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+  int rc = upload_params();
+  if (!rc) rc = enqueue();
+  hipError_t e = hipStreamEndCapture(stream, &graph);
+  if (rc) return rc;
+  SHP_CHECK(e);
+  SHP_CHECK(hipGraphInstantiate(&graph_exec, graph, nullptr, nullptr, 0));
+  return 0;
+}
+
+int Engine::launch_graph() {
+  SHP_CHECK(hipGraphLaunch(graph_exec, stream));
+  return 0;
+}
+
+int Engine::sync() {
+  SHP_CHECK(hipStreamSynchronize(stream));
+  return 0;
+}
+
+int Engine::download_evals(std::vector<EvalRec> &out) {
+  out.resize(nevals);
+  SHP_CHECK(hipMemcpyAsync(out.data(), d_evals, sizeof(EvalRec) * nevals, hipMemcpyDeviceToHost, stream));
+  SHP_CHECK(hipStreamSynchronize(stream));
+  return 0;
+}
+int Engine::download_frames(std::vector<FrameRec> &out) {
+  out.resize(nframes);
+  SHP_CHECK(hipMemcpyAsync(out.data(), d_frames, sizeof(FrameRec) * nframes, hipMemcpyDeviceToHost, stream));
+  SHP_CHECK(hipStreamSynchronize(stream));
+  return 0;
+}
+int Engine::download_view_out(std::vector<double> &out) {
+  out.resize((size_t)VIEW_OUT * nviews);
+  SHP_CHECK(hipMemcpyAsync(out.data(), d_view_out, sizeof(double) * out.size(), hipMemcpyDeviceToHost, stream));
+  SHP_CHECK(hipStreamSynchronize(stream));
+  return 0;
+}
+int Engine::download_views(std::vector<ViewRec> &out) {
+  out.resize(nviews);
+  SHP_CHECK(hipMemcpyAsync(out.data(), d_views, sizeof(ViewRec) * nviews, hipMemcpyDeviceToHost, stream));
+  SHP_CHECK(hipStreamSynchronize(stream));
+  return 0;
+}
+int Engine::error_flags() {
+  int h[4] = {0, 0, 0, 0};
+  if (hipMemcpy(h, pd.nstale, sizeof(int) * 4, hipMemcpyDeviceToHost) != hipSuccess) return -1;
+  return h[1];
+}
+
+// copy a device POS array of one view into an NR [-n..n][-n..n] host matrix
+template <class T>
+static int pos_to_host(hipStream_t st, T **dst, const T *src, long long off, int n) {
+  int w = 2 * n + 1;
+  std::vector<T> tmp((size_t)w * w);
+  SHP_CHECK(hipMemcpyAsync(tmp.data(), src + off, sizeof(T) * tmp.size(), hipMemcpyDeviceToHost, st));
+  SHP_CHECK(hipStreamSynchronize(st));
+  for (int i = -n; i <= n; i++) memcpy(&dst[i][-n], &tmp[(size_t)(i + n) * w], sizeof(T) * w);
+  return 0;
+}
+
+int Engine::writeback(int e, struct par_t *par, struct mod_t *mod, struct dat_t *dat, int write_pos) {
+  std::vector<FrameRec> fr;
+  std::vector<EvalRec> ev;
+  std::vector<double> vo;
+  std::vector<ViewRec> vw;
+  int rc;
+  if ((rc = download_frames(fr)) || (rc = download_evals(ev)) || (rc = download_view_out(vo)) ||
+      (rc = download_views(vw)))
+    return rc;
+  std::vector<double> fit((size_t)std::max(1LL, fit_len));
+  SHP_CHECK(hipMemcpy(fit.data(), d_fit, sizeof(double) * fit_len, hipMemcpyDeviceToHost));
+  const EvalRec &E = ev[e];
+  for (int q = 0; q < E.nframes; q++) {
+    const FrameRec &F = fr[E.first_frame + q];
+    struct set_t *set = &dat->set[F.set];
+    const double *img = fit.data() + F.off;
+    double inv = 1.0 / F.nviews;
+    if (F.type == VT_DELAY) {
+      struct deldopfrm_t *h = &set->desc.deldop.frame[F.frm];
+      for (int i = 1; i <= F.ndel; i++)
+        for (int j = 1; j <= F.ndop; j++) {
+          double v = img[(size_t)(i - 1) * F.ndop + (j - 1)];
+          h->fit[i][j] = F.nviews > 1 ? v * inv : v;
+        }
+      h->overflow_o2 = h->overflow_m2 = h->overflow_xsec = h->overflow_delmean = h->overflow_dopmean = 0.0;
+      h->badradar_logfactor = 0.0;
+    } else if (F.type == VT_DOPPLER) {
+      struct dopfrm_t *h = &set->desc.doppler.frame[F.frm];
+      for (int j = 1; j <= F.ndop; j++) h->fit[j] = F.nviews > 1 ? img[j - 1] * inv : img[j - 1];
+      h->overflow_o2 = F.overflow_o2; h->overflow_m2 = F.overflow_m2;
+      h->overflow_xsec = F.overflow_xsec; h->overflow_dopmean = F.overflow_dopmean;
+    } else if (F.type == VT_LGHTCRV) {
+      struct lghtcrv_t *L = &set->desc.lghtcrv;
+      for (int i = 1; i <= F.lc_n; i++) L->fit[i] = img[i - 1];
+      std::vector<double> y(F.lc_ncalc + 1), y2(2 * (F.lc_ncalc + 1));
+      SHP_CHECK(hipMemcpy(y.data(), d_lc_y + F.lc_y_off, sizeof(double) * y.size(), hipMemcpyDeviceToHost));
+      SHP_CHECK(hipMemcpy(y2.data(), d_lc_y2 + F.lc_y2_off, sizeof(double) * y2.size(), hipMemcpyDeviceToHost));
+      for (int i = 1; i <= F.lc_ncalc; i++) { L->y[i] = y[i]; L->y2[i] = y2[i]; }
+    }
+    // view-level side effects; the frame keeps the values of view v0
+    for (int v = 0; v < F.nviews && F.type != VT_LGHTCRV; v++) {
+      const ViewRec &V = vw[F.first_view + v];
+      const double *o = vo.data() + (size_t)(F.first_view + v) * VIEW_OUT;
+      if (F.type == VT_DELAY) {
+        struct deldopfrm_t *h = &set->desc.deldop.frame[F.frm];
+        struct deldopview_t *w = &h->view[V.v];
+        w->km2Hz = o[OUT_KM2HZ];
+        if (!opt.host_spin) {
+          for (int i = 0; i < 3; i++) for (int j = 0; j < 3; j++) w->ae[i][j] = V.ae[3 * i + j];
+          for (int j = 0; j < 3; j++) { w->spin[j] = V.spin[j]; w->intspin[j] = V.intspin[j]; }
+          w->deloff = V.deloff; w->dopoff = V.dopoff;
+        }
+        if (V.is_v0) {
+          h->idellim[0] = (int)o[OUT_IDELLIM]; h->idellim[1] = (int)o[OUT_IDELLIM + 1];
+          h->idoplim[0] = (int)o[OUT_IDOPLIM]; h->idoplim[1] = (int)o[OUT_IDOPLIM + 1];
+          h->dellim[0] = o[OUT_DELLIM]; h->dellim[1] = o[OUT_DELLIM + 1];
+          h->doplim[0] = o[OUT_DOPLIM]; h->doplim[1] = o[OUT_DOPLIM + 1];
+          h->badradar_logfactor = o[OUT_BADRADAR_LOG];
+          h->pos.posbnd_logfactor = o[OUT_POSBND_LOG];
+        }
+      } else {
+        struct dopfrm_t *h = &set->desc.doppler.frame[F.frm];
+        struct dopview_t *w = &h->view[V.v];
+        w->km2Hz = o[OUT_KM2HZ];
+        if (!opt.host_spin) {
+          for (int i = 0; i < 3; i++) for (int j = 0; j < 3; j++) w->ae[i][j] = V.ae[3 * i + j];
+          for (int j = 0; j < 3; j++) { w->spin[j] = V.spin[j]; w->intspin[j] = V.intspin[j]; }
+          w->dopoff = V.dopoff;
+        }
+        if (V.is_v0) {
+          h->idoplim[0] = (int)o[OUT_IDOPLIM]; h->idoplim[1] = (int)o[OUT_IDOPLIM + 1];
+          h->doplim[0] = o[OUT_DOPLIM]; h->doplim[1] = o[OUT_DOPLIM + 1];
+          h->badradar_logfactor = o[OUT_BADRADAR_LOG];
+          h->pos.posbnd_logfactor = o[OUT_POSBND_LOG];
+        }
+      }
+    }
+  }
+  // par flags (calc_fits.c:349-354, 407-409)
+  par->posbnd = (unsigned char)E.posbnd;
+  par->badradar = (unsigned char)E.badradar;
+  par->badposet = 0;
+  par->posbnd_logfactor = E.posbnd_logfactor;
+  par->badradar_logfactor = E.badradar_logfactor;
+  par->badposet_logfactor = 0.0 / dat->dof_poset;
+  // seen flags
+  const GeomSlot &G = h_slots[h_eval_slot[e]];
+  std::vector<unsigned char> seen(G.nf);
+  SHP_CHECK(hipMemcpy(seen.data(), G.seen, G.nf, hipMemcpyDeviceToHost));
+  struct vertices_t *rv = &mod->shape.comp[0].real;
+  for (int f = 0; f < G.nf && f < rv->nf; f++) rv->f[f].seen = seen[f];
+  // POS arrays of every view (with pos_scope GLOBAL the last view's survive)
+  if (write_pos && pd.z) {
+    for (int v = 0; v < nviews; v++) {
+      const ViewRec &V = vw[v];
+      if (V.eval != e) continue;
+      struct set_t *set = &dat->set[V.set];
+      struct pos_t *pos = V.type == VT_DELAY ? &set->desc.deldop.frame[V.frm].pos
+                          : V.type == VT_DOPPLER ? &set->desc.doppler.frame[V.frm].pos
+                                                 : &set->desc.lghtcrv.rend[V.frm + 1].pos;
+      if ((rc = pos_to_host(stream, pos->z, pd.z, V.pos_off, V.n))) return rc;
+      if ((rc = pos_to_host(stream, pos->cose, pd.cose, V.pos_off, V.n))) return rc;
+      if ((rc = pos_to_host(stream, pos->f, pd.f, V.pos_off, V.n))) return rc;
+      if (V.bistatic) {
+        if ((rc = pos_to_host(stream, pos->cosi, pd.cosi, V.pos_off, V.n))) return rc;
+        if ((rc = pos_to_host(stream, pos->b, pd.b, V.pos_off, V.n))) return rc;
+        if ((rc = pos_to_host(stream, pos->zill, pd.zill, V.pos_off, V.n))) return rc;
+        if ((rc = pos_to_host(stream, pos->cosill, pd.cosill, V.pos_off, V.n))) return rc;
+        if ((rc = pos_to_host(stream, pos->fill, pd.fill, V.pos_off, V.n))) return rc;
+      }
+      pos->xlim[0] = V.xlim[0]; pos->xlim[1] = V.xlim[1]; pos->ylim[0] = V.ylim[0]; pos->ylim[1] = V.ylim[1];
+      if (V.bistatic) {
+        pos->xlim2[0] = V.xlim2[0]; pos->xlim2[1] = V.xlim2[1]; pos->ylim2[0] = V.ylim2[0]; pos->ylim2[1] = V.ylim2[1];
+      }
+      for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) { pos->ae[i][j] = V.ae[3 * i + j]; pos->oe[i][j] = V.oe[3 * i + j]; }
+    }
+  }
+  return 0;
+}
+
+}  // namespace shp

@@ -1,0 +1,99 @@
+// engine_host.h -- host side of the evaluation engine.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <map>
+#include <vector>
+
+#include "engine.h"
+#include "kernels.h"
+#include "shape_abi.h"
+
+namespace shp {
+
+// Where a fittable scalar lives inside an evaluation's parameter block, or
+// which vertex deviation it is.
+struct ParamTarget {
+  int kind;    // 0: parameter block offset, 1: vertex deviation r
+  int model;
+  int offset;  // block offset or vertex index
+};
+
+struct EvalSpec {
+  int model;
+  std::vector<const double *> targets;  // host addresses (e.g. par->fpntr[p])
+  std::vector<double> values;
+};
+
+struct RunOptions {
+  int method = METHOD_FP64;
+  int host_spin = 0;        // 1: take ae/spin/deloff/dopoff from the host structs
+  int host_geometry = 0;    // 1: upload realized x / normals instead of realizing on device
+  int keep_fit = 1;         // 1: leave fit images in place (0: read-and-clear in chi2)
+  int with_body = 0;        // allocate body/comp arrays (drop-ins)
+  int views_only = 0;       // pack parameters and views only (spin / offset drop-ins)
+};
+
+class Engine {
+ public:
+  ~Engine();
+  // Pack one scene per model; evaluations reference models.  Returns 0 or an error code.
+  int build(struct par_t *par, struct mod_t **mods, struct dat_t **dats, int nmodels,
+            const std::vector<EvalSpec> &evals, const RunOptions &opt);
+  // Rewrite the trial values of evaluation e (same targets as at build time).
+  int set_values(int e, const std::vector<double> &values);
+  int upload_params();
+  // Enqueue one full evaluation of every evaluation on the stream.
+  int enqueue();
+  int capture_graph();
+  int launch_graph();
+  int sync();
+  // results
+  int download_evals(std::vector<EvalRec> &out);
+  int download_frames(std::vector<FrameRec> &out);
+  int download_view_out(std::vector<double> &out);
+  int download_views(std::vector<ViewRec> &out);
+  // write fits, frame scalars, view data and flags of evaluation e into the host structs
+  int writeback(int e, struct par_t *par, struct mod_t *mod, struct dat_t *dat, int write_pos);
+  int error_flags();
+
+  hipStream_t stream = nullptr;
+  RunOptions opt;
+  // sizes
+  int nviews = 0, nframes = 0, nevals = 0, nslots = 0, pb_stride = 0;
+  int max_nf = 0, max_pix = 0, nsinc2 = 1, sinc2width = 6;
+  double mask_tol = 0.0, mincosine_seen = 0.0;
+  int exclude_seen = -1;
+  long long total_pix = 0, fit_len = 0, obs_len = 0;
+  bool any_lc = false, any_dd = false, any_dop = false;
+  std::vector<ViewRec> h_views;
+  std::vector<FrameRec> h_frames;
+  std::vector<EvalRec> h_evals;
+  std::vector<GeomSlot> h_slots;
+  std::vector<int> h_eval_slot;
+  std::vector<double> h_params;
+  std::vector<std::vector<ParamTarget>> eval_targets;
+  std::vector<std::map<const double *, ParamTarget>> model_params;
+  // device
+  ViewRec *d_views = nullptr;
+  FrameRec *d_frames = nullptr;
+  EvalRec *d_evals = nullptr;
+  GeomSlot *d_slots = nullptr;
+  int *d_eval_slot = nullptr, *d_slot_flags = nullptr;
+  double *d_slot_logf = nullptr;
+  double *d_params = nullptr, *d_fit = nullptr, *d_obs = nullptr, *d_oov = nullptr;
+  double *d_overflow = nullptr, *d_view_out = nullptr;
+  double *d_lc_x = nullptr, *d_lc_t = nullptr, *d_lc_y = nullptr, *d_lc_y2 = nullptr;
+  float *d_obs32 = nullptr, *d_oov32 = nullptr;
+  PosDev pd{};
+  std::vector<void *> allocs;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t graph_exec = nullptr;
+
+ private:
+  int alloc(void **p, size_t bytes);
+  int pack_model(int m, struct par_t *par, struct mod_t *mod, struct dat_t *dat);
+};
+
+}  // namespace shp

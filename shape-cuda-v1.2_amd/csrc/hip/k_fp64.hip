@@ -1,0 +1,1132 @@
+// k_fp64.hip -- FP64 kernels: per-view setup, realization, and the plain
+// FP64 forward model (exact z-buffer, radar synthesis, photometry, chi-square).
+//
+// This translation unit is compiled with -ffp-contract=off: every expression
+// below is evaluated in the same order and with the same roundings as the
+// reference's CPU code (no fused multiply-adds), so the FP64 path reproduces
+// the CPU's z values, facet ownership and cos(scattering) bit for bit; only
+// transcendental functions (sin, cos, pow, exp) may differ in the last ulp.
+#include "common.h"
+#include "engine.h"
+#include "kernels.h"
+
+namespace shp {
+
+// ---------------------------------------------------------------------------
+// small FP64 helpers (same operation order as src/util/*.c)
+// ---------------------------------------------------------------------------
+__device__ inline void cotrans3(double y[3], const double a[9], const double x[3]) {
+  double t0 = 0.0, t1 = 0.0, t2 = 0.0;
+  t0 += a[0] * x[0]; t0 += a[1] * x[1]; t0 += a[2] * x[2];
+  t1 += a[3] * x[0]; t1 += a[4] * x[1]; t1 += a[5] * x[2];
+  t2 += a[6] * x[0]; t2 += a[7] * x[1]; t2 += a[8] * x[2];
+  y[0] = t0; y[1] = t1; y[2] = t2;
+}
+__device__ inline void cotrans3T(double y[3], const double a[9], const double x[3]) {
+  double t0 = 0.0, t1 = 0.0, t2 = 0.0;
+  t0 += a[0] * x[0]; t0 += a[3] * x[1]; t0 += a[6] * x[2];
+  t1 += a[1] * x[0]; t1 += a[4] * x[1]; t1 += a[7] * x[2];
+  t2 += a[2] * x[0]; t2 += a[5] * x[1]; t2 += a[8] * x[2];
+  y[0] = t0; y[1] = t1; y[2] = t2;
+}
+// oa = b * a^T  (mtrnsps + mmmul of posvis.c:101,108,122)
+__device__ inline void mul_bat(double oa[9], const double b[9], const double a[9]) {
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < 3; j++) {
+      double t = 0.0;
+      for (int k = 0; k < 3; k++) t += b[3 * i + k] * a[3 * j + k];
+      oa[3 * i + j] = t;
+    }
+}
+__device__ inline double normalize3(double u[3]) {
+  double nn = 0.0;
+  nn += u[0] * u[0]; nn += u[1] * u[1]; nn += u[2] * u[2];
+  nn = sqrt(nn);
+  if (nn != 0.0) { u[0] /= nn; u[1] /= nn; u[2] /= nn; }
+  return nn;
+}
+__device__ inline void euler2mat9(double m[9], double phi, double theta, double psi) {
+  double ct = cos(theta), st = sin(theta), cp = cos(phi), sp = sin(phi), cs = cos(psi), ss = sin(psi);
+  m[0] = cs * cp - ct * sp * ss;
+  m[3] = -ss * cp - ct * sp * cs;
+  m[6] = st * sp;
+  m[1] = cs * sp + ct * cp * ss;
+  m[4] = -ss * sp + ct * cp * cs;
+  m[7] = -st * cp;
+  m[2] = ss * st;
+  m[5] = cs * st;
+  m[8] = ct;
+}
+__device__ inline double sinc2d(double x) {
+  double t = fabs(3.141592653589793 * x);
+  if (t > 1.0e-4) { t = sin(t) / t; return t * t; }
+  return 1.0;
+}
+__device__ inline double trid(double x) { double t = 1 - fabs(x); return t > 0.0 ? t : 0.0; }
+__device__ inline double tri2d(double x) { double t = 1 - fabs(x); return t > 0.0 ? t * t : 0.0; }
+
+__device__ inline double dmin(double a, double b) { return a < b ? a : b; }
+__device__ inline double dmax(double a, double b) { return a > b ? a : b; }
+
+// wavefront reductions (wave64)
+__device__ inline int wave_min_i(int v) { for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o, 64)); return v; }
+__device__ inline int wave_max_i(int v) { for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, 64)); return v; }
+__device__ inline double wave_min_d(double v) { for (int o = 32; o > 0; o >>= 1) v = dmin(v, __shfl_xor(v, o, 64)); return v; }
+__device__ inline double wave_max_d(double v) { for (int o = 32; o > 0; o >>= 1) v = dmax(v, __shfl_xor(v, o, 64)); return v; }
+__device__ inline bool lane0() { return (threadIdx.x & 63) == 0; }
+
+// ---------------------------------------------------------------------------
+// per-view setup: spin state, COM offsets, projection and Doppler constants
+// (realize_spin.c:41-280, inteuler.c:93-113, deldopoffs.c, dopoffs.c,
+//  posvis.c:96-135, pos2deldop.c:224-295, pos2doppler.c:179-232)
+// ---------------------------------------------------------------------------
+__global__ void k_setup_views(ViewRec *views, int nviews, const double *params) {
+  int iv = blockIdx.x * blockDim.x + threadIdx.x;
+  if (iv >= nviews) return;
+  ViewRec &V = views[iv];
+  const double *pb = params + V.pb;
+  if (!V.host_spin) {
+    const double *ps = pb + V.pb_set;
+    double ang[3];
+    for (int j = 0; j < 3; j++) ang[j] = pb[PB_ANGLE + j] + ps[PB_SET_ANGLEOFF + j];
+    // impulse list from t0 to t (realize_spin.c:350-398)
+    double t0 = pb[PB_T0], t = V.t;
+    double tint[MAXIMP + 2], imp[MAXIMP + 2];
+    int k = 0, nimp = (int)pb[PB_NIMP];
+    tint[0] = t0; imp[0] = 0.0;
+    if (t >= t0) {
+      for (int q = 0; q < nimp; q++) {
+        double tq = pb[PB_TIMP + q];
+        if (tq > t0 && tq <= t) { k++; tint[k] = tq; imp[k] = pb[PB_IMP + 3 * q + 2]; }
+      }
+      if (tint[k] < t) { k++; tint[k] = t; imp[k] = 0.0; }
+    } else {
+      for (int q = nimp - 1; q >= 0; q--) {
+        double tq = pb[PB_TIMP + q];
+        if (tq < t0 && tq >= t) { k++; tint[k] = tq; imp[k] = -pb[PB_IMP + 3 * q + 2]; }
+      }
+      if (tint[k] > t) { k++; tint[k] = t; imp[k] = 0.0; }
+    }
+    int n = k + 1;
+    double dt = tint[n - 1] - tint[0];
+    double psi = ang[2] + pb[PB_OMEGA + 2] * dt + 0.5 * pb[PB_OMEGADOT + 2] * dt * dt +
+                 pb[PB_LIBAMP] * sin(pb[PB_LIBFREQ] * dt + pb[PB_LIBPHASE]);
+    double w[3];
+    for (int i = 0; i <= 1; i++) w[i] = pb[PB_OMEGA + i] + pb[PB_OMEGADOT + i] * dt;
+    w[2] = pb[PB_OMEGA + 2] + pb[PB_OMEGADOT + 2] * dt +
+           pb[PB_LIBAMP] * pb[PB_LIBFREQ] * cos(pb[PB_LIBFREQ] * dt + pb[PB_LIBPHASE]);
+    for (int q = 0; q < n; q++) { psi += imp[q] * (tint[n - 1] - tint[q]); w[2] += imp[q]; }
+    euler2mat9(V.ae, ang[0], ang[1], psi);
+    for (int j = 0; j < 3; j++) w[j] += ps[PB_SET_OMEGAOFF + j];
+    double is[3];
+    cotrans3T(is, V.ae, w);
+    for (int j = 0; j < 3; j++) { V.intspin[j] = is[j]; V.spin[j] = V.orbspin[j] + is[j]; }
+    // COM offsets from the delay-correction polynomial
+    if (V.type == VT_DELAY || V.type == VT_DOPPLER) {
+      const double *pdc = pb + V.pb_dc;
+      int nd = (int)pdc[PB_SET_DCN];
+      double x = 1.0, dop = 0.0, del = pdc[PB_SET_DCA];
+      double arg = V.t - pdc[PB_SET_DCT0];
+      for (int q = 1; q <= nd; q++) {
+        dop += q * pdc[PB_SET_DCA + q] * x;
+        if (V.type == VT_DELAY) del += pdc[PB_SET_DCA + q] * (x *= arg);
+        else x *= arg;
+      }
+      V.deloff = (V.type == VT_DELAY) ? del / V.del_per_pixel : 0.0;
+      V.dopoff = -dop * V.Ftx / (V.dop_per_pixel * 86400.0);
+    }
+  }
+  mul_bat(V.oa, V.oe, V.ae);
+  if (V.bistatic) {
+    mul_bat(V.sa, V.se, V.ae);
+    double u[3] = {0.0, 0.0, 1.0};
+    cotrans3T(u, V.se, u);
+    cotrans3(V.usrc, V.oe, u);
+  }
+  // radar constants
+  if (V.type == VT_DELAY || V.type == VT_DOPPLER) {
+    const double *ps = pb + V.pb_set;
+    double w[3];
+    cotrans3(w, V.oe, V.spin);
+    double dopfact = ps[PB_SET_DOPSCALE] * 7.72139109254982e-5 * V.kmpp * V.Ftx / V.dop_per_pixel;
+    V.ax = -w[1] * dopfact;
+    V.ay = w[0] * dopfact;
+    V.km2Hz = sqrt(V.ax * V.ax + V.ay * V.ay) * V.dop_per_pixel / V.kmpp;
+    if (w[0] != 0.0 || w[1] != 0.0)
+      V.dop_extra = V.km2Hz * 0.5 * V.kmpp * sqrt(w[0] * w[0] + w[1] * w[1]) / dmax(fabs(w[0]), fabs(w[1]));
+    else
+      V.dop_extra = 0.0;
+    int nsinc2 = V.nsinc2, width = V.sinc2width;
+    V.xincr = V.ax / nsinc2;
+    V.yincr = V.ay / nsinc2;
+    V.dopdiff_bl = -(nsinc2 - 1) * (V.xincr + V.yincr) / 2;
+    V.dopdiff_max = (nsinc2 - 1) * (fabs(V.xincr) + fabs(V.yincr)) / 2;
+    V.badradar_sinc = 0;
+    V.badradar_sinc_log = 0.0;
+    if (2 * V.dopdiff_max + width + 1 > 100) {
+      V.badradar_sinc = 1;
+      V.badradar_sinc_log = log((2 * V.dopdiff_max + width + 1) / 100);
+    }
+    if (V.codemethod != 1) {
+      V.const1 = (3 * V.spb - 1) / (2.0 * V.stride);
+      V.const2 = (V.spb - 1) / (2.0 * V.stride);
+    } else {
+      V.const1 = (double)(V.spb / V.stride);
+      V.const2 = 0.0;
+    }
+    V.delfact = -6.67128190396304 / V.del_per_pixel;
+    V.delshift = V.delcom_vig + V.deloff;
+    V.dopshift = V.dopcom_vig + V.dopoff;
+    V.idop0 = 1000 - (int)floor(V.dopshift + 0.5);
+  }
+  // reset accumulators
+  V.xlim[0] = V.ylim[0] = V.xlim2[0] = V.ylim2[0] = V.n;
+  V.xlim[1] = V.ylim[1] = V.xlim2[1] = V.ylim2[1] = -V.n;
+  V.outbnd = V.outbnd2 = V.any_overflow = V.nstale = 0;
+  for (int q = 0; q < 4; q++) V.ov[q] = V.ov2[q] = ord64(1.0e20);
+  V.idellim[0] = V.ndel + 999999; V.idellim[1] = -999999;
+  V.idoplim[0] = V.ndop + 999999; V.idoplim[1] = -999999;
+  V.dellim[0] = V.dellim[1] = V.doplim[0] = V.doplim[1] = ord64(1.0e20);
+  V.ysum = 0.0;
+}
+
+// ---------------------------------------------------------------------------
+// realization (realize_mod.c:444-563)
+// ---------------------------------------------------------------------------
+__global__ void k_realize_vertices(GeomSlot *slots, int nslot, const double *params, int *slot_flags,
+                                   double *slot_logf) {
+  int s = blockIdx.y;
+  GeomSlot &G = slots[s];
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const double *pc = params + G.comp_pb;
+  double sf[3];
+  int bad = 0;
+  double blog = 0.0;
+  for (int j = 0; j < 3; j++) {
+    sf[j] = pc[j];
+    if (sf[j] <= 0.01) {
+      bad = 1;
+      blog += log(1 + 0.01 - sf[j]);
+      sf[j] = 0.01 / (1 + 0.01 - sf[j]);
+    }
+  }
+  if (i == 0) { slot_flags[s] = bad; slot_logf[s] = blog; }
+  if (i >= G.nv) return;
+  double x[3];
+  for (int j = 0; j < 3; j++) x[j] = sf[j] * (G.u[3 * i + j] * G.r[i] + G.a[3 * i + j]);
+  const double *rot = pc + 3, *off = pc + 6;
+  if (!(rot[0] == 0 && rot[1] == 0 && rot[2] == 0)) {
+    double m[9], y[3];
+    euler2mat9(m, rot[0], rot[1], rot[2]);
+    for (int j = 0; j < 3; j++) {
+      y[j] = 0.0;
+      for (int k = 0; k < 3; k++) y[j] += m[3 * j + k] * x[k];
+    }
+    for (int j = 0; j < 3; j++) x[j] = y[j];
+  }
+  if (!(off[0] == 0.0 && off[1] == 0.0 && off[2] == 0.0))
+    for (int j = 0; j < 3; j++) x[j] += off[j];
+  for (int j = 0; j < 3; j++) G.x[3 * i + j] = x[j];
+  G.x32[i] = make_float4((float)x[0], (float)x[1], (float)x[2], 0.0f);
+}
+
+__global__ void k_realize_facets(GeomSlot *slots, int nslot) {
+  GeomSlot &G = slots[blockIdx.y];
+  int f = blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= G.nf) return;
+  const int *fv = G.fv + 3 * f;
+  const double *p0 = G.x + 3 * fv[0], *p1 = G.x + 3 * fv[1], *p2 = G.x + 3 * fv[2];
+  double a[3], b[3], n[3];
+  for (int i = 0; i < 3; i++) { a[i] = p1[i] - p0[i]; b[i] = p2[i] - p1[i]; }
+  n[0] = a[1] * b[2] - a[2] * b[1];
+  n[1] = a[2] * b[0] - a[0] * b[2];
+  n[2] = a[0] * b[1] - a[1] * b[0];
+  double area = 0.5 * sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+  normalize3(n);
+  for (int i = 0; i < 3; i++) {
+    G.fn[3 * f + i] = n[i];
+    G.fx[3 * f + i] = (p0[i] + p1[i] + p2[i]) / 3;
+  }
+  G.area[f] = area;
+  G.fn32[f] = make_float4((float)n[0], (float)n[1], (float)n[2], 0.0f);
+}
+
+__global__ void k_realize_vnormals(GeomSlot *slots, int nslot) {
+  GeomSlot &G = slots[blockIdx.y];
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= G.nv) return;
+  double n[3] = {0.0, 0.0, 0.0};
+  for (int q = G.af_off[i]; q < G.af_off[i + 1]; q++) {
+    int f = G.af_idx[q];
+    for (int k = 0; k < 3; k++) n[k] += G.fn[3 * f + k];
+  }
+  normalize3(n);
+  for (int k = 0; k < 3; k++) G.vn[3 * i + k] = n[k];
+  G.vn32[i] = make_float4((float)n[0], (float)n[1], (float)n[2], 0.0f);
+}
+
+// ---------------------------------------------------------------------------
+// FP64 rasterization (posvis.c:142-295) in three passes over the facets:
+//   1: z-buffer  -- atomicMax of the ordered depth of every covered pixel
+//   2: ownership -- lowest facet index whose depth equals the maximum
+//   3: history   -- facets covering pixels whose winner has a back-facing
+//                   smoothed normal (their cos(scattering) is inherited from
+//                   an earlier claimant; replayed in k_replay64)
+// ---------------------------------------------------------------------------
+struct FacetProj {
+  double v0[3], v1[3], v2[3], nrm[3], den;
+  int i1, i2, j1, j2;
+  double imin_d, imax_d, jmin_d, jmax_d;
+  bool front, inside, outbnd;
+};
+
+__device__ inline void project_facet(const ViewRec &V, const GeomSlot &G, int f, int src, FacetProj &P) {
+  const double *oa = src ? V.sa : V.oa;
+  double nb[3] = {G.fn[3 * f], G.fn[3 * f + 1], G.fn[3 * f + 2]};
+  cotrans3(P.nrm, oa, nb);
+  P.front = P.nrm[2] > 0.0;
+  if (!P.front) return;
+  const int *fv = G.fv + 3 * f;
+  cotrans3(P.v0, oa, G.x + 3 * fv[0]);
+  cotrans3(P.v1, oa, G.x + 3 * fv[1]);
+  cotrans3(P.v2, oa, G.x + 3 * fv[2]);
+  // orbit offset is zero for single bodies: v += 0.0 keeps values unchanged
+  double kmpp = V.kmpp;
+  int n = V.n;
+  P.imin_d = floor(dmin(P.v0[0], dmin(P.v1[0], P.v2[0])) / kmpp - 1.0e-10 + 0.5);
+  P.imax_d = floor(dmax(P.v0[0], dmax(P.v1[0], P.v2[0])) / kmpp + 1.0e-10 + 0.5);
+  P.jmin_d = floor(dmin(P.v0[1], dmin(P.v1[1], P.v2[1])) / kmpp - 1.0e-10 + 0.5);
+  P.jmax_d = floor(dmax(P.v0[1], dmax(P.v1[1], P.v2[1])) / kmpp + 1.0e-10 + 0.5);
+  int imin = P.imin_d < -2147483648.0 ? INT_MIN : (int)P.imin_d;
+  int imax = P.imax_d > 2147483647.0 ? INT_MAX : (int)P.imax_d;
+  int jmin = P.jmin_d < -2147483648.0 ? INT_MIN : (int)P.jmin_d;
+  int jmax = P.jmax_d > 2147483647.0 ? INT_MAX : (int)P.jmax_d;
+  P.outbnd = (imin < -n || imax > n || jmin < -n || jmax > n);
+  P.i1 = max(imin, -n); P.i2 = min(imax, n);
+  P.j1 = max(jmin, -n); P.j2 = min(jmax, n);
+  P.inside = !(P.i1 > n || P.i2 < -n || P.j1 > n || P.j2 < -n);
+  P.den = 1 / ((P.v1[0] - P.v0[0]) * (P.v2[1] - P.v1[1]) - (P.v2[0] - P.v1[0]) * (P.v1[1] - P.v0[1]));
+}
+
+// barycentric test and depth at pixel (i, j); same expressions as posvis.c:217-228
+__device__ inline bool pixel_hit(const FacetProj &P, double kmpp, int i, int j, double &s, double &t, double &z) {
+  double x0 = i * kmpp, x1 = j * kmpp;
+  s = ((x0 - P.v0[0]) * (P.v2[1] - P.v1[1]) - (P.v2[0] - P.v1[0]) * (x1 - P.v0[1])) * P.den;
+  if (!((s >= -1.0e-10) && (s <= 1.0 + 1.0e-10))) return false;
+  t = ((P.v1[0] - P.v0[0]) * (x1 - P.v0[1]) - (x0 - P.v0[0]) * (P.v1[1] - P.v0[1])) * P.den;
+  if (!((t >= -1.0e-10) && (t <= s + 1.0e-10))) return false;
+  z = P.v0[2] + s * (P.v1[2] - P.v0[2]) + t * (P.v2[2] - P.v1[2]);
+  return true;
+}
+
+__global__ void k_raster64(ViewRec *views, const GeomSlot *slots, PosDev pd, int src, int pass) {
+  ViewRec &V = views[blockIdx.y];
+  const GeomSlot &G = slots[V.slot];
+  int f = blockIdx.x * blockDim.x + threadIdx.x;
+  int n = V.n, w = 2 * n + 1;
+  long long base = V.pos_off;
+  // pass-1 accumulators: in-frame rectangle and overall extent
+  int xl0 = n, xl1 = -n, yl0 = n, yl1 = -n, ob = 0;
+  double o0 = 1.0e20, o1 = -1.0e20, o2 = 1.0e20, o3 = -1.0e20;
+  if (f < G.nf) {
+    FacetProj P;
+    project_facet(V, G, f, src, P);
+    if (P.front) {
+      ob = P.outbnd;
+      if (!P.inside) {
+        // posvis.c:194-199: facet wholly outside -> only the rectangles grow
+        o0 = P.imin_d; o1 = P.imax_d; o2 = P.jmin_d; o3 = P.jmax_d;
+        int imin = P.imin_d < -2147483648.0 ? INT_MIN : (int)P.imin_d;
+        int imax = P.imax_d > 2147483647.0 ? INT_MAX : (int)P.imax_d;
+        int jmin = P.jmin_d < -2147483648.0 ? INT_MIN : (int)P.jmin_d;
+        int jmax = P.jmax_d > 2147483647.0 ? INT_MAX : (int)P.jmax_d;
+        xl0 = max(imin, -n); xl1 = min(imax, n); yl0 = max(jmin, -n); yl1 = min(jmax, n);
+        if (xl0 > n) xl0 = n;       // MAX(imin,-n) only lowers xlim[0] when below it
+        if (xl1 < -n) xl1 = -n;
+        if (yl0 > n) yl0 = n;
+        if (yl1 < -n) yl1 = -n;
+      } else {
+        for (int i = P.i1; i <= P.i2; i++)
+          for (int j = P.j1; j <= P.j2; j++) {
+            double s, t, z;
+            if (!pixel_hit(P, V.kmpp, i, j, s, t, z)) continue;
+            long long p = base + (long long)(i + n) * w + (j + n);
+            if (pass == 1) {
+              atomicMax(&pd.zkey[p], (unsigned long long)ord64(z));
+              xl0 = min(xl0, i); xl1 = max(xl1, i); yl0 = min(yl0, j); yl1 = max(yl1, j);
+              o0 = dmin(o0, (double)i); o1 = dmax(o1, (double)i);
+              o2 = dmin(o2, (double)j); o3 = dmax(o3, (double)j);
+            } else if (pass == 2) {
+              if ((unsigned long long)ord64(z) == pd.zkey[p]) atomicMin(&pd.fbuf[p], f);
+            } else {
+              int si = pd.stale_idx[p];
+              if (si >= 0) {
+                int q = atomicAdd(&pd.stale[si].ncand, 1);
+                if (q < STALE_CAND) pd.stale[si].cand[q] = f;
+              }
+            }
+          }
+      }
+    }
+  }
+  if (pass != 1) return;
+  xl0 = wave_min_i(xl0); xl1 = wave_max_i(xl1); yl0 = wave_min_i(yl0); yl1 = wave_max_i(yl1);
+  ob = wave_max_i(ob);
+  o0 = wave_min_d(o0); o1 = wave_max_d(o1); o2 = wave_min_d(o2); o3 = wave_max_d(o3);
+  if (lane0()) {
+    int *xl = src ? V.xlim2 : V.xlim, *yl = src ? V.ylim2 : V.ylim;
+    unsigned long long *ov = src ? V.ov2 : V.ov;
+    if (xl0 < n) atomicMin(&xl[0], xl0);
+    if (xl1 > -n) atomicMax(&xl[1], xl1);
+    if (yl0 < n) atomicMin(&yl[0], yl0);
+    if (yl1 > -n) atomicMax(&yl[1], yl1);
+    if (ob) atomicOr(src ? &V.outbnd2 : &V.outbnd, 1);
+    if (o0 < 1.0e20) atomic_min_d(&ov[0], o0);
+    if (o1 > -1.0e20) atomic_min_d(&ov[1], -o1);
+    if (o2 < 1.0e20) atomic_min_d(&ov[2], o2);
+    if (o3 > -1.0e20) atomic_min_d(&ov[3], -o3);
+  }
+}
+
+// shading of the winning facet at one pixel (posvis.c:243-275)
+__device__ inline void shade64(const ViewRec &V, const GeomSlot &G, int f, int src, int smooth, double s, double t,
+                               double &nz, double &cosb) {
+  double nrm[3];
+  const double *oa = src ? V.sa : V.oa;
+  if (smooth) {
+    const int *fv = G.fv + 3 * f;
+    const double *a = G.vn + 3 * fv[0], *b = G.vn + 3 * fv[1], *c = G.vn + 3 * fv[2];
+    for (int k = 0; k < 3; k++) nrm[k] = a[k] + s * (b[k] - a[k]) + t * (c[k] - b[k]);
+    double tmp[3];
+    cotrans3(tmp, oa, nrm);
+    nrm[0] = tmp[0]; nrm[1] = tmp[1]; nrm[2] = tmp[2];
+    normalize3(nrm);
+  } else {
+    double nb[3] = {G.fn[3 * f], G.fn[3 * f + 1], G.fn[3 * f + 2]};
+    cotrans3(nrm, oa, nb);
+  }
+  nz = nrm[2];
+  cosb = 0.0;
+  if (!src && V.bistatic) cosb = nrm[0] * V.usrc[0] + nrm[1] * V.usrc[1] + nrm[2] * V.usrc[2];
+}
+
+// Resolve the z-buffer into POS arrays.  merge: pixels already owned before
+// this call keep their owner on depth ties (the `fac < 0` rule of
+// posvis.c:236); blank: pixels without a new claimant are written as blank sky.
+__global__ void k_resolve64(ViewRec *views, const GeomSlot *slots, PosDev pd, int src, int merge, int blank,
+                            int body, int comp) {
+  ViewRec &V = views[blockIdx.y];
+  int n = V.n, w = 2 * n + 1;
+  int pix = blockIdx.x * blockDim.x + threadIdx.x;
+  if (pix >= w * w) return;
+  int i = pix / w - n, j = pix % w - n;
+  const int *xl = src ? V.xlim2 : V.xlim, *yl = src ? V.ylim2 : V.ylim;
+  if (!merge && (i < xl[0] || i > xl[1] || j < yl[0] || j > yl[1])) return;
+  long long p = V.pos_off + pix;
+  unsigned long long k = pd.zkey[p];
+  int fw = pd.fbuf[p];
+  pd.zkey[p] = 0ULL;
+  pd.fbuf[p] = 0x7fffffff;
+  double *zz = src ? pd.zill : pd.z;
+  double *cosa = src ? pd.cosill : pd.cose;
+  int *fac = src ? pd.fill : pd.f;
+  if (fw == 0x7fffffff) {
+    if (blank) {
+      zz[p] = -1.0e20; cosa[p] = 0.0; fac[p] = -1;
+      if (!src && pd.cosi) pd.cosi[p] = 0.0;
+    }
+    return;
+  }
+  int oldf = fac[p];
+  double oldz = zz[p];
+  if (merge && oldf >= 0 && k == (unsigned long long)ord64(oldz)) return;  // earlier owner keeps the tie
+  const GeomSlot &G = slots[V.slot];
+  FacetProj P;
+  project_facet(V, G, fw, src, P);
+  double s, t, z;
+  pixel_hit(P, V.kmpp, i, j, s, t, z);
+  int smooth = src ? 0 : V.smooth;
+  double nz, cosb;
+  shade64(V, G, fw, src, smooth, s, t, nz, cosb);
+  zz[p] = z;
+  fac[p] = fw;
+  if (pd.body) {
+    (src ? pd.bodyill : pd.body)[p] = body;
+    (src ? pd.compill : pd.comp)[p] = comp;
+  }
+  if (nz > 0.0) {
+    cosa[p] = nz;
+    if (!src && V.bistatic) {
+      pd.cosi[p] = cosb;
+      if (cosb <= 0.0) cosa[p] = 0.0;
+    }
+  } else {
+    // cos(scattering) is inherited from an earlier claimant: record for replay
+    int si = atomicAdd(pd.nstale, 1);
+    if (si < pd.stale_cap) {
+      StaleRec &S = pd.stale[si];
+      S.view = blockIdx.y; S.pix = pix;
+      S.oldf = merge ? oldf : -1;          // without merging the pixel started blank
+      S.oldz = merge ? oldz : -1.0e20;
+      S.oldcose = merge ? cosa[p] : 0.0;
+      S.oldcosi = (!src && V.bistatic) ? (merge ? pd.cosi[p] : 0.0) : 0.0;
+      S.ncand = 0;
+      S.src = src;
+      pd.stale_idx[p] = si;
+      if (!merge) { cosa[p] = 0.0; if (!src && V.bistatic) pd.cosi[p] = 0.0; }
+    } else {
+      atomicOr(pd.error, 1);
+    }
+  }
+}
+
+// Replay, in facet order, the claims made on one pixel whose final owner has a
+// back-facing smoothed normal, reproducing the CPU's inherited values.
+__global__ void k_replay64(ViewRec *views, const GeomSlot *slots, PosDev pd) {
+  int si = blockIdx.x * blockDim.x + threadIdx.x;
+  int ns = min(*pd.nstale, pd.stale_cap);
+  if (si >= ns) return;
+  StaleRec &S = pd.stale[si];
+  const ViewRec &V = views[S.view];
+  const GeomSlot &G = slots[V.slot];
+  int n = V.n, w = 2 * n + 1;
+  int i = S.pix / w - n, j = S.pix % w - n;
+  long long p = V.pos_off + S.pix;
+  int nc = min(S.ncand, STALE_CAND);
+  if (S.ncand > STALE_CAND) atomicOr(pd.error, 2);
+  int c[STALE_CAND];
+  for (int q = 0; q < nc; q++) c[q] = S.cand[q];
+  for (int a = 1; a < nc; a++) {  // insertion sort: facet order
+    int v = c[a], b = a - 1;
+    while (b >= 0 && c[b] > v) { c[b + 1] = c[b]; b--; }
+    c[b + 1] = v;
+  }
+  double zz = S.oldz, cosa = S.oldcose, cosb = S.oldcosi;
+  int fac = S.oldf;
+  int src = S.src, smooth = src ? 0 : V.smooth;
+  for (int q = 0; q < nc; q++) {
+    FacetProj P;
+    project_facet(V, G, c[q], src, P);
+    double s, t, z;
+    if (!pixel_hit(P, V.kmpp, i, j, s, t, z)) continue;
+    if (!((z > zz) || (fac < 0))) continue;
+    zz = z;
+    double nz, cb;
+    shade64(V, G, c[q], src, smooth, s, t, nz, cb);
+    if (nz > 0.0) {
+      cosa = nz;
+      if (!src && V.bistatic) { cosb = cb; if (cb <= 0.0) cosa = 0.0; }
+    }
+    fac = c[q];
+  }
+  (src ? pd.cosill : pd.cose)[p] = cosa;
+  if (!src && V.bistatic) pd.cosi[p] = cosb;
+  pd.stale_idx[p] = -1;
+}
+
+// ---------------------------------------------------------------------------
+// shadowing (posmask.c:58-174); pixels outside the sun-view rectangle are
+// blank sky by construction
+// ---------------------------------------------------------------------------
+__device__ inline double zill_at(const ViewRec &V, const PosDev &pd, int i, int j) {
+  if (i < V.xlim2[0] || i > V.xlim2[1] || j < V.ylim2[0] || j > V.ylim2[1]) return -1.0e20;
+  int w = 2 * V.n + 1;
+  return pd.zill[V.pos_off + (long long)(i + V.n) * w + (j + V.n)];
+}
+__device__ inline int fill_at(const ViewRec &V, const PosDev &pd, int i, int j) {
+  if (i < V.xlim2[0] || i > V.xlim2[1] || j < V.ylim2[0] || j > V.ylim2[1]) return -1;
+  int w = 2 * V.n + 1;
+  return pd.fill[V.pos_off + (long long)(i + V.n) * w + (j + V.n)];
+}
+
+__global__ void k_posmask64(ViewRec *views, PosDev pd, double tol) {
+  const ViewRec &V = views[blockIdx.y];
+  int n = V.n, w = 2 * n + 1;
+  int pix = blockIdx.x * blockDim.x + threadIdx.x;
+  if (pix >= w * w) return;
+  int i = pix / w - n, j = pix % w - n;
+  if (i < V.xlim[0] || i > V.xlim[1] || j < V.ylim[0] || j > V.ylim[1]) return;
+  long long p = V.pos_off + pix;
+  if (pd.cose[p] == 0.0) return;
+  double so[9];
+  mul_bat(so, V.se, V.oe);  // so = se * oe^T: observer -> source
+  double kmpp = V.kmpp, ppk = 1 / kmpp;
+  double x[3] = {i * kmpp, j * kmpp, pd.z[p]}, y[3];
+  cotrans3(y, so, x);
+  double i0 = y[0] * ppk, j0 = y[1] * ppk;
+  int im = i0 < 0.0 ? (int)(i0 - 0.5) : (int)(i0 + 0.5);
+  int jm = j0 < 0.0 ? (int)(j0 - 0.5) : (int)(j0 + 0.5);
+  double big = 0.99 * 1.0e20;
+  if (!(fabs(i0) < n && fabs(j0) < n)) return;
+  if (!(zill_at(V, pd, im, jm) > -big)) return;
+  if (!(pd.f[p] != fill_at(V, pd, im, jm))) return;  // body/comp are 0 for every pixel here
+  int i1 = (int)floor(i0), j1 = (int)floor(j0);
+  double zill;
+  double z11 = zill_at(V, pd, i1, j1), z21 = zill_at(V, pd, i1 + 1, j1), z12 = zill_at(V, pd, i1, j1 + 1),
+         z22 = zill_at(V, pd, i1 + 1, j1 + 1);
+  if (z11 > -big && z21 > -big && z12 > -big && z22 > -big) {
+    double t = i0 - i1, u = j0 - j1;
+    zill = (1 - t) * (1 - u) * z11 + t * (1 - u) * z21 + t * u * z22 + (1 - t) * u * z12;
+  } else {
+    double zm = zill_at(V, pd, im, jm);
+    zill = zm;
+    int is = (i0 >= im) ? 1 : -1;
+    int i2 = im + is;
+    if (abs(i2) <= n && zill_at(V, pd, i2, jm) > -big) {
+      zill += fabs(i0 - im) * (zill_at(V, pd, i2, jm) - zm);
+    } else {
+      i2 = im - is;
+      if (abs(i2) <= n && zill_at(V, pd, i2, jm) > -big) zill -= fabs(i0 - im) * (zill_at(V, pd, i2, jm) - zm);
+    }
+    int js = (j0 >= jm) ? 1 : -1;
+    int j2 = jm + js;
+    if (abs(j2) <= n && zill_at(V, pd, im, j2) > -big) {
+      zill += fabs(j0 - jm) * (zill_at(V, pd, im, j2) - zm);
+    } else {
+      j2 = jm - js;
+      if (abs(j2) <= n && zill_at(V, pd, im, j2) > -big) zill -= fabs(j0 - jm) * (zill_at(V, pd, im, j2) - zm);
+    }
+  }
+  if (zill - y[2] > tol) pd.cose[p] = 0.0;
+}
+
+// ---------------------------------------------------------------------------
+// optical brightness (apply_photo.c:78-309, Lambert / Lommel-Seeliger /
+// geometric / Kaasalainen) summed per lightcurve point
+// ---------------------------------------------------------------------------
+__global__ void k_photo64(ViewRec *views, PosDev pd, const double *params) {
+  ViewRec &V = views[blockIdx.y];
+  int n = V.n, w = 2 * n + 1;
+  int pix = blockIdx.x * blockDim.x + threadIdx.x;
+  double b = 0.0;
+  if (pix < w * w) {
+    int i = pix / w - n, j = pix % w - n;
+    if (i >= V.xlim[0] && i <= V.xlim[1] && j >= V.ylim[0] && j <= V.ylim[1]) {
+      long long p = V.pos_off + pix;
+      double ce = pd.cose[p], ci = pd.cosi[p];
+      if (ce > 0.0 && ci > 0.0) {
+        const double *pl = params + V.pb + V.pb_law;
+        int type = V.law_type;
+        if (type == LAMBERTLAW) b = V.intensityfactor * (pl[1] / 3.141592653589793) * ci;
+        else if (type == LOMMEL) b = V.intensityfactor * (pl[1] / (4 * 3.141592653589793)) * ci / (ci + ce);
+        else if (type == GEOMETRICAL) b = V.intensityfactor * pl[1];
+        else if (type == KAASALAINEN) {
+          double pf = pl[3] * exp(-V.phase / pl[4]) + pl[5] * V.phase + 1;
+          double sls = (1 - pl[2]) * pf * pl[1] / (4 * 3.141592653589793);
+          double sl = pl[2] * pf * pl[1] / 3.141592653589793;
+          b = V.intensityfactor * ci * (sls / (ci + ce) + sl);
+        }
+        if (pd.b) pd.b[p] = b;
+      }
+    }
+  }
+  // block reduction, then one FP64 atomic per block
+  __shared__ double red[256];
+  red[threadIdx.x] = b;
+  __syncthreads();
+  for (int o = blockDim.x / 2; o > 0; o >>= 1) {
+    if (threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0 && red[0] != 0.0) atomicAdd(&V.ysum, red[0]);
+}
+
+// ---------------------------------------------------------------------------
+// delay-Doppler synthesis (pos2deldop.c:305-535), one thread per POS pixel,
+// FP64 atomics into the fit image
+// ---------------------------------------------------------------------------
+constexpr int DELMAX = 8, DOPMAX = 32;
+
+__device__ inline double sinc2_mean64(const ViewRec &V, double dopPOS, int idop) {
+  int ns = V.nsinc2;
+  if (ns == 1) return sinc2d(dopPOS - idop);
+  if (ns == 2) {
+    double bl = dopPOS + V.dopdiff_bl - idop;
+    return (sinc2d(bl) + sinc2d(bl + V.xincr) + sinc2d(bl + V.yincr) + sinc2d(bl + V.xincr + V.yincr)) / 4;
+  }
+  double al = dopPOS + V.dopdiff_bl - idop, sum = 0.0;
+  for (int a = 0; a < ns; a++) {
+    double g = al;
+    for (int b = 0; b < ns; b++) { sum += sinc2d(g); g += V.xincr; }
+    al += V.yincr;
+  }
+  return sum / (ns * ns);
+}
+
+__device__ inline double delay_weight64(const ViewRec &V, double delPOS, int idel) {
+  if (V.codemethod != 1) {
+    if (V.spb == 1) return tri2d(delPOS - idel);
+    int sos = V.spb / V.stride;
+    if (V.spb == 2) {
+      double arg = (delPOS - (idel - V.const2)) / sos;
+      double d = trid(arg) + trid(arg - 0.5);
+      return d * (d / 4);
+    }
+    double d = 0.0, osp = 1.0 / V.spb;
+    int m_min = max((int)floor((delPOS - idel - V.const2) * V.stride), 0);
+    int m_max = min((int)ceil((delPOS - idel + V.const1) * V.stride), V.spb) - 1;
+    double arg = (delPOS - (idel - V.const2)) / sos - m_min * osp;
+    for (int m = m_min; m <= m_max; m++) { d += trid(arg); arg -= osp; }
+    return d * (d / (V.spb * V.spb));
+  }
+  return tri2d((delPOS - idel) / (V.spb / V.stride));
+}
+
+__device__ inline double radlaw64(const double *pl, int type, double c) {
+  // pl: [type, R, C, cos_cutoff, qsR, qsC, qs_cutoff]
+  double d = 0.0;
+  switch (type) {
+  case COSINELAW_DIFF: d = pl[1] * (pl[2] + 1) * pow(c, 2 * pl[2] - 1); break;
+  case GAUSSIANLAW:
+    if (c >= pl[3]) { double t2 = 1 / (c * c) - 1; d = pl[1] * pl[2] * exp(-pl[2] * t2) / (c * c * c * c * c); }
+    break;
+  case HAGFORSLAW:
+    if (c >= pl[3]) { double s2 = 1 - c * c, a = c * c * c * c + pl[2] * s2; d = 0.5 * pl[1] * pl[2] * pow(a, -1.5) / c; }
+    break;
+  case COSINELAW_QS:
+    if (c >= pl[3]) d = pl[1] * (pl[2] + 1) * pow(c, 2 * pl[2] - 1);
+    break;
+  case GAUSSIAN_COSINE:
+    d = pl[1] * (pl[2] + 1) * pow(c, 2 * pl[2] - 1);
+    if (c >= pl[6]) { double t2 = 1 / (c * c) - 1; d += pl[4] * pl[5] * exp(-pl[5] * t2) / (c * c * c * c * c); }
+    break;
+  case HAGFORS_COSINE:
+    d = pl[1] * (pl[2] + 1) * pow(c, 2 * pl[2] - 1);
+    if (c >= pl[6]) { double s2 = 1 - c * c, a = c * c * c * c + pl[5] * s2; d += 0.5 * pl[4] * pl[5] * pow(a, -1.5) / c; }
+    break;
+  case COSINE_COSINE:
+    d = pl[1] * (pl[2] + 1) * pow(c, 2 * pl[2] - 1);
+    if (c >= pl[6]) d += pl[4] * (pl[5] + 1) * pow(c, 2 * pl[5] - 1);
+    break;
+  }
+  return d;
+}
+
+__global__ void k_deldop64(ViewRec *views, FrameRec *frames, PosDev pd, double *fit, const double *params) {
+  ViewRec &V = views[blockIdx.y];
+  int n = V.n, w = 2 * n + 1;
+  int pix = blockIdx.x * blockDim.x + threadIdx.x;
+  int il0 = 0x7fffffff, il1 = -0x7fffffff, jl0 = 0x7fffffff, jl1 = -0x7fffffff;
+  double dl0 = 1.0e20, dl1 = -1.0e20, pl0 = 1.0e20, pl1 = -1.0e20;
+  if (pix < w * w && V.type == VT_DELAY) {
+    int x = pix / w - n, y = pix % w - n;
+    long long p = V.pos_off + pix;
+    if (x >= V.xlim[0] && x <= V.xlim[1] && y >= V.ylim[0] && y <= V.ylim[1] && pd.cose[p] > 0.0 &&
+        (pd.body ? pd.body[p] == 0 : true) && pd.z[p] != -1e20) {
+      const FrameRec &F = frames[V.frame_id];
+      int ndel = V.ndel, ndop = V.ndop, width = V.sinc2width;
+      double delPOS = pd.z[p] * V.delfact + V.delshift;
+      int idel_min = (int)floor(delPOS - V.const1) + 1;
+      int idel_max = (int)ceil(delPOS + V.const1) - 1;
+      double dopPOS = V.ax * (x - 0.0) + V.ay * (y - 0.0) + V.dopshift;
+      int idop_min = (int)floor(dopPOS - V.dopdiff_max + 1 - width / 2.0);
+      int idop_max = (int)floor(dopPOS + V.dopdiff_max + width / 2.0);
+      double codefactor = (V.codemethod == 0) ? sinc2d((dopPOS - V.dopDC_vig) / V.dopfftlen) : 1.0;
+      il0 = idel_min; il1 = idel_max; jl0 = idop_min; jl1 = idop_max;
+      dl0 = dl1 = delPOS; pl0 = pl1 = dopPOS;
+      bool in_bounds = (idel_min >= 1) && (idel_max <= ndel) && (idop_min >= 1) && (idop_max <= ndop);
+      int nd = idel_max - idel_min + 1, np = idop_max - idop_min + 1;
+      if (in_bounds && nd <= DELMAX && np <= DOPMAX) {
+        double dc[DELMAX], pc[DOPMAX];
+        for (int a = 0; a < nd; a++) dc[a] = delay_weight64(V, delPOS, idel_min + a);
+        for (int b = 0; b < np; b++) pc[b] = sinc2_mean64(V, dopPOS, idop_min + b);
+        double sumw = 0.0;
+        for (int a = 0; a < nd; a++)
+          for (int b = 0; b < np; b++) sumw += dc[a] * pc[b];
+        const double *pl = params + V.pb + V.pb_law;
+        double amp = radlaw64(pl, V.law_type, pd.cose[p]) * V.kmpp * V.kmpp * codefactor / sumw;
+        double *img = fit + F.off;
+        for (int a = 0; a < nd; a++)
+          for (int b = 0; b < np; b++)
+            atomicAdd(&img[(long long)(idel_min + a - 1) * ndop + (idop_min + b - 1)], amp * dc[a] * pc[b]);
+      } else if (in_bounds) {
+        // wide footprints: recompute weights instead of holding them in registers
+        double sumw = 0.0;
+        for (int a = idel_min; a <= idel_max; a++) {
+          double d = delay_weight64(V, delPOS, a);
+          for (int b = idop_min; b <= idop_max; b++) sumw += d * sinc2_mean64(V, dopPOS, b);
+        }
+        const double *pl = params + V.pb + V.pb_law;
+        double amp = radlaw64(pl, V.law_type, pd.cose[p]) * V.kmpp * V.kmpp * codefactor / sumw;
+        double *img = fit + F.off;
+        for (int a = idel_min; a <= idel_max; a++) {
+          double d = delay_weight64(V, delPOS, a);
+          for (int b = idop_min; b <= idop_max; b++)
+            atomicAdd(&img[(long long)(a - 1) * ndop + (b - 1)], amp * d * sinc2_mean64(V, dopPOS, b));
+        }
+      }
+    }
+  }
+  il0 = wave_min_i(il0); il1 = wave_max_i(il1); jl0 = wave_min_i(jl0); jl1 = wave_max_i(jl1);
+  dl0 = wave_min_d(dl0); dl1 = wave_max_d(dl1); pl0 = wave_min_d(pl0); pl1 = wave_max_d(pl1);
+  if (lane0() && il0 != 0x7fffffff) {
+    atomicMin(&V.idellim[0], il0); atomicMax(&V.idellim[1], il1);
+    atomicMin(&V.idoplim[0], jl0); atomicMax(&V.idoplim[1], jl1);
+    atomic_min_d(&V.dellim[0], dl0); atomic_min_d(&V.dellim[1], -dl1);
+    atomic_min_d(&V.doplim[0], pl0); atomic_min_d(&V.doplim[1], -pl1);
+  }
+}
+
+// Doppler synthesis with the overflow spectrum (pos2doppler.c:244-439)
+__global__ void k_doppler64(ViewRec *views, FrameRec *frames, PosDev pd, double *fit, double *overflow,
+                            const double *params) {
+  ViewRec &V = views[blockIdx.y];
+  int n = V.n, w = 2 * n + 1;
+  int pix = blockIdx.x * blockDim.x + threadIdx.x;
+  int jl0 = 0x7fffffff, jl1 = -0x7fffffff, ovf = 0;
+  double pl0 = 1.0e20, pl1 = -1.0e20;
+  if (pix < w * w && V.type == VT_DOPPLER) {
+    int x = pix / w - n, y = pix % w - n;
+    long long p = V.pos_off + pix;
+    if (x >= V.xlim[0] && x <= V.xlim[1] && y >= V.ylim[0] && y <= V.ylim[1] && pd.cose[p] > 0.0 &&
+        (pd.body ? pd.body[p] == 0 : true)) {
+      const FrameRec &F = frames[V.frame_id];
+      int ndop = V.ndop, width = V.sinc2width;
+      double dopPOS = V.ax * (x - 0.0) + V.ay * (y - 0.0) + V.dopshift;
+      int idop_min = (int)floor(dopPOS - V.dopdiff_max + 1 - width / 2.0);
+      int idop_max = (int)floor(dopPOS + V.dopdiff_max + width / 2.0);
+      jl0 = idop_min; jl1 = idop_max;
+      pl0 = pl1 = dopPOS;
+      bool in_bounds = (idop_min >= 1) && (idop_max <= ndop);
+      int np = idop_max - idop_min + 1;
+      double sumw = 0.0;
+      double pc[DOPMAX];
+      for (int b = 0; b < np; b++) {
+        double v = sinc2_mean64(V, dopPOS, idop_min + b);
+        if (b < DOPMAX) pc[b] = v;
+        sumw += v;
+      }
+      const double *pl = params + V.pb + V.pb_law;
+      double amp = radlaw64(pl, V.law_type, pd.cose[p]) * V.kmpp * V.kmpp / sumw;
+      if (in_bounds) {
+        double *img = fit + F.off;
+        for (int b = 0; b < np; b++)
+          atomicAdd(&img[idop_min + b - 1], amp * (b < DOPMAX ? pc[b] : sinc2_mean64(V, dopPOS, idop_min + b)));
+      } else {
+        ovf = 1;
+        int idop0 = V.idop0;
+        int i1 = max(idop_min, -idop0), i2 = min(idop_max, -idop0 + 2000 - 1);
+        double *ov = overflow + (long long)blockIdx.y * 2000;
+        for (int b = i1; b <= i2; b++) {
+          int kk = b - idop_min;
+          atomicAdd(&ov[b + idop0], amp * (kk < DOPMAX ? pc[kk] : sinc2_mean64(V, dopPOS, b)));
+        }
+      }
+    }
+  }
+  jl0 = wave_min_i(jl0); jl1 = wave_max_i(jl1); ovf = wave_max_i(ovf);
+  pl0 = wave_min_d(pl0); pl1 = wave_max_d(pl1);
+  if (lane0() && jl0 != 0x7fffffff) {
+    atomicMin(&V.idoplim[0], jl0); atomicMax(&V.idoplim[1], jl1);
+    atomic_min_d(&V.doplim[0], pl0); atomic_min_d(&V.doplim[1], -pl1);
+    if (ovf) atomicOr(&V.any_overflow, 1);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// per-view finishing: POS-bound log factor (posvis.c:301-307), delay-Doppler
+// limits in usec/Hz (pos2deldop.c:543-548), Doppler overflow terms and the
+// beyond-overflow check (pos2doppler.c:447-499)
+// ---------------------------------------------------------------------------
+__global__ void k_finish_views(ViewRec *views, int nviews, double *overflow, const FrameRec *frames,
+                               double *view_out) {
+  int iv = blockIdx.x * blockDim.x + threadIdx.x;
+  if (iv >= nviews) return;
+  ViewRec &V = views[iv];
+  double *o = view_out + (long long)iv * VIEW_OUT;
+  for (int q = 0; q < VIEW_OUT; q++) o[q] = 0.0;
+  int n = V.n;
+  for (int src = 0; src < (V.bistatic ? 2 : 1); src++) {
+    int ob = src ? V.outbnd2 : V.outbnd;
+    const unsigned long long *ov = src ? V.ov2 : V.ov;
+    double lf = 0.0;
+    if (ob) {
+      double imin = unord64(ov[0]), imax = -unord64(ov[1]), jmin = unord64(ov[2]), jmax = -unord64(ov[3]);
+      double xf = (dmax(imax, (double)n) - dmin(imin, (double)-n) + 1) / (2 * n + 1);
+      double yf = (dmax(jmax, (double)n) - dmin(jmin, (double)-n) + 1) / (2 * n + 1);
+      lf = log(xf * yf);
+    }
+    o[OUT_OUTBND + src] = ob;
+    o[OUT_POSBND_LOG + src] = lf;
+  }
+  if (V.type == VT_DELAY || V.type == VT_DOPPLER) {
+    double dl0 = unord64(V.dellim[0]), dl1 = -unord64(V.dellim[1]);
+    double pl0 = unord64(V.doplim[0]), pl1 = -unord64(V.doplim[1]);
+    if (V.type == VT_DELAY) {
+      o[OUT_DELLIM] = (dl0 - V.delshift) * V.del_per_pixel;
+      o[OUT_DELLIM + 1] = (dl1 - V.delshift) * V.del_per_pixel;
+    }
+    o[OUT_DOPLIM] = (pl0 - V.dopshift) * V.dop_per_pixel - V.dop_extra;
+    o[OUT_DOPLIM + 1] = (pl1 - V.dopshift) * V.dop_per_pixel + V.dop_extra;
+    int bad = V.badradar_sinc;
+    double blog = V.badradar_sinc_log;
+    double o2 = 0.0, m2 = 0.0, xs = 0.0, dm = 0.0;
+    if (V.type == VT_DOPPLER && V.any_overflow) {
+      double *ovb = overflow + (long long)iv * 2000;
+      int idop0 = V.idop0;
+      int j1 = max(V.idoplim[0] + idop0, 0), j2 = min(V.idoplim[1] + idop0, 1999);
+      const FrameRec &F = frames[V.frame_id];
+      double sd2 = F.sdev_sq, var = sd2;
+      for (int j = j1; j <= j2; j++) {
+        double fo = ovb[j];
+        if (fo != 0.0) {
+          if (F.speckle) var = sd2 + F.lookfact * fo * fo;
+          o2 += 1.0;
+          m2 += fo * fo / var;
+          xs += fo;
+          dm += (j - idop0) * fo;
+        }
+      }
+      if (xs != 0.0) dm /= xs;
+      if ((V.idoplim[0] + idop0) < 0 || (V.idoplim[1] + idop0) >= 2000) {
+        bad = 1;
+        double df = (max(V.idoplim[1] + idop0, 2000) - min(V.idoplim[0] + idop0, 0)) / (1.0 * 2000);
+        blog += log(df);
+      }
+      for (int j = 0; j < 2000; j++) ovb[j] = 0.0;
+    }
+    o[OUT_OVF] = o2; o[OUT_OVF + 1] = m2; o[OUT_OVF + 2] = xs; o[OUT_OVF + 3] = dm;
+    o[OUT_BADRADAR] = bad;
+    o[OUT_BADRADAR_LOG] = blog;
+    o[OUT_IDELLIM] = V.idellim[0]; o[OUT_IDELLIM + 1] = V.idellim[1];
+    o[OUT_IDOPLIM] = V.idoplim[0]; o[OUT_IDOPLIM + 1] = V.idoplim[1];
+    o[OUT_KM2HZ] = V.km2Hz;
+  }
+  if (V.type == VT_LGHTCRV) o[OUT_YSUM] = (V.ysum == 0.0) ? 1.0e-40 : V.ysum;
+}
+
+// ---------------------------------------------------------------------------
+// lightcurve interpolation: natural cubic spline of y(x) over the calculated
+// points, evaluated at each observation epoch and view (calc_fits.c:1923-1933)
+// ---------------------------------------------------------------------------
+__global__ void k_spline(FrameRec *frames, int nframes, const double *view_out, const ViewRec *views,
+                         const double *lc_x, const double *lc_t, double *lc_y, double *lc_y2, double *fit) {
+  int fi = blockIdx.x;
+  if (fi >= nframes) return;
+  const FrameRec &F = frames[fi];
+  if (F.type != VT_LGHTCRV) return;
+  int nc = F.lc_ncalc;
+  const double *x = lc_x + F.lc_x_off;     // 1-based
+  double *y = lc_y + F.lc_y_off, *y2 = lc_y2 + F.lc_y2_off;
+  if (threadIdx.x == 0) {
+    for (int i = 1; i <= nc; i++) y[i] = view_out[(long long)(F.first_view + i - 1) * VIEW_OUT + OUT_YSUM];
+    // nr/spline.c with natural end conditions; u reuses y2's scratch half
+    double *u = y2 + nc + 1;
+    y2[1] = u[1] = 0.0;
+    for (int i = 2; i <= nc - 1; i++) {
+      double sig = (x[i] - x[i - 1]) / (x[i + 1] - x[i - 1]);
+      double p = sig * y2[i - 1] + 2.0;
+      y2[i] = (sig - 1.0) / p;
+      u[i] = (y[i + 1] - y[i]) / (x[i + 1] - x[i]) - (y[i] - y[i - 1]) / (x[i] - x[i - 1]);
+      u[i] = (6.0 * u[i] / (x[i + 1] - x[i - 1]) - sig * u[i - 1]) / p;
+    }
+    double qn = 0.0, un = 0.0;
+    y2[nc] = (un - qn * u[nc - 1]) / (qn * y2[nc - 1] + 1.0);
+    for (int k = nc - 1; k >= 1; k--) y2[k] = y2[k] * y2[k + 1] + u[k];
+  }
+  __syncthreads();
+  const double *t = lc_t + F.lc_t_off;     // t[(i-1)*nviews + v]
+  double *out = fit + F.off;               // 0-based fit[i-1]
+  for (int i = 1 + threadIdx.x; i <= F.lc_n; i += blockDim.x) {
+    double acc = 0.0;
+    for (int v = 0; v < F.lc_nviews; v++) {
+      double xv = t[(long long)(i - 1) * F.lc_nviews + v];
+      int klo = 1, khi = nc;
+      while (khi - klo > 1) {
+        int k = (khi + klo) >> 1;
+        if (x[k] > xv) khi = k; else klo = k;
+      }
+      double h = x[khi] - x[klo];
+      double a = (x[khi] - xv) / h, b = (xv - x[klo]) / h;
+      acc += a * y[klo] + b * y[khi] + ((a * a * a - a) * y2[klo] + (b * b * b - b) * y2[khi]) * (h * h) / 6.0;
+    }
+    out[i - 1] = acc / F.lc_nviews;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// chi-square per frame (chi2.c:418-461, :752-785, :1137-1170): FP64 partial
+// sums with Neumaier compensation and a fixed-order block reduction
+// ---------------------------------------------------------------------------
+__device__ inline void neumaier(double &s, double &c, double x) {
+  double t = s + x;
+  if (fabs(s) >= fabs(x)) c += (s - t) + x; else c += (x - t) + s;
+  s = t;
+}
+
+__global__ void k_chi2_frames64(FrameRec *frames, int nframes, double *fit, const double *obs, const double *oov,
+                                const double *view_out, int clear_fit, int ovf_from_host) {
+  int fi = blockIdx.x;
+  if (fi >= nframes) return;
+  FrameRec &F = frames[fi];
+  long long N = (F.type == VT_DELAY) ? (long long)F.ndel * F.ndop : (F.type == VT_DOPPLER ? F.ndop : F.lc_n);
+  double *ft = fit + F.off;
+  const double *ob = obs + F.obs_off, *w = oov + F.obs_off;
+  double inv_nv = 1.0 / F.nviews;
+  double m2s = 0, m2c = 0, oms = 0, omc = 0;
+  for (long long q = threadIdx.x; q < N; q += blockDim.x) {
+    double fv = ft[q];
+    if (F.nviews > 1 && F.type != VT_LGHTCRV) { fv = fv * inv_nv; if (!clear_fit) ft[q] = fv; }
+    if (clear_fit) ft[q] = 0.0;
+    neumaier(m2s, m2c, fv * fv * w[q]);
+    neumaier(oms, omc, fv * ob[q] * w[q]);
+  }
+  __shared__ double r0[256], r1[256], r2[256], r3[256];
+  r0[threadIdx.x] = m2s; r1[threadIdx.x] = m2c; r2[threadIdx.x] = oms; r3[threadIdx.x] = omc;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double a = 0, ac = 0, b = 0, bc = 0;
+    for (int q = 0; q < (int)blockDim.x; q++) {
+      neumaier(a, ac, r0[q]); ac += r1[q];
+      neumaier(b, bc, r2[q]); bc += r3[q];
+    }
+    double o2 = F.o2, m2 = a + ac, om = b + bc;
+    if (ovf_from_host) {
+      // chi2() on host-resident fits: overflow terms come from the frame (chi2.c:430-431)
+      o2 = F.overflow_o2 + F.o2;
+      m2 = F.overflow_m2 + m2;
+    } else if (F.type == VT_DOPPLER) {
+      // overflow terms, averaged over the views of the frame (calc_fits.c:939-960)
+      double ovo2 = 0, ovm2 = 0, ovxs = 0, ovdm = 0, blog = 0;
+      int bad = 0;
+      for (int v = 0; v < F.nviews; v++) {
+        const double *o = view_out + (long long)(F.first_view + v) * VIEW_OUT;
+        ovo2 += o[OUT_OVF]; ovm2 += o[OUT_OVF + 1]; ovxs += o[OUT_OVF + 2]; ovdm += o[OUT_OVF + 3];
+        if (o[OUT_BADRADAR] != 0.0) { bad = 1; blog += o[OUT_BADRADAR_LOG]; }
+      }
+      if (F.nviews > 1) { ovo2 /= F.nviews; ovm2 /= F.nviews; ovxs /= F.nviews; ovdm /= F.nviews; }
+      F.overflow_o2 = ovo2; F.overflow_m2 = ovm2; F.overflow_xsec = ovxs; F.overflow_dopmean = ovdm;
+      o2 = ovo2 + F.o2;
+      m2 = ovm2 + m2;
+      F.badradar = bad; F.badradar_logfactor = blog;
+    } else if (F.type == VT_DELAY) {
+      int bad = 0;
+      double blog = 0;
+      for (int v = 0; v < F.nviews; v++) {
+        const double *o = view_out + (long long)(F.first_view + v) * VIEW_OUT;
+        if (o[OUT_BADRADAR] != 0.0) { bad = 1; blog += o[OUT_BADRADAR_LOG]; }
+      }
+      F.overflow_o2 = F.overflow_m2 = F.overflow_xsec = F.overflow_dopmean = 0.0;
+      F.badradar = bad; F.badradar_logfactor = blog;
+    }
+    double cal = F.cal_fixed;
+    if (F.cal_floats) cal = (om > 0.0) ? om / m2 : 1.0e-40;
+    F.cal = cal;
+    F.m2 = m2;
+    F.om = om;
+    F.chi2 = F.weight * (o2 - 2 * cal * om + cal * cal * m2);
+  }
+}
+
+// objective per evaluation (bestfit.c:789-866 with zero penalties), in
+// fixed frame order; log factors follow calc_fits.c:407-409, :511-512,
+// :547-550, :1654-1677, :1947
+__global__ void k_objective(EvalRec *evals, int nevals, const FrameRec *frames, const ViewRec *views,
+                            const double *view_out, const int *slot_flags, const double *slot_logf,
+                            const int *eval_slot) {
+  int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= nevals) return;
+  EvalRec &E = evals[e];
+  double chi2 = 0.0, pb = 0.0, br = 0.0;
+  int posbnd = 0, badradar = 0;
+  for (int q = 0; q < E.nframes; q++) {
+    const FrameRec &F = frames[E.first_frame + q];
+    chi2 += F.chi2;
+    if (F.type == VT_LGHTCRV) {
+      double lf = 0.0;
+      for (int i = 0; i < F.lc_ncalc; i++) {
+        const double *o = view_out + (long long)(F.first_view + i) * VIEW_OUT;
+        if (o[OUT_OUTBND] != 0.0) { posbnd = 1; lf += 0.5 * o[OUT_POSBND_LOG]; }
+        if (o[OUT_OUTBND + 1] != 0.0) { posbnd = 1; lf += 0.5 * o[OUT_POSBND_LOG + 1]; }
+      }
+      pb += F.dof * (lf / F.lc_ncalc);
+    } else {
+      for (int v = 0; v < F.nviews; v++) {
+        const ViewRec &V = views[F.first_view + v];
+        const double *o = view_out + (long long)(F.first_view + v) * VIEW_OUT;
+        if (V.is_v0 && o[OUT_OUTBND] != 0.0) { posbnd = 1; pb += F.dof * o[OUT_POSBND_LOG]; }
+        if (o[OUT_BADRADAR] != 0.0) { badradar = 1; br += F.dof * o[OUT_BADRADAR_LOG] / F.nviews; }
+      }
+    }
+  }
+  E.chi2 = chi2;
+  E.posbnd = posbnd;
+  E.badradar = badradar;
+  E.posbnd_logfactor = pb / E.dof;
+  E.badradar_logfactor = br / (E.dof_deldop + E.dof_doppler);
+  int sl = eval_slot[e];
+  E.baddiam = slot_flags[sl];
+  E.baddiam_logfactor = slot_logf[sl];
+  double err = chi2 / E.dof;
+  if (E.baddiam) err *= E.bad_objfactor * exp(E.baddiam_logfactor);
+  if (E.posbnd) err *= E.bad_objfactor * exp(E.posbnd_logfactor);
+  if (E.badradar) err *= E.bad_objfactor * exp(E.badradar_logfactor);
+  E.objective = err;
+}
+
+}  // namespace shp
+
+namespace shp {
+
+// facets seen from Earth (calc_fits.c:522-532, :903-913, :1699-1710)
+__global__ void k_mark_seen(ViewRec *views, GeomSlot *slots, PosDev pd, double mincos) {
+  const ViewRec &V = views[blockIdx.y];
+  if (V.exclude_seen || !V.is_v0) return;
+  int n = V.n, w = 2 * n + 1;
+  int pix = blockIdx.x * blockDim.x + threadIdx.x;
+  if (pix >= w * w) return;
+  int i = pix / w - n, j = pix % w - n;
+  if (i < V.xlim[0] || i > V.xlim[1] || j < V.ylim[0] || j > V.ylim[1]) return;
+  long long p = V.pos_off + pix;
+  int f = pd.f[p];
+  if (f < 0 || !(pd.cose[p] > mincos)) return;
+  if (V.type == VT_LGHTCRV && !(pd.cosi[p] > mincos)) return;
+  slots[V.slot].seen[f] = 1;
+}
+
+__global__ void k_fill_i32(int *p, long long n, int v) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+    p[i] = v;
+}
+__global__ void k_fill_u8(unsigned char *p, long long n) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+    p[i] = 0;
+}
+__global__ void k_zero_counters(int *p, int n) {
+  if ((int)threadIdx.x < n) p[threadIdx.x] = 0;
+}
+
+}  // namespace shp
+
+namespace shp {
+
+// o2 = sum obs^2/variance per frame (chi2.c:438), computed once per scene
+__global__ void k_frame_o2(FrameRec *frames, int nframes, const double *obs, const double *oov) {
+  int fi = blockIdx.x;
+  if (fi >= nframes) return;
+  FrameRec &F = frames[fi];
+  long long N = (F.type == VT_DELAY) ? (long long)F.ndel * F.ndop : (F.type == VT_DOPPLER ? F.ndop : F.lc_n);
+  const double *ob = obs + F.obs_off, *w = oov + F.obs_off;
+  double s = 0, c = 0;
+  for (long long q = threadIdx.x; q < N; q += blockDim.x) neumaier(s, c, ob[q] * ob[q] * w[q]);
+  __shared__ double r0[256], r1[256];
+  r0[threadIdx.x] = s; r1[threadIdx.x] = c;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double a = 0, ac = 0;
+    for (int q = 0; q < (int)blockDim.x; q++) { neumaier(a, ac, r0[q]); ac += r1[q]; }
+    F.o2 = a + ac;
+  }
+}
+
+// chi2 totals per set and for the data set, in set order (chi2.c:270-300)
+__global__ void k_chi2_sets(const FrameRec *frames, int nframes, double *set_chi2, int nsets, double *total) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  for (int s = 0; s < nsets; s++) set_chi2[s] = 0.0;
+  for (int q = 0; q < nframes; q++) set_chi2[frames[q].set] += frames[q].chi2;
+  double t = 0.0;
+  for (int s = 0; s < nsets; s++) t += set_chi2[s];
+  *total = t;
+}
+
+}  // namespace shp

@@ -1,0 +1,79 @@
+// kernels.h -- device buffer bundles and kernel launchers shared by the
+// engine and the drop-in entry points.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "engine.h"
+
+namespace shp {
+
+constexpr int STALE_CAND = 16;  // max facets recorded per replayed pixel
+
+// A pixel whose final owner has a back-facing smoothed normal; its
+// cos(scattering) is whatever an earlier claimant left (posvis.c:243-275).
+struct StaleRec {
+  int view, pix, oldf, ncand, src;
+  double oldz, oldcose, oldcosi;
+  int cand[STALE_CAND];
+};
+
+// POS buffers of all views, each view at ViewRec::pos_off.  FP64 arrays are
+// present for the FP64 method and the drop-ins; the mixed path uses zkey only.
+struct PosDev {
+  unsigned long long *zkey;  // z-buffer keys (FP64: ordered depth; mixed: packed key)
+  int *fbuf;                 // FP64 pass-2 owners
+  double *z, *cose, *cosi, *b;
+  int *f, *body, *comp;      // body/comp only for the drop-ins (NULL otherwise)
+  double *zill, *cosill;
+  int *fill, *bodyill, *compill;
+  int *stale_idx;            // per pixel: index into stale[], or -1
+  StaleRec *stale;
+  int *nstale;
+  int stale_cap;
+  int *error;                // bit flags: 1 stale list full, 2 too many candidates
+};
+
+// per-view outputs written by k_finish_views (doubles)
+constexpr int OUT_OUTBND = 0, OUT_POSBND_LOG = 2, OUT_DELLIM = 4, OUT_DOPLIM = 6, OUT_OVF = 8,
+              OUT_BADRADAR = 12, OUT_BADRADAR_LOG = 13, OUT_IDELLIM = 14, OUT_IDOPLIM = 16,
+              OUT_KM2HZ = 18, OUT_YSUM = 19, VIEW_OUT = 20;
+
+// ---- FP64 kernels (k_fp64.hip) ----
+__global__ void k_setup_views(ViewRec *views, int nviews, const double *params);
+__global__ void k_realize_vertices(GeomSlot *slots, int nslot, const double *params, int *slot_flags,
+                                   double *slot_logf);
+__global__ void k_realize_facets(GeomSlot *slots, int nslot);
+__global__ void k_realize_vnormals(GeomSlot *slots, int nslot);
+__global__ void k_raster64(ViewRec *views, const GeomSlot *slots, PosDev pd, int src, int pass);
+__global__ void k_resolve64(ViewRec *views, const GeomSlot *slots, PosDev pd, int src, int merge, int blank,
+                            int body, int comp);
+__global__ void k_replay64(ViewRec *views, const GeomSlot *slots, PosDev pd);
+__global__ void k_posmask64(ViewRec *views, PosDev pd, double tol);
+__global__ void k_photo64(ViewRec *views, PosDev pd, const double *params);
+__global__ void k_deldop64(ViewRec *views, FrameRec *frames, PosDev pd, double *fit, const double *params);
+__global__ void k_doppler64(ViewRec *views, FrameRec *frames, PosDev pd, double *fit, double *overflow,
+                            const double *params);
+__global__ void k_finish_views(ViewRec *views, int nviews, double *overflow, const FrameRec *frames,
+                               double *view_out);
+__global__ void k_spline(FrameRec *frames, int nframes, const double *view_out, const ViewRec *views,
+                         const double *lc_x, const double *lc_t, double *lc_y, double *lc_y2, double *fit);
+__global__ void k_chi2_frames64(FrameRec *frames, int nframes, double *fit, const double *obs, const double *oov,
+                                const double *view_out, int clear_fit, int ovf_from_host);
+__global__ void k_objective(EvalRec *evals, int nevals, const FrameRec *frames, const ViewRec *views,
+                            const double *view_out, const int *slot_flags, const double *slot_logf,
+                            const int *eval_slot);
+
+}  // namespace shp
+
+namespace shp {
+__global__ void k_mark_seen(ViewRec *views, GeomSlot *slots, PosDev pd, double mincos);
+__global__ void k_fill_i32(int *p, long long n, int v);
+__global__ void k_fill_u8(unsigned char *p, long long n);
+__global__ void k_zero_counters(int *p, int n);
+}  // namespace shp
+
+namespace shp {
+__global__ void k_frame_o2(FrameRec *frames, int nframes, const double *obs, const double *oov);
+__global__ void k_chi2_sets(const FrameRec *frames, int nframes, double *set_chi2, int nsets, double *total);
+}  // namespace shp

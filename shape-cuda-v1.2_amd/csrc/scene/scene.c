@@ -1050,6 +1050,62 @@ void shp_facet_seen(struct mod_t *mod, unsigned char *out)
   for (int f = 0; f < rv->nf; f++) out[f] = rv->f[f].seen;
 }
 
+/* frame constants: delcom_vig, dopcom_vig, dopDC_vig, epoch of view 0,
+   del_per_pixel, dop_per_pixel (or dop_per_bin), Ftx, codemethod, dopfftlen */
+void shp_frame_consts(struct dat_t *dat, int s, int f, double out[9])
+{
+  struct set_t *set = &dat->set[s];
+  memset(out, 0, sizeof(double) * 9);
+  if (set->type == DELAY) {
+    struct deldop_t *d = &set->desc.deldop;
+    struct deldopfrm_t *fr = &d->frame[f];
+    out[0] = fr->delcom_vig; out[1] = fr->dopcom_vig; out[2] = fr->dopDC_vig; out[3] = fr->view[0].t;
+    out[4] = d->del_per_pixel; out[5] = d->dop_per_pixel; out[6] = d->Ftx; out[7] = d->codemethod;
+    out[8] = d->dopfftlen;
+  } else if (set->type == DOPPLER) {
+    struct doppler_t *d = &set->desc.doppler;
+    struct dopfrm_t *fr = &d->frame[f];
+    out[1] = fr->dopcom_vig; out[3] = fr->view[0].t; out[5] = d->dop_per_bin; out[6] = d->Ftx;
+  } else if (set->type == LGHTCRV) {
+    out[3] = set->desc.lghtcrv.x[f + 1];
+  }
+}
+
+/* par knobs used by tests: 0 nsinc2, 1 sinc2width, 2 pos_smooth, 3 exclude_seen,
+   4 mask_tol, 5 speckle, 6 dd_gamma */
+void shp_par_set(struct par_t *par, int which, double v)
+{
+  switch (which) {
+  case 0: par->nsinc2 = (int)v; break;
+  case 1: par->sinc2width = (int)v; break;
+  case 2: par->pos_smooth = (unsigned char)v; break;
+  case 3: par->exclude_seen = (int)v; break;
+  case 4: par->mask_tol = v; break;
+  case 5: par->speckle = (unsigned char)v; break;
+  case 6: par->dd_gamma = v; break;
+  }
+}
+
+/* attached facets of vertex v (up to cap); returns the count */
+int shp_vertex_facets(struct mod_t *mod, int v, int *out, int cap)
+{
+  struct vertex_t *vx = &mod->shape.comp[0].real.v[v];
+  for (int q = 0; q < vx->naf && q < cap; q++) out[q] = vx->af[q];
+  return vx->naf < cap ? vx->naf : cap;
+}
+
+/* cosine-law R and C of the radar law used by set s */
+void shp_radar_law(struct mod_t *mod, struct dat_t *dat, int s, double out[2])
+{
+  struct set_t *set = &dat->set[s];
+  int l = set->type == DELAY ? set->desc.deldop.iradlaw : set->desc.doppler.iradlaw;
+  out[0] = mod->photo.radar[l].RC.R.val;
+  out[1] = mod->photo.radar[l].RC.C.val;
+}
+
+/* pointer to the model's photo_t (for radlaw / apply_photo calls) */
+struct photo_t *shp_mod_photo(struct mod_t *mod) { return &mod->photo; }
+
 void shp_par_flags(struct par_t *par, double out[6])
 {
   out[0] = par->posbnd; out[1] = par->badradar; out[2] = par->baddiam;

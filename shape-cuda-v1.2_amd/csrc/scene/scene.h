@@ -121,6 +121,11 @@ void shp_geometry(struct mod_t *mod, double *vx, double *vn, double *fn, double 
 void shp_facet_seen(struct mod_t *mod, unsigned char *out);
 /* par flags: posbnd, badradar, baddiam; logfactors posbnd, badradar, baddiam */
 void shp_par_flags(struct par_t *par, double out[6]);
+void shp_frame_consts(struct dat_t *dat, int s, int f, double out[9]);
+void shp_par_set(struct par_t *par, int which, double v);
+struct photo_t *shp_mod_photo(struct mod_t *mod);
+int shp_vertex_facets(struct mod_t *mod, int v, int *out, int cap);
+void shp_radar_law(struct mod_t *mod, struct dat_t *dat, int s, double out[2]);
 double *shp_param_ptr(shp_scene *sc, int i);
 int shp_param_count(shp_scene *sc);
 int shp_param_type(shp_scene *sc, int i);

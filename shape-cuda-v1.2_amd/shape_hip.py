@@ -87,6 +87,9 @@ def load_scene():
             "shp_geometry": (None, [vp, dp, dp, dp, dp]),
             "shp_facet_seen": (None, [vp, C.POINTER(C.c_ubyte)]),
             "shp_par_flags": (None, [vp, dp]),
+            "shp_frame_consts": (None, [vp, C.c_int, C.c_int, dp]),
+            "shp_par_set": (None, [vp, C.c_int, C.c_double]),
+            "shp_mod_photo": (vp, [vp]),
             "shp_param_ptr": (dp, [vp, C.c_int]),
             "shp_param_count": (C.c_int, [vp]),
             "shp_param_type": (C.c_int, [vp, C.c_int]),
@@ -97,6 +100,42 @@ def load_scene():
             fn.argtypes = args
         _scene_lib = lib
     return _scene_lib
+
+
+_hip_lib = None
+
+HIP_EXPORTS = ["realize_mod", "realize_spin", "realize_delcor", "calc_fits", "chi2",
+               "shape_hip_last_error", "shape_hip_clear_error", "shape_hip_set_strict",
+               "shape_hip_device_count", "shape_hip_realize_delcor"]
+
+
+def load_hip():
+    """Load the HIP library.  Raises if it is missing: there is no fallback."""
+    global _hip_lib
+    if _hip_lib is None:
+        path = os.path.join(LIBDIR, "libshape_hip.so")
+        if not os.path.exists(path):
+            raise RuntimeError("libshape_hip.so is not built (run __graft_entry__.build())")
+        lib = C.CDLL(path)
+        vp = C.c_void_p
+        sigs = {
+            "realize_mod": (None, [vp, vp]),
+            "realize_spin": (None, [vp, vp, vp]),
+            "realize_delcor": (None, [vp, C.c_double, C.c_int]),
+            "calc_fits": (None, [vp, vp, vp]),
+            "chi2": (C.c_double, [vp, vp, C.c_int]),
+            "shape_hip_last_error": (C.c_int, []),
+            "shape_hip_clear_error": (None, []),
+            "shape_hip_set_strict": (None, [C.c_int]),
+            "shape_hip_device_count": (C.c_int, []),
+            "shape_hip_realize_delcor": (None, [vp, vp, vp, C.c_double, C.c_int]),
+        }
+        for name, (res, args) in sigs.items():
+            fn = getattr(lib, name)
+            fn.restype = res
+            fn.argtypes = args
+        _hip_lib = lib
+    return _hip_lib
 
 
 # ---------------------------------------------------------------------------
@@ -293,6 +332,22 @@ class Scene:
     def set_view(self, s, f, v, arr):
         arr = np.ascontiguousarray(arr, dtype=np.float64)
         self.lib.shp_set_view_data(self.dat, s, f, v, arr.ctypes.data_as(C.POINTER(C.c_double)))
+
+    def consts(self, s, f):
+        out = np.zeros(9)
+        self.lib.shp_frame_consts(self.dat, s, f, out.ctypes.data_as(C.POINTER(C.c_double)))
+        return dict(zip(["delcom_vig", "dopcom_vig", "dopDC_vig", "t", "del_per_pixel",
+                         "dop_per_pixel", "Ftx", "codemethod", "dopfftlen"], out))
+
+    def par_set(self, which, value):
+        """which: nsinc2, sinc2width, pos_smooth, exclude_seen, mask_tol, speckle, dd_gamma"""
+        idx = ["nsinc2", "sinc2width", "pos_smooth", "exclude_seen", "mask_tol", "speckle",
+               "dd_gamma"].index(which)
+        self.lib.shp_par_set(self.par, idx, float(value))
+
+    @property
+    def photo(self):
+        return self.lib.shp_mod_photo(self.mod)
 
     def par_flags(self):
         out = np.zeros(6)
