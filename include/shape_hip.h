@@ -56,6 +56,41 @@ int shape_hip_device_count(void);
 void shape_hip_realize_delcor(struct par_t *par, struct mod_t *mod, struct dat_t *dat,
                               double delta_delcor0, int delcor0_mode);
 
+/* number of POS pixels whose cos(scattering) the last calc_fits inherited
+   from an earlier-claiming facet (diagnostic) */
+int shape_hip_last_replayed(void);
+
+/* ---------------- Part 2: batched evaluation (no reference counterpart) ----
+   Evaluates many trial parameter values of bestfit's objective (bestfit.c:718,
+   with zero penalties) at once: realization, rendering, synthesis, chi-square
+   and the bad-model factors for every evaluation, device-resident.  Each
+   evaluation e uses model eval_model[e] (an index into mods/dats) and sets
+   n_targets[e] parameters, given as addresses of `val` fields inside that
+   model's structs (e.g. par->fpntr entries), to the given trial values.
+   targets/values are flattened over evaluations. */
+typedef struct shape_batch shape_batch;
+
+#define SHAPE_METHOD_FP64 0   /* plain FP64 (the drop-in's kernels) */
+#define SHAPE_METHOD_MIXED 1  /* FP32 bulk, FP64 setup and reductions */
+
+#define SHAPE_BATCH_GRAPH 1     /* capture one replay into a hipGraph */
+#define SHAPE_BATCH_KEEP_FIT 2  /* keep fit images for inspection (single replay) */
+
+shape_batch *shape_batch_create(struct par_t *par, struct mod_t **mods, struct dat_t **dats, int nmodels,
+                                int nevals, const int *eval_model, const int *n_targets,
+                                double *const *targets, const double *values, int method, int flags);
+int shape_batch_set_values(shape_batch *b, const double *values);
+int shape_batch_launch(shape_batch *b);                 /* enqueue one replay */
+int shape_batch_wait(shape_batch *b, double *objective, double *chi2v);
+int shape_batch_run(shape_batch *b, double *objective, double *chi2v);
+float shape_batch_last_ms(shape_batch *b);              /* device time of the last replay */
+int shape_batch_nframes(shape_batch *b, int e);
+int shape_batch_frame_results(shape_batch *b, int e, double *chi2v, double *cal);
+int shape_batch_fit(shape_batch *b, int e, int frame, double *out);
+int shape_batch_counters(shape_batch *b, int *out4);
+int shape_batch_sizes(shape_batch *b, long long *out8);
+void shape_batch_destroy(shape_batch *b);
+
 #ifdef __cplusplus
 }
 #endif

@@ -22,6 +22,7 @@ using namespace shp;
 
 static int g_err = 0;
 static int g_strict_pos = 0;
+static int g_last_nstale = 0;
 
 static void set_err(int rc, const char *where) {
   if (rc) {
@@ -59,9 +60,14 @@ extern "C" void calc_fits(struct par_t *par, struct mod_t *mod, struct dat_t *da
   if (!rc) rc = E.enqueue();
   if (!rc) rc = E.sync();
   if (!rc && E.error_flags()) rc = -30;
+  g_last_nstale = E.last_nstale;
   if (!rc) rc = E.writeback(0, par, mod, dat, g_strict_pos);
   set_err(rc, "calc_fits");
 }
+
+// number of POS pixels whose cos(scattering) was inherited from an earlier
+// claimant in the last calc_fits (the replayed pixels); diagnostic only
+extern "C" int shape_hip_last_replayed(void) { return g_last_nstale; }
 
 // ---------------------------------------------------------------------------
 // chi2 (chi2.c:240): chi-square of the fits currently held by the caller

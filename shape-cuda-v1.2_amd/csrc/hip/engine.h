@@ -73,6 +73,7 @@ struct ViewRec {
   int smooth, bistatic, host_spin;  // host_spin: ae/spin/deloff/dopoff supplied by host
   int exclude_seen;      // 1: do not mark facets seen from this view
   long long pos_off;     // pixel offset of this view's POS buffers
+  long long tile_off;    // offset of this view's stale-tile bitmap (mixed method)
   int pb;                // parameter block of the evaluation
   int pb_set, pb_law;    // offsets (within the block) of the set and scattering-law params
   int pb_dc;             // offset of the set block whose delay polynomial applies (= states)
@@ -92,6 +93,12 @@ struct ViewRec {
   double xincr, yincr, dopdiff_bl, dopdiff_max;
   int badradar_sinc; double badradar_sinc_log;
   int idop0;
+  // FP32 copies for the mixed method: rotations, and the delay / Doppler COM
+  // bins split into an integer anchor plus an FP32 fraction so that pixel bin
+  // coordinates stay small (|rel| < ~1e3) when formed in FP32
+  float oa32[9], sa32[9], usrc32[3];
+  int D0, P0;
+  float dfr, pfr, delfact32, ax32, ay32, const1_32, kmpp32, inv_kmpp32, dcrel32;
   // ---- accumulated by kernels ----
   int xlim[2], ylim[2], xlim2[2], ylim2[2];
   int outbnd, outbnd2, any_overflow, nstale;

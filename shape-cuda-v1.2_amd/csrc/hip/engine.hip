@@ -18,8 +18,9 @@ static int g_last_err = 0;
 Engine::~Engine() {
   if (graph_exec) hipGraphExecDestroy(graph_exec);
   if (graph) hipGraphDestroy(graph);
-  for (void *p : allocs) hipFree(p);
-  if (stream) hipStreamDestroy(stream);
+  for (void *p : allocs) (void)hipFree(p);
+  if (h_pin) (void)hipHostFree(h_pin);
+  if (stream) (void)hipStreamDestroy(stream);
 }
 
 int Engine::alloc(void **p, size_t bytes) {
@@ -418,7 +419,7 @@ int Engine::build(struct par_t *par, struct mod_t **mods, struct dat_t **dats, i
   h_views.clear();
   h_frames.clear();
   h_evals.clear();
-  long long fit_cursor = 0, pos_cursor = 0, y_cursor = 0, y2_cursor = 0;
+  long long fit_cursor = 0, pos_cursor = 0, y_cursor = 0, y2_cursor = 0, tile_cursor = 0;
   std::vector<long long> model_obs_base(nmodels, 0), model_x_base(nmodels, 0), model_t_base(nmodels, 0);
   long long ob = 0, xb = 0, tb = 0;
   for (int m = 0; m < nmodels; m++) {
@@ -478,6 +479,8 @@ int Engine::build(struct par_t *par, struct mod_t **mods, struct dat_t **dats, i
       V.frame_id += frame_base;
       V.pos_off = pos_cursor;
       pos_cursor += (long long)(2 * V.n + 1) * (2 * V.n + 1);
+      V.tile_off = tile_cursor;
+      tile_cursor += (long long)((2 * V.n + STALE_TILE) / STALE_TILE) * ((2 * V.n + STALE_TILE) / STALE_TILE);
       max_pix = std::max(max_pix, (2 * V.n + 1) * (2 * V.n + 1));
       if (V.type == VT_LGHTCRV) any_lc = true;
       if (V.type == VT_DELAY) any_dd = true;
@@ -505,6 +508,7 @@ int Engine::build(struct par_t *par, struct mod_t **mods, struct dat_t **dats, i
   nframes = (int)h_frames.size();
   nslots = (int)h_slots.size();
   total_pix = pos_cursor;
+  total_tiles = tile_cursor;
   fit_len = fit_cursor;
   // ---- device arenas ----
   max_nf = 0;
@@ -543,6 +547,8 @@ int Engine::build(struct par_t *par, struct mod_t **mods, struct dat_t **dats, i
   if ((rc = upload(this, &d_evals, h_evals))) return rc;
   if ((rc = upload(this, &d_eval_slot, h_eval_slot))) return rc;
   if ((rc = upload(this, &d_params, h_params))) return rc;
+  SHP_CHECK(hipHostMalloc((void **)&h_pin, sizeof(double) * h_params.size(), hipHostMallocDefault));
+  memcpy(h_pin, h_params.data(), sizeof(double) * h_params.size());
   if (opt.views_only) { g_packs.clear(); return 0; }
   if ((rc = alloc((void **)&d_slot_flags, sizeof(int) * nslots))) return rc;
   if ((rc = alloc((void **)&d_slot_logf, sizeof(double) * nslots))) return rc;
@@ -589,6 +595,7 @@ int Engine::build(struct par_t *par, struct mod_t **mods, struct dat_t **dats, i
     hipLaunchKernelGGL(k_fill_i32, dim3(1024), dim3(256), 0, stream, pd.stale_idx, (long long)P, -1);
     hipLaunchKernelGGL(k_fill_i32, dim3(1024), dim3(256), 0, stream, pd.f, (long long)P, -1);
     SHP_CHECK(hipMemsetAsync(pd.cose, 0, sizeof(double) * P, stream));
+    hipLaunchKernelGGL(k_fill_f64, dim3(1024), dim3(256), 0, stream, pd.z, (long long)P, -1.0e20);
     if (any_lc) {
       if ((rc = alloc((void **)&pd.cosi, sizeof(double) * P))) return rc;
       if ((rc = alloc((void **)&pd.b, sizeof(double) * P))) return rc;
@@ -597,6 +604,9 @@ int Engine::build(struct par_t *par, struct mod_t **mods, struct dat_t **dats, i
       if ((rc = alloc((void **)&pd.fill, sizeof(int) * P))) return rc;
       SHP_CHECK(hipMemsetAsync(pd.cosi, 0, sizeof(double) * P, stream));
       SHP_CHECK(hipMemsetAsync(pd.b, 0, sizeof(double) * P, stream));
+      SHP_CHECK(hipMemsetAsync(pd.cosill, 0, sizeof(double) * P, stream));
+      hipLaunchKernelGGL(k_fill_f64, dim3(1024), dim3(256), 0, stream, pd.zill, (long long)P, -1.0e20);
+      hipLaunchKernelGGL(k_fill_i32, dim3(1024), dim3(256), 0, stream, pd.fill, (long long)P, -1);
     }
     if (opt.with_body) {
       if ((rc = alloc((void **)&pd.body, sizeof(int) * P))) return rc;
@@ -608,9 +618,17 @@ int Engine::build(struct par_t *par, struct mod_t **mods, struct dat_t **dats, i
         if ((rc = alloc((void **)&pd.compill, sizeof(int) * P))) return rc;
       }
     }
-    pd.stale_cap = (int)std::min<long long>(std::max<long long>(P / 8, 4096), 4LL << 20);
-    if ((rc = alloc((void **)&pd.stale, sizeof(StaleRec) * (size_t)pd.stale_cap))) return rc;
   }
+  if (opt.method == METHOD_MIXED && !pd.stale_idx) {
+    if ((rc = alloc((void **)&pd.stale_idx, sizeof(int) * P))) return rc;
+    hipLaunchKernelGGL(k_fill_i32, dim3(1024), dim3(256), 0, stream, pd.stale_idx, (long long)P, -1);
+  }
+  if (opt.method == METHOD_MIXED) {
+    if ((rc = alloc((void **)&pd.stale_tiles, (size_t)std::max(1LL, total_tiles)))) return rc;
+    SHP_CHECK(hipMemsetAsync(pd.stale_tiles, 0, (size_t)std::max(1LL, total_tiles), stream));
+  }
+  pd.stale_cap = (int)std::min<long long>(std::max<long long>((long long)P / 32, 4096), 2LL << 20);
+  if ((rc = alloc((void **)&pd.stale, sizeof(StaleRec) * (size_t)pd.stale_cap))) return rc;
   if ((rc = alloc((void **)&pd.nstale, sizeof(int) * 4))) return rc;
   pd.error = pd.nstale + 1;
   SHP_CHECK(hipMemset(pd.nstale, 0, sizeof(int) * 4));
@@ -624,8 +642,10 @@ int Engine::set_values(int e, const std::vector<double> &values) {
   if (e < 0 || e >= nevals || values.size() != eval_targets[e].size()) return -20;
   for (size_t q = 0; q < values.size(); q++) {
     const ParamTarget &T = eval_targets[e][q];
-    if (T.kind == 0) h_params[(size_t)pb_stride * e + T.offset] = values[q];
-    else {
+    if (T.kind == 0) {
+      h_params[(size_t)pb_stride * e + T.offset] = values[q];
+      if (h_pin) h_pin[(size_t)pb_stride * e + T.offset] = values[q];
+    } else {
       int sl = h_eval_slot[e];
       SHP_CHECK(hipMemcpyAsync(h_slots[sl].r + T.offset, &values[q], sizeof(double), hipMemcpyHostToDevice, stream));
       SHP_CHECK(hipStreamSynchronize(stream));
@@ -635,7 +655,8 @@ int Engine::set_values(int e, const std::vector<double> &values) {
 }
 
 int Engine::upload_params() {
-  SHP_CHECK(hipMemcpyAsync(d_params, h_params.data(), sizeof(double) * h_params.size(), hipMemcpyHostToDevice, stream));
+  const double *src = h_pin ? h_pin : h_params.data();
+  SHP_CHECK(hipMemcpyAsync(d_params, src, sizeof(double) * h_params.size(), hipMemcpyHostToDevice, stream));
   return 0;
 }
 
@@ -735,6 +756,7 @@ int Engine::download_views(std::vector<ViewRec> &out) {
 int Engine::error_flags() {
   int h[4] = {0, 0, 0, 0};
   if (hipMemcpy(h, pd.nstale, sizeof(int) * 4, hipMemcpyDeviceToHost) != hipSuccess) return -1;
+  last_nstale = h[0];
   return h[1];
 }
 
@@ -859,6 +881,13 @@ int Engine::writeback(int e, struct par_t *par, struct mod_t *mod, struct dat_t 
         if ((rc = pos_to_host(stream, pos->cosill, pd.cosill, V.pos_off, V.n))) return rc;
         if ((rc = pos_to_host(stream, pos->fill, pd.fill, V.pos_off, V.n))) return rc;
       }
+      // body / component maps of a single-body, single-component model (posvis.c:284-286)
+      for (int i = -V.n; i <= V.n; i++)
+        for (int j = -V.n; j <= V.n; j++) {
+          int own = pos->f[i][j] >= 0 ? 0 : -1;
+          pos->body[i][j] = pos->comp[i][j] = own;
+          if (V.bistatic) pos->bodyill[i][j] = pos->compill[i][j] = pos->fill[i][j] >= 0 ? 0 : -1;
+        }
       pos->xlim[0] = V.xlim[0]; pos->xlim[1] = V.xlim[1]; pos->ylim[0] = V.ylim[0]; pos->ylim[1] = V.ylim[1];
       if (V.bistatic) {
         pos->xlim2[0] = V.xlim2[0]; pos->xlim2[1] = V.xlim2[1]; pos->ylim2[0] = V.ylim2[0]; pos->ylim2[1] = V.ylim2[1];

@@ -65,7 +65,8 @@ class Engine {
   int max_nf = 0, max_pix = 0, nsinc2 = 1, sinc2width = 6;
   double mask_tol = 0.0, mincosine_seen = 0.0;
   int exclude_seen = -1;
-  long long total_pix = 0, fit_len = 0, obs_len = 0;
+  int last_nstale = 0;      // pixels replayed in the last evaluation (diagnostic)
+  long long total_pix = 0, fit_len = 0, obs_len = 0, total_tiles = 0;
   bool any_lc = false, any_dd = false, any_dop = false;
   std::vector<ViewRec> h_views;
   std::vector<FrameRec> h_frames;
@@ -73,6 +74,7 @@ class Engine {
   std::vector<GeomSlot> h_slots;
   std::vector<int> h_eval_slot;
   std::vector<double> h_params;
+  double *h_pin = nullptr;  // pinned staging copy of h_params (source of the graph's upload node)
   std::vector<std::vector<ParamTarget>> eval_targets;
   std::vector<std::map<const double *, ParamTarget>> model_params;
   // device

@@ -179,6 +179,21 @@ __global__ void k_setup_views(ViewRec *views, int nviews, const double *params) 
     V.dopshift = V.dopcom_vig + V.dopoff;
     V.idop0 = 1000 - (int)floor(V.dopshift + 0.5);
   }
+  // FP32 constants of the mixed method (anchored bin coordinates)
+  for (int q = 0; q < 9; q++) { V.oa32[q] = (float)V.oa[q]; V.sa32[q] = (float)V.sa[q]; }
+  for (int q = 0; q < 3; q++) V.usrc32[q] = (float)V.usrc[q];
+  V.kmpp32 = (float)V.kmpp;
+  V.inv_kmpp32 = (float)(1.0 / V.kmpp);
+  if (V.type == VT_DELAY || V.type == VT_DOPPLER) {
+    double d0 = floor(V.delshift), p0 = floor(V.dopshift);
+    V.D0 = (int)d0; V.P0 = (int)p0;
+    V.dfr = (float)(V.delshift - d0);
+    V.pfr = (float)(V.dopshift - p0);
+    V.delfact32 = (float)V.delfact;
+    V.ax32 = (float)V.ax; V.ay32 = (float)V.ay;
+    V.const1_32 = (float)V.const1;
+    V.dcrel32 = (float)(p0 - V.dopDC_vig);
+  }
   // reset accumulators
   V.xlim[0] = V.ylim[0] = V.xlim2[0] = V.ylim2[0] = V.n;
   V.xlim[1] = V.ylim[1] = V.xlim2[1] = V.ylim2[1] = -V.n;
@@ -1085,6 +1100,10 @@ __global__ void k_mark_seen(ViewRec *views, GeomSlot *slots, PosDev pd, double m
 }
 
 __global__ void k_fill_i32(int *p, long long n, int v) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+    p[i] = v;
+}
+__global__ void k_fill_f64(double *p, long long n, double v) {
   for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
     p[i] = v;
 }

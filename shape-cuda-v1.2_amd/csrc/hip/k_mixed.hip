@@ -1,7 +1,213 @@
-// k_mixed.hip -- FP32/FP64 mixed method (filled in below).
-#include "common.h"
+// k_mixed.hip -- the FP32/FP64 mixed method: rasterization, chi-square and
+// the launch sequence.
+//
+// FP64 stays where precision is structural: the per-view setup (rotation
+// composition at large epoch offsets, COM delay/Doppler offsets; k_fp64.hip)
+// and every reduction into chi-square (compensated FP64).  The bulk -- vertex
+// projection, rasterization, shading and delay-Doppler binning -- runs in FP32:
+//   * raster: facet per thread, one 64-bit atomicMax per covered pixel on a
+//     packed (FP32 depth, inverted facet index) key, so the nearest facet wins
+//     and equal depths go to the lower facet index, as in the CPU z-test;
+//   * resolve+synthesis (k_radar32.hip): one workgroup per 16x16-pixel tile
+//     reads and clears the keys, shades the owner, and accumulates the tile's
+//     delay-Doppler footprint in an LDS window before one flush to HBM;
+//   * pixels whose owner's smoothed normal faces away are replayed from the
+//     facets covering them (a raster pass restricted to 8x8 tiles that hold
+//     such pixels), reproducing the CPU's inherited cos(scattering);
+//   * chi-square: FP64 compensated partial sums over each frame's occupied
+//     window, reading and clearing the FP32 fit.
+// Delay and Doppler bin coordinates are formed relative to an integer anchor
+// (ViewRec::D0, P0) so FP32 rounding acts on |x| < ~1e3 bins, not on the
+// absolute bin number.
+#include "mixed_dev.h"
 #include "mixed.h"
 
 namespace shp {
-int enqueue_mixed(Engine &E) { (void)E; return -40; }
+
+__global__ void k_shade_radar32(ViewRec *views, const GeomSlot *slots, PosDev pd, double *fit, const FrameRec *frames,
+                                double *overflow, const double *params, int *counters);
+__global__ void k_replay32(ViewRec *views, const GeomSlot *slots, PosDev pd, double *fit, const FrameRec *frames,
+                           double *overflow, const double *params);
+
+// ---------------------------------------------------------------------------
+// raster (posvis.c:142-295).  pass 1: packed-key z-buffer and the POS
+// rectangles; pass 3: record the facets covering replayed pixels, visiting
+// only facets whose rectangle touches a tile flagged in the stale bitmap
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_raster32(ViewRec *views, const GeomSlot *slots, PosDev pd,
+                                                  long long sun_off, int src, int pass) {
+  ViewRec &V = views[blockIdx.y];
+  if (src && !V.bistatic) return;
+  if (pass == 3 && *pd.nstale == 0) return;
+  const GeomSlot &G = slots[V.slot];
+  int f = blockIdx.x * blockDim.x + threadIdx.x;
+  int n = V.n, w = 2 * n + 1;
+  unsigned long long *kb = pd.zkey + V.pos_off + (src ? sun_off : 0);
+  int xl0 = n, xl1 = -n, yl0 = n, yl1 = -n, ob = 0;
+  float o0 = 1e30f, o1 = -1e30f, o2 = 1e30f, o3 = -1e30f;
+  Proj32 P;
+  if (f < G.nf && project32(V, G, f, src ? V.sa32 : V.oa32, P)) {
+    float imin_d = floorf(fminf(P.x0, fminf(P.x1, P.x2)) + 0.5f);
+    float imax_d = floorf(fmaxf(P.x0, fmaxf(P.x1, P.x2)) + 0.5f);
+    float jmin_d = floorf(fminf(P.y0, fminf(P.y1, P.y2)) + 0.5f);
+    float jmax_d = floorf(fmaxf(P.y0, fmaxf(P.y1, P.y2)) + 0.5f);
+    imin_d = fmaxf(imin_d, -1.0e9f); imax_d = fminf(imax_d, 1.0e9f);
+    jmin_d = fmaxf(jmin_d, -1.0e9f); jmax_d = fminf(jmax_d, 1.0e9f);
+    int imin = (int)imin_d, imax = (int)imax_d, jmin = (int)jmin_d, jmax = (int)jmax_d;
+    ob = (imin < -n || imax > n || jmin < -n || jmax > n);
+    int i1 = max(imin, -n), i2 = min(imax, n), j1 = max(jmin, -n), j2 = min(jmax, n);
+    if (i1 > n || i2 < -n || j1 > n || j2 < -n) {
+      o0 = imin_d; o1 = imax_d; o2 = jmin_d; o3 = jmax_d;
+      xl0 = min(max(imin, -n), n); xl1 = max(min(imax, n), -n);
+      yl0 = min(max(jmin, -n), n); yl1 = max(min(jmax, n), -n);
+    } else if (pass == 1) {
+      for (int i = i1; i <= i2; i++)
+        for (int j = j1; j <= j2; j++) {
+          float s, t, z;
+          if (!hit32(P, i, j, s, t, z)) continue;
+          atomicMax(&kb[(long long)(i + n) * w + (j + n)], (unsigned long long)pack_key(z, f));
+          xl0 = min(xl0, i); xl1 = max(xl1, i); yl0 = min(yl0, j); yl1 = max(yl1, j);
+          o0 = fminf(o0, (float)i); o1 = fmaxf(o1, (float)i);
+          o2 = fminf(o2, (float)j); o3 = fmaxf(o3, (float)j);
+        }
+    } else {
+      int tw = (2 * n + STALE_TILE) / STALE_TILE;
+      const unsigned char *tb = pd.stale_tiles + V.tile_off;
+      bool any = false;
+      for (int a = (i1 + n) / STALE_TILE; a <= (i2 + n) / STALE_TILE && !any; a++)
+        for (int b = (j1 + n) / STALE_TILE; b <= (j2 + n) / STALE_TILE; b++)
+          if (tb[(long long)a * tw + b]) { any = true; break; }
+      if (any)
+        for (int i = i1; i <= i2; i++)
+          for (int j = j1; j <= j2; j++) {
+            float s, t, z;
+            if (!hit32(P, i, j, s, t, z)) continue;
+            int si = pd.stale_idx[V.pos_off + (long long)(i + n) * w + (j + n)];
+            if (si >= 0) {
+              int q = atomicAdd(&pd.stale[si].ncand, 1);
+              if (q < STALE_CAND) pd.stale[si].cand[q] = f;
+            }
+          }
+    }
+  }
+  if (pass != 1) return;
+  xl0 = wmin(xl0); xl1 = wmax(xl1); yl0 = wmin(yl0); yl1 = wmax(yl1); ob = wmax(ob);
+  o0 = wminf(o0); o1 = wmaxf(o1); o2 = wminf(o2); o3 = wmaxf(o3);
+  if ((threadIdx.x & 63) == 0) {
+    int *xl = src ? V.xlim2 : V.xlim, *yl = src ? V.ylim2 : V.ylim;
+    unsigned long long *ov = src ? V.ov2 : V.ov;
+    if (xl0 < n) atomicMin(&xl[0], xl0);
+    if (xl1 > -n) atomicMax(&xl[1], xl1);
+    if (yl0 < n) atomicMin(&yl[0], yl0);
+    if (yl1 > -n) atomicMax(&yl[1], yl1);
+    if (ob) atomicOr(src ? &V.outbnd2 : &V.outbnd, 1);
+    if (o0 < 1e30f) atomic_min_d(&ov[0], (double)o0);
+    if (o1 > -1e30f) atomic_min_d(&ov[1], -(double)o1);
+    if (o2 < 1e30f) atomic_min_d(&ov[2], (double)o2);
+    if (o3 > -1e30f) atomic_min_d(&ov[3], -(double)o3);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// chi-square of radar frames from the FP32 fit: FP64 compensated partials over
+// the frame's occupied window (chi2.c:418-461, :752-785); unless keep is set
+// the fit is cleared as it is read, ready for the next evaluation
+// ---------------------------------------------------------------------------
+__device__ inline void neu(double &s, double &c, double x) {
+  double t = s + x;
+  if (fabs(s) >= fabs(x)) c += (s - t) + x; else c += (x - t) + s;
+  s = t;
+}
+
+__global__ __launch_bounds__(256) void k_chi2_mixed(FrameRec *frames, int nframes, const ViewRec *views, double *fit,
+                                                    const float *ow, const float *w32, const double *view_out,
+                                                    int keep) {
+  FrameRec &F = frames[blockIdx.x];
+  if (F.type != VT_DELAY && F.type != VT_DOPPLER) return;
+  int r0 = 1, r1 = F.ndel, c0 = 0x7fffffff, c1 = -0x7fffffff;
+  int d0 = 0x7fffffff, d1 = -0x7fffffff;
+  for (int v = 0; v < F.nviews; v++) {
+    const ViewRec &V = views[F.first_view + v];
+    c0 = min(c0, V.idoplim[0]); c1 = max(c1, V.idoplim[1]);
+    d0 = min(d0, V.idellim[0]); d1 = max(d1, V.idellim[1]);
+  }
+  if (F.type == VT_DELAY) { r0 = max(1, d0); r1 = min(F.ndel, d1); }
+  c0 = max(1, c0); c1 = min(F.ndop, c1);
+  int nr = r1 - r0 + 1, nc = c1 - c0 + 1;
+  double m2s = 0, m2c = 0, oms = 0, omc = 0;
+  double *ft = fit + F.off;
+  const float *o = ow + F.obs_off, *wt = w32 + F.obs_off;
+  double inv_nv = 1.0 / F.nviews;
+  if (nr > 0 && nc > 0) {
+    for (int q = threadIdx.x; q < nr * nc; q += blockDim.x) {
+      long long idx = (long long)(r0 - 1 + q / nc) * F.ndop + (c0 - 1 + q % nc);
+      double fd = ft[idx];
+      if (fd != 0.0) {
+        if (!keep) ft[idx] = 0.0;
+        if (F.nviews > 1) fd *= inv_nv;
+        neu(m2s, m2c, fd * fd * (double)wt[idx]);
+        neu(oms, omc, fd * (double)o[idx]);
+      }
+    }
+  }
+  __shared__ double r[4][256];
+  r[0][threadIdx.x] = m2s; r[1][threadIdx.x] = m2c; r[2][threadIdx.x] = oms; r[3][threadIdx.x] = omc;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double a = 0, ac = 0, b = 0, bc = 0;
+    for (int q = 0; q < (int)blockDim.x; q++) {
+      neu(a, ac, r[0][q]); ac += r[1][q];
+      neu(b, bc, r[2][q]); bc += r[3][q];
+    }
+    double o2 = F.o2, m2 = a + ac, om = b + bc;
+    double ovo2 = 0, ovm2 = 0, ovxs = 0, ovdm = 0, blog = 0;
+    int bad = 0;
+    for (int v = 0; v < F.nviews; v++) {
+      const double *vo = view_out + (long long)(F.first_view + v) * VIEW_OUT;
+      ovo2 += vo[OUT_OVF]; ovm2 += vo[OUT_OVF + 1]; ovxs += vo[OUT_OVF + 2]; ovdm += vo[OUT_OVF + 3];
+      if (vo[OUT_BADRADAR] != 0.0) { bad = 1; blog += vo[OUT_BADRADAR_LOG]; }
+    }
+    if (F.nviews > 1) { ovo2 /= F.nviews; ovm2 /= F.nviews; ovxs /= F.nviews; ovdm /= F.nviews; }
+    F.overflow_o2 = ovo2; F.overflow_m2 = ovm2; F.overflow_xsec = ovxs; F.overflow_dopmean = ovdm;
+    o2 += ovo2;
+    m2 += ovm2;
+    F.badradar = bad;
+    F.badradar_logfactor = blog;
+    double cal = F.cal_fixed;
+    if (F.cal_floats) cal = (om > 0.0) ? om / m2 : 1.0e-40;
+    F.cal = cal;
+    F.m2 = m2;
+    F.om = om;
+    F.chi2 = F.weight * (o2 - 2 * cal * om + cal * cal * m2);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// launch sequence of one batch (radar datasets)
+// ---------------------------------------------------------------------------
+static inline unsigned cdivm(long long a, long long b) { return (unsigned)((a + b - 1) / b); }
+
+int enqueue_mixed(Engine &E) {
+  if (E.any_lc) return -41;  // lightcurve datasets use the FP64 method
+  hipStream_t st = E.stream;
+  dim3 gf(cdivm(E.max_nf, 256), E.nviews), bt(256);
+  int wmax_ = 0;
+  for (auto &V : E.h_views) wmax_ = std::max(wmax_, 2 * V.n + 1);
+  int tiles = cdivm(wmax_, TILE);
+  hipLaunchKernelGGL(k_fill_u8, dim3(256), bt, 0, st, E.pd.stale_tiles, std::max(1LL, E.total_tiles));
+  hipLaunchKernelGGL(k_raster32, gf, bt, 0, st, E.d_views, E.d_slots, E.pd, E.total_pix, 0, 1);
+  hipLaunchKernelGGL(k_shade_radar32, dim3(tiles * tiles, E.nviews), bt, 0, st, E.d_views, E.d_slots, E.pd,
+                     E.d_fit, E.d_frames, E.d_overflow, E.d_params, E.pd.nstale);
+  hipLaunchKernelGGL(k_raster32, gf, bt, 0, st, E.d_views, E.d_slots, E.pd, E.total_pix, 0, 3);
+  hipLaunchKernelGGL(k_replay32, dim3(cdivm(E.pd.stale_cap, 256)), bt, 0, st, E.d_views, E.d_slots, E.pd,
+                     E.d_fit, E.d_frames, E.d_overflow, E.d_params);
+  hipLaunchKernelGGL(k_finish_views, dim3(cdivm(E.nviews, 64)), dim3(64), 0, st, E.d_views, E.nviews, E.d_overflow,
+                     E.d_frames, E.d_view_out);
+  hipLaunchKernelGGL(k_chi2_mixed, dim3(E.nframes), bt, 0, st, E.d_frames, E.nframes, E.d_views, E.d_fit,
+                     E.d_obs32, E.d_oov32, E.d_view_out, E.opt.keep_fit);
+  hipLaunchKernelGGL(k_objective, dim3(cdivm(E.nevals, 64)), dim3(64), 0, st, E.d_evals, E.nevals, E.d_frames,
+                     E.d_views, E.d_view_out, E.d_slot_flags, E.d_slot_logf, E.d_eval_slot);
+  return hipGetLastError() == hipSuccess ? 0 : -42;
+}
+
 }  // namespace shp

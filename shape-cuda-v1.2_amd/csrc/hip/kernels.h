@@ -8,7 +8,8 @@
 
 namespace shp {
 
-constexpr int STALE_CAND = 16;  // max facets recorded per replayed pixel
+constexpr int STALE_CAND = 64;  // max facets recorded per replayed pixel
+constexpr int STALE_TILE = 8;   // stale-pixel bitmap granularity (pixels)
 
 // A pixel whose final owner has a back-facing smoothed normal; its
 // cos(scattering) is whatever an earlier claimant left (posvis.c:243-275).
@@ -28,6 +29,7 @@ struct PosDev {
   double *zill, *cosill;
   int *fill, *bodyill, *compill;
   int *stale_idx;            // per pixel: index into stale[], or -1
+  unsigned char *stale_tiles;// mixed path: per view (at ViewRec::tile_off), 1 where an 8x8 tile holds a stale pixel
   StaleRec *stale;
   int *nstale;
   int stale_cap;
@@ -70,6 +72,7 @@ namespace shp {
 __global__ void k_mark_seen(ViewRec *views, GeomSlot *slots, PosDev pd, double mincos);
 __global__ void k_fill_i32(int *p, long long n, int v);
 __global__ void k_fill_u8(unsigned char *p, long long n);
+__global__ void k_fill_f64(double *p, long long n, double v);
 __global__ void k_zero_counters(int *p, int n);
 }  // namespace shp
 

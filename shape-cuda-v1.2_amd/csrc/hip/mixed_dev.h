@@ -1,0 +1,255 @@
+// mixed_dev.h -- FP32 device helpers of the mixed method (projection, barycentric
+// test, owner shading, delay/Doppler responses, scattering laws).
+#pragma once
+
+#include "common.h"
+#include "kernels.h"
+
+namespace shp {
+
+constexpr int TILE = 16;         // resolve tiles: 16x16 pixels per workgroup, one per thread
+constexpr int WIN_MAX = 2048;    // LDS delay-Doppler window (FP64 bins, 16 KiB)
+constexpr int DW = 4, PW = 16;   // register footprint limits (delay, Doppler bins)
+
+__device__ inline int wmin(int v) { for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o, 64)); return v; }
+__device__ inline int wmax(int v) { for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, 64)); return v; }
+__device__ inline float wminf(float v) { for (int o = 32; o > 0; o >>= 1) v = fminf(v, __shfl_xor(v, o, 64)); return v; }
+__device__ inline float wmaxf(float v) { for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, 64)); return v; }
+
+__device__ inline void rot3f(const float m[9], float x, float y, float z, float &a, float &b, float &c) {
+  a = m[0] * x + m[1] * y + m[2] * z;
+  b = m[3] * x + m[4] * y + m[5] * z;
+  c = m[6] * x + m[7] * y + m[8] * z;
+}
+
+struct Proj32 {
+  float x0, y0, z0, x1, y1, z1, x2, y2, z2;  // vertices: x, y in pixel units, z in km
+  float den;
+};
+
+// project a facet's corners (posvis.c:159-166) in pixel units
+__device__ inline void corners32(const ViewRec &V, const GeomSlot &G, int f, const float *m, Proj32 &P) {
+  const int *fv = G.fv + 3 * f;
+  float4 a = G.x32[fv[0]], b = G.x32[fv[1]], c = G.x32[fv[2]];
+  float s = V.inv_kmpp32;
+  rot3f(m, a.x, a.y, a.z, P.x0, P.y0, P.z0);
+  rot3f(m, b.x, b.y, b.z, P.x1, P.y1, P.z1);
+  rot3f(m, c.x, c.y, c.z, P.x2, P.y2, P.z2);
+  P.x0 *= s; P.y0 *= s; P.x1 *= s; P.y1 *= s; P.x2 *= s; P.y2 *= s;
+  P.den = 1.0f / ((P.x1 - P.x0) * (P.y2 - P.y1) - (P.x2 - P.x1) * (P.y1 - P.y0));
+}
+
+__device__ inline bool project32(const ViewRec &V, const GeomSlot &G, int f, const float *m, Proj32 &P) {
+  float4 fn = G.fn32[f];
+  float nz = m[6] * fn.x + m[7] * fn.y + m[8] * fn.z;
+  if (!(nz > 0.0f)) return false;  // back-facing (posvis.c:152)
+  corners32(V, G, f, m, P);
+  return true;
+}
+
+// barycentric coverage test and depth (posvis.c:217-228).  The CPU admits
+// pixel centers within 1e-10 of an edge so neighbouring facets both claim a
+// shared edge; FP32 rounds s and t at ~1e-7, so the band is widened to 2e-6:
+// otherwise a center on a shared edge could be rejected by both facets.
+constexpr float EDGE_TOL = 2.0e-6f;
+__device__ inline bool hit32(const Proj32 &P, int i, int j, float &s, float &t, float &z) {
+  float xi = (float)i, yj = (float)j;
+  s = ((xi - P.x0) * (P.y2 - P.y1) - (P.x2 - P.x1) * (yj - P.y0)) * P.den;
+  if (!(s >= -EDGE_TOL && s <= 1.0f + EDGE_TOL)) return false;
+  t = ((P.x1 - P.x0) * (yj - P.y0) - (xi - P.x0) * (P.y1 - P.y0)) * P.den;
+  if (!(t >= -EDGE_TOL && t <= s + EDGE_TOL)) return false;
+  z = P.z0 + s * (P.z1 - P.z0) + t * (P.z2 - P.z1);
+  return true;
+}
+
+// normal at (s, t) of facet f in observer coordinates (posvis.c:243-258)
+__device__ inline void normal32(const GeomSlot &G, int f, const float *m, int smooth, float s, float t, float &nx,
+                                float &ny, float &nz) {
+  float bx, by, bz;
+  if (smooth) {
+    const int *fv = G.fv + 3 * f;
+    float4 na = G.vn32[fv[0]], nb = G.vn32[fv[1]], nc = G.vn32[fv[2]];
+    bx = na.x + s * (nb.x - na.x) + t * (nc.x - nb.x);
+    by = na.y + s * (nb.y - na.y) + t * (nc.y - nb.y);
+    bz = na.z + s * (nb.z - na.z) + t * (nc.z - nb.z);
+  } else {
+    float4 fn = G.fn32[f];
+    bx = fn.x; by = fn.y; bz = fn.z;
+  }
+  rot3f(m, bx, by, bz, nx, ny, nz);
+  if (smooth) {
+    float inv = rsqrtf(nx * nx + ny * ny + nz * nz);
+    nx *= inv; ny *= inv; nz *= inv;
+  }
+}
+
+__device__ inline float sinc2f(float x) {
+  float t = fabsf(3.14159265358979f * x);
+  if (t > 1.0e-4f) { t = __sinf(t) / t; return t * t; }
+  return 1.0f;
+}
+__device__ inline float trif(float x) { float t = 1.0f - fabsf(x); return t > 0.0f ? t : 0.0f; }
+
+// delay response of one image row (pos2deldop.c:371-424); drel = delPOS - idel
+__device__ inline float delay_w32(const ViewRec &V, float drel) {
+  if (V.codemethod != 1) {
+    if (V.spb == 1) { float t = trif(drel); return t * t; }
+    int sos = V.spb / V.stride;
+    float c2 = (float)V.const2;
+    if (V.spb == 2) {
+      float arg = (drel + c2) / sos;
+      float d = trif(arg) + trif(arg - 0.5f);
+      return d * (d / 4.0f);
+    }
+    float d = 0.0f, osp = 1.0f / V.spb;
+    int m_min = max((int)floorf((drel - c2) * V.stride), 0);
+    int m_max = min((int)ceilf((drel + (float)V.const1) * V.stride), V.spb) - 1;
+    float arg = (drel + c2) / sos - m_min * osp;
+    for (int m = m_min; m <= m_max; m++) { d += trif(arg); arg -= osp; }
+    return d * (d / (V.spb * V.spb));
+  }
+  float t = trif(drel / (V.spb / V.stride));
+  return t * t;
+}
+
+// radar scattering laws (radlaw.c:51-121); pl = [type, R, C, cutoff, qsR, qsC, qs_cutoff]
+__device__ inline float radlaw32(const double *pl, int type, float c) {
+  float R = (float)pl[1], Cc = (float)pl[2];
+  float d = 0.0f;
+  switch (type) {
+  case COSINELAW_DIFF: d = R * (Cc + 1.0f) * exp2f((2.0f * Cc - 1.0f) * log2f(c)); break;
+  case GAUSSIANLAW:
+    if (c >= (float)pl[3]) { float t2 = 1.0f / (c * c) - 1.0f; d = R * Cc * __expf(-Cc * t2) / (c * c * c * c * c); }
+    break;
+  case HAGFORSLAW:
+    if (c >= (float)pl[3]) { float a = c * c * c * c + Cc * (1.0f - c * c); d = 0.5f * R * Cc * rsqrtf(a * a * a) / c; }
+    break;
+  case COSINELAW_QS:
+    if (c >= (float)pl[3]) d = R * (Cc + 1.0f) * exp2f((2.0f * Cc - 1.0f) * log2f(c));
+    break;
+  default: {
+    d = R * (Cc + 1.0f) * exp2f((2.0f * Cc - 1.0f) * log2f(c));
+    float qR = (float)pl[4], qC = (float)pl[5];
+    if (c >= (float)pl[6]) {
+      if (type == GAUSSIAN_COSINE) { float t2 = 1.0f / (c * c) - 1.0f; d += qR * qC * __expf(-qC * t2) / (c * c * c * c * c); }
+      else if (type == HAGFORS_COSINE) { float a = c * c * c * c + qC * (1.0f - c * c); d += 0.5f * qR * qC * rsqrtf(a * a * a) / c; }
+      else if (type == COSINE_COSINE) d += qR * (qC + 1.0f) * exp2f((2.0f * qC - 1.0f) * log2f(c));
+    }
+  }
+  }
+  return d;
+}
+
+// One lit POS pixel's delay-Doppler footprint (pos2deldop.c:320-482 /
+// pos2doppler.c:259-362): bins relative to the anchors D0 / P0, response
+// weights and the amplitude of cross section per unit weight.
+struct RadarPix {
+  bool inb, wide;
+  float amp, delrel, doprel;
+  int id0, id1, ip0, ip1;
+  float dw[DW], pw[PW];
+};
+
+__device__ inline void radar_pixel(const ViewRec &V, const double *pl, int i, int j, float z, float cose, RadarPix &R) {
+  const bool dd = V.type == VT_DELAY;
+  int width = V.sinc2width, ns = V.nsinc2;
+  R.wide = false;
+  // Bin coordinates and the integer footprint decisions in FP64, as the CPU
+  // forms them (pos2deldop.c:320-329): a footprint that straddles a frame edge
+  // moves a pixel's whole power between the image and the overflow, so these
+  // floors must not flip under FP32 rounding.  Weights then use FP32 offsets
+  // from the anchors.
+  double dop64 = V.ax * (double)i + V.ay * (double)j + V.dopshift;
+  R.ip0 = (int)floor(dop64 - V.dopdiff_max + 1 - width / 2.0) - V.P0;
+  R.ip1 = (int)floor(dop64 + V.dopdiff_max + width / 2.0) - V.P0;
+  R.doprel = (float)(dop64 - (double)V.P0);
+  int np = R.ip1 - R.ip0 + 1;
+  float codef = 1.0f;
+  if (dd && V.codemethod == 0) codef = sinc2f((R.doprel + V.dcrel32) / (float)V.dopfftlen);
+  float sump = 0.0f;
+  if (ns == 1 && np <= PW) {
+    // sin(pi (x - k)) = +-sin(pi x): one sine serves every Doppler bin
+    float fr = R.doprel - floorf(R.doprel);
+    float sp = sinf(3.14159265358979f * fr);
+    float s2 = sp * sp * (1.0f / (3.14159265358979f * 3.14159265358979f));
+#pragma unroll
+    for (int b = 0; b < PW; b++) {
+      if (b < np) {
+        float a = R.doprel - (float)(R.ip0 + b);
+        float v = (fabsf(3.14159265358979f * a) > 1.0e-4f) ? s2 / (a * a) : 1.0f;
+        R.pw[b] = v;
+        sump += v;
+      }
+    }
+  } else if (np <= PW) {
+    float xi = (float)V.xincr, yi = (float)V.yincr, bl = (float)V.dopdiff_bl;
+#pragma unroll
+    for (int b = 0; b < PW; b++) {
+      if (b < np) {
+        float al = R.doprel + bl - (float)(R.ip0 + b), acc = 0.0f;
+        for (int q = 0; q < ns; q++) {
+          float g = al;
+          for (int r = 0; r < ns; r++) { acc += sinc2f(g); g += xi; }
+          al += yi;
+        }
+        R.pw[b] = acc / (ns * ns);
+        sump += R.pw[b];
+      }
+    }
+  } else {
+    R.wide = true;
+  }
+  float sumd = 1.0f;
+  R.delrel = 0.0f;
+  R.id0 = 0;
+  R.id1 = 0;
+  if (dd) {
+    double del64 = (double)z * V.delfact + V.delshift;
+    R.id0 = (int)floor(del64 - V.const1) + 1 - V.D0;
+    R.id1 = (int)ceil(del64 + V.const1) - 1 - V.D0;
+    R.delrel = (float)(del64 - (double)V.D0);
+    int nd = R.id1 - R.id0 + 1;
+    sumd = 0.0f;
+    if (nd <= DW) {
+#pragma unroll
+      for (int a = 0; a < DW; a++)
+        if (a < nd) { R.dw[a] = delay_w32(V, R.delrel - (float)(R.id0 + a)); sumd += R.dw[a]; }
+    } else {
+      R.wide = true;
+    }
+    R.inb = (V.D0 + R.id0 >= 1) && (V.D0 + R.id1 <= V.ndel) && (V.P0 + R.ip0 >= 1) && (V.P0 + R.ip1 <= V.ndop);
+  } else {
+    R.inb = (V.P0 + R.ip0 >= 1) && (V.P0 + R.ip1 <= V.ndop);
+  }
+  R.amp = R.wide ? 0.0f : radlaw32(pl, V.law_type, cose) * V.kmpp32 * V.kmpp32 * codef / (sumd * sump);
+}
+
+// scatter one pixel straight into the HBM fit image (FP32 atomics) or, for
+// Doppler pixels beyond the spectrum, into the FP64 overflow spectrum
+__device__ inline void radar_scatter_global(const ViewRec &V, const RadarPix &R, double *img, double *ov) {
+  const bool dd = V.type == VT_DELAY;
+  int np = R.ip1 - R.ip0 + 1, nd = dd ? R.id1 - R.id0 + 1 : 1, ndop = V.ndop;
+  if (R.wide) return;
+  if (R.inb) {
+#pragma unroll
+    for (int a = 0; a < DW; a++) {
+      if (a < nd) {
+        float ad = dd ? R.amp * R.dw[a] : R.amp;
+        int idel = dd ? V.D0 + R.id0 + a : 1;
+#pragma unroll
+        for (int b = 0; b < PW; b++)
+          if (b < np) atomicAdd(&img[(long long)(idel - 1) * ndop + (V.P0 + R.ip0 + b - 1)], (double)(ad * R.pw[b]));
+      }
+    }
+  } else if (!dd) {
+    int idop0 = V.idop0;
+    int lo = max(V.P0 + R.ip0, -idop0), hi = min(V.P0 + R.ip1, -idop0 + 2000 - 1);
+#pragma unroll
+    for (int b = 0; b < PW; b++) {
+      int idop = V.P0 + R.ip0 + b;
+      if (idop >= lo && idop <= hi) atomicAdd(&ov[idop + idop0], (double)(R.amp * R.pw[b]));
+    }
+  }
+}
+
+}  // namespace shp

@@ -129,6 +129,23 @@ def load_hip():
             "shape_hip_set_strict": (None, [C.c_int]),
             "shape_hip_device_count": (C.c_int, []),
             "shape_hip_realize_delcor": (None, [vp, vp, vp, C.c_double, C.c_int]),
+            "shape_hip_last_replayed": (C.c_int, []),
+            "shape_batch_create": (vp, [vp, C.POINTER(vp), C.POINTER(vp), C.c_int, C.c_int,
+                                        C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(vp),
+                                        C.POINTER(C.c_double), C.c_int, C.c_int]),
+            "shape_batch_set_values": (C.c_int, [vp, C.POINTER(C.c_double)]),
+            "shape_batch_launch": (C.c_int, [vp]),
+            "shape_batch_wait": (C.c_int, [vp, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
+            "shape_batch_run": (C.c_int, [vp, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
+            "shape_batch_last_ms": (C.c_float, [vp]),
+            "shape_batch_nframes": (C.c_int, [vp, C.c_int]),
+            "shape_batch_frame_results": (C.c_int, [vp, C.c_int, C.POINTER(C.c_double),
+                                                    C.POINTER(C.c_double)]),
+            "shape_batch_fit": (C.c_int, [vp, C.c_int, C.c_int, C.POINTER(C.c_double)]),
+            "shape_batch_frame_terms": (C.c_int, [vp, C.c_int, C.POINTER(C.c_double)]),
+            "shape_batch_counters": (C.c_int, [vp, C.POINTER(C.c_int)]),
+            "shape_batch_sizes": (C.c_int, [vp, C.POINTER(C.c_longlong)]),
+            "shape_batch_destroy": (None, [vp]),
         }
         for name, (res, args) in sigs.items():
             fn = getattr(lib, name)
@@ -382,3 +399,109 @@ class Scene:
                 for f in range(self.nframes(s)):
                     out.append((s, f))
         return out
+
+
+METHOD_FP64, METHOD_MIXED = 0, 1
+BATCH_GRAPH, BATCH_KEEP_FIT = 1, 2
+
+
+class Batch:
+    """Many objective evaluations in one device-resident replay.
+
+    scenes: list of Scene (one model each; all share the first scene's par_t).
+    evals:  list of (scene_index, [(param_index, value), ...]).
+    """
+
+    def __init__(self, scenes, evals, method=METHOD_MIXED, flags=BATCH_GRAPH):
+        self.lib = load_hip()
+        self.scenes = scenes
+        nm = len(scenes)
+        mods = (C.c_void_p * nm)(*[s.mod for s in scenes])
+        dats = (C.c_void_p * nm)(*[s.dat for s in scenes])
+        self.nevals = len(evals)
+        self.eval_model = (C.c_int * self.nevals)(*[m for m, _ in evals])
+        self.ntarg = (C.c_int * self.nevals)(*[len(t) for _, t in evals])
+        addrs, vals = [], []
+        for m, tv in evals:
+            for pi, v in tv:
+                addrs.append(C.cast(scenes[m].lib.shp_param_ptr(scenes[m].h, pi), C.c_void_p).value)
+                vals.append(v)
+        self.nvals = len(vals)
+        tarr = (C.c_void_p * max(1, len(addrs)))(*addrs)
+        varr = (C.c_double * max(1, len(vals)))(*vals)
+        self.h = self.lib.shape_batch_create(scenes[0].par, mods, dats, nm, self.nevals, self.eval_model,
+                                             self.ntarg, tarr, varr, method, flags)
+        if not self.h:
+            raise RuntimeError("shape_batch_create failed")
+
+    def set_values(self, values):
+        arr = (C.c_double * max(1, self.nvals))(*values)
+        rc = self.lib.shape_batch_set_values(self.h, arr)
+        if rc:
+            raise RuntimeError("shape_batch_set_values failed (%d)" % rc)
+
+    def run(self):
+        obj = np.zeros(self.nevals)
+        chi = np.zeros(self.nevals)
+        dp = C.POINTER(C.c_double)
+        rc = self.lib.shape_batch_run(self.h, obj.ctypes.data_as(dp), chi.ctypes.data_as(dp))
+        if rc:
+            raise RuntimeError("shape_batch_run failed (%d)" % rc)
+        return obj, chi
+
+    def launch(self):
+        rc = self.lib.shape_batch_launch(self.h)
+        if rc:
+            raise RuntimeError("shape_batch_launch failed (%d)" % rc)
+
+    def wait(self, want=True):
+        dp = C.POINTER(C.c_double)
+        if not want:
+            rc = self.lib.shape_batch_wait(self.h, None, None)
+            if rc:
+                raise RuntimeError("shape_batch_wait failed (%d)" % rc)
+            return None
+        obj, chi = np.zeros(self.nevals), np.zeros(self.nevals)
+        rc = self.lib.shape_batch_wait(self.h, obj.ctypes.data_as(dp), chi.ctypes.data_as(dp))
+        if rc:
+            raise RuntimeError("shape_batch_wait failed (%d)" % rc)
+        return obj, chi
+
+    def last_ms(self):
+        return float(self.lib.shape_batch_last_ms(self.h))
+
+    def frame_results(self, e):
+        nf = self.lib.shape_batch_nframes(self.h, e)
+        chi, cal = np.zeros(nf), np.zeros(nf)
+        dp = C.POINTER(C.c_double)
+        self.lib.shape_batch_frame_results(self.h, e, chi.ctypes.data_as(dp), cal.ctypes.data_as(dp))
+        return chi, cal
+
+    def fit(self, e, q, shape):
+        out = np.zeros(int(np.prod(shape)))
+        rc = self.lib.shape_batch_fit(self.h, e, q, out.ctypes.data_as(C.POINTER(C.c_double)))
+        if rc:
+            raise RuntimeError("shape_batch_fit failed (%d)" % rc)
+        return out.reshape(shape)
+
+    def counters(self):
+        out = (C.c_int * 4)()
+        self.lib.shape_batch_counters(self.h, out)
+        return list(out)
+
+    def sizes(self):
+        out = (C.c_longlong * 8)()
+        self.lib.shape_batch_sizes(self.h, out)
+        return dict(zip(["nevals", "nviews", "nframes", "nslots", "total_pix", "fit_len", "max_nf",
+                         "max_pix"], list(out)))
+
+    def close(self):
+        if self.h:
+            self.lib.shape_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,94 @@
+"""Batched evaluation (FP64 and mixed methods) against the FP64 CPU oracle.
+
+Tolerances: FP64 method -- objective equal to 1e-10 relative (the same
+kernels as the drop-in); mixed method -- objective and every frame's chi2
+within 1e-6 relative, model images within 1e-4 of each frame's peak.
+"""
+import numpy as np
+import pytest
+
+import helpers as H
+
+sh = H.sh
+pytestmark = pytest.mark.gpu
+
+TRIALS = [[], [(0, 1.02)], [(1, 0.97), (2, 1.01)], [(3, 25.2)], [(4, 2.1)], [(6, 0.25)], [(7, 1.9)],
+          [(8, 0.3)], [(9, -0.1)]]
+
+
+def _oracle_eval(orc, sc, trial):
+    saved = [sc.get_param(i) for i in range(sc.nparams())]
+    for i, v in trial:
+        sc.set_param(i, v)
+    H.oracle_forward(orc, sc)
+    obj = orc.oracle_objective_tail(sc.par, sc.dat)
+    chi = orc.oracle_chi2(sc.par, sc.dat, 0)
+    frames = []
+    for s, f in sc.all_frames():
+        frames.append((sc.scalars(s, f)["chi2"] if f >= 0 else None, sc.image(s, max(f, 0), 0)))
+    for i, v in enumerate(saved):
+        sc.set_param(i, v)
+    return obj, chi, frames
+
+
+def _check(config, method, over, tol_chi, tol_img, flags=sh.BATCH_KEEP_FIT):
+    orc = H.load_oracle()
+    sc = H.make_scene_with_obs(config, **over)
+    ntr = [t for t in TRIALS if all(i < sc.nparams() for i, _ in t)]
+    B = sh.Batch([sc], [(0, t) for t in ntr], method=method, flags=flags)
+    obj, chi = B.run()
+    cnt = B.counters()
+    print("config", config, "method", method, "counters", cnt, "sizes", B.sizes())
+    assert cnt[1] == 0 and cnt[3] == 0
+    worst_chi, worst_img = 0.0, 0.0
+    for e, t in enumerate(ntr):
+        o_obj, o_chi, frames = _oracle_eval(orc, sc, t)
+        assert abs(obj[e] - o_obj) <= tol_chi * abs(o_obj), (e, t, obj[e], o_obj)
+        assert abs(chi[e] - o_chi) <= tol_chi * abs(o_chi), (e, t, chi[e], o_chi)
+        worst_chi = max(worst_chi, abs(chi[e] - o_chi) / abs(o_chi))
+        fchi, _ = B.frame_results(e)
+        for q, (c, img) in enumerate(frames):
+            if c is not None:
+                assert abs(fchi[q] - c) <= tol_chi * abs(c), (e, q, fchi[q], c)
+            if e == 0:
+                got = B.fit(e, q, img.shape)
+                err = np.max(np.abs(got - img)) / max(np.max(np.abs(img)), 1e-300)
+                worst_img = max(worst_img, err)
+    print("worst chi2 rel err", worst_chi, "worst image rel err", worst_img)
+    assert worst_img <= tol_img
+
+
+@pytest.mark.parametrize("config,over", [(1, {}), (2, dict(n_deldop=16)),
+                                         (1, dict(n_deldop=0, n_doppler=2, nbin_min=30, nbin_max=40)),
+                                         (3, dict(n_deldop=3, n_doppler=3, n_lightcurves=2, lc_points=30,
+                                                  lc_ncalc=10))])
+def test_fp64_batch_matches_oracle(config, over):
+    _check(config, sh.METHOD_FP64, over, 1e-10, 1e-12)
+
+
+@pytest.mark.parametrize("config,over", [(1, {}), (2, dict(n_deldop=16)),
+                                         (1, dict(n_deldop=0, n_doppler=2, nbin_min=30, nbin_max=40)),
+                                         (5, dict(n_deldop=6))])
+def test_mixed_batch_matches_oracle(config, over):
+    _check(config, sh.METHOD_MIXED, over, 1e-6, 1e-4)
+
+
+@pytest.mark.parametrize("method", [sh.METHOD_FP64, sh.METHOD_MIXED])
+def test_graph_replays_match_eager(method):
+    """Replays of the captured graph, with new trial values between them and
+    eager work in between, reproduce eager evaluations of the same values."""
+    sc = H.make_scene_with_obs(2, n_deldop=8)
+    trials = [[(0, 1.0)], [(0, 1.01)], [(0, 0.99)]]
+    G = sh.Batch([sc], [(0, t) for t in trials], method=method, flags=sh.BATCH_GRAPH)
+    Eg = sh.Batch([sc], [(0, t) for t in trials], method=method, flags=0)
+    for k, scale in enumerate((1.0, 1.02, 0.98, 1.0)):
+        vals = [v * scale for t in trials for _, v in t]
+        G.set_values(vals)
+        Eg.set_values(vals)
+        og, cg = G.run()
+        oe, ce = Eg.run()
+        # FP64: same kernels and order of everything but atomic accumulations (1e-12);
+        # mixed: FP32 atomic accumulation order varies from run to run (1e-6)
+        tol = 1e-12 if method == sh.METHOD_FP64 else 1e-6
+        assert np.allclose(og, oe, rtol=tol, atol=0), (k, og, oe)
+        assert np.allclose(cg, ce, rtol=tol, atol=0), (k, cg, ce)
