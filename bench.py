@@ -1,0 +1,276 @@
+#!/usr/bin/env python3
+"""Chi-square evaluations/sec of SHAPE's model-evaluation path on one MI355X.
+
+Workload (default, the large-model headline): a 163,842-vertex / 327,680-facet
+icosphere model (0.9 x 0.72 x 0.54 km, 0.2% radial noise), POS 1023 x 1023 at
+2 m/px, 256 delay-Doppler frames of 128-512 rows x 64-256 columns, one view
+per frame, cosine radar law.  Synthetic: the scene comes from the package's
+scene builder with fixed seeds; the "observed" images are the plain FP64 GPU
+drop-in's fits plus Gaussian noise.
+
+One step is one replay of a batch of `--evals` objective evaluations (trial
+values of the first size parameter, as in a line search): vertex realization,
+rasterization, delay-Doppler synthesis, chi-square and the objective of every
+evaluation, on the device.  The timed region launches K replays, each followed
+by a wait on its stream, so every step is a complete, synchronous evaluation
+round as the fitting loop uses it.
+
+Printed on one JSON line: the mixed method's evaluations/sec (`value`), the
+plain FP64 drop-in and the device-resident FP64 batch beside it, the mixed
+method's chi-square error against the FP64 method on the same evaluations,
+the roofline of the dominant kernel (algorithmic bytes / event-timed launch
+duration on the batch's stream), and the oracle (CPU restatement of the
+reference's FP64 path) timed on a bounded sample on the host cores.
+"""
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "shape-cuda-v1.2_amd"))
+
+import shape_hip as sh  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+NFRAMES_FULL = 256
+KSTEPS = 5
+
+
+def log(msg):
+    print("[bench] " + msg, file=sys.stderr, flush=True)
+
+
+def parse():
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--evals", type=int, default=4, help="objective evaluations per replay")
+    ap.add_argument("--frames", type=int, default=NFRAMES_FULL, help="delay-Doppler frames (256: full workload)")
+    ap.add_argument("--fp64-steps", type=int, default=3)
+    ap.add_argument("--kernel-steps", type=int, default=5, help="eager replays timed per kernel")
+    ap.add_argument("--cpu-frames", type=int, default=64, help="frames in the CPU-baseline sample")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--mixed-only", action="store_true",
+                    help="time only the mixed replays (for profiler runs)")
+    return ap.parse_args()
+
+
+# ---------------------------------------------------------------------------
+# scene and observations
+# ---------------------------------------------------------------------------
+def make_scene(frames):
+    sc = sh.Scene(sh.spec_for(4, n_deldop=frames))
+    hip = sh.load_hip()
+    # fits of the plain FP64 drop-in are the noiseless "truth"
+    hip.shape_hip_clear_error()
+    hip.realize_mod(sc.par, sc.mod)
+    hip.realize_spin(sc.par, sc.mod, sc.dat)
+    hip.realize_delcor(sc.dat, 0.0, 0)
+    hip.calc_fits(sc.par, sc.mod, sc.dat)
+    err = hip.shape_hip_last_error()
+    if err:
+        raise RuntimeError("FP64 drop-in failed while synthesizing observations (%d)" % err)
+    sc.synthesize_obs(snr=20.0)
+    return sc
+
+
+def trial_evals(sc, n):
+    """n trial values of parameter 0 (first scale factor) around its value."""
+    p0 = sc.get_param(0)
+    vals = [p0 * (1.0 + 0.01 * (k - (n - 1) / 2.0)) for k in range(n)]
+    return [(0, [(0, v)]) for v in vals]
+
+
+# ---------------------------------------------------------------------------
+# timing
+# ---------------------------------------------------------------------------
+def time_batch(B, steps, warmup):
+    for _ in range(warmup):
+        B.launch()
+        B.wait(want=False)
+    lat = []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        B.launch()
+        B.wait(want=False)
+        lat.append(B.last_ms())
+    t1 = time.perf_counter()
+    return (t1 - t0), np.array(lat)
+
+
+def time_dropin(sc, steps):
+    hip = sh.load_hip()
+    hip.shape_hip_clear_error()
+    chis = []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        hip.realize_mod(sc.par, sc.mod)
+        hip.realize_spin(sc.par, sc.mod, sc.dat)
+        hip.realize_delcor(sc.dat, 0.0, 0)
+        hip.calc_fits(sc.par, sc.mod, sc.dat)
+        chis.append(hip.chi2(sc.par, sc.dat, 0))
+    t1 = time.perf_counter()
+    if hip.shape_hip_last_error():
+        raise RuntimeError("FP64 drop-in failed (%d)" % hip.shape_hip_last_error())
+    return (t1 - t0) / steps, chis[-1]
+
+
+# Algorithmic HBM bytes per launch of the mixed kernels (one launch covers every
+# evaluation of the batch).  fp: footprint of the replay; claimed: z-buffer keys
+# written by the raster (one per covered pixel).
+def kernel_bytes(name, fp, claimed):
+    model = 16 * fp["nv"] + 28 * fp["nf"]       # float4 vertices; 3 indices + float4 normal per facet
+    if name == "k_raster32/1":
+        return model + 8 * claimed                # one 8-byte key per covered pixel
+    if name == "k_shade_radar32":
+        # read every key of the views' model rectangles, clear the claimed
+        # ones, write each occupied image cell once (FP64)
+        return 8 * fp["bbox_pix"] + 8 * claimed + 8 * fp["window_cells"]
+    if name == "k_chi2_mixed":
+        return 24 * fp["window_cells"]            # fit read + clear (FP64), obs*w and w (FP32)
+    return None
+
+
+def kernel_roofline(sc, nev):
+    B = sh.Batch([sc], trial_evals(sc, nev), method=sh.METHOD_MIXED, flags=0)
+    B.run()
+    B.time_kernels(True)
+    for _ in range(KSTEPS):
+        B.launch()
+        B.wait(want=False)
+    stats = B.kernel_stats()
+    fp = B.footprint()
+    claimed = fp["claimed"]
+    B.close()
+    rows = {}
+    for name, (ms, n) in stats.items():
+        avg = ms / max(n, 1)
+        by = kernel_bytes(name, fp, claimed)
+        rows[name] = {"avg_ms": round(avg, 4), "launches": n,
+                      "GBps": None if by is None or avg <= 0 else round(by / (avg * 1e-3) / 1e9, 1),
+                      "bytes": by}
+    dom = max(stats.items(), key=lambda kv: kv[1][0])[0]
+    return dom, rows, fp, claimed
+
+
+# ---------------------------------------------------------------------------
+# CPU baseline: the oracle on the first --cpu-frames frames, all host threads
+# ---------------------------------------------------------------------------
+def cpu_baseline(nframes):
+    path = os.path.join(ROOT, "oracle", "_build", "liboracle.so")
+    if not os.path.exists(path):
+        return None
+    orc = C.CDLL(path)
+    vp = C.c_void_p
+    for name, res, args in [("oracle_realize_mod", None, [vp, vp]), ("oracle_realize_spin", None, [vp, vp, vp]),
+                            ("oracle_realize_delcor", None, [vp, C.c_double, C.c_int]),
+                            ("oracle_calc_fits_threads", None, [vp, vp, vp, C.c_int]),
+                            ("oracle_chi2", C.c_double, [vp, vp, C.c_int])]:
+        fn = getattr(orc, name)
+        fn.restype, fn.argtypes = res, args
+    threads = max(1, min(16, len(os.sched_getaffinity(0))))
+    sc = sh.Scene(sh.spec_for(4, n_deldop=nframes))
+
+    def forward():
+        orc.oracle_realize_mod(sc.par, sc.mod)
+        orc.oracle_realize_spin(sc.par, sc.mod, sc.dat)
+        orc.oracle_realize_delcor(sc.dat, 0.0, 0)
+        orc.oracle_calc_fits_threads(sc.par, sc.mod, sc.dat, threads)
+        return orc.oracle_chi2(sc.par, sc.dat, 0)
+
+    forward()
+    sc.synthesize_obs(snr=20.0)
+    t0 = time.perf_counter()
+    forward()
+    dt = time.perf_counter() - t0
+    sc.close()
+    return {"value": round(nframes / (NFRAMES_FULL * dt), 4), "unit": "chi-square evals/s",
+            "cores": threads, "kind": "port",
+            "sample": "oracle (FP64 CPU restatement) realize+calc_fits+chi2 over the first %d of the "
+                      "%d frames, frames spread over %d threads, %.2f s wall; scaled to one "
+                      "%d-frame evaluation" % (nframes, NFRAMES_FULL, threads, dt, NFRAMES_FULL)}
+
+
+def main():
+    global KSTEPS
+    a = parse()
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if a.gpus != 1 or world != 1:
+        print("bench.py measures one GPU: this build runs the evaluation path on a single MI355X "
+              "(--gpus %d requested). Run with --gpus 1." % a.gpus, file=sys.stderr)
+        sys.exit(2)
+    KSTEPS = a.kernel_steps
+    hip = sh.load_hip()
+    if hip.shape_hip_device_count() < 1:
+        raise RuntimeError("no HIP device")
+    log("building scene: %d frames" % a.frames)
+    sc = make_scene(a.frames)
+    evals = trial_evals(sc, a.evals)
+
+    log("mixed batch: %d evaluations per replay" % a.evals)
+    Bm = sh.Batch([sc], evals, method=sh.METHOD_MIXED, flags=sh.BATCH_GRAPH)
+    obj_m, chi_m = Bm.run()
+    t_m, lat_m = time_batch(Bm, a.steps, a.warmup)
+    obj_m, chi_m = Bm.run()
+    cnt = Bm.counters()
+    fpm = Bm.footprint()
+    Bm.close()
+    evals_per_s = a.evals * a.steps / t_m
+    ms_step = 1e3 * t_m / a.steps
+    log("mixed: %.2f evals/s, %.3f ms/step" % (evals_per_s, ms_step))
+    out = {"metric": "chi-square evals/sec (batched mixed vs plain FP64) at <=1e-6 rel chi2 err",
+           "value": round(evals_per_s, 3), "unit": "chi-square evals/s", "n_gpus": 1,
+           "steps": a.steps, "warmup": a.warmup, "ms_per_step": round(ms_step, 4),
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32+f64",
+           "data": "synthetic (seeded scene builder; observations = FP64 drop-in fits + noise)",
+           "config": {"workload": "large model: 163842 V / 327680 F icosphere, POS 1023^2 at 2 m/px, "
+                                  "%d delay-Doppler frames 128-512 x 64-256" % a.frames,
+                      "evals_per_replay": a.evals, "method": "mixed", "graph": True},
+           "replay_ms_p50": round(float(np.percentile(lat_m, 50)), 4),
+           "replay_ms_p99": round(float(np.percentile(lat_m, 99)), 4),
+           "replayed_pixels": cnt[0], "claimed_keys": fpm["claimed"]}
+    if a.mixed_only:
+        print(json.dumps(out), flush=True)
+        return
+
+    log("FP64 batch")
+    Bf = sh.Batch([sc], evals, method=sh.METHOD_FP64, flags=sh.BATCH_GRAPH)
+    obj_f, chi_f = Bf.run()
+    t_f, _ = time_batch(Bf, a.fp64_steps, 1)
+    Bf.close()
+    out["fp64_batch_evals_per_s"] = round(a.evals * a.fp64_steps / t_f, 3)
+    rel = np.abs(chi_m - chi_f) / np.abs(chi_f)
+    out["chi2_rel_err_vs_fp64"] = float(np.max(rel))
+    out["objective_rel_err_vs_fp64"] = float(np.max(np.abs(obj_m - obj_f) / np.abs(obj_f)))
+
+    log("FP64 drop-in")
+    t_d, _ = time_dropin(sc, a.fp64_steps)
+    out["fp64_dropin_evals_per_s"] = round(1.0 / t_d, 3)
+    out["speedup_vs_fp64_dropin"] = round(evals_per_s * t_d, 2)
+    out["speedup_vs_fp64_batch"] = round(evals_per_s / out["fp64_batch_evals_per_s"], 2)
+
+    log("per-kernel timing (eager replays)")
+    dom, rows, fp, claimed = kernel_roofline(sc, a.evals)
+    r = rows[dom]
+    ach = r["GBps"]
+    out["roofline"] = {"kernel": dom, "bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                       "frac": None if ach is None else round(ach / HBM_PEAK_GBS, 4),
+                       "traffic": None, "avg_ms": r["avg_ms"], "algorithmic_bytes": r["bytes"]}
+    out["kernels"] = rows
+    out["footprint"] = fp
+    sc.close()
+
+    if not a.no_cpu:
+        log("CPU baseline (oracle, %d-frame sample)" % a.cpu_frames)
+        out["cpu_baseline"] = cpu_baseline(a.cpu_frames)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

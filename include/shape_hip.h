@@ -87,9 +87,25 @@ float shape_batch_last_ms(shape_batch *b);              /* device time of the la
 int shape_batch_nframes(shape_batch *b, int e);
 int shape_batch_frame_results(shape_batch *b, int e, double *chi2v, double *cal);
 int shape_batch_fit(shape_batch *b, int e, int frame, double *out);
+/* per frame of evaluation e: o2, m2, om, overflow_o2, overflow_m2, cal, chi2,
+   badradar_logfactor (8 doubles per frame, set/frame order) */
+int shape_batch_frame_terms(shape_batch *b, int e, double *out);
+/* [0] replayed pixels, [1] error flags, [2] unused,
+   [3] mixed: footprints too wide for the register window */
 int shape_batch_counters(shape_batch *b, int *out4);
+/* nevals, nviews, nframes, geometry slots, POS pixels, fit cells, max facets, max POS pixels */
 int shape_batch_sizes(shape_batch *b, long long *out8);
 void shape_batch_destroy(shape_batch *b);
+
+/* measurement: the batch's hipStream_t; per-kernel HIP-event timing of eager
+   replays (batches created without SHAPE_BATCH_GRAPH); and the last replay's
+   footprint: model-rectangle pixels, POS pixels, occupied image cells, image
+   cells, vertices, facets, claimed z-buffer keys (7 values) */
+void *shape_batch_stream(shape_batch *b);
+int shape_batch_time_kernels(shape_batch *b, int on);
+int shape_batch_kernel_stat(shape_batch *b, int idx, char *name, int namelen, double *total_ms,
+                            long long *launches);
+int shape_batch_footprint(shape_batch *b, long long *out7);
 
 #ifdef __cplusplus
 }

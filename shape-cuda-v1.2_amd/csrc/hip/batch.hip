@@ -9,6 +9,7 @@
 // from a host staging copy as the graph's first node.
 #include <string.h>
 
+#include <algorithm>
 #include <vector>
 
 #include "common.h"
@@ -109,7 +110,7 @@ extern "C" int shape_batch_run(shape_batch *B, double *objective, double *chi2v)
 
 extern "C" float shape_batch_last_ms(shape_batch *B) { return B->last_ms; }
 
-extern "C" hipStream_t shape_batch_stream(shape_batch *B) { return B->E.stream; }
+extern "C" void *shape_batch_stream(shape_batch *B) { return (void *)B->E.stream; }
 
 extern "C" int shape_batch_nframes(shape_batch *B, int e) {
   return (e >= 0 && e < B->E.nevals) ? B->E.h_evals[e].nframes : -1;
@@ -156,8 +157,9 @@ extern "C" int shape_batch_fit(shape_batch *B, int e, int q, double *out) {
   return 0;
 }
 
-// counters of the last replay: [0] replayed / inherited-normal pixels,
-// [1] error flags, [2] mixed-path pixels whose owner faces away, [3] footprints too wide
+// counters of the last replay: [0] replayed pixels (FP64 method: owner faces
+// away; mixed: owner faces away or pixel on a facet edge), [1] error flags,
+// [2] unused, [3] mixed: footprints too wide
 extern "C" int shape_batch_counters(shape_batch *B, int *out4) {
   return hipMemcpy(out4, B->E.pd.nstale, sizeof(int) * 4, hipMemcpyDeviceToHost) == hipSuccess ? 0 : -4;
 }
@@ -166,5 +168,75 @@ extern "C" int shape_batch_sizes(shape_batch *B, long long *out8) {
   Engine &E = B->E;
   out8[0] = E.nevals; out8[1] = E.nviews; out8[2] = E.nframes; out8[3] = E.nslots;
   out8[4] = E.total_pix; out8[5] = E.fit_len; out8[6] = E.max_nf; out8[7] = E.max_pix;
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
+// measurement helpers (benchmarks and profiling)
+// ---------------------------------------------------------------------------
+// Time every kernel of eager launches with HIP events on the batch's stream
+// (batches created without SHAPE_BATCH_GRAPH only).  Totals accumulate over
+// launches until shape_batch_time_kernels is called again with on = 1.
+extern "C" int shape_batch_time_kernels(shape_batch *B, int on) {
+  if (B->use_graph) return -21;
+  B->E.time_kernels = on != 0;
+  if (on) { B->E.kt_names.clear(); B->E.kt_ms.clear(); B->E.kt_count.clear(); }
+  return 0;
+}
+
+// Entry idx of the kernel totals: name, total milliseconds and launches.
+// Returns the number of entries, or -1 when idx is out of range.
+extern "C" int shape_batch_kernel_stat(shape_batch *B, int idx, char *name, int namelen, double *total_ms,
+                                       long long *launches) {
+  int rc = B->E.collect_kernel_times();
+  if (rc) return rc;
+  int nk = (int)B->E.kt_names.size();
+  if (idx < 0 || idx >= nk) return -1;
+  if (name && namelen > 0) {
+    strncpy(name, B->E.kt_names[idx].c_str(), (size_t)namelen - 1);
+    name[namelen - 1] = 0;
+  }
+  if (total_ms) *total_ms = B->E.kt_ms[idx];
+  if (launches) *launches = B->E.kt_count[idx];
+  return nk;
+}
+
+// Footprint of the last replay: [0] POS pixels inside the views' model
+// rectangles, [1] pixels of all views' POS grids, [2] image cells inside the
+// frames' occupied windows, [3] image cells of all radar frames,
+// [4] vertices and [5] facets summed over the geometry slots, [6] z-buffer
+// keys claimed (mixed method).
+extern "C" int shape_batch_footprint(shape_batch *B, long long *out7) {
+  long long *out6 = out7;
+  std::vector<ViewRec> vs;
+  std::vector<FrameRec> fr;
+  int rc = B->E.download_views(vs);
+  if (!rc) rc = B->E.download_frames(fr);
+  if (rc) return rc;
+  long long bbox = 0, grid = 0, win = 0, cells = 0, nv = 0, nf = 0, claimed = 0;
+  for (const ViewRec &V : vs) {
+    long long w = 2LL * V.n + 1;
+    grid += w * w;
+    claimed += V.nclaimed;
+    if (V.xlim[1] >= V.xlim[0] && V.ylim[1] >= V.ylim[0])
+      bbox += (long long)(V.xlim[1] - V.xlim[0] + 1) * (V.ylim[1] - V.ylim[0] + 1);
+  }
+  for (const FrameRec &F : fr) {
+    if (F.type != VT_DELAY && F.type != VT_DOPPLER) continue;
+    int r0 = 1, r1 = F.type == VT_DELAY ? F.ndel : 1, c0 = 0x7fffffff, c1 = -0x7fffffff;
+    int d0 = 0x7fffffff, d1 = -0x7fffffff;
+    for (int v = 0; v < F.nviews; v++) {
+      const ViewRec &V = vs[F.first_view + v];
+      c0 = std::min(c0, V.idoplim[0]); c1 = std::max(c1, V.idoplim[1]);
+      d0 = std::min(d0, V.idellim[0]); d1 = std::max(d1, V.idellim[1]);
+    }
+    if (F.type == VT_DELAY) { r0 = std::max(1, d0); r1 = std::min(F.ndel, d1); }
+    c0 = std::max(1, c0); c1 = std::min(F.ndop, c1);
+    if (r1 >= r0 && c1 >= c0) win += (long long)(r1 - r0 + 1) * (c1 - c0 + 1);
+    cells += (long long)(F.type == VT_DELAY ? F.ndel : 1) * F.ndop;
+  }
+  for (const GeomSlot &G : B->E.h_slots) { nv += G.nv; nf += G.nf; }
+  out6[0] = bbox; out6[1] = grid; out6[2] = win; out6[3] = cells; out6[4] = nv; out6[5] = nf;
+  out6[6] = claimed;
   return 0;
 }

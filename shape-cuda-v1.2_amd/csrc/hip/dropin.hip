@@ -74,7 +74,7 @@ extern "C" int shape_hip_last_replayed(void) { return g_last_nstale; }
 // ---------------------------------------------------------------------------
 struct DevBuf {
   void *p = nullptr;
-  ~DevBuf() { if (p) hipFree(p); }
+  ~DevBuf() { if (p) (void)hipFree(p); }
   int get(size_t bytes) { return hipMalloc(&p, bytes ? bytes : 8) == hipSuccess ? 0 : -2; }
 };
 

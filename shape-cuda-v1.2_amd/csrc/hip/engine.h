@@ -101,7 +101,7 @@ struct ViewRec {
   float dfr, pfr, delfact32, ax32, ay32, const1_32, kmpp32, inv_kmpp32, dcrel32;
   // ---- accumulated by kernels ----
   int xlim[2], ylim[2], xlim2[2], ylim2[2];
-  int outbnd, outbnd2, any_overflow, nstale;
+  int outbnd, outbnd2, any_overflow, nstale, nclaimed;  // nclaimed: mixed z-buffer keys claimed
   unsigned long long ov[4], ov2[4];       // ordered doubles: imin, -imax, jmin, -jmax (min-reduced)
   int idellim[2], idoplim[2];
   unsigned long long dellim[2], doplim[2]; // ordered doubles: min delPOS, -max delPOS ...

@@ -13,11 +13,10 @@ extern "C" int shp_fail(hipError_t e) { return (int)e == 0 ? -1 : (int)e; }
 
 namespace shp {
 
-static int g_last_err = 0;
-
 Engine::~Engine() {
-  if (graph_exec) hipGraphExecDestroy(graph_exec);
-  if (graph) hipGraphDestroy(graph);
+  if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
+  if (graph) (void)hipGraphDestroy(graph);
+  for (hipEvent_t e : t_events) (void)hipEventDestroy(e);
   for (void *p : allocs) (void)hipFree(p);
   if (h_pin) (void)hipHostFree(h_pin);
   if (stream) (void)hipStreamDestroy(stream);
@@ -627,7 +626,9 @@ int Engine::build(struct par_t *par, struct mod_t **mods, struct dat_t **dats, i
     if ((rc = alloc((void **)&pd.stale_tiles, (size_t)std::max(1LL, total_tiles)))) return rc;
     SHP_CHECK(hipMemsetAsync(pd.stale_tiles, 0, (size_t)std::max(1LL, total_tiles), stream));
   }
-  pd.stale_cap = (int)std::min<long long>(std::max<long long>((long long)P / 32, 4096), 2LL << 20);
+  // replay records: 1 per 32 POS pixels (a limb band and the facet-edge band
+  // of every view take well under that), at least 4096, at most 64M
+  pd.stale_cap = (int)std::min<long long>(std::max<long long>((long long)P / 32, 4096), 64LL << 20);
   if ((rc = alloc((void **)&pd.stale, sizeof(StaleRec) * (size_t)pd.stale_cap))) return rc;
   if ((rc = alloc((void **)&pd.nstale, sizeof(int) * 4))) return rc;
   pd.error = pd.nstale + 1;
@@ -665,50 +666,50 @@ static inline unsigned cdiv(long long a, long long b) { return (unsigned)((a + b
 int Engine::enqueue() {
   const int T = 256;
   dim3 gv(cdiv(nviews, 64)), bv(64);
-  hipLaunchKernelGGL(k_zero_counters, dim3(1), dim3(64), 0, stream, pd.nstale, 4);
-  hipLaunchKernelGGL(k_setup_views, gv, bv, 0, stream, d_views, nviews, d_params);
+  SHP_LAUNCH(*this, "k_zero_counters", k_zero_counters, dim3(1), dim3(64), 0, stream, pd.nstale, 4);
+  SHP_LAUNCH(*this, "k_setup_views", k_setup_views, gv, bv, 0, stream, d_views, nviews, d_params);
   if (!opt.host_geometry) {
     int max_nv = 0;
     for (auto &G : h_slots) max_nv = std::max(max_nv, G.nv);
-    hipLaunchKernelGGL(k_realize_vertices, dim3(cdiv(max_nv, T), nslots), dim3(T), 0, stream, d_slots, nslots,
+    SHP_LAUNCH(*this, "k_realize_vertices", k_realize_vertices, dim3(cdiv(max_nv, T), nslots), dim3(T), 0, stream, d_slots, nslots,
                        d_params, d_slot_flags, d_slot_logf);
-    hipLaunchKernelGGL(k_realize_facets, dim3(cdiv(max_nf, T), nslots), dim3(T), 0, stream, d_slots, nslots);
-    hipLaunchKernelGGL(k_realize_vnormals, dim3(cdiv(max_nv, T), nslots), dim3(T), 0, stream, d_slots, nslots);
+    SHP_LAUNCH(*this, "k_realize_facets", k_realize_facets, dim3(cdiv(max_nf, T), nslots), dim3(T), 0, stream, d_slots, nslots);
+    SHP_LAUNCH(*this, "k_realize_vnormals", k_realize_vnormals, dim3(cdiv(max_nv, T), nslots), dim3(T), 0, stream, d_slots, nslots);
   }
   if (opt.method == METHOD_MIXED) return enqueue_mixed(*this);
   dim3 gf(cdiv(max_nf, T), nviews), gp(cdiv(max_pix, T), nviews), bt(T);
   // observer view
-  hipLaunchKernelGGL(k_raster64, gf, bt, 0, stream, d_views, d_slots, pd, 0, 1);
-  hipLaunchKernelGGL(k_raster64, gf, bt, 0, stream, d_views, d_slots, pd, 0, 2);
-  hipLaunchKernelGGL(k_resolve64, gp, bt, 0, stream, d_views, d_slots, pd, 0, 0, 1, 0, 0);
-  hipLaunchKernelGGL(k_raster64, gf, bt, 0, stream, d_views, d_slots, pd, 0, 3);
-  hipLaunchKernelGGL(k_replay64, dim3(cdiv(pd.stale_cap, T)), bt, 0, stream, d_views, d_slots, pd);
+  SHP_LAUNCH(*this, "k_raster64/1", k_raster64, gf, bt, 0, stream, d_views, d_slots, pd, 0, 1);
+  SHP_LAUNCH(*this, "k_raster64/2", k_raster64, gf, bt, 0, stream, d_views, d_slots, pd, 0, 2);
+  SHP_LAUNCH(*this, "k_resolve64", k_resolve64, gp, bt, 0, stream, d_views, d_slots, pd, 0, 0, 1, 0, 0);
+  SHP_LAUNCH(*this, "k_raster64/3", k_raster64, gf, bt, 0, stream, d_views, d_slots, pd, 0, 3);
+  SHP_LAUNCH(*this, "k_replay64", k_replay64, dim3(std::min(cdiv(pd.stale_cap, T), 2048u)), bt, 0, stream, d_views, d_slots, pd);
   if (any_lc) {
-    hipLaunchKernelGGL(k_zero_counters, dim3(1), dim3(64), 0, stream, pd.nstale, 1);
-    hipLaunchKernelGGL(k_raster64, gf, bt, 0, stream, d_views, d_slots, pd, 1, 1);
-    hipLaunchKernelGGL(k_raster64, gf, bt, 0, stream, d_views, d_slots, pd, 1, 2);
-    hipLaunchKernelGGL(k_resolve64, gp, bt, 0, stream, d_views, d_slots, pd, 1, 0, 1, 0, 0);
-    hipLaunchKernelGGL(k_posmask64, gp, bt, 0, stream, d_views, pd, mask_tol);
-    hipLaunchKernelGGL(k_photo64, gp, bt, 0, stream, d_views, pd, d_params);
+    SHP_LAUNCH(*this, "k_zero_counters", k_zero_counters, dim3(1), dim3(64), 0, stream, pd.nstale, 1);
+    SHP_LAUNCH(*this, "k_raster64/1/sun", k_raster64, gf, bt, 0, stream, d_views, d_slots, pd, 1, 1);
+    SHP_LAUNCH(*this, "k_raster64/2/sun", k_raster64, gf, bt, 0, stream, d_views, d_slots, pd, 1, 2);
+    SHP_LAUNCH(*this, "k_resolve64", k_resolve64, gp, bt, 0, stream, d_views, d_slots, pd, 1, 0, 1, 0, 0);
+    SHP_LAUNCH(*this, "k_posmask64", k_posmask64, gp, bt, 0, stream, d_views, pd, mask_tol);
+    SHP_LAUNCH(*this, "k_photo64", k_photo64, gp, bt, 0, stream, d_views, pd, d_params);
   }
-  hipLaunchKernelGGL(k_mark_seen, gp, bt, 0, stream, d_views, d_slots, pd, mincosine_seen);
-  if (any_dd) hipLaunchKernelGGL(k_deldop64, gp, bt, 0, stream, d_views, d_frames, pd, d_fit, d_params);
-  if (any_dop) hipLaunchKernelGGL(k_doppler64, gp, bt, 0, stream, d_views, d_frames, pd, d_fit, d_overflow, d_params);
-  hipLaunchKernelGGL(k_finish_views, gv, bv, 0, stream, d_views, nviews, d_overflow, d_frames, d_view_out);
+  SHP_LAUNCH(*this, "k_mark_seen", k_mark_seen, gp, bt, 0, stream, d_views, d_slots, pd, mincosine_seen);
+  if (any_dd) SHP_LAUNCH(*this, "k_deldop64", k_deldop64, gp, bt, 0, stream, d_views, d_frames, pd, d_fit, d_params);
+  if (any_dop) SHP_LAUNCH(*this, "k_doppler64", k_doppler64, gp, bt, 0, stream, d_views, d_frames, pd, d_fit, d_overflow, d_params);
+  SHP_LAUNCH(*this, "k_finish_views", k_finish_views, gv, bv, 0, stream, d_views, nviews, d_overflow, d_frames, d_view_out);
   if (any_lc)
-    hipLaunchKernelGGL(k_spline, dim3(nframes), bt, 0, stream, d_frames, nframes, d_view_out, d_views, d_lc_x,
+    SHP_LAUNCH(*this, "k_spline", k_spline, dim3(nframes), bt, 0, stream, d_frames, nframes, d_view_out, d_views, d_lc_x,
                        d_lc_t, d_lc_y, d_lc_y2, d_fit);
-  hipLaunchKernelGGL(k_chi2_frames64, dim3(nframes), bt, 0, stream, d_frames, nframes, d_fit, d_obs, d_oov,
+  SHP_LAUNCH(*this, "k_chi2_frames64", k_chi2_frames64, dim3(nframes), bt, 0, stream, d_frames, nframes, d_fit, d_obs, d_oov,
                      d_view_out, opt.keep_fit ? 0 : 1, 0);
-  hipLaunchKernelGGL(k_objective, dim3(cdiv(nevals, 64)), dim3(64), 0, stream, d_evals, nevals, d_frames, d_views,
+  SHP_LAUNCH(*this, "k_objective", k_objective, dim3(cdiv(nevals, 64)), dim3(64), 0, stream, d_evals, nevals, d_frames, d_views,
                      d_view_out, d_slot_flags, d_slot_logf, d_eval_slot);
   SHP_CHECK(hipGetLastError());
   return 0;
 }
 
 int Engine::capture_graph() {
-  if (graph_exec) { hipGraphExecDestroy(graph_exec); graph_exec = nullptr; }
-  if (graph) { hipGraphDestroy(graph); graph = nullptr; }
+  if (graph_exec) { (void)hipGraphExecDestroy(graph_exec); graph_exec = nullptr; }
+  if (graph) { (void)hipGraphDestroy(graph); graph = nullptr; }
   SHP_CHECK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
   int rc = upload_params();
   if (!rc) rc = enqueue();
@@ -726,6 +727,44 @@ int Engine::launch_graph() {
 
 int Engine::sync() {
   SHP_CHECK(hipStreamSynchronize(stream));
+  return 0;
+}
+
+void Engine::tbegin(const char *name) {
+  if (!time_kernels) return;
+  if ((size_t)(2 * t_used + 2) > t_events.size()) {
+    hipEvent_t a = nullptr, b = nullptr;
+    if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) return;
+    t_events.push_back(a);
+    t_events.push_back(b);
+  }
+  if ((int)t_names.size() <= t_used) t_names.resize(t_used + 1);
+  t_names[t_used] = name;
+  (void)hipEventRecord(t_events[2 * t_used], stream);
+}
+
+void Engine::tend() {
+  if (!time_kernels || (size_t)(2 * t_used + 2) > t_events.size()) return;
+  (void)hipEventRecord(t_events[2 * t_used + 1], stream);
+  t_used++;
+}
+
+int Engine::collect_kernel_times() {
+  SHP_CHECK(hipStreamSynchronize(stream));
+  for (int q = 0; q < t_used; q++) {
+    float ms = 0.0f;
+    SHP_CHECK(hipEventElapsedTime(&ms, t_events[2 * q], t_events[2 * q + 1]));
+    size_t k = 0;
+    while (k < kt_names.size() && kt_names[k] != t_names[q]) k++;
+    if (k == kt_names.size()) {
+      kt_names.push_back(t_names[q]);
+      kt_ms.push_back(0.0);
+      kt_count.push_back(0);
+    }
+    kt_ms[k] += ms;
+    kt_count[k]++;
+  }
+  t_used = 0;
   return 0;
 }
 

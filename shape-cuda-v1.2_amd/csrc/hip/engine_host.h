@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <map>
+#include <string>
 #include <vector>
 
 #include "engine.h"
@@ -92,10 +93,30 @@ class Engine {
   std::vector<void *> allocs;
   hipGraph_t graph = nullptr;
   hipGraphExec_t graph_exec = nullptr;
+  // Optional timing of eager launches: HIP events around every kernel on
+  // `stream`; collect_kernel_times() folds the last launches into the totals.
+  bool time_kernels = false;
+  void tbegin(const char *name);
+  void tend();
+  int collect_kernel_times();
+  std::vector<std::string> kt_names;
+  std::vector<double> kt_ms;
+  std::vector<long long> kt_count;
 
  private:
+  std::vector<hipEvent_t> t_events;
+  std::vector<const char *> t_names;
+  int t_used = 0;
   int alloc(void **p, size_t bytes);
   int pack_model(int m, struct par_t *par, struct mod_t *mod, struct dat_t *dat);
 };
+
+// launch a kernel, bracketed by timing events when the engine times kernels
+#define SHP_LAUNCH(E, name, ...)  \
+  do {                            \
+    (E).tbegin(name);             \
+    hipLaunchKernelGGL(__VA_ARGS__); \
+    (E).tend();                   \
+  } while (0)
 
 }  // namespace shp

@@ -1,0 +1,105 @@
+// fp64_dev.h -- FP64 projection, coverage test and shading of one facet at one
+// pixel, in the reference's operation order (posvis.c:142-275, src/util
+// cotrans / normalize).  Every function body disables FP contraction itself,
+// so translation units built with contraction on (the mixed kernels) still
+// reproduce the CPU's z, facet ownership and cos(scattering) bit for bit.
+#pragma once
+
+#include <climits>
+
+#include "common.h"
+#include "engine.h"
+
+namespace shp {
+
+__device__ inline void cotrans3(double y[3], const double a[9], const double x[3]) {
+#pragma clang fp contract(off)
+  double t0 = 0.0, t1 = 0.0, t2 = 0.0;
+  t0 += a[0] * x[0]; t0 += a[1] * x[1]; t0 += a[2] * x[2];
+  t1 += a[3] * x[0]; t1 += a[4] * x[1]; t1 += a[5] * x[2];
+  t2 += a[6] * x[0]; t2 += a[7] * x[1]; t2 += a[8] * x[2];
+  y[0] = t0; y[1] = t1; y[2] = t2;
+}
+__device__ inline double normalize3(double u[3]) {
+#pragma clang fp contract(off)
+  double nn = 0.0;
+  nn += u[0] * u[0]; nn += u[1] * u[1]; nn += u[2] * u[2];
+  nn = sqrt(nn);
+  if (nn != 0.0) { u[0] /= nn; u[1] /= nn; u[2] /= nn; }
+  return nn;
+}
+__device__ inline double dmin(double a, double b) { return a < b ? a : b; }
+__device__ inline double dmax(double a, double b) { return a > b ? a : b; }
+
+struct FacetProj {
+  double v0[3], v1[3], v2[3], nrm[3], den;
+  int i1, i2, j1, j2;
+  double imin_d, imax_d, jmin_d, jmax_d;
+  bool front, inside, outbnd;
+};
+
+__device__ inline void project_facet(const ViewRec &V, const GeomSlot &G, int f, int src, FacetProj &P) {
+#pragma clang fp contract(off)
+  const double *oa = src ? V.sa : V.oa;
+  double nb[3] = {G.fn[3 * f], G.fn[3 * f + 1], G.fn[3 * f + 2]};
+  cotrans3(P.nrm, oa, nb);
+  P.front = P.nrm[2] > 0.0;
+  if (!P.front) return;
+  const int *fv = G.fv + 3 * f;
+  cotrans3(P.v0, oa, G.x + 3 * fv[0]);
+  cotrans3(P.v1, oa, G.x + 3 * fv[1]);
+  cotrans3(P.v2, oa, G.x + 3 * fv[2]);
+  // orbit offset is zero for single bodies: v += 0.0 keeps values unchanged
+  double kmpp = V.kmpp;
+  int n = V.n;
+  P.imin_d = floor(dmin(P.v0[0], dmin(P.v1[0], P.v2[0])) / kmpp - 1.0e-10 + 0.5);
+  P.imax_d = floor(dmax(P.v0[0], dmax(P.v1[0], P.v2[0])) / kmpp + 1.0e-10 + 0.5);
+  P.jmin_d = floor(dmin(P.v0[1], dmin(P.v1[1], P.v2[1])) / kmpp - 1.0e-10 + 0.5);
+  P.jmax_d = floor(dmax(P.v0[1], dmax(P.v1[1], P.v2[1])) / kmpp + 1.0e-10 + 0.5);
+  int imin = P.imin_d < -2147483648.0 ? INT_MIN : (int)P.imin_d;
+  int imax = P.imax_d > 2147483647.0 ? INT_MAX : (int)P.imax_d;
+  int jmin = P.jmin_d < -2147483648.0 ? INT_MIN : (int)P.jmin_d;
+  int jmax = P.jmax_d > 2147483647.0 ? INT_MAX : (int)P.jmax_d;
+  P.outbnd = (imin < -n || imax > n || jmin < -n || jmax > n);
+  P.i1 = max(imin, -n); P.i2 = min(imax, n);
+  P.j1 = max(jmin, -n); P.j2 = min(jmax, n);
+  P.inside = !(P.i1 > n || P.i2 < -n || P.j1 > n || P.j2 < -n);
+  P.den = 1 / ((P.v1[0] - P.v0[0]) * (P.v2[1] - P.v1[1]) - (P.v2[0] - P.v1[0]) * (P.v1[1] - P.v0[1]));
+}
+
+// barycentric test and depth at pixel (i, j); same expressions as posvis.c:217-228
+__device__ inline bool pixel_hit(const FacetProj &P, double kmpp, int i, int j, double &s, double &t, double &z) {
+#pragma clang fp contract(off)
+  double x0 = i * kmpp, x1 = j * kmpp;
+  s = ((x0 - P.v0[0]) * (P.v2[1] - P.v1[1]) - (P.v2[0] - P.v1[0]) * (x1 - P.v0[1])) * P.den;
+  if (!((s >= -1.0e-10) && (s <= 1.0 + 1.0e-10))) return false;
+  t = ((P.v1[0] - P.v0[0]) * (x1 - P.v0[1]) - (x0 - P.v0[0]) * (P.v1[1] - P.v0[1])) * P.den;
+  if (!((t >= -1.0e-10) && (t <= s + 1.0e-10))) return false;
+  z = P.v0[2] + s * (P.v1[2] - P.v0[2]) + t * (P.v2[2] - P.v1[2]);
+  return true;
+}
+
+// shading of the winning facet at one pixel (posvis.c:243-275)
+__device__ inline void shade64(const ViewRec &V, const GeomSlot &G, int f, int src, int smooth, double s, double t,
+                               double &nz, double &cosb) {
+#pragma clang fp contract(off)
+  double nrm[3];
+  const double *oa = src ? V.sa : V.oa;
+  if (smooth) {
+    const int *fv = G.fv + 3 * f;
+    const double *a = G.vn + 3 * fv[0], *b = G.vn + 3 * fv[1], *c = G.vn + 3 * fv[2];
+    for (int k = 0; k < 3; k++) nrm[k] = a[k] + s * (b[k] - a[k]) + t * (c[k] - b[k]);
+    double tmp[3];
+    cotrans3(tmp, oa, nrm);
+    nrm[0] = tmp[0]; nrm[1] = tmp[1]; nrm[2] = tmp[2];
+    normalize3(nrm);
+  } else {
+    double nb[3] = {G.fn[3 * f], G.fn[3 * f + 1], G.fn[3 * f + 2]};
+    cotrans3(nrm, oa, nb);
+  }
+  nz = nrm[2];
+  cosb = 0.0;
+  if (!src && V.bistatic) cosb = nrm[0] * V.usrc[0] + nrm[1] * V.usrc[1] + nrm[2] * V.usrc[2];
+}
+
+}  // namespace shp

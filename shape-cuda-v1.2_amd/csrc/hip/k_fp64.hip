@@ -9,19 +9,13 @@
 #include "common.h"
 #include "engine.h"
 #include "kernels.h"
+#include "fp64_dev.h"
 
 namespace shp {
 
 // ---------------------------------------------------------------------------
 // small FP64 helpers (same operation order as src/util/*.c)
 // ---------------------------------------------------------------------------
-__device__ inline void cotrans3(double y[3], const double a[9], const double x[3]) {
-  double t0 = 0.0, t1 = 0.0, t2 = 0.0;
-  t0 += a[0] * x[0]; t0 += a[1] * x[1]; t0 += a[2] * x[2];
-  t1 += a[3] * x[0]; t1 += a[4] * x[1]; t1 += a[5] * x[2];
-  t2 += a[6] * x[0]; t2 += a[7] * x[1]; t2 += a[8] * x[2];
-  y[0] = t0; y[1] = t1; y[2] = t2;
-}
 __device__ inline void cotrans3T(double y[3], const double a[9], const double x[3]) {
   double t0 = 0.0, t1 = 0.0, t2 = 0.0;
   t0 += a[0] * x[0]; t0 += a[3] * x[1]; t0 += a[6] * x[2];
@@ -37,13 +31,6 @@ __device__ inline void mul_bat(double oa[9], const double b[9], const double a[9
       for (int k = 0; k < 3; k++) t += b[3 * i + k] * a[3 * j + k];
       oa[3 * i + j] = t;
     }
-}
-__device__ inline double normalize3(double u[3]) {
-  double nn = 0.0;
-  nn += u[0] * u[0]; nn += u[1] * u[1]; nn += u[2] * u[2];
-  nn = sqrt(nn);
-  if (nn != 0.0) { u[0] /= nn; u[1] /= nn; u[2] /= nn; }
-  return nn;
 }
 __device__ inline void euler2mat9(double m[9], double phi, double theta, double psi) {
   double ct = cos(theta), st = sin(theta), cp = cos(phi), sp = sin(phi), cs = cos(psi), ss = sin(psi);
@@ -65,8 +52,6 @@ __device__ inline double sinc2d(double x) {
 __device__ inline double trid(double x) { double t = 1 - fabs(x); return t > 0.0 ? t : 0.0; }
 __device__ inline double tri2d(double x) { double t = 1 - fabs(x); return t > 0.0 ? t * t : 0.0; }
 
-__device__ inline double dmin(double a, double b) { return a < b ? a : b; }
-__device__ inline double dmax(double a, double b) { return a > b ? a : b; }
 
 // wavefront reductions (wave64)
 __device__ inline int wave_min_i(int v) { for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o, 64)); return v; }
@@ -197,7 +182,7 @@ __global__ void k_setup_views(ViewRec *views, int nviews, const double *params) 
   // reset accumulators
   V.xlim[0] = V.ylim[0] = V.xlim2[0] = V.ylim2[0] = V.n;
   V.xlim[1] = V.ylim[1] = V.xlim2[1] = V.ylim2[1] = -V.n;
-  V.outbnd = V.outbnd2 = V.any_overflow = V.nstale = 0;
+  V.outbnd = V.outbnd2 = V.any_overflow = V.nstale = V.nclaimed = 0;
   for (int q = 0; q < 4; q++) V.ov[q] = V.ov2[q] = ord64(1.0e20);
   V.idellim[0] = V.ndel + 999999; V.idellim[1] = -999999;
   V.idoplim[0] = V.ndop + 999999; V.idoplim[1] = -999999;
@@ -288,52 +273,6 @@ __global__ void k_realize_vnormals(GeomSlot *slots, int nslot) {
 //                   smoothed normal (their cos(scattering) is inherited from
 //                   an earlier claimant; replayed in k_replay64)
 // ---------------------------------------------------------------------------
-struct FacetProj {
-  double v0[3], v1[3], v2[3], nrm[3], den;
-  int i1, i2, j1, j2;
-  double imin_d, imax_d, jmin_d, jmax_d;
-  bool front, inside, outbnd;
-};
-
-__device__ inline void project_facet(const ViewRec &V, const GeomSlot &G, int f, int src, FacetProj &P) {
-  const double *oa = src ? V.sa : V.oa;
-  double nb[3] = {G.fn[3 * f], G.fn[3 * f + 1], G.fn[3 * f + 2]};
-  cotrans3(P.nrm, oa, nb);
-  P.front = P.nrm[2] > 0.0;
-  if (!P.front) return;
-  const int *fv = G.fv + 3 * f;
-  cotrans3(P.v0, oa, G.x + 3 * fv[0]);
-  cotrans3(P.v1, oa, G.x + 3 * fv[1]);
-  cotrans3(P.v2, oa, G.x + 3 * fv[2]);
-  // orbit offset is zero for single bodies: v += 0.0 keeps values unchanged
-  double kmpp = V.kmpp;
-  int n = V.n;
-  P.imin_d = floor(dmin(P.v0[0], dmin(P.v1[0], P.v2[0])) / kmpp - 1.0e-10 + 0.5);
-  P.imax_d = floor(dmax(P.v0[0], dmax(P.v1[0], P.v2[0])) / kmpp + 1.0e-10 + 0.5);
-  P.jmin_d = floor(dmin(P.v0[1], dmin(P.v1[1], P.v2[1])) / kmpp - 1.0e-10 + 0.5);
-  P.jmax_d = floor(dmax(P.v0[1], dmax(P.v1[1], P.v2[1])) / kmpp + 1.0e-10 + 0.5);
-  int imin = P.imin_d < -2147483648.0 ? INT_MIN : (int)P.imin_d;
-  int imax = P.imax_d > 2147483647.0 ? INT_MAX : (int)P.imax_d;
-  int jmin = P.jmin_d < -2147483648.0 ? INT_MIN : (int)P.jmin_d;
-  int jmax = P.jmax_d > 2147483647.0 ? INT_MAX : (int)P.jmax_d;
-  P.outbnd = (imin < -n || imax > n || jmin < -n || jmax > n);
-  P.i1 = max(imin, -n); P.i2 = min(imax, n);
-  P.j1 = max(jmin, -n); P.j2 = min(jmax, n);
-  P.inside = !(P.i1 > n || P.i2 < -n || P.j1 > n || P.j2 < -n);
-  P.den = 1 / ((P.v1[0] - P.v0[0]) * (P.v2[1] - P.v1[1]) - (P.v2[0] - P.v1[0]) * (P.v1[1] - P.v0[1]));
-}
-
-// barycentric test and depth at pixel (i, j); same expressions as posvis.c:217-228
-__device__ inline bool pixel_hit(const FacetProj &P, double kmpp, int i, int j, double &s, double &t, double &z) {
-  double x0 = i * kmpp, x1 = j * kmpp;
-  s = ((x0 - P.v0[0]) * (P.v2[1] - P.v1[1]) - (P.v2[0] - P.v1[0]) * (x1 - P.v0[1])) * P.den;
-  if (!((s >= -1.0e-10) && (s <= 1.0 + 1.0e-10))) return false;
-  t = ((P.v1[0] - P.v0[0]) * (x1 - P.v0[1]) - (x0 - P.v0[0]) * (P.v1[1] - P.v0[1])) * P.den;
-  if (!((t >= -1.0e-10) && (t <= s + 1.0e-10))) return false;
-  z = P.v0[2] + s * (P.v1[2] - P.v0[2]) + t * (P.v2[2] - P.v1[2]);
-  return true;
-}
-
 __global__ void k_raster64(ViewRec *views, const GeomSlot *slots, PosDev pd, int src, int pass) {
   ViewRec &V = views[blockIdx.y];
   const GeomSlot &G = slots[V.slot];
@@ -401,28 +340,6 @@ __global__ void k_raster64(ViewRec *views, const GeomSlot *slots, PosDev pd, int
     if (o2 < 1.0e20) atomic_min_d(&ov[2], o2);
     if (o3 > -1.0e20) atomic_min_d(&ov[3], -o3);
   }
-}
-
-// shading of the winning facet at one pixel (posvis.c:243-275)
-__device__ inline void shade64(const ViewRec &V, const GeomSlot &G, int f, int src, int smooth, double s, double t,
-                               double &nz, double &cosb) {
-  double nrm[3];
-  const double *oa = src ? V.sa : V.oa;
-  if (smooth) {
-    const int *fv = G.fv + 3 * f;
-    const double *a = G.vn + 3 * fv[0], *b = G.vn + 3 * fv[1], *c = G.vn + 3 * fv[2];
-    for (int k = 0; k < 3; k++) nrm[k] = a[k] + s * (b[k] - a[k]) + t * (c[k] - b[k]);
-    double tmp[3];
-    cotrans3(tmp, oa, nrm);
-    nrm[0] = tmp[0]; nrm[1] = tmp[1]; nrm[2] = tmp[2];
-    normalize3(nrm);
-  } else {
-    double nb[3] = {G.fn[3 * f], G.fn[3 * f + 1], G.fn[3 * f + 2]};
-    cotrans3(nrm, oa, nb);
-  }
-  nz = nrm[2];
-  cosb = 0.0;
-  if (!src && V.bistatic) cosb = nrm[0] * V.usrc[0] + nrm[1] * V.usrc[1] + nrm[2] * V.usrc[2];
 }
 
 // Resolve the z-buffer into POS arrays.  merge: pixels already owned before
@@ -497,10 +414,7 @@ __global__ void k_resolve64(ViewRec *views, const GeomSlot *slots, PosDev pd, in
 
 // Replay, in facet order, the claims made on one pixel whose final owner has a
 // back-facing smoothed normal, reproducing the CPU's inherited values.
-__global__ void k_replay64(ViewRec *views, const GeomSlot *slots, PosDev pd) {
-  int si = blockIdx.x * blockDim.x + threadIdx.x;
-  int ns = min(*pd.nstale, pd.stale_cap);
-  if (si >= ns) return;
+__device__ void replay64_one(ViewRec *views, const GeomSlot *slots, const PosDev &pd, int si) {
   StaleRec &S = pd.stale[si];
   const ViewRec &V = views[S.view];
   const GeomSlot &G = slots[V.slot];
@@ -537,6 +451,13 @@ __global__ void k_replay64(ViewRec *views, const GeomSlot *slots, PosDev pd) {
   (src ? pd.cosill : pd.cose)[p] = cosa;
   if (!src && V.bistatic) pd.cosi[p] = cosb;
   pd.stale_idx[p] = -1;
+}
+
+// grid-stride over the records (the grid is sized for the cap, not the count)
+__global__ void k_replay64(ViewRec *views, const GeomSlot *slots, PosDev pd) {
+  int ns = min(*pd.nstale, pd.stale_cap);
+  for (int si = blockIdx.x * blockDim.x + threadIdx.x; si < ns; si += gridDim.x * blockDim.x)
+    replay64_one(views, slots, pd, si);
 }
 
 // ---------------------------------------------------------------------------

@@ -11,11 +11,13 @@
 //   * resolve+synthesis (k_radar32.hip): one workgroup per 16x16-pixel tile
 //     reads and clears the keys, shades the owner, and accumulates the tile's
 //     delay-Doppler footprint in an LDS window before one flush to HBM;
-//   * pixels whose owner's smoothed normal faces away are replayed from the
-//     facets covering them (a raster pass restricted to 8x8 tiles that hold
-//     such pixels), reproducing the CPU's inherited cos(scattering);
-//   * chi-square: FP64 compensated partial sums over each frame's occupied
-//     window, reading and clearing the FP32 fit.
+//   * pixels whose owner's smoothed normal faces away, and pixels within the
+//     FP32 rounding band of a facet edge, are replayed in FP64 from the facets
+//     covering them (a raster pass restricted to 8x8 tiles that hold such
+//     pixels), reproducing the CPU's owner and inherited cos(scattering);
+//   * per-pixel contributions are FP32, every accumulation FP64: the fit
+//     image, and chi-square as compensated partial sums over each frame's
+//     occupied window, reading and clearing the fit.
 // Delay and Doppler bin coordinates are formed relative to an integer anchor
 // (ViewRec::D0, P0) so FP32 rounding acts on |x| < ~1e3 bins, not on the
 // absolute bin number.
@@ -30,7 +32,8 @@ __global__ void k_replay32(ViewRec *views, const GeomSlot *slots, PosDev pd, dou
                            double *overflow, const double *params);
 
 // ---------------------------------------------------------------------------
-// raster (posvis.c:142-295).  pass 1: packed-key z-buffer and the POS
+// raster (posvis.c:142-295).  pass 1: packed-key z-buffer over pixels well
+// inside a facet, marginal pixels recorded for the FP64 replay, and the POS
 // rectangles; pass 3: record the facets covering replayed pixels, visiting
 // only facets whose rectangle touches a tile flagged in the stale bitmap
 // ---------------------------------------------------------------------------
@@ -46,7 +49,8 @@ __global__ __launch_bounds__(256) void k_raster32(ViewRec *views, const GeomSlot
   int xl0 = n, xl1 = -n, yl0 = n, yl1 = -n, ob = 0;
   float o0 = 1e30f, o1 = -1e30f, o2 = 1e30f, o3 = -1e30f;
   Proj32 P;
-  if (f < G.nf && project32(V, G, f, src ? V.sa32 : V.oa32, P)) {
+  // pass 3 also admits nearly edge-on facets: the FP64 replay tests facing itself
+  if (f < G.nf && project32(V, G, f, src ? V.sa32 : V.oa32, pass == 3 ? -FRONT_TOL : 0.0f, P)) {
     float imin_d = floorf(fminf(P.x0, fminf(P.x1, P.x2)) + 0.5f);
     float imax_d = floorf(fmaxf(P.x0, fmaxf(P.x1, P.x2)) + 0.5f);
     float jmin_d = floorf(fminf(P.y0, fminf(P.y1, P.y2)) + 0.5f);
@@ -63,9 +67,15 @@ __global__ __launch_bounds__(256) void k_raster32(ViewRec *views, const GeomSlot
     } else if (pass == 1) {
       for (int i = i1; i <= i2; i++)
         for (int j = j1; j <= j2; j++) {
-          float s, t, z;
-          if (!hit32(P, i, j, s, t, z)) continue;
-          atomicMax(&kb[(long long)(i + n) * w + (j + n)], (unsigned long long)pack_key(z, f));
+          float s, t;
+          float m = bary32(P, i, j, s, t);
+          if (!(m >= -AMB_TOL)) continue;
+          long long pix = (long long)(i + n) * w + (j + n);
+          if (m >= AMB_TOL || src) {
+            atomicMax(&kb[pix], (unsigned long long)pack_key(depth32(P, s, t), f));
+          } else {
+            mark_marginal(V, pd, (int)pix, i, j);
+          }
           xl0 = min(xl0, i); xl1 = max(xl1, i); yl0 = min(yl0, j); yl1 = max(yl1, j);
           o0 = fminf(o0, (float)i); o1 = fmaxf(o1, (float)i);
           o2 = fminf(o2, (float)j); o3 = fmaxf(o3, (float)j);
@@ -80,8 +90,9 @@ __global__ __launch_bounds__(256) void k_raster32(ViewRec *views, const GeomSlot
       if (any)
         for (int i = i1; i <= i2; i++)
           for (int j = j1; j <= j2; j++) {
-            float s, t, z;
-            if (!hit32(P, i, j, s, t, z)) continue;
+            // candidates: a superset of the facets covering the pixel in FP64
+            float s, t;
+            if (!(bary32(P, i, j, s, t) >= -AMB_TOL)) continue;
             int si = pd.stale_idx[V.pos_off + (long long)(i + n) * w + (j + n)];
             if (si >= 0) {
               int q = atomicAdd(&pd.stale[si].ncand, 1);
@@ -93,23 +104,27 @@ __global__ __launch_bounds__(256) void k_raster32(ViewRec *views, const GeomSlot
   if (pass != 1) return;
   xl0 = wmin(xl0); xl1 = wmax(xl1); yl0 = wmin(yl0); yl1 = wmax(yl1); ob = wmax(ob);
   o0 = wminf(o0); o1 = wmaxf(o1); o2 = wminf(o2); o3 = wmaxf(o3);
+  // one update per wave, skipped when a read of the current value shows it
+  // cannot move the limits (they only move outward)
   if ((threadIdx.x & 63) == 0) {
     int *xl = src ? V.xlim2 : V.xlim, *yl = src ? V.ylim2 : V.ylim;
     unsigned long long *ov = src ? V.ov2 : V.ov;
-    if (xl0 < n) atomicMin(&xl[0], xl0);
-    if (xl1 > -n) atomicMax(&xl[1], xl1);
-    if (yl0 < n) atomicMin(&yl[0], yl0);
-    if (yl1 > -n) atomicMax(&yl[1], yl1);
-    if (ob) atomicOr(src ? &V.outbnd2 : &V.outbnd, 1);
-    if (o0 < 1e30f) atomic_min_d(&ov[0], (double)o0);
-    if (o1 > -1e30f) atomic_min_d(&ov[1], -(double)o1);
-    if (o2 < 1e30f) atomic_min_d(&ov[2], (double)o2);
-    if (o3 > -1e30f) atomic_min_d(&ov[3], -(double)o3);
+    int *obp = src ? &V.outbnd2 : &V.outbnd;
+    if (xl0 < n && xl0 < xl[0]) atomicMin(&xl[0], xl0);
+    if (xl1 > -n && xl1 > xl[1]) atomicMax(&xl[1], xl1);
+    if (yl0 < n && yl0 < yl[0]) atomicMin(&yl[0], yl0);
+    if (yl1 > -n && yl1 > yl[1]) atomicMax(&yl[1], yl1);
+    if (ob && !*obp) atomicOr(obp, 1);
+    unsigned long long k;
+    if (o0 < 1e30f && (k = ord64((double)o0)) < ov[0]) atomicMin(&ov[0], k);
+    if (o1 > -1e30f && (k = ord64(-(double)o1)) < ov[1]) atomicMin(&ov[1], k);
+    if (o2 < 1e30f && (k = ord64((double)o2)) < ov[2]) atomicMin(&ov[2], k);
+    if (o3 > -1e30f && (k = ord64(-(double)o3)) < ov[3]) atomicMin(&ov[3], k);
   }
 }
 
 // ---------------------------------------------------------------------------
-// chi-square of radar frames from the FP32 fit: FP64 compensated partials over
+// chi-square of radar frames from the fit image: FP64 compensated partials over
 // the frame's occupied window (chi2.c:418-461, :752-785); unless keep is set
 // the fit is cleared as it is read, ready for the next evaluation
 // ---------------------------------------------------------------------------
@@ -194,18 +209,18 @@ int enqueue_mixed(Engine &E) {
   int wmax_ = 0;
   for (auto &V : E.h_views) wmax_ = std::max(wmax_, 2 * V.n + 1);
   int tiles = cdivm(wmax_, TILE);
-  hipLaunchKernelGGL(k_fill_u8, dim3(256), bt, 0, st, E.pd.stale_tiles, std::max(1LL, E.total_tiles));
-  hipLaunchKernelGGL(k_raster32, gf, bt, 0, st, E.d_views, E.d_slots, E.pd, E.total_pix, 0, 1);
-  hipLaunchKernelGGL(k_shade_radar32, dim3(tiles * tiles, E.nviews), bt, 0, st, E.d_views, E.d_slots, E.pd,
+  SHP_LAUNCH(E, "k_fill_u8", k_fill_u8, dim3(256), bt, 0, st, E.pd.stale_tiles, std::max(1LL, E.total_tiles));
+  SHP_LAUNCH(E, "k_raster32/1", k_raster32, gf, bt, 0, st, E.d_views, E.d_slots, E.pd, E.total_pix, 0, 1);
+  SHP_LAUNCH(E, "k_shade_radar32", k_shade_radar32, dim3(tiles * tiles, E.nviews), bt, 0, st, E.d_views, E.d_slots, E.pd,
                      E.d_fit, E.d_frames, E.d_overflow, E.d_params, E.pd.nstale);
-  hipLaunchKernelGGL(k_raster32, gf, bt, 0, st, E.d_views, E.d_slots, E.pd, E.total_pix, 0, 3);
-  hipLaunchKernelGGL(k_replay32, dim3(cdivm(E.pd.stale_cap, 256)), bt, 0, st, E.d_views, E.d_slots, E.pd,
+  SHP_LAUNCH(E, "k_raster32/3", k_raster32, gf, bt, 0, st, E.d_views, E.d_slots, E.pd, E.total_pix, 0, 3);
+  SHP_LAUNCH(E, "k_replay32", k_replay32, dim3(std::min(cdivm(E.pd.stale_cap, 256), 2048u)), bt, 0, st, E.d_views, E.d_slots, E.pd,
                      E.d_fit, E.d_frames, E.d_overflow, E.d_params);
-  hipLaunchKernelGGL(k_finish_views, dim3(cdivm(E.nviews, 64)), dim3(64), 0, st, E.d_views, E.nviews, E.d_overflow,
+  SHP_LAUNCH(E, "k_finish_views", k_finish_views, dim3(cdivm(E.nviews, 64)), dim3(64), 0, st, E.d_views, E.nviews, E.d_overflow,
                      E.d_frames, E.d_view_out);
-  hipLaunchKernelGGL(k_chi2_mixed, dim3(E.nframes), bt, 0, st, E.d_frames, E.nframes, E.d_views, E.d_fit,
+  SHP_LAUNCH(E, "k_chi2_mixed", k_chi2_mixed, dim3(E.nframes), bt, 0, st, E.d_frames, E.nframes, E.d_views, E.d_fit,
                      E.d_obs32, E.d_oov32, E.d_view_out, E.opt.keep_fit);
-  hipLaunchKernelGGL(k_objective, dim3(cdivm(E.nevals, 64)), dim3(64), 0, st, E.d_evals, E.nevals, E.d_frames,
+  SHP_LAUNCH(E, "k_objective", k_objective, dim3(cdivm(E.nevals, 64)), dim3(64), 0, st, E.d_evals, E.nevals, E.d_frames,
                      E.d_views, E.d_view_out, E.d_slot_flags, E.d_slot_logf, E.d_eval_slot);
   return hipGetLastError() == hipSuccess ? 0 : -42;
 }

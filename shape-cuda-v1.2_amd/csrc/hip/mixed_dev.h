@@ -39,27 +39,59 @@ __device__ inline void corners32(const ViewRec &V, const GeomSlot &G, int f, con
   P.den = 1.0f / ((P.x1 - P.x0) * (P.y2 - P.y1) - (P.x2 - P.x1) * (P.y1 - P.y0));
 }
 
-__device__ inline bool project32(const ViewRec &V, const GeomSlot &G, int f, const float *m, Proj32 &P) {
+constexpr float FRONT_TOL = 1.0e-4f;
+__device__ inline bool project32(const ViewRec &V, const GeomSlot &G, int f, const float *m, float nz_min, Proj32 &P) {
   float4 fn = G.fn32[f];
   float nz = m[6] * fn.x + m[7] * fn.y + m[8] * fn.z;
-  if (!(nz > 0.0f)) return false;  // back-facing (posvis.c:152)
+  if (!(nz > nz_min)) return false;  // back-facing (posvis.c:152)
   corners32(V, G, f, m, P);
   return true;
 }
 
-// barycentric coverage test and depth (posvis.c:217-228).  The CPU admits
-// pixel centers within 1e-10 of an edge so neighbouring facets both claim a
-// shared edge; FP32 rounds s and t at ~1e-7, so the band is widened to 2e-6:
-// otherwise a center on a shared edge could be rejected by both facets.
-constexpr float EDGE_TOL = 2.0e-6f;
-__device__ inline bool hit32(const Proj32 &P, int i, int j, float &s, float &t, float &z) {
+// Barycentric coordinates of pixel (i, j) (posvis.c:217-228): the facet
+// covers the pixel center when t >= 0, t <= s and s <= 1, i.e. when
+// margin = min(t, s - t, 1 - s) >= 0.
+//
+// FP32 vertex positions carry ~1e-4 px of rounding (|x| up to 512 px), so a
+// center within that distance of an edge may be covered by a different facet
+// than in FP64, or by none.  Such a center is "marginal": no facet claims it
+// in the FP32 z-buffer; it is recorded and resolved in FP64, in facet order,
+// exactly as the CPU does.  AMB_TOL bounds the band in barycentric units:
+// >= 5x the FP32 error for facets of >= 2 px.
+constexpr float AMB_TOL = 2.0e-4f;
+__device__ inline float bary32(const Proj32 &P, int i, int j, float &s, float &t) {
   float xi = (float)i, yj = (float)j;
   s = ((xi - P.x0) * (P.y2 - P.y1) - (P.x2 - P.x1) * (yj - P.y0)) * P.den;
-  if (!(s >= -EDGE_TOL && s <= 1.0f + EDGE_TOL)) return false;
   t = ((P.x1 - P.x0) * (yj - P.y0) - (xi - P.x0) * (P.y1 - P.y0)) * P.den;
-  if (!(t >= -EDGE_TOL && t <= s + EDGE_TOL)) return false;
-  z = P.z0 + s * (P.z1 - P.z0) + t * (P.z2 - P.z1);
-  return true;
+  return fminf(t, fminf(s - t, 1.0f - s));
+}
+__device__ inline float depth32(const Proj32 &P, float s, float t) {
+  return P.z0 + s * (P.z1 - P.z0) + t * (P.z2 - P.z1);
+}
+
+// tile flag of pixel (i, j) in the view's 8x8-tile bitmap
+__device__ inline long long stale_tile(const ViewRec &V, int i, int j) {
+  int tw = (2 * V.n + STALE_TILE) / STALE_TILE;
+  return V.tile_off + (long long)((i + V.n) / STALE_TILE) * tw + (j + V.n) / STALE_TILE;
+}
+
+// Record a marginal pixel for the FP64 replay, once: the first facet to find
+// it swaps its stale index from -1 and appends a record that starts from
+// blank sky (every claimant of the pixel is then replayed in facet order).
+__device__ inline void mark_marginal(const ViewRec &V, const PosDev &pd, int pix, int i, int j) {
+  long long p = V.pos_off + pix;
+  if (atomicCAS(&pd.stale_idx[p], -1, -2) != -1) return;
+  int si = atomicAdd(pd.nstale, 1);
+  if (si >= pd.stale_cap) {
+    pd.stale_idx[p] = -1;
+    atomicOr(pd.error, 1);
+    return;
+  }
+  StaleRec &S = pd.stale[si];
+  S.view = blockIdx.y; S.pix = pix; S.oldf = -1; S.oldz = -1.0e20; S.oldcose = 0.0;
+  S.oldcosi = 0.0; S.ncand = 0; S.src = 0;
+  pd.stale_idx[p] = si;
+  pd.stale_tiles[stale_tile(V, i, j)] = 1;
 }
 
 // normal at (s, t) of facet f in observer coordinates (posvis.c:243-258)
@@ -150,32 +182,35 @@ struct RadarPix {
   float dw[DW], pw[PW];
 };
 
-__device__ inline void radar_pixel(const ViewRec &V, const double *pl, int i, int j, float z, float cose, RadarPix &R) {
+__device__ inline void radar_pixel(const ViewRec &V, const double *pl, int i, int j, double z, float cose, RadarPix &R) {
   const bool dd = V.type == VT_DELAY;
   int width = V.sinc2width, ns = V.nsinc2;
   R.wide = false;
   // Bin coordinates and the integer footprint decisions in FP64, as the CPU
   // forms them (pos2deldop.c:320-329): a footprint that straddles a frame edge
   // moves a pixel's whole power between the image and the overflow, so these
-  // floors must not flip under FP32 rounding.  Weights then use FP32 offsets
-  // from the anchors.
+  // floors must not flip under FP32 rounding.  The weights take FP32 offsets
+  // built from the FP64 fractional bin coordinate plus a small integer, so an
+  // offset carries one FP32 rounding of itself, not of the absolute bin number.
   double dop64 = V.ax * (double)i + V.ay * (double)j + V.dopshift;
   R.ip0 = (int)floor(dop64 - V.dopdiff_max + 1 - width / 2.0) - V.P0;
   R.ip1 = (int)floor(dop64 + V.dopdiff_max + width / 2.0) - V.P0;
   R.doprel = (float)(dop64 - (double)V.P0);
+  double pfl = floor(dop64);
+  float pfr = (float)(dop64 - pfl);
+  int pm0 = (int)pfl - (V.P0 + R.ip0);  // dop64 - first bin = pfr + pm0
   int np = R.ip1 - R.ip0 + 1;
   float codef = 1.0f;
   if (dd && V.codemethod == 0) codef = sinc2f((R.doprel + V.dcrel32) / (float)V.dopfftlen);
   float sump = 0.0f;
   if (ns == 1 && np <= PW) {
     // sin(pi (x - k)) = +-sin(pi x): one sine serves every Doppler bin
-    float fr = R.doprel - floorf(R.doprel);
-    float sp = sinf(3.14159265358979f * fr);
+    float sp = sinf(3.14159265358979f * pfr);
     float s2 = sp * sp * (1.0f / (3.14159265358979f * 3.14159265358979f));
 #pragma unroll
     for (int b = 0; b < PW; b++) {
       if (b < np) {
-        float a = R.doprel - (float)(R.ip0 + b);
+        float a = pfr + (float)(pm0 - b);
         float v = (fabsf(3.14159265358979f * a) > 1.0e-4f) ? s2 / (a * a) : 1.0f;
         R.pw[b] = v;
         sump += v;
@@ -186,7 +221,7 @@ __device__ inline void radar_pixel(const ViewRec &V, const double *pl, int i, in
 #pragma unroll
     for (int b = 0; b < PW; b++) {
       if (b < np) {
-        float al = R.doprel + bl - (float)(R.ip0 + b), acc = 0.0f;
+        float al = pfr + (float)(pm0 - b) + bl, acc = 0.0f;
         for (int q = 0; q < ns; q++) {
           float g = al;
           for (int r = 0; r < ns; r++) { acc += sinc2f(g); g += xi; }
@@ -208,12 +243,15 @@ __device__ inline void radar_pixel(const ViewRec &V, const double *pl, int i, in
     R.id0 = (int)floor(del64 - V.const1) + 1 - V.D0;
     R.id1 = (int)ceil(del64 + V.const1) - 1 - V.D0;
     R.delrel = (float)(del64 - (double)V.D0);
+    double dfl = floor(del64);
+    float dfr = (float)(del64 - dfl);
+    int dm0 = (int)dfl - (V.D0 + R.id0);  // del64 - first row = dfr + dm0
     int nd = R.id1 - R.id0 + 1;
     sumd = 0.0f;
     if (nd <= DW) {
 #pragma unroll
       for (int a = 0; a < DW; a++)
-        if (a < nd) { R.dw[a] = delay_w32(V, R.delrel - (float)(R.id0 + a)); sumd += R.dw[a]; }
+        if (a < nd) { R.dw[a] = delay_w32(V, dfr + (float)(dm0 - a)); sumd += R.dw[a]; }
     } else {
       R.wide = true;
     }
@@ -224,7 +262,7 @@ __device__ inline void radar_pixel(const ViewRec &V, const double *pl, int i, in
   R.amp = R.wide ? 0.0f : radlaw32(pl, V.law_type, cose) * V.kmpp32 * V.kmpp32 * codef / (sumd * sump);
 }
 
-// scatter one pixel straight into the HBM fit image (FP32 atomics) or, for
+// scatter one pixel straight into the HBM fit image (FP64 atomics) or, for
 // Doppler pixels beyond the spectrum, into the FP64 overflow spectrum
 __device__ inline void radar_scatter_global(const ViewRec &V, const RadarPix &R, double *img, double *ov) {
   const bool dd = V.type == VT_DELAY;
