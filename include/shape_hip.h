@@ -48,6 +48,43 @@ void calc_fits(struct par_t *par, struct mod_t *mod, struct dat_t *dat);
    writes frame->chi2, cal.val, set chi2 and dat->chi2, returns dat->chi2. */
 double chi2(struct par_t *par, struct dat_t *dat, int list_breakdown);
 
+/* posclr.c:24  blank sky over the whole POS (z = -1e20, cos = b = 0,
+   f/body/comp = -1, and the *ill arrays for a bistatic POS); empty rectangles. */
+void posclr(struct pos_t *pos);
+
+/* posvis.c:84  rasterize a realized vertex set (x, facet and vertex normals)
+   into the observer (src = 0) or source (src = 1) view of the POS with the
+   CPU's z-buffer rule; a pixel the POS already owns goes only to a nearer
+   facet.  Writes z, cos, f/body/comp and the rectangle; returns outbnd and
+   sets pos->posbnd_logfactor. */
+int posvis(struct vertices_t *verts, double orbit_offset[3], struct pos_t *pos, int smooth, int src, int body,
+           int comp);
+
+/* posmask.c:58  shadowing: zero pos->cose where the source view shows a
+   different surface element nearer than tol. */
+void posmask(struct pos_t *pos, double tol);
+
+/* radlaw.c:44  radar scattering law ilaw at cos(incidence) cosinc (cosine,
+   Gaussian, Hagfors, cosine-QS and the hybrid laws). */
+double radlaw(struct photo_t *photo, int ilaw, double cosinc, int c, int f);
+
+/* pos2deldop.c:176  add view v's echo power from frame frm's POS into
+   frame->fit (accumulated); sets idellim, idoplim, dellim, doplim, km2Hz,
+   the overflow terms and badradar_logfactor; returns badradar. */
+int pos2deldop(struct par_t *par, struct photo_t *photo, double orbit_xoff, double orbit_yoff,
+               double orbit_dopoff, struct deldop_t *deldop, int body, int set, int frm, int v);
+
+/* pos2doppler.c:149  as pos2deldop for a Doppler spectrum, with the
+   overflow spectrum's o2, m2, cross section and mean Doppler. */
+int pos2doppler(struct par_t *par, struct photo_t *photo, double orbit_xoff, double orbit_yoff,
+                double orbit_dopoff, struct doppler_t *doppler, int body, int set, int frm, int v);
+
+/* apply_photo.c:78  brightness of each lit and illuminated pixel of body into
+   pos->b (Lambert, Lommel-Seeliger, geometric, Kaasalainen); returns the
+   sum, or 1e-40 when it is zero. */
+double apply_photo(struct mod_t *mod, int ilaw, double phase, double intensityfactor, struct pos_t *pos,
+                   int body, int s, int frm);
+
 /* ---------------- library control ---------------- */
 int shape_hip_last_error(void);       /* 0, or the code of the last failure */
 void shape_hip_clear_error(void);

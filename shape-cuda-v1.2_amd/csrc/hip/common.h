@@ -23,20 +23,20 @@
 namespace shp {
 
 __host__ __device__ inline uint64_t ord64(double d) {
-  uint64_t b = (uint64_t)__double_as_longlong(d);
+  uint64_t b = __builtin_bit_cast(uint64_t, d);
   return (b & 0x8000000000000000ULL) ? ~b : (b | 0x8000000000000000ULL);
 }
 __host__ __device__ inline double unord64(uint64_t k) {
   uint64_t b = (k & 0x8000000000000000ULL) ? (k & 0x7FFFFFFFFFFFFFFFULL) : ~k;
-  return __longlong_as_double((long long)b);
+  return __builtin_bit_cast(double, b);
 }
 __host__ __device__ inline uint32_t ord32(float f) {
-  uint32_t b = (uint32_t)__float_as_uint(f);
+  uint32_t b = __builtin_bit_cast(uint32_t, f);
   return (b & 0x80000000u) ? ~b : (b | 0x80000000u);
 }
 __host__ __device__ inline float unord32(uint32_t k) {
   uint32_t b = (k & 0x80000000u) ? (k & 0x7FFFFFFFu) : ~k;
-  return __uint_as_float(b);
+  return __builtin_bit_cast(float, b);
 }
 
 // Packed z-buffer key of the mixed path: depth in the high word, inverted

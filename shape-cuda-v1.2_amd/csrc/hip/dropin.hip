@@ -31,6 +31,10 @@ static void set_err(int rc, const char *where) {
   }
 }
 
+namespace shp {
+void record_error(int rc, const char *where) { set_err(rc, where); }
+}  // namespace shp
+
 extern "C" int shape_hip_last_error(void) { return g_err; }
 extern "C" void shape_hip_clear_error(void) { g_err = 0; }
 extern "C" void shape_hip_set_strict(int write_pos) { g_strict_pos = write_pos; }

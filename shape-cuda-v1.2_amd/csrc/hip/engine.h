@@ -71,7 +71,14 @@ struct ViewRec {
   int type, set, frm, v, eval, slot, frame_id, is_v0, nviews;
   int n;                 // POS half width
   int smooth, bistatic, host_spin;  // host_spin: ae/spin/deloff/dopoff supplied by host
+  int smooth_src;        // smoothing of the source view (calc_fits draws it unsmoothed: 0)
   int exclude_seen;      // 1: do not mark facets seen from this view
+  // per-POS entry points: orbit offsets (km, pixels, Doppler bins), the body
+  // whose pixels count, rectangles supplied by the caller, and rectangles
+  // grown only by pixels this call claims (posvis onto an occupied POS).
+  // Zero in the engine's own views.
+  double orb[3], orb_x, orb_y, orb_dop;
+  int body, keep_rect, rect_by_claim;
   long long pos_off;     // pixel offset of this view's POS buffers
   long long tile_off;    // offset of this view's stale-tile bitmap (mixed method)
   int pb;                // parameter block of the evaluation

@@ -615,6 +615,8 @@ int Engine::build(struct par_t *par, struct mod_t **mods, struct dat_t **dats, i
       if (any_lc) {
         if ((rc = alloc((void **)&pd.bodyill, sizeof(int) * P))) return rc;
         if ((rc = alloc((void **)&pd.compill, sizeof(int) * P))) return rc;
+        hipLaunchKernelGGL(k_fill_i32, dim3(1024), dim3(256), 0, stream, pd.bodyill, (long long)P, -1);
+        hipLaunchKernelGGL(k_fill_i32, dim3(1024), dim3(256), 0, stream, pd.compill, (long long)P, -1);
       }
     }
   }

@@ -111,6 +111,9 @@ class Engine {
   int pack_model(int m, struct par_t *par, struct mod_t *mod, struct dat_t *dat);
 };
 
+// record a failure of a reference entry point for shape_hip_last_error()
+void record_error(int rc, const char *where);
+
 // launch a kernel, bracketed by timing events when the engine times kernels
 #define SHP_LAUNCH(E, name, ...)  \
   do {                            \

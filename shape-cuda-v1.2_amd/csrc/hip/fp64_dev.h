@@ -49,7 +49,11 @@ __device__ inline void project_facet(const ViewRec &V, const GeomSlot &G, int f,
   cotrans3(P.v0, oa, G.x + 3 * fv[0]);
   cotrans3(P.v1, oa, G.x + 3 * fv[1]);
   cotrans3(P.v2, oa, G.x + 3 * fv[2]);
-  // orbit offset is zero for single bodies: v += 0.0 keeps values unchanged
+  for (int k = 0; k < 3; k++) {  // orbit offset (posvis.c:166-170; zero for single bodies)
+    P.v0[k] += V.orb[k];
+    P.v1[k] += V.orb[k];
+    P.v2[k] += V.orb[k];
+  }
   double kmpp = V.kmpp;
   int n = V.n;
   P.imin_d = floor(dmin(P.v0[0], dmin(P.v1[0], P.v2[0])) / kmpp - 1.0e-10 + 0.5);

@@ -161,7 +161,7 @@ __global__ void k_setup_views(ViewRec *views, int nviews, const double *params) 
     }
     V.delfact = -6.67128190396304 / V.del_per_pixel;
     V.delshift = V.delcom_vig + V.deloff;
-    V.dopshift = V.dopcom_vig + V.dopoff;
+    V.dopshift = V.dopcom_vig + V.dopoff + V.orb_dop;
     V.idop0 = 1000 - (int)floor(V.dopshift + 0.5);
   }
   // FP32 constants of the mixed method (anchored bin coordinates)
@@ -180,14 +180,45 @@ __global__ void k_setup_views(ViewRec *views, int nviews, const double *params) 
     V.dcrel32 = (float)(p0 - V.dopDC_vig);
   }
   // reset accumulators
-  V.xlim[0] = V.ylim[0] = V.xlim2[0] = V.ylim2[0] = V.n;
-  V.xlim[1] = V.ylim[1] = V.xlim2[1] = V.ylim2[1] = -V.n;
+  if (!V.keep_rect) {
+    V.xlim[0] = V.ylim[0] = V.xlim2[0] = V.ylim2[0] = V.n;
+    V.xlim[1] = V.ylim[1] = V.xlim2[1] = V.ylim2[1] = -V.n;
+  }
   V.outbnd = V.outbnd2 = V.any_overflow = V.nstale = V.nclaimed = 0;
   for (int q = 0; q < 4; q++) V.ov[q] = V.ov2[q] = ord64(1.0e20);
   V.idellim[0] = V.ndel + 999999; V.idellim[1] = -999999;
   V.idoplim[0] = V.ndop + 999999; V.idoplim[1] = -999999;
   V.dellim[0] = V.dellim[1] = V.doplim[0] = V.doplim[1] = ord64(1.0e20);
   V.ysum = 0.0;
+}
+
+// projection matrices of a caller's POS (posvis.c:96-135): oa = oe * ae^T,
+// sa = se * ae^T and the source direction in observer coordinates
+__global__ void k_pos_setup(ViewRec *views) {
+  ViewRec &V = views[blockIdx.x];
+  if (threadIdx.x != 0) return;
+  mul_bat(V.oa, V.oe, V.ae);
+  if (V.bistatic) {
+    mul_bat(V.sa, V.se, V.ae);
+    double u[3] = {0.0, 0.0, 1.0};
+    cotrans3T(u, V.se, u);
+    cotrans3(V.usrc, V.oe, u);
+  }
+}
+
+// seed the z-buffer from a POS that already holds claims: a pixel with an
+// owner starts at its depth, so only nearer facets claim it (posvis.c:236)
+__global__ void k_pos_seed(ViewRec *views, PosDev pd, int src) {
+  const ViewRec &V = views[blockIdx.y];
+  int w = 2 * V.n + 1;
+  int pix = blockIdx.x * blockDim.x + threadIdx.x;
+  if (pix >= w * w) return;
+  long long p = V.pos_off + pix;
+  int fac = (src ? pd.fill : pd.f)[p];
+  double z = (src ? pd.zill : pd.z)[p];
+  pd.zkey[p] = fac >= 0 ? (unsigned long long)ord64(z) : 0ULL;
+  pd.fbuf[p] = 0x7fffffff;
+  pd.stale_idx[p] = -1;
 }
 
 // ---------------------------------------------------------------------------
@@ -307,7 +338,9 @@ __global__ void k_raster64(ViewRec *views, const GeomSlot *slots, PosDev pd, int
             long long p = base + (long long)(i + n) * w + (j + n);
             if (pass == 1) {
               atomicMax(&pd.zkey[p], (unsigned long long)ord64(z));
-              xl0 = min(xl0, i); xl1 = max(xl1, i); yl0 = min(yl0, j); yl1 = max(yl1, j);
+              if (!V.rect_by_claim) {
+                xl0 = min(xl0, i); xl1 = max(xl1, i); yl0 = min(yl0, j); yl1 = max(yl1, j);
+              }
               o0 = dmin(o0, (double)i); o1 = dmax(o1, (double)i);
               o2 = dmin(o2, (double)j); o3 = dmax(o3, (double)j);
             } else if (pass == 2) {
@@ -377,11 +410,18 @@ __global__ void k_resolve64(ViewRec *views, const GeomSlot *slots, PosDev pd, in
   project_facet(V, G, fw, src, P);
   double s, t, z;
   pixel_hit(P, V.kmpp, i, j, s, t, z);
-  int smooth = src ? 0 : V.smooth;
+  int smooth = src ? V.smooth_src : V.smooth;
   double nz, cosb;
   shade64(V, G, fw, src, smooth, s, t, nz, cosb);
   zz[p] = z;
   fac[p] = fw;
+  if (V.rect_by_claim) {  // posvis.c:322-364 for a pixel this call claims
+    int *xl = src ? V.xlim2 : V.xlim, *yl = src ? V.ylim2 : V.ylim;
+    if (i < xl[0]) atomicMin(&xl[0], i);
+    if (i > xl[1]) atomicMax(&xl[1], i);
+    if (j < yl[0]) atomicMin(&yl[0], j);
+    if (j > yl[1]) atomicMax(&yl[1], j);
+  }
   if (pd.body) {
     (src ? pd.bodyill : pd.body)[p] = body;
     (src ? pd.compill : pd.comp)[p] = comp;
@@ -432,7 +472,7 @@ __device__ void replay64_one(ViewRec *views, const GeomSlot *slots, const PosDev
   }
   double zz = S.oldz, cosa = S.oldcose, cosb = S.oldcosi;
   int fac = S.oldf;
-  int src = S.src, smooth = src ? 0 : V.smooth;
+  int src = S.src, smooth = src ? V.smooth_src : V.smooth;
   for (int q = 0; q < nc; q++) {
     FacetProj P;
     project_facet(V, G, c[q], src, P);
@@ -469,11 +509,12 @@ __device__ inline double zill_at(const ViewRec &V, const PosDev &pd, int i, int 
   int w = 2 * V.n + 1;
   return pd.zill[V.pos_off + (long long)(i + V.n) * w + (j + V.n)];
 }
-__device__ inline int fill_at(const ViewRec &V, const PosDev &pd, int i, int j) {
+__device__ inline int ill_at(const ViewRec &V, const int *a, int i, int j) {
   if (i < V.xlim2[0] || i > V.xlim2[1] || j < V.ylim2[0] || j > V.ylim2[1]) return -1;
   int w = 2 * V.n + 1;
-  return pd.fill[V.pos_off + (long long)(i + V.n) * w + (j + V.n)];
+  return a[V.pos_off + (long long)(i + V.n) * w + (j + V.n)];
 }
+__device__ inline int fill_at(const ViewRec &V, const PosDev &pd, int i, int j) { return ill_at(V, pd.fill, i, j); }
 
 __global__ void k_posmask64(ViewRec *views, PosDev pd, double tol) {
   const ViewRec &V = views[blockIdx.y];
@@ -495,7 +536,10 @@ __global__ void k_posmask64(ViewRec *views, PosDev pd, double tol) {
   double big = 0.99 * 1.0e20;
   if (!(fabs(i0) < n && fabs(j0) < n)) return;
   if (!(zill_at(V, pd, im, jm) > -big)) return;
-  if (!(pd.f[p] != fill_at(V, pd, im, jm))) return;  // body/comp are 0 for every pixel here
+  bool other = pd.f[p] != fill_at(V, pd, im, jm);
+  if (pd.comp && pd.compill) other = other || pd.comp[p] != ill_at(V, pd.compill, im, jm) ||
+                                    pd.body[p] != ill_at(V, pd.bodyill, im, jm);
+  if (!other) return;
   int i1 = (int)floor(i0), j1 = (int)floor(j0);
   double zill;
   double z11 = zill_at(V, pd, i1, j1), z21 = zill_at(V, pd, i1 + 1, j1), z12 = zill_at(V, pd, i1, j1 + 1),
@@ -540,7 +584,7 @@ __global__ void k_photo64(ViewRec *views, PosDev pd, const double *params) {
     if (i >= V.xlim[0] && i <= V.xlim[1] && j >= V.ylim[0] && j <= V.ylim[1]) {
       long long p = V.pos_off + pix;
       double ce = pd.cose[p], ci = pd.cosi[p];
-      if (ce > 0.0 && ci > 0.0) {
+      if (ce > 0.0 && ci > 0.0 && (pd.body ? pd.body[p] == V.body : true)) {
         const double *pl = params + V.pb + V.pb_law;
         int type = V.law_type;
         if (type == LAMBERTLAW) b = V.intensityfactor * (pl[1] / 3.141592653589793) * ci;
@@ -638,6 +682,11 @@ __device__ inline double radlaw64(const double *pl, int type, double c) {
   return d;
 }
 
+// one radar scattering-law value (radlaw.c:44-148)
+__global__ void k_radlaw_one(const double *pl, int type, double c, double *out) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *out = radlaw64(pl, type, c);
+}
+
 __global__ void k_deldop64(ViewRec *views, FrameRec *frames, PosDev pd, double *fit, const double *params) {
   ViewRec &V = views[blockIdx.y];
   int n = V.n, w = 2 * n + 1;
@@ -648,13 +697,13 @@ __global__ void k_deldop64(ViewRec *views, FrameRec *frames, PosDev pd, double *
     int x = pix / w - n, y = pix % w - n;
     long long p = V.pos_off + pix;
     if (x >= V.xlim[0] && x <= V.xlim[1] && y >= V.ylim[0] && y <= V.ylim[1] && pd.cose[p] > 0.0 &&
-        (pd.body ? pd.body[p] == 0 : true) && pd.z[p] != -1e20) {
+        (pd.body ? pd.body[p] == V.body : true) && pd.z[p] != -1e20) {
       const FrameRec &F = frames[V.frame_id];
       int ndel = V.ndel, ndop = V.ndop, width = V.sinc2width;
       double delPOS = pd.z[p] * V.delfact + V.delshift;
       int idel_min = (int)floor(delPOS - V.const1) + 1;
       int idel_max = (int)ceil(delPOS + V.const1) - 1;
-      double dopPOS = V.ax * (x - 0.0) + V.ay * (y - 0.0) + V.dopshift;
+      double dopPOS = V.ax * (x - V.orb_x) + V.ay * (y - V.orb_y) + V.dopshift;
       int idop_min = (int)floor(dopPOS - V.dopdiff_max + 1 - width / 2.0);
       int idop_max = (int)floor(dopPOS + V.dopdiff_max + width / 2.0);
       double codefactor = (V.codemethod == 0) ? sinc2d((dopPOS - V.dopDC_vig) / V.dopfftlen) : 1.0;
@@ -715,10 +764,10 @@ __global__ void k_doppler64(ViewRec *views, FrameRec *frames, PosDev pd, double 
     int x = pix / w - n, y = pix % w - n;
     long long p = V.pos_off + pix;
     if (x >= V.xlim[0] && x <= V.xlim[1] && y >= V.ylim[0] && y <= V.ylim[1] && pd.cose[p] > 0.0 &&
-        (pd.body ? pd.body[p] == 0 : true)) {
+        (pd.body ? pd.body[p] == V.body : true)) {
       const FrameRec &F = frames[V.frame_id];
       int ndop = V.ndop, width = V.sinc2width;
-      double dopPOS = V.ax * (x - 0.0) + V.ay * (y - 0.0) + V.dopshift;
+      double dopPOS = V.ax * (x - V.orb_x) + V.ay * (y - V.orb_y) + V.dopshift;
       int idop_min = (int)floor(dopPOS - V.dopdiff_max + 1 - width / 2.0);
       int idop_max = (int)floor(dopPOS + V.dopdiff_max + width / 2.0);
       jl0 = idop_min; jl1 = idop_max;

@@ -43,6 +43,9 @@ constexpr int OUT_OUTBND = 0, OUT_POSBND_LOG = 2, OUT_DELLIM = 4, OUT_DOPLIM = 6
 
 // ---- FP64 kernels (k_fp64.hip) ----
 __global__ void k_setup_views(ViewRec *views, int nviews, const double *params);
+__global__ void k_pos_setup(ViewRec *views);
+__global__ void k_pos_seed(ViewRec *views, PosDev pd, int src);
+__global__ void k_radlaw_one(const double *pl, int type, double c, double *out);
 __global__ void k_realize_vertices(GeomSlot *slots, int nslot, const double *params, int *slot_flags,
                                    double *slot_logf);
 __global__ void k_realize_facets(GeomSlot *slots, int nslot);

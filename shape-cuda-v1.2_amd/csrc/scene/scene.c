@@ -984,6 +984,40 @@ void shp_pos_array(struct pos_t *pos, int which, double *out)
     for (int j = -n; j <= n; j++) *out++ = d ? d[i][j] : (double)iv[i][j];
 }
 
+void shp_pos_copy(struct pos_t *dst, const struct pos_t *src)
+{
+  int n = src->n;
+  double **dd[] = {dst->b, dst->cose, dst->cosi, dst->z, dst->cosill, dst->zill};
+  double **sd[] = {src->b, src->cose, src->cosi, src->z, src->cosill, src->zill};
+  int **di[] = {dst->f, dst->body, dst->comp, dst->fill, dst->bodyill, dst->compill};
+  int **si[] = {src->f, src->body, src->comp, src->fill, src->bodyill, src->compill};
+  for (int a = 0; a < 6; a++) {
+    for (int i = -n; i <= n; i++) {
+      if (dd[a] && sd[a]) memcpy(&dd[a][i][-n], &sd[a][i][-n], sizeof(double) * (2 * n + 1));
+      if (di[a] && si[a]) memcpy(&di[a][i][-n], &si[a][i][-n], sizeof(int) * (2 * n + 1));
+    }
+  }
+  for (int k = 0; k < 2; k++) {
+    dst->xlim[k] = src->xlim[k]; dst->ylim[k] = src->ylim[k];
+    dst->xlim2[k] = src->xlim2[k]; dst->ylim2[k] = src->ylim2[k];
+  }
+  memcpy(dst->ae, src->ae, sizeof(src->ae));
+  memcpy(dst->oe, src->oe, sizeof(src->oe));
+  memcpy(dst->se, src->se, sizeof(src->se));
+  dst->bistatic = src->bistatic;
+  dst->km_per_pixel = src->km_per_pixel;
+  dst->posbnd_logfactor = src->posbnd_logfactor;
+}
+
+void shp_pos_scalars(struct pos_t *pos, double out[4])
+{
+  out[0] = pos->posbnd_logfactor; out[1] = pos->bistatic; out[2] = pos->km_per_pixel; out[3] = pos->n;
+}
+
+struct vertices_t *shp_mod_vertices(struct mod_t *mod) { return &mod->shape.comp[0].real; }
+
+void *shp_set_desc(struct dat_t *dat, int s) { return &dat->set[s].desc; }
+
 void shp_pos_limits(struct pos_t *pos, int out[8])
 {
   out[0] = pos->xlim[0]; out[1] = pos->xlim[1]; out[2] = pos->ylim[0]; out[3] = pos->ylim[1];

@@ -113,6 +113,13 @@ struct pos_t *shp_frame_pos(struct dat_t *dat, int s, int f);
           10 f, 11 body, 12 comp, 13 fill               (ints, written as doubles) */
 void shp_pos_array(struct pos_t *pos, int which, double *out);
 void shp_pos_limits(struct pos_t *pos, int out[8]); /* xlim, ylim, xlim2, ylim2 */
+/* Copy every array, rectangle and matrix of src into dst (same n) */
+void shp_pos_copy(struct pos_t *dst, const struct pos_t *src);
+/* posbnd_logfactor, bistatic, km_per_pixel, n */
+void shp_pos_scalars(struct pos_t *pos, double out[4]);
+/* the realized vertex set of component 0, and a set's deldop_t / doppler_t */
+struct vertices_t *shp_mod_vertices(struct mod_t *mod);
+void *shp_set_desc(struct dat_t *dat, int s);
 /* view data: ae[9], oe[9], spin[3], intspin[3], deloff, dopoff -> 29 doubles */
 void shp_view_data(struct dat_t *dat, int s, int f, int v, double *out);
 void shp_set_view_data(struct dat_t *dat, int s, int f, int v, const double *in);

@@ -82,6 +82,10 @@ def load_scene():
             "shp_frame_pos": (vp, [vp, C.c_int, C.c_int]),
             "shp_pos_array": (None, [vp, C.c_int, dp]),
             "shp_pos_limits": (None, [vp, ip]),
+            "shp_pos_copy": (None, [vp, vp]),
+            "shp_pos_scalars": (None, [vp, dp]),
+            "shp_mod_vertices": (vp, [vp]),
+            "shp_set_desc": (vp, [vp, C.c_int]),
             "shp_view_data": (None, [vp, C.c_int, C.c_int, C.c_int, dp]),
             "shp_set_view_data": (None, [vp, C.c_int, C.c_int, C.c_int, dp]),
             "shp_geometry": (None, [vp, dp, dp, dp, dp]),
@@ -119,6 +123,15 @@ def load_hip():
         lib = C.CDLL(path)
         vp = C.c_void_p
         sigs = {
+            "posclr": (None, [vp]),
+            "posvis": (C.c_int, [vp, C.POINTER(C.c_double), vp, C.c_int, C.c_int, C.c_int, C.c_int]),
+            "posmask": (None, [vp, C.c_double]),
+            "radlaw": (C.c_double, [vp, C.c_int, C.c_double, C.c_int, C.c_int]),
+            "pos2deldop": (C.c_int, [vp, vp, C.c_double, C.c_double, C.c_double, vp, C.c_int, C.c_int,
+                                     C.c_int, C.c_int]),
+            "pos2doppler": (C.c_int, [vp, vp, C.c_double, C.c_double, C.c_double, vp, C.c_int, C.c_int,
+                                      C.c_int, C.c_int]),
+            "apply_photo": (C.c_double, [vp, C.c_int, C.c_double, C.c_double, vp, C.c_int, C.c_int, C.c_int]),
             "realize_mod": (None, [vp, vp]),
             "realize_spin": (None, [vp, vp, vp]),
             "realize_delcor": (None, [vp, C.c_double, C.c_int]),
@@ -343,6 +356,21 @@ class Scene:
         out = np.zeros((2 * n + 1, 2 * n + 1))
         self.lib.shp_pos_array(pos, which, out.ctypes.data_as(C.POINTER(C.c_double)))
         return out
+
+    def pos_copy(self, dst, src):
+        self.lib.shp_pos_copy(dst, src)
+
+    def pos_scalars(self, pos):
+        out = np.zeros(4)
+        self.lib.shp_pos_scalars(pos, out.ctypes.data_as(C.POINTER(C.c_double)))
+        return dict(zip(["posbnd_logfactor", "bistatic", "km_per_pixel", "n"], out))
+
+    @property
+    def vertices(self):
+        return self.lib.shp_mod_vertices(self.mod)
+
+    def set_desc(self, s):
+        return self.lib.shp_set_desc(self.dat, s)
 
     def pos_limits(self, pos):
         out = (C.c_int * 8)()
