@@ -175,7 +175,8 @@ def cpu_baseline(nframes):
         fn = getattr(orc, name)
         fn.restype, fn.argtypes = res, args
     threads = max(1, min(16, len(os.sched_getaffinity(0))))
-    sc = sh.Scene(sh.spec_for(4, n_deldop=nframes))
+    # one POS per frame (LOCAL scope): the threads render frames concurrently
+    sc = sh.Scene(sh.spec_for(4, n_deldop=nframes, pos_scope=sh.LOCAL))
 
     def forward():
         orc.oracle_realize_mod(sc.par, sc.mod)
@@ -184,7 +185,10 @@ def cpu_baseline(nframes):
         orc.oracle_calc_fits_threads(sc.par, sc.mod, sc.dat, threads)
         return orc.oracle_chi2(sc.par, sc.dat, 0)
 
+    orc.oracle_last_error.restype = C.c_char_p
     forward()
+    if orc.oracle_last_error():
+        raise RuntimeError("oracle: %s" % orc.oracle_last_error().decode())
     sc.synthesize_obs(snr=20.0)
     t0 = time.perf_counter()
     forward()

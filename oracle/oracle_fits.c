@@ -253,6 +253,11 @@ static void *o_thread_main(void *p)
 void oracle_calc_fits_threads(struct par_t *par, struct mod_t *mod, struct dat_t *dat, int nthreads)
 {
   if (nthreads < 1) nthreads = 1;
+  if (nthreads > 1 && par->pos_scope != LOCAL) {
+    /* frames share one POS under GLOBAL scope: threads would race on it */
+    oracle_set_error("oracle_calc_fits_threads: threads need pos_scope LOCAL");
+    return;
+  }
   reset_flags(par, mod);
   struct o_flags *flags = (struct o_flags *)calloc(dat->nsets, sizeof(struct o_flags));
   pthread_t *th = (pthread_t *)malloc(sizeof(pthread_t) * nthreads);

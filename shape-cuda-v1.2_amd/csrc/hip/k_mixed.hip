@@ -43,7 +43,9 @@ __global__ __launch_bounds__(256) void k_raster32(ViewRec *views, const GeomSlot
   if (src && !V.bistatic) return;
   if (pass == 3 && *pd.nstale == 0) return;
   const GeomSlot &G = slots[V.slot];
-  int f = blockIdx.x * blockDim.x + threadIdx.x;
+  // pass 1 walks every facet of every view: XCD-local eighths of the model.
+  // Pass 3 works only near flagged tiles, which cluster: plain order balances it.
+  int f = (pass == 1 ? xcd_block(blockIdx.x, gridDim.x) : (int)blockIdx.x) * blockDim.x + threadIdx.x;
   int n = V.n, w = 2 * n + 1;
   unsigned long long *kb = pd.zkey + V.pos_off + (src ? sun_off : 0);
   int xl0 = n, xl1 = -n, yl0 = n, yl1 = -n, ob = 0;
@@ -205,7 +207,7 @@ static inline unsigned cdivm(long long a, long long b) { return (unsigned)((a + 
 int enqueue_mixed(Engine &E) {
   if (E.any_lc) return -41;  // lightcurve datasets use the FP64 method
   hipStream_t st = E.stream;
-  dim3 gf(cdivm(E.max_nf, 256), E.nviews), bt(256);
+  dim3 gf(pad_xcd(cdivm(E.max_nf, 256)), E.nviews), bt(256);
   int wmax_ = 0;
   for (auto &V : E.h_views) wmax_ = std::max(wmax_, 2 * V.n + 1);
   int tiles = cdivm(wmax_, TILE);

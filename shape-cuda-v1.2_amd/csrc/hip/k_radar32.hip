@@ -6,36 +6,193 @@
 
 namespace shp {
 
-// Per-view bin limits of lit pixels (pos2deldop.c:337-354) and the count of
-// claimed keys: wave reductions, then one update per workgroup.  The limits
-// only move outward, so an update that a (possibly stale) read of the current
-// value shows cannot move them is skipped: interior tiles issue no atomics.
-__device__ inline void view_limits_block(ViewRec &V, bool lit, const RadarPix &R, bool claimed) {
+// ---------------------------------------------------------------------------
+// one workgroup per TILE x TILE tile of a view's POS (TPP pixels per thread):
+//   A. read and clear the tile's keys; the claimed pixels' delay rows and
+//      Doppler columns bound the tile's footprint window exactly
+//   B. shade each owner, compute its delay-Doppler footprint and add it into
+//      the FP64 LDS window at once (footprints leaving the frame, or a window
+//      too large for LDS, go straight to HBM)
+//   C. flush the window to the FP64 fit image; one update of the view's
+//      bin limits and claimed-key count per workgroup
+// ---------------------------------------------------------------------------
+// block-wide minimum of four ints at once (one barrier pair)
+__device__ inline void blk_min4(int &a, int &b, int &c, int &d, int (*sh)[4]) {
+  a = wmin(a); b = wmin(b); c = wmin(c); d = wmin(d);
+  int wv = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) { sh[wv][0] = a; sh[wv][1] = b; sh[wv][2] = c; sh[wv][3] = d; }
+  __syncthreads();
+  a = sh[0][0]; b = sh[0][1]; c = sh[0][2]; d = sh[0][3];
+  for (int q = 1; q < (int)(blockDim.x >> 6); q++) {
+    a = min(a, sh[q][0]); b = min(b, sh[q][1]); c = min(c, sh[q][2]); d = min(d, sh[q][3]);
+  }
+  __syncthreads();
+}
+
+__global__ __launch_bounds__(256) void k_shade_radar32(ViewRec *views, const GeomSlot *slots, PosDev pd,
+                                                       double *fit, const FrameRec *frames, double *overflow,
+                                                       const double *params, int *counters) {
+  ViewRec &V = views[blockIdx.y];
+  if (V.type != VT_DELAY && V.type != VT_DOPPLER) return;
   const bool dd = V.type == VT_DELAY;
-  __shared__ int si[4][6];
-  __shared__ float sf[4][4];
-  int a0 = lit ? R.id0 : 0x7fffffff, a1 = lit ? R.id1 : -0x7fffffff;
-  int b0 = lit ? R.ip0 : 0x7fffffff, b1 = lit ? R.ip1 : -0x7fffffff;
-  float d0 = lit ? R.delrel : 1e30f, d1 = lit ? R.delrel : -1e30f;
-  float p0 = lit ? R.doprel : 1e30f, p1 = lit ? R.doprel : -1e30f;
+  const int n = V.n, w = 2 * n + 1;
+  const int tx = (w + TILE - 1) / TILE;
+  const int ti = blockIdx.x / tx, tj = blockIdx.x % tx;
+  if (ti >= tx) return;
+  const int i0 = ti * TILE - n, j0 = tj * TILE - n;
+  const int xl0 = V.xlim[0], xl1 = V.xlim[1], yl0 = V.ylim[0], yl1 = V.ylim[1];
+  if (i0 > xl1 || i0 + TILE - 1 < xl0 || j0 > yl1 || j0 + TILE - 1 < yl0) return;
+  __shared__ double win[WIN_MAX];
+  __shared__ int shi[8][4];
+  __shared__ float shf[4][4];
+  __shared__ int shw[4][4];
+  const GeomSlot &G = slots[V.slot];
+  const double *pl = params + V.pb + V.pb_law;
+  const int width = V.sinc2width;
+
+  // Pixel order within the tile: a wave's 64 pixels lie along the axis on
+  // which the Doppler coordinate changes faster, so they spread over more
+  // Doppler bins and collide less in the LDS window.
+  const bool along_j = fabs(V.ay) >= fabs(V.ax);
+
+  // ---- A: keys, claimed pixels and the footprint window ----
+  unsigned long long key[TPP];
+  int lo_d = 0x7fffffff, hi_d = -0x7fffffff, lo_p = 0x7fffffff, hi_p = -0x7fffffff, ncl = 0;
+#pragma unroll
+  for (int m = 0; m < TPP; m++) {
+    int q = threadIdx.x + m * (int)blockDim.x;
+    int i = i0 + (along_j ? q / TILE : q % TILE), j = j0 + (along_j ? q % TILE : q / TILE);
+    key[m] = 0ULL;
+    if (i <= n && j <= n && i >= xl0 && i <= xl1 && j >= yl0 && j <= yl1) {
+      long long p = V.pos_off + (long long)(i + n) * w + (j + n);
+      unsigned long long k = pd.zkey[p];
+      if (k) {
+        pd.zkey[p] = 0ULL;
+        ncl++;
+        // marginal pixels (near a facet edge) are resolved by the FP64 replay
+        if (pd.stale_tiles[stale_tile(V, i, j)] && pd.stale_idx[p] >= 0) k = 0ULL;
+      }
+      key[m] = k;
+      if (k) {
+        double dop64 = V.ax * (double)i + V.ay * (double)j + V.dopshift;
+        lo_p = min(lo_p, (int)floor(dop64 - V.dopdiff_max + 1 - width / 2.0) - V.P0);
+        hi_p = max(hi_p, (int)floor(dop64 + V.dopdiff_max + width / 2.0) - V.P0);
+        if (dd) {
+          double del64 = (double)key_z(k) * V.delfact + V.delshift;
+          lo_d = min(lo_d, (int)floor(del64 - V.const1) + 1 - V.D0);
+          hi_d = max(hi_d, (int)ceil(del64 + V.const1) - 1 - V.D0);
+        }
+      }
+    }
+  }
+  {
+    int nhp = -hi_p, nhd = -hi_d;
+    blk_min4(lo_p, nhp, lo_d, nhd, shw);
+    hi_p = -nhp; hi_d = -nhd;
+    if (!dd) lo_d = hi_d = 0;
+  }
+  const bool any = lo_p != 0x7fffffff;
+  if (any) {  // one bin of margin: phase B recomputes the same floors
+    lo_p--; hi_p++;
+    if (dd) { lo_d--; hi_d++; }
+  }
+  const int wd = hi_d - lo_d + 1, wp = hi_p - lo_p + 1;
+  const bool use_lds = any && (long long)wd * wp <= WIN_MAX;
+  if (use_lds) {
+    for (int q = threadIdx.x; q < wd * wp; q += blockDim.x) win[q] = 0.0;
+    __syncthreads();
+  }
+
+  // ---- B: shade, footprint, accumulate ----
+  double *img = fit + frames[V.frame_id].off;
+  double *ov = overflow + (long long)blockIdx.y * 2000;
+  int a0 = 0x7fffffff, a1 = -0x7fffffff, b0 = 0x7fffffff, b1 = -0x7fffffff, anyo = 0;
+  float d0 = 1e30f, d1 = -1e30f, p0 = 1e30f, p1 = -1e30f;
+  for (int m = 0; m < TPP; m++) {
+    unsigned long long k = key[m];
+    if (!k) continue;
+    int q = threadIdx.x + m * (int)blockDim.x;
+    int i = i0 + (along_j ? q / TILE : q % TILE), j = j0 + (along_j ? q % TILE : q / TILE);
+    int f = key_facet(k);
+    Proj32 P;
+    corners32(V, G, f, V.oa32, P);
+    float s, t;
+    bary32(P, i, j, s, t);  // same arithmetic as the raster
+    float nx, ny, nz;
+    normal32(G, f, V.oa32, V.smooth, s, t, nx, ny, nz);
+    if (!(nz > 0.0f)) {
+      // cos(scattering) is inherited from an earlier claimant: FP64 replay
+      long long p = V.pos_off + (long long)(i + n) * w + (j + n);
+      int si = atomicAdd(pd.nstale, 1);
+      if (si < pd.stale_cap) {
+        StaleRec &S = pd.stale[si];
+        S.view = blockIdx.y; S.pix = (int)(p - V.pos_off); S.oldf = -1; S.oldz = -1.0e20; S.oldcose = 0.0;
+        S.oldcosi = 0.0; S.ncand = 0; S.src = 0;
+        pd.stale_idx[p] = si;
+        pd.stale_tiles[stale_tile(V, i, j)] = 1;
+      } else {
+        atomicOr(pd.error, 1);
+      }
+      continue;
+    }
+    RadarPix R;
+    radar_pixel(V, pl, i, j, (double)key_z(k), nz, R);
+    a0 = min(a0, R.id0); a1 = max(a1, R.id1); b0 = min(b0, R.ip0); b1 = max(b1, R.ip1);
+    d0 = fminf(d0, R.delrel); d1 = fmaxf(d1, R.delrel); p0 = fminf(p0, R.doprel); p1 = fmaxf(p1, R.doprel);
+    if (!R.inb) anyo = 1;
+    if (R.wide) { atomicAdd(&counters[3], 1); continue; }
+    if (use_lds && R.inb) {
+      int np = R.ip1 - R.ip0 + 1, nd = dd ? R.id1 - R.id0 + 1 : 1;
+#pragma unroll
+      for (int a = 0; a < DW; a++) {
+        if (a < nd) {
+          float ad = dd ? R.amp * R.dw[a] : R.amp;
+          double *row = win + (dd ? (R.id0 + a - lo_d) * wp : 0) + (R.ip0 - lo_p);
+#pragma unroll
+          for (int b = 0; b < PW; b++)
+            if (b < np) atomicAdd(&row[b], (double)(ad * R.pw[b]));
+        }
+      }
+    } else {
+      radar_scatter_global(V, R, img, ov);
+    }
+  }
+
+  // ---- C: flush the window; limits and claimed keys, once per workgroup ----
+  if (use_lds) {
+    __syncthreads();
+    int ndop = V.ndop;
+    for (int q = threadIdx.x; q < wd * wp; q += blockDim.x) {
+      double v = win[q];
+      if (v != 0.0) {
+        int r = q / wp, c = q % wp;
+        int idel = V.D0 + lo_d + r, idop = V.P0 + lo_p + c;
+        if (dd) atomicAdd(&img[(long long)(idel - 1) * ndop + (idop - 1)], v);
+        else atomicAdd(&img[idop - 1], v);
+      }
+    }
+  }
   a0 = wmin(a0); a1 = wmax(a1); b0 = wmin(b0); b1 = wmax(b1);
   d0 = wminf(d0); d1 = wmaxf(d1); p0 = wminf(p0); p1 = wmaxf(p1);
-  int anyo = __ballot(lit && !R.inb) != 0ULL;
-  int ncl = __popcll(__ballot(claimed));
+  anyo = wmax(anyo);
+  ncl = wsum(ncl);
   int wv = threadIdx.x >> 6;
   if ((threadIdx.x & 63) == 0) {
-    si[wv][0] = a0; si[wv][1] = a1; si[wv][2] = b0; si[wv][3] = b1; si[wv][4] = anyo; si[wv][5] = ncl;
-    sf[wv][0] = d0; sf[wv][1] = d1; sf[wv][2] = p0; sf[wv][3] = p1;
+    shi[4][wv] = a0; shi[5][wv] = a1; shi[6][wv] = b0; shi[7][wv] = b1;
+    shf[0][wv] = d0; shf[1][wv] = d1; shf[2][wv] = p0; shf[3][wv] = p1;
+    shi[0][wv] = anyo; shi[1][wv] = ncl;
   }
   __syncthreads();
   if (threadIdx.x != 0) return;
   for (int q = 1; q < (int)(blockDim.x >> 6); q++) {
-    a0 = min(a0, si[q][0]); a1 = max(a1, si[q][1]); b0 = min(b0, si[q][2]); b1 = max(b1, si[q][3]);
-    anyo |= si[q][4]; ncl += si[q][5];
-    d0 = fminf(d0, sf[q][0]); d1 = fmaxf(d1, sf[q][1]); p0 = fminf(p0, sf[q][2]); p1 = fmaxf(p1, sf[q][3]);
+    a0 = min(a0, shi[4][q]); a1 = max(a1, shi[5][q]); b0 = min(b0, shi[6][q]); b1 = max(b1, shi[7][q]);
+    d0 = fminf(d0, shf[0][q]); d1 = fmaxf(d1, shf[1][q]); p0 = fminf(p0, shf[2][q]); p1 = fmaxf(p1, shf[3][q]);
+    anyo |= shi[0][q]; ncl += shi[1][q];
   }
   if (ncl) atomicAdd(&V.nclaimed, ncl);
   if (a0 == 0x7fffffff) return;
+  // the limits only move outward: skip updates a read of the current value
+  // shows cannot move them
   if (dd) {
     if (V.D0 + a0 < V.idellim[0]) atomicMin(&V.idellim[0], V.D0 + a0);
     if (V.D0 + a1 > V.idellim[1]) atomicMax(&V.idellim[1], V.D0 + a1);
@@ -49,121 +206,6 @@ __device__ inline void view_limits_block(ViewRec &V, bool lit, const RadarPix &R
   if (k2 < V.doplim[0]) atomicMin(&V.doplim[0], k2);
   if (k3 < V.doplim[1]) atomicMin(&V.doplim[1], k3);
   if (!dd && anyo && !V.any_overflow) atomicOr(&V.any_overflow, 1);
-}
-
-// ---------------------------------------------------------------------------
-// one workgroup per 16x16 tile of a view's POS: read and clear the keys,
-// shade the owners, accumulate the tile's delay-Doppler footprint in LDS,
-// flush once to the FP64 fit image
-// ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_shade_radar32(ViewRec *views, const GeomSlot *slots, PosDev pd,
-                                                       double *fit, const FrameRec *frames, double *overflow,
-                                                       const double *params, int *counters) {
-  ViewRec &V = views[blockIdx.y];
-  if (V.type != VT_DELAY && V.type != VT_DOPPLER) return;
-  const bool dd = V.type == VT_DELAY;
-  int n = V.n, w = 2 * n + 1;
-  int tx = (w + TILE - 1) / TILE;
-  int ti = blockIdx.x / tx, tj = blockIdx.x % tx;
-  if (ti >= tx) return;
-  int i0 = ti * TILE - n, j0 = tj * TILE - n;
-  if (i0 > V.xlim[1] || i0 + TILE - 1 < V.xlim[0] || j0 > V.ylim[1] || j0 + TILE - 1 < V.ylim[0]) return;
-  int i = i0 + (threadIdx.x >> 4), j = j0 + (threadIdx.x & 15);
-  const GeomSlot &G = slots[V.slot];
-  const FrameRec &F = frames[V.frame_id];
-  const double *pl = params + V.pb + V.pb_law;
-  bool lit = false;
-  RadarPix R;
-  R.inb = false; R.wide = false; R.amp = 0.0f; R.delrel = R.doprel = 0.0f;
-  R.id0 = R.id1 = R.ip0 = R.ip1 = 0;
-  bool claimed = false;
-  if (i <= n && j <= n && i >= V.xlim[0] && i <= V.xlim[1] && j >= V.ylim[0] && j <= V.ylim[1]) {
-    long long pix = (long long)(i + n) * w + (j + n);
-    long long p = V.pos_off + pix;
-    unsigned long long k = pd.zkey[p];
-    if (k) pd.zkey[p] = 0ULL;
-    claimed = k != 0ULL;
-    // marginal pixels (near a facet edge) are resolved by the FP64 replay
-    if (k && pd.stale_tiles[stale_tile(V, i, j)] && pd.stale_idx[p] >= 0) k = 0ULL;
-    if (k) {
-      int f = key_facet(k);
-      float z = key_z(k);
-      Proj32 P;
-      corners32(V, G, f, V.oa32, P);
-      float s, t;
-      bary32(P, i, j, s, t);  // same arithmetic as the raster
-      float nx, ny, nz;
-      normal32(G, f, V.oa32, V.smooth, s, t, nx, ny, nz);
-      if (nz > 0.0f) {
-        lit = true;
-        radar_pixel(V, pl, i, j, z, nz, R);
-        if (R.wide) atomicAdd(&counters[3], 1);
-      } else {
-        // cos(scattering) is inherited from an earlier claimant: replay later
-        int si = atomicAdd(pd.nstale, 1);
-        if (si < pd.stale_cap) {
-          StaleRec &S = pd.stale[si];
-          S.view = blockIdx.y; S.pix = (int)pix; S.oldf = -1; S.oldz = -1.0e20; S.oldcose = 0.0;
-          S.oldcosi = 0.0; S.ncand = 0; S.src = 0;
-          pd.stale_idx[p] = si;
-          pd.stale_tiles[stale_tile(V, i, j)] = 1;
-        } else {
-          atomicOr(pd.error, 1);
-        }
-      }
-    }
-  }
-  view_limits_block(V, lit, R, claimed);
-  // ---- LDS window over the tile's in-bounds footprints ----
-  // FP32 contributions, FP64 accumulation: a bin gathers thousands of pixels
-  // and chi-square later cancels o2 - om^2/m2, so the sums must not round at
-  // FP32 precision
-  __shared__ double win[WIN_MAX];
-  __shared__ int wb[4];
-  if (threadIdx.x == 0) { wb[0] = 0x7fffffff; wb[1] = -0x7fffffff; wb[2] = 0x7fffffff; wb[3] = -0x7fffffff; }
-  __syncthreads();
-  bool scat = lit && R.inb && !R.wide;
-  if (scat) {
-    if (dd) { atomicMin(&wb[0], R.id0); atomicMax(&wb[1], R.id1); }
-    atomicMin(&wb[2], R.ip0); atomicMax(&wb[3], R.ip1);
-  }
-  __syncthreads();
-  int lo_d = dd ? wb[0] : 0, hi_d = dd ? wb[1] : 0, lo_p = wb[2], hi_p = wb[3];
-  int wd = hi_d - lo_d + 1, wp = hi_p - lo_p + 1;
-  bool use_lds = (lo_p != 0x7fffffff) && (long long)wd * wp <= WIN_MAX;
-  double *img = fit + F.off;
-  int ndop = V.ndop;
-  double *ov = overflow + (long long)blockIdx.y * 2000;
-  if (use_lds) {
-    for (int q = threadIdx.x; q < wd * wp; q += blockDim.x) win[q] = 0.0;
-    __syncthreads();
-    if (scat) {
-      int np = R.ip1 - R.ip0 + 1, nd = dd ? R.id1 - R.id0 + 1 : 1;
-#pragma unroll
-      for (int a = 0; a < DW; a++) {
-        if (a < nd) {
-          float ad = dd ? R.amp * R.dw[a] : R.amp;
-          double *row = win + (dd ? (R.id0 + a - lo_d) * wp : 0) + (R.ip0 - lo_p);
-#pragma unroll
-          for (int b = 0; b < PW; b++)
-            if (b < np) atomicAdd(&row[b], (double)(ad * R.pw[b]));
-        }
-      }
-    }
-    __syncthreads();
-    for (int q = threadIdx.x; q < wd * wp; q += blockDim.x) {
-      double v = win[q];
-      if (v != 0.0) {
-        int r = q / wp, c = q % wp;
-        int idel = V.D0 + lo_d + r, idop = V.P0 + lo_p + c;
-        if (dd) atomicAdd(&img[(long long)(idel - 1) * ndop + (idop - 1)], v);
-        else atomicAdd(&img[idop - 1], v);
-      }
-    }
-    if (lit && !R.inb && !dd) radar_scatter_global(V, R, img, ov);
-  } else if (lit) {
-    radar_scatter_global(V, R, img, ov);
-  }
 }
 
 __device__ void replay32_one(ViewRec *views, const GeomSlot *slots, const PosDev &pd, double *fit,
