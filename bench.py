@@ -54,7 +54,9 @@ def parse():
     ap.add_argument("--frames", type=int, default=NFRAMES_FULL, help="delay-Doppler frames (256: full workload)")
     ap.add_argument("--fp64-steps", type=int, default=3)
     ap.add_argument("--kernel-steps", type=int, default=5, help="eager replays timed per kernel")
-    ap.add_argument("--cpu-frames", type=int, default=64, help="frames in the CPU-baseline sample")
+    ap.add_argument("--cpu-frames", type=int, default=NFRAMES_FULL, help="frames in the CPU-baseline sample")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="only copy 1 GiB device to device (calibrates HBM byte counters), then exit")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--mixed-only", action="store_true",
                     help="time only the mixed replays (for profiler runs)")
@@ -201,9 +203,40 @@ def cpu_baseline(nframes):
                       "%d-frame evaluation" % (nframes, NFRAMES_FULL, threads, dt, NFRAMES_FULL)}
 
 
+def copy_probe():
+    """One plain 1 GiB device-to-device copy: a kernel of known traffic, read
+    by the same FETCH_SIZE / WRITE_SIZE counters as the evaluation kernels."""
+    import torch
+    n = 1 << 30
+    src = torch.ones(n // 4, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes": n}), flush=True)
+
+
+def measured_traffic(kernel, workload):
+    """HBM bytes per launch of `kernel` on `workload`, from the counter runs
+    summarized in profiles/hbm_traffic.json (FETCH_SIZE + WRITE_SIZE, scaled by
+    the copy probe); None when no summary for this kernel and workload exists."""
+    path = os.path.join(ROOT, "profiles", "hbm_traffic.json")
+    try:
+        rec = json.load(open(path))
+    except (OSError, ValueError):
+        return None
+    k = rec.get("kernels", {}).get(kernel)
+    if not k or rec.get("workload") != workload:
+        return None
+    return k.get("bytes_per_launch")
+
+
 def main():
     global KSTEPS
     a = parse()
+    if a.copy_probe:
+        copy_probe()
+        return
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if a.gpus != 1 or world != 1:
         print("bench.py measures one GPU: this build runs the evaluation path on a single MI355X "
@@ -263,9 +296,11 @@ def main():
     dom, rows, fp, claimed = kernel_roofline(sc, a.evals)
     r = rows[dom]
     ach = r["GBps"]
+    wl = "config4/frames=%d/evals=%d" % (a.frames, a.evals)
     out["roofline"] = {"kernel": dom, "bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                        "frac": None if ach is None else round(ach / HBM_PEAK_GBS, 4),
-                       "traffic": None, "avg_ms": r["avg_ms"], "algorithmic_bytes": r["bytes"]}
+                       "traffic": measured_traffic(dom, wl), "avg_ms": r["avg_ms"],
+                       "algorithmic_bytes": r["bytes"]}
     out["kernels"] = rows
     out["footprint"] = fp
     sc.close()
