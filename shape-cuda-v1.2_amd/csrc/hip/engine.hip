@@ -695,8 +695,11 @@ int Engine::enqueue() {
     SHP_LAUNCH(*this, "k_photo64", k_photo64, gp, bt, 0, stream, d_views, pd, d_params);
   }
   SHP_LAUNCH(*this, "k_mark_seen", k_mark_seen, gp, bt, 0, stream, d_views, d_slots, pd, mincosine_seen);
-  if (any_dd) SHP_LAUNCH(*this, "k_deldop64", k_deldop64, gp, bt, 0, stream, d_views, d_frames, pd, d_fit, d_params);
-  if (any_dop) SHP_LAUNCH(*this, "k_doppler64", k_doppler64, gp, bt, 0, stream, d_views, d_frames, pd, d_fit, d_overflow, d_params);
+  int max_n = 0;
+  for (auto &V : h_views) max_n = std::max(max_n, V.n);
+  dim3 gs(syn_blocks(max_n), nviews);
+  if (any_dd) SHP_LAUNCH(*this, "k_deldop64", k_deldop64, gs, bt, 0, stream, d_views, d_frames, pd, d_fit, d_params);
+  if (any_dop) SHP_LAUNCH(*this, "k_doppler64", k_doppler64, gs, bt, 0, stream, d_views, d_frames, pd, d_fit, d_overflow, d_params);
   SHP_LAUNCH(*this, "k_finish_views", k_finish_views, gv, bv, 0, stream, d_views, nviews, d_overflow, d_frames, d_view_out);
   if (any_lc)
     SHP_LAUNCH(*this, "k_spline", k_spline, dim3(nframes), bt, 0, stream, d_frames, nframes, d_view_out, d_views, d_lc_x,

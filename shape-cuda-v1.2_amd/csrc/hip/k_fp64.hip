@@ -612,10 +612,62 @@ __global__ void k_photo64(ViewRec *views, PosDev pd, const double *params) {
 }
 
 // ---------------------------------------------------------------------------
-// delay-Doppler synthesis (pos2deldop.c:305-535), one thread per POS pixel,
-// FP64 atomics into the fit image
+// delay-Doppler and Doppler synthesis (pos2deldop.c:305-535,
+// pos2doppler.c:244-439) in FP64: one workgroup per SYN_TILE x SYN_TILE tile
+// of a view's POS, one pixel per thread.  The tile's in-frame footprints are
+// accumulated in an FP64 LDS window bounded exactly by the tile's own delay
+// rows and Doppler columns, then flushed once to the fit image.
 // ---------------------------------------------------------------------------
-constexpr int DELMAX = 8, DOPMAX = 32;
+constexpr int DELMAX = 4, DOPMAX = 16;  // register footprint; wider ones take the direct path
+constexpr int WIN64 = 2048;             // LDS window (doubles)
+
+// block-wide minimum of four ints (one barrier pair); sh holds 4 per wave
+__device__ inline void blk_min4_64(int &a, int &b, int &c, int &d, int (*sh)[4]) {
+  a = wave_min_i(a); b = wave_min_i(b); c = wave_min_i(c); d = wave_min_i(d);
+  int wv = threadIdx.x >> 6;
+  if (lane0()) { sh[wv][0] = a; sh[wv][1] = b; sh[wv][2] = c; sh[wv][3] = d; }
+  __syncthreads();
+  a = sh[0][0]; b = sh[0][1]; c = sh[0][2]; d = sh[0][3];
+  for (int q = 1; q < (int)(blockDim.x >> 6); q++) {
+    a = min(a, sh[q][0]); b = min(b, sh[q][1]); c = min(c, sh[q][2]); d = min(d, sh[q][3]);
+  }
+  __syncthreads();
+}
+
+// the view's delay-Doppler limits, once per workgroup; limits only move
+// outward, so updates a read of the current value shows useless are skipped
+__device__ inline void limits_block64(ViewRec &V, bool dd, int il0, int il1, int jl0, int jl1, double dl0, double dl1,
+                                      double pl0, double pl1, int ovf, int (*shi)[8], double (*shd)[4]) {
+  il0 = wave_min_i(il0); il1 = wave_max_i(il1); jl0 = wave_min_i(jl0); jl1 = wave_max_i(jl1);
+  dl0 = wave_min_d(dl0); dl1 = wave_max_d(dl1); pl0 = wave_min_d(pl0); pl1 = wave_max_d(pl1);
+  ovf = wave_max_i(ovf);
+  int wv = threadIdx.x >> 6;
+  if (lane0()) {
+    shi[wv][0] = il0; shi[wv][1] = il1; shi[wv][2] = jl0; shi[wv][3] = jl1; shi[wv][4] = ovf;
+    shd[wv][0] = dl0; shd[wv][1] = dl1; shd[wv][2] = pl0; shd[wv][3] = pl1;
+  }
+  __syncthreads();
+  if (threadIdx.x != 0) return;
+  for (int q = 1; q < (int)(blockDim.x >> 6); q++) {
+    il0 = min(il0, shi[q][0]); il1 = max(il1, shi[q][1]); jl0 = min(jl0, shi[q][2]); jl1 = max(jl1, shi[q][3]);
+    ovf |= shi[q][4];
+    dl0 = dmin(dl0, shd[q][0]); dl1 = dmax(dl1, shd[q][1]); pl0 = dmin(pl0, shd[q][2]); pl1 = dmax(pl1, shd[q][3]);
+  }
+  if (jl0 == 0x7fffffff) return;
+  if (dd) {
+    if (il0 < V.idellim[0]) atomicMin(&V.idellim[0], il0);
+    if (il1 > V.idellim[1]) atomicMax(&V.idellim[1], il1);
+    unsigned long long k0 = ord64(dl0), k1 = ord64(-dl1);
+    if (k0 < V.dellim[0]) atomicMin(&V.dellim[0], k0);
+    if (k1 < V.dellim[1]) atomicMin(&V.dellim[1], k1);
+  }
+  if (jl0 < V.idoplim[0]) atomicMin(&V.idoplim[0], jl0);
+  if (jl1 > V.idoplim[1]) atomicMax(&V.idoplim[1], jl1);
+  unsigned long long k2 = ord64(pl0), k3 = ord64(-pl1);
+  if (k2 < V.doplim[0]) atomicMin(&V.doplim[0], k2);
+  if (k3 < V.doplim[1]) atomicMin(&V.doplim[1], k3);
+  if (ovf && !V.any_overflow) atomicOr(&V.any_overflow, 1);
+}
 
 __device__ inline double sinc2_mean64(const ViewRec &V, double dopPOS, int idop) {
   int ns = V.nsinc2;
@@ -687,125 +739,177 @@ __global__ void k_radlaw_one(const double *pl, int type, double c, double *out) 
   if (threadIdx.x == 0 && blockIdx.x == 0) *out = radlaw64(pl, type, c);
 }
 
-__global__ void k_deldop64(ViewRec *views, FrameRec *frames, PosDev pd, double *fit, const double *params) {
+__global__ __launch_bounds__(256) void k_deldop64(ViewRec *views, FrameRec *frames, PosDev pd, double *fit,
+                                                  const double *params) {
   ViewRec &V = views[blockIdx.y];
-  int n = V.n, w = 2 * n + 1;
-  int pix = blockIdx.x * blockDim.x + threadIdx.x;
-  int il0 = 0x7fffffff, il1 = -0x7fffffff, jl0 = 0x7fffffff, jl1 = -0x7fffffff;
-  double dl0 = 1.0e20, dl1 = -1.0e20, pl0 = 1.0e20, pl1 = -1.0e20;
-  if (pix < w * w && V.type == VT_DELAY) {
-    int x = pix / w - n, y = pix % w - n;
-    long long p = V.pos_off + pix;
-    if (x >= V.xlim[0] && x <= V.xlim[1] && y >= V.ylim[0] && y <= V.ylim[1] && pd.cose[p] > 0.0 &&
-        (pd.body ? pd.body[p] == V.body : true) && pd.z[p] != -1e20) {
-      const FrameRec &F = frames[V.frame_id];
-      int ndel = V.ndel, ndop = V.ndop, width = V.sinc2width;
-      double delPOS = pd.z[p] * V.delfact + V.delshift;
-      int idel_min = (int)floor(delPOS - V.const1) + 1;
-      int idel_max = (int)ceil(delPOS + V.const1) - 1;
-      double dopPOS = V.ax * (x - V.orb_x) + V.ay * (y - V.orb_y) + V.dopshift;
-      int idop_min = (int)floor(dopPOS - V.dopdiff_max + 1 - width / 2.0);
-      int idop_max = (int)floor(dopPOS + V.dopdiff_max + width / 2.0);
-      double codefactor = (V.codemethod == 0) ? sinc2d((dopPOS - V.dopDC_vig) / V.dopfftlen) : 1.0;
-      il0 = idel_min; il1 = idel_max; jl0 = idop_min; jl1 = idop_max;
-      dl0 = dl1 = delPOS; pl0 = pl1 = dopPOS;
-      bool in_bounds = (idel_min >= 1) && (idel_max <= ndel) && (idop_min >= 1) && (idop_max <= ndop);
-      int nd = idel_max - idel_min + 1, np = idop_max - idop_min + 1;
-      if (in_bounds && nd <= DELMAX && np <= DOPMAX) {
-        double dc[DELMAX], pc[DOPMAX];
-        for (int a = 0; a < nd; a++) dc[a] = delay_weight64(V, delPOS, idel_min + a);
-        for (int b = 0; b < np; b++) pc[b] = sinc2_mean64(V, dopPOS, idop_min + b);
-        double sumw = 0.0;
-        for (int a = 0; a < nd; a++)
-          for (int b = 0; b < np; b++) sumw += dc[a] * pc[b];
-        const double *pl = params + V.pb + V.pb_law;
-        double amp = radlaw64(pl, V.law_type, pd.cose[p]) * V.kmpp * V.kmpp * codefactor / sumw;
-        double *img = fit + F.off;
-        for (int a = 0; a < nd; a++)
-          for (int b = 0; b < np; b++)
-            atomicAdd(&img[(long long)(idel_min + a - 1) * ndop + (idop_min + b - 1)], amp * dc[a] * pc[b]);
-      } else if (in_bounds) {
-        // wide footprints: recompute weights instead of holding them in registers
-        double sumw = 0.0;
-        for (int a = idel_min; a <= idel_max; a++) {
-          double d = delay_weight64(V, delPOS, a);
-          for (int b = idop_min; b <= idop_max; b++) sumw += d * sinc2_mean64(V, dopPOS, b);
-        }
-        const double *pl = params + V.pb + V.pb_law;
-        double amp = radlaw64(pl, V.law_type, pd.cose[p]) * V.kmpp * V.kmpp * codefactor / sumw;
-        double *img = fit + F.off;
-        for (int a = idel_min; a <= idel_max; a++) {
-          double d = delay_weight64(V, delPOS, a);
-          for (int b = idop_min; b <= idop_max; b++)
-            atomicAdd(&img[(long long)(a - 1) * ndop + (b - 1)], amp * d * sinc2_mean64(V, dopPOS, b));
-        }
+  if (V.type != VT_DELAY) return;
+  const int n = V.n, w = 2 * n + 1, tx = (w + SYN_TILE - 1) / SYN_TILE;
+  const int ti = blockIdx.x / tx, tj = blockIdx.x % tx;
+  if (ti >= tx) return;
+  const int i0 = ti * SYN_TILE - n, j0 = tj * SYN_TILE - n;
+  if (i0 > V.xlim[1] || i0 + SYN_TILE - 1 < V.xlim[0] || j0 > V.ylim[1] || j0 + SYN_TILE - 1 < V.ylim[0]) return;
+  __shared__ double win[WIN64];
+  __shared__ int shw[4][4];
+  __shared__ int shi[4][8];
+  __shared__ double shd[4][4];
+  const int x = i0 + (int)threadIdx.x / SYN_TILE, y = j0 + (int)threadIdx.x % SYN_TILE;
+  const long long p = V.pos_off + (long long)(x + n) * w + (y + n);
+  const int ndel = V.ndel, ndop = V.ndop, width = V.sinc2width;
+  bool valid = x <= n && y <= n && x >= V.xlim[0] && x <= V.xlim[1] && y >= V.ylim[0] && y <= V.ylim[1] &&
+               pd.cose[p] > 0.0 && (pd.body ? pd.body[p] == V.body : true) && pd.z[p] != -1e20;
+  double delPOS = 0.0, dopPOS = 0.0;
+  int idel_min = 0, idel_max = -1, idop_min = 0, idop_max = -1;
+  if (valid) {
+    delPOS = pd.z[p] * V.delfact + V.delshift;
+    idel_min = (int)floor(delPOS - V.const1) + 1;
+    idel_max = (int)ceil(delPOS + V.const1) - 1;
+    dopPOS = V.ax * (x - V.orb_x) + V.ay * (y - V.orb_y) + V.dopshift;
+    idop_min = (int)floor(dopPOS - V.dopdiff_max + 1 - width / 2.0);
+    idop_max = (int)floor(dopPOS + V.dopdiff_max + width / 2.0);
+  }
+  const bool in_bounds = valid && (idel_min >= 1) && (idel_max <= ndel) && (idop_min >= 1) && (idop_max <= ndop);
+  const int nd = idel_max - idel_min + 1, np = idop_max - idop_min + 1;
+  const bool small = in_bounds && nd <= DELMAX && np <= DOPMAX;
+  // window over the tile's in-frame register footprints
+  int lo_d = small ? idel_min : 0x7fffffff, nhi_d = small ? -idel_max : 0x7fffffff;
+  int lo_p = small ? idop_min : 0x7fffffff, nhi_p = small ? -idop_max : 0x7fffffff;
+  blk_min4_64(lo_d, nhi_d, lo_p, nhi_p, shw);
+  const int wd = -nhi_d - lo_d + 1, wp = -nhi_p - lo_p + 1;
+  const bool use_lds = lo_d != 0x7fffffff && (long long)wd * wp <= WIN64;
+  if (use_lds) {
+    for (int q = threadIdx.x; q < wd * wp; q += blockDim.x) win[q] = 0.0;
+    __syncthreads();
+  }
+  double *img = fit + frames[V.frame_id].off;
+  if (small) {
+    double codefactor = (V.codemethod == 0) ? sinc2d((dopPOS - V.dopDC_vig) / V.dopfftlen) : 1.0;
+    double dc[DELMAX], pc[DOPMAX];
+#pragma unroll
+    for (int a = 0; a < DELMAX; a++) dc[a] = a < nd ? delay_weight64(V, delPOS, idel_min + a) : 0.0;
+#pragma unroll
+    for (int b = 0; b < DOPMAX; b++) pc[b] = b < np ? sinc2_mean64(V, dopPOS, idop_min + b) : 0.0;
+    double sumw = 0.0;
+#pragma unroll
+    for (int a = 0; a < DELMAX; a++)
+#pragma unroll
+      for (int b = 0; b < DOPMAX; b++)
+        if (a < nd && b < np) sumw += dc[a] * pc[b];
+    const double *pl = params + V.pb + V.pb_law;
+    double amp = radlaw64(pl, V.law_type, pd.cose[p]) * V.kmpp * V.kmpp * codefactor / sumw;
+#pragma unroll
+    for (int a = 0; a < DELMAX; a++) {
+      if (a < nd) {
+        double ad = amp * dc[a];
+        double *row = use_lds ? win + (long long)(idel_min + a - lo_d) * wp + (idop_min - lo_p)
+                              : img + (long long)(idel_min + a - 1) * ndop + (idop_min - 1);
+#pragma unroll
+        for (int b = 0; b < DOPMAX; b++)
+          if (b < np) atomicAdd(&row[b], ad * pc[b]);
       }
     }
+  } else if (in_bounds) {
+    // wide footprints: recompute weights instead of holding them in registers
+    double codefactor = (V.codemethod == 0) ? sinc2d((dopPOS - V.dopDC_vig) / V.dopfftlen) : 1.0;
+    double sumw = 0.0;
+    for (int a = idel_min; a <= idel_max; a++) {
+      double d = delay_weight64(V, delPOS, a);
+      for (int b = idop_min; b <= idop_max; b++) sumw += d * sinc2_mean64(V, dopPOS, b);
+    }
+    const double *pl = params + V.pb + V.pb_law;
+    double amp = radlaw64(pl, V.law_type, pd.cose[p]) * V.kmpp * V.kmpp * codefactor / sumw;
+    for (int a = idel_min; a <= idel_max; a++) {
+      double d = delay_weight64(V, delPOS, a);
+      for (int b = idop_min; b <= idop_max; b++)
+        atomicAdd(&img[(long long)(a - 1) * ndop + (b - 1)], amp * d * sinc2_mean64(V, dopPOS, b));
+    }
   }
-  il0 = wave_min_i(il0); il1 = wave_max_i(il1); jl0 = wave_min_i(jl0); jl1 = wave_max_i(jl1);
-  dl0 = wave_min_d(dl0); dl1 = wave_max_d(dl1); pl0 = wave_min_d(pl0); pl1 = wave_max_d(pl1);
-  if (lane0() && il0 != 0x7fffffff) {
-    atomicMin(&V.idellim[0], il0); atomicMax(&V.idellim[1], il1);
-    atomicMin(&V.idoplim[0], jl0); atomicMax(&V.idoplim[1], jl1);
-    atomic_min_d(&V.dellim[0], dl0); atomic_min_d(&V.dellim[1], -dl1);
-    atomic_min_d(&V.doplim[0], pl0); atomic_min_d(&V.doplim[1], -pl1);
+  if (use_lds) {
+    __syncthreads();
+    for (int q = threadIdx.x; q < wd * wp; q += blockDim.x) {
+      double v = win[q];
+      if (v != 0.0) atomicAdd(&img[(long long)(lo_d + q / wp - 1) * ndop + (lo_p + q % wp - 1)], v);
+    }
   }
+  limits_block64(V, true, valid ? idel_min : 0x7fffffff, valid ? idel_max : -0x7fffffff,
+                 valid ? idop_min : 0x7fffffff, valid ? idop_max : -0x7fffffff, valid ? delPOS : 1.0e20,
+                 valid ? delPOS : -1.0e20, valid ? dopPOS : 1.0e20, valid ? dopPOS : -1.0e20, 0, shi, shd);
 }
 
 // Doppler synthesis with the overflow spectrum (pos2doppler.c:244-439)
-__global__ void k_doppler64(ViewRec *views, FrameRec *frames, PosDev pd, double *fit, double *overflow,
-                            const double *params) {
+__global__ __launch_bounds__(256) void k_doppler64(ViewRec *views, FrameRec *frames, PosDev pd, double *fit,
+                                                   double *overflow, const double *params) {
   ViewRec &V = views[blockIdx.y];
-  int n = V.n, w = 2 * n + 1;
-  int pix = blockIdx.x * blockDim.x + threadIdx.x;
-  int jl0 = 0x7fffffff, jl1 = -0x7fffffff, ovf = 0;
-  double pl0 = 1.0e20, pl1 = -1.0e20;
-  if (pix < w * w && V.type == VT_DOPPLER) {
-    int x = pix / w - n, y = pix % w - n;
-    long long p = V.pos_off + pix;
-    if (x >= V.xlim[0] && x <= V.xlim[1] && y >= V.ylim[0] && y <= V.ylim[1] && pd.cose[p] > 0.0 &&
-        (pd.body ? pd.body[p] == V.body : true)) {
-      const FrameRec &F = frames[V.frame_id];
-      int ndop = V.ndop, width = V.sinc2width;
-      double dopPOS = V.ax * (x - V.orb_x) + V.ay * (y - V.orb_y) + V.dopshift;
-      int idop_min = (int)floor(dopPOS - V.dopdiff_max + 1 - width / 2.0);
-      int idop_max = (int)floor(dopPOS + V.dopdiff_max + width / 2.0);
-      jl0 = idop_min; jl1 = idop_max;
-      pl0 = pl1 = dopPOS;
-      bool in_bounds = (idop_min >= 1) && (idop_max <= ndop);
-      int np = idop_max - idop_min + 1;
-      double sumw = 0.0;
-      double pc[DOPMAX];
-      for (int b = 0; b < np; b++) {
-        double v = sinc2_mean64(V, dopPOS, idop_min + b);
-        if (b < DOPMAX) pc[b] = v;
-        sumw += v;
-      }
-      const double *pl = params + V.pb + V.pb_law;
-      double amp = radlaw64(pl, V.law_type, pd.cose[p]) * V.kmpp * V.kmpp / sumw;
-      if (in_bounds) {
-        double *img = fit + F.off;
-        for (int b = 0; b < np; b++)
-          atomicAdd(&img[idop_min + b - 1], amp * (b < DOPMAX ? pc[b] : sinc2_mean64(V, dopPOS, idop_min + b)));
-      } else {
-        ovf = 1;
-        int idop0 = V.idop0;
-        int i1 = max(idop_min, -idop0), i2 = min(idop_max, -idop0 + 2000 - 1);
-        double *ov = overflow + (long long)blockIdx.y * 2000;
-        for (int b = i1; b <= i2; b++) {
-          int kk = b - idop_min;
-          atomicAdd(&ov[b + idop0], amp * (kk < DOPMAX ? pc[kk] : sinc2_mean64(V, dopPOS, b)));
-        }
-      }
+  if (V.type != VT_DOPPLER) return;
+  const int n = V.n, w = 2 * n + 1, tx = (w + SYN_TILE - 1) / SYN_TILE;
+  const int ti = blockIdx.x / tx, tj = blockIdx.x % tx;
+  if (ti >= tx) return;
+  const int i0 = ti * SYN_TILE - n, j0 = tj * SYN_TILE - n;
+  if (i0 > V.xlim[1] || i0 + SYN_TILE - 1 < V.xlim[0] || j0 > V.ylim[1] || j0 + SYN_TILE - 1 < V.ylim[0]) return;
+  __shared__ double win[WIN64];
+  __shared__ int shw[4][4];
+  __shared__ int shi[4][8];
+  __shared__ double shd[4][4];
+  const int x = i0 + (int)threadIdx.x / SYN_TILE, y = j0 + (int)threadIdx.x % SYN_TILE;
+  const long long p = V.pos_off + (long long)(x + n) * w + (y + n);
+  const int ndop = V.ndop, width = V.sinc2width;
+  bool valid = x <= n && y <= n && x >= V.xlim[0] && x <= V.xlim[1] && y >= V.ylim[0] && y <= V.ylim[1] &&
+               pd.cose[p] > 0.0 && (pd.body ? pd.body[p] == V.body : true);
+  double dopPOS = 0.0;
+  int idop_min = 0, idop_max = -1;
+  if (valid) {
+    dopPOS = V.ax * (x - V.orb_x) + V.ay * (y - V.orb_y) + V.dopshift;
+    idop_min = (int)floor(dopPOS - V.dopdiff_max + 1 - width / 2.0);
+    idop_max = (int)floor(dopPOS + V.dopdiff_max + width / 2.0);
+  }
+  const bool in_bounds = valid && (idop_min >= 1) && (idop_max <= ndop);
+  const int np = idop_max - idop_min + 1;
+  const bool small = in_bounds && np <= DOPMAX;
+  int lo_p = small ? idop_min : 0x7fffffff, nhi_p = small ? -idop_max : 0x7fffffff, u0 = 0, u1 = 0;
+  blk_min4_64(lo_p, nhi_p, u0, u1, shw);
+  const int wp = -nhi_p - lo_p + 1;
+  const bool use_lds = lo_p != 0x7fffffff && wp <= WIN64;
+  if (use_lds) {
+    for (int q = threadIdx.x; q < wp; q += blockDim.x) win[q] = 0.0;
+    __syncthreads();
+  }
+  double *img = fit + frames[V.frame_id].off;
+  if (valid) {
+    // weights in bin order, as the CPU sums them; held in registers for the
+    // first DOPMAX bins, recomputed (same value) beyond
+    double sumw = 0.0;
+    double pc[DOPMAX];
+#pragma unroll
+    for (int b = 0; b < DOPMAX; b++) {
+      double v = b < np ? sinc2_mean64(V, dopPOS, idop_min + b) : 0.0;
+      pc[b] = v;
+      if (b < np) sumw += v;
+    }
+    for (int b = DOPMAX; b < np; b++) sumw += sinc2_mean64(V, dopPOS, idop_min + b);
+    const double *pl = params + V.pb + V.pb_law;
+    double amp = radlaw64(pl, V.law_type, pd.cose[p]) * V.kmpp * V.kmpp / sumw;
+    if (small) {
+      double *row = use_lds ? win + (idop_min - lo_p) : img + (idop_min - 1);
+#pragma unroll
+      for (int b = 0; b < DOPMAX; b++)
+        if (b < np) atomicAdd(&row[b], amp * pc[b]);
+    } else if (in_bounds) {
+      for (int b = 0; b < np; b++) atomicAdd(&img[idop_min + b - 1], amp * sinc2_mean64(V, dopPOS, idop_min + b));
+    } else {
+      int idop0 = V.idop0;
+      int i1 = max(idop_min, -idop0), i2 = min(idop_max, -idop0 + 2000 - 1);
+      double *ov = overflow + (long long)blockIdx.y * 2000;
+      for (int b = i1; b <= i2; b++) atomicAdd(&ov[b + idop0], amp * sinc2_mean64(V, dopPOS, b));
     }
   }
-  jl0 = wave_min_i(jl0); jl1 = wave_max_i(jl1); ovf = wave_max_i(ovf);
-  pl0 = wave_min_d(pl0); pl1 = wave_max_d(pl1);
-  if (lane0() && jl0 != 0x7fffffff) {
-    atomicMin(&V.idoplim[0], jl0); atomicMax(&V.idoplim[1], jl1);
-    atomic_min_d(&V.doplim[0], pl0); atomic_min_d(&V.doplim[1], -pl1);
-    if (ovf) atomicOr(&V.any_overflow, 1);
+  if (use_lds) {
+    __syncthreads();
+    for (int q = threadIdx.x; q < wp; q += blockDim.x) {
+      double v = win[q];
+      if (v != 0.0) atomicAdd(&img[lo_p + q - 1], v);
+    }
   }
+  limits_block64(V, false, 0, 0, valid ? idop_min : 0x7fffffff, valid ? idop_max : -0x7fffffff, 0.0, 0.0,
+                 valid ? dopPOS : 1.0e20, valid ? dopPOS : -1.0e20, valid && !in_bounds ? 1 : 0, shi, shd);
 }
 
 // ---------------------------------------------------------------------------

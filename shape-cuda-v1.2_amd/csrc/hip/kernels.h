@@ -10,6 +10,8 @@ namespace shp {
 
 constexpr int STALE_CAND = 64;  // max facets recorded per replayed pixel
 constexpr int STALE_TILE = 8;   // stale-pixel bitmap granularity (pixels)
+constexpr int SYN_TILE = 16;    // FP64 synthesis: 16 x 16 POS pixels per workgroup
+inline unsigned syn_blocks(int n) { unsigned t = (2 * n + 1 + SYN_TILE - 1) / SYN_TILE; return t * t; }
 
 // A pixel whose final owner has a back-facing smoothed normal; its
 // cos(scattering) is whatever an earlier claimant left (posvis.c:243-275).

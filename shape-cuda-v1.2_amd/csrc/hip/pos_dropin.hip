@@ -386,9 +386,9 @@ int radar_view(struct par_t *par, struct photo_t *photo, double orbit_xoff, doub
   hipLaunchKernelGGL(k_fill_f64, dim3(8), dim3(256), 0, 0, dov, 2000LL, 0.0);
   hipLaunchKernelGGL(k_setup_views, dim3(1), dim3(64), 0, 0, dV, 1, (const double *)dpb);
   if (is_dd)
-    hipLaunchKernelGGL(k_deldop64, dim3(cdiv(P, 256), 1), dim3(256), 0, 0, dV, dF, pd, dfit, (const double *)dpb);
+    hipLaunchKernelGGL(k_deldop64, dim3(syn_blocks(n), 1), dim3(256), 0, 0, dV, dF, pd, dfit, (const double *)dpb);
   else
-    hipLaunchKernelGGL(k_doppler64, dim3(cdiv(P, 256), 1), dim3(256), 0, 0, dV, dF, pd, dfit, dov,
+    hipLaunchKernelGGL(k_doppler64, dim3(syn_blocks(n), 1), dim3(256), 0, 0, dV, dF, pd, dfit, dov,
                        (const double *)dpb);
   hipLaunchKernelGGL(k_finish_views, dim3(1), dim3(64), 0, 0, dV, 1, dov, (const FrameRec *)dF, vout);
   double out[VIEW_OUT];
