@@ -147,7 +147,7 @@ extern "C" double chi2(struct par_t *par, struct dat_t *dat, int list_breakdown)
   FrameRec *df = (FrameRec *)dfr.p;
   hipLaunchKernelGGL(k_frame_o2, dim3(std::max(1, nf)), dim3(256), 0, 0, df, nf, (double *)dobs.p, (double *)doov.p);
   hipLaunchKernelGGL(k_chi2_frames64, dim3(std::max(1, nf)), dim3(256), 0, 0, df, nf, (double *)dfit.p,
-                     (double *)dobs.p, (double *)doov.p, (const double *)nullptr, 0, 1);
+                     (double *)dobs.p, (double *)doov.p, (const double *)nullptr, 0, 1, 0);
   hipLaunchKernelGGL(k_chi2_sets, dim3(1), dim3(64), 0, 0, (const FrameRec *)df, nf, (double *)dset.p, nsets,
                      (double *)dtot.p);
   std::vector<double> setc(nsets);

@@ -79,6 +79,7 @@ struct ViewRec {
   // Zero in the engine's own views.
   double orb[3], orb_x, orb_y, orb_dop;
   int body, keep_rect, rect_by_claim;
+  int fp64_skip;         // mixed method: radar view, skipped by the FP64 kernels
   long long pos_off;     // pixel offset of this view's POS buffers
   long long tile_off;    // offset of this view's stale-tile bitmap (mixed method)
   int pb;                // parameter block of the evaluation

@@ -472,6 +472,7 @@ int Engine::build(struct par_t *par, struct mod_t **mods, struct dat_t **dats, i
     int view_base = (int)h_views.size(), frame_base = (int)h_frames.size();
     for (size_t q = 0; q < P.views.size(); q++) {
       ViewRec V = P.views[q];
+      V.fp64_skip = (opt.method == METHOD_MIXED && V.type != VT_LGHTCRV) ? 1 : 0;
       V.eval = e;
       V.slot = h_eval_slot[e];
       V.pb = pb;
@@ -705,7 +706,7 @@ int Engine::enqueue() {
     SHP_LAUNCH(*this, "k_spline", k_spline, dim3(nframes), bt, 0, stream, d_frames, nframes, d_view_out, d_views, d_lc_x,
                        d_lc_t, d_lc_y, d_lc_y2, d_fit);
   SHP_LAUNCH(*this, "k_chi2_frames64", k_chi2_frames64, dim3(nframes), bt, 0, stream, d_frames, nframes, d_fit, d_obs, d_oov,
-                     d_view_out, opt.keep_fit ? 0 : 1, 0);
+                     d_view_out, opt.keep_fit ? 0 : 1, 0, 0);
   SHP_LAUNCH(*this, "k_objective", k_objective, dim3(cdiv(nevals, 64)), dim3(64), 0, stream, d_evals, nevals, d_frames, d_views,
                      d_view_out, d_slot_flags, d_slot_logf, d_eval_slot);
   SHP_CHECK(hipGetLastError());

@@ -306,6 +306,7 @@ __global__ void k_realize_vnormals(GeomSlot *slots, int nslot) {
 // ---------------------------------------------------------------------------
 __global__ void k_raster64(ViewRec *views, const GeomSlot *slots, PosDev pd, int src, int pass) {
   ViewRec &V = views[blockIdx.y];
+  if (V.fp64_skip) return;
   const GeomSlot &G = slots[V.slot];
   int f = blockIdx.x * blockDim.x + threadIdx.x;
   int n = V.n, w = 2 * n + 1;
@@ -381,6 +382,7 @@ __global__ void k_raster64(ViewRec *views, const GeomSlot *slots, PosDev pd, int
 __global__ void k_resolve64(ViewRec *views, const GeomSlot *slots, PosDev pd, int src, int merge, int blank,
                             int body, int comp) {
   ViewRec &V = views[blockIdx.y];
+  if (V.fp64_skip) return;
   int n = V.n, w = 2 * n + 1;
   int pix = blockIdx.x * blockDim.x + threadIdx.x;
   if (pix >= w * w) return;
@@ -518,6 +520,7 @@ __device__ inline int fill_at(const ViewRec &V, const PosDev &pd, int i, int j) 
 
 __global__ void k_posmask64(ViewRec *views, PosDev pd, double tol) {
   const ViewRec &V = views[blockIdx.y];
+  if (V.fp64_skip) return;
   int n = V.n, w = 2 * n + 1;
   int pix = blockIdx.x * blockDim.x + threadIdx.x;
   if (pix >= w * w) return;
@@ -576,6 +579,7 @@ __global__ void k_posmask64(ViewRec *views, PosDev pd, double tol) {
 // ---------------------------------------------------------------------------
 __global__ void k_photo64(ViewRec *views, PosDev pd, const double *params) {
   ViewRec &V = views[blockIdx.y];
+  if (V.fp64_skip) return;
   int n = V.n, w = 2 * n + 1;
   int pix = blockIdx.x * blockDim.x + threadIdx.x;
   double b = 0.0;
@@ -1044,10 +1048,11 @@ __device__ inline void neumaier(double &s, double &c, double x) {
 }
 
 __global__ void k_chi2_frames64(FrameRec *frames, int nframes, double *fit, const double *obs, const double *oov,
-                                const double *view_out, int clear_fit, int ovf_from_host) {
+                                const double *view_out, int clear_fit, int ovf_from_host, int lc_only) {
   int fi = blockIdx.x;
   if (fi >= nframes) return;
   FrameRec &F = frames[fi];
+  if (lc_only && F.type != VT_LGHTCRV) return;  // mixed method: radar frames have their own kernel
   long long N = (F.type == VT_DELAY) ? (long long)F.ndel * F.ndop : (F.type == VT_DOPPLER ? F.ndop : F.lc_n);
   double *ft = fit + F.off;
   const double *ob = obs + F.obs_off, *w = oov + F.obs_off;
@@ -1160,6 +1165,7 @@ namespace shp {
 // facets seen from Earth (calc_fits.c:522-532, :903-913, :1699-1710)
 __global__ void k_mark_seen(ViewRec *views, GeomSlot *slots, PosDev pd, double mincos) {
   const ViewRec &V = views[blockIdx.y];
+  if (V.fp64_skip) return;
   if (V.exclude_seen || !V.is_v0) return;
   int n = V.n, w = 2 * n + 1;
   int pix = blockIdx.x * blockDim.x + threadIdx.x;

@@ -40,6 +40,7 @@ __global__ void k_replay32(ViewRec *views, const GeomSlot *slots, PosDev pd, dou
 __global__ __launch_bounds__(256) void k_raster32(ViewRec *views, const GeomSlot *slots, PosDev pd,
                                                   long long sun_off, int src, int pass) {
   ViewRec &V = views[blockIdx.y];
+  if (V.type != VT_DELAY && V.type != VT_DOPPLER) return;  // lightcurve views take the FP64 kernels
   if (src && !V.bistatic) return;
   if (pass == 3 && *pd.nstale == 0) return;
   const GeomSlot &G = slots[V.slot];
@@ -205,7 +206,6 @@ __global__ __launch_bounds__(256) void k_chi2_mixed(FrameRec *frames, int nframe
 static inline unsigned cdivm(long long a, long long b) { return (unsigned)((a + b - 1) / b); }
 
 int enqueue_mixed(Engine &E) {
-  if (E.any_lc) return -41;  // lightcurve datasets use the FP64 method
   hipStream_t st = E.stream;
   dim3 gf(pad_xcd(cdivm(E.max_nf, 256)), E.nviews), bt(256);
   int wmax_ = 0;
@@ -218,10 +218,37 @@ int enqueue_mixed(Engine &E) {
   SHP_LAUNCH(E, "k_raster32/3", k_raster32, gf, bt, 0, st, E.d_views, E.d_slots, E.pd, E.total_pix, 0, 3);
   SHP_LAUNCH(E, "k_replay32", k_replay32, dim3(std::min(cdivm(E.pd.stale_cap, 256), 2048u)), bt, 0, st, E.d_views, E.d_slots, E.pd,
                      E.d_fit, E.d_frames, E.d_overflow, E.d_params);
+  if (E.any_lc) {
+    // Lightcurves: the FP64 kernels draw the observer and source views of
+    // every lightcurve point (radar views carry fp64_skip), mask shadows and
+    // sum brightness, exactly as the FP64 method does.  Replay records are
+    // shared with the radar replay above, which has finished with them.
+    const int T = 256;
+    dim3 gfl(cdivm(E.max_nf, T), E.nviews), gpl(cdivm(E.max_pix, T), E.nviews);
+    SHP_LAUNCH(E, "k_zero_counters", k_zero_counters, dim3(1), dim3(64), 0, st, E.pd.nstale, 1);
+    SHP_LAUNCH(E, "k_raster64/1", k_raster64, gfl, bt, 0, st, E.d_views, E.d_slots, E.pd, 0, 1);
+    SHP_LAUNCH(E, "k_raster64/2", k_raster64, gfl, bt, 0, st, E.d_views, E.d_slots, E.pd, 0, 2);
+    SHP_LAUNCH(E, "k_resolve64", k_resolve64, gpl, bt, 0, st, E.d_views, E.d_slots, E.pd, 0, 0, 1, 0, 0);
+    SHP_LAUNCH(E, "k_raster64/3", k_raster64, gfl, bt, 0, st, E.d_views, E.d_slots, E.pd, 0, 3);
+    SHP_LAUNCH(E, "k_replay64", k_replay64, dim3(std::min(cdivm(E.pd.stale_cap, T), 2048u)), bt, 0, st, E.d_views,
+               E.d_slots, E.pd);
+    SHP_LAUNCH(E, "k_zero_counters", k_zero_counters, dim3(1), dim3(64), 0, st, E.pd.nstale, 1);
+    SHP_LAUNCH(E, "k_raster64/1/sun", k_raster64, gfl, bt, 0, st, E.d_views, E.d_slots, E.pd, 1, 1);
+    SHP_LAUNCH(E, "k_raster64/2/sun", k_raster64, gfl, bt, 0, st, E.d_views, E.d_slots, E.pd, 1, 2);
+    SHP_LAUNCH(E, "k_resolve64", k_resolve64, gpl, bt, 0, st, E.d_views, E.d_slots, E.pd, 1, 0, 1, 0, 0);
+    SHP_LAUNCH(E, "k_posmask64", k_posmask64, gpl, bt, 0, st, E.d_views, E.pd, E.mask_tol);
+    SHP_LAUNCH(E, "k_photo64", k_photo64, gpl, bt, 0, st, E.d_views, E.pd, E.d_params);
+  }
   SHP_LAUNCH(E, "k_finish_views", k_finish_views, dim3(cdivm(E.nviews, 64)), dim3(64), 0, st, E.d_views, E.nviews, E.d_overflow,
                      E.d_frames, E.d_view_out);
   SHP_LAUNCH(E, "k_chi2_mixed", k_chi2_mixed, dim3(E.nframes), bt, 0, st, E.d_frames, E.nframes, E.d_views, E.d_fit,
                      E.d_obs32, E.d_oov32, E.d_view_out, E.opt.keep_fit);
+  if (E.any_lc) {
+    SHP_LAUNCH(E, "k_spline", k_spline, dim3(E.nframes), bt, 0, st, E.d_frames, E.nframes, E.d_view_out, E.d_views,
+               E.d_lc_x, E.d_lc_t, E.d_lc_y, E.d_lc_y2, E.d_fit);
+    SHP_LAUNCH(E, "k_chi2_frames64", k_chi2_frames64, dim3(E.nframes), bt, 0, st, E.d_frames, E.nframes, E.d_fit,
+               E.d_obs, E.d_oov, E.d_view_out, 0, 0, 1);
+  }
   SHP_LAUNCH(E, "k_objective", k_objective, dim3(cdivm(E.nevals, 64)), dim3(64), 0, st, E.d_evals, E.nevals, E.d_frames,
                      E.d_views, E.d_view_out, E.d_slot_flags, E.d_slot_logf, E.d_eval_slot);
   return hipGetLastError() == hipSuccess ? 0 : -42;

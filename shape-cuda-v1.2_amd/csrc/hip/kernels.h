@@ -66,7 +66,7 @@ __global__ void k_finish_views(ViewRec *views, int nviews, double *overflow, con
 __global__ void k_spline(FrameRec *frames, int nframes, const double *view_out, const ViewRec *views,
                          const double *lc_x, const double *lc_t, double *lc_y, double *lc_y2, double *fit);
 __global__ void k_chi2_frames64(FrameRec *frames, int nframes, double *fit, const double *obs, const double *oov,
-                                const double *view_out, int clear_fit, int ovf_from_host);
+                                const double *view_out, int clear_fit, int ovf_from_host, int lc_only);
 __global__ void k_objective(EvalRec *evals, int nevals, const FrameRec *frames, const ViewRec *views,
                             const double *view_out, const int *slot_flags, const double *slot_logf,
                             const int *eval_slot);

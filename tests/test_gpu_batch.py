@@ -68,7 +68,9 @@ def test_fp64_batch_matches_oracle(config, over):
 
 @pytest.mark.parametrize("config,over", [(1, {}), (2, dict(n_deldop=16)),
                                          (1, dict(n_deldop=0, n_doppler=2, nbin_min=30, nbin_max=40)),
-                                         (5, dict(n_deldop=6))])
+                                         (5, dict(n_deldop=6)),
+                                         (3, dict(n_deldop=3, n_doppler=3, n_lightcurves=2, lc_points=30,
+                                                  lc_ncalc=10))])
 def test_mixed_batch_matches_oracle(config, over):
     _check(config, sh.METHOD_MIXED, over, 1e-6, 1e-4)
 
@@ -92,3 +94,22 @@ def test_graph_replays_match_eager(method):
         tol = 1e-12 if method == sh.METHOD_FP64 else 1e-6
         assert np.allclose(og, oe, rtol=tol, atol=0), (k, og, oe)
         assert np.allclose(cg, ce, rtol=tol, atol=0), (k, cg, ce)
+
+
+@pytest.mark.parametrize("method,tol", [(sh.METHOD_FP64, 1e-10), (sh.METHOD_MIXED, 1e-6)])
+def test_multi_model_batch(method, tol):
+    """Several independent models (own seeds, frame counts and sizes) and
+    trial values in one replay, each evaluation checked against the oracle
+    on its own model (the shape of the batched-fit workload)."""
+    orc = H.load_oracle()
+    scenes = [H.make_scene_with_obs(5, n_deldop=3, seed_geom=5000 + m, seed_frames=5500 + m) for m in range(4)]
+    evals = [(m, t) for m in range(4) for t in ([], [(0, 1.015)])]
+    B = sh.Batch(scenes, evals, method=method, flags=sh.BATCH_GRAPH)
+    obj, chi = B.run()
+    assert B.counters()[1] == 0
+    for e, (m, t) in enumerate(evals):
+        o_obj, o_chi, _ = _oracle_eval(orc, scenes[m], t)
+        assert abs(obj[e] - o_obj) <= tol * abs(o_obj), (e, m, t, obj[e], o_obj)
+        assert abs(chi[e] - o_chi) <= tol * abs(o_chi), (e, m, t, chi[e], o_chi)
+    # distinct models really differ
+    assert len({round(float(c), 6) for c in chi[::2]}) == 4
