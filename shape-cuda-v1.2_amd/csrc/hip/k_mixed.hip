@@ -26,9 +26,9 @@
 
 namespace shp {
 
-__global__ void k_shade_radar32(ViewRec *views, const GeomSlot *slots, PosDev pd, double *fit, const FrameRec *frames,
+__global__ void k_shade_radar32(ViewRec *__restrict__ views, const GeomSlot *slots, PosDev pd, double *fit, const FrameRec *frames,
                                 double *overflow, const double *params, int *counters);
-__global__ void k_replay32(ViewRec *views, const GeomSlot *slots, PosDev pd, double *fit, const FrameRec *frames,
+__global__ void k_replay32(ViewRec *__restrict__ views, const GeomSlot *slots, PosDev pd, double *fit, const FrameRec *frames,
                            double *overflow, const double *params);
 
 // ---------------------------------------------------------------------------
@@ -37,7 +37,7 @@ __global__ void k_replay32(ViewRec *views, const GeomSlot *slots, PosDev pd, dou
 // rectangles; pass 3: record the facets covering replayed pixels, visiting
 // only facets whose rectangle touches a tile flagged in the stale bitmap
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_raster32(ViewRec *views, const GeomSlot *slots, PosDev pd,
+__global__ __launch_bounds__(256) void k_raster32(ViewRec *__restrict__ views, const GeomSlot *slots, PosDev pd,
                                                   long long sun_off, int src, int pass) {
   ViewRec &V = views[blockIdx.y];
   if (V.type != VT_DELAY && V.type != VT_DOPPLER) return;  // lightcurve views take the FP64 kernels
@@ -68,21 +68,29 @@ __global__ __launch_bounds__(256) void k_raster32(ViewRec *views, const GeomSlot
       xl0 = min(max(imin, -n), n); xl1 = max(min(imax, n), -n);
       yl0 = min(max(jmin, -n), n); yl1 = max(min(jmax, n), -n);
     } else if (pass == 1) {
-      for (int i = i1; i <= i2; i++)
-        for (int j = j1; j <= j2; j++) {
-          float s, t;
-          float m = bary32(P, i, j, s, t);
-          if (!(m >= -AMB_TOL)) continue;
-          long long pix = (long long)(i + n) * w + (j + n);
-          if (m >= AMB_TOL || src) {
-            atomicMax(&kb[pix], (unsigned long long)pack_key(depth32(P, s, t), f));
-          } else {
-            mark_marginal(V, pd, (int)pix, i, j);
-          }
-          xl0 = min(xl0, i); xl1 = max(xl1, i); yl0 = min(yl0, j); yl1 = max(yl1, j);
-          o0 = fminf(o0, (float)i); o1 = fmaxf(o1, (float)i);
-          o2 = fminf(o2, (float)j); o3 = fmaxf(o3, (float)j);
+      // one flat loop over the bounding box: a wave runs max(cells) iterations,
+      // not max(rows) * max(columns)
+      int nc = (i2 - i1 + 1) * (j2 - j1 + 1);
+      bool hit = false;
+      for (int c = 0, i = i1, j = j1; c < nc; c++) {
+        float s, t;
+        float m = bary32(P, i, j, s, t);
+        long long pix = (long long)(i + n) * w + (j + n);
+        if (m >= AMB_TOL || (src && m >= -AMB_TOL)) {
+          hit = true;
+          atomicMax(&kb[pix], (unsigned long long)pack_key(depth32(P, s, t), f));
         }
+        bool marginal = !src && m >= -AMB_TOL && m < AMB_TOL;
+        hit = hit || marginal;
+        mark_marginal(V, pd, marginal, (int)pix, i, j);
+        if (++j > j2) { j = j1; i++; }
+      }
+      // The model rectangle grows by the box of every facet that covers a
+      // pixel: at most a pixel wider than the covered pixels themselves, which
+      // only widens the resolve's tile range.  Pixel positions inside the POS
+      // never change the posbnd factor, so the overall extent takes boxes of
+      // facets wholly outside only (above).
+      if (hit) { xl0 = i1; xl1 = i2; yl0 = j1; yl1 = j2; }
     } else {
       int tw = (2 * n + STALE_TILE) / STALE_TILE;
       const unsigned char *tb = pd.stale_tiles + V.tile_off;
@@ -90,18 +98,21 @@ __global__ __launch_bounds__(256) void k_raster32(ViewRec *views, const GeomSlot
       for (int a = (i1 + n) / STALE_TILE; a <= (i2 + n) / STALE_TILE && !any; a++)
         for (int b = (j1 + n) / STALE_TILE; b <= (j2 + n) / STALE_TILE; b++)
           if (tb[(long long)a * tw + b]) { any = true; break; }
-      if (any)
-        for (int i = i1; i <= i2; i++)
-          for (int j = j1; j <= j2; j++) {
-            // candidates: a superset of the facets covering the pixel in FP64
-            float s, t;
-            if (!(bary32(P, i, j, s, t) >= -AMB_TOL)) continue;
+      if (any) {
+        int nc = (i2 - i1 + 1) * (j2 - j1 + 1);
+        for (int c = 0, i = i1, j = j1; c < nc; c++) {
+          // candidates: a superset of the facets covering the pixel in FP64
+          float s, t;
+          if (bary32(P, i, j, s, t) >= -AMB_TOL) {
             int si = pd.stale_idx[V.pos_off + (long long)(i + n) * w + (j + n)];
             if (si >= 0) {
               int q = atomicAdd(&pd.stale[si].ncand, 1);
               if (q < STALE_CAND) pd.stale[si].cand[q] = f;
             }
           }
+          if (++j > j2) { j = j1; i++; }
+        }
+      }
     }
   }
   if (pass != 1) return;

@@ -14,7 +14,8 @@ namespace shp {
 //      the FP64 LDS window at once (footprints leaving the frame, or a window
 //      too large for LDS, go straight to HBM)
 //   C. flush the window to the FP64 fit image; one update of the view's
-//      bin limits and claimed-key count per workgroup
+//      bin limits (the claimed pixels' bins: the chi-square window) and
+//      claimed-key count per workgroup
 // ---------------------------------------------------------------------------
 // block-wide minimum of four ints at once (one barrier pair)
 __device__ inline void blk_min4(int &a, int &b, int &c, int &d, int (*sh)[4]) {
@@ -29,7 +30,7 @@ __device__ inline void blk_min4(int &a, int &b, int &c, int &d, int (*sh)[4]) {
   __syncthreads();
 }
 
-__global__ __launch_bounds__(256) void k_shade_radar32(ViewRec *views, const GeomSlot *slots, PosDev pd,
+__global__ __launch_bounds__(256) void k_shade_radar32(ViewRec *__restrict__ views, const GeomSlot *slots, PosDev pd,
                                                        double *fit, const FrameRec *frames, double *overflow,
                                                        const double *params, int *counters) {
   ViewRec &V = views[blockIdx.y];
@@ -43,8 +44,7 @@ __global__ __launch_bounds__(256) void k_shade_radar32(ViewRec *views, const Geo
   const int xl0 = V.xlim[0], xl1 = V.xlim[1], yl0 = V.ylim[0], yl1 = V.ylim[1];
   if (i0 > xl1 || i0 + TILE - 1 < xl0 || j0 > yl1 || j0 + TILE - 1 < yl0) return;
   __shared__ double win[WIN_MAX];
-  __shared__ int shi[8][4];
-  __shared__ float shf[4][4];
+  __shared__ int shi[2][4];
   __shared__ int shw[4][4];
   const GeomSlot &G = slots[V.slot];
   const double *pl = params + V.pb + V.pb_law;
@@ -92,6 +92,9 @@ __global__ __launch_bounds__(256) void k_shade_radar32(ViewRec *views, const Geo
     if (!dd) lo_d = hi_d = 0;
   }
   const bool any = lo_p != 0x7fffffff;
+  // the claimed pixels' bins bound the lit pixels' bins: they are the tile's
+  // contribution to the frame's delay / Doppler limits (the chi-square window)
+  const int lim_d0 = lo_d, lim_d1 = hi_d, lim_p0 = lo_p, lim_p1 = hi_p;
   if (any) {  // one bin of margin: phase B recomputes the same floors
     lo_p--; hi_p++;
     if (dd) { lo_d--; hi_d++; }
@@ -106,8 +109,7 @@ __global__ __launch_bounds__(256) void k_shade_radar32(ViewRec *views, const Geo
   // ---- B: shade, footprint, accumulate ----
   double *img = fit + frames[V.frame_id].off;
   double *ov = overflow + (long long)blockIdx.y * 2000;
-  int a0 = 0x7fffffff, a1 = -0x7fffffff, b0 = 0x7fffffff, b1 = -0x7fffffff, anyo = 0;
-  float d0 = 1e30f, d1 = -1e30f, p0 = 1e30f, p1 = -1e30f;
+  int anyo = 0;
   for (int m = 0; m < TPP; m++) {
     unsigned long long k = key[m];
     if (!k) continue;
@@ -120,25 +122,15 @@ __global__ __launch_bounds__(256) void k_shade_radar32(ViewRec *views, const Geo
     bary32(P, i, j, s, t);  // same arithmetic as the raster
     float nx, ny, nz;
     normal32(G, f, V.oa32, V.smooth, s, t, nx, ny, nz);
+    // cos(scattering) of an owner facing away is inherited from an earlier
+    // claimant: FP64 replay (records reserved once per wave)
+    int si = stale_slot(pd, !(nz > 0.0f));
     if (!(nz > 0.0f)) {
-      // cos(scattering) is inherited from an earlier claimant: FP64 replay
-      long long p = V.pos_off + (long long)(i + n) * w + (j + n);
-      int si = atomicAdd(pd.nstale, 1);
-      if (si < pd.stale_cap) {
-        StaleRec &S = pd.stale[si];
-        S.view = blockIdx.y; S.pix = (int)(p - V.pos_off); S.oldf = -1; S.oldz = -1.0e20; S.oldcose = 0.0;
-        S.oldcosi = 0.0; S.ncand = 0; S.src = 0;
-        pd.stale_idx[p] = si;
-        pd.stale_tiles[stale_tile(V, i, j)] = 1;
-      } else {
-        atomicOr(pd.error, 1);
-      }
+      if (si >= 0) stale_init(V, pd, si, blockIdx.y, (i + n) * w + (j + n), i, j);
       continue;
     }
     RadarPix R;
     radar_pixel(V, pl, i, j, (double)key_z(k), nz, R);
-    a0 = min(a0, R.id0); a1 = max(a1, R.id1); b0 = min(b0, R.ip0); b1 = max(b1, R.ip1);
-    d0 = fminf(d0, R.delrel); d1 = fmaxf(d1, R.delrel); p0 = fminf(p0, R.doprel); p1 = fmaxf(p1, R.doprel);
     if (!R.inb) anyo = 1;
     if (R.wide) { atomicAdd(&counters[3], 1); continue; }
     if (use_lds && R.inb) {
@@ -172,39 +164,23 @@ __global__ __launch_bounds__(256) void k_shade_radar32(ViewRec *views, const Geo
       }
     }
   }
-  a0 = wmin(a0); a1 = wmax(a1); b0 = wmin(b0); b1 = wmax(b1);
-  d0 = wminf(d0); d1 = wmaxf(d1); p0 = wminf(p0); p1 = wmaxf(p1);
   anyo = wmax(anyo);
   ncl = wsum(ncl);
   int wv = threadIdx.x >> 6;
-  if ((threadIdx.x & 63) == 0) {
-    shi[4][wv] = a0; shi[5][wv] = a1; shi[6][wv] = b0; shi[7][wv] = b1;
-    shf[0][wv] = d0; shf[1][wv] = d1; shf[2][wv] = p0; shf[3][wv] = p1;
-    shi[0][wv] = anyo; shi[1][wv] = ncl;
-  }
+  if ((threadIdx.x & 63) == 0) { shi[0][wv] = anyo; shi[1][wv] = ncl; }
   __syncthreads();
   if (threadIdx.x != 0) return;
-  for (int q = 1; q < (int)(blockDim.x >> 6); q++) {
-    a0 = min(a0, shi[4][q]); a1 = max(a1, shi[5][q]); b0 = min(b0, shi[6][q]); b1 = max(b1, shi[7][q]);
-    d0 = fminf(d0, shf[0][q]); d1 = fmaxf(d1, shf[1][q]); p0 = fminf(p0, shf[2][q]); p1 = fmaxf(p1, shf[3][q]);
-    anyo |= shi[0][q]; ncl += shi[1][q];
-  }
+  for (int q = 1; q < (int)(blockDim.x >> 6); q++) { anyo |= shi[0][q]; ncl += shi[1][q]; }
   if (ncl) atomicAdd(&V.nclaimed, ncl);
-  if (a0 == 0x7fffffff) return;
+  if (!any) return;
   // the limits only move outward: skip updates a read of the current value
   // shows cannot move them
   if (dd) {
-    if (V.D0 + a0 < V.idellim[0]) atomicMin(&V.idellim[0], V.D0 + a0);
-    if (V.D0 + a1 > V.idellim[1]) atomicMax(&V.idellim[1], V.D0 + a1);
-    unsigned long long k0 = ord64((double)V.D0 + (double)d0), k1 = ord64(-((double)V.D0 + (double)d1));
-    if (k0 < V.dellim[0]) atomicMin(&V.dellim[0], k0);
-    if (k1 < V.dellim[1]) atomicMin(&V.dellim[1], k1);
+    if (V.D0 + lim_d0 < V.idellim[0]) atomicMin(&V.idellim[0], V.D0 + lim_d0);
+    if (V.D0 + lim_d1 > V.idellim[1]) atomicMax(&V.idellim[1], V.D0 + lim_d1);
   }
-  if (V.P0 + b0 < V.idoplim[0]) atomicMin(&V.idoplim[0], V.P0 + b0);
-  if (V.P0 + b1 > V.idoplim[1]) atomicMax(&V.idoplim[1], V.P0 + b1);
-  unsigned long long k2 = ord64((double)V.P0 + (double)p0), k3 = ord64(-((double)V.P0 + (double)p1));
-  if (k2 < V.doplim[0]) atomicMin(&V.doplim[0], k2);
-  if (k3 < V.doplim[1]) atomicMin(&V.doplim[1], k3);
+  if (V.P0 + lim_p0 < V.idoplim[0]) atomicMin(&V.idoplim[0], V.P0 + lim_p0);
+  if (V.P0 + lim_p1 > V.idoplim[1]) atomicMax(&V.idoplim[1], V.P0 + lim_p1);
   if (!dd && anyo && !V.any_overflow) atomicOr(&V.any_overflow, 1);
 }
 
@@ -270,7 +246,7 @@ __device__ void replay32_one(ViewRec *views, const GeomSlot *slots, const PosDev
 // (posvis.c:142-275); then the pixel's power is scattered directly to HBM.
 // Grid-stride over the records: the grid is sized for the cap, not the count.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_replay32(ViewRec *views, const GeomSlot *slots, PosDev pd, double *fit,
+__global__ __launch_bounds__(256) void k_replay32(ViewRec *__restrict__ views, const GeomSlot *slots, PosDev pd, double *fit,
                                                   const FrameRec *frames, double *overflow, const double *params) {
   int ns = min(*pd.nstale, pd.stale_cap);
   for (int si = blockIdx.x * blockDim.x + threadIdx.x; si < ns; si += gridDim.x * blockDim.x)

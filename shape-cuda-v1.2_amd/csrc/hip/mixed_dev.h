@@ -87,23 +87,44 @@ __device__ inline long long stale_tile(const ViewRec &V, int i, int j) {
   return V.tile_off + (long long)((i + V.n) / STALE_TILE) * tw + (j + V.n) / STALE_TILE;
 }
 
-// Record a marginal pixel for the FP64 replay, once: the first facet to find
-// it swaps its stale index from -1 and appends a record that starts from
-// blank sky (every claimant of the pixel is then replayed in facet order).
-__device__ inline void mark_marginal(const ViewRec &V, const PosDev &pd, int pix, int i, int j) {
-  long long p = V.pos_off + pix;
-  if (atomicCAS(&pd.stale_idx[p], -1, -2) != -1) return;
-  int si = atomicAdd(pd.nstale, 1);
+// Reserve replay records for the lanes of the calling wave that need one:
+// one atomic on the shared counter per wave (a per-lane atomic on that single
+// address serializes).  Every active lane of the wave must call it.  Returns
+// the lane's record index, or -1 when it needs none or the list is full.
+__device__ inline int stale_slot(const PosDev &pd, bool need) {
+  unsigned long long mask = __ballot(need);
+  if (!mask) return -1;
+  int lane = threadIdx.x & 63, leader = __ffsll((long long)mask) - 1, base = 0;
+  if (lane == leader) base = atomicAdd(pd.nstale, __popcll(mask));
+  base = __shfl(base, leader, 64);
+  if (!need) return -1;
+  int si = base + __popcll(mask & ((1ULL << lane) - 1ULL));
   if (si >= pd.stale_cap) {
-    pd.stale_idx[p] = -1;
     atomicOr(pd.error, 1);
-    return;
+    return -1;
   }
+  return si;
+}
+
+// an empty replay record: the pixel's claims start from blank sky and are
+// replayed in facet order
+__device__ inline void stale_init(const ViewRec &V, const PosDev &pd, int si, int view, int pix, int i, int j) {
   StaleRec &S = pd.stale[si];
-  S.view = blockIdx.y; S.pix = pix; S.oldf = -1; S.oldz = -1.0e20; S.oldcose = 0.0;
+  S.view = view; S.pix = pix; S.oldf = -1; S.oldz = -1.0e20; S.oldcose = 0.0;
   S.oldcosi = 0.0; S.ncand = 0; S.src = 0;
-  pd.stale_idx[p] = si;
+  pd.stale_idx[V.pos_off + pix] = si;
   pd.stale_tiles[stale_tile(V, i, j)] = 1;
+}
+
+// Record a marginal pixel for the FP64 replay, once: the first facet to find
+// it swaps its stale index from -1.  Every active lane of the wave calls this
+// (marginal or not) so that records are reserved once per wave.
+__device__ inline void mark_marginal(const ViewRec &V, const PosDev &pd, bool marginal, int pix, int i, int j) {
+  long long p = V.pos_off + pix;
+  bool first = marginal && atomicCAS(&pd.stale_idx[p], -1, -2) == -1;
+  int si = stale_slot(pd, first);
+  if (si >= 0) stale_init(V, pd, si, blockIdx.y, pix, i, j);
+  else if (first) pd.stale_idx[p] = -1;  // list full: the error flag is set
 }
 
 // normal at (s, t) of facet f in observer coordinates (posvis.c:243-258)
