@@ -112,6 +112,8 @@ typedef struct shape_batch shape_batch;
 
 #define SHAPE_BATCH_GRAPH 1     /* capture one replay into a hipGraph */
 #define SHAPE_BATCH_KEEP_FIT 2  /* keep fit images for inspection (single replay) */
+#define SHAPE_BATCH_GLOBAL_ZBUF 4 /* mixed method: rasterize every view through a global z-buffer
+                                     instead of per-tile LDS z-buffers (same results; for checking) */
 
 shape_batch *shape_batch_create(struct par_t *par, struct mod_t **mods, struct dat_t **dats, int nmodels,
                                 int nevals, const int *eval_model, const int *n_targets,
@@ -127,8 +129,9 @@ int shape_batch_fit(shape_batch *b, int e, int frame, double *out);
 /* per frame of evaluation e: o2, m2, om, overflow_o2, overflow_m2, cal, chi2,
    badradar_logfactor (8 doubles per frame, set/frame order) */
 int shape_batch_frame_terms(shape_batch *b, int e, double *out);
-/* [0] replayed pixels, [1] error flags, [2] unused,
-   [3] mixed: footprints too wide for the register window */
+/* [0] pixels replayed through the global replay list, [1] error flags,
+   [2] mixed: pixels replayed inside their tile, [3] mixed: footprints too wide
+   for the register window */
 int shape_batch_counters(shape_batch *b, int *out4);
 /* nevals, nviews, nframes, geometry slots, POS pixels, fit cells, max facets, max POS pixels */
 int shape_batch_sizes(shape_batch *b, long long *out8);

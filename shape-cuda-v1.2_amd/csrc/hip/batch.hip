@@ -43,6 +43,7 @@ extern "C" shape_batch *shape_batch_create(struct par_t *par, struct mod_t **mod
   RunOptions o;
   o.method = method == 1 ? METHOD_MIXED : METHOD_FP64;
   o.keep_fit = (flags & SHAPE_BATCH_KEEP_FIT) ? 1 : 0;
+  o.global_zbuf = (flags & SHAPE_BATCH_GLOBAL_ZBUF) ? 1 : 0;
   B->use_graph = (flags & SHAPE_BATCH_GRAPH) ? 1 : 0;
   int rc = B->E.build(par, mods, dats, nmodels, ev, o);
   if (!rc && B->use_graph) rc = B->E.capture_graph();

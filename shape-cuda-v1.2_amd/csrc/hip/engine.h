@@ -60,6 +60,7 @@ struct GeomSlot {
   double *fn, *area, *fx;// 3*nf, nf, 3*nf
   float4 *x32, *vn32;    // nv
   float4 *fn32;          // nf
+  float4 *blk;           // mixed method: per 64-facet block, {center, radius} and {cone axis, min cos}
   unsigned char *seen;   // nf
 };
 
@@ -82,6 +83,8 @@ struct ViewRec {
   int fp64_skip;         // mixed method: radar view, skipped by the FP64 kernels
   long long pos_off;     // pixel offset of this view's POS buffers
   long long tile_off;    // offset of this view's stale-tile bitmap (mixed method)
+  long long bin_off;     // binned mixed raster: first 32x32 tile of this view in the bin arrays
+  int bin_ntx, binned;   // tiles per side; 1 when the view takes the binned tile kernel
   int pb;                // parameter block of the evaluation
   int pb_set, pb_law;    // offsets (within the block) of the set and scattering-law params
   int pb_dc;             // offset of the set block whose delay polynomial applies (= states)

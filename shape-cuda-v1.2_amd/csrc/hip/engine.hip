@@ -418,7 +418,7 @@ int Engine::build(struct par_t *par, struct mod_t **mods, struct dat_t **dats, i
   h_views.clear();
   h_frames.clear();
   h_evals.clear();
-  long long fit_cursor = 0, pos_cursor = 0, y_cursor = 0, y2_cursor = 0, tile_cursor = 0;
+  long long fit_cursor = 0, pos_cursor = 0, y_cursor = 0, y2_cursor = 0, tile_cursor = 0, bin_cursor = 0;
   std::vector<long long> model_obs_base(nmodels, 0), model_x_base(nmodels, 0), model_t_base(nmodels, 0);
   long long ob = 0, xb = 0, tb = 0;
   for (int m = 0; m < nmodels; m++) {
@@ -481,6 +481,20 @@ int Engine::build(struct par_t *par, struct mod_t **mods, struct dat_t **dats, i
       pos_cursor += (long long)(2 * V.n + 1) * (2 * V.n + 1);
       V.tile_off = tile_cursor;
       tile_cursor += (long long)((2 * V.n + STALE_TILE) / STALE_TILE) * ((2 * V.n + STALE_TILE) / STALE_TILE);
+      V.binned = 0; V.bin_ntx = 0; V.bin_off = 0;
+      if (opt.method == METHOD_MIXED && V.type != VT_LGHTCRV) {
+        // binned tile kernel unless the POS is wider than its tile grid, the
+        // facet index does not fit an event, or the global z-buffer is asked for
+        int ntx = (2 * V.n + 1 + BT - 1) / BT;
+        if (!opt.global_zbuf && ntx <= BIN_MAX_NTX && P.nf <= (1 << EV_FACET_BITS)) {
+          V.binned = 1; V.bin_ntx = ntx; V.bin_off = bin_cursor;
+          bin_cursor += (long long)ntx * ntx;
+          max_bin_ntx = std::max(max_bin_ntx, ntx);
+          any_binned = true;
+        } else {
+          any_global = true;
+        }
+      }
       max_pix = std::max(max_pix, (2 * V.n + 1) * (2 * V.n + 1));
       if (V.type == VT_LGHTCRV) any_lc = true;
       if (V.type == VT_DELAY) any_dd = true;
@@ -509,6 +523,7 @@ int Engine::build(struct par_t *par, struct mod_t **mods, struct dat_t **dats, i
   nslots = (int)h_slots.size();
   total_pix = pos_cursor;
   total_tiles = tile_cursor;
+  total_bins = bin_cursor;
   fit_len = fit_cursor;
   // ---- device arenas ----
   max_nf = 0;
@@ -527,6 +542,11 @@ int Engine::build(struct par_t *par, struct mod_t **mods, struct dat_t **dats, i
     if ((rc = alloc((void **)&G.vn32, sizeof(float4) * G.nv))) return rc;
     if ((rc = alloc((void **)&G.fn32, sizeof(float4) * G.nf))) return rc;
     if ((rc = alloc((void **)&G.seen, G.nf))) return rc;
+    if (opt.method == METHOD_MIXED) {
+      int nblk = (G.nf + BF - 1) / BF;
+      max_nblk = std::max(max_nblk, nblk);
+      if ((rc = alloc((void **)&G.blk, sizeof(float4) * 2 * (size_t)nblk))) return rc;
+    }
     SHP_CHECK(hipMemset(G.seen, 0, G.nf));
     if (opt.host_geometry) {
       SHP_CHECK(hipMemcpy(G.x, P.hx.data(), sizeof(double) * 3 * G.nv, hipMemcpyHostToDevice));
@@ -624,6 +644,12 @@ int Engine::build(struct par_t *par, struct mod_t **mods, struct dat_t **dats, i
   if (opt.method == METHOD_MIXED && !pd.stale_idx) {
     if ((rc = alloc((void **)&pd.stale_idx, sizeof(int) * P))) return rc;
     hipLaunchKernelGGL(k_fill_i32, dim3(1024), dim3(256), 0, stream, pd.stale_idx, (long long)P, -1);
+  }
+  if (any_binned) {
+    // tile lists: counts start (and are left by every tile) at zero
+    if ((rc = alloc((void **)&d_bin_cnt, sizeof(int) * (size_t)total_bins))) return rc;
+    SHP_CHECK(hipMemsetAsync(d_bin_cnt, 0, sizeof(int) * (size_t)total_bins, stream));
+    if ((rc = alloc((void **)&d_bin_list, sizeof(unsigned short) * BIN_CAP * (size_t)total_bins))) return rc;
   }
   if (opt.method == METHOD_MIXED) {
     if ((rc = alloc((void **)&pd.stale_tiles, (size_t)std::max(1LL, total_tiles)))) return rc;

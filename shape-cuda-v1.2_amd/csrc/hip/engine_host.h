@@ -34,6 +34,7 @@ struct RunOptions {
   int keep_fit = 1;         // 1: leave fit images in place (0: read-and-clear in chi2)
   int with_body = 0;        // allocate body/comp arrays (drop-ins)
   int views_only = 0;       // pack parameters and views only (spin / offset drop-ins)
+  int global_zbuf = 0;      // mixed method: rasterize every view through the global z-buffer
 };
 
 class Engine {
@@ -69,6 +70,10 @@ class Engine {
   int last_nstale = 0;      // pixels replayed in the last evaluation (diagnostic)
   long long total_pix = 0, fit_len = 0, obs_len = 0, total_tiles = 0;
   bool any_lc = false, any_dd = false, any_dop = false;
+  // mixed method: views taking the binned tile kernel / the global z-buffer
+  bool any_binned = false, any_global = false;
+  int max_bin_ntx = 0, max_nblk = 0;
+  long long total_bins = 0;
   std::vector<ViewRec> h_views;
   std::vector<FrameRec> h_frames;
   std::vector<EvalRec> h_evals;
@@ -89,6 +94,8 @@ class Engine {
   double *d_overflow = nullptr, *d_view_out = nullptr;
   double *d_lc_x = nullptr, *d_lc_t = nullptr, *d_lc_y = nullptr, *d_lc_y2 = nullptr;
   float *d_obs32 = nullptr, *d_oov32 = nullptr;
+  int *d_bin_cnt = nullptr;                 // per 32x32 tile of every binned view
+  unsigned short *d_bin_list = nullptr;     // BIN_CAP block ids per tile
   PosDev pd{};
   std::vector<void *> allocs;
   hipGraph_t graph = nullptr;

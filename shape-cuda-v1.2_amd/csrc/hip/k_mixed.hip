@@ -5,16 +5,21 @@
 // composition at large epoch offsets, COM delay/Doppler offsets; k_fp64.hip)
 // and every reduction into chi-square (compensated FP64).  The bulk -- vertex
 // projection, rasterization, shading and delay-Doppler binning -- runs in FP32:
-//   * raster: facet per thread, one 64-bit atomicMax per covered pixel on a
-//     packed (FP32 depth, inverted facet index) key, so the nearest facet wins
-//     and equal depths go to the lower facet index, as in the CPU z-test;
-//   * resolve+synthesis (k_radar32.hip): one workgroup per 16x16-pixel tile
-//     reads and clears the keys, shades the owner, and accumulates the tile's
-//     delay-Doppler footprint in an LDS window before one flush to HBM;
+//   * raster: one 64-bit atomicMax per covered pixel on a packed (FP32 depth,
+//     inverted facet index) key, so the nearest facet wins and equal depths
+//     go to the lower facet index, as in the CPU z-test;
+//   * binned views (k_tile32.hip): facets are binned into 32x32-pixel tiles
+//     and one workgroup per tile rasterizes into an LDS z-buffer, shades the
+//     owners and accumulates the tile's delay-Doppler footprint in an LDS
+//     window before one flush to HBM;
+//   * other views (POS wider than 2048 pixels, or the global path on
+//     request): facet per thread into a global z-buffer, then a resolve
+//     kernel (k_radar32.hip) per 16x16-pixel tile that reads and clears the
+//     keys and synthesizes the same way;
 //   * pixels whose owner's smoothed normal faces away, and pixels within the
 //     FP32 rounding band of a facet edge, are replayed in FP64 from the facets
-//     covering them (a raster pass restricted to 8x8 tiles that hold such
-//     pixels), reproducing the CPU's owner and inherited cos(scattering);
+//     covering them, reproducing the CPU's owner and inherited
+//     cos(scattering);
 //   * per-pixel contributions are FP32, every accumulation FP64: the fit
 //     image, and chi-square as compensated partial sums over each frame's
 //     occupied window, reading and clearing the fit.
@@ -30,12 +35,21 @@ __global__ void k_shade_radar32(ViewRec *__restrict__ views, const GeomSlot *slo
                                 double *overflow, const double *params, int *counters);
 __global__ void k_replay32(ViewRec *__restrict__ views, const GeomSlot *slots, PosDev pd, double *fit, const FrameRec *frames,
                            double *overflow, const double *params);
+__global__ void k_block_bounds(GeomSlot *slots, int nslots);
+__global__ void k_bin32(const ViewRec *__restrict__ views, const GeomSlot *slots, int *bin_cnt,
+                        unsigned short *bin_list);
+__global__ void k_tile32(ViewRec *__restrict__ views, const GeomSlot *slots, PosDev pd, int *bin_cnt,
+                         const unsigned short *bin_list, double *fit, const FrameRec *frames, double *overflow,
+                         const double *params, int *counters);
 
 // ---------------------------------------------------------------------------
-// raster (posvis.c:142-295).  pass 1: packed-key z-buffer over pixels well
-// inside a facet, marginal pixels recorded for the FP64 replay, and the POS
-// rectangles; pass 3: record the facets covering replayed pixels, visiting
-// only facets whose rectangle touches a tile flagged in the stale bitmap
+// raster (posvis.c:142-295).  pass 1 (views that are not binned): packed-key
+// z-buffer over pixels well inside a facet, marginal pixels recorded for the
+// FP64 replay, and the POS rectangles; pass 0 (binned views): only the facets
+// reaching outside the POS window (posbnd and the outside extent), visiting
+// only blocks whose rectangle leaves the window; pass 3: record the facets
+// covering pixels in the global replay list, visiting only facets whose
+// rectangle touches a tile flagged in the stale bitmap
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_raster32(ViewRec *__restrict__ views, const GeomSlot *slots, PosDev pd,
                                                   long long sun_off, int src, int pass) {
@@ -43,11 +57,19 @@ __global__ __launch_bounds__(256) void k_raster32(ViewRec *__restrict__ views, c
   if (V.type != VT_DELAY && V.type != VT_DOPPLER) return;  // lightcurve views take the FP64 kernels
   if (src && !V.bistatic) return;
   if (pass == 3 && *pd.nstale == 0) return;
+  if ((pass == 0 && !V.binned) || (pass == 1 && V.binned)) return;
   const GeomSlot &G = slots[V.slot];
-  // pass 1 walks every facet of every view: XCD-local eighths of the model.
-  // Pass 3 works only near flagged tiles, which cluster: plain order balances it.
-  int f = (pass == 1 ? xcd_block(blockIdx.x, gridDim.x) : (int)blockIdx.x) * blockDim.x + threadIdx.x;
+  // passes 0 and 1 walk every facet of every view: XCD-local eighths of the
+  // model.  Pass 3 works only near flagged tiles, which cluster: plain order
+  // balances it.
+  int f = (pass != 3 ? xcd_block(blockIdx.x, gridDim.x) : (int)blockIdx.x) * blockDim.x + threadIdx.x;
   int n = V.n, w = 2 * n + 1;
+  if (pass == 0) {
+    // a wave is one bin block: skip it when its rectangle lies inside the window
+    int b = f / BF, r0, r1, c0, c1;
+    if (b * BF >= G.nf) return;
+    if (!block_rect(V, G, b, r0, r1, c0, c1) || (r0 >= -n && r1 <= n && c0 >= -n && c1 <= n)) return;
+  }
   unsigned long long *kb = pd.zkey + V.pos_off + (src ? sun_off : 0);
   int xl0 = n, xl1 = -n, yl0 = n, yl1 = -n, ob = 0;
   float o0 = 1e30f, o1 = -1e30f, o2 = 1e30f, o3 = -1e30f;
@@ -91,7 +113,7 @@ __global__ __launch_bounds__(256) void k_raster32(ViewRec *__restrict__ views, c
       // never change the posbnd factor, so the overall extent takes boxes of
       // facets wholly outside only (above).
       if (hit) { xl0 = i1; xl1 = i2; yl0 = j1; yl1 = j2; }
-    } else {
+    } else if (pass == 3) {
       int tw = (2 * n + STALE_TILE) / STALE_TILE;
       const unsigned char *tb = pd.stale_tiles + V.tile_off;
       bool any = false;
@@ -115,7 +137,7 @@ __global__ __launch_bounds__(256) void k_raster32(ViewRec *__restrict__ views, c
       }
     }
   }
-  if (pass != 1) return;
+  if (pass == 3) return;
   xl0 = wmin(xl0); xl1 = wmax(xl1); yl0 = wmin(yl0); yl1 = wmax(yl1); ob = wmax(ob);
   o0 = wminf(o0); o1 = wmaxf(o1); o2 = wminf(o2); o3 = wmaxf(o3);
   // one update per wave, skipped when a read of the current value shows it
@@ -223,9 +245,21 @@ int enqueue_mixed(Engine &E) {
   for (auto &V : E.h_views) wmax_ = std::max(wmax_, 2 * V.n + 1);
   int tiles = cdivm(wmax_, TILE);
   SHP_LAUNCH(E, "k_fill_u8", k_fill_u8, dim3(256), bt, 0, st, E.pd.stale_tiles, std::max(1LL, E.total_tiles));
-  SHP_LAUNCH(E, "k_raster32/1", k_raster32, gf, bt, 0, st, E.d_views, E.d_slots, E.pd, E.total_pix, 0, 1);
-  SHP_LAUNCH(E, "k_shade_radar32", k_shade_radar32, dim3(tiles * tiles, E.nviews), bt, 0, st, E.d_views, E.d_slots, E.pd,
-                     E.d_fit, E.d_frames, E.d_overflow, E.d_params, E.pd.nstale);
+  if (E.any_binned) {
+    SHP_LAUNCH(E, "k_block_bounds", k_block_bounds, dim3(cdivm(E.max_nblk, 4), E.nslots), bt, 0, st, E.d_slots,
+               E.nslots);
+    SHP_LAUNCH(E, "k_bin32", k_bin32, dim3(cdivm(E.max_nblk, 256), E.nviews), bt, 0, st, E.d_views, E.d_slots,
+               E.d_bin_cnt, E.d_bin_list);
+    SHP_LAUNCH(E, "k_raster32/0", k_raster32, gf, bt, 0, st, E.d_views, E.d_slots, E.pd, E.total_pix, 0, 0);
+    SHP_LAUNCH(E, "k_tile32", k_tile32, dim3(pad_xcd((unsigned)(E.max_bin_ntx * E.max_bin_ntx)), E.nviews), bt, 0, st,
+               E.d_views, E.d_slots, E.pd, E.d_bin_cnt, E.d_bin_list, E.d_fit, E.d_frames, E.d_overflow, E.d_params,
+               E.pd.nstale);
+  }
+  if (E.any_global) {
+    SHP_LAUNCH(E, "k_raster32/1", k_raster32, gf, bt, 0, st, E.d_views, E.d_slots, E.pd, E.total_pix, 0, 1);
+    SHP_LAUNCH(E, "k_shade_radar32", k_shade_radar32, dim3(tiles * tiles, E.nviews), bt, 0, st, E.d_views, E.d_slots,
+               E.pd, E.d_fit, E.d_frames, E.d_overflow, E.d_params, E.pd.nstale);
+  }
   SHP_LAUNCH(E, "k_raster32/3", k_raster32, gf, bt, 0, st, E.d_views, E.d_slots, E.pd, E.total_pix, 0, 3);
   SHP_LAUNCH(E, "k_replay32", k_replay32, dim3(std::min(cdivm(E.pd.stale_cap, 256), 2048u)), bt, 0, st, E.d_views, E.d_slots, E.pd,
                      E.d_fit, E.d_frames, E.d_overflow, E.d_params);

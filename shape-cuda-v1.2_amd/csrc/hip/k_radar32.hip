@@ -1,8 +1,7 @@
 // k_radar32.hip -- mixed method: resolve of the packed-key z-buffer fused with
 // delay-Doppler / Doppler synthesis, and the FP64 replay of pixels whose owner
 // faces away or that lie on a facet edge.
-#include "fp64_dev.h"
-#include "mixed_dev.h"
+#include "replay_dev.h"
 
 namespace shp {
 
@@ -17,24 +16,11 @@ namespace shp {
 //      bin limits (the claimed pixels' bins: the chi-square window) and
 //      claimed-key count per workgroup
 // ---------------------------------------------------------------------------
-// block-wide minimum of four ints at once (one barrier pair)
-__device__ inline void blk_min4(int &a, int &b, int &c, int &d, int (*sh)[4]) {
-  a = wmin(a); b = wmin(b); c = wmin(c); d = wmin(d);
-  int wv = threadIdx.x >> 6;
-  if ((threadIdx.x & 63) == 0) { sh[wv][0] = a; sh[wv][1] = b; sh[wv][2] = c; sh[wv][3] = d; }
-  __syncthreads();
-  a = sh[0][0]; b = sh[0][1]; c = sh[0][2]; d = sh[0][3];
-  for (int q = 1; q < (int)(blockDim.x >> 6); q++) {
-    a = min(a, sh[q][0]); b = min(b, sh[q][1]); c = min(c, sh[q][2]); d = min(d, sh[q][3]);
-  }
-  __syncthreads();
-}
-
 __global__ __launch_bounds__(256) void k_shade_radar32(ViewRec *__restrict__ views, const GeomSlot *slots, PosDev pd,
                                                        double *fit, const FrameRec *frames, double *overflow,
                                                        const double *params, int *counters) {
   ViewRec &V = views[blockIdx.y];
-  if (V.type != VT_DELAY && V.type != VT_DOPPLER) return;
+  if ((V.type != VT_DELAY && V.type != VT_DOPPLER) || V.binned) return;
   const bool dd = V.type == VT_DELAY;
   const int n = V.n, w = 2 * n + 1;
   const int tx = (w + TILE - 1) / TILE;
@@ -196,46 +182,13 @@ __device__ void replay32_one(ViewRec *views, const GeomSlot *slots, const PosDev
   if (S.ncand > STALE_CAND) atomicOr(pd.error, 2);
   int c[STALE_CAND];
   for (int q = 0; q < nc; q++) c[q] = S.cand[q];
-  for (int a = 1; a < nc; a++) {
-    int v = c[a], b = a - 1;
-    while (b >= 0 && c[b] > v) { c[b + 1] = c[b]; b--; }
-    c[b + 1] = v;
-  }
-  double zz = -1.0e20, cosa = 0.0;
-  int fac = -1;
-  for (int q = 0; q < nc; q++) {
-    if (q > 0 && c[q] == c[q - 1]) continue;
-    FacetProj P;
-    project_facet(V, G, c[q], 0, P);
-    if (!P.front) continue;
-    double s, t, z;
-    if (!pixel_hit(P, V.kmpp, i, j, s, t, z)) continue;
-    if (!((z > zz) || (fac < 0))) continue;
-    zz = z;
-    double nz, cb;
-    shade64(V, G, c[q], 0, V.smooth, s, t, nz, cb);
-    if (nz > 0.0) cosa = nz;
-    fac = c[q];
-  }
+  sort_candidates(c, nc);
+  double zz, cosa;
+  replay_owner64(V, G, c, nc, i, j, zz, cosa);
   pd.stale_idx[V.pos_off + S.pix] = -1;
   if (!(cosa > 0.0)) return;
-  RadarPix R;
-  const double *pl = params + V.pb + V.pb_law;
-  radar_pixel(V, pl, i, j, zz, (float)cosa, R);
-  radar_scatter_global(V, R, fit + frames[V.frame_id].off, overflow + (long long)vi * 2000);
-  // limits: a replayed pixel is lit like any other
-  const bool dd = V.type == VT_DELAY;
-  if (dd) {
-    if (V.D0 + R.id0 < V.idellim[0]) atomicMin(&V.idellim[0], V.D0 + R.id0);
-    if (V.D0 + R.id1 > V.idellim[1]) atomicMax(&V.idellim[1], V.D0 + R.id1);
-    atomic_min_d(&V.dellim[0], (double)V.D0 + (double)R.delrel);
-    atomic_min_d(&V.dellim[1], -((double)V.D0 + (double)R.delrel));
-  }
-  if (V.P0 + R.ip0 < V.idoplim[0]) atomicMin(&V.idoplim[0], V.P0 + R.ip0);
-  if (V.P0 + R.ip1 > V.idoplim[1]) atomicMax(&V.idoplim[1], V.P0 + R.ip1);
-  atomic_min_d(&V.doplim[0], (double)V.P0 + (double)R.doprel);
-  atomic_min_d(&V.doplim[1], -((double)V.P0 + (double)R.doprel));
-  if (!dd && !R.inb) atomicOr(&V.any_overflow, 1);
+  replay_emit(V, params + V.pb + V.pb_law, i, j, zz, cosa, fit + frames[V.frame_id].off,
+              overflow + (long long)vi * 2000);
 }
 
 // ---------------------------------------------------------------------------

@@ -1,0 +1,482 @@
+// k_tile32.hip -- mixed method, binned raster: no z-buffer key reaches HBM.
+//
+//   k_block_bounds  per slot: bounding sphere and normal cone of every block of
+//                   BF consecutive facets (FP32, from the realized model)
+//   k_bin32         per view: each block's conservative pixel rectangle, entered
+//                   in the lists of the BT x BT tiles it touches
+//   k_tile32        per tile: rasterize the listed blocks into an LDS z-buffer
+//                   (the packed keys of the global path), shade the owners
+//                   straight from it into an FP64 LDS delay-Doppler window, and
+//                   replay the tile's edge and faces-away pixels in FP64 from an
+//                   LDS list of the facets covering them
+//
+// Facets wholly or partly outside the POS window (posbnd, posvis.c:172-196)
+// are handled by k_raster32 pass 0, which skips every block whose rectangle
+// lies inside the window.  A tile whose lists overflow falls back to testing
+// every block, or to the global replay records of k_raster32 pass 3 /
+// k_replay32; results do not depend on which route a pixel takes.
+#include "replay_dev.h"
+
+namespace shp {
+
+// ---------------------------------------------------------------------------
+// block bounds: one wave per block, lane = facet
+// ---------------------------------------------------------------------------
+__device__ inline float wsumf(float v) { for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64); return v; }
+
+__global__ __launch_bounds__(256) void k_block_bounds(GeomSlot *slots, int nslots) {
+  const GeomSlot &G = slots[blockIdx.y];
+  const int nb = (G.nf + BF - 1) / BF;
+  const int b = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (b >= nb) return;  // whole wave
+  const int f = b * BF + (threadIdx.x & 63);
+  const bool ok = f < G.nf;
+  float4 va = {0, 0, 0, 0}, vb = va, vc = va, fn = va;
+  if (ok) {
+    const int *fv = G.fv + 3 * f;
+    va = G.x32[fv[0]]; vb = G.x32[fv[1]]; vc = G.x32[fv[2]];
+    fn = G.fn32[f];
+  }
+  float lo[3], hi[3];
+  {
+    float a[3] = {va.x, va.y, va.z}, bb[3] = {vb.x, vb.y, vb.z}, c[3] = {vc.x, vc.y, vc.z};
+    for (int k = 0; k < 3; k++) {
+      lo[k] = ok ? fminf(a[k], fminf(bb[k], c[k])) : 1.0e30f;
+      hi[k] = ok ? fmaxf(a[k], fmaxf(bb[k], c[k])) : -1.0e30f;
+      lo[k] = wminf(lo[k]);
+      hi[k] = wmaxf(hi[k]);
+    }
+  }
+  float cx = 0.5f * (lo[0] + hi[0]), cy = 0.5f * (lo[1] + hi[1]), cz = 0.5f * (lo[2] + hi[2]);
+  float r2 = 0.0f;
+  if (ok) {
+    float d0x = va.x - cx, d0y = va.y - cy, d0z = va.z - cz;
+    float d1x = vb.x - cx, d1y = vb.y - cy, d1z = vb.z - cz;
+    float d2x = vc.x - cx, d2y = vc.y - cy, d2z = vc.z - cz;
+    r2 = fmaxf(d0x * d0x + d0y * d0y + d0z * d0z,
+               fmaxf(d1x * d1x + d1y * d1y + d1z * d1z, d2x * d2x + d2y * d2y + d2z * d2z));
+  }
+  r2 = wmaxf(r2);
+  float r = sqrtf(r2) * 1.00001f + 1.0e-9f;
+  float ax = wsumf(fn.x), ay = wsumf(fn.y), az = wsumf(fn.z);
+  float al = sqrtf(ax * ax + ay * ay + az * az);
+  float cmin = -1.0f;  // no cone: the block may face every way
+  if (al > 1.0e-6f) {
+    ax /= al; ay /= al; az /= al;
+    float c = ok ? ax * fn.x + ay * fn.y + az * fn.z : 1.0f;
+    cmin = wminf(c);
+    cmin = fmaxf(cmin - 1.0e-5f, -1.0f);  // rounding of the FP32 normals
+  }
+  if ((threadIdx.x & 63) == 0) {
+    G.blk[2 * b] = make_float4(cx, cy, cz, r);
+    G.blk[2 * b + 1] = make_float4(ax, ay, az, cmin);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// binning: thread per block.  A workgroup's blocks are neighbours on the model,
+// so entries are counted in LDS over the workgroup's tile rectangle and each
+// touched tile's list is reserved with one global atomic.
+// ---------------------------------------------------------------------------
+constexpr int BIN_LDS = 1024;  // tile counters of a workgroup rectangle held in LDS
+
+// a, c: workgroup minima; b, d: workgroup maxima
+__device__ inline void blk_minmax4(int &a, int &b, int &c, int &d, int (*sh)[4]) {
+  int nb = -b, nd = -d;
+  blk_min4(a, nb, c, nd, sh);
+  b = -nb; d = -nd;
+}
+
+__global__ __launch_bounds__(256) void k_bin32(const ViewRec *__restrict__ views, const GeomSlot *slots,
+                                               int *bin_cnt, unsigned short *bin_list) {
+  const ViewRec &V = views[blockIdx.y];
+  if (!V.binned) return;
+  const GeomSlot &G = slots[V.slot];
+  const int nb = (G.nf + BF - 1) / BF;
+  if ((int)blockIdx.x * 256 >= nb) return;
+  const int b = blockIdx.x * 256 + threadIdx.x;
+  const int n = V.n, ntx = V.bin_ntx;
+  int tx0 = 0x7fffffff, tx1 = -1, ty0 = 0x7fffffff, ty1 = -1;
+  int i0, i1, j0, j1;
+  if (b < nb && block_rect(V, G, b, i0, i1, j0, j1)) {
+    i0 = max(i0, -n); i1 = min(i1, n); j0 = max(j0, -n); j1 = min(j1, n);
+    if (i0 <= i1 && j0 <= j1) {
+      tx0 = (i0 + n) / BT; tx1 = (i1 + n) / BT;
+      ty0 = (j0 + n) / BT; ty1 = (j1 + n) / BT;
+    }
+  }
+  __shared__ int shr[4][4];
+  __shared__ int lc[BIN_LDS];
+  int rx0 = tx0, rx1 = tx1, ry0 = ty0, ry1 = ty1;
+  blk_minmax4(rx0, rx1, ry0, ry1, shr);
+  if (rx1 < rx0) return;  // nothing of this workgroup lands in the window
+  const int rw = ry1 - ry0 + 1, rn = (rx1 - rx0 + 1) * rw;
+  const long long base = V.bin_off;
+  if (rn > BIN_LDS) {
+    // a scattered workgroup (unordered facets): one global atomic per entry
+    for (int x = tx0; x <= tx1; x++)
+      for (int y = ty0; y <= ty1; y++) {
+        long long t = base + (long long)x * ntx + y;
+        int q = atomicAdd(&bin_cnt[t], 1);
+        if (q < BIN_CAP) bin_list[t * BIN_CAP + q] = (unsigned short)b;
+      }
+    return;
+  }
+  for (int q = threadIdx.x; q < rn; q += blockDim.x) lc[q] = 0;
+  __syncthreads();
+  for (int x = tx0; x <= tx1; x++)
+    for (int y = ty0; y <= ty1; y++) atomicAdd(&lc[(x - rx0) * rw + (y - ry0)], 1);
+  __syncthreads();
+  for (int q = threadIdx.x; q < rn; q += blockDim.x) {
+    int c = lc[q];
+    if (c) lc[q] = atomicAdd(&bin_cnt[base + (long long)(rx0 + q / rw) * ntx + (ry0 + q % rw)], c);
+  }
+  __syncthreads();
+  for (int x = tx0; x <= tx1; x++)
+    for (int y = ty0; y <= ty1; y++) {
+      long long t = base + (long long)x * ntx + y;
+      int q = atomicAdd(&lc[(x - rx0) * rw + (y - ry0)], 1);
+      if (q < BIN_CAP) bin_list[t * BIN_CAP + q] = (unsigned short)b;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// the tile kernel
+// ---------------------------------------------------------------------------
+// Append event e for the lanes of the calling wave that need it: one LDS
+// atomic per wave.  Every active lane of the wave must call it.
+__device__ inline void ev_append(bool need, unsigned int e, unsigned int *ev, int *nev) {
+  unsigned long long mask = __ballot(need);
+  if (!mask) return;
+  int lane = threadIdx.x & 63, leader = __ffsll((long long)mask) - 1, base = 0;
+  if (lane == leader) base = atomicAdd(nev, __popcll(mask));
+  base = __shfl(base, leader, 64);
+  if (need) {
+    int q = base + __popcll(mask & ((1ULL << lane) - 1ULL));
+    if (q < EV_CAP) ev[q] = e;
+  }
+}
+
+// Record pixel (i, j) (local index lp) for the FP64 replay: in the tile's LDS
+// list while its events are complete and the list has room, else in the
+// global replay records (k_raster32 pass 3 collects its candidates).
+__device__ inline void tile_stale(const ViewRec &V, const PosDev &pd, bool ev_ok, int lp, int i, int j,
+                                  unsigned char *flag, short *spix, int *nts) {
+  if (ev_ok) {
+    int s = atomicAdd(nts, 1);
+    if (s < TS_CAP) {
+      spix[s] = (short)lp;
+      flag[lp] = (unsigned char)(0x80 | (s << 1) | (flag[lp] & 1));
+      return;
+    }
+  }
+  int si = atomicAdd(pd.nstale, 1);
+  if (si >= pd.stale_cap) {
+    atomicOr(pd.error, 1);
+    return;
+  }
+  stale_init(V, pd, si, blockIdx.y, (i + V.n) * (2 * V.n + 1) + (j + V.n), i, j);
+}
+
+// Pixel (a, c) of the tile taken by this thread in its m-th turn: each wave
+// takes an 8 x 8 grid of pixels at stride 4 across the whole tile, so its
+// lanes spread over the tile's range of delay rows and Doppler bins and
+// their LDS window atomics collide far less than along one pixel row.
+static_assert(BT == 32 && BT_PIX == 4 * 256, "tile_pixel assumes 32x32 tiles, 4 pixels per thread");
+__device__ inline void tile_pixel(int m, int &a, int &c) {
+  const int g = (threadIdx.x >> 6) * (BT_PIX / 256) + m, l = threadIdx.x & 63;
+  a = (l >> 3) * 4 + (g >> 2);
+  c = (l & 7) * 4 + (g & 3);
+}
+
+// Six workgroups per CU (LDS 26 KiB each): the register cap of six waves per
+// SIMD costs a few spills but hides more of the gathers' latency (measured).
+__global__ __launch_bounds__(256, 6) void k_tile32(ViewRec *__restrict__ views, const GeomSlot *slots, PosDev pd,
+                                                int *bin_cnt, const unsigned short *bin_list, double *fit,
+                                                const FrameRec *frames, double *overflow, const double *params,
+                                                int *counters) {
+  ViewRec &V = views[blockIdx.y];
+  if (!V.binned) return;
+  const int n = V.n, ntx = V.bin_ntx;
+  // each XCD takes a contiguous eighth of the view's tiles: neighbouring
+  // tiles share blocks, whose vertices then stay in one L2
+  const int bx = xcd_block(blockIdx.x, gridDim.x);
+  if (bx >= ntx * ntx) return;
+  const long long bin = V.bin_off + bx;
+  const int cnt = bin_cnt[bin];
+  if (cnt == 0) return;
+  const int i0 = (bx / ntx) * BT - n, j0 = (bx % ntx) * BT - n;
+  const GeomSlot &G = slots[V.slot];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+
+  __shared__ unsigned long long key[BT_PIX];
+  __shared__ unsigned char flag[BT_PIX];  // bit 0: marginal; bit 7: LDS replay slot in bits 1-6
+  __shared__ unsigned int ev[EV_CAP];
+  __shared__ unsigned short blist[BIN_CAP];
+  __shared__ double wbuf[WIN_T];          // delay-Doppler window, then replay candidates
+  __shared__ short spix[TS_CAP];
+  __shared__ int scnt[TS_CAP];
+  __shared__ int nev, nts;
+  __shared__ int shw[4][4];
+  __shared__ int shr[4][8];
+
+  const bool full = cnt > BIN_CAP;
+  if (!full)
+    for (int q = threadIdx.x; q < cnt; q += blockDim.x) blist[q] = bin_list[bin * BIN_CAP + q];
+  for (int q = threadIdx.x; q < BT_PIX; q += blockDim.x) { key[q] = 0ULL; flag[q] = 0; }
+  if (threadIdx.x == 0) { nev = 0; nts = 0; }
+  __syncthreads();
+
+  // ---- raster: one wave per block, lane = facet ----
+  int xl0 = n, xl1 = -n, yl0 = n, yl1 = -n;
+  {
+    const int nb = (G.nf + BF - 1) / BF;
+    const int nbl = full ? nb : cnt;
+    for (int k = wv; k < nbl; k += 4) {
+      int b = k;
+      if (full) {  // lists overflowed: test every block (wave-uniform)
+        int r0, r1, c0, c1;
+        if (!block_rect(V, G, b, r0, r1, c0, c1) || r0 > i0 + BT - 1 || r1 < i0 || c0 > j0 + BT - 1 || c1 < j0)
+          continue;
+      } else {
+        b = blist[k];
+      }
+      const int f = b * BF + lane;
+      Proj32 P;
+      float nz;
+      // nearly edge-on facets are replay candidates only: the FP64 replay
+      // tests facing itself
+      if (!(f < G.nf && project32n(V, G, f, V.oa32, -FRONT_TOL, P, nz))) continue;
+      const bool front = nz > 0.0f;
+      float imin_d = floorf(fminf(P.x0, fminf(P.x1, P.x2)) + 0.5f);
+      float imax_d = floorf(fmaxf(P.x0, fmaxf(P.x1, P.x2)) + 0.5f);
+      float jmin_d = floorf(fminf(P.y0, fminf(P.y1, P.y2)) + 0.5f);
+      float jmax_d = floorf(fmaxf(P.y0, fmaxf(P.y1, P.y2)) + 0.5f);
+      int imin = (int)fmaxf(imin_d, -1.0e9f), imax = (int)fminf(imax_d, 1.0e9f);
+      int jmin = (int)fmaxf(jmin_d, -1.0e9f), jmax = (int)fminf(jmax_d, 1.0e9f);
+      // window-clipped box (the POS rectangle grows by it), then tile-clipped
+      const int wi1 = max(imin, -n), wi2 = min(imax, n), wj1 = max(jmin, -n), wj2 = min(jmax, n);
+      const int ti1 = max(wi1, i0), ti2 = min(wi2, i0 + BT - 1), tj1 = max(wj1, j0), tj2 = min(wj2, j0 + BT - 1);
+      if (ti1 > ti2 || tj1 > tj2) continue;
+      const int nc = (ti2 - ti1 + 1) * (tj2 - tj1 + 1);
+      bool hit = false;
+      for (int c = 0, i = ti1, j = tj1; c < nc; c++) {
+        float s, t;
+        float m = bary32(P, i, j, s, t);
+        const int lp = (i - i0) * BT + (j - j0);
+        const bool cov = m >= -AMB_TOL;
+        if (front && m >= AMB_TOL) {
+          atomicMax(&key[lp], (unsigned long long)pack_key(depth32(P, s, t), f));
+          hit = true;
+        } else if (front && cov) {
+          flag[lp] = 1;  // marginal: claimed by no facet, replayed in FP64
+          hit = true;
+        }
+        ev_append(cov, ((unsigned int)lp << EV_FACET_BITS) | (unsigned int)f, ev, &nev);
+        if (++j > tj2) { j = tj1; i++; }
+      }
+      if (hit) {
+        xl0 = min(xl0, wi1); xl1 = max(xl1, wi2);
+        yl0 = min(yl0, wj1); yl1 = max(yl1, wj2);
+      }
+    }
+  }
+  __syncthreads();
+
+  // ---- A: keys, marginal pixels and the footprint window ----
+  const bool dd = V.type == VT_DELAY;
+  const int width = V.sinc2width;
+  const bool ev_ok = nev <= EV_CAP;
+  constexpr int TP = BT_PIX / 256;
+  unsigned long long kk[TP];
+  int lo_d = 0x7fffffff, hi_d = -0x7fffffff, lo_p = 0x7fffffff, hi_p = -0x7fffffff, ncl = 0;
+#pragma unroll
+  for (int m = 0; m < TP; m++) {
+    int a, c;
+    tile_pixel(m, a, c);
+    const int i = i0 + a, j = j0 + c, lp = a * BT + c;
+    unsigned long long k = 0ULL;
+    if (i <= n && j <= n) {
+      k = key[lp];
+      if (k) ncl++;
+      if (flag[lp] & 1) {
+        k = 0ULL;
+        tile_stale(V, pd, ev_ok, lp, i, j, flag, spix, &nts);
+      }
+      if (k) {
+        double dop64 = V.ax * (double)i + V.ay * (double)j + V.dopshift;
+        lo_p = min(lo_p, (int)floor(dop64 - V.dopdiff_max + 1 - width / 2.0) - V.P0);
+        hi_p = max(hi_p, (int)floor(dop64 + V.dopdiff_max + width / 2.0) - V.P0);
+        if (dd) {
+          double del64 = (double)key_z(k) * V.delfact + V.delshift;
+          lo_d = min(lo_d, (int)floor(del64 - V.const1) + 1 - V.D0);
+          hi_d = max(hi_d, (int)ceil(del64 + V.const1) - 1 - V.D0);
+        }
+      }
+    }
+    kk[m] = k;
+  }
+  {
+    int nhp = -hi_p, nhd = -hi_d;
+    blk_min4(lo_p, nhp, lo_d, nhd, shw);
+    hi_p = -nhp; hi_d = -nhd;
+    if (!dd) lo_d = hi_d = 0;
+  }
+  const bool any = lo_p != 0x7fffffff;
+  const int lim_d0 = lo_d, lim_d1 = hi_d, lim_p0 = lo_p, lim_p1 = hi_p;
+  if (any) {  // one bin of margin: phase B recomputes the same floors
+    lo_p--; hi_p++;
+    if (dd) { lo_d--; hi_d++; }
+  }
+  const int wd = hi_d - lo_d + 1, wp = hi_p - lo_p + 1;
+  const bool use_lds = any && (long long)wd * wp <= WIN_T;
+  double *win = wbuf;
+  if (use_lds) {
+    for (int q = threadIdx.x; q < wd * wp; q += blockDim.x) win[q] = 0.0;
+    __syncthreads();
+  }
+
+  // ---- B: shade, footprint, accumulate ----
+  const double *pl = params + V.pb + V.pb_law;
+  double *img = fit + frames[V.frame_id].off;
+  double *ov = overflow + (long long)blockIdx.y * 2000;
+  int anyo = 0;
+  for (int m = 0; m < TP; m++) {
+    const unsigned long long k = kk[m];
+    if (!k) continue;
+    int a, c;
+    tile_pixel(m, a, c);
+    const int i = i0 + a, j = j0 + c;
+    const int f = key_facet(k);
+    Proj32 P;
+    corners32(V, G, f, V.oa32, P);
+    float s, t;
+    bary32(P, i, j, s, t);  // same arithmetic as the raster
+    float nx, ny, nz;
+    normal32(G, f, V.oa32, V.smooth, s, t, nx, ny, nz);
+    if (!(nz > 0.0f)) {
+      // cos(scattering) of an owner facing away is inherited from an earlier
+      // claimant: FP64 replay
+      tile_stale(V, pd, ev_ok, a * BT + c, i, j, flag, spix, &nts);
+      continue;
+    }
+    RadarPix R;
+    radar_pixel(V, pl, i, j, (double)key_z(k), nz, R);
+    if (!R.inb) anyo = 1;
+    if (R.wide) { atomicAdd(&counters[3], 1); continue; }
+    if (use_lds && R.inb && R.sinc1) {
+      // Lanes whose footprints start at the same bin would add into the same
+      // LDS cells in lockstep, and same-address FP64 atomics serialize: each
+      // lane walks its Doppler bins from its own starting offset instead,
+      // recomputing each bin's weight (same value as pw[b]).
+      const int np = R.ip1 - R.ip0 + 1, nd = dd ? R.id1 - R.id0 + 1 : 1;
+      int b = lane & 15;
+      while (b >= np) b -= np;
+      double *col = win + (R.ip0 - lo_p) + (dd ? (R.id0 - lo_d) * wp : 0);
+      for (int st = 0; st < np; st++) {
+        const float pwb = dop_weight(R.pfr, R.pm0, R.s2, b);
+#pragma unroll
+        for (int aa = 0; aa < DW; aa++) {
+          if (aa < nd) {
+            float ad = dd ? R.amp * R.dw[aa] : R.amp;
+            atomicAdd(&col[aa * wp + b], (double)(ad * pwb));
+          }
+        }
+        if (++b == np) b = 0;
+      }
+    } else if (use_lds && R.inb) {
+      int np = R.ip1 - R.ip0 + 1, nd = dd ? R.id1 - R.id0 + 1 : 1;
+#pragma unroll
+      for (int aa = 0; aa < DW; aa++) {
+        if (aa < nd) {
+          float ad = dd ? R.amp * R.dw[aa] : R.amp;
+          double *row = win + (dd ? (R.id0 + aa - lo_d) * wp : 0) + (R.ip0 - lo_p);
+#pragma unroll
+          for (int bb = 0; bb < PW; bb++)
+            if (bb < np) atomicAdd(&row[bb], (double)(ad * R.pw[bb]));
+        }
+      }
+    } else {
+      radar_scatter_global(V, R, img, ov);
+    }
+  }
+
+  // ---- C: flush the window ----
+  __syncthreads();
+  if (use_lds) {
+    const int ndop = V.ndop;
+    for (int q = threadIdx.x; q < wd * wp; q += blockDim.x) {
+      double v = win[q];
+      if (v != 0.0) {
+        int r = q / wp, c = q % wp;
+        int idel = V.D0 + lo_d + r, idop = V.P0 + lo_p + c;
+        if (dd) atomicAdd(&img[(long long)(idel - 1) * ndop + (idop - 1)], v);
+        else atomicAdd(&img[idop - 1], v);
+      }
+    }
+  }
+  __syncthreads();
+
+  // ---- D: FP64 replay of the tile's recorded pixels ----
+  const int nt = min(nts, TS_CAP);
+  if (nt > 0) {
+    int (*cand)[STALE_CAND] = reinterpret_cast<int (*)[STALE_CAND]>(wbuf);
+    for (int q = threadIdx.x; q < nt; q += blockDim.x) scnt[q] = 0;
+    __syncthreads();
+    const int ne = min(nev, EV_CAP);
+    for (int e = threadIdx.x; e < ne; e += blockDim.x) {
+      const unsigned int x = ev[e];
+      const unsigned char fl = flag[x >> EV_FACET_BITS];
+      if (fl & 0x80) {
+        const int s = (fl >> 1) & 63;
+        const int q = atomicAdd(&scnt[s], 1);
+        if (q < STALE_CAND) cand[s][q] = (int)(x & ((1u << EV_FACET_BITS) - 1u));
+      }
+    }
+    __syncthreads();
+    if ((int)threadIdx.x < nt) {
+      const int s = threadIdx.x, lp = spix[s];
+      int nc = scnt[s];
+      if (nc > STALE_CAND) { atomicOr(pd.error, 2); nc = STALE_CAND; }
+      int *c = cand[s];
+      sort_candidates(c, nc);
+      const int i = i0 + lp / BT, j = j0 + lp % BT;
+      double zz, cosa;
+      replay_owner64(V, G, c, nc, i, j, zz, cosa);
+      if (cosa > 0.0) replay_emit(V, pl, i, j, zz, cosa, img, ov);
+    }
+  }
+
+  // ---- E: rectangle, limits and claimed keys, once per workgroup ----
+  anyo = wmax(anyo);
+  ncl = wsum(ncl);
+  xl0 = wmin(xl0); xl1 = wmax(xl1); yl0 = wmin(yl0); yl1 = wmax(yl1);
+  if (lane == 0) {
+    shr[wv][0] = anyo; shr[wv][1] = ncl; shr[wv][2] = xl0; shr[wv][3] = xl1;
+    shr[wv][4] = yl0; shr[wv][5] = yl1;
+  }
+  __syncthreads();
+  if (threadIdx.x != 0) return;
+  bin_cnt[bin] = 0;  // ready for the next replay's binning
+  for (int q = 1; q < 4; q++) {
+    anyo |= shr[q][0]; ncl += shr[q][1];
+    xl0 = min(xl0, shr[q][2]); xl1 = max(xl1, shr[q][3]);
+    yl0 = min(yl0, shr[q][4]); yl1 = max(yl1, shr[q][5]);
+  }
+  if (nt) atomicAdd(&counters[2], nt);
+  if (ncl) atomicAdd(&V.nclaimed, ncl);
+  if (xl0 < n && xl0 < V.xlim[0]) atomicMin(&V.xlim[0], xl0);
+  if (xl1 > -n && xl1 > V.xlim[1]) atomicMax(&V.xlim[1], xl1);
+  if (yl0 < n && yl0 < V.ylim[0]) atomicMin(&V.ylim[0], yl0);
+  if (yl1 > -n && yl1 > V.ylim[1]) atomicMax(&V.ylim[1], yl1);
+  if (!dd && anyo && !V.any_overflow) atomicOr(&V.any_overflow, 1);
+  if (!any) return;
+  if (dd) {
+    if (V.D0 + lim_d0 < V.idellim[0]) atomicMin(&V.idellim[0], V.D0 + lim_d0);
+    if (V.D0 + lim_d1 > V.idellim[1]) atomicMax(&V.idellim[1], V.D0 + lim_d1);
+  }
+  if (V.P0 + lim_p0 < V.idoplim[0]) atomicMin(&V.idoplim[0], V.P0 + lim_p0);
+  if (V.P0 + lim_p1 > V.idoplim[1]) atomicMax(&V.idoplim[1], V.P0 + lim_p1);
+}
+
+}  // namespace shp

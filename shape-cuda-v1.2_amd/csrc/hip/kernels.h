@@ -13,6 +13,17 @@ constexpr int STALE_TILE = 8;   // stale-pixel bitmap granularity (pixels)
 constexpr int SYN_TILE = 16;    // FP64 synthesis: 16 x 16 POS pixels per workgroup
 inline unsigned syn_blocks(int n) { unsigned t = (2 * n + 1 + SYN_TILE - 1) / SYN_TILE; return t * t; }
 
+// Binned raster (k_tile32.hip): facets are binned by blocks of BF consecutive
+// facets into BT x BT pixel tiles of each view; one workgroup per tile.
+constexpr int BT = 32, BT_PIX = BT * BT;
+constexpr int BF = 64;           // facets per block: one wave
+constexpr int BIN_CAP = 128;     // block ids stored per tile (more: the tile tests every block)
+constexpr int BIN_MAX_NTX = 64;  // tiles per side of a binned view (POS width <= 2048)
+constexpr int EV_CAP = 2048;     // covering (pixel, facet) events kept per tile
+constexpr int TS_CAP = 32;       // pixels per tile replayed from those events
+constexpr int WIN_T = 1024;      // FP64 LDS delay-Doppler window of a tile (8 KiB)
+constexpr int EV_FACET_BITS = 22;  // event = local pixel << 22 | facet (facets < 2^22)
+
 // A pixel whose final owner has a back-facing smoothed normal; its
 // cos(scattering) is whatever an earlier claimant left (posvis.c:243-275).
 struct StaleRec {

@@ -28,6 +28,19 @@ __device__ inline int wsum(int v) { for (int o = 32; o > 0; o >>= 1) v += __shfl
 __device__ inline float wminf(float v) { for (int o = 32; o > 0; o >>= 1) v = fminf(v, __shfl_xor(v, o, 64)); return v; }
 __device__ inline float wmaxf(float v) { for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, 64)); return v; }
 
+// block-wide minimum of four ints at once (one barrier pair)
+__device__ inline void blk_min4(int &a, int &b, int &c, int &d, int (*sh)[4]) {
+  a = wmin(a); b = wmin(b); c = wmin(c); d = wmin(d);
+  int wv = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) { sh[wv][0] = a; sh[wv][1] = b; sh[wv][2] = c; sh[wv][3] = d; }
+  __syncthreads();
+  a = sh[0][0]; b = sh[0][1]; c = sh[0][2]; d = sh[0][3];
+  for (int q = 1; q < (int)(blockDim.x >> 6); q++) {
+    a = min(a, sh[q][0]); b = min(b, sh[q][1]); c = min(c, sh[q][2]); d = min(d, sh[q][3]);
+  }
+  __syncthreads();
+}
+
 __device__ inline void rot3f(const float m[9], float x, float y, float z, float &a, float &b, float &c) {
   a = m[0] * x + m[1] * y + m[2] * z;
   b = m[3] * x + m[4] * y + m[5] * z;
@@ -52,11 +65,46 @@ __device__ inline void corners32(const ViewRec &V, const GeomSlot &G, int f, con
 }
 
 constexpr float FRONT_TOL = 1.0e-4f;
+// as project32, also returning the facet normal's line-of-sight component
+__device__ inline bool project32n(const ViewRec &V, const GeomSlot &G, int f, const float *m, float nz_min,
+                                  Proj32 &P, float &nz) {
+  float4 fn = G.fn32[f];
+  nz = m[6] * fn.x + m[7] * fn.y + m[8] * fn.z;
+  if (!(nz > nz_min)) return false;
+  corners32(V, G, f, m, P);
+  return true;
+}
 __device__ inline bool project32(const ViewRec &V, const GeomSlot &G, int f, const float *m, float nz_min, Proj32 &P) {
   float4 fn = G.fn32[f];
   float nz = m[6] * fn.x + m[7] * fn.y + m[8] * fn.z;
   if (!(nz > nz_min)) return false;  // back-facing (posvis.c:152)
   corners32(V, G, f, m, P);
+  return true;
+}
+
+// Conservative pixel rectangle [i0, i1] x [j0, j1] of bin block b in view V:
+// it contains the bounding box (posvis.c:172-180) of every facet of the
+// block, whose corners lie within the block's radius of its center, widened
+// for FP32 rounding.  Not clipped to the POS window.  False when every facet
+// of the block faces away by more than FRONT_TOL (normal cone).
+__device__ inline bool block_rect(const ViewRec &V, const GeomSlot &G, int b, int &i0, int &i1, int &j0, int &j1) {
+  float4 cr = G.blk[2 * b], ax = G.blk[2 * b + 1];
+  const float *m = V.oa32;
+  if (ax.w > -1.0f) {
+    // every normal is within acos(ax.w) of the axis, so its line-of-sight
+    // component is at most cos(angle(axis, sight) - acos(ax.w))
+    float cphi = m[6] * ax.x + m[7] * ax.y + m[8] * ax.z;
+    float d = acosf(fminf(fmaxf(cphi, -1.0f), 1.0f)) - acosf(fminf(fmaxf(ax.w, -1.0f), 1.0f));
+    if (d > 0.0f && cosf(d) < -2.0f * FRONT_TOL) return false;
+  }
+  float cx, cy, cz;
+  rot3f(m, cr.x, cr.y, cr.z, cx, cy, cz);
+  float s = V.inv_kmpp32, px = cx * s, py = cy * s;
+  float R = cr.w * s * 1.0001f + 0.01f + 1.0e-6f * (fabsf(px) + fabsf(py));
+  i0 = (int)fminf(fmaxf(floorf(px - R + 0.5f), -1.0e9f), 1.0e9f);
+  i1 = (int)fminf(fmaxf(floorf(px + R + 0.5f), -1.0e9f), 1.0e9f);
+  j0 = (int)fminf(fmaxf(floorf(py - R + 0.5f), -1.0e9f), 1.0e9f);
+  j1 = (int)fminf(fmaxf(floorf(py + R + 0.5f), -1.0e9f), 1.0e9f);
   return true;
 }
 
@@ -210,10 +258,21 @@ __device__ inline float radlaw32(const double *pl, int type, float c) {
 // weights and the amplitude of cross section per unit weight.
 struct RadarPix {
   bool inb, wide;
+  bool sinc1;        // single-sample sinc^2 Doppler response: pw[b] == dop_weight(b)
   float amp, delrel, doprel;
   int id0, id1, ip0, ip1;
+  float pfr, s2;     // fractional Doppler bin and sin^2(pi pfr) / pi^2
+  int pm0;           // Doppler coordinate - first bin = pfr + pm0
   float dw[DW], pw[PW];
 };
+
+// Doppler weight of bin b of a single-sample sinc^2 footprint: the expression
+// radar_pixel evaluates into pw[b], so both give the same value.
+__device__ inline float dop_weight(float pfr, int pm0, float s2, int b) {
+  float a = pfr + (float)(pm0 - b);
+  // hardware reciprocal (1 ulp): the weights are FP32 and normalized by their sum
+  return (fabsf(3.14159265358979f * a) > 1.0e-4f) ? s2 * __builtin_amdgcn_rcpf(a * a) : 1.0f;
+}
 
 __device__ inline void radar_pixel(const ViewRec &V, const double *pl, int i, int j, double z, float cose, RadarPix &R) {
   const bool dd = V.type == VT_DELAY;
@@ -236,16 +295,18 @@ __device__ inline void radar_pixel(const ViewRec &V, const double *pl, int i, in
   float codef = 1.0f;
   if (dd && V.codemethod == 0) codef = sinc2f((R.doprel + V.dcrel32) / (float)V.dopfftlen);
   float sump = 0.0f;
-  if (ns == 1 && np <= PW) {
+  R.pfr = pfr;
+  R.pm0 = pm0;
+  R.s2 = 0.0f;
+  R.sinc1 = ns == 1 && np >= 1 && np <= PW;
+  if (R.sinc1) {
     // sin(pi (x - k)) = +-sin(pi x): one sine serves every Doppler bin
     float sp = sinf(3.14159265358979f * pfr);
-    float s2 = sp * sp * (1.0f / (3.14159265358979f * 3.14159265358979f));
+    R.s2 = sp * sp * (1.0f / (3.14159265358979f * 3.14159265358979f));
 #pragma unroll
     for (int b = 0; b < PW; b++) {
       if (b < np) {
-        float a = pfr + (float)(pm0 - b);
-        // hardware reciprocal (1 ulp): the weights are FP32 and normalized by their sum
-        float v = (fabsf(3.14159265358979f * a) > 1.0e-4f) ? s2 * __builtin_amdgcn_rcpf(a * a) : 1.0f;
+        float v = dop_weight(pfr, pm0, R.s2, b);
         R.pw[b] = v;
         sump += v;
       }

@@ -438,7 +438,7 @@ class Scene:
 
 
 METHOD_FP64, METHOD_MIXED = 0, 1
-BATCH_GRAPH, BATCH_KEEP_FIT = 1, 2
+BATCH_GRAPH, BATCH_KEEP_FIT, BATCH_GLOBAL_ZBUF = 1, 2, 4
 
 
 class Batch:

@@ -113,3 +113,46 @@ def test_multi_model_batch(method, tol):
         assert abs(chi[e] - o_chi) <= tol * abs(o_chi), (e, m, t, chi[e], o_chi)
     # distinct models really differ
     assert len({round(float(c), 6) for c in chi[::2]}) == 4
+
+
+@pytest.mark.parametrize("config,over", [(2, dict(n_deldop=16)),
+                                         (1, dict(n_deldop=0, n_doppler=2, nbin_min=30, nbin_max=40)),
+                                         (3, dict(n_deldop=3, n_doppler=3, n_lightcurves=2, lc_points=30,
+                                                  lc_ncalc=10)),
+                                         # 81,920 facets on a 63x63 POS: tile lists overflow and the
+                                         # tiles test every block
+                                         (3, dict(n_deldop=3, n_doppler=0, n_lightcurves=0, pos_n=31,
+                                                  km_per_pixel=0.04))])
+def test_mixed_binned_matches_global_zbuf(config, over):
+    """The binned tile kernel (LDS z-buffer per 32x32 tile) and the global
+    z-buffer path build the same packed keys, replay the same pixels and do
+    the same FP32 work per pixel; only the order of FP64 accumulation
+    differs.  Chi-square, every frame's terms and the fit images agree to
+    1e-11, and the replayed and claimed pixel counts are equal."""
+    sc = H.make_scene_with_obs(config, **over)
+    ntr = [t for t in TRIALS[:5] if all(i < sc.nparams() for i, _ in t)]
+    evals = [(0, t) for t in ntr]
+    Bt = sh.Batch([sc], evals, method=sh.METHOD_MIXED, flags=sh.BATCH_KEEP_FIT)
+    Bg = sh.Batch([sc], evals, method=sh.METHOD_MIXED, flags=sh.BATCH_KEEP_FIT | sh.BATCH_GLOBAL_ZBUF)
+    ot, ct = Bt.run()
+    og, cg = Bg.run()
+    nt, ng = Bt.counters(), Bg.counters()
+    ft, fg = Bt.footprint(), Bg.footprint()
+    print("binned counters", nt, "global counters", ng, "claimed", ft["claimed"], fg["claimed"])
+    assert nt[1] == 0 and ng[1] == 0 and nt[3] == 0 and ng[3] == 0
+    assert ng[2] == 0                      # the global path replays through the list only
+    if not any(sc.set_type(s) == sh.LGHTCRV for s in range(sc.nsets)):
+        # (lightcurve replays reuse counter 0 after the radar replay)
+        assert nt[0] + nt[2] == ng[0]      # the same pixels are replayed
+    assert ft["claimed"] == fg["claimed"]  # the same keys are claimed
+    assert np.allclose(ot, og, rtol=1e-11, atol=0), (ot, og)
+    assert np.allclose(ct, cg, rtol=1e-11, atol=0), (ct, cg)
+    for e in range(len(evals)):
+        tt, tg = Bt.frame_terms(e), Bg.frame_terms(e)
+        assert np.allclose(tt, tg, rtol=1e-11, atol=1e-300), e
+    for q, (s, f) in enumerate(sc.all_frames()):
+        if f < 0 or sc.set_type(s) == sh.LGHTCRV:
+            continue
+        shape = sc.image(s, f, 0).shape
+        it, ig = Bt.fit(0, q, shape), Bg.fit(0, q, shape)
+        assert np.max(np.abs(it - ig)) <= 1e-11 * max(np.max(np.abs(ig)), 1e-300), q
