@@ -271,7 +271,7 @@ def main():
                       "evals_per_replay": a.evals, "method": "mixed", "graph": True},
            "replay_ms_p50": round(float(np.percentile(lat_m, 50)), 4),
            "replay_ms_p99": round(float(np.percentile(lat_m, 99)), 4),
-           "replayed_pixels": cnt[0] + cnt[2], "claimed_keys": fpm["claimed"]}
+           "replayed_pixels": cnt[0], "claimed_keys": fpm["claimed"]}
     if a.mixed_only:
         print(json.dumps(out), flush=True)
         return

@@ -129,9 +129,10 @@ int shape_batch_fit(shape_batch *b, int e, int frame, double *out);
 /* per frame of evaluation e: o2, m2, om, overflow_o2, overflow_m2, cal, chi2,
    badradar_logfactor (8 doubles per frame, set/frame order) */
 int shape_batch_frame_terms(shape_batch *b, int e, double *out);
-/* [0] pixels replayed through the global replay list, [1] error flags,
-   [2] mixed: pixels replayed inside their tile, [3] mixed: footprints too wide
-   for the register window */
+/* [0] pixels replayed in FP64, [1] error flags, [2] mixed: replayed pixels
+   whose candidate facets a second raster pass collected (the rest come from
+   their tile's own list), [3] mixed: footprints too wide for the register
+   window */
 int shape_batch_counters(shape_batch *b, int *out4);
 /* nevals, nviews, nframes, geometry slots, POS pixels, fit cells, max facets, max POS pixels */
 int shape_batch_sizes(shape_batch *b, long long *out8);

@@ -61,6 +61,7 @@ struct GeomSlot {
   float4 *x32, *vn32;    // nv
   float4 *fn32;          // nf
   float4 *blk;           // mixed method: per 64-facet block, {center, radius} and {cone axis, min cos}
+  float4 *fb;            // mixed method: per facet, bounding sphere {center, radius}
   unsigned char *seen;   // nf
 };
 

@@ -546,6 +546,7 @@ int Engine::build(struct par_t *par, struct mod_t **mods, struct dat_t **dats, i
       int nblk = (G.nf + BF - 1) / BF;
       max_nblk = std::max(max_nblk, nblk);
       if ((rc = alloc((void **)&G.blk, sizeof(float4) * 2 * (size_t)nblk))) return rc;
+      if ((rc = alloc((void **)&G.fb, sizeof(float4) * (size_t)G.nf))) return rc;
     }
     SHP_CHECK(hipMemset(G.seen, 0, G.nf));
     if (opt.host_geometry) {
@@ -661,6 +662,7 @@ int Engine::build(struct par_t *par, struct mod_t **mods, struct dat_t **dats, i
   if ((rc = alloc((void **)&pd.stale, sizeof(StaleRec) * (size_t)pd.stale_cap))) return rc;
   if ((rc = alloc((void **)&pd.nstale, sizeof(int) * 4))) return rc;
   pd.error = pd.nstale + 1;
+  pd.ncollect = pd.nstale + 2;
   SHP_CHECK(hipMemset(pd.nstale, 0, sizeof(int) * 4));
   hipLaunchKernelGGL(k_frame_o2, dim3(std::max(1, nframes)), dim3(256), 0, stream, d_frames, nframes, d_obs, d_oov);
   SHP_CHECK(hipStreamSynchronize(stream));

@@ -56,7 +56,7 @@ __global__ __launch_bounds__(256) void k_raster32(ViewRec *__restrict__ views, c
   ViewRec &V = views[blockIdx.y];
   if (V.type != VT_DELAY && V.type != VT_DOPPLER) return;  // lightcurve views take the FP64 kernels
   if (src && !V.bistatic) return;
-  if (pass == 3 && *pd.nstale == 0) return;
+  if (pass == 3 && *pd.ncollect == 0) return;
   if ((pass == 0 && !V.binned) || (pass == 1 && V.binned)) return;
   const GeomSlot &G = slots[V.slot];
   // passes 0 and 1 walk every facet of every view: XCD-local eighths of the

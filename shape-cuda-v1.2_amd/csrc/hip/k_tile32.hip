@@ -7,15 +7,16 @@
 //   k_tile32        per tile: rasterize the listed blocks into an LDS z-buffer
 //                   (the packed keys of the global path), shade the owners
 //                   straight from it into an FP64 LDS delay-Doppler window, and
-//                   replay the tile's edge and faces-away pixels in FP64 from an
-//                   LDS list of the facets covering them
+//                   hand the tile's edge and faces-away pixels, with the facets
+//                   covering them (an LDS list of covering events), to the
+//                   FP64 replay (k_replay32)
 //
 // Facets wholly or partly outside the POS window (posbnd, posvis.c:172-196)
 // are handled by k_raster32 pass 0, which skips every block whose rectangle
 // lies inside the window.  A tile whose lists overflow falls back to testing
 // every block, or to the global replay records of k_raster32 pass 3 /
 // k_replay32; results do not depend on which route a pixel takes.
-#include "replay_dev.h"
+#include "mixed_dev.h"
 
 namespace shp {
 
@@ -55,6 +56,14 @@ __global__ __launch_bounds__(256) void k_block_bounds(GeomSlot *slots, int nslot
     float d2x = vc.x - cx, d2y = vc.y - cy, d2z = vc.z - cz;
     r2 = fmaxf(d0x * d0x + d0y * d0y + d0z * d0z,
                fmaxf(d1x * d1x + d1y * d1y + d1z * d1z, d2x * d2x + d2y * d2y + d2z * d2z));
+  }
+  if (ok) {  // the facet's own sphere: centroid and farthest corner
+    float fx = (va.x + vb.x + vc.x) * (1.0f / 3.0f), fy = (va.y + vb.y + vc.y) * (1.0f / 3.0f),
+          fz = (va.z + vb.z + vc.z) * (1.0f / 3.0f);
+    float e0 = (va.x - fx) * (va.x - fx) + (va.y - fy) * (va.y - fy) + (va.z - fz) * (va.z - fz);
+    float e1 = (vb.x - fx) * (vb.x - fx) + (vb.y - fy) * (vb.y - fy) + (vb.z - fz) * (vb.z - fz);
+    float e2 = (vc.x - fx) * (vc.x - fx) + (vc.y - fy) * (vc.y - fy) + (vc.z - fz) * (vc.z - fz);
+    G.fb[f] = make_float4(fx, fy, fz, sqrtf(fmaxf(e0, fmaxf(e1, e2))) * 1.00001f + 1.0e-9f);
   }
   r2 = wmaxf(r2);
   float r = sqrtf(r2) * 1.00001f + 1.0e-9f;
@@ -143,18 +152,22 @@ __global__ __launch_bounds__(256) void k_bin32(const ViewRec *__restrict__ views
 // ---------------------------------------------------------------------------
 // the tile kernel
 // ---------------------------------------------------------------------------
-// Append event e for the lanes of the calling wave that need it: one LDS
-// atomic per wave.  Every active lane of the wave must call it.
-__device__ inline void ev_append(bool need, unsigned int e, unsigned int *ev, int *nev) {
+// Append value e to list[] (capacity cap; the count may run past it) for the
+// lanes of the calling wave that need it: one LDS atomic per wave.  Every
+// active lane of the wave must call it.
+__device__ inline void lds_append(bool need, unsigned int e, void *list, int *cnt, int cap) {
   unsigned long long mask = __ballot(need);
   if (!mask) return;
   int lane = threadIdx.x & 63, leader = __ffsll((long long)mask) - 1, base = 0;
-  if (lane == leader) base = atomicAdd(nev, __popcll(mask));
+  if (lane == leader) base = atomicAdd(cnt, __popcll(mask));
   base = __shfl(base, leader, 64);
   if (need) {
     int q = base + __popcll(mask & ((1ULL << lane) - 1ULL));
-    if (q < EV_CAP) ev[q] = e;
+    if (q < cap) static_cast<unsigned int *>(list)[q] = e;
   }
+}
+__device__ inline void ev_append(bool need, unsigned int e, unsigned int *ev, int *nev) {
+  lds_append(need, e, ev, nev, EV_CAP);
 }
 
 // Record pixel (i, j) (local index lp) for the FP64 replay: in the tile's LDS
@@ -171,6 +184,7 @@ __device__ inline void tile_stale(const ViewRec &V, const PosDev &pd, bool ev_ok
     }
   }
   int si = atomicAdd(pd.nstale, 1);
+  atomicAdd(pd.ncollect, 1);
   if (si >= pd.stale_cap) {
     atomicOr(pd.error, 1);
     return;
@@ -214,9 +228,11 @@ __global__ __launch_bounds__(256, 6) void k_tile32(ViewRec *__restrict__ views, 
   __shared__ unsigned int ev[EV_CAP];
   __shared__ unsigned short blist[BIN_CAP];
   __shared__ double wbuf[WIN_T];          // delay-Doppler window, then replay candidates
+  static_assert(TS_CAP * TS_CAND * sizeof(int) <= sizeof(wbuf), "candidates must fit the window buffer");
+  static_assert(TS_CAP <= 64, "replay slot must fit flag bits 1-6");
   __shared__ short spix[TS_CAP];
   __shared__ int scnt[TS_CAP];
-  __shared__ int nev, nts;
+  __shared__ int nev, nts, sbase, nfl;
   __shared__ int shw[4][4];
   __shared__ int shr[4][8];
 
@@ -227,61 +243,94 @@ __global__ __launch_bounds__(256, 6) void k_tile32(ViewRec *__restrict__ views, 
   if (threadIdx.x == 0) { nev = 0; nts = 0; }
   __syncthreads();
 
-  // ---- raster: one wave per block, lane = facet ----
+  // ---- raster, in rounds: (1) waves test the facets of up to FL_CAP / BF
+  // listed blocks against the tile and compact the survivors into an LDS
+  // list (the window buffer, free until phase A); (2) every thread
+  // rasterizes listed facets, so lanes stay busy in the pixel loops ----
   int xl0 = n, xl1 = -n, yl0 = n, yl1 = -n;
   {
+    constexpr int FL_CAP = WIN_T * 2;  // ints in the window buffer
+    int *flist = reinterpret_cast<int *>(wbuf);
     const int nb = (G.nf + BF - 1) / BF;
     const int nbl = full ? nb : cnt;
-    for (int k = wv; k < nbl; k += 4) {
-      int b = k;
-      if (full) {  // lists overflowed: test every block (wave-uniform)
-        int r0, r1, c0, c1;
-        if (!block_rect(V, G, b, r0, r1, c0, c1) || r0 > i0 + BT - 1 || r1 < i0 || c0 > j0 + BT - 1 || c1 < j0)
-          continue;
-      } else {
-        b = blist[k];
-      }
-      const int f = b * BF + lane;
-      Proj32 P;
-      float nz;
-      // nearly edge-on facets are replay candidates only: the FP64 replay
-      // tests facing itself
-      if (!(f < G.nf && project32n(V, G, f, V.oa32, -FRONT_TOL, P, nz))) continue;
-      const bool front = nz > 0.0f;
-      float imin_d = floorf(fminf(P.x0, fminf(P.x1, P.x2)) + 0.5f);
-      float imax_d = floorf(fmaxf(P.x0, fmaxf(P.x1, P.x2)) + 0.5f);
-      float jmin_d = floorf(fminf(P.y0, fminf(P.y1, P.y2)) + 0.5f);
-      float jmax_d = floorf(fmaxf(P.y0, fmaxf(P.y1, P.y2)) + 0.5f);
-      int imin = (int)fmaxf(imin_d, -1.0e9f), imax = (int)fminf(imax_d, 1.0e9f);
-      int jmin = (int)fmaxf(jmin_d, -1.0e9f), jmax = (int)fminf(jmax_d, 1.0e9f);
-      // window-clipped box (the POS rectangle grows by it), then tile-clipped
-      const int wi1 = max(imin, -n), wi2 = min(imax, n), wj1 = max(jmin, -n), wj2 = min(jmax, n);
-      const int ti1 = max(wi1, i0), ti2 = min(wi2, i0 + BT - 1), tj1 = max(wj1, j0), tj2 = min(wj2, j0 + BT - 1);
-      if (ti1 > ti2 || tj1 > tj2) continue;
-      const int nc = (ti2 - ti1 + 1) * (tj2 - tj1 + 1);
-      bool hit = false;
-      for (int c = 0, i = ti1, j = tj1; c < nc; c++) {
-        float s, t;
-        float m = bary32(P, i, j, s, t);
-        const int lp = (i - i0) * BT + (j - j0);
-        const bool cov = m >= -AMB_TOL;
-        if (front && m >= AMB_TOL) {
-          atomicMax(&key[lp], (unsigned long long)pack_key(depth32(P, s, t), f));
-          hit = true;
-        } else if (front && cov) {
-          flag[lp] = 1;  // marginal: claimed by no facet, replayed in FP64
-          hit = true;
+    const float *m = V.oa32;
+    const float sc = V.inv_kmpp32;
+    for (int kb = 0; kb < nbl; kb += FL_CAP / BF) {
+      const int kend = min(nbl, kb + FL_CAP / BF);
+      if (threadIdx.x == 0) nfl = 0;
+      __syncthreads();
+      for (int k = kb + wv; k < kend; k += 4) {
+        int b = k;
+        if (full) {  // lists overflowed: test every block (wave-uniform)
+          int r0, r1, c0, c1;
+          if (!block_rect(V, G, b, r0, r1, c0, c1) || r0 > i0 + BT - 1 || r1 < i0 || c0 > j0 + BT - 1 || c1 < j0)
+            continue;
+        } else {
+          b = blist[k];
         }
-        ev_append(cov, ((unsigned int)lp << EV_FACET_BITS) | (unsigned int)f, ev, &nev);
-        if (++j > tj2) { j = tj1; i++; }
+        const int f = b * BF + lane;
+        bool keep = false;
+        if (f < G.nf) {
+          // the facet's sphere against the tile, before any dependent gather:
+          // its bounding box (posvis.c:172-180) lies within the projected sphere
+          const float4 fb = G.fb[f];
+          const float px = (m[0] * fb.x + m[1] * fb.y + m[2] * fb.z) * sc;
+          const float py = (m[3] * fb.x + m[4] * fb.y + m[5] * fb.z) * sc;
+          const float R = fb.w * sc * 1.0001f + 0.01f + 1.0e-6f * (fabsf(px) + fabsf(py));
+          keep = !(floorf(px + R + 0.5f) < (float)i0 || floorf(px - R + 0.5f) > (float)(i0 + BT - 1) ||
+                   floorf(py + R + 0.5f) < (float)j0 || floorf(py - R + 0.5f) > (float)(j0 + BT - 1));
+          if (keep) {
+            // nearly edge-on facets are replay candidates only: the FP64
+            // replay tests facing itself (posvis.c:152)
+            const float4 fn = G.fn32[f];
+            keep = m[6] * fn.x + m[7] * fn.y + m[8] * fn.z > -FRONT_TOL;
+          }
+        }
+        lds_append(keep, f, flist, &nfl, FL_CAP);
       }
-      if (hit) {
-        xl0 = min(xl0, wi1); xl1 = max(xl1, wi2);
-        yl0 = min(yl0, wj1); yl1 = max(yl1, wj2);
+      __syncthreads();
+      const int nl = min(nfl, FL_CAP);
+      for (int q = threadIdx.x; q < nl; q += blockDim.x) {
+        const int f = flist[q];
+        Proj32 P;
+        float nz;
+        if (!project32n(V, G, f, V.oa32, -FRONT_TOL, P, nz)) continue;
+        const bool front = nz > 0.0f;
+        float imin_d = floorf(fminf(P.x0, fminf(P.x1, P.x2)) + 0.5f);
+        float imax_d = floorf(fmaxf(P.x0, fmaxf(P.x1, P.x2)) + 0.5f);
+        float jmin_d = floorf(fminf(P.y0, fminf(P.y1, P.y2)) + 0.5f);
+        float jmax_d = floorf(fmaxf(P.y0, fmaxf(P.y1, P.y2)) + 0.5f);
+        int imin = (int)fmaxf(imin_d, -1.0e9f), imax = (int)fminf(imax_d, 1.0e9f);
+        int jmin = (int)fmaxf(jmin_d, -1.0e9f), jmax = (int)fminf(jmax_d, 1.0e9f);
+        // window-clipped box (the POS rectangle grows by it), then tile-clipped
+        const int wi1 = max(imin, -n), wi2 = min(imax, n), wj1 = max(jmin, -n), wj2 = min(jmax, n);
+        const int ti1 = max(wi1, i0), ti2 = min(wi2, i0 + BT - 1), tj1 = max(wj1, j0), tj2 = min(wj2, j0 + BT - 1);
+        if (ti1 > ti2 || tj1 > tj2) continue;
+        const int nc = (ti2 - ti1 + 1) * (tj2 - tj1 + 1);
+        bool hit = false;
+        for (int c = 0, i = ti1, j = tj1; c < nc; c++) {
+          float s, t;
+          float mg = bary32(P, i, j, s, t);
+          const int lp = (i - i0) * BT + (j - j0);
+          const bool cov = mg >= -AMB_TOL;
+          if (front && mg >= AMB_TOL) {
+            atomicMax(&key[lp], (unsigned long long)pack_key(depth32(P, s, t), f));
+            hit = true;
+          } else if (front && cov) {
+            flag[lp] = 1;  // marginal: claimed by no facet, replayed in FP64
+            hit = true;
+          }
+          ev_append(cov, ((unsigned int)lp << EV_FACET_BITS) | (unsigned int)f, ev, &nev);
+          if (++j > tj2) { j = tj1; i++; }
+        }
+        if (hit) {
+          xl0 = min(xl0, wi1); xl1 = max(xl1, wi2);
+          yl0 = min(yl0, wj1); yl1 = max(yl1, wj2);
+        }
       }
+      __syncthreads();
     }
   }
-  __syncthreads();
 
   // ---- A: keys, marginal pixels and the footprint window ----
   const bool dd = V.type == VT_DELAY;
@@ -417,10 +466,12 @@ __global__ __launch_bounds__(256, 6) void k_tile32(ViewRec *__restrict__ views, 
   }
   __syncthreads();
 
-  // ---- D: FP64 replay of the tile's recorded pixels ----
+  // ---- D: the tile's recorded pixels join the global replay records, with
+  // the candidates gathered from the tile's events; k_replay32 replays them
+  // in FP64 (its own launch keeps this kernel's registers for the bulk) ----
   const int nt = min(nts, TS_CAP);
   if (nt > 0) {
-    int (*cand)[STALE_CAND] = reinterpret_cast<int (*)[STALE_CAND]>(wbuf);
+    int (*cand)[TS_CAND] = reinterpret_cast<int (*)[TS_CAND]>(wbuf);
     for (int q = threadIdx.x; q < nt; q += blockDim.x) scnt[q] = 0;
     __syncthreads();
     const int ne = min(nev, EV_CAP);
@@ -430,20 +481,41 @@ __global__ __launch_bounds__(256, 6) void k_tile32(ViewRec *__restrict__ views, 
       if (fl & 0x80) {
         const int s = (fl >> 1) & 63;
         const int q = atomicAdd(&scnt[s], 1);
-        if (q < STALE_CAND) cand[s][q] = (int)(x & ((1u << EV_FACET_BITS) - 1u));
+        if (q < TS_CAND) cand[s][q] = (int)(x & ((1u << EV_FACET_BITS) - 1u));
       }
     }
+    if (threadIdx.x == 0) {
+      int b0 = atomicAdd(pd.nstale, nt);
+      if (b0 + nt > pd.stale_cap) {
+        atomicOr(pd.error, 1);
+        b0 = -1;
+      }
+      sbase = b0;
+    }
     __syncthreads();
-    if ((int)threadIdx.x < nt) {
-      const int s = threadIdx.x, lp = spix[s];
-      int nc = scnt[s];
-      if (nc > STALE_CAND) { atomicOr(pd.error, 2); nc = STALE_CAND; }
-      int *c = cand[s];
-      sort_candidates(c, nc);
-      const int i = i0 + lp / BT, j = j0 + lp % BT;
-      double zz, cosa;
-      replay_owner64(V, G, c, nc, i, j, zz, cosa);
-      if (cosa > 0.0) replay_emit(V, pl, i, j, zz, cosa, img, ov);
+    const int base = sbase;
+    if (base >= 0) {
+      const int w = 2 * n + 1;
+      for (int s = wv; s < nt; s += 4) {
+        StaleRec &S = pd.stale[base + s];
+        const int nc = scnt[s];
+        const bool own = nc <= TS_CAND;  // else the global raster collects them (pass 3)
+        if (own)
+          for (int q = lane; q < nc; q += 64) S.cand[q] = cand[s][q];
+        if (lane == 0) {
+          const int lp = spix[s], i = i0 + lp / BT, j = j0 + lp % BT;
+          S.view = blockIdx.y;
+          S.pix = (i + n) * w + (j + n);
+          S.oldf = -1; S.oldz = -1.0e20; S.oldcose = 0.0; S.oldcosi = 0.0;
+          S.src = 0;
+          S.ncand = own ? nc : 0;
+          if (!own) {
+            pd.stale_idx[V.pos_off + S.pix] = base + s;
+            pd.stale_tiles[stale_tile(V, i, j)] = 1;
+            atomicAdd(pd.ncollect, 1);
+          }
+        }
+      }
     }
   }
 
@@ -463,7 +535,6 @@ __global__ __launch_bounds__(256, 6) void k_tile32(ViewRec *__restrict__ views, 
     xl0 = min(xl0, shr[q][2]); xl1 = max(xl1, shr[q][3]);
     yl0 = min(yl0, shr[q][4]); yl1 = max(yl1, shr[q][5]);
   }
-  if (nt) atomicAdd(&counters[2], nt);
   if (ncl) atomicAdd(&V.nclaimed, ncl);
   if (xl0 < n && xl0 < V.xlim[0]) atomicMin(&V.xlim[0], xl0);
   if (xl1 > -n && xl1 > V.xlim[1]) atomicMax(&V.xlim[1], xl1);

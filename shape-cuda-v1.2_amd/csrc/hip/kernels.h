@@ -20,7 +20,8 @@ constexpr int BF = 64;           // facets per block: one wave
 constexpr int BIN_CAP = 128;     // block ids stored per tile (more: the tile tests every block)
 constexpr int BIN_MAX_NTX = 64;  // tiles per side of a binned view (POS width <= 2048)
 constexpr int EV_CAP = 2048;     // covering (pixel, facet) events kept per tile
-constexpr int TS_CAP = 32;       // pixels per tile replayed from those events
+constexpr int TS_CAP = 64;       // replayed pixels per tile whose candidates come from those events
+constexpr int TS_CAND = 32;      // candidates kept per such pixel (more: the global raster collects them)
 constexpr int WIN_T = 1024;      // FP64 LDS delay-Doppler window of a tile (8 KiB)
 constexpr int EV_FACET_BITS = 22;  // event = local pixel << 22 | facet (facets < 2^22)
 
@@ -44,7 +45,8 @@ struct PosDev {
   int *stale_idx;            // per pixel: index into stale[], or -1
   unsigned char *stale_tiles;// mixed path: per view (at ViewRec::tile_off), 1 where an 8x8 tile holds a stale pixel
   StaleRec *stale;
-  int *nstale;
+  int *nstale;               // replay records in use
+  int *ncollect;             // mixed path: records whose candidates the global raster (pass 3) collects
   int stale_cap;
   int *error;                // bit flags: 1 stale list full, 2 too many candidates
 };

@@ -143,7 +143,10 @@ __device__ inline int stale_slot(const PosDev &pd, bool need) {
   unsigned long long mask = __ballot(need);
   if (!mask) return -1;
   int lane = threadIdx.x & 63, leader = __ffsll((long long)mask) - 1, base = 0;
-  if (lane == leader) base = atomicAdd(pd.nstale, __popcll(mask));
+  if (lane == leader) {
+    base = atomicAdd(pd.nstale, __popcll(mask));
+    atomicAdd(pd.ncollect, __popcll(mask));  // candidates come from the global raster
+  }
   base = __shfl(base, leader, 64);
   if (!need) return -1;
   int si = base + __popcll(mask & ((1ULL << lane) - 1ULL));

@@ -169,6 +169,7 @@ extern "C" int posvis(struct vertices_t *verts, double orbit_offset[3], struct p
   pd.stale = S.get<StaleRec>(pd.stale_cap);
   pd.nstale = S.get<int>(4);
   pd.error = pd.nstale + 1;
+  pd.ncollect = pd.nstale + 2;
   double *zz = S.get<double>(P), *cosa = S.get<double>(P);
   int *fac = S.get<int>(P), *bod = S.get<int>(P), *cmp = S.get<int>(P);
   double *cosb = (!src && pos->bistatic) ? S.get<double>(P) : nullptr;
