@@ -120,17 +120,7 @@ __global__ __launch_bounds__(256) void k_shade_radar32(ViewRec *__restrict__ vie
     if (!R.inb) anyo = 1;
     if (R.wide) { atomicAdd(&counters[3], 1); continue; }
     if (use_lds && R.inb) {
-      int np = R.ip1 - R.ip0 + 1, nd = dd ? R.id1 - R.id0 + 1 : 1;
-#pragma unroll
-      for (int a = 0; a < DW; a++) {
-        if (a < nd) {
-          float ad = dd ? R.amp * R.dw[a] : R.amp;
-          double *row = win + (dd ? (R.id0 + a - lo_d) * wp : 0) + (R.ip0 - lo_p);
-#pragma unroll
-          for (int b = 0; b < PW; b++)
-            if (b < np) atomicAdd(&row[b], (double)(ad * R.pw[b]));
-        }
-      }
+      radar_scatter_lds(V, R, win + (dd ? (R.id0 - lo_d) * wp : 0) + (R.ip0 - lo_p), wp);
     } else {
       radar_scatter_global(V, R, img, ov);
     }

@@ -390,6 +390,17 @@ __global__ __launch_bounds__(256, 6) void k_tile32(ViewRec *__restrict__ views, 
   double *img = fit + frames[V.frame_id].off;
   double *ov = overflow + (long long)blockIdx.y * 2000;
   int anyo = 0;
+  // the owners' vertex indices for all of this thread's pixels at once: one
+  // gather latency instead of one per pixel
+  int vid[TP][3];
+#pragma unroll
+  for (int m = 0; m < TP; m++) {
+    if (kk[m]) {
+      const int *fv = G.fv + 3 * key_facet(kk[m]);
+      vid[m][0] = fv[0]; vid[m][1] = fv[1]; vid[m][2] = fv[2];
+    }
+  }
+#pragma unroll
   for (int m = 0; m < TP; m++) {
     const unsigned long long k = kk[m];
     if (!k) continue;
@@ -398,11 +409,11 @@ __global__ __launch_bounds__(256, 6) void k_tile32(ViewRec *__restrict__ views, 
     const int i = i0 + a, j = j0 + c;
     const int f = key_facet(k);
     Proj32 P;
-    corners32(V, G, f, V.oa32, P);
+    corners32v(V, G, vid[m], V.oa32, P);
     float s, t;
     bary32(P, i, j, s, t);  // same arithmetic as the raster
     float nx, ny, nz;
-    normal32(G, f, V.oa32, V.smooth, s, t, nx, ny, nz);
+    normal32v(G, f, vid[m], V.oa32, V.smooth, s, t, nx, ny, nz);
     if (!(nz > 0.0f)) {
       // cos(scattering) of an owner facing away is inherited from an earlier
       // claimant: FP64 replay
@@ -413,38 +424,8 @@ __global__ __launch_bounds__(256, 6) void k_tile32(ViewRec *__restrict__ views, 
     radar_pixel(V, pl, i, j, (double)key_z(k), nz, R);
     if (!R.inb) anyo = 1;
     if (R.wide) { atomicAdd(&counters[3], 1); continue; }
-    if (use_lds && R.inb && R.sinc1) {
-      // Lanes whose footprints start at the same bin would add into the same
-      // LDS cells in lockstep, and same-address FP64 atomics serialize: each
-      // lane walks its Doppler bins from its own starting offset instead,
-      // recomputing each bin's weight (same value as pw[b]).
-      const int np = R.ip1 - R.ip0 + 1, nd = dd ? R.id1 - R.id0 + 1 : 1;
-      int b = lane & 15;
-      while (b >= np) b -= np;
-      double *col = win + (R.ip0 - lo_p) + (dd ? (R.id0 - lo_d) * wp : 0);
-      for (int st = 0; st < np; st++) {
-        const float pwb = dop_weight(R.pfr, R.pm0, R.s2, b);
-#pragma unroll
-        for (int aa = 0; aa < DW; aa++) {
-          if (aa < nd) {
-            float ad = dd ? R.amp * R.dw[aa] : R.amp;
-            atomicAdd(&col[aa * wp + b], (double)(ad * pwb));
-          }
-        }
-        if (++b == np) b = 0;
-      }
-    } else if (use_lds && R.inb) {
-      int np = R.ip1 - R.ip0 + 1, nd = dd ? R.id1 - R.id0 + 1 : 1;
-#pragma unroll
-      for (int aa = 0; aa < DW; aa++) {
-        if (aa < nd) {
-          float ad = dd ? R.amp * R.dw[aa] : R.amp;
-          double *row = win + (dd ? (R.id0 + aa - lo_d) * wp : 0) + (R.ip0 - lo_p);
-#pragma unroll
-          for (int bb = 0; bb < PW; bb++)
-            if (bb < np) atomicAdd(&row[bb], (double)(ad * R.pw[bb]));
-        }
-      }
+    if (use_lds && R.inb) {
+      radar_scatter_lds(V, R, win + (dd ? (R.id0 - lo_d) * wp : 0) + (R.ip0 - lo_p), wp);
     } else {
       radar_scatter_global(V, R, img, ov);
     }

@@ -52,16 +52,20 @@ struct Proj32 {
   float den;
 };
 
-// project a facet's corners (posvis.c:159-166) in pixel units
-__device__ inline void corners32(const ViewRec &V, const GeomSlot &G, int f, const float *m, Proj32 &P) {
-  const int *fv = G.fv + 3 * f;
-  float4 a = G.x32[fv[0]], b = G.x32[fv[1]], c = G.x32[fv[2]];
+// project a facet's corners (posvis.c:159-166) in pixel units; v: its vertices
+__device__ inline void corners32v(const ViewRec &V, const GeomSlot &G, const int v[3], const float *m, Proj32 &P) {
+  float4 a = G.x32[v[0]], b = G.x32[v[1]], c = G.x32[v[2]];
   float s = V.inv_kmpp32;
   rot3f(m, a.x, a.y, a.z, P.x0, P.y0, P.z0);
   rot3f(m, b.x, b.y, b.z, P.x1, P.y1, P.z1);
   rot3f(m, c.x, c.y, c.z, P.x2, P.y2, P.z2);
   P.x0 *= s; P.y0 *= s; P.x1 *= s; P.y1 *= s; P.x2 *= s; P.y2 *= s;
   P.den = 1.0f / ((P.x1 - P.x0) * (P.y2 - P.y1) - (P.x2 - P.x1) * (P.y1 - P.y0));
+}
+__device__ inline void corners32(const ViewRec &V, const GeomSlot &G, int f, const float *m, Proj32 &P) {
+  const int *fv = G.fv + 3 * f;
+  const int v[3] = {fv[0], fv[1], fv[2]};
+  corners32v(V, G, v, m, P);
 }
 
 constexpr float FRONT_TOL = 1.0e-4f;
@@ -178,13 +182,12 @@ __device__ inline void mark_marginal(const ViewRec &V, const PosDev &pd, bool ma
   else if (first) pd.stale_idx[p] = -1;  // list full: the error flag is set
 }
 
-// normal at (s, t) of facet f in observer coordinates (posvis.c:243-258)
-__device__ inline void normal32(const GeomSlot &G, int f, const float *m, int smooth, float s, float t, float &nx,
-                                float &ny, float &nz) {
+// normal at (s, t) of facet f (vertices v) in observer coordinates (posvis.c:243-258)
+__device__ inline void normal32v(const GeomSlot &G, int f, const int v[3], const float *m, int smooth, float s, float t,
+                                 float &nx, float &ny, float &nz) {
   float bx, by, bz;
   if (smooth) {
-    const int *fv = G.fv + 3 * f;
-    float4 na = G.vn32[fv[0]], nb = G.vn32[fv[1]], nc = G.vn32[fv[2]];
+    float4 na = G.vn32[v[0]], nb = G.vn32[v[1]], nc = G.vn32[v[2]];
     bx = na.x + s * (nb.x - na.x) + t * (nc.x - nb.x);
     by = na.y + s * (nb.y - na.y) + t * (nc.y - nb.y);
     bz = na.z + s * (nb.z - na.z) + t * (nc.z - nb.z);
@@ -198,6 +201,12 @@ __device__ inline void normal32(const GeomSlot &G, int f, const float *m, int sm
     nx *= inv; ny *= inv; nz *= inv;
   }
 }
+__device__ inline void normal32(const GeomSlot &G, int f, const float *m, int smooth, float s, float t, float &nx,
+                                float &ny, float &nz) {
+  const int *fv = G.fv + 3 * f;
+  const int v[3] = {fv[0], fv[1], fv[2]};
+  normal32v(G, f, v, m, smooth, s, t, nx, ny, nz);
+}
 
 __device__ inline float sinc2f(float x) {
   float t = fabsf(3.14159265358979f * x);
@@ -206,10 +215,10 @@ __device__ inline float sinc2f(float x) {
 }
 __device__ inline float trif(float x) { float t = 1.0f - fabsf(x); return t > 0.0f ? t : 0.0f; }
 
-// delay response of one image row (pos2deldop.c:371-424); drel = delPOS - idel
-__device__ inline float delay_w32(const ViewRec &V, float drel) {
+// delay response of one image row (pos2deldop.c:371-424); drel = delPOS - idel.
+// One sample per baud (the common case) inline, the rest out of line.
+__device__ __attribute__((noinline)) float delay_w32_other(const ViewRec &V, float drel) {
   if (V.codemethod != 1) {
-    if (V.spb == 1) { float t = trif(drel); return t * t; }
     int sos = V.spb / V.stride;
     float c2 = (float)V.const2;
     if (V.spb == 2) {
@@ -227,13 +236,21 @@ __device__ inline float delay_w32(const ViewRec &V, float drel) {
   float t = trif(drel / (V.spb / V.stride));
   return t * t;
 }
+__device__ inline float delay_w32(const ViewRec &V, float drel) {
+  if (V.codemethod != 1 && V.spb == 1) { float t = trif(drel); return t * t; }
+  return delay_w32_other(V, drel);
+}
 
-// radar scattering laws (radlaw.c:51-121); pl = [type, R, C, cutoff, qsR, qsC, qs_cutoff]
-__device__ inline float radlaw32(const double *pl, int type, float c) {
+// c^e for 0 < c <= 1 by the hardware base-2 log and exponential (~1 ulp
+// each; the per-pixel FP32 contributions carry ~1e-7 relative anyway)
+__device__ inline float powc(float c, float e) { return __builtin_amdgcn_exp2f(e * __builtin_amdgcn_logf(c)); }
+
+// radar scattering laws (radlaw.c:51-121); pl = [type, R, C, cutoff, qsR, qsC, qs_cutoff].
+// The laws other than the diffuse cosine law are out of line (cold code).
+__device__ __attribute__((noinline)) float radlaw32_other(const double *pl, int type, float c) {
   float R = (float)pl[1], Cc = (float)pl[2];
   float d = 0.0f;
   switch (type) {
-  case COSINELAW_DIFF: d = R * (Cc + 1.0f) * exp2f((2.0f * Cc - 1.0f) * log2f(c)); break;
   case GAUSSIANLAW:
     if (c >= (float)pl[3]) { float t2 = 1.0f / (c * c) - 1.0f; d = R * Cc * __expf(-Cc * t2) / (c * c * c * c * c); }
     break;
@@ -241,40 +258,71 @@ __device__ inline float radlaw32(const double *pl, int type, float c) {
     if (c >= (float)pl[3]) { float a = c * c * c * c + Cc * (1.0f - c * c); d = 0.5f * R * Cc * rsqrtf(a * a * a) / c; }
     break;
   case COSINELAW_QS:
-    if (c >= (float)pl[3]) d = R * (Cc + 1.0f) * exp2f((2.0f * Cc - 1.0f) * log2f(c));
+    if (c >= (float)pl[3]) d = R * (Cc + 1.0f) * powc(c, 2.0f * Cc - 1.0f);
     break;
   default: {
-    d = R * (Cc + 1.0f) * exp2f((2.0f * Cc - 1.0f) * log2f(c));
+    d = R * (Cc + 1.0f) * powc(c, 2.0f * Cc - 1.0f);
     float qR = (float)pl[4], qC = (float)pl[5];
     if (c >= (float)pl[6]) {
       if (type == GAUSSIAN_COSINE) { float t2 = 1.0f / (c * c) - 1.0f; d += qR * qC * __expf(-qC * t2) / (c * c * c * c * c); }
       else if (type == HAGFORS_COSINE) { float a = c * c * c * c + qC * (1.0f - c * c); d += 0.5f * qR * qC * rsqrtf(a * a * a) / c; }
-      else if (type == COSINE_COSINE) d += qR * (qC + 1.0f) * exp2f((2.0f * qC - 1.0f) * log2f(c));
+      else if (type == COSINE_COSINE) d += qR * (qC + 1.0f) * powc(c, 2.0f * qC - 1.0f);
     }
   }
   }
   return d;
 }
+__device__ inline float radlaw32(const double *pl, int type, float c) {
+  if (type == COSINELAW_DIFF) return (float)pl[1] * ((float)pl[2] + 1.0f) * powc(c, 2.0f * (float)pl[2] - 1.0f);
+  return radlaw32_other(pl, type, c);
+}
 
 // One lit POS pixel's delay-Doppler footprint (pos2deldop.c:320-482 /
-// pos2doppler.c:259-362): bins relative to the anchors D0 / P0, response
-// weights and the amplitude of cross section per unit weight.
+// pos2doppler.c:259-362): bins relative to the anchors D0 / P0, the fractional
+// bin positions its response weights are computed from (dop_w / del_w, at
+// the point of use, so no weight array stays live) and the amplitude of
+// cross section per unit weight.
 struct RadarPix {
   bool inb, wide;
-  bool sinc1;        // single-sample sinc^2 Doppler response: pw[b] == dop_weight(b)
+  bool sinc1;        // single-sample sinc^2 Doppler response
   float amp, delrel, doprel;
   int id0, id1, ip0, ip1;
-  float pfr, s2;     // fractional Doppler bin and sin^2(pi pfr) / pi^2
+  float pfr, s2;     // fractional Doppler bin; sin^2(pi pfr) / pi^2 (sinc1)
   int pm0;           // Doppler coordinate - first bin = pfr + pm0
-  float dw[DW], pw[PW];
+  float dfr;         // fractional delay bin
+  int dm0;           // delay coordinate - first row = dfr + dm0
 };
 
-// Doppler weight of bin b of a single-sample sinc^2 footprint: the expression
-// radar_pixel evaluates into pw[b], so both give the same value.
+// Doppler weight of bin b of a single-sample sinc^2 footprint
 __device__ inline float dop_weight(float pfr, int pm0, float s2, int b) {
   float a = pfr + (float)(pm0 - b);
   // hardware reciprocal (1 ulp): the weights are FP32 and normalized by their sum
   return (fabsf(3.14159265358979f * a) > 1.0e-4f) ? s2 * __builtin_amdgcn_rcpf(a * a) : 1.0f;
+}
+
+// Doppler response of bin ip0 + b averaged over nsinc2 x nsinc2 subpixels
+// (pos2deldop.c:437-455); out of line: only nsinc2 > 1 takes it
+__device__ __attribute__((noinline)) float dop_w_sub(const ViewRec &V, float pfr, int pm0, int b) {
+  const int ns = V.nsinc2;
+  const float xi = (float)V.xincr, yi = (float)V.yincr;
+  float al = pfr + (float)(pm0 - b) + (float)V.dopdiff_bl, acc = 0.0f;
+  for (int q = 0; q < ns; q++) {
+    float g = al;
+    for (int r = 0; r < ns; r++) { acc += sinc2f(g); g += xi; }
+    al += yi;
+  }
+  return acc / (ns * ns);
+}
+
+// Doppler response of bin ip0 + b: sinc^2 at the pixel center, or its
+// subpixel mean
+__device__ inline float dop_w(const ViewRec &V, const RadarPix &R, int b) {
+  return R.sinc1 ? dop_weight(R.pfr, R.pm0, R.s2, b) : dop_w_sub(V, R.pfr, R.pm0, b);
+}
+
+// delay response of row id0 + a
+__device__ inline float del_w(const ViewRec &V, const RadarPix &R, int a) {
+  return delay_w32(V, R.dfr + (float)(R.dm0 - a));
 }
 
 __device__ inline void radar_pixel(const ViewRec &V, const double *pl, int i, int j, double z, float cose, RadarPix &R) {
@@ -292,43 +340,24 @@ __device__ inline void radar_pixel(const ViewRec &V, const double *pl, int i, in
   R.ip1 = (int)floor(dop64 + V.dopdiff_max + width / 2.0) - V.P0;
   R.doprel = (float)(dop64 - (double)V.P0);
   double pfl = floor(dop64);
-  float pfr = (float)(dop64 - pfl);
-  int pm0 = (int)pfl - (V.P0 + R.ip0);  // dop64 - first bin = pfr + pm0
-  int np = R.ip1 - R.ip0 + 1;
+  R.pfr = (float)(dop64 - pfl);
+  R.pm0 = (int)pfl - (V.P0 + R.ip0);
+  const int np = R.ip1 - R.ip0 + 1;
   float codef = 1.0f;
   if (dd && V.codemethod == 0) codef = sinc2f((R.doprel + V.dcrel32) / (float)V.dopfftlen);
-  float sump = 0.0f;
-  R.pfr = pfr;
-  R.pm0 = pm0;
   R.s2 = 0.0f;
-  R.sinc1 = ns == 1 && np >= 1 && np <= PW;
+  R.sinc1 = ns == 1;
   if (R.sinc1) {
-    // sin(pi (x - k)) = +-sin(pi x): one sine serves every Doppler bin
-    float sp = sinf(3.14159265358979f * pfr);
+    // sin(pi (x - k)) = +-sin(pi x): one sine serves every Doppler bin.  The
+    // hardware sine (argument in revolutions) suffices: s2 scales every
+    // weight alike and cancels in their normalization, except for a bin
+    // within 1e-4 of the center, whose neighbours then weigh < 1e-8.
+    float sp = __builtin_amdgcn_sinf(0.5f * R.pfr);
     R.s2 = sp * sp * (1.0f / (3.14159265358979f * 3.14159265358979f));
-#pragma unroll
-    for (int b = 0; b < PW; b++) {
-      if (b < np) {
-        float v = dop_weight(pfr, pm0, R.s2, b);
-        R.pw[b] = v;
-        sump += v;
-      }
-    }
-  } else if (np <= PW) {
-    float xi = (float)V.xincr, yi = (float)V.yincr, bl = (float)V.dopdiff_bl;
-#pragma unroll
-    for (int b = 0; b < PW; b++) {
-      if (b < np) {
-        float al = pfr + (float)(pm0 - b) + bl, acc = 0.0f;
-        for (int q = 0; q < ns; q++) {
-          float g = al;
-          for (int r = 0; r < ns; r++) { acc += sinc2f(g); g += xi; }
-          al += yi;
-        }
-        R.pw[b] = acc / (ns * ns);
-        sump += R.pw[b];
-      }
-    }
+  }
+  float sump = 0.0f;
+  if (np >= 1 && np <= PW) {
+    for (int b = 0; b < np; b++) sump += dop_w(V, R, b);
   } else {
     R.wide = true;
   }
@@ -336,20 +365,20 @@ __device__ inline void radar_pixel(const ViewRec &V, const double *pl, int i, in
   R.delrel = 0.0f;
   R.id0 = 0;
   R.id1 = 0;
+  R.dfr = 0.0f;
+  R.dm0 = 0;
   if (dd) {
     double del64 = (double)z * V.delfact + V.delshift;
     R.id0 = (int)floor(del64 - V.const1) + 1 - V.D0;
     R.id1 = (int)ceil(del64 + V.const1) - 1 - V.D0;
     R.delrel = (float)(del64 - (double)V.D0);
     double dfl = floor(del64);
-    float dfr = (float)(del64 - dfl);
-    int dm0 = (int)dfl - (V.D0 + R.id0);  // del64 - first row = dfr + dm0
-    int nd = R.id1 - R.id0 + 1;
+    R.dfr = (float)(del64 - dfl);
+    R.dm0 = (int)dfl - (V.D0 + R.id0);
+    const int nd = R.id1 - R.id0 + 1;
     sumd = 0.0f;
     if (nd <= DW) {
-#pragma unroll
-      for (int a = 0; a < DW; a++)
-        if (a < nd) { R.dw[a] = delay_w32(V, dfr + (float)(dm0 - a)); sumd += R.dw[a]; }
+      for (int a = 0; a < nd; a++) sumd += del_w(V, R, a);
     } else {
       R.wide = true;
     }
@@ -357,7 +386,8 @@ __device__ inline void radar_pixel(const ViewRec &V, const double *pl, int i, in
   } else {
     R.inb = (V.P0 + R.ip0 >= 1) && (V.P0 + R.ip1 <= V.ndop);
   }
-  R.amp = R.wide ? 0.0f : radlaw32(pl, V.law_type, cose) * V.kmpp32 * V.kmpp32 * codef / (sumd * sump);
+  R.amp = R.wide ? 0.0f
+                  : radlaw32(pl, V.law_type, cose) * V.kmpp32 * V.kmpp32 * codef * __builtin_amdgcn_rcpf(sumd * sump);
 }
 
 // scatter one pixel straight into the HBM fit image (FP64 atomics) or, for
@@ -367,24 +397,40 @@ __device__ inline void radar_scatter_global(const ViewRec &V, const RadarPix &R,
   int np = R.ip1 - R.ip0 + 1, nd = dd ? R.id1 - R.id0 + 1 : 1, ndop = V.ndop;
   if (R.wide) return;
   if (R.inb) {
-#pragma unroll
-    for (int a = 0; a < DW; a++) {
-      if (a < nd) {
-        float ad = dd ? R.amp * R.dw[a] : R.amp;
-        int idel = dd ? V.D0 + R.id0 + a : 1;
-#pragma unroll
-        for (int b = 0; b < PW; b++)
-          if (b < np) atomicAdd(&img[(long long)(idel - 1) * ndop + (V.P0 + R.ip0 + b - 1)], (double)(ad * R.pw[b]));
-      }
+    for (int a = 0; a < nd; a++) {
+      const float ad = dd ? R.amp * del_w(V, R, a) : R.amp;
+      const long long row = (long long)((dd ? V.D0 + R.id0 + a : 1) - 1) * ndop + (V.P0 + R.ip0 - 1);
+      for (int b = 0; b < np; b++) atomicAdd(&img[row + b], (double)(ad * dop_w(V, R, b)));
     }
   } else if (!dd) {
     int idop0 = V.idop0;
     int lo = max(V.P0 + R.ip0, -idop0), hi = min(V.P0 + R.ip1, -idop0 + 2000 - 1);
-#pragma unroll
-    for (int b = 0; b < PW; b++) {
+    for (int b = 0; b < np; b++) {
       int idop = V.P0 + R.ip0 + b;
-      if (idop >= lo && idop <= hi) atomicAdd(&ov[idop + idop0], (double)(R.amp * R.pw[b]));
+      if (idop >= lo && idop <= hi) atomicAdd(&ov[idop + idop0], (double)(R.amp * dop_w(V, R, b)));
     }
+  }
+}
+
+// Add one pixel's footprint into an FP64 LDS window (rows of wp cells; origin
+// = window cell of the footprint's first row and bin).  Lanes whose
+// footprints start at the same bin would add into the same cells in lockstep,
+// and same-address FP64 LDS atomics serialize, so each lane walks its Doppler
+// bins from its own starting offset.
+__device__ inline void radar_scatter_lds(const ViewRec &V, const RadarPix &R, double *origin, int wp) {
+  const bool dd = V.type == VT_DELAY;
+  const int np = R.ip1 - R.ip0 + 1, nd = dd ? R.id1 - R.id0 + 1 : 1;
+  float ad[DW];
+#pragma unroll
+  for (int a = 0; a < DW; a++) ad[a] = (a < nd) ? (dd ? R.amp * del_w(V, R, a) : R.amp) : 0.0f;
+  int b = threadIdx.x & 15;
+  while (b >= np) b -= np;
+  for (int st = 0; st < np; st++) {
+    const float pwb = dop_w(V, R, b);
+#pragma unroll
+    for (int a = 0; a < DW; a++)
+      if (a < nd) atomicAdd(&origin[a * wp + b], (double)(ad[a] * pwb));
+    if (++b == np) b = 0;
   }
 }
 
