@@ -134,6 +134,10 @@ def kernel_bytes(name, fp, claimed):
         # read every key of the views' model rectangles, clear the claimed
         # ones, write each occupied image cell once (FP64)
         return 8 * fp["bbox_pix"] + 8 * claimed + 8 * fp["window_cells"]
+    if name == "k_tile32":
+        # keys stay in LDS: the model once per evaluation (plus a 16-byte
+        # bounding sphere per facet) and each occupied image cell written once
+        return model + 16 * fp["nf"] + 8 * fp["window_cells"]
     if name == "k_chi2_mixed":
         return 24 * fp["window_cells"]            # fit read + clear (FP64), obs*w and w (FP32)
     return None
