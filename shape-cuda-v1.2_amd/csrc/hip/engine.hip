@@ -547,6 +547,8 @@ int Engine::build(struct par_t *par, struct mod_t **mods, struct dat_t **dats, i
       max_nblk = std::max(max_nblk, nblk);
       if ((rc = alloc((void **)&G.blk, sizeof(float4) * 2 * (size_t)nblk))) return rc;
       if ((rc = alloc((void **)&G.fb, sizeof(float4) * (size_t)G.nf))) return rc;
+      if ((rc = alloc((void **)&G.fvx, sizeof(float4) * 3 * (size_t)G.nf))) return rc;
+      if ((rc = alloc((void **)&G.fvn, sizeof(float4) * 3 * (size_t)G.nf))) return rc;
     }
     SHP_CHECK(hipMemset(G.seen, 0, G.nf));
     if (opt.host_geometry) {

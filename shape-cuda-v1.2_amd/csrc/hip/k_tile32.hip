@@ -1,7 +1,9 @@
 // k_tile32.hip -- mixed method, binned raster: no z-buffer key reaches HBM.
 //
 //   k_block_bounds  per slot: bounding sphere and normal cone of every block of
-//                   BF consecutive facets (FP32, from the realized model)
+//                   BF consecutive facets and a sphere per facet (FP32, from
+//                   the realized model), and facet-major copies of each
+//                   facet's vertices and vertex normals
 //   k_bin32         per view: each block's conservative pixel rectangle, entered
 //                   in the lists of the BT x BT tiles it touches
 //   k_tile32        per tile: rasterize the listed blocks into an LDS z-buffer
@@ -37,6 +39,9 @@ __global__ __launch_bounds__(256) void k_block_bounds(GeomSlot *slots, int nslot
     const int *fv = G.fv + 3 * f;
     va = G.x32[fv[0]]; vb = G.x32[fv[1]]; vc = G.x32[fv[2]];
     fn = G.fn32[f];
+    // facet-major copies for the tile kernel: one gather per facet, not two
+    G.fvx[3 * f] = va; G.fvx[3 * f + 1] = vb; G.fvx[3 * f + 2] = vc;
+    G.fvn[3 * f] = G.vn32[fv[0]]; G.fvn[3 * f + 1] = G.vn32[fv[1]]; G.fvn[3 * f + 2] = G.vn32[fv[2]];
   }
   float lo[3], hi[3];
   {
@@ -271,42 +276,50 @@ __global__ __launch_bounds__(256, 6) void k_tile32(ViewRec *__restrict__ views, 
         const int f = b * BF + lane;
         bool keep = false;
         if (f < G.nf) {
-          // the facet's sphere against the tile, before any dependent gather:
-          // its bounding box (posvis.c:172-180) lies within the projected sphere
-          const float4 fb = G.fb[f];
+          // the facet's sphere against the tile, before any dependent gather
+          // (its bounding box, posvis.c:172-180, lies within the projected
+          // sphere), and its facing: nearly edge-on facets are replay
+          // candidates only, the FP64 replay tests facing itself
+          // (posvis.c:152).  Both loads are coalesced and issued together.
+          const float4 fb = G.fb[f], fn = G.fn32[f];
           const float px = (m[0] * fb.x + m[1] * fb.y + m[2] * fb.z) * sc;
           const float py = (m[3] * fb.x + m[4] * fb.y + m[5] * fb.z) * sc;
           const float R = fb.w * sc * 1.0001f + 0.01f + 1.0e-6f * (fabsf(px) + fabsf(py));
           keep = !(floorf(px + R + 0.5f) < (float)i0 || floorf(px - R + 0.5f) > (float)(i0 + BT - 1) ||
-                   floorf(py + R + 0.5f) < (float)j0 || floorf(py - R + 0.5f) > (float)(j0 + BT - 1));
-          if (keep) {
-            // nearly edge-on facets are replay candidates only: the FP64
-            // replay tests facing itself (posvis.c:152)
-            const float4 fn = G.fn32[f];
-            keep = m[6] * fn.x + m[7] * fn.y + m[8] * fn.z > -FRONT_TOL;
-          }
+                   floorf(py + R + 0.5f) < (float)j0 || floorf(py - R + 0.5f) > (float)(j0 + BT - 1)) &&
+                 m[6] * fn.x + m[7] * fn.y + m[8] * fn.z > -FRONT_TOL;
         }
         lds_append(keep, f, flist, &nfl, FL_CAP);
       }
       __syncthreads();
       const int nl = min(nfl, FL_CAP);
-      for (int q = threadIdx.x; q < nl; q += blockDim.x) {
-        const int f = flist[q];
+      // every lane of a wave takes each trip (q >= nl: idle lane), so the
+      // wave-wide event reservation below sees all 64 lanes
+      for (int q0 = threadIdx.x & ~63; q0 < nl; q0 += blockDim.x) {
+        const int q = q0 + lane;
         Proj32 P;
-        float nz;
-        if (!project32n(V, G, f, V.oa32, -FRONT_TOL, P, nz)) continue;
+        float nz = 0.0f;
+        int f = 0, ti1 = 0, ti2 = -1, tj1 = 0, tj2 = -1, wi1 = 0, wi2 = 0, wj1 = 0, wj2 = 0;
+        if (q < nl) {
+          f = flist[q];
+          if (project32nf(V, G, f, V.oa32, -FRONT_TOL, P, nz)) {
+            float imin_d = floorf(fminf(P.x0, fminf(P.x1, P.x2)) + 0.5f);
+            float imax_d = floorf(fmaxf(P.x0, fmaxf(P.x1, P.x2)) + 0.5f);
+            float jmin_d = floorf(fminf(P.y0, fminf(P.y1, P.y2)) + 0.5f);
+            float jmax_d = floorf(fmaxf(P.y0, fmaxf(P.y1, P.y2)) + 0.5f);
+            int imin = (int)fmaxf(imin_d, -1.0e9f), imax = (int)fminf(imax_d, 1.0e9f);
+            int jmin = (int)fmaxf(jmin_d, -1.0e9f), jmax = (int)fminf(jmax_d, 1.0e9f);
+            // window-clipped box (the POS rectangle grows by it), then tile-clipped
+            wi1 = max(imin, -n); wi2 = min(imax, n); wj1 = max(jmin, -n); wj2 = min(jmax, n);
+            ti1 = max(wi1, i0); ti2 = min(wi2, i0 + BT - 1); tj1 = max(wj1, j0); tj2 = min(wj2, j0 + BT - 1);
+          }
+        }
         const bool front = nz > 0.0f;
-        float imin_d = floorf(fminf(P.x0, fminf(P.x1, P.x2)) + 0.5f);
-        float imax_d = floorf(fmaxf(P.x0, fmaxf(P.x1, P.x2)) + 0.5f);
-        float jmin_d = floorf(fminf(P.y0, fminf(P.y1, P.y2)) + 0.5f);
-        float jmax_d = floorf(fmaxf(P.y0, fmaxf(P.y1, P.y2)) + 0.5f);
-        int imin = (int)fmaxf(imin_d, -1.0e9f), imax = (int)fminf(imax_d, 1.0e9f);
-        int jmin = (int)fmaxf(jmin_d, -1.0e9f), jmax = (int)fminf(jmax_d, 1.0e9f);
-        // window-clipped box (the POS rectangle grows by it), then tile-clipped
-        const int wi1 = max(imin, -n), wi2 = min(imax, n), wj1 = max(jmin, -n), wj2 = min(jmax, n);
-        const int ti1 = max(wi1, i0), ti2 = min(wi2, i0 + BT - 1), tj1 = max(wj1, j0), tj2 = min(wj2, j0 + BT - 1);
-        if (ti1 > ti2 || tj1 > tj2) continue;
-        const int nc = (ti2 - ti1 + 1) * (tj2 - tj1 + 1);
+        const int nc = (ti1 <= ti2 && tj1 <= tj2) ? (ti2 - ti1 + 1) * (tj2 - tj1 + 1) : 0;
+        // covered cells (cell c of the box: bit c); boxes over 64 cells, rare,
+        // append their further events one by one
+        const bool big = __ballot(nc > 64) != 0;
+        unsigned long long covm = 0ULL;
         bool hit = false;
         for (int c = 0, i = ti1, j = tj1; c < nc; c++) {
           float s, t;
@@ -320,12 +333,33 @@ __global__ __launch_bounds__(256, 6) void k_tile32(ViewRec *__restrict__ views, 
             flag[lp] = 1;  // marginal: claimed by no facet, replayed in FP64
             hit = true;
           }
-          ev_append(cov, ((unsigned int)lp << EV_FACET_BITS) | (unsigned int)f, ev, &nev);
+          if (cov && c < 64) covm |= 1ULL << c;
+          if (big) ev_append(cov && c >= 64, ((unsigned int)lp << EV_FACET_BITS) | (unsigned int)f, ev, &nev);
           if (++j > tj2) { j = tj1; i++; }
         }
         if (hit) {
           xl0 = min(xl0, wi1); xl1 = max(xl1, wi2);
           yl0 = min(yl0, wj1); yl1 = max(yl1, wj2);
+        }
+        // one reservation for the whole wave's events: exclusive prefix sum
+        // of the lanes' counts, one LDS atomic by the last lane
+        const int ne = __popcll(covm);
+        int incl = ne;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+          const int y = __shfl_up(incl, o, 64);
+          if (lane >= o) incl += y;
+        }
+        int base = 0;
+        if (lane == 63) base = atomicAdd(&nev, incl);
+        base = __shfl(base, 63, 64) + incl - ne;
+        const int wj = tj2 - tj1 + 1;
+        while (covm) {
+          const int c = __builtin_ctzll(covm);
+          covm &= covm - 1ULL;
+          const int lp = (ti1 + c / wj - i0) * BT + (tj1 + c % wj - j0);
+          if (base < EV_CAP) ev[base] = ((unsigned int)lp << EV_FACET_BITS) | (unsigned int)f;
+          base++;
         }
       }
       __syncthreads();
@@ -390,17 +424,6 @@ __global__ __launch_bounds__(256, 6) void k_tile32(ViewRec *__restrict__ views, 
   double *img = fit + frames[V.frame_id].off;
   double *ov = overflow + (long long)blockIdx.y * 2000;
   int anyo = 0;
-  // the owners' vertex indices for all of this thread's pixels at once: one
-  // gather latency instead of one per pixel
-  int vid[TP][3];
-#pragma unroll
-  for (int m = 0; m < TP; m++) {
-    if (kk[m]) {
-      const int *fv = G.fv + 3 * key_facet(kk[m]);
-      vid[m][0] = fv[0]; vid[m][1] = fv[1]; vid[m][2] = fv[2];
-    }
-  }
-#pragma unroll
   for (int m = 0; m < TP; m++) {
     const unsigned long long k = kk[m];
     if (!k) continue;
@@ -409,11 +432,11 @@ __global__ __launch_bounds__(256, 6) void k_tile32(ViewRec *__restrict__ views, 
     const int i = i0 + a, j = j0 + c;
     const int f = key_facet(k);
     Proj32 P;
-    corners32v(V, G, vid[m], V.oa32, P);
+    corners32f(V, G, f, V.oa32, P);
     float s, t;
     bary32(P, i, j, s, t);  // same arithmetic as the raster
     float nx, ny, nz;
-    normal32v(G, f, vid[m], V.oa32, V.smooth, s, t, nx, ny, nz);
+    normal32f(G, f, V.oa32, V.smooth, s, t, nx, ny, nz);
     if (!(nz > 0.0f)) {
       // cos(scattering) of an owner facing away is inherited from an earlier
       // claimant: FP64 replay

@@ -52,9 +52,8 @@ struct Proj32 {
   float den;
 };
 
-// project a facet's corners (posvis.c:159-166) in pixel units; v: its vertices
-__device__ inline void corners32v(const ViewRec &V, const GeomSlot &G, const int v[3], const float *m, Proj32 &P) {
-  float4 a = G.x32[v[0]], b = G.x32[v[1]], c = G.x32[v[2]];
+// project a facet's corners a, b, c (posvis.c:159-166) in pixel units
+__device__ inline void corners32p(const ViewRec &V, float4 a, float4 b, float4 c, const float *m, Proj32 &P) {
   float s = V.inv_kmpp32;
   rot3f(m, a.x, a.y, a.z, P.x0, P.y0, P.z0);
   rot3f(m, b.x, b.y, b.z, P.x1, P.y1, P.z1);
@@ -64,8 +63,11 @@ __device__ inline void corners32v(const ViewRec &V, const GeomSlot &G, const int
 }
 __device__ inline void corners32(const ViewRec &V, const GeomSlot &G, int f, const float *m, Proj32 &P) {
   const int *fv = G.fv + 3 * f;
-  const int v[3] = {fv[0], fv[1], fv[2]};
-  corners32v(V, G, v, m, P);
+  corners32p(V, G.x32[fv[0]], G.x32[fv[1]], G.x32[fv[2]], m, P);
+}
+// the same from the facet-major vertex copies (one gather level)
+__device__ inline void corners32f(const ViewRec &V, const GeomSlot &G, int f, const float *m, Proj32 &P) {
+  corners32p(V, G.fvx[3 * f], G.fvx[3 * f + 1], G.fvx[3 * f + 2], m, P);
 }
 
 constexpr float FRONT_TOL = 1.0e-4f;
@@ -76,6 +78,15 @@ __device__ inline bool project32n(const ViewRec &V, const GeomSlot &G, int f, co
   nz = m[6] * fn.x + m[7] * fn.y + m[8] * fn.z;
   if (!(nz > nz_min)) return false;
   corners32(V, G, f, m, P);
+  return true;
+}
+// as project32n, from the facet-major vertex copies
+__device__ inline bool project32nf(const ViewRec &V, const GeomSlot &G, int f, const float *m, float nz_min,
+                                   Proj32 &P, float &nz) {
+  float4 fn = G.fn32[f];
+  nz = m[6] * fn.x + m[7] * fn.y + m[8] * fn.z;
+  if (!(nz > nz_min)) return false;
+  corners32f(V, G, f, m, P);
   return true;
 }
 __device__ inline bool project32(const ViewRec &V, const GeomSlot &G, int f, const float *m, float nz_min, Proj32 &P) {
@@ -182,12 +193,13 @@ __device__ inline void mark_marginal(const ViewRec &V, const PosDev &pd, bool ma
   else if (first) pd.stale_idx[p] = -1;  // list full: the error flag is set
 }
 
-// normal at (s, t) of facet f (vertices v) in observer coordinates (posvis.c:243-258)
-__device__ inline void normal32v(const GeomSlot &G, int f, const int v[3], const float *m, int smooth, float s, float t,
-                                 float &nx, float &ny, float &nz) {
+// normal at (s, t) of facet f in observer coordinates (posvis.c:243-258);
+// vn: the facet's 3 vertex normals (smooth) from the facet-major copies
+__device__ inline void normal32f(const GeomSlot &G, int f, const float *m, int smooth, float s, float t, float &nx,
+                                 float &ny, float &nz) {
   float bx, by, bz;
   if (smooth) {
-    float4 na = G.vn32[v[0]], nb = G.vn32[v[1]], nc = G.vn32[v[2]];
+    float4 na = G.fvn[3 * f], nb = G.fvn[3 * f + 1], nc = G.fvn[3 * f + 2];
     bx = na.x + s * (nb.x - na.x) + t * (nc.x - nb.x);
     by = na.y + s * (nb.y - na.y) + t * (nc.y - nb.y);
     bz = na.z + s * (nb.z - na.z) + t * (nc.z - nb.z);
@@ -203,9 +215,22 @@ __device__ inline void normal32v(const GeomSlot &G, int f, const int v[3], const
 }
 __device__ inline void normal32(const GeomSlot &G, int f, const float *m, int smooth, float s, float t, float &nx,
                                 float &ny, float &nz) {
-  const int *fv = G.fv + 3 * f;
-  const int v[3] = {fv[0], fv[1], fv[2]};
-  normal32v(G, f, v, m, smooth, s, t, nx, ny, nz);
+  float bx, by, bz;
+  if (smooth) {
+    const int *fv = G.fv + 3 * f;
+    float4 na = G.vn32[fv[0]], nb = G.vn32[fv[1]], nc = G.vn32[fv[2]];
+    bx = na.x + s * (nb.x - na.x) + t * (nc.x - nb.x);
+    by = na.y + s * (nb.y - na.y) + t * (nc.y - nb.y);
+    bz = na.z + s * (nb.z - na.z) + t * (nc.z - nb.z);
+  } else {
+    float4 fn = G.fn32[f];
+    bx = fn.x; by = fn.y; bz = fn.z;
+  }
+  rot3f(m, bx, by, bz, nx, ny, nz);
+  if (smooth) {
+    float inv = rsqrtf(nx * nx + ny * ny + nz * nz);
+    nx *= inv; ny *= inv; nz *= inv;
+  }
 }
 
 __device__ inline float sinc2f(float x) {
