@@ -653,6 +653,11 @@ int Engine::build(struct par_t *par, struct mod_t **mods, struct dat_t **dats, i
     if ((rc = alloc((void **)&d_bin_cnt, sizeof(int) * (size_t)total_bins))) return rc;
     SHP_CHECK(hipMemsetAsync(d_bin_cnt, 0, sizeof(int) * (size_t)total_bins, stream));
     if ((rc = alloc((void **)&d_bin_list, sizeof(unsigned short) * BIN_CAP * (size_t)total_bins))) return rc;
+    // every block of every binned view may reach outside its window
+    long long out_cap = 0;
+    for (auto &V : h_views)
+      if (V.binned) out_cap += (h_slots[V.slot].nf + BF - 1) / BF;
+    if ((rc = alloc((void **)&d_out_list, sizeof(int2) * (size_t)std::max(1LL, out_cap)))) return rc;
   }
   if (opt.method == METHOD_MIXED) {
     if ((rc = alloc((void **)&pd.stale_tiles, (size_t)std::max(1LL, total_tiles)))) return rc;
@@ -662,10 +667,12 @@ int Engine::build(struct par_t *par, struct mod_t **mods, struct dat_t **dats, i
   // of every view take well under that), at least 4096, at most 64M
   pd.stale_cap = (int)std::min<long long>(std::max<long long>((long long)P / 32, 4096), 64LL << 20);
   if ((rc = alloc((void **)&pd.stale, sizeof(StaleRec) * (size_t)pd.stale_cap))) return rc;
-  if ((rc = alloc((void **)&pd.nstale, sizeof(int) * 4))) return rc;
+  // counters: [0] replay records, [1] error flags, [2] records to collect,
+  // [3] wide footprints, [4] blocks reaching outside their POS window
+  if ((rc = alloc((void **)&pd.nstale, sizeof(int) * 8))) return rc;
   pd.error = pd.nstale + 1;
   pd.ncollect = pd.nstale + 2;
-  SHP_CHECK(hipMemset(pd.nstale, 0, sizeof(int) * 4));
+  SHP_CHECK(hipMemset(pd.nstale, 0, sizeof(int) * 8));
   hipLaunchKernelGGL(k_frame_o2, dim3(std::max(1, nframes)), dim3(256), 0, stream, d_frames, nframes, d_obs, d_oov);
   SHP_CHECK(hipStreamSynchronize(stream));
   g_packs.clear();
@@ -699,7 +706,7 @@ static inline unsigned cdiv(long long a, long long b) { return (unsigned)((a + b
 int Engine::enqueue() {
   const int T = 256;
   dim3 gv(cdiv(nviews, 64)), bv(64);
-  SHP_LAUNCH(*this, "k_zero_counters", k_zero_counters, dim3(1), dim3(64), 0, stream, pd.nstale, 4);
+  SHP_LAUNCH(*this, "k_zero_counters", k_zero_counters, dim3(1), dim3(64), 0, stream, pd.nstale, 8);
   SHP_LAUNCH(*this, "k_setup_views", k_setup_views, gv, bv, 0, stream, d_views, nviews, d_params);
   if (!opt.host_geometry) {
     int max_nv = 0;
@@ -737,7 +744,7 @@ int Engine::enqueue() {
                        d_lc_t, d_lc_y, d_lc_y2, d_fit);
   SHP_LAUNCH(*this, "k_chi2_frames64", k_chi2_frames64, dim3(nframes), bt, 0, stream, d_frames, nframes, d_fit, d_obs, d_oov,
                      d_view_out, opt.keep_fit ? 0 : 1, 0, 0);
-  SHP_LAUNCH(*this, "k_objective", k_objective, dim3(cdiv(nevals, 64)), dim3(64), 0, stream, d_evals, nevals, d_frames, d_views,
+  SHP_LAUNCH(*this, "k_objective", k_objective, dim3(std::max(1, nevals)), dim3(256), 0, stream, d_evals, nevals, d_frames, d_views,
                      d_view_out, d_slot_flags, d_slot_logf, d_eval_slot);
   SHP_CHECK(hipGetLastError());
   return 0;

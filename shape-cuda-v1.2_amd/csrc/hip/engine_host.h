@@ -96,6 +96,7 @@ class Engine {
   float *d_obs32 = nullptr, *d_oov32 = nullptr;
   int *d_bin_cnt = nullptr;                 // per 32x32 tile of every binned view
   unsigned short *d_bin_list = nullptr;     // BIN_CAP block ids per tile
+  int2 *d_out_list = nullptr;               // (view, block) of blocks reaching outside the POS window
   PosDev pd{};
   std::vector<void *> allocs;
   hipGraph_t graph = nullptr;

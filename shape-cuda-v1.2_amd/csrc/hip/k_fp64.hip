@@ -1115,34 +1115,53 @@ __global__ void k_chi2_frames64(FrameRec *frames, int nframes, double *fit, cons
 // objective per evaluation (bestfit.c:789-866 with zero penalties), in
 // fixed frame order; log factors follow calc_fits.c:407-409, :511-512,
 // :547-550, :1654-1677, :1947
-__global__ void k_objective(EvalRec *evals, int nevals, const FrameRec *frames, const ViewRec *views,
-                            const double *view_out, const int *slot_flags, const double *slot_logf,
-                            const int *eval_slot) {
-  int e = blockIdx.x * blockDim.x + threadIdx.x;
+__global__ __launch_bounds__(256) void k_objective(EvalRec *evals, int nevals, const FrameRec *frames,
+                                                   const ViewRec *views, const double *view_out, const int *slot_flags,
+                                                   const double *slot_logf, const int *eval_slot) {
+  // one workgroup per evaluation: the threads fetch 256 frames' terms at a
+  // time into LDS, one thread adds them in frame order (the CPU's order)
+  const int e = blockIdx.x;
   if (e >= nevals) return;
   EvalRec &E = evals[e];
+  __shared__ double s_chi[256], s_pb[256], s_br[256];
+  __shared__ int s_fl[256];
   double chi2 = 0.0, pb = 0.0, br = 0.0;
   int posbnd = 0, badradar = 0;
-  for (int q = 0; q < E.nframes; q++) {
-    const FrameRec &F = frames[E.first_frame + q];
-    chi2 += F.chi2;
-    if (F.type == VT_LGHTCRV) {
-      double lf = 0.0;
-      for (int i = 0; i < F.lc_ncalc; i++) {
-        const double *o = view_out + (long long)(F.first_view + i) * VIEW_OUT;
-        if (o[OUT_OUTBND] != 0.0) { posbnd = 1; lf += 0.5 * o[OUT_POSBND_LOG]; }
-        if (o[OUT_OUTBND + 1] != 0.0) { posbnd = 1; lf += 0.5 * o[OUT_POSBND_LOG + 1]; }
+  for (int q0 = 0; q0 < E.nframes; q0 += 256) {
+    const int q = q0 + threadIdx.x;
+    if (q < E.nframes) {
+      const FrameRec &F = frames[E.first_frame + q];
+      double fpb = 0.0, fbr = 0.0;
+      int fl = 0;
+      if (F.type == VT_LGHTCRV) {
+        double lf = 0.0;
+        for (int i = 0; i < F.lc_ncalc; i++) {
+          const double *o = view_out + (long long)(F.first_view + i) * VIEW_OUT;
+          if (o[OUT_OUTBND] != 0.0) { fl |= 1; lf += 0.5 * o[OUT_POSBND_LOG]; }
+          if (o[OUT_OUTBND + 1] != 0.0) { fl |= 1; lf += 0.5 * o[OUT_POSBND_LOG + 1]; }
+        }
+        fpb = F.dof * (lf / F.lc_ncalc);
+      } else {
+        for (int v = 0; v < F.nviews; v++) {
+          const ViewRec &V = views[F.first_view + v];
+          const double *o = view_out + (long long)(F.first_view + v) * VIEW_OUT;
+          if (V.is_v0 && o[OUT_OUTBND] != 0.0) { fl |= 1; fpb += F.dof * o[OUT_POSBND_LOG]; }
+          if (o[OUT_BADRADAR] != 0.0) { fl |= 2; fbr += F.dof * o[OUT_BADRADAR_LOG] / F.nviews; }
+        }
       }
-      pb += F.dof * (lf / F.lc_ncalc);
-    } else {
-      for (int v = 0; v < F.nviews; v++) {
-        const ViewRec &V = views[F.first_view + v];
-        const double *o = view_out + (long long)(F.first_view + v) * VIEW_OUT;
-        if (V.is_v0 && o[OUT_OUTBND] != 0.0) { posbnd = 1; pb += F.dof * o[OUT_POSBND_LOG]; }
-        if (o[OUT_BADRADAR] != 0.0) { badradar = 1; br += F.dof * o[OUT_BADRADAR_LOG] / F.nviews; }
+      s_chi[threadIdx.x] = F.chi2; s_pb[threadIdx.x] = fpb; s_br[threadIdx.x] = fbr; s_fl[threadIdx.x] = fl;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      for (int k = 0; k < min(256, E.nframes - q0); k++) {
+        chi2 += s_chi[k];
+        if (s_fl[k] & 1) { posbnd = 1; pb += s_pb[k]; }
+        if (s_fl[k] & 2) { badradar = 1; br += s_br[k]; }
       }
     }
+    __syncthreads();
   }
+  if (threadIdx.x != 0) return;
   E.chi2 = chi2;
   E.posbnd = posbnd;
   E.badradar = badradar;

@@ -37,125 +37,162 @@ __global__ void k_replay32(ViewRec *__restrict__ views, const GeomSlot *slots, P
                            double *overflow, const double *params);
 __global__ void k_block_bounds(GeomSlot *slots, int nslots);
 __global__ void k_bin32(const ViewRec *__restrict__ views, const GeomSlot *slots, int *bin_cnt,
-                        unsigned short *bin_list);
+                        unsigned short *bin_list, int2 *out_list, int *out_count);
 __global__ void k_tile32(ViewRec *__restrict__ views, const GeomSlot *slots, PosDev pd, int *bin_cnt,
                          const unsigned short *bin_list, double *fit, const FrameRec *frames, double *overflow,
                          const double *params, int *counters);
 
 // ---------------------------------------------------------------------------
-// raster (posvis.c:142-295).  pass 1 (views that are not binned): packed-key
-// z-buffer over pixels well inside a facet, marginal pixels recorded for the
-// FP64 replay, and the POS rectangles; pass 0 (binned views): only the facets
-// reaching outside the POS window (posbnd and the outside extent), visiting
-// only blocks whose rectangle leaves the window; pass 3: record the facets
-// covering pixels in the global replay list, visiting only facets whose
-// rectangle touches a tile flagged in the stale bitmap
+// raster (posvis.c:142-295), one facet per lane:
+//   pass 0 (binned views, k_outside32): only the POS extent of facets
+//     reaching outside the window (posbnd and the outside extent); the
+//     binned tile kernel covers the pixels;
+//   pass 1 (views that are not binned, k_raster32): packed-key z-buffer over
+//     pixels well inside a facet, marginal pixels recorded for the FP64
+//     replay, and the POS rectangles;
+//   pass 3 (k_collect32): record the facets covering pixels of the replay
+//     list, visiting only facets whose rectangle touches a tile flagged in
+//     the stale bitmap.
+// Passes 0 and 1 fold the wave's POS rectangle, extent and posbnd flag into
+// the view's with one update per wave (raster_limits).
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_raster32(ViewRec *__restrict__ views, const GeomSlot *slots, PosDev pd,
-                                                  long long sun_off, int src, int pass) {
-  ViewRec &V = views[blockIdx.y];
-  if (V.type != VT_DELAY && V.type != VT_DOPPLER) return;  // lightcurve views take the FP64 kernels
-  if (src && !V.bistatic) return;
-  if (pass == 3 && *pd.ncollect == 0) return;
-  if ((pass == 0 && !V.binned) || (pass == 1 && V.binned)) return;
-  const GeomSlot &G = slots[V.slot];
-  // passes 0 and 1 walk every facet of every view: XCD-local eighths of the
-  // model.  Pass 3 works only near flagged tiles, which cluster: plain order
-  // balances it.
-  int f = (pass != 3 ? xcd_block(blockIdx.x, gridDim.x) : (int)blockIdx.x) * blockDim.x + threadIdx.x;
-  int n = V.n, w = 2 * n + 1;
-  if (pass == 0) {
-    // a wave is one bin block: skip it when its rectangle lies inside the window
-    int b = f / BF, r0, r1, c0, c1;
-    if (b * BF >= G.nf) return;
-    if (!block_rect(V, G, b, r0, r1, c0, c1) || (r0 >= -n && r1 <= n && c0 >= -n && c1 <= n)) return;
-  }
-  unsigned long long *kb = pd.zkey + V.pos_off + (src ? sun_off : 0);
-  int xl0 = n, xl1 = -n, yl0 = n, yl1 = -n, ob = 0;
-  float o0 = 1e30f, o1 = -1e30f, o2 = 1e30f, o3 = -1e30f;
+struct RasterLim {
+  int xl0, xl1, yl0, yl1, ob;
+  float o0, o1, o2, o3;
+};
+
+__device__ inline void raster_facet(ViewRec &V, const GeomSlot &G, const PosDev &pd, long long sun_off, int src,
+                                    int pass, int f, RasterLim &L) {
+  const int n = V.n, w = 2 * n + 1;
+  L.xl0 = n; L.xl1 = -n; L.yl0 = n; L.yl1 = -n; L.ob = 0;
+  L.o0 = 1e30f; L.o1 = -1e30f; L.o2 = 1e30f; L.o3 = -1e30f;
   Proj32 P;
   // pass 3 also admits nearly edge-on facets: the FP64 replay tests facing itself
-  if (f < G.nf && project32(V, G, f, src ? V.sa32 : V.oa32, pass == 3 ? -FRONT_TOL : 0.0f, P)) {
-    float imin_d = floorf(fminf(P.x0, fminf(P.x1, P.x2)) + 0.5f);
-    float imax_d = floorf(fmaxf(P.x0, fmaxf(P.x1, P.x2)) + 0.5f);
-    float jmin_d = floorf(fminf(P.y0, fminf(P.y1, P.y2)) + 0.5f);
-    float jmax_d = floorf(fmaxf(P.y0, fmaxf(P.y1, P.y2)) + 0.5f);
-    imin_d = fmaxf(imin_d, -1.0e9f); imax_d = fminf(imax_d, 1.0e9f);
-    jmin_d = fmaxf(jmin_d, -1.0e9f); jmax_d = fminf(jmax_d, 1.0e9f);
-    int imin = (int)imin_d, imax = (int)imax_d, jmin = (int)jmin_d, jmax = (int)jmax_d;
-    ob = (imin < -n || imax > n || jmin < -n || jmax > n);
-    int i1 = max(imin, -n), i2 = min(imax, n), j1 = max(jmin, -n), j2 = min(jmax, n);
-    if (i1 > n || i2 < -n || j1 > n || j2 < -n) {
-      o0 = imin_d; o1 = imax_d; o2 = jmin_d; o3 = jmax_d;
-      xl0 = min(max(imin, -n), n); xl1 = max(min(imax, n), -n);
-      yl0 = min(max(jmin, -n), n); yl1 = max(min(jmax, n), -n);
-    } else if (pass == 1) {
-      // one flat loop over the bounding box: a wave runs max(cells) iterations,
-      // not max(rows) * max(columns)
-      int nc = (i2 - i1 + 1) * (j2 - j1 + 1);
-      bool hit = false;
-      for (int c = 0, i = i1, j = j1; c < nc; c++) {
-        float s, t;
-        float m = bary32(P, i, j, s, t);
-        long long pix = (long long)(i + n) * w + (j + n);
-        if (m >= AMB_TOL || (src && m >= -AMB_TOL)) {
-          hit = true;
-          atomicMax(&kb[pix], (unsigned long long)pack_key(depth32(P, s, t), f));
-        }
-        bool marginal = !src && m >= -AMB_TOL && m < AMB_TOL;
-        hit = hit || marginal;
-        mark_marginal(V, pd, marginal, (int)pix, i, j);
-        if (++j > j2) { j = j1; i++; }
+  if (!(f < G.nf && project32(V, G, f, src ? V.sa32 : V.oa32, pass == 3 ? -FRONT_TOL : 0.0f, P))) return;
+  float imin_d = floorf(fminf(P.x0, fminf(P.x1, P.x2)) + 0.5f);
+  float imax_d = floorf(fmaxf(P.x0, fmaxf(P.x1, P.x2)) + 0.5f);
+  float jmin_d = floorf(fminf(P.y0, fminf(P.y1, P.y2)) + 0.5f);
+  float jmax_d = floorf(fmaxf(P.y0, fmaxf(P.y1, P.y2)) + 0.5f);
+  imin_d = fmaxf(imin_d, -1.0e9f); imax_d = fminf(imax_d, 1.0e9f);
+  jmin_d = fmaxf(jmin_d, -1.0e9f); jmax_d = fminf(jmax_d, 1.0e9f);
+  int imin = (int)imin_d, imax = (int)imax_d, jmin = (int)jmin_d, jmax = (int)jmax_d;
+  L.ob = (imin < -n || imax > n || jmin < -n || jmax > n);
+  int i1 = max(imin, -n), i2 = min(imax, n), j1 = max(jmin, -n), j2 = min(jmax, n);
+  if (i1 > n || i2 < -n || j1 > n || j2 < -n) {
+    L.o0 = imin_d; L.o1 = imax_d; L.o2 = jmin_d; L.o3 = jmax_d;
+    L.xl0 = min(max(imin, -n), n); L.xl1 = max(min(imax, n), -n);
+    L.yl0 = min(max(jmin, -n), n); L.yl1 = max(min(jmax, n), -n);
+  } else if (pass == 1) {
+    unsigned long long *kb = pd.zkey + V.pos_off + (src ? sun_off : 0);
+    // one flat loop over the bounding box: a wave runs max(cells) iterations,
+    // not max(rows) * max(columns)
+    int nc = (i2 - i1 + 1) * (j2 - j1 + 1);
+    bool hit = false;
+    for (int c = 0, i = i1, j = j1; c < nc; c++) {
+      float s, t;
+      float m = bary32(P, i, j, s, t);
+      long long pix = (long long)(i + n) * w + (j + n);
+      if (m >= AMB_TOL || (src && m >= -AMB_TOL)) {
+        hit = true;
+        atomicMax(&kb[pix], (unsigned long long)pack_key(depth32(P, s, t), f));
       }
-      // The model rectangle grows by the box of every facet that covers a
-      // pixel: at most a pixel wider than the covered pixels themselves, which
-      // only widens the resolve's tile range.  Pixel positions inside the POS
-      // never change the posbnd factor, so the overall extent takes boxes of
-      // facets wholly outside only (above).
-      if (hit) { xl0 = i1; xl1 = i2; yl0 = j1; yl1 = j2; }
-    } else if (pass == 3) {
-      int tw = (2 * n + STALE_TILE) / STALE_TILE;
-      const unsigned char *tb = pd.stale_tiles + V.tile_off;
-      bool any = false;
-      for (int a = (i1 + n) / STALE_TILE; a <= (i2 + n) / STALE_TILE && !any; a++)
-        for (int b = (j1 + n) / STALE_TILE; b <= (j2 + n) / STALE_TILE; b++)
-          if (tb[(long long)a * tw + b]) { any = true; break; }
-      if (any) {
-        int nc = (i2 - i1 + 1) * (j2 - j1 + 1);
-        for (int c = 0, i = i1, j = j1; c < nc; c++) {
-          // candidates: a superset of the facets covering the pixel in FP64
-          float s, t;
-          if (bary32(P, i, j, s, t) >= -AMB_TOL) {
-            int si = pd.stale_idx[V.pos_off + (long long)(i + n) * w + (j + n)];
-            if (si >= 0) {
-              int q = atomicAdd(&pd.stale[si].ncand, 1);
-              if (q < STALE_CAND) pd.stale[si].cand[q] = f;
-            }
+      bool marginal = !src && m >= -AMB_TOL && m < AMB_TOL;
+      hit = hit || marginal;
+      mark_marginal(V, pd, marginal, (int)pix, i, j);
+      if (++j > j2) { j = j1; i++; }
+    }
+    // The model rectangle grows by the box of every facet that covers a
+    // pixel: at most a pixel wider than the covered pixels themselves, which
+    // only widens the resolve's tile range.  Pixel positions inside the POS
+    // never change the posbnd factor, so the overall extent takes boxes of
+    // facets wholly outside only (above).
+    if (hit) { L.xl0 = i1; L.xl1 = i2; L.yl0 = j1; L.yl1 = j2; }
+  } else if (pass == 3) {
+    int tw = (2 * n + STALE_TILE) / STALE_TILE;
+    const unsigned char *tb = pd.stale_tiles + V.tile_off;
+    bool any = false;
+    for (int a = (i1 + n) / STALE_TILE; a <= (i2 + n) / STALE_TILE && !any; a++)
+      for (int b = (j1 + n) / STALE_TILE; b <= (j2 + n) / STALE_TILE; b++)
+        if (tb[(long long)a * tw + b]) { any = true; break; }
+    if (any) {
+      int nc = (i2 - i1 + 1) * (j2 - j1 + 1);
+      for (int c = 0, i = i1, j = j1; c < nc; c++) {
+        // candidates: a superset of the facets covering the pixel in FP64
+        float s, t;
+        if (bary32(P, i, j, s, t) >= -AMB_TOL) {
+          int si = pd.stale_idx[V.pos_off + (long long)(i + n) * w + (j + n)];
+          if (si >= 0) {
+            int q = atomicAdd(&pd.stale[si].ncand, 1);
+            if (q < STALE_CAND) pd.stale[si].cand[q] = f;
           }
-          if (++j > j2) { j = j1; i++; }
         }
+        if (++j > j2) { j = j1; i++; }
       }
     }
   }
-  if (pass == 3) return;
-  xl0 = wmin(xl0); xl1 = wmax(xl1); yl0 = wmin(yl0); yl1 = wmax(yl1); ob = wmax(ob);
-  o0 = wminf(o0); o1 = wmaxf(o1); o2 = wminf(o2); o3 = wmaxf(o3);
-  // one update per wave, skipped when a read of the current value shows it
-  // cannot move the limits (they only move outward)
+}
+
+// Fold the lanes' rectangles, extents and posbnd flags into the view's: one
+// update per wave, skipped when a read of the current value shows it cannot
+// move the limits (they only move outward).  Every lane of the wave calls it.
+__device__ inline void raster_limits(ViewRec &V, int src, RasterLim L) {
+  const int n = V.n;
+  L.xl0 = wmin(L.xl0); L.xl1 = wmax(L.xl1); L.yl0 = wmin(L.yl0); L.yl1 = wmax(L.yl1); L.ob = wmax(L.ob);
+  L.o0 = wminf(L.o0); L.o1 = wmaxf(L.o1); L.o2 = wminf(L.o2); L.o3 = wmaxf(L.o3);
   if ((threadIdx.x & 63) == 0) {
     int *xl = src ? V.xlim2 : V.xlim, *yl = src ? V.ylim2 : V.ylim;
     unsigned long long *ov = src ? V.ov2 : V.ov;
     int *obp = src ? &V.outbnd2 : &V.outbnd;
-    if (xl0 < n && xl0 < xl[0]) atomicMin(&xl[0], xl0);
-    if (xl1 > -n && xl1 > xl[1]) atomicMax(&xl[1], xl1);
-    if (yl0 < n && yl0 < yl[0]) atomicMin(&yl[0], yl0);
-    if (yl1 > -n && yl1 > yl[1]) atomicMax(&yl[1], yl1);
-    if (ob && !*obp) atomicOr(obp, 1);
+    if (L.xl0 < n && L.xl0 < xl[0]) atomicMin(&xl[0], L.xl0);
+    if (L.xl1 > -n && L.xl1 > xl[1]) atomicMax(&xl[1], L.xl1);
+    if (L.yl0 < n && L.yl0 < yl[0]) atomicMin(&yl[0], L.yl0);
+    if (L.yl1 > -n && L.yl1 > yl[1]) atomicMax(&yl[1], L.yl1);
+    if (L.ob && !*obp) atomicOr(obp, 1);
     unsigned long long k;
-    if (o0 < 1e30f && (k = ord64((double)o0)) < ov[0]) atomicMin(&ov[0], k);
-    if (o1 > -1e30f && (k = ord64(-(double)o1)) < ov[1]) atomicMin(&ov[1], k);
-    if (o2 < 1e30f && (k = ord64((double)o2)) < ov[2]) atomicMin(&ov[2], k);
-    if (o3 > -1e30f && (k = ord64(-(double)o3)) < ov[3]) atomicMin(&ov[3], k);
+    if (L.o0 < 1e30f && (k = ord64((double)L.o0)) < ov[0]) atomicMin(&ov[0], k);
+    if (L.o1 > -1e30f && (k = ord64(-(double)L.o1)) < ov[1]) atomicMin(&ov[1], k);
+    if (L.o2 < 1e30f && (k = ord64((double)L.o2)) < ov[2]) atomicMin(&ov[2], k);
+    if (L.o3 > -1e30f && (k = ord64(-(double)L.o3)) < ov[3]) atomicMin(&ov[3], k);
+  }
+}
+
+// pass 1 over every facet of every view that is not binned: XCD-local
+// eighths of the model (each XCD's L2 then keeps one eighth of it)
+__global__ __launch_bounds__(256) void k_raster32(ViewRec *__restrict__ views, const GeomSlot *slots, PosDev pd,
+                                                  long long sun_off, int src) {
+  ViewRec &V = views[blockIdx.y];
+  if (V.type != VT_DELAY && V.type != VT_DOPPLER) return;  // lightcurve views take the FP64 kernels
+  if ((src && !V.bistatic) || V.binned) return;
+  const int f = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+  RasterLim L;
+  raster_facet(V, slots[V.slot], pd, sun_off, src, 1, f, L);
+  raster_limits(V, src, L);
+}
+
+// pass 0 over the bin blocks (view, block) that k_bin32 found reaching
+// outside their view's POS window; one wave per block
+__global__ __launch_bounds__(256) void k_outside32(ViewRec *__restrict__ views, const GeomSlot *slots, PosDev pd,
+                                                   const int2 *list, const int *count) {
+  const int nl = *count;
+  for (int e = blockIdx.x * 4 + (threadIdx.x >> 6); e < nl; e += gridDim.x * 4) {
+    const int2 vb = list[e];
+    ViewRec &V = views[vb.x];
+    RasterLim L;
+    raster_facet(V, slots[V.slot], pd, 0, 0, 0, vb.y * BF + (threadIdx.x & 63), L);
+    raster_limits(V, 0, L);
+  }
+}
+
+// pass 3, grid-stride over (view, 256 facets): a fixed small grid, so an
+// empty replay list costs one early exit per workgroup
+__global__ __launch_bounds__(256) void k_collect32(ViewRec *__restrict__ views, const GeomSlot *slots, PosDev pd,
+                                                   int nviews, int nchunk) {
+  if (*pd.ncollect == 0) return;
+  for (long long w = blockIdx.x; w < (long long)nviews * nchunk; w += gridDim.x) {
+    ViewRec &V = views[w / nchunk];
+    if (V.type != VT_DELAY && V.type != VT_DOPPLER) continue;
+    RasterLim L;
+    raster_facet(V, slots[V.slot], pd, 0, 0, 3, (int)(w % nchunk) * 256 + threadIdx.x, L);
   }
 }
 
@@ -249,18 +286,20 @@ int enqueue_mixed(Engine &E) {
     SHP_LAUNCH(E, "k_block_bounds", k_block_bounds, dim3(cdivm(E.max_nblk, 4), E.nslots), bt, 0, st, E.d_slots,
                E.nslots);
     SHP_LAUNCH(E, "k_bin32", k_bin32, dim3(cdivm(E.max_nblk, 256), E.nviews), bt, 0, st, E.d_views, E.d_slots,
-               E.d_bin_cnt, E.d_bin_list);
-    SHP_LAUNCH(E, "k_raster32/0", k_raster32, gf, bt, 0, st, E.d_views, E.d_slots, E.pd, E.total_pix, 0, 0);
+               E.d_bin_cnt, E.d_bin_list, E.d_out_list, E.pd.nstale + 4);
+    SHP_LAUNCH(E, "k_outside32", k_outside32, dim3(512), bt, 0, st, E.d_views, E.d_slots, E.pd, E.d_out_list,
+               E.pd.nstale + 4);
     SHP_LAUNCH(E, "k_tile32", k_tile32, dim3(pad_xcd((unsigned)(E.max_bin_ntx * E.max_bin_ntx)), E.nviews), bt, 0, st,
                E.d_views, E.d_slots, E.pd, E.d_bin_cnt, E.d_bin_list, E.d_fit, E.d_frames, E.d_overflow, E.d_params,
                E.pd.nstale);
   }
   if (E.any_global) {
-    SHP_LAUNCH(E, "k_raster32/1", k_raster32, gf, bt, 0, st, E.d_views, E.d_slots, E.pd, E.total_pix, 0, 1);
+    SHP_LAUNCH(E, "k_raster32", k_raster32, gf, bt, 0, st, E.d_views, E.d_slots, E.pd, E.total_pix, 0);
     SHP_LAUNCH(E, "k_shade_radar32", k_shade_radar32, dim3(tiles * tiles, E.nviews), bt, 0, st, E.d_views, E.d_slots,
                E.pd, E.d_fit, E.d_frames, E.d_overflow, E.d_params, E.pd.nstale);
   }
-  SHP_LAUNCH(E, "k_raster32/3", k_raster32, gf, bt, 0, st, E.d_views, E.d_slots, E.pd, E.total_pix, 0, 3);
+  SHP_LAUNCH(E, "k_collect32", k_collect32, dim3(2048), bt, 0, st, E.d_views, E.d_slots, E.pd, E.nviews,
+             (int)cdivm(E.max_nf, 256));
   SHP_LAUNCH(E, "k_replay32", k_replay32, dim3(std::min(cdivm(E.pd.stale_cap, 256), 2048u)), bt, 0, st, E.d_views, E.d_slots, E.pd,
                      E.d_fit, E.d_frames, E.d_overflow, E.d_params);
   if (E.any_lc) {
@@ -294,7 +333,7 @@ int enqueue_mixed(Engine &E) {
     SHP_LAUNCH(E, "k_chi2_frames64", k_chi2_frames64, dim3(E.nframes), bt, 0, st, E.d_frames, E.nframes, E.d_fit,
                E.d_obs, E.d_oov, E.d_view_out, 0, 0, 1);
   }
-  SHP_LAUNCH(E, "k_objective", k_objective, dim3(cdivm(E.nevals, 64)), dim3(64), 0, st, E.d_evals, E.nevals, E.d_frames,
+  SHP_LAUNCH(E, "k_objective", k_objective, dim3(std::max(1, E.nevals)), dim3(256), 0, st, E.d_evals, E.nevals, E.d_frames,
                      E.d_views, E.d_view_out, E.d_slot_flags, E.d_slot_logf, E.d_eval_slot);
   return hipGetLastError() == hipSuccess ? 0 : -42;
 }

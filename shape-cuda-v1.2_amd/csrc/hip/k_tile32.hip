@@ -14,9 +14,9 @@
 //                   FP64 replay (k_replay32)
 //
 // Facets wholly or partly outside the POS window (posbnd, posvis.c:172-196)
-// are handled by k_raster32 pass 0, which skips every block whose rectangle
-// lies inside the window.  A tile whose lists overflow falls back to testing
-// every block, or to the global replay records of k_raster32 pass 3 /
+// are handled by k_outside32, over the blocks k_bin32 lists as reaching
+// outside the window.  A tile whose lists overflow falls back to testing
+// every block, or to the global replay records of k_collect32 /
 // k_replay32; results do not depend on which route a pixel takes.
 #include "mixed_dev.h"
 
@@ -102,7 +102,8 @@ __device__ inline void blk_minmax4(int &a, int &b, int &c, int &d, int (*sh)[4])
 }
 
 __global__ __launch_bounds__(256) void k_bin32(const ViewRec *__restrict__ views, const GeomSlot *slots,
-                                               int *bin_cnt, unsigned short *bin_list) {
+                                               int *bin_cnt, unsigned short *bin_list, int2 *out_list,
+                                               int *out_count) {
   const ViewRec &V = views[blockIdx.y];
   if (!V.binned) return;
   const GeomSlot &G = slots[V.slot];
@@ -113,6 +114,10 @@ __global__ __launch_bounds__(256) void k_bin32(const ViewRec *__restrict__ views
   int tx0 = 0x7fffffff, tx1 = -1, ty0 = 0x7fffffff, ty1 = -1;
   int i0, i1, j0, j1;
   if (b < nb && block_rect(V, G, b, i0, i1, j0, j1)) {
+    // a block reaching outside the POS window: its facets' posbnd and
+    // outside extent are taken by k_outside32 (rare; one slot per block
+    // of every view exists)
+    if (i0 < -n || i1 > n || j0 < -n || j1 > n) out_list[atomicAdd(out_count, 1)] = make_int2(blockIdx.y, b);
     i0 = max(i0, -n); i1 = min(i1, n); j0 = max(j0, -n); j1 = min(j1, n);
     if (i0 <= i1 && j0 <= j1) {
       tx0 = (i0 + n) / BT; tx1 = (i1 + n) / BT;
@@ -177,7 +182,7 @@ __device__ inline void ev_append(bool need, unsigned int e, unsigned int *ev, in
 
 // Record pixel (i, j) (local index lp) for the FP64 replay: in the tile's LDS
 // list while its events are complete and the list has room, else in the
-// global replay records (k_raster32 pass 3 collects its candidates).
+// global replay records (k_collect32 collects its candidates).
 __device__ inline void tile_stale(const ViewRec &V, const PosDev &pd, bool ev_ok, int lp, int i, int j,
                                   unsigned char *flag, short *spix, int *nts) {
   if (ev_ok) {
@@ -287,7 +292,7 @@ __global__ __launch_bounds__(256, 6) void k_tile32(ViewRec *__restrict__ views, 
           const float R = fb.w * sc * 1.0001f + 0.01f + 1.0e-6f * (fabsf(px) + fabsf(py));
           keep = !(floorf(px + R + 0.5f) < (float)i0 || floorf(px - R + 0.5f) > (float)(i0 + BT - 1) ||
                    floorf(py + R + 0.5f) < (float)j0 || floorf(py - R + 0.5f) > (float)(j0 + BT - 1)) &&
-                 m[6] * fn.x + m[7] * fn.y + m[8] * fn.z > -FRONT_TOL;
+                 facing32(m, fn) > -FRONT_TOL;  // as project32nf
         }
         lds_append(keep, f, flist, &nfl, FL_CAP);
       }
@@ -503,7 +508,7 @@ __global__ __launch_bounds__(256, 6) void k_tile32(ViewRec *__restrict__ views, 
       for (int s = wv; s < nt; s += 4) {
         StaleRec &S = pd.stale[base + s];
         const int nc = scnt[s];
-        const bool own = nc <= TS_CAND;  // else the global raster collects them (pass 3)
+        const bool own = nc <= TS_CAND;  // else k_collect32 collects them
         if (own)
           for (int q = lane; q < nc; q += 64) S.cand[q] = cand[s][q];
         if (lane == 0) {

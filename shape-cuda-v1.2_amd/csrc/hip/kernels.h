@@ -46,7 +46,7 @@ struct PosDev {
   unsigned char *stale_tiles;// mixed path: per view (at ViewRec::tile_off), 1 where an 8x8 tile holds a stale pixel
   StaleRec *stale;
   int *nstale;               // replay records in use
-  int *ncollect;             // mixed path: records whose candidates the global raster (pass 3) collects
+  int *ncollect;             // mixed path: records whose candidates the global raster (k_collect32) collects
   int stale_cap;
   int *error;                // bit flags: 1 stale list full, 2 too many candidates
 };

@@ -41,10 +41,17 @@ __device__ inline void blk_min4(int &a, int &b, int &c, int &d, int (*sh)[4]) {
   __syncthreads();
 }
 
+// The FP32 geometry below (projection, barycentric test, depth, normals)
+// spells out its fused multiply-adds and turns contraction off otherwise:
+// left to the compiler, contraction differs between kernels and inlining
+// sites, and a pixel at a claim threshold (AMB_TOL, facing) would then take
+// different routes in the binned and global-z-buffer rasters.  Written out,
+// every kernel computes the same bits.
 __device__ inline void rot3f(const float m[9], float x, float y, float z, float &a, float &b, float &c) {
-  a = m[0] * x + m[1] * y + m[2] * z;
-  b = m[3] * x + m[4] * y + m[5] * z;
-  c = m[6] * x + m[7] * y + m[8] * z;
+#pragma clang fp contract(off)
+  a = fmaf(m[2], z, fmaf(m[1], y, m[0] * x));
+  b = fmaf(m[5], z, fmaf(m[4], y, m[3] * x));
+  c = fmaf(m[8], z, fmaf(m[7], y, m[6] * x));
 }
 
 struct Proj32 {
@@ -54,12 +61,13 @@ struct Proj32 {
 
 // project a facet's corners a, b, c (posvis.c:159-166) in pixel units
 __device__ inline void corners32p(const ViewRec &V, float4 a, float4 b, float4 c, const float *m, Proj32 &P) {
+#pragma clang fp contract(off)
   float s = V.inv_kmpp32;
   rot3f(m, a.x, a.y, a.z, P.x0, P.y0, P.z0);
   rot3f(m, b.x, b.y, b.z, P.x1, P.y1, P.z1);
   rot3f(m, c.x, c.y, c.z, P.x2, P.y2, P.z2);
   P.x0 *= s; P.y0 *= s; P.x1 *= s; P.y1 *= s; P.x2 *= s; P.y2 *= s;
-  P.den = 1.0f / ((P.x1 - P.x0) * (P.y2 - P.y1) - (P.x2 - P.x1) * (P.y1 - P.y0));
+  P.den = 1.0f / fmaf(P.x1 - P.x0, P.y2 - P.y1, -((P.x2 - P.x1) * (P.y1 - P.y0)));
 }
 __device__ inline void corners32(const ViewRec &V, const GeomSlot &G, int f, const float *m, Proj32 &P) {
   const int *fv = G.fv + 3 * f;
@@ -71,11 +79,17 @@ __device__ inline void corners32f(const ViewRec &V, const GeomSlot &G, int f, co
 }
 
 constexpr float FRONT_TOL = 1.0e-4f;
+// line-of-sight component of facet normal fn (the facing test of the raster)
+__device__ inline float facing32(const float *m, float4 fn) {
+#pragma clang fp contract(off)
+  return fmaf(m[8], fn.z, fmaf(m[7], fn.y, m[6] * fn.x));
+}
 // as project32, also returning the facet normal's line-of-sight component
 __device__ inline bool project32n(const ViewRec &V, const GeomSlot &G, int f, const float *m, float nz_min,
                                   Proj32 &P, float &nz) {
+#pragma clang fp contract(off)
   float4 fn = G.fn32[f];
-  nz = m[6] * fn.x + m[7] * fn.y + m[8] * fn.z;
+  nz = facing32(m, fn);
   if (!(nz > nz_min)) return false;
   corners32(V, G, f, m, P);
   return true;
@@ -83,15 +97,17 @@ __device__ inline bool project32n(const ViewRec &V, const GeomSlot &G, int f, co
 // as project32n, from the facet-major vertex copies
 __device__ inline bool project32nf(const ViewRec &V, const GeomSlot &G, int f, const float *m, float nz_min,
                                    Proj32 &P, float &nz) {
+#pragma clang fp contract(off)
   float4 fn = G.fn32[f];
-  nz = m[6] * fn.x + m[7] * fn.y + m[8] * fn.z;
+  nz = facing32(m, fn);
   if (!(nz > nz_min)) return false;
   corners32f(V, G, f, m, P);
   return true;
 }
 __device__ inline bool project32(const ViewRec &V, const GeomSlot &G, int f, const float *m, float nz_min, Proj32 &P) {
+#pragma clang fp contract(off)
   float4 fn = G.fn32[f];
-  float nz = m[6] * fn.x + m[7] * fn.y + m[8] * fn.z;
+  float nz = facing32(m, fn);
   if (!(nz > nz_min)) return false;  // back-facing (posvis.c:152)
   corners32(V, G, f, m, P);
   return true;
@@ -135,13 +151,15 @@ __device__ inline bool block_rect(const ViewRec &V, const GeomSlot &G, int b, in
 // >= 5x the FP32 error for facets of >= 2 px.
 constexpr float AMB_TOL = 2.0e-4f;
 __device__ inline float bary32(const Proj32 &P, int i, int j, float &s, float &t) {
+#pragma clang fp contract(off)
   float xi = (float)i, yj = (float)j;
-  s = ((xi - P.x0) * (P.y2 - P.y1) - (P.x2 - P.x1) * (yj - P.y0)) * P.den;
-  t = ((P.x1 - P.x0) * (yj - P.y0) - (xi - P.x0) * (P.y1 - P.y0)) * P.den;
+  s = fmaf(xi - P.x0, P.y2 - P.y1, -((P.x2 - P.x1) * (yj - P.y0))) * P.den;
+  t = fmaf(P.x1 - P.x0, yj - P.y0, -((xi - P.x0) * (P.y1 - P.y0))) * P.den;
   return fminf(t, fminf(s - t, 1.0f - s));
 }
 __device__ inline float depth32(const Proj32 &P, float s, float t) {
-  return P.z0 + s * (P.z1 - P.z0) + t * (P.z2 - P.z1);
+#pragma clang fp contract(off)
+  return fmaf(t, P.z2 - P.z1, fmaf(s, P.z1 - P.z0, P.z0));
 }
 
 // tile flag of pixel (i, j) in the view's 8x8-tile bitmap
@@ -197,38 +215,40 @@ __device__ inline void mark_marginal(const ViewRec &V, const PosDev &pd, bool ma
 // vn: the facet's 3 vertex normals (smooth) from the facet-major copies
 __device__ inline void normal32f(const GeomSlot &G, int f, const float *m, int smooth, float s, float t, float &nx,
                                  float &ny, float &nz) {
+#pragma clang fp contract(off)
   float bx, by, bz;
   if (smooth) {
     float4 na = G.fvn[3 * f], nb = G.fvn[3 * f + 1], nc = G.fvn[3 * f + 2];
-    bx = na.x + s * (nb.x - na.x) + t * (nc.x - nb.x);
-    by = na.y + s * (nb.y - na.y) + t * (nc.y - nb.y);
-    bz = na.z + s * (nb.z - na.z) + t * (nc.z - nb.z);
+    bx = fmaf(t, nc.x - nb.x, fmaf(s, nb.x - na.x, na.x));
+    by = fmaf(t, nc.y - nb.y, fmaf(s, nb.y - na.y, na.y));
+    bz = fmaf(t, nc.z - nb.z, fmaf(s, nb.z - na.z, na.z));
   } else {
     float4 fn = G.fn32[f];
     bx = fn.x; by = fn.y; bz = fn.z;
   }
   rot3f(m, bx, by, bz, nx, ny, nz);
   if (smooth) {
-    float inv = rsqrtf(nx * nx + ny * ny + nz * nz);
+    float inv = rsqrtf(fmaf(nz, nz, fmaf(ny, ny, nx * nx)));
     nx *= inv; ny *= inv; nz *= inv;
   }
 }
 __device__ inline void normal32(const GeomSlot &G, int f, const float *m, int smooth, float s, float t, float &nx,
                                 float &ny, float &nz) {
+#pragma clang fp contract(off)
   float bx, by, bz;
   if (smooth) {
     const int *fv = G.fv + 3 * f;
     float4 na = G.vn32[fv[0]], nb = G.vn32[fv[1]], nc = G.vn32[fv[2]];
-    bx = na.x + s * (nb.x - na.x) + t * (nc.x - nb.x);
-    by = na.y + s * (nb.y - na.y) + t * (nc.y - nb.y);
-    bz = na.z + s * (nb.z - na.z) + t * (nc.z - nb.z);
+    bx = fmaf(t, nc.x - nb.x, fmaf(s, nb.x - na.x, na.x));
+    by = fmaf(t, nc.y - nb.y, fmaf(s, nb.y - na.y, na.y));
+    bz = fmaf(t, nc.z - nb.z, fmaf(s, nb.z - na.z, na.z));
   } else {
     float4 fn = G.fn32[f];
     bx = fn.x; by = fn.y; bz = fn.z;
   }
   rot3f(m, bx, by, bz, nx, ny, nz);
   if (smooth) {
-    float inv = rsqrtf(nx * nx + ny * ny + nz * nz);
+    float inv = rsqrtf(fmaf(nz, nz, fmaf(ny, ny, nx * nx)));
     nx *= inv; ny *= inv; nz *= inv;
   }
 }
