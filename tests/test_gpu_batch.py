@@ -68,6 +68,8 @@ def test_fp64_batch_matches_oracle(config, over):
 
 @pytest.mark.parametrize("config,over", [(1, {}), (2, dict(n_deldop=16)),
                                          (1, dict(n_deldop=0, n_doppler=2, nbin_min=30, nbin_max=40)),
+                                         # the model reaches past a 41x41 POS: posbnd
+                                         (1, dict(pos_n=20, n_deldop=2)),
                                          (5, dict(n_deldop=6)),
                                          (3, dict(n_deldop=3, n_doppler=3, n_lightcurves=2, lc_points=30,
                                                   lc_ncalc=10))])
@@ -119,6 +121,8 @@ def test_multi_model_batch(method, tol):
                                          (1, dict(n_deldop=0, n_doppler=2, nbin_min=30, nbin_max=40)),
                                          (3, dict(n_deldop=3, n_doppler=3, n_lightcurves=2, lc_points=30,
                                                   lc_ncalc=10)),
+                                         # the model reaches past a 41x41 POS (posbnd, k_outside32)
+                                         (1, dict(pos_n=20, n_deldop=2)),
                                          # 81,920 facets on a 63x63 POS: tile lists overflow and the
                                          # tiles test every block
                                          (3, dict(n_deldop=3, n_doppler=0, n_lightcurves=0, pos_n=31,
