@@ -84,17 +84,8 @@ __device__ inline float facing32(const float *m, float4 fn) {
 #pragma clang fp contract(off)
   return fmaf(m[8], fn.z, fmaf(m[7], fn.y, m[6] * fn.x));
 }
-// as project32, also returning the facet normal's line-of-sight component
-__device__ inline bool project32n(const ViewRec &V, const GeomSlot &G, int f, const float *m, float nz_min,
-                                  Proj32 &P, float &nz) {
-#pragma clang fp contract(off)
-  float4 fn = G.fn32[f];
-  nz = facing32(m, fn);
-  if (!(nz > nz_min)) return false;
-  corners32(V, G, f, m, P);
-  return true;
-}
-// as project32n, from the facet-major vertex copies
+// as project32 from the facet-major vertex copies, also returning the facet
+// normal's line-of-sight component
 __device__ inline bool project32nf(const ViewRec &V, const GeomSlot &G, int f, const float *m, float nz_min,
                                    Proj32 &P, float &nz) {
 #pragma clang fp contract(off)
