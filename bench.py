@@ -124,23 +124,45 @@ def time_dropin(sc, steps):
 
 
 # Algorithmic HBM bytes per launch of the mixed kernels (one launch covers every
-# evaluation of the batch).  fp: footprint of the replay; claimed: z-buffer keys
-# written by the raster (one per covered pixel).
+# evaluation of the batch), on the path's byte contract: per frame view, 16 B
+# per pixel of the model's POS bounding box (an 8-byte (depth, facet) key
+# written and read once) and 8 B per image bin (FP32 obs*w and w); per
+# evaluation, 28 B per vertex read and 24 B written by the realization and
+# 12 B per facet of indices.  fp: footprint of the replay (bbox_pix and
+# image_cells summed over the views; nv and nf over the geometry slots).
 def kernel_bytes(name, fp, claimed):
-    model = 16 * fp["nv"] + 28 * fp["nf"]       # float4 vertices; 3 indices + float4 normal per facet
-    if name == "k_raster32/1":
-        return model + 8 * claimed                # one 8-byte key per covered pixel
-    if name == "k_shade_radar32":
-        # read every key of the views' model rectangles, clear the claimed
-        # ones, write each occupied image cell once (FP64)
-        return 8 * fp["bbox_pix"] + 8 * claimed + 8 * fp["window_cells"]
-    if name == "k_tile32":
-        # keys stay in LDS: the model once per evaluation (plus a 16-byte
-        # bounding sphere per facet) and each occupied image cell written once
-        return model + 16 * fp["nf"] + 8 * fp["window_cells"]
+    if name in ("k_tile32", "k_shade_radar32"):
+        # the POS raster and synthesis: the key traffic of the contract (the
+        # binned kernel keeps its keys in LDS, so it moves far less) plus the
+        # realized vertices (positions and normals) and facet indices it reads
+        return 16 * fp["bbox_pix"] + 24 * fp["nv"] + 12 * fp["nf"]
     if name == "k_chi2_mixed":
-        return 24 * fp["window_cells"]            # fit read + clear (FP64), obs*w and w (FP32)
+        return 8 * fp["image_cells"]              # FP32 obs*w and w per bin
     return None
+
+
+def path_bytes(fp, nevals, nv, nf):
+    """The contract's bytes for one replay of nevals evaluations (B_eval summed)."""
+    return nevals * (52 * nv + 12 * nf) + 16 * fp["bbox_pix"] + 8 * fp["image_cells"]
+
+
+def image_error(sc, evals, nframes, stride=32):
+    """Largest |mixed - FP64| / peak(FP64) over every stride-th frame's model
+    image, for the first evaluation (the per-pixel image contract: <= 1e-4)."""
+    Bm = sh.Batch([sc], evals, method=sh.METHOD_MIXED, flags=sh.BATCH_KEEP_FIT)
+    Bf = sh.Batch([sc], evals, method=sh.METHOD_FP64, flags=sh.BATCH_KEEP_FIT)
+    Bm.run()
+    Bf.run()
+    worst = 0.0
+    frames = sc.all_frames()
+    for q in range(0, min(nframes, len(frames)), stride):
+        s_, f_ = frames[q]
+        shape = sc.image(s_, max(f_, 0), 0).shape
+        im, imf = Bm.fit(0, q, shape), Bf.fit(0, q, shape)
+        worst = max(worst, float(np.max(np.abs(im - imf)) / max(float(np.max(np.abs(imf))), 1e-300)))
+    Bm.close()
+    Bf.close()
+    return worst
 
 
 def kernel_roofline(sc, nev):
@@ -307,6 +329,13 @@ def main():
                        "algorithmic_bytes": r["bytes"]}
     out["kernels"] = rows
     out["footprint"] = fp
+    pb = path_bytes(fp, a.evals, sc.nv, sc.nf)
+    pach = pb / (ms_step * 1e-3) / 1e9
+    out["path_roofline"] = {"bytes_per_replay": pb, "bytes_per_eval": pb // a.evals, "ms_per_replay": round(ms_step, 4),
+                            "achieved": round(pach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                            "frac": round(pach / HBM_PEAK_GBS, 4)}
+    log("model images: mixed against FP64 fits, sampled frames")
+    out["image_rel_err_vs_fp64"] = image_error(sc, evals[:1], a.frames)
     sc.close()
 
     if not a.no_cpu:
