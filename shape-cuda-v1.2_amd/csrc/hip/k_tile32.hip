@@ -222,9 +222,12 @@ __global__ __launch_bounds__(256, 6) void k_tile32(ViewRec *__restrict__ views, 
   ViewRec &V = views[blockIdx.y];
   if (!V.binned) return;
   const int n = V.n, ntx = V.bin_ntx;
-  // each XCD takes a contiguous eighth of the view's tiles: neighbouring
-  // tiles share blocks, whose vertices then stay in one L2
-  const int bx = xcd_block(blockIdx.x, gridDim.x);
+  // each XCD takes a contiguous eighth of the view's tiles (neighbouring
+  // tiles share blocks, whose vertices then stay in one L2); which eighth
+  // rotates with the view, so that the edge bands, mostly empty sky, are
+  // spread over all XCDs instead of idling two of them
+  const int per = gridDim.x / NXCD;
+  const int bx = (int)((blockIdx.x % NXCD + blockIdx.y) % NXCD) * per + (int)(blockIdx.x / NXCD);
   if (bx >= ntx * ntx) return;
   const long long bin = V.bin_off + bx;
   const int cnt = bin_cnt[bin];
