@@ -176,9 +176,6 @@ __device__ inline void lds_append(bool need, unsigned int e, void *list, int *cn
     if (q < cap) static_cast<unsigned int *>(list)[q] = e;
   }
 }
-__device__ inline void ev_append(bool need, unsigned int e, unsigned int *ev, int *nev) {
-  lds_append(need, e, ev, nev, EV_CAP);
-}
 
 // Record pixel (i, j) (local index lp) for the FP64 replay: in the tile's LDS
 // list while its events are complete and the list has room, else in the
@@ -324,9 +321,11 @@ __global__ __launch_bounds__(256, 6) void k_tile32(ViewRec *__restrict__ views, 
         }
         const bool front = nz > 0.0f;
         const int nc = (ti1 <= ti2 && tj1 <= tj2) ? (ti2 - ti1 + 1) * (tj2 - tj1 + 1) : 0;
-        // covered cells (cell c of the box: bit c); boxes over 64 cells, rare,
-        // append their further events one by one
-        const bool big = __ballot(nc > 64) != 0;
+        // covered cells (cell c of the box: bit c).  A box over 64 cells
+        // (a facet wider than 8 pixels, none at config 4) leaves the tile's
+        // event list incomplete: its replayed pixels then take the global
+        // candidate collection (k_collect32)
+        if (nc > 64) atomicMax(&nev, EV_CAP + 1);  // (other waves add to nev concurrently)
         unsigned long long covm = 0ULL;
         bool hit = false;
         for (int c = 0, i = ti1, j = tj1; c < nc; c++) {
@@ -342,7 +341,6 @@ __global__ __launch_bounds__(256, 6) void k_tile32(ViewRec *__restrict__ views, 
             hit = true;
           }
           if (cov && c < 64) covm |= 1ULL << c;
-          if (big) ev_append(cov && c >= 64, ((unsigned int)lp << EV_FACET_BITS) | (unsigned int)f, ev, &nev);
           if (++j > tj2) { j = tj1; i++; }
         }
         if (hit) {
