@@ -61,6 +61,7 @@ struct GeomSlot {
   float4 *x32, *vn32;    // nv
   float4 *fn32;          // nf
   float4 *blk;           // mixed method: per 64-facet block, {center, radius} and {cone axis, min cos}
+  float4 *sblk;          // mixed method: per 16-facet quarter of each block, {center, radius}
   float4 *fb;            // mixed method: per facet, bounding sphere {center, radius}
   float4 *fvx, *fvn;     // mixed method: per facet, its 3 vertices' x32 / vn32 (one gather, not two)
   unsigned char *seen;   // nf

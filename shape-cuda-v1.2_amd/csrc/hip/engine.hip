@@ -546,6 +546,7 @@ int Engine::build(struct par_t *par, struct mod_t **mods, struct dat_t **dats, i
       int nblk = (G.nf + BF - 1) / BF;
       max_nblk = std::max(max_nblk, nblk);
       if ((rc = alloc((void **)&G.blk, sizeof(float4) * 2 * (size_t)nblk))) return rc;
+      if ((rc = alloc((void **)&G.sblk, sizeof(float4) * 4 * (size_t)nblk))) return rc;
       if ((rc = alloc((void **)&G.fb, sizeof(float4) * (size_t)G.nf))) return rc;
       if ((rc = alloc((void **)&G.fvx, sizeof(float4) * 3 * (size_t)G.nf))) return rc;
       if ((rc = alloc((void **)&G.fvn, sizeof(float4) * 3 * (size_t)G.nf))) return rc;
@@ -652,7 +653,7 @@ int Engine::build(struct par_t *par, struct mod_t **mods, struct dat_t **dats, i
     // tile lists: counts start (and are left by every tile) at zero
     if ((rc = alloc((void **)&d_bin_cnt, sizeof(int) * (size_t)total_bins))) return rc;
     SHP_CHECK(hipMemsetAsync(d_bin_cnt, 0, sizeof(int) * (size_t)total_bins, stream));
-    if ((rc = alloc((void **)&d_bin_list, sizeof(unsigned short) * BIN_CAP * (size_t)total_bins))) return rc;
+    if ((rc = alloc((void **)&d_bin_list, sizeof(unsigned int) * BIN_CAP * (size_t)total_bins))) return rc;
     // every block of every binned view may reach outside its window
     long long out_cap = 0;
     for (auto &V : h_views)

@@ -95,7 +95,7 @@ class Engine {
   double *d_lc_x = nullptr, *d_lc_t = nullptr, *d_lc_y = nullptr, *d_lc_y2 = nullptr;
   float *d_obs32 = nullptr, *d_oov32 = nullptr;
   int *d_bin_cnt = nullptr;                 // per 32x32 tile of every binned view
-  unsigned short *d_bin_list = nullptr;     // BIN_CAP block ids per tile
+  unsigned int *d_bin_list = nullptr;       // BIN_CAP entries per tile: block id << 4 | quarter mask
   int2 *d_out_list = nullptr;               // (view, block) of blocks reaching outside the POS window
   PosDev pd{};
   std::vector<void *> allocs;

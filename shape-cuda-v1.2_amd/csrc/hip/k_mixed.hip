@@ -37,9 +37,9 @@ __global__ void k_replay32(ViewRec *__restrict__ views, const GeomSlot *slots, P
                            double *overflow, const double *params);
 __global__ void k_block_bounds(GeomSlot *slots, int nslots);
 __global__ void k_bin32(const ViewRec *__restrict__ views, const GeomSlot *slots, int *bin_cnt,
-                        unsigned short *bin_list, int2 *out_list, int *out_count);
+                        unsigned int *bin_list, int2 *out_list, int *out_count);
 __global__ void k_tile32(ViewRec *__restrict__ views, const GeomSlot *slots, PosDev pd, int *bin_cnt,
-                         const unsigned short *bin_list, double *fit, const FrameRec *frames, double *overflow,
+                         const unsigned int *bin_list, double *fit, const FrameRec *frames, double *overflow,
                          const double *params, int *counters);
 
 // ---------------------------------------------------------------------------

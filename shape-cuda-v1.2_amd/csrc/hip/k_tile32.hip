@@ -26,6 +26,9 @@ namespace shp {
 // block bounds: one wave per block, lane = facet
 // ---------------------------------------------------------------------------
 __device__ inline float wsumf(float v) { for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64); return v; }
+// minimum / maximum over each group of 16 lanes
+__device__ inline float seg16_minf(float v) { for (int o = 8; o > 0; o >>= 1) v = fminf(v, __shfl_xor(v, o, 64)); return v; }
+__device__ inline float seg16_maxf(float v) { for (int o = 8; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, 64)); return v; }
 
 __global__ __launch_bounds__(256) void k_block_bounds(GeomSlot *slots, int nslots) {
   const GeomSlot &G = slots[blockIdx.y];
@@ -70,6 +73,27 @@ __global__ __launch_bounds__(256) void k_block_bounds(GeomSlot *slots, int nslot
     float e2 = (vc.x - fx) * (vc.x - fx) + (vc.y - fy) * (vc.y - fy) + (vc.z - fz) * (vc.z - fz);
     G.fb[f] = make_float4(fx, fy, fz, sqrtf(fmaxf(e0, fmaxf(e1, e2))) * 1.00001f + 1.0e-9f);
   }
+  {
+    // the quarter's sphere (16 consecutive facets: lanes 16q .. 16q + 15)
+    float qlo[3], qhi[3];
+    float a[3] = {va.x, va.y, va.z}, bb[3] = {vb.x, vb.y, vb.z}, c[3] = {vc.x, vc.y, vc.z};
+    for (int k = 0; k < 3; k++) {
+      qlo[k] = seg16_minf(ok ? fminf(a[k], fminf(bb[k], c[k])) : 1.0e30f);
+      qhi[k] = seg16_maxf(ok ? fmaxf(a[k], fmaxf(bb[k], c[k])) : -1.0e30f);
+    }
+    float qx = 0.5f * (qlo[0] + qhi[0]), qy = 0.5f * (qlo[1] + qhi[1]), qz = 0.5f * (qlo[2] + qhi[2]);
+    float q2 = 0.0f;
+    if (ok) {
+      float d0x = va.x - qx, d0y = va.y - qy, d0z = va.z - qz;
+      float d1x = vb.x - qx, d1y = vb.y - qy, d1z = vb.z - qz;
+      float d2x = vc.x - qx, d2y = vc.y - qy, d2z = vc.z - qz;
+      q2 = fmaxf(d0x * d0x + d0y * d0y + d0z * d0z,
+                 fmaxf(d1x * d1x + d1y * d1y + d1z * d1z, d2x * d2x + d2y * d2y + d2z * d2z));
+    }
+    q2 = seg16_maxf(q2);
+    if ((threadIdx.x & 15) == 0)
+      G.sblk[4 * b + ((threadIdx.x & 63) >> 4)] = make_float4(qx, qy, qz, sqrtf(q2) * 1.00001f + 1.0e-9f);
+  }
   r2 = wmaxf(r2);
   float r = sqrtf(r2) * 1.00001f + 1.0e-9f;
   float ax = wsumf(fn.x), ay = wsumf(fn.y), az = wsumf(fn.z);
@@ -101,8 +125,20 @@ __device__ inline void blk_minmax4(int &a, int &b, int &c, int &d, int (*sh)[4])
   b = -nb; d = -nd;
 }
 
+static_assert(BF == 64, "a block is one wave of 64 facets, in four quarters of 16 lanes");
+// Which quarters of a block touch tile (x, y): bit q for quarter q, from the
+// quarters' conservative rectangles qr[q] = {i0, i1, j0, j1}
+__device__ inline unsigned quarter_mask(const int (&qr)[4][4], int x, int y, int n) {
+  const int ti0 = x * BT - n, tj0 = y * BT - n;
+  unsigned mk = 0;
+#pragma unroll
+  for (int q = 0; q < 4; q++)
+    if (!(qr[q][0] > ti0 + BT - 1 || qr[q][1] < ti0 || qr[q][2] > tj0 + BT - 1 || qr[q][3] < tj0)) mk |= 1u << q;
+  return mk;
+}
+
 __global__ __launch_bounds__(256) void k_bin32(const ViewRec *__restrict__ views, const GeomSlot *slots,
-                                               int *bin_cnt, unsigned short *bin_list, int2 *out_list,
+                                               int *bin_cnt, unsigned int *bin_list, int2 *out_list,
                                                int *out_count) {
   const ViewRec &V = views[blockIdx.y];
   if (!V.binned) return;
@@ -113,6 +149,7 @@ __global__ __launch_bounds__(256) void k_bin32(const ViewRec *__restrict__ views
   const int n = V.n, ntx = V.bin_ntx;
   int tx0 = 0x7fffffff, tx1 = -1, ty0 = 0x7fffffff, ty1 = -1;
   int i0, i1, j0, j1;
+  int qr[4][4];
   if (b < nb && block_rect(V, G, b, i0, i1, j0, j1)) {
     // a block reaching outside the POS window: its facets' posbnd and
     // outside extent are taken by k_outside32 (rare; one slot per block
@@ -122,6 +159,8 @@ __global__ __launch_bounds__(256) void k_bin32(const ViewRec *__restrict__ views
     if (i0 <= i1 && j0 <= j1) {
       tx0 = (i0 + n) / BT; tx1 = (i1 + n) / BT;
       ty0 = (j0 + n) / BT; ty1 = (j1 + n) / BT;
+#pragma unroll
+      for (int q = 0; q < 4; q++) sphere_rect(V, G.sblk[4 * b + q], qr[q][0], qr[q][1], qr[q][2], qr[q][3]);
     }
   }
   __shared__ int shr[4][4];
@@ -131,20 +170,25 @@ __global__ __launch_bounds__(256) void k_bin32(const ViewRec *__restrict__ views
   if (rx1 < rx0) return;  // nothing of this workgroup lands in the window
   const int rw = ry1 - ry0 + 1, rn = (rx1 - rx0 + 1) * rw;
   const long long base = V.bin_off;
+  // an entry is the block id and the mask of its quarters that touch the
+  // tile; tiles that no quarter touches get no entry
   if (rn > BIN_LDS) {
     // a scattered workgroup (unordered facets): one global atomic per entry
     for (int x = tx0; x <= tx1; x++)
       for (int y = ty0; y <= ty1; y++) {
+        const unsigned mk = quarter_mask(qr, x, y, n);
+        if (!mk) continue;
         long long t = base + (long long)x * ntx + y;
         int q = atomicAdd(&bin_cnt[t], 1);
-        if (q < BIN_CAP) bin_list[t * BIN_CAP + q] = (unsigned short)b;
+        if (q < BIN_CAP) bin_list[t * BIN_CAP + q] = ((unsigned)b << 4) | mk;
       }
     return;
   }
   for (int q = threadIdx.x; q < rn; q += blockDim.x) lc[q] = 0;
   __syncthreads();
   for (int x = tx0; x <= tx1; x++)
-    for (int y = ty0; y <= ty1; y++) atomicAdd(&lc[(x - rx0) * rw + (y - ry0)], 1);
+    for (int y = ty0; y <= ty1; y++)
+      if (quarter_mask(qr, x, y, n)) atomicAdd(&lc[(x - rx0) * rw + (y - ry0)], 1);
   __syncthreads();
   for (int q = threadIdx.x; q < rn; q += blockDim.x) {
     int c = lc[q];
@@ -153,9 +197,11 @@ __global__ __launch_bounds__(256) void k_bin32(const ViewRec *__restrict__ views
   __syncthreads();
   for (int x = tx0; x <= tx1; x++)
     for (int y = ty0; y <= ty1; y++) {
+      const unsigned mk = quarter_mask(qr, x, y, n);
+      if (!mk) continue;
       long long t = base + (long long)x * ntx + y;
       int q = atomicAdd(&lc[(x - rx0) * rw + (y - ry0)], 1);
-      if (q < BIN_CAP) bin_list[t * BIN_CAP + q] = (unsigned short)b;
+      if (q < BIN_CAP) bin_list[t * BIN_CAP + q] = ((unsigned)b << 4) | mk;
     }
 }
 
@@ -213,7 +259,7 @@ __device__ inline void tile_pixel(int m, int &a, int &c) {
 // Six workgroups per CU (LDS 26 KiB each): the register cap of six waves per
 // SIMD costs a few spills but hides more of the gathers' latency (measured).
 __global__ __launch_bounds__(256, 6) void k_tile32(ViewRec *__restrict__ views, const GeomSlot *slots, PosDev pd,
-                                                int *bin_cnt, const unsigned short *bin_list, double *fit,
+                                                int *bin_cnt, const unsigned int *bin_list, double *fit,
                                                 const FrameRec *frames, double *overflow, const double *params,
                                                 int *counters) {
   ViewRec &V = views[blockIdx.y];
@@ -236,7 +282,7 @@ __global__ __launch_bounds__(256, 6) void k_tile32(ViewRec *__restrict__ views, 
   __shared__ unsigned long long key[BT_PIX];
   __shared__ unsigned char flag[BT_PIX];  // bit 0: marginal; bit 7: LDS replay slot in bits 1-6
   __shared__ unsigned int ev[EV_CAP];
-  __shared__ unsigned short blist[BIN_CAP];
+  __shared__ unsigned int blist[BIN_CAP];
   __shared__ double wbuf[WIN_T];          // delay-Doppler window, then replay candidates
   static_assert(TS_CAP * TS_CAND * sizeof(int) <= sizeof(wbuf), "candidates must fit the window buffer");
   static_assert(TS_CAP <= 64, "replay slot must fit flag bits 1-6");
@@ -271,16 +317,18 @@ __global__ __launch_bounds__(256, 6) void k_tile32(ViewRec *__restrict__ views, 
       __syncthreads();
       for (int k = kb + wv; k < kend; k += 4) {
         int b = k;
+        unsigned mk = 15u;  // quarters of the block touching the tile
         if (full) {  // lists overflowed: test every block (wave-uniform)
           int r0, r1, c0, c1;
           if (!block_rect(V, G, b, r0, r1, c0, c1) || r0 > i0 + BT - 1 || r1 < i0 || c0 > j0 + BT - 1 || c1 < j0)
             continue;
         } else {
-          b = blist[k];
+          b = (int)(blist[k] >> 4);
+          mk = blist[k] & 15u;
         }
         const int f = b * BF + lane;
         bool keep = false;
-        if (f < G.nf) {
+        if (f < G.nf && ((mk >> (lane >> 4)) & 1u)) {
           // the facet's sphere against the tile, before any dependent gather
           // (its bounding box, posvis.c:172-180, lies within the projected
           // sphere), and its facing: nearly edge-on facets are replay

@@ -104,6 +104,21 @@ __device__ inline bool project32(const ViewRec &V, const GeomSlot &G, int f, con
   return true;
 }
 
+// Conservative pixel rectangle of a body-frame sphere cr = {center, radius}
+// in view V: it contains the bounding box (posvis.c:172-180) of every facet
+// whose corners lie in the sphere, widened for FP32 rounding.
+__device__ inline void sphere_rect(const ViewRec &V, float4 cr, int &i0, int &i1, int &j0, int &j1) {
+  const float *m = V.oa32;
+  float cx, cy, cz;
+  rot3f(m, cr.x, cr.y, cr.z, cx, cy, cz);
+  float s = V.inv_kmpp32, px = cx * s, py = cy * s;
+  float R = cr.w * s * 1.0001f + 0.01f + 1.0e-6f * (fabsf(px) + fabsf(py));
+  i0 = (int)fminf(fmaxf(floorf(px - R + 0.5f), -1.0e9f), 1.0e9f);
+  i1 = (int)fminf(fmaxf(floorf(px + R + 0.5f), -1.0e9f), 1.0e9f);
+  j0 = (int)fminf(fmaxf(floorf(py - R + 0.5f), -1.0e9f), 1.0e9f);
+  j1 = (int)fminf(fmaxf(floorf(py + R + 0.5f), -1.0e9f), 1.0e9f);
+}
+
 // Conservative pixel rectangle [i0, i1] x [j0, j1] of bin block b in view V:
 // it contains the bounding box (posvis.c:172-180) of every facet of the
 // block, whose corners lie within the block's radius of its center, widened
@@ -119,14 +134,7 @@ __device__ inline bool block_rect(const ViewRec &V, const GeomSlot &G, int b, in
     float d = acosf(fminf(fmaxf(cphi, -1.0f), 1.0f)) - acosf(fminf(fmaxf(ax.w, -1.0f), 1.0f));
     if (d > 0.0f && cosf(d) < -2.0f * FRONT_TOL) return false;
   }
-  float cx, cy, cz;
-  rot3f(m, cr.x, cr.y, cr.z, cx, cy, cz);
-  float s = V.inv_kmpp32, px = cx * s, py = cy * s;
-  float R = cr.w * s * 1.0001f + 0.01f + 1.0e-6f * (fabsf(px) + fabsf(py));
-  i0 = (int)fminf(fmaxf(floorf(px - R + 0.5f), -1.0e9f), 1.0e9f);
-  i1 = (int)fminf(fmaxf(floorf(px + R + 0.5f), -1.0e9f), 1.0e9f);
-  j0 = (int)fminf(fmaxf(floorf(py - R + 0.5f), -1.0e9f), 1.0e9f);
-  j1 = (int)fminf(fmaxf(floorf(py + R + 0.5f), -1.0e9f), 1.0e9f);
+  sphere_rect(V, cr, i0, i1, j0, j1);
   return true;
 }
 
