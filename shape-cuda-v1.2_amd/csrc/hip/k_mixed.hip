@@ -88,9 +88,12 @@ __device__ inline void raster_facet(ViewRec &V, const GeomSlot &G, const PosDev 
     // not max(rows) * max(columns)
     int nc = (i2 - i1 + 1) * (j2 - j1 + 1);
     bool hit = false;
+    Edge32 E;
+    edge32_init(P, i1, j1, E);
+    float di = 0.0f, dj = 0.0f;
     for (int c = 0, i = i1, j = j1; c < nc; c++) {
       float s, t;
-      float m = bary32(P, i, j, s, t);
+      float m = edge32(E, di, dj, s, t);
       long long pix = (long long)(i + n) * w + (j + n);
       if (m >= AMB_TOL || (src && m >= -AMB_TOL)) {
         hit = true;
@@ -99,7 +102,7 @@ __device__ inline void raster_facet(ViewRec &V, const GeomSlot &G, const PosDev 
       bool marginal = !src && m >= -AMB_TOL && m < AMB_TOL;
       hit = hit || marginal;
       mark_marginal(V, pd, marginal, (int)pix, i, j);
-      if (++j > j2) { j = j1; i++; }
+      if (++j > j2) { j = j1; i++; dj = 0.0f; di += 1.0f; } else { dj += 1.0f; }
     }
     // The model rectangle grows by the box of every facet that covers a
     // pixel: at most a pixel wider than the covered pixels themselves, which
@@ -116,17 +119,20 @@ __device__ inline void raster_facet(ViewRec &V, const GeomSlot &G, const PosDev 
         if (tb[(long long)a * tw + b]) { any = true; break; }
     if (any) {
       int nc = (i2 - i1 + 1) * (j2 - j1 + 1);
+      Edge32 E;
+      edge32_init(P, i1, j1, E);
+      float di = 0.0f, dj = 0.0f;
       for (int c = 0, i = i1, j = j1; c < nc; c++) {
         // candidates: a superset of the facets covering the pixel in FP64
         float s, t;
-        if (bary32(P, i, j, s, t) >= -AMB_TOL) {
+        if (edge32(E, di, dj, s, t) >= -AMB_TOL) {
           int si = pd.stale_idx[V.pos_off + (long long)(i + n) * w + (j + n)];
           if (si >= 0) {
             int q = atomicAdd(&pd.stale[si].ncand, 1);
             if (q < STALE_CAND) pd.stale[si].cand[q] = f;
           }
         }
-        if (++j > j2) { j = j1; i++; }
+        if (++j > j2) { j = j1; i++; dj = 0.0f; di += 1.0f; } else { dj += 1.0f; }
       }
     }
   }

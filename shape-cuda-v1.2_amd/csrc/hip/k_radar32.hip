@@ -105,7 +105,7 @@ __global__ __launch_bounds__(256) void k_shade_radar32(ViewRec *__restrict__ vie
     Proj32 P;
     corners32(V, G, f, V.oa32, P);
     float s, t;
-    bary32(P, i, j, s, t);  // same arithmetic as the raster
+    bary32(P, i, j, s, t);  // the raster's coordinates, to FP32 rounding
     float nx, ny, nz;
     normal32(G, f, V.oa32, V.smooth, s, t, nx, ny, nz);
     // cos(scattering) of an owner facing away is inherited from an earlier

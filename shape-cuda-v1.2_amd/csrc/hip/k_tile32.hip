@@ -281,7 +281,7 @@ __global__ __launch_bounds__(256, 6) void k_tile32(ViewRec *__restrict__ views, 
 
   __shared__ unsigned long long key[BT_PIX];
   __shared__ unsigned char flag[BT_PIX];  // bit 0: marginal; bit 7: LDS replay slot in bits 1-6
-  __shared__ unsigned int ev[EV_CAP];
+  __shared__ unsigned int ev[EV_CAP + 1];  // (pixel << 22 | facet); the last slot takes overflowing writes
   __shared__ unsigned int blist[BIN_CAP];
   __shared__ double wbuf[WIN_T];          // delay-Doppler window, then replay candidates
   static_assert(TS_CAP * TS_CAND * sizeof(int) <= sizeof(wbuf), "candidates must fit the window buffer");
@@ -351,6 +351,7 @@ __global__ __launch_bounds__(256, 6) void k_tile32(ViewRec *__restrict__ views, 
       for (int q0 = threadIdx.x & ~63; q0 < nl; q0 += blockDim.x) {
         const int q = q0 + lane;
         Proj32 P;
+        Edge32 E;
         float nz = 0.0f;
         int f = 0, ti1 = 0, ti2 = -1, tj1 = 0, tj2 = -1, wi1 = 0, wi2 = 0, wj1 = 0, wj2 = 0;
         if (q < nl) {
@@ -365,6 +366,7 @@ __global__ __launch_bounds__(256, 6) void k_tile32(ViewRec *__restrict__ views, 
             // window-clipped box (the POS rectangle grows by it), then tile-clipped
             wi1 = max(imin, -n); wi2 = min(imax, n); wj1 = max(jmin, -n); wj2 = min(jmax, n);
             ti1 = max(wi1, i0); ti2 = min(wi2, i0 + BT - 1); tj1 = max(wj1, j0); tj2 = min(wj2, j0 + BT - 1);
+            edge32_init(P, wi1, wj1, E);  // anchored as in the global raster
           }
         }
         const bool front = nz > 0.0f;
@@ -374,22 +376,34 @@ __global__ __launch_bounds__(256, 6) void k_tile32(ViewRec *__restrict__ views, 
         // event list incomplete: its replayed pixels then take the global
         // candidate collection (k_collect32)
         if (nc > 64) atomicMax(&nev, EV_CAP + 1);  // (other waves add to nev concurrently)
+        // covered cells, shifted in: cell c of the box (row-major) is bit
+        // nc - 1 - c, for the box's last 64 cells
         unsigned long long covm = 0ULL;
         bool hit = false;
-        for (int c = 0, i = ti1, j = tj1; c < nc; c++) {
-          float s, t;
-          float mg = bary32(P, i, j, s, t);
-          const int lp = (i - i0) * BT + (j - j0);
-          const bool cov = mg >= -AMB_TOL;
-          if (front && mg >= AMB_TOL) {
-            atomicMax(&key[lp], (unsigned long long)pack_key(depth32(P, s, t), f));
-            hit = true;
-          } else if (front && cov) {
-            flag[lp] = 1;  // marginal: claimed by no facet, replayed in FP64
-            hit = true;
+        const int wj = tj2 - tj1 + 1;
+        const int lp0 = (ti1 - i0) * BT + (tj1 - j0);
+        {
+          const float dj0 = (float)(tj1 - wj1), dj2 = (float)(tj2 - wj1);
+          float di = (float)(ti1 - wi1), dj = dj0;
+          // a row advances lp by BT, so the box ends at lp0 + rows * BT
+          const int lp_end = nc > 0 ? lp0 + (ti2 - ti1 + 1) * BT : lp0;
+          for (int lp = lp0; lp < lp_end;) {
+            float s, t;
+            const float mg = edge32(E, di, dj, s, t);
+            const bool cov = mg >= -AMB_TOL;
+            if (front && mg >= AMB_TOL) {
+              atomicMax(&key[lp], (unsigned long long)pack_key(depth32(P, s, t), f));
+              hit = true;
+            } else if (front && cov) {
+              flag[lp] = 1;  // marginal: claimed by no facet, replayed in FP64
+              hit = true;
+            }
+            covm = covm + covm + (cov ? 1ULL : 0ULL);
+            const bool wrap = dj >= dj2;
+            dj = wrap ? dj0 : dj + 1.0f;
+            di = wrap ? di + 1.0f : di;
+            lp += wrap ? BT - wj + 1 : 1;
           }
-          if (cov && c < 64) covm |= 1ULL << c;
-          if (++j > tj2) { j = tj1; i++; }
         }
         if (hit) {
           xl0 = min(xl0, wi1); xl1 = max(xl1, wi2);
@@ -407,12 +421,16 @@ __global__ __launch_bounds__(256, 6) void k_tile32(ViewRec *__restrict__ views, 
         int base = 0;
         if (lane == 63) base = atomicAdd(&nev, incl);
         base = __shfl(base, 63, 64) + incl - ne;
-        const int wj = tj2 - tj1 + 1;
+        // row of cell c: (c + 0.5) / wj lies at least 0.5 / wj from an
+        // integer, far beyond the FP32 error, so the float quotient is exact
+        // for the boxes whose events count (nc <= 64)
+        const float inv_wj = __builtin_amdgcn_rcpf((float)wj);
         while (covm) {
-          const int c = __builtin_ctzll(covm);
+          const int c = nc - 1 - __builtin_ctzll(covm);
           covm &= covm - 1ULL;
-          const int lp = (ti1 + c / wj - i0) * BT + (tj1 + c % wj - j0);
-          if (base < EV_CAP) ev[base] = ((unsigned int)lp << EV_FACET_BITS) | (unsigned int)f;
+          const int r = (int)(((float)c + 0.5f) * inv_wj);
+          const int lp = lp0 + c + __mul24(r, BT - wj);
+          ev[min(base, EV_CAP)] = ((unsigned int)lp << EV_FACET_BITS) | (unsigned int)f;  // slot EV_CAP: spill
           base++;
         }
       }
@@ -488,7 +506,7 @@ __global__ __launch_bounds__(256, 6) void k_tile32(ViewRec *__restrict__ views, 
     Proj32 P;
     corners32f(V, G, f, V.oa32, P);
     float s, t;
-    bary32(P, i, j, s, t);  // same arithmetic as the raster
+    bary32(P, i, j, s, t);  // the raster's coordinates, to FP32 rounding
     float nx, ny, nz;
     normal32f(G, f, V.oa32, V.smooth, s, t, nx, ny, nz);
     if (!(nz > 0.0f)) {

@@ -156,6 +156,30 @@ __device__ inline float bary32(const Proj32 &P, int i, int j, float &s, float &t
   t = fmaf(P.x1 - P.x0, yj - P.y0, -((xi - P.x0) * (P.y1 - P.y0))) * P.den;
   return fminf(t, fminf(s - t, 1.0f - s));
 }
+// The rasters' barycentric coordinates over a facet's bounding box. s and t
+// are linear in the pixel indices, so each pixel takes two FMAs per
+// coordinate from the values at an anchor pixel: (di, dj) = pixel - anchor,
+// both small integers held exactly in float.  Every raster of a view anchors
+// a facet at its window-clipped box corner, so the tile and global-z-buffer
+// rasters and the candidate collection make identical decisions.  Rounding
+// stays ~1e-7 barycentric, far inside AMB_TOL.
+struct Edge32 {
+  float s0, t0, dsi, dsj, dti, dtj;
+};
+__device__ inline void edge32_init(const Proj32 &P, int ia, int ja, Edge32 &E) {
+#pragma clang fp contract(off)
+  bary32(P, ia, ja, E.s0, E.t0);
+  E.dsi = (P.y2 - P.y1) * P.den;
+  E.dsj = -((P.x2 - P.x1) * P.den);
+  E.dti = -((P.y1 - P.y0) * P.den);
+  E.dtj = (P.x1 - P.x0) * P.den;
+}
+__device__ inline float edge32(const Edge32 &E, float di, float dj, float &s, float &t) {
+#pragma clang fp contract(off)
+  s = fmaf(di, E.dsi, fmaf(dj, E.dsj, E.s0));
+  t = fmaf(di, E.dti, fmaf(dj, E.dtj, E.t0));
+  return fminf(t, fminf(s - t, 1.0f - s));
+}
 __device__ inline float depth32(const Proj32 &P, float s, float t) {
 #pragma clang fp contract(off)
   return fmaf(t, P.z2 - P.z1, fmaf(s, P.z1 - P.z0, P.z0));
