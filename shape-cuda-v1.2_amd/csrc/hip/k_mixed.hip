@@ -38,9 +38,10 @@ __global__ void k_replay32(ViewRec *__restrict__ views, const GeomSlot *slots, P
 __global__ void k_block_bounds(GeomSlot *slots, int nslots);
 __global__ void k_bin32(const ViewRec *__restrict__ views, const GeomSlot *slots, int *bin_cnt,
                         unsigned int *bin_list, int2 *out_list, int *out_count);
-__global__ void k_tile32(ViewRec *__restrict__ views, const GeomSlot *slots, PosDev pd, int *bin_cnt,
-                         const unsigned int *bin_list, double *fit, const FrameRec *frames, double *overflow,
-                         const double *params, int *counters);
+__global__ void k_tile32(const ViewRec *__restrict__ vrd, ViewRec *views, const GeomSlot *__restrict__ slots,
+                         PosDev pd, int *bin_cnt, const unsigned int *__restrict__ bin_list, double *fit,
+                         const FrameRec *__restrict__ frames, double *overflow, const double *__restrict__ params,
+                         int *counters);
 
 // ---------------------------------------------------------------------------
 // raster (posvis.c:142-295), one facet per lane:
@@ -296,8 +297,8 @@ int enqueue_mixed(Engine &E) {
     SHP_LAUNCH(E, "k_outside32", k_outside32, dim3(512), bt, 0, st, E.d_views, E.d_slots, E.pd, E.d_out_list,
                E.pd.nstale + 4);
     SHP_LAUNCH(E, "k_tile32", k_tile32, dim3(pad_xcd((unsigned)(E.max_bin_ntx * E.max_bin_ntx)), E.nviews), bt, 0, st,
-               E.d_views, E.d_slots, E.pd, E.d_bin_cnt, E.d_bin_list, E.d_fit, E.d_frames, E.d_overflow, E.d_params,
-               E.pd.nstale);
+               E.d_views, E.d_views, E.d_slots, E.pd, E.d_bin_cnt, E.d_bin_list, E.d_fit, E.d_frames, E.d_overflow,
+               E.d_params, E.pd.nstale);
   }
   if (E.any_global) {
     SHP_LAUNCH(E, "k_raster32", k_raster32, gf, bt, 0, st, E.d_views, E.d_slots, E.pd, E.total_pix, 0);

@@ -258,11 +258,18 @@ __device__ inline void tile_pixel(int m, int &a, int &c) {
 
 // Six workgroups per CU (LDS 26 KiB each): the register cap of six waves per
 // SIMD costs a few spills but hides more of the gathers' latency (measured).
-__global__ __launch_bounds__(256, 6) void k_tile32(ViewRec *__restrict__ views, const GeomSlot *slots, PosDev pd,
-                                                int *bin_cnt, const unsigned int *bin_list, double *fit,
-                                                const FrameRec *frames, double *overflow, const double *params,
-                                                int *counters) {
-  ViewRec &V = views[blockIdx.y];
+// vrd and views are the same view records: every read goes through the
+// read-only, non-aliased vrd (the fields it reads stay fixed during the
+// launch), so the compiler keeps them in scalar loads instead of reloading
+// them per pixel after the kernel's global stores; the limits that workgroups
+// update are read and written through views only.
+__global__ __launch_bounds__(256, 6) void k_tile32(const ViewRec *__restrict__ vrd, ViewRec *views,
+                                                const GeomSlot *__restrict__ slots, PosDev pd, int *bin_cnt,
+                                                const unsigned int *__restrict__ bin_list, double *fit,
+                                                const FrameRec *__restrict__ frames, double *overflow,
+                                                const double *__restrict__ params, int *counters) {
+  const ViewRec &V = vrd[blockIdx.y];
+  ViewRec &Vw = views[blockIdx.y];
   if (!V.binned) return;
   const int n = V.n, ntx = V.bin_ntx;
   // each XCD takes a contiguous eighth of the view's tiles (neighbouring
@@ -611,19 +618,19 @@ __global__ __launch_bounds__(256, 6) void k_tile32(ViewRec *__restrict__ views, 
     xl0 = min(xl0, shr[q][2]); xl1 = max(xl1, shr[q][3]);
     yl0 = min(yl0, shr[q][4]); yl1 = max(yl1, shr[q][5]);
   }
-  if (ncl) atomicAdd(&V.nclaimed, ncl);
-  if (xl0 < n && xl0 < V.xlim[0]) atomicMin(&V.xlim[0], xl0);
-  if (xl1 > -n && xl1 > V.xlim[1]) atomicMax(&V.xlim[1], xl1);
-  if (yl0 < n && yl0 < V.ylim[0]) atomicMin(&V.ylim[0], yl0);
-  if (yl1 > -n && yl1 > V.ylim[1]) atomicMax(&V.ylim[1], yl1);
-  if (!dd && anyo && !V.any_overflow) atomicOr(&V.any_overflow, 1);
+  if (ncl) atomicAdd(&Vw.nclaimed, ncl);
+  if (xl0 < n && xl0 < Vw.xlim[0]) atomicMin(&Vw.xlim[0], xl0);
+  if (xl1 > -n && xl1 > Vw.xlim[1]) atomicMax(&Vw.xlim[1], xl1);
+  if (yl0 < n && yl0 < Vw.ylim[0]) atomicMin(&Vw.ylim[0], yl0);
+  if (yl1 > -n && yl1 > Vw.ylim[1]) atomicMax(&Vw.ylim[1], yl1);
+  if (!dd && anyo && !Vw.any_overflow) atomicOr(&Vw.any_overflow, 1);
   if (!any) return;
   if (dd) {
-    if (V.D0 + lim_d0 < V.idellim[0]) atomicMin(&V.idellim[0], V.D0 + lim_d0);
-    if (V.D0 + lim_d1 > V.idellim[1]) atomicMax(&V.idellim[1], V.D0 + lim_d1);
+    if (V.D0 + lim_d0 < Vw.idellim[0]) atomicMin(&Vw.idellim[0], V.D0 + lim_d0);
+    if (V.D0 + lim_d1 > Vw.idellim[1]) atomicMax(&Vw.idellim[1], V.D0 + lim_d1);
   }
-  if (V.P0 + lim_p0 < V.idoplim[0]) atomicMin(&V.idoplim[0], V.P0 + lim_p0);
-  if (V.P0 + lim_p1 > V.idoplim[1]) atomicMax(&V.idoplim[1], V.P0 + lim_p1);
+  if (V.P0 + lim_p0 < Vw.idoplim[0]) atomicMin(&Vw.idoplim[0], V.P0 + lim_p0);
+  if (V.P0 + lim_p1 > Vw.idoplim[1]) atomicMax(&Vw.idoplim[1], V.P0 + lim_p1);
 }
 
 }  // namespace shp
