@@ -341,7 +341,7 @@ __global__ __launch_bounds__(256, 6) void k_tile32(const ViewRec *__restrict__ v
           // sphere), and its facing: nearly edge-on facets are replay
           // candidates only, the FP64 replay tests facing itself
           // (posvis.c:152).  Both loads are coalesced and issued together.
-          const float4 fb = G.fb[f], fn = G.fn32[f];
+          const float4 fb = gload4(G.fb + f), fn = gload4(G.fn32 + f);
           const float px = (m[0] * fb.x + m[1] * fb.y + m[2] * fb.z) * sc;
           const float py = (m[3] * fb.x + m[4] * fb.y + m[5] * fb.z) * sc;
           const float R = fb.w * sc * 1.0001f + 0.01f + 1.0e-6f * (fabsf(px) + fabsf(py));

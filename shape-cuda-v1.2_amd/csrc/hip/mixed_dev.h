@@ -59,6 +59,19 @@ struct Proj32 {
   float den;
 };
 
+// Loads of geometry through the pointers a GeomSlot holds (device
+// allocations).  The compiler cannot see their address space and emits flat
+// loads, which count against the LDS counter too: a wait for an LDS result
+// then also waits for them.  The tile raster's facet loads use these global
+// loads (measured faster); in its shading phase they cost register spills,
+// so the other helpers keep plain loads.  HIP's float4 copies through a
+// generic reference, so the load goes through the native vector type.
+typedef float gvec4 __attribute__((ext_vector_type(4)));
+__device__ inline float4 gload4(const float4 *p) {
+  const gvec4 v = *(const __attribute__((address_space(1))) gvec4 *)p;
+  return make_float4(v.x, v.y, v.z, v.w);
+}
+
 // project a facet's corners a, b, c (posvis.c:159-166) in pixel units
 __device__ inline void corners32p(const ViewRec &V, float4 a, float4 b, float4 c, const float *m, Proj32 &P) {
 #pragma clang fp contract(off)
@@ -89,10 +102,10 @@ __device__ inline float facing32(const float *m, float4 fn) {
 __device__ inline bool project32nf(const ViewRec &V, const GeomSlot &G, int f, const float *m, float nz_min,
                                    Proj32 &P, float &nz) {
 #pragma clang fp contract(off)
-  float4 fn = G.fn32[f];
+  float4 fn = gload4(G.fn32 + f);
   nz = facing32(m, fn);
   if (!(nz > nz_min)) return false;
-  corners32f(V, G, f, m, P);
+  corners32p(V, gload4(G.fvx + 3 * f), gload4(G.fvx + 3 * f + 1), gload4(G.fvx + 3 * f + 2), m, P);
   return true;
 }
 __device__ inline bool project32(const ViewRec &V, const GeomSlot &G, int f, const float *m, float nz_min, Proj32 &P) {
