@@ -106,8 +106,7 @@ __global__ __launch_bounds__(256) void k_shade_radar32(ViewRec *__restrict__ vie
     corners32(V, G, f, V.oa32, P);
     float s, t;
     bary32(P, i, j, s, t);  // the raster's coordinates, to FP32 rounding
-    float nx, ny, nz;
-    normal32(G, f, V.oa32, V.smooth, s, t, nx, ny, nz);
+    const float nz = normal32_los(G, f, V.oa32, V.smooth, s, t);
     // cos(scattering) of an owner facing away is inherited from an earlier
     // claimant: FP64 replay (records reserved once per wave)
     int si = stale_slot(pd, !(nz > 0.0f));

@@ -514,8 +514,7 @@ __global__ __launch_bounds__(256, 6) void k_tile32(const ViewRec *__restrict__ v
     corners32f(V, G, f, V.oa32, P);
     float s, t;
     bary32(P, i, j, s, t);  // the raster's coordinates, to FP32 rounding
-    float nx, ny, nz;
-    normal32f(G, f, V.oa32, V.smooth, s, t, nx, ny, nz);
+    const float nz = normal32f_los(G, f, V.oa32, V.smooth, s, t);
     if (!(nz > 0.0f)) {
       // cos(scattering) of an owner facing away is inherited from an earlier
       // claimant: FP64 replay

@@ -80,7 +80,8 @@ __device__ inline void corners32p(const ViewRec &V, float4 a, float4 b, float4 c
   rot3f(m, b.x, b.y, b.z, P.x1, P.y1, P.z1);
   rot3f(m, c.x, c.y, c.z, P.x2, P.y2, P.z2);
   P.x0 *= s; P.y0 *= s; P.x1 *= s; P.y1 *= s; P.x2 *= s; P.y2 *= s;
-  P.den = 1.0f / fmaf(P.x1 - P.x0, P.y2 - P.y1, -((P.x2 - P.x1) * (P.y1 - P.y0)));
+  // hardware reciprocal (1 ulp): every raster and shader takes the same value
+  P.den = __builtin_amdgcn_rcpf(fmaf(P.x1 - P.x0, P.y2 - P.y1, -((P.x2 - P.x1) * (P.y1 - P.y0))));
 }
 __device__ inline void corners32(const ViewRec &V, const GeomSlot &G, int f, const float *m, Proj32 &P) {
   const int *fv = G.fv + 3 * f;
@@ -247,46 +248,31 @@ __device__ inline void mark_marginal(const ViewRec &V, const PosDev &pd, bool ma
   else if (first) pd.stale_idx[p] = -1;  // list full: the error flag is set
 }
 
-// normal at (s, t) of facet f in observer coordinates (posvis.c:243-258);
-// vn: the facet's 3 vertex normals (smooth) from the facet-major copies
-__device__ inline void normal32f(const GeomSlot &G, int f, const float *m, int smooth, float s, float t, float &nx,
-                                 float &ny, float &nz) {
+// Line-of-sight component of the normal at (s, t) of facet f in observer
+// coordinates (posvis.c:243-258): the radar's cos(scattering).  A smoothed
+// normal interpolates the facet's vertex normals na, nb, nc and is
+// normalized by its body-frame length (a rotation keeps lengths), with the
+// hardware reciprocal square root (1 ulp; the length is near 1).
+__device__ inline float normal_los(const float *m, float4 na, float4 nb, float4 nc, float s, float t) {
 #pragma clang fp contract(off)
-  float bx, by, bz;
-  if (smooth) {
-    float4 na = G.fvn[3 * f], nb = G.fvn[3 * f + 1], nc = G.fvn[3 * f + 2];
-    bx = fmaf(t, nc.x - nb.x, fmaf(s, nb.x - na.x, na.x));
-    by = fmaf(t, nc.y - nb.y, fmaf(s, nb.y - na.y, na.y));
-    bz = fmaf(t, nc.z - nb.z, fmaf(s, nb.z - na.z, na.z));
-  } else {
-    float4 fn = G.fn32[f];
-    bx = fn.x; by = fn.y; bz = fn.z;
-  }
-  rot3f(m, bx, by, bz, nx, ny, nz);
-  if (smooth) {
-    float inv = rsqrtf(fmaf(nz, nz, fmaf(ny, ny, nx * nx)));
-    nx *= inv; ny *= inv; nz *= inv;
-  }
+  const float bx = fmaf(t, nc.x - nb.x, fmaf(s, nb.x - na.x, na.x));
+  const float by = fmaf(t, nc.y - nb.y, fmaf(s, nb.y - na.y, na.y));
+  const float bz = fmaf(t, nc.z - nb.z, fmaf(s, nb.z - na.z, na.z));
+  const float nz = fmaf(m[8], bz, fmaf(m[7], by, m[6] * bx));
+  return nz * __builtin_amdgcn_rsqf(fmaf(bz, bz, fmaf(by, by, bx * bx)));
 }
-__device__ inline void normal32(const GeomSlot &G, int f, const float *m, int smooth, float s, float t, float &nx,
-                                float &ny, float &nz) {
-#pragma clang fp contract(off)
-  float bx, by, bz;
+// from the facet-major vertex-normal copies
+__device__ inline float normal32f_los(const GeomSlot &G, int f, const float *m, int smooth, float s, float t) {
+  if (smooth) return normal_los(m, G.fvn[3 * f], G.fvn[3 * f + 1], G.fvn[3 * f + 2], s, t);
+  return facing32(m, G.fn32[f]);
+}
+// from the vertex normals through the facet's vertex indices
+__device__ inline float normal32_los(const GeomSlot &G, int f, const float *m, int smooth, float s, float t) {
   if (smooth) {
     const int *fv = G.fv + 3 * f;
-    float4 na = G.vn32[fv[0]], nb = G.vn32[fv[1]], nc = G.vn32[fv[2]];
-    bx = fmaf(t, nc.x - nb.x, fmaf(s, nb.x - na.x, na.x));
-    by = fmaf(t, nc.y - nb.y, fmaf(s, nb.y - na.y, na.y));
-    bz = fmaf(t, nc.z - nb.z, fmaf(s, nb.z - na.z, na.z));
-  } else {
-    float4 fn = G.fn32[f];
-    bx = fn.x; by = fn.y; bz = fn.z;
+    return normal_los(m, G.vn32[fv[0]], G.vn32[fv[1]], G.vn32[fv[2]], s, t);
   }
-  rot3f(m, bx, by, bz, nx, ny, nz);
-  if (smooth) {
-    float inv = rsqrtf(fmaf(nz, nz, fmaf(ny, ny, nx * nx)));
-    nx *= inv; ny *= inv; nz *= inv;
-  }
+  return facing32(m, G.fn32[f]);
 }
 
 __device__ inline float sinc2f(float x) {
@@ -377,8 +363,9 @@ struct RadarPix {
 // Doppler weight of bin b of a single-sample sinc^2 footprint
 __device__ inline float dop_weight(float pfr, int pm0, float s2, int b) {
   float a = pfr + (float)(pm0 - b);
-  // hardware reciprocal (1 ulp): the weights are FP32 and normalized by their sum
-  return (fabsf(3.14159265358979f * a) > 1.0e-4f) ? s2 * __builtin_amdgcn_rcpf(a * a) : 1.0f;
+  // hardware reciprocal (1 ulp): the weights are FP32 and normalized by their
+  // sum.  Within 1e-4 / pi of the center sinc^2 is 1 to 3e-9.
+  return (fabsf(a) > 1.0e-4f / 3.14159265358979f) ? s2 * __builtin_amdgcn_rcpf(a * a) : 1.0f;
 }
 
 // Doppler response of bin ip0 + b averaged over nsinc2 x nsinc2 subpixels
