@@ -450,7 +450,6 @@ __global__ __launch_bounds__(256, 6) void k_tile32(const ViewRec *__restrict__ v
   const int width = V.sinc2width;
   const bool ev_ok = nev <= EV_CAP;
   constexpr int TP = BT_PIX / 256;
-  unsigned long long kk[TP];
   int lo_d = 0x7fffffff, hi_d = -0x7fffffff, lo_p = 0x7fffffff, hi_p = -0x7fffffff, ncl = 0;
 #pragma unroll
   for (int m = 0; m < TP; m++) {
@@ -463,6 +462,7 @@ __global__ __launch_bounds__(256, 6) void k_tile32(const ViewRec *__restrict__ v
       if (k) ncl++;
       if (flag[lp] & 1) {
         k = 0ULL;
+        key[lp] = 0ULL;  // not shaded: replayed in FP64
         tile_stale(V, pd, ev_ok, lp, i, j, flag, spix, &nts);
       }
       if (k) {
@@ -476,7 +476,6 @@ __global__ __launch_bounds__(256, 6) void k_tile32(const ViewRec *__restrict__ v
         }
       }
     }
-    kk[m] = k;
   }
   {
     int nhp = -hi_p, nhd = -hi_d;
@@ -504,10 +503,10 @@ __global__ __launch_bounds__(256, 6) void k_tile32(const ViewRec *__restrict__ v
   double *ov = overflow + (long long)blockIdx.y * 2000;
   int anyo = 0;
   for (int m = 0; m < TP; m++) {
-    const unsigned long long k = kk[m];
-    if (!k) continue;
     int a, c;
     tile_pixel(m, a, c);
+    const unsigned long long k = key[a * BT + c];  // (phase A cleared the marginal ones)
+    if (!k) continue;
     const int i = i0 + a, j = j0 + c;
     const int f = key_facet(k);
     Proj32 P;

@@ -425,7 +425,10 @@ __device__ inline void radar_pixel(const ViewRec &V, const double *pl, int i, in
   }
   float sump = 0.0f;
   if (np >= 1 && np <= PW) {
-    for (int b = 0; b < np; b++) sump += dop_w(V, R, b);
+    if (R.sinc1)
+      for (int b = 0; b < np; b++) sump += dop_weight(R.pfr, R.pm0, R.s2, b);
+    else
+      for (int b = 0; b < np; b++) sump += dop_w_sub(V, R.pfr, R.pm0, b);
   } else {
     R.wide = true;
   }
@@ -493,12 +496,22 @@ __device__ inline void radar_scatter_lds(const ViewRec &V, const RadarPix &R, do
   for (int a = 0; a < DW; a++) ad[a] = (a < nd) ? (dd ? R.amp * del_w(V, R, a) : R.amp) : 0.0f;
   int b = threadIdx.x & 15;
   while (b >= np) b -= np;
-  for (int st = 0; st < np; st++) {
-    const float pwb = dop_w(V, R, b);
+  if (R.sinc1) {
+    for (int st = 0; st < np; st++) {
+      const float pwb = dop_weight(R.pfr, R.pm0, R.s2, b);
 #pragma unroll
-    for (int a = 0; a < DW; a++)
-      if (a < nd) atomicAdd(&origin[a * wp + b], (double)(ad[a] * pwb));
-    if (++b == np) b = 0;
+      for (int a = 0; a < DW; a++)
+        if (a < nd) atomicAdd(&origin[a * wp + b], (double)(ad[a] * pwb));
+      if (++b == np) b = 0;
+    }
+  } else {
+    for (int st = 0; st < np; st++) {
+      const float pwb = dop_w_sub(V, R.pfr, R.pm0, b);
+#pragma unroll
+      for (int a = 0; a < DW; a++)
+        if (a < nd) atomicAdd(&origin[a * wp + b], (double)(ad[a] * pwb));
+      if (++b == np) b = 0;
+    }
   }
 }
 
