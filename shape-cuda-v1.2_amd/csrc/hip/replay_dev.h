@@ -43,6 +43,15 @@ __device__ inline int replay_owner64(const ViewRec &V, const GeomSlot &G, const 
   return fac;
 }
 
+// Lower the ordered FP64 limit at a to v, reading it first: the limits only
+// move outward, so a value read stale can only cost an atomic, never skip
+// one that is needed, and most replayed pixels leave the limits as they are
+// (same-address atomics serialize at the memory side).
+__device__ inline void lower_limit_d(unsigned long long *a, double v) {
+  const unsigned long long o = (unsigned long long)ord64(v);
+  if (o < *(volatile unsigned long long *)a) atomicMin(a, o);
+}
+
 // Scatter a replayed lit pixel straight into the HBM fit image (or the
 // overflow spectrum) and grow the view's bin limits by it: a replayed pixel
 // is lit like any other.
@@ -55,13 +64,13 @@ __device__ inline void replay_emit(ViewRec &V, const double *pl, int i, int j, d
   if (dd) {
     if (V.D0 + R.id0 < V.idellim[0]) atomicMin(&V.idellim[0], V.D0 + R.id0);
     if (V.D0 + R.id1 > V.idellim[1]) atomicMax(&V.idellim[1], V.D0 + R.id1);
-    atomic_min_d(&V.dellim[0], (double)V.D0 + (double)R.delrel);
-    atomic_min_d(&V.dellim[1], -((double)V.D0 + (double)R.delrel));
+    lower_limit_d(&V.dellim[0], (double)V.D0 + (double)R.delrel);
+    lower_limit_d(&V.dellim[1], -((double)V.D0 + (double)R.delrel));
   }
   if (V.P0 + R.ip0 < V.idoplim[0]) atomicMin(&V.idoplim[0], V.P0 + R.ip0);
   if (V.P0 + R.ip1 > V.idoplim[1]) atomicMax(&V.idoplim[1], V.P0 + R.ip1);
-  atomic_min_d(&V.doplim[0], (double)V.P0 + (double)R.doprel);
-  atomic_min_d(&V.doplim[1], -((double)V.P0 + (double)R.doprel));
+  lower_limit_d(&V.doplim[0], (double)V.P0 + (double)R.doprel);
+  lower_limit_d(&V.doplim[1], -((double)V.P0 + (double)R.doprel));
   if (!dd && !R.inb) atomicOr(&V.any_overflow, 1);
 }
 
