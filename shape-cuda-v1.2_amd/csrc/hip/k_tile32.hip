@@ -451,6 +451,7 @@ __global__ __launch_bounds__(256, 6) void k_tile32(const ViewRec *__restrict__ v
   const bool ev_ok = nev <= EV_CAP;
   constexpr int TP = BT_PIX / 256;
   int lo_d = 0x7fffffff, hi_d = -0x7fffffff, lo_p = 0x7fffffff, hi_p = -0x7fffffff, ncl = 0;
+  unsigned long long kk[TP];
 #pragma unroll
   for (int m = 0; m < TP; m++) {
     int a, c;
@@ -461,8 +462,7 @@ __global__ __launch_bounds__(256, 6) void k_tile32(const ViewRec *__restrict__ v
       k = key[lp];
       if (k) ncl++;
       if (flag[lp] & 1) {
-        k = 0ULL;
-        key[lp] = 0ULL;  // not shaded: replayed in FP64
+        k = 0ULL;  // not shaded: replayed in FP64
         tile_stale(V, pd, ev_ok, lp, i, j, flag, spix, &nts);
       }
       if (k) {
@@ -476,12 +476,32 @@ __global__ __launch_bounds__(256, 6) void k_tile32(const ViewRec *__restrict__ v
         }
       }
     }
+    kk[m] = k;
   }
   {
     int nhp = -hi_p, nhd = -hi_d;
-    blk_min4(lo_p, nhp, lo_d, nhd, shw);
+    blk_min4(lo_p, nhp, lo_d, nhd, shw);  // (its barriers also end every read of the keys)
     hi_p = -nhp; hi_d = -nhd;
     if (!dd) lo_d = hi_d = 0;
+  }
+  // Each wave lists its pixels to shade contiguously in its own quarter of
+  // the key array, so that the shading loop runs no lane idle for an
+  // unclaimed pixel.  A facet below 2^22 leaves the top ten bits of ~facet
+  // set: they carry the pixel's tile index meanwhile.
+  static_assert(BT_PIX == 4 * 256 && EV_FACET_BITS == 22 && BT_PIX <= 1024, "a wave's list and packed pixel index");
+  constexpr unsigned long long PIX_BITS = 0x3FFULL << EV_FACET_BITS;
+  unsigned long long *shade = key + wv * 256;
+  int nsh = 0;
+#pragma unroll
+  for (int m = 0; m < TP; m++) {
+    const unsigned long long mask = __ballot(kk[m] != 0ULL);
+    if (kk[m]) {
+      int a, c;
+      tile_pixel(m, a, c);
+      shade[nsh + __popcll(mask & ((1ULL << lane) - 1ULL))] =
+          (kk[m] & ~PIX_BITS) | ((unsigned long long)(a * BT + c) << EV_FACET_BITS);
+    }
+    nsh += __popcll(mask);
   }
   const bool any = lo_p != 0x7fffffff;
   const int lim_d0 = lo_d, lim_d1 = hi_d, lim_p0 = lo_p, lim_p1 = hi_p;
@@ -502,11 +522,10 @@ __global__ __launch_bounds__(256, 6) void k_tile32(const ViewRec *__restrict__ v
   double *img = fit + frames[V.frame_id].off;
   double *ov = overflow + (long long)blockIdx.y * 2000;
   int anyo = 0;
-  for (int m = 0; m < TP; m++) {
-    int a, c;
-    tile_pixel(m, a, c);
-    const unsigned long long k = key[a * BT + c];  // (phase A cleared the marginal ones)
-    if (!k) continue;
+  for (int q = lane; q < nsh; q += 64) {
+    const unsigned long long e = shade[q];
+    const int lp = (int)((e & PIX_BITS) >> EV_FACET_BITS), a = lp / BT, c = lp % BT;
+    const unsigned long long k = e | PIX_BITS;
     const int i = i0 + a, j = j0 + c;
     const int f = key_facet(k);
     Proj32 P;
