@@ -85,30 +85,32 @@ struct DevBuf {
 extern "C" double chi2(struct par_t *par, struct dat_t *dat, int list_breakdown) {
   (void)par;
   (void)list_breakdown;
+  // frame records first (offsets into flat arrays), then the fit,
+  // observation and weight arrays copied from the frames' row arrays in
+  // parallel
   std::vector<FrameRec> fr;
-  std::vector<double> fit, obs, oov;
+  std::vector<const void *> src;  // the frame (deldopfrm_t, dopfrm_t) or lightcurve (lghtcrv_t)
+  long long ncell = 0;
+  auto add = [&](FrameRec &F, int s, long long N, const void *h) {
+    F.set = s;
+    F.nviews = 1;
+    F.off = ncell;
+    F.obs_off = ncell;
+    ncell += N;
+    fr.push_back(F);
+    src.push_back(h);
+  };
   for (int s = 0; s < dat->nsets; s++) {
     struct set_t *set = &dat->set[s];
-    auto add = [&](FrameRec &F, long long N) {
-      F.set = s;
-      F.nviews = 1;
-      F.off = (long long)fit.size() - N;
-      F.obs_off = F.off;
-      fr.push_back(F);
-    };
     if (set->type == DELAY) {
       for (int f = 0; f < set->desc.deldop.nframes; f++) {
         struct deldopfrm_t *h = &set->desc.deldop.frame[f];
         FrameRec F;
         memset(&F, 0, sizeof(F));
         F.type = VT_DELAY; F.frm = f; F.ndel = h->ndel; F.ndop = h->ndop;
-        for (int i = 1; i <= h->ndel; i++)
-          for (int j = 1; j <= h->ndop; j++) {
-            fit.push_back(h->fit[i][j]); obs.push_back(h->obs[i][j]); oov.push_back(h->oneovervar[i][j]);
-          }
         F.cal_floats = h->cal.state == 'f'; F.cal_fixed = h->cal.val; F.weight = h->weight;
         F.overflow_o2 = h->overflow_o2; F.overflow_m2 = h->overflow_m2;
-        add(F, (long long)h->ndel * h->ndop);
+        add(F, s, (long long)h->ndel * h->ndop, h);
       }
     } else if (set->type == DOPPLER) {
       for (int f = 0; f < set->desc.doppler.nframes; f++) {
@@ -116,24 +118,38 @@ extern "C" double chi2(struct par_t *par, struct dat_t *dat, int list_breakdown)
         FrameRec F;
         memset(&F, 0, sizeof(F));
         F.type = VT_DOPPLER; F.frm = f; F.ndel = 1; F.ndop = h->ndop;
-        for (int j = 1; j <= h->ndop; j++) { fit.push_back(h->fit[j]); obs.push_back(h->obs[j]); oov.push_back(h->oneovervar[j]); }
         F.cal_floats = h->cal.state == 'f'; F.cal_fixed = h->cal.val; F.weight = h->weight;
         F.overflow_o2 = h->overflow_o2; F.overflow_m2 = h->overflow_m2;
-        add(F, h->ndop);
+        add(F, s, h->ndop, h);
       }
     } else if (set->type == LGHTCRV) {
       struct lghtcrv_t *L = &set->desc.lghtcrv;
       FrameRec F;
       memset(&F, 0, sizeof(F));
       F.type = VT_LGHTCRV; F.lc_n = L->n;
-      for (int i = 1; i <= L->n; i++) { fit.push_back(L->fit[i]); obs.push_back(L->obs[i]); oov.push_back(L->oneovervar[i]); }
       F.cal_floats = L->cal.state == 'f'; F.cal_fixed = L->cal.val; F.weight = L->weight;
-      add(F, L->n);
+      add(F, s, L->n, L);
     } else {
       set_err(-15, "chi2 (plane-of-sky sets are outside this library)");
       return dat->chi2;
     }
   }
+  std::vector<double> fit(ncell), obs(ncell), oov(ncell);
+  host_parallel_for((int)fr.size(), [&](int k) {
+    const FrameRec &F = fr[k];
+    double *a = fit.data() + F.off, *o = obs.data() + F.off, *w = oov.data() + F.off;
+    if (F.type == VT_DELAY) {
+      const struct deldopfrm_t *h = (const struct deldopfrm_t *)src[k];
+      for (int i = 1; i <= h->ndel; i++)
+        for (int j = 1; j <= h->ndop; j++) { *a++ = h->fit[i][j]; *o++ = h->obs[i][j]; *w++ = h->oneovervar[i][j]; }
+    } else if (F.type == VT_DOPPLER) {
+      const struct dopfrm_t *h = (const struct dopfrm_t *)src[k];
+      for (int j = 1; j <= h->ndop; j++) { *a++ = h->fit[j]; *o++ = h->obs[j]; *w++ = h->oneovervar[j]; }
+    } else {
+      const struct lghtcrv_t *L = (const struct lghtcrv_t *)src[k];
+      for (int i = 1; i <= L->n; i++) { *a++ = L->fit[i]; *o++ = L->obs[i]; *w++ = L->oneovervar[i]; }
+    }
+  });
   int nf = (int)fr.size(), nsets = dat->nsets;
   DevBuf dfr, dfit, dobs, doov, dset, dtot;
   int rc = dfr.get(sizeof(FrameRec) * nf) | dfit.get(sizeof(double) * fit.size()) | dobs.get(sizeof(double) * obs.size()) |

@@ -62,7 +62,15 @@ struct ModelPack {
   std::vector<FrameRec> frames;
   std::vector<int> frame_first_view;  // template view index of each frame's first view
   long long obs_off = 0;              // into the shared obs arena
-  std::vector<double> obs, oov;
+  // observations and weights to copy (after the pack, in parallel): the
+  // source frame and its offset in the model's observations
+  struct ObsJob {
+    int kind;         // VT_DELAY, VT_DOPPLER, VT_LGHTCRV
+    const void *src;  // deldopfrm_t, dopfrm_t or lghtcrv_t
+    long long off, n;
+  };
+  std::vector<ObsJob> obs_jobs;
+  long long nobs = 0;
   std::vector<double> lc_x, lc_t;
   long long lc_x_off = 0, lc_t_off = 0;
   double dof = 0, dof_dd = 0, dof_dop = 0, dof_lc = 0;
@@ -290,13 +298,11 @@ int Engine::pack_model(int mi, struct par_t *par, struct mod_t *mod, struct dat_
           struct deldopfrm_t *fr = &D->frame[f];
           F.ndel = fr->ndel; F.ndop = fr->ndop;
           N = (long long)fr->ndel * fr->ndop;
-          F.obs_off = (long long)P.obs.size();
-          if (!opt.views_only)
-            for (int i = 1; i <= fr->ndel; i++)
-              for (int j = 1; j <= fr->ndop; j++) {
-                P.obs.push_back(fr->obs[i][j]);
-                P.oov.push_back(fr->oneovervar[i][j]);
-              }
+          F.obs_off = P.nobs;
+          if (!opt.views_only) {
+            P.obs_jobs.push_back({VT_DELAY, fr, P.nobs, N});
+            P.nobs += N;
+          }
           F.cal_floats = fr->cal.state == 'f'; F.cal_fixed = fr->cal.val; F.weight = fr->weight; F.dof = fr->dof;
           F.sdev_sq = fr->sdev * fr->sdev; F.lookfact = fr->nlooks > 0.0 ? 1.0 / fr->nlooks : 0.0;
           F.overflow_o2 = fr->overflow_o2; F.overflow_m2 = fr->overflow_m2;
@@ -304,12 +310,11 @@ int Engine::pack_model(int mi, struct par_t *par, struct mod_t *mod, struct dat_
           struct dopfrm_t *fr = &Q->frame[f];
           F.ndel = 1; F.ndop = fr->ndop;
           N = fr->ndop;
-          F.obs_off = (long long)P.obs.size();
-          if (!opt.views_only)
-            for (int j = 1; j <= fr->ndop; j++) {
-              P.obs.push_back(fr->obs[j]);
-              P.oov.push_back(fr->oneovervar[j]);
-            }
+          F.obs_off = P.nobs;
+          if (!opt.views_only) {
+            P.obs_jobs.push_back({VT_DOPPLER, fr, P.nobs, N});
+            P.nobs += N;
+          }
           F.cal_floats = fr->cal.state == 'f'; F.cal_fixed = fr->cal.val; F.weight = fr->weight; F.dof = fr->dof;
           F.sdev_sq = fr->sdev * fr->sdev; F.lookfact = fr->nlooks > 0.0 ? 1.0 / fr->nlooks : 0.0;
           F.overflow_o2 = fr->overflow_o2; F.overflow_m2 = fr->overflow_m2;
@@ -348,12 +353,11 @@ int Engine::pack_model(int mi, struct par_t *par, struct mod_t *mod, struct dat_
         V.intensityfactor = pow(rd->pos.km_per_pixel / AU, 2.0);
         P.views.push_back(V);
       }
-      F.obs_off = (long long)P.obs.size();
-      if (!opt.views_only)
-        for (int i = 1; i <= L->n; i++) {
-          P.obs.push_back(L->obs[i]);
-          P.oov.push_back(L->oneovervar[i]);
-        }
+      F.obs_off = P.nobs;
+      if (!opt.views_only) {
+        P.obs_jobs.push_back({VT_LGHTCRV, L, P.nobs, (long long)L->n});
+        P.nobs += L->n;
+      }
       F.cal_floats = L->cal.state == 'f'; F.cal_fixed = L->cal.val; F.weight = L->weight; F.dof = L->dof;
       F.lc_x_off = (long long)P.lc_x.size();
       P.lc_x.push_back(0.0);
@@ -422,7 +426,7 @@ int Engine::build(struct par_t *par, struct mod_t **mods, struct dat_t **dats, i
   std::vector<long long> model_obs_base(nmodels, 0), model_x_base(nmodels, 0), model_t_base(nmodels, 0);
   long long ob = 0, xb = 0, tb = 0;
   for (int m = 0; m < nmodels; m++) {
-    model_obs_base[m] = ob; ob += (long long)g_packs[m].obs.size();
+    model_obs_base[m] = ob; ob += g_packs[m].nobs;
     model_x_base[m] = xb; xb += (long long)g_packs[m].lc_x.size();
     model_t_base[m] = tb; tb += (long long)g_packs[m].lc_t.size();
   }
@@ -579,12 +583,31 @@ int Engine::build(struct par_t *par, struct mod_t **mods, struct dat_t **dats, i
   SHP_CHECK(hipMemset(d_slot_flags, 0, sizeof(int) * nslots));
   SHP_CHECK(hipMemset(d_slot_logf, 0, sizeof(double) * nslots));
   std::vector<double> obs_all, oov_all, x_all, t_all;
+  std::vector<std::pair<const ModelPack::ObsJob *, long long>> jobs;  // (job, offset in obs_all)
+  long long nobs_all = 0;
   for (auto &P : g_packs) {
-    obs_all.insert(obs_all.end(), P.obs.begin(), P.obs.end());
-    oov_all.insert(oov_all.end(), P.oov.begin(), P.oov.end());
+    for (auto &J : P.obs_jobs) jobs.push_back({&J, nobs_all + J.off});
+    nobs_all += P.nobs;
     x_all.insert(x_all.end(), P.lc_x.begin(), P.lc_x.end());
     t_all.insert(t_all.end(), P.lc_t.begin(), P.lc_t.end());
   }
+  obs_all.resize(nobs_all);
+  oov_all.resize(nobs_all);
+  host_parallel_for((int)jobs.size(), [&](int k) {
+    const ModelPack::ObsJob &J = *jobs[k].first;
+    double *o = obs_all.data() + jobs[k].second, *w = oov_all.data() + jobs[k].second;
+    if (J.kind == VT_DELAY) {
+      const struct deldopfrm_t *fr = (const struct deldopfrm_t *)J.src;
+      for (int i = 1; i <= fr->ndel; i++)
+        for (int j = 1; j <= fr->ndop; j++) { *o++ = fr->obs[i][j]; *w++ = fr->oneovervar[i][j]; }
+    } else if (J.kind == VT_DOPPLER) {
+      const struct dopfrm_t *fr = (const struct dopfrm_t *)J.src;
+      for (int j = 1; j <= fr->ndop; j++) { *o++ = fr->obs[j]; *w++ = fr->oneovervar[j]; }
+    } else {
+      const struct lghtcrv_t *L = (const struct lghtcrv_t *)J.src;
+      for (int i = 1; i <= L->n; i++) { *o++ = L->obs[i]; *w++ = L->oneovervar[i]; }
+    }
+  });
   obs_len = (long long)obs_all.size();
   if ((rc = upload(this, &d_obs, obs_all))) return rc;
   if ((rc = upload(this, &d_oov, oov_all))) return rc;
@@ -866,11 +889,12 @@ int Engine::writeback(int e, struct par_t *par, struct mod_t *mod, struct dat_t 
   std::vector<double> fit((size_t)std::max(1LL, fit_len));
   SHP_CHECK(hipMemcpy(fit.data(), d_fit, sizeof(double) * fit_len, hipMemcpyDeviceToHost));
   const EvalRec &E = ev[e];
-  for (int q = 0; q < E.nframes; q++) {
+  // the radar fit images into the frames' row arrays, frames in parallel
+  host_parallel_for(E.nframes, [&](int q) {
     const FrameRec &F = fr[E.first_frame + q];
     struct set_t *set = &dat->set[F.set];
     const double *img = fit.data() + F.off;
-    double inv = 1.0 / F.nviews;
+    const double inv = 1.0 / F.nviews;
     if (F.type == VT_DELAY) {
       struct deldopfrm_t *h = &set->desc.deldop.frame[F.frm];
       for (int i = 1; i <= F.ndel; i++)
@@ -878,11 +902,21 @@ int Engine::writeback(int e, struct par_t *par, struct mod_t *mod, struct dat_t 
           double v = img[(size_t)(i - 1) * F.ndop + (j - 1)];
           h->fit[i][j] = F.nviews > 1 ? v * inv : v;
         }
+    } else if (F.type == VT_DOPPLER) {
+      struct dopfrm_t *h = &set->desc.doppler.frame[F.frm];
+      for (int j = 1; j <= F.ndop; j++) h->fit[j] = F.nviews > 1 ? img[j - 1] * inv : img[j - 1];
+    }
+  });
+  for (int q = 0; q < E.nframes; q++) {
+    const FrameRec &F = fr[E.first_frame + q];
+    struct set_t *set = &dat->set[F.set];
+    const double *img = fit.data() + F.off;
+    if (F.type == VT_DELAY) {
+      struct deldopfrm_t *h = &set->desc.deldop.frame[F.frm];
       h->overflow_o2 = h->overflow_m2 = h->overflow_xsec = h->overflow_delmean = h->overflow_dopmean = 0.0;
       h->badradar_logfactor = 0.0;
     } else if (F.type == VT_DOPPLER) {
       struct dopfrm_t *h = &set->desc.doppler.frame[F.frm];
-      for (int j = 1; j <= F.ndop; j++) h->fit[j] = F.nviews > 1 ? img[j - 1] * inv : img[j - 1];
       h->overflow_o2 = F.overflow_o2; h->overflow_m2 = F.overflow_m2;
       h->overflow_xsec = F.overflow_xsec; h->overflow_dopmean = F.overflow_dopmean;
     } else if (F.type == VT_LGHTCRV) {

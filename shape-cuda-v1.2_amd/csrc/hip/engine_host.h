@@ -3,13 +3,35 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <map>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "engine.h"
 #include "kernels.h"
 #include "shape_abi.h"
+
+// Run f(k) for k in [0, n) on up to 16 host threads, interleaved so that
+// frames of different sizes balance.  The drop-ins copy the reference's
+// per-row host arrays to and from flat buffers: memory-bound host work that
+// one thread would take most of a call on.  f must write disjoint data.
+template <class F> inline void host_parallel_for(int n, F &&f) {
+  const int hw = (int)std::thread::hardware_concurrency();
+  const int nt = std::max(1, std::min(std::min(n, 16), hw > 0 ? hw : 1));
+  if (nt <= 1) {
+    for (int k = 0; k < n; k++) f(k);
+    return;
+  }
+  std::vector<std::thread> th;
+  th.reserve(nt);
+  for (int t = 0; t < nt; t++)
+    th.emplace_back([&f, t, nt, n] {
+      for (int k = t; k < n; k += nt) f(k);
+    });
+  for (auto &x : th) x.join();
+}
 
 namespace shp {
 
