@@ -134,10 +134,18 @@ extern "C" double chi2(struct par_t *par, struct dat_t *dat, int list_breakdown)
       return dat->chi2;
     }
   }
-  std::vector<double> fit(ncell), obs(ncell), oov(ncell);
+  // fit, obs and 1/var back to back in the thread's pinned stage (plain
+  // host memory if it cannot be pinned)
+  std::vector<double> plain;
+  double *fit = host_stage(3 * (size_t)ncell);
+  if (!fit) {
+    plain.resize(3 * (size_t)ncell);
+    fit = plain.data();
+  }
+  double *obs = fit + ncell, *oov = obs + ncell;
   host_parallel_for((int)fr.size(), [&](int k) {
     const FrameRec &F = fr[k];
-    double *a = fit.data() + F.off, *o = obs.data() + F.off, *w = oov.data() + F.off;
+    double *a = fit + F.off, *o = obs + F.off, *w = oov + F.off;
     if (F.type == VT_DELAY) {
       const struct deldopfrm_t *h = (const struct deldopfrm_t *)src[k];
       for (int i = 1; i <= h->ndel; i++)
@@ -152,13 +160,14 @@ extern "C" double chi2(struct par_t *par, struct dat_t *dat, int list_breakdown)
   });
   int nf = (int)fr.size(), nsets = dat->nsets;
   DevBuf dfr, dfit, dobs, doov, dset, dtot;
-  int rc = dfr.get(sizeof(FrameRec) * nf) | dfit.get(sizeof(double) * fit.size()) | dobs.get(sizeof(double) * obs.size()) |
-           doov.get(sizeof(double) * oov.size()) | dset.get(sizeof(double) * nsets) | dtot.get(sizeof(double));
+  const size_t nb = sizeof(double) * (size_t)ncell;
+  int rc = dfr.get(sizeof(FrameRec) * nf) | dfit.get(nb) | dobs.get(nb) | doov.get(nb) | dset.get(sizeof(double) * nsets) |
+           dtot.get(sizeof(double));
   if (rc) { set_err(rc, "chi2 allocation"); return dat->chi2; }
   bool ok = hipMemcpy(dfr.p, fr.data(), sizeof(FrameRec) * nf, hipMemcpyHostToDevice) == hipSuccess &&
-            hipMemcpy(dfit.p, fit.data(), sizeof(double) * fit.size(), hipMemcpyHostToDevice) == hipSuccess &&
-            hipMemcpy(dobs.p, obs.data(), sizeof(double) * obs.size(), hipMemcpyHostToDevice) == hipSuccess &&
-            hipMemcpy(doov.p, oov.data(), sizeof(double) * oov.size(), hipMemcpyHostToDevice) == hipSuccess;
+            (ncell == 0 || (hipMemcpy(dfit.p, fit, nb, hipMemcpyHostToDevice) == hipSuccess &&
+                            hipMemcpy(dobs.p, obs, nb, hipMemcpyHostToDevice) == hipSuccess &&
+                            hipMemcpy(doov.p, oov, nb, hipMemcpyHostToDevice) == hipSuccess));
   if (!ok) { set_err(-3, "chi2 upload"); return dat->chi2; }
   FrameRec *df = (FrameRec *)dfr.p;
   hipLaunchKernelGGL(k_frame_o2, dim3(std::max(1, nf)), dim3(256), 0, 0, df, nf, (double *)dobs.p, (double *)doov.p);

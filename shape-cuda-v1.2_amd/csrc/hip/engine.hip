@@ -78,6 +78,31 @@ struct ModelPack {
 
 static std::vector<ModelPack> g_packs;  // scratch used during build
 
+namespace {
+struct HostStage {
+  double *p = nullptr;
+  size_t n = 0;
+  ~HostStage() {
+    if (p) (void)hipHostFree(p);
+  }
+};
+thread_local HostStage t_stage;
+}  // namespace
+
+double *host_stage(size_t n) {
+  if (n <= t_stage.n) return t_stage.p;
+  if (t_stage.p) (void)hipHostFree(t_stage.p);
+  t_stage.p = nullptr;
+  t_stage.n = 0;
+  size_t want = std::max(n, (size_t)1 << 20);
+  if (hipHostMalloc((void **)&t_stage.p, sizeof(double) * want, hipHostMallocDefault) != hipSuccess) {
+    t_stage.p = nullptr;
+    return nullptr;
+  }
+  t_stage.n = want;
+  return t_stage.p;
+}
+
 static void set_param_target(std::map<const double *, ParamTarget> &m, const double *addr, int model, int off) {
   m[addr] = ParamTarget{0, model, off};
 }
@@ -582,7 +607,7 @@ int Engine::build(struct par_t *par, struct mod_t **mods, struct dat_t **dats, i
   if ((rc = alloc((void **)&d_slot_logf, sizeof(double) * nslots))) return rc;
   SHP_CHECK(hipMemset(d_slot_flags, 0, sizeof(int) * nslots));
   SHP_CHECK(hipMemset(d_slot_logf, 0, sizeof(double) * nslots));
-  std::vector<double> obs_all, oov_all, x_all, t_all;
+  std::vector<double> x_all, t_all;
   std::vector<std::pair<const ModelPack::ObsJob *, long long>> jobs;  // (job, offset in obs_all)
   long long nobs_all = 0;
   for (auto &P : g_packs) {
@@ -591,11 +616,18 @@ int Engine::build(struct par_t *par, struct mod_t **mods, struct dat_t **dats, i
     x_all.insert(x_all.end(), P.lc_x.begin(), P.lc_x.end());
     t_all.insert(t_all.end(), P.lc_t.begin(), P.lc_t.end());
   }
-  obs_all.resize(nobs_all);
-  oov_all.resize(nobs_all);
+  // observations and weights back to back in the thread's pinned stage
+  // (plain host memory if it cannot be pinned)
+  std::vector<double> plain;
+  double *obs_all = host_stage(2 * (size_t)nobs_all);
+  if (!obs_all) {
+    plain.resize(2 * (size_t)nobs_all);
+    obs_all = plain.data();
+  }
+  double *oov_all = obs_all + nobs_all;
   host_parallel_for((int)jobs.size(), [&](int k) {
     const ModelPack::ObsJob &J = *jobs[k].first;
-    double *o = obs_all.data() + jobs[k].second, *w = oov_all.data() + jobs[k].second;
+    double *o = obs_all + jobs[k].second, *w = oov_all + jobs[k].second;
     if (J.kind == VT_DELAY) {
       const struct deldopfrm_t *fr = (const struct deldopfrm_t *)J.src;
       for (int i = 1; i <= fr->ndel; i++)
@@ -608,9 +640,13 @@ int Engine::build(struct par_t *par, struct mod_t **mods, struct dat_t **dats, i
       for (int i = 1; i <= L->n; i++) { *o++ = L->obs[i]; *w++ = L->oneovervar[i]; }
     }
   });
-  obs_len = (long long)obs_all.size();
-  if ((rc = upload(this, &d_obs, obs_all))) return rc;
-  if ((rc = upload(this, &d_oov, oov_all))) return rc;
+  obs_len = nobs_all;
+  if ((rc = alloc((void **)&d_obs, sizeof(double) * std::max(1LL, obs_len)))) return rc;
+  if ((rc = alloc((void **)&d_oov, sizeof(double) * std::max(1LL, obs_len)))) return rc;
+  if (obs_len) {
+    SHP_CHECK(hipMemcpy(d_obs, obs_all, sizeof(double) * obs_len, hipMemcpyHostToDevice));
+    SHP_CHECK(hipMemcpy(d_oov, oov_all, sizeof(double) * obs_len, hipMemcpyHostToDevice));
+  }
   if ((rc = upload(this, &d_lc_x, x_all))) return rc;
   if ((rc = upload(this, &d_lc_t, t_all))) return rc;
   if ((rc = alloc((void **)&d_lc_y, sizeof(double) * std::max(1LL, y_cursor)))) return rc;
@@ -886,14 +922,22 @@ int Engine::writeback(int e, struct par_t *par, struct mod_t *mod, struct dat_t 
   if ((rc = download_frames(fr)) || (rc = download_evals(ev)) || (rc = download_view_out(vo)) ||
       (rc = download_views(vw)))
     return rc;
-  std::vector<double> fit((size_t)std::max(1LL, fit_len));
-  SHP_CHECK(hipMemcpy(fit.data(), d_fit, sizeof(double) * fit_len, hipMemcpyDeviceToHost));
+  // the fit images through the thread's pinned stage (plain host memory if
+  // it cannot be pinned)
+  std::vector<double> plain;
+  double *fitp = host_stage((size_t)std::max(1LL, fit_len));
+  if (!fitp) {
+    plain.resize((size_t)std::max(1LL, fit_len));
+    fitp = plain.data();
+  }
+  SHP_CHECK(hipMemcpy(fitp, d_fit, sizeof(double) * fit_len, hipMemcpyDeviceToHost));
+  const double *fit = fitp;
   const EvalRec &E = ev[e];
   // the radar fit images into the frames' row arrays, frames in parallel
   host_parallel_for(E.nframes, [&](int q) {
     const FrameRec &F = fr[E.first_frame + q];
     struct set_t *set = &dat->set[F.set];
-    const double *img = fit.data() + F.off;
+    const double *img = fit + F.off;
     const double inv = 1.0 / F.nviews;
     if (F.type == VT_DELAY) {
       struct deldopfrm_t *h = &set->desc.deldop.frame[F.frm];
@@ -910,7 +954,7 @@ int Engine::writeback(int e, struct par_t *par, struct mod_t *mod, struct dat_t 
   for (int q = 0; q < E.nframes; q++) {
     const FrameRec &F = fr[E.first_frame + q];
     struct set_t *set = &dat->set[F.set];
-    const double *img = fit.data() + F.off;
+    const double *img = fit + F.off;
     if (F.type == VT_DELAY) {
       struct deldopfrm_t *h = &set->desc.deldop.frame[F.frm];
       h->overflow_o2 = h->overflow_m2 = h->overflow_xsec = h->overflow_delmean = h->overflow_dopmean = 0.0;

@@ -35,6 +35,12 @@ template <class F> inline void host_parallel_for(int n, F &&f) {
 
 namespace shp {
 
+// A pinned host staging buffer per host thread, grown on demand and kept
+// for the thread's life: the drop-ins stage their flat arrays in it, so the
+// copies to and from the device run at full link rate without pinning pages
+// on every call.  Returns nullptr if the allocation fails.
+double *host_stage(size_t n);
+
 // Where a fittable scalar lives inside an evaluation's parameter block, or
 // which vertex deviation it is.
 struct ParamTarget {
