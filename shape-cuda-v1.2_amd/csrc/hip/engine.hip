@@ -71,6 +71,7 @@ struct ModelPack {
   };
   std::vector<ObsJob> obs_jobs;
   long long nobs = 0;
+  const struct vertex_t *verts = nullptr;  // the model's vertices: deviation targets by address
   std::vector<double> lc_x, lc_t;
   long long lc_x_off = 0, lc_t_off = 0;
   double dof = 0, dof_dd = 0, dof_dop = 0, dof_lc = 0;
@@ -228,9 +229,11 @@ int Engine::pack_model(int mi, struct par_t *par, struct mod_t *mod, struct dat_
     pc[3 + j] = comp->rot[j].val; set_param_target(tmap, &comp->rot[j].val, mi, off_comp + 3 + j);
     pc[6 + j] = comp->off[j].val; set_param_target(tmap, &comp->off[j].val, mi, off_comp + 6 + j);
   }
-  // ---- topology and inputs ----
+  // ---- topology and inputs (none for the views-only drop-ins) ----
   P.nv = rv->nv;
   P.nf = rv->nf;
+  P.verts = rv->v;
+  if (opt.views_only) P.nv = P.nf = 0;
   P.fv.resize(3 * (size_t)P.nf);
   for (int f = 0; f < P.nf; f++)
     for (int k = 0; k < 3; k++) P.fv[3 * f + k] = rv->f[f].v[k];
@@ -246,7 +249,6 @@ int Engine::pack_model(int mi, struct par_t *par, struct mod_t *mod, struct dat_
     for (int k = 0; k < 3; k++) { P.u[3 * i + k] = v->u[k]; P.a[3 * i + k] = v->a[k]; }
     int src = v->r.state == '=' ? v->v_mirror : i;
     P.r[i] = rv->v[src].r.val;
-    tmap[&v->r.val] = ParamTarget{1, mi, i};
   }
   if (opt.host_geometry) {
     P.hx.resize(3 * (size_t)P.nv); P.hvn.resize(3 * (size_t)P.nv); P.hfn.resize(3 * (size_t)P.nf);
@@ -465,8 +467,17 @@ int Engine::build(struct par_t *par, struct mod_t **mods, struct dat_t **dats, i
     std::vector<double> r;
     for (size_t q = 0; q < S.targets.size(); q++) {
       auto it = model_params[m].find(S.targets[q]);
-      if (it == model_params[m].end()) return -16;  // not a parameter of this model
-      ParamTarget T = it->second;
+      ParamTarget T;
+      if (it != model_params[m].end()) {
+        T = it->second;
+      } else {
+        // a vertex deviation: the r.val field of one of the model's vertices
+        const ptrdiff_t d = (const char *)S.targets[q] - (const char *)&P.verts[0].r.val;
+        if (P.nv == 0 || d < 0 || d % (ptrdiff_t)sizeof(struct vertex_t) != 0 ||
+            d / (ptrdiff_t)sizeof(struct vertex_t) >= P.nv)
+          return -16;  // not a parameter of this model
+        T = ParamTarget{1, m, (int)(d / (ptrdiff_t)sizeof(struct vertex_t))};
+      }
       eval_targets[e].push_back(T);
       if (T.kind == 0) {
         h_params[pb + T.offset] = S.values[q];

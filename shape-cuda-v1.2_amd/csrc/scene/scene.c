@@ -765,6 +765,10 @@ static void build_data(shp_scene *sc)
     add_param(sc, &dc->a[0].val, SHP_DELCORPAR);
     add_param(sc, &dc->a[1].val, SHP_DELCORPAR);
   }
+  /* last, a vertex deviation: the radius of a vertex a third of the way
+     through the list (a shape parameter that moves only its own facets) */
+  struct vertices_t *vt = &sc->mod.shape.comp[0].desc.ver;
+  if (vt->nv > 0) add_param(sc, &vt->v[vt->nv / 3].r.val, SHP_SHAPEPAR);
 }
 
 static void build_par(shp_scene *sc)

@@ -31,10 +31,24 @@ def _oracle_eval(orc, sc, trial):
     return obj, chi, frames
 
 
+SHAPEPAR = 2  # scene parameter type of a vertex deviation
+
+
+def _trials(sc):
+    """TRIALS that fit the scene, plus its vertex deviation (the last
+    parameter) 5% out: a target addressed by a vertex, not the parameter
+    block."""
+    ntr = [t for t in TRIALS if all(i < sc.nparams() for i, _ in t)]
+    iv = sc.nparams() - 1
+    if iv >= 0 and sc.param_type(iv) == SHAPEPAR:
+        ntr.append([(iv, sc.get_param(iv) * 1.05)])
+    return ntr
+
+
 def _check(config, method, over, tol_chi, tol_img, flags=sh.BATCH_KEEP_FIT):
     orc = H.load_oracle()
     sc = H.make_scene_with_obs(config, **over)
-    ntr = [t for t in TRIALS if all(i < sc.nparams() for i, _ in t)]
+    ntr = _trials(sc)
     B = sh.Batch([sc], [(0, t) for t in ntr], method=method, flags=flags)
     obj, chi = B.run()
     cnt = B.counters()
