@@ -114,6 +114,7 @@ typedef struct shape_batch shape_batch;
 #define SHAPE_BATCH_KEEP_FIT 2  /* keep fit images for inspection (single replay) */
 #define SHAPE_BATCH_GLOBAL_ZBUF 4 /* mixed method: rasterize every view through a global z-buffer
                                      instead of per-tile LDS z-buffers (same results; for checking) */
+#define SHAPE_BATCH_FACET_MAP 8   /* keep every POS pixel's owner facet (shape_batch_facet_map) */
 
 shape_batch *shape_batch_create(struct par_t *par, struct mod_t **mods, struct dat_t **dats, int nmodels,
                                 int nevals, const int *eval_model, const int *n_targets,
@@ -126,6 +127,10 @@ float shape_batch_last_ms(shape_batch *b);              /* device time of the la
 int shape_batch_nframes(shape_batch *b, int e);
 int shape_batch_frame_results(shape_batch *b, int e, double *chi2v, double *cal);
 int shape_batch_fit(shape_batch *b, int e, int frame, double *out);
+/* owner facet of every POS pixel of view v of frame `frame` of evaluation e
+   ((2n+1)^2 ints, row i = -n..n, column j = -n..n; -1: no owner), from the
+   last replay; needs SHAPE_BATCH_FACET_MAP; radar frames only */
+int shape_batch_facet_map(shape_batch *b, int e, int frame, int v, int *out);
 /* per frame of evaluation e: o2, m2, om, overflow_o2, overflow_m2, cal, chi2,
    badradar_logfactor (8 doubles per frame, set/frame order) */
 int shape_batch_frame_terms(shape_batch *b, int e, double *out);

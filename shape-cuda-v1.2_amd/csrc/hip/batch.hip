@@ -44,6 +44,7 @@ extern "C" shape_batch *shape_batch_create(struct par_t *par, struct mod_t **mod
   o.method = method == 1 ? METHOD_MIXED : METHOD_FP64;
   o.keep_fit = (flags & SHAPE_BATCH_KEEP_FIT) ? 1 : 0;
   o.global_zbuf = (flags & SHAPE_BATCH_GLOBAL_ZBUF) ? 1 : 0;
+  o.facet_map = (flags & SHAPE_BATCH_FACET_MAP) ? 1 : 0;
   B->use_graph = (flags & SHAPE_BATCH_GRAPH) ? 1 : 0;
   int rc = B->E.build(par, mods, dats, nmodels, ev, o);
   if (!rc && B->use_graph) rc = B->E.capture_graph();
@@ -156,6 +157,22 @@ extern "C" int shape_batch_fit(shape_batch *B, int e, int q, double *out) {
   if (F.nviews > 1 && F.type != VT_LGHTCRV)
     for (long long i = 0; i < N; i++) out[i] /= F.nviews;
   return 0;
+}
+
+// owner facets of view v of frame q of evaluation e (needs SHAPE_BATCH_FACET_MAP)
+extern "C" int shape_batch_facet_map(shape_batch *B, int e, int q, int v, int *out) {
+  Engine &E = B->E;
+  if (!E.opt.facet_map) return -21;
+  if (e < 0 || e >= E.nevals) return -20;
+  const EvalRec &R = E.h_evals[e];
+  if (q < 0 || q >= R.nframes) return -20;
+  const FrameRec &F = E.h_frames[R.first_frame + q];
+  if (F.type == VT_LGHTCRV || v < 0 || v >= F.nviews) return -20;
+  const ViewRec &V = E.h_views[F.first_view + v];
+  const int *src = E.pd.fmap ? E.pd.fmap : E.pd.f;
+  if (!src) return -21;
+  const size_t np = (size_t)(2 * V.n + 1) * (2 * V.n + 1);
+  return hipMemcpy(out, src + V.pos_off, sizeof(int) * np, hipMemcpyDeviceToHost) == hipSuccess ? 0 : -4;
 }
 
 // counters of the last replay: [0] replayed pixels (FP64 method: owner faces

@@ -677,6 +677,8 @@ int Engine::build(struct par_t *par, struct mod_t **mods, struct dat_t **dats, i
   // POS buffers
   size_t P = (size_t)std::max(1LL, total_pix);
   memset(&pd, 0, sizeof(pd));
+  if (opt.facet_map && opt.method == METHOD_MIXED)
+    if ((rc = alloc((void **)&pd.fmap, sizeof(int) * P))) return rc;
   if ((rc = alloc((void **)&pd.zkey, sizeof(unsigned long long) * P))) return rc;
   SHP_CHECK(hipMemset(pd.zkey, 0, sizeof(unsigned long long) * P));
   if (opt.method == METHOD_FP64 || any_lc) {
@@ -778,6 +780,10 @@ int Engine::enqueue() {
   const int T = 256;
   dim3 gv(cdiv(nviews, 64)), bv(64);
   SHP_LAUNCH(*this, "k_zero_counters", k_zero_counters, dim3(1), dim3(64), 0, stream, pd.nstale, 8);
+  if (opt.facet_map) {  // owners of this replay only
+    int *fm = pd.fmap ? pd.fmap : pd.f;
+    if (fm) SHP_LAUNCH(*this, "k_fill_i32", k_fill_i32, dim3(1024), dim3(256), 0, stream, fm, std::max(1LL, total_pix), -1);
+  }
   SHP_LAUNCH(*this, "k_setup_views", k_setup_views, gv, bv, 0, stream, d_views, nviews, d_params);
   if (!opt.host_geometry) {
     int max_nv = 0;

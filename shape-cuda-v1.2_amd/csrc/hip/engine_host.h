@@ -63,6 +63,7 @@ struct RunOptions {
   int with_body = 0;        // allocate body/comp arrays (drop-ins)
   int views_only = 0;       // pack parameters and views only (spin / offset drop-ins)
   int global_zbuf = 0;      // mixed method: rasterize every view through the global z-buffer
+  int facet_map = 0;        // keep every POS pixel's owner facet for inspection
 };
 
 class Engine {

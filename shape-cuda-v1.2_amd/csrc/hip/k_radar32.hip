@@ -102,6 +102,7 @@ __global__ __launch_bounds__(256) void k_shade_radar32(ViewRec *__restrict__ vie
     int q = threadIdx.x + m * (int)blockDim.x;
     int i = i0 + (along_j ? q / TILE : q % TILE), j = j0 + (along_j ? q % TILE : q / TILE);
     int f = key_facet(k);
+    if (pd.fmap) pd.fmap[V.pos_off + (long long)(i + n) * w + (j + n)] = f;  // (inspection)
     Proj32 P;
     corners32(V, G, f, V.oa32, P);
     float s, t;
@@ -173,7 +174,8 @@ __device__ void replay32_one(ViewRec *views, const GeomSlot *slots, const PosDev
   for (int q = 0; q < nc; q++) c[q] = S.cand[q];
   sort_candidates(c, nc);
   double zz, cosa;
-  replay_owner64(V, G, c, nc, i, j, zz, cosa);
+  const int fac = replay_owner64(V, G, c, nc, i, j, zz, cosa);
+  if (pd.fmap) pd.fmap[V.pos_off + S.pix] = fac;  // (inspection)
   pd.stale_idx[V.pos_off + S.pix] = -1;
   if (!(cosa > 0.0)) return;
   replay_emit(V, params + V.pb + V.pb_law, i, j, zz, cosa, fit + frames[V.frame_id].off,

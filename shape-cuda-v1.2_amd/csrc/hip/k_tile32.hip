@@ -528,6 +528,7 @@ __global__ __launch_bounds__(256, 6) void k_tile32(const ViewRec *__restrict__ v
     const unsigned long long k = e | PIX_BITS;
     const int i = i0 + a, j = j0 + c;
     const int f = key_facet(k);
+    if (pd.fmap) pd.fmap[V.pos_off + (long long)(i + n) * (2 * n + 1) + (j + n)] = f;  // (inspection)
     Proj32 P;
     corners32f(V, G, f, V.oa32, P);
     float s, t;

@@ -49,6 +49,7 @@ struct PosDev {
   int *ncollect;             // mixed path: records whose candidates the global raster (k_collect32) collects
   int stale_cap;
   int *error;                // bit flags: 1 stale list full, 2 too many candidates
+  int *fmap;                 // mixed path, when asked for: each POS pixel's owner facet (-1: none)
 };
 
 // per-view outputs written by k_finish_views (doubles)

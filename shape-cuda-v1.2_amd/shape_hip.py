@@ -155,6 +155,7 @@ def load_hip():
             "shape_batch_frame_results": (C.c_int, [vp, C.c_int, C.POINTER(C.c_double),
                                                     C.POINTER(C.c_double)]),
             "shape_batch_fit": (C.c_int, [vp, C.c_int, C.c_int, C.POINTER(C.c_double)]),
+            "shape_batch_facet_map": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_int)]),
             "shape_batch_frame_terms": (C.c_int, [vp, C.c_int, C.POINTER(C.c_double)]),
             "shape_batch_counters": (C.c_int, [vp, C.POINTER(C.c_int)]),
             "shape_batch_sizes": (C.c_int, [vp, C.POINTER(C.c_longlong)]),
@@ -438,7 +439,7 @@ class Scene:
 
 
 METHOD_FP64, METHOD_MIXED = 0, 1
-BATCH_GRAPH, BATCH_KEEP_FIT, BATCH_GLOBAL_ZBUF = 1, 2, 4
+BATCH_GRAPH, BATCH_KEEP_FIT, BATCH_GLOBAL_ZBUF, BATCH_FACET_MAP = 1, 2, 4, 8
 
 
 class Batch:
@@ -519,6 +520,16 @@ class Batch:
         if rc:
             raise RuntimeError("shape_batch_fit failed (%d)" % rc)
         return out.reshape(shape)
+
+    def facet_map(self, e, q, n, v=0):
+        """Owner facet of every pixel of view v of frame q of evaluation e
+        (POS half-width n; -1: none), from the last replay; needs
+        BATCH_FACET_MAP."""
+        out = np.zeros((2 * n + 1, 2 * n + 1), dtype=np.int32)
+        rc = self.lib.shape_batch_facet_map(self.h, e, q, v, out.ctypes.data_as(C.POINTER(C.c_int)))
+        if rc:
+            raise RuntimeError("shape_batch_facet_map failed (%d)" % rc)
+        return out
 
     def counters(self):
         out = (C.c_int * 4)()

@@ -175,3 +175,45 @@ def test_mixed_binned_matches_global_zbuf(config, over):
         shape = sc.image(s, f, 0).shape
         it, ig = Bt.fit(0, q, shape), Bg.fit(0, q, shape)
         assert np.max(np.abs(it - ig)) <= 1e-11 * max(np.max(np.abs(ig)), 1e-300), q
+
+
+def _edge_pixels(fm):
+    """Pixels of an owner map whose 4-neighbourhood holds another owner."""
+    e = np.zeros(fm.shape, dtype=bool)
+    e[1:, :] |= fm[1:, :] != fm[:-1, :]
+    e[:-1, :] |= fm[:-1, :] != fm[1:, :]
+    e[:, 1:] |= fm[:, 1:] != fm[:, :-1]
+    e[:, :-1] |= fm[:, :-1] != fm[:, 1:]
+    return e
+
+
+@pytest.mark.parametrize("config,over", [(1, {}), (2, dict(n_deldop=16)),
+                                         (3, dict(n_deldop=4, n_doppler=4, n_lightcurves=1, lc_points=20,
+                                                  lc_ncalc=8))])
+def test_mixed_facet_map_matches_fp64(config, over):
+    """Owner facet of every POS pixel: the mixed method against the FP64
+    method on the same trial values.  Pixels near a facet edge are replayed
+    in FP64, so owners may differ only where two surfaces cross within FP32
+    depth resolution: at most 1e-4 of the owned pixels, all on facet edges of
+    the FP64 map.  The binned and global-z-buffer rasters agree exactly."""
+    sc = H.make_scene_with_obs(config, **over)
+    evals = [(0, [])]
+    flags = sh.BATCH_KEEP_FIT | sh.BATCH_FACET_MAP
+    Bm = sh.Batch([sc], evals, method=sh.METHOD_MIXED, flags=flags)
+    Bg = sh.Batch([sc], evals, method=sh.METHOD_MIXED, flags=flags | sh.BATCH_GLOBAL_ZBUF)
+    Bf = sh.Batch([sc], evals, method=sh.METHOD_FP64, flags=flags)
+    Bm.run(); Bg.run(); Bf.run()
+    n = sc.spec.pos_n
+    owned = mism = off_edge = 0
+    for q, (s, f) in enumerate(sc.all_frames()):
+        if f < 0 or sc.set_type(s) == sh.LGHTCRV:
+            continue
+        fm, fg, ff = Bm.facet_map(0, q, n), Bg.facet_map(0, q, n), Bf.facet_map(0, q, n)
+        assert np.array_equal(fm, fg), q
+        d = fm != ff
+        owned += int(np.count_nonzero(ff >= 0))
+        mism += int(np.count_nonzero(d))
+        off_edge += int(np.count_nonzero(d & ~_edge_pixels(ff)))
+    print("config", config, "owned", owned, "mismatched", mism, "off edges", off_edge)
+    assert owned > 0
+    assert mism <= 1e-4 * owned and off_edge == 0
