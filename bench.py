@@ -58,6 +58,9 @@ def parse():
     ap.add_argument("--copy-probe", action="store_true",
                     help="only copy 1 GiB device to device (calibrates HBM byte counters), then exit")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--configs", type=str, default=None,
+                    help="comma-separated workload configs (1-5) to report one JSON line each for: "
+                         "mixed and FP64 rates, latency, accuracy, facet-map mismatch, CPU baselines")
     ap.add_argument("--mixed-only", action="store_true",
                     help="time only the mixed replays (for profiler runs)")
     return ap.parse_args()
@@ -67,7 +70,11 @@ def parse():
 # scene and observations
 # ---------------------------------------------------------------------------
 def make_scene(frames):
-    sc = sh.Scene(sh.spec_for(4, n_deldop=frames))
+    return dropin_obs(sh.Scene(sh.spec_for(4, n_deldop=frames)))
+
+
+def dropin_obs(sc):
+    """Observations = the plain FP64 drop-in's fits (the noiseless truth) + noise."""
     hip = sh.load_hip()
     # fits of the plain FP64 drop-in are the noiseless "truth"
     hip.shape_hip_clear_error()
@@ -257,6 +264,144 @@ def measured_traffic(kernel, workload):
     return k.get("bytes_per_launch")
 
 
+# ---------------------------------------------------------------------------
+# per-config report (--configs): every workload of the measurement contract
+# ---------------------------------------------------------------------------
+def load_oracle():
+    path = os.path.join(ROOT, "oracle", "_build", "liboracle.so")
+    if not os.path.exists(path):
+        return None
+    orc = C.CDLL(path)
+    vp = C.c_void_p
+    for name, res, args in [("oracle_realize_mod", None, [vp, vp]), ("oracle_realize_spin", None, [vp, vp, vp]),
+                            ("oracle_realize_delcor", None, [vp, C.c_double, C.c_int]),
+                            ("oracle_calc_fits_threads", None, [vp, vp, vp, C.c_int]),
+                            ("oracle_chi2", C.c_double, [vp, vp, C.c_int])]:
+        fn = getattr(orc, name)
+        fn.restype, fn.argtypes = res, args
+    return orc
+
+
+def cpu_rate(orc, config, threads, max_frames):
+    """Oracle evaluations/s on `threads` host threads: realize + calc_fits +
+    chi2 of one model of `config`, on at most max_frames radar frames (scaled
+    to the config's full frame count; lightcurves always whole)."""
+    spec = sh.spec_for(config)
+    full = spec.n_deldop + spec.n_doppler
+    over = {"pos_scope": sh.LOCAL}
+    if full > max_frames:
+        nd = min(spec.n_deldop, max_frames)
+        over.update(n_deldop=nd, n_doppler=min(spec.n_doppler, max_frames - nd))
+    sc = sh.Scene(sh.spec_for(config, **over))
+    frames = sc.spec.n_deldop + sc.spec.n_doppler
+
+    def forward():
+        orc.oracle_realize_mod(sc.par, sc.mod)
+        orc.oracle_realize_spin(sc.par, sc.mod, sc.dat)
+        orc.oracle_realize_delcor(sc.dat, 0.0, 0)
+        orc.oracle_calc_fits_threads(sc.par, sc.mod, sc.dat, threads)
+        return orc.oracle_chi2(sc.par, sc.dat, 0)
+
+    forward()
+    sc.synthesize_obs(snr=20.0)
+    t0 = time.perf_counter()
+    forward()
+    dt = time.perf_counter() - t0
+    sc.close()
+    return frames / max(full, 1) / dt, frames
+
+
+def config_scenes(config):
+    if config == 5:  # 32 models, per-model seeds; 8 trial values of the first size parameter each
+        return [dropin_obs(sh.Scene(sh.spec_for(5, seed_geom=5000 + m, seed_frames=5500 + m))) for m in range(32)]
+    return [dropin_obs(sh.Scene(sh.spec_for(config)))]
+
+
+def config_evals(scenes, per_model):
+    ev = []
+    for m, sc in enumerate(scenes):
+        ev += [(m, t) for _, t in trial_evals(sc, per_model)]
+    return ev
+
+
+def facet_map_mismatch(sc, ev0):
+    """Owner-facet mismatch of the mixed method against FP64 over the radar
+    frames of one evaluation: (mismatched, owned, mismatched off facet edges)."""
+    flags = sh.BATCH_KEEP_FIT | sh.BATCH_FACET_MAP
+    Bm = sh.Batch([sc], [ev0], method=sh.METHOD_MIXED, flags=flags)
+    Bf = sh.Batch([sc], [ev0], method=sh.METHOD_FP64, flags=flags)
+    Bm.run()
+    Bf.run()
+    n = sc.spec.pos_n
+    mism = owned = off = 0
+    img = 0.0
+    for q, (s_, f_) in enumerate(sc.all_frames()):
+        if f_ < 0 or sc.set_type(s_) == sh.LGHTCRV:
+            continue
+        fm, ff = Bm.facet_map(0, q, n), Bf.facet_map(0, q, n)
+        d = fm != ff
+        edge = np.zeros(ff.shape, dtype=bool)
+        edge[1:, :] |= ff[1:, :] != ff[:-1, :]
+        edge[:-1, :] |= ff[:-1, :] != ff[1:, :]
+        edge[:, 1:] |= ff[:, 1:] != ff[:, :-1]
+        edge[:, :-1] |= ff[:, :-1] != ff[:, 1:]
+        mism += int(np.count_nonzero(d))
+        owned += int(np.count_nonzero(ff >= 0))
+        off += int(np.count_nonzero(d & ~edge))
+        shape = sc.image(s_, f_, 0).shape
+        im, imf = Bm.fit(0, q, shape), Bf.fit(0, q, shape)
+        img = max(img, float(np.max(np.abs(im - imf)) / max(float(np.max(np.abs(imf))), 1e-300)))
+    Bm.close()
+    Bf.close()
+    return mism, owned, off, img
+
+
+def config_report(config, a, orc):
+    log("config %d: scenes" % config)
+    scenes = config_scenes(config)
+    per_model = 8 if config == 5 else a.evals
+    ev = config_evals(scenes, per_model)
+    ne = len(ev)
+    steps = a.steps if config == 4 else max(a.steps, 100)
+    Bm = sh.Batch(scenes, ev, method=sh.METHOD_MIXED, flags=sh.BATCH_GRAPH)
+    Bm.run()
+    t_m, lat = time_batch(Bm, steps, a.warmup)
+    obj_m, chi_m = Bm.run()
+    fp = Bm.footprint()
+    Bm.close()
+    fsteps = max(3, min(20, steps // 5))
+    Bf = sh.Batch(scenes, ev, method=sh.METHOD_FP64, flags=sh.BATCH_GRAPH)
+    obj_f, chi_f = Bf.run()
+    t_f, _ = time_batch(Bf, fsteps, 1)
+    Bf.close()
+    t_d, _ = time_dropin(scenes[0], 3)
+    mism, owned, off, img = facet_map_mismatch(scenes[0], ev[0])
+    nv = sum(sc.nv for sc in scenes) // len(scenes)
+    nf = sum(sc.nf for sc in scenes) // len(scenes)
+    ms = 1e3 * t_m / steps
+    pb = path_bytes(fp, ne, nv, nf)
+    out = {"config": config, "models": len(scenes), "evals_per_replay": ne, "vertices": nv, "facets": nf,
+           "frames_per_model": len(scenes[0].all_frames()),
+           "mixed_evals_per_s": round(ne * steps / t_m, 2), "replay_ms_p50": round(float(np.percentile(lat, 50)), 4),
+           "replay_ms_p99": round(float(np.percentile(lat, 99)), 4), "replays": steps,
+           "fp64_batch_evals_per_s": round(ne * fsteps / t_f, 2), "fp64_dropin_evals_per_s": round(1.0 / t_d, 2),
+           "chi2_rel_err_vs_fp64": float(np.max(np.abs(chi_m - chi_f) / np.abs(chi_f))),
+           "image_rel_err_vs_fp64": img, "facet_map_mismatch": mism / max(owned, 1),
+           "facet_map_mismatch_off_edges": off, "owned_pixels": owned,
+           "path_roofline_frac": round(pb / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+    if orc is not None and not a.no_cpu:
+        threads = max(1, min(16, len(os.sched_getaffinity(0))))
+        r1, n1 = cpu_rate(orc, config, 1, 16)
+        rt, nt = cpu_rate(orc, config, threads, 256)
+        out["cpu_1thread_evals_per_s"] = round(r1, 4)
+        out["cpu_threads"] = threads
+        out["cpu_threads_evals_per_s"] = round(rt, 4)
+        out["cpu_sample_frames"] = [n1, nt]
+    for sc in scenes:
+        sc.close()
+    return out
+
+
 def main():
     global KSTEPS
     a = parse()
@@ -272,6 +417,11 @@ def main():
     hip = sh.load_hip()
     if hip.shape_hip_device_count() < 1:
         raise RuntimeError("no HIP device")
+    if a.configs:
+        orc = load_oracle()
+        for c in [int(x) for x in a.configs.split(",")]:
+            print(json.dumps(config_report(c, a, orc)), flush=True)
+        return
     log("building scene: %d frames" % a.frames)
     sc = make_scene(a.frames)
     evals = trial_evals(sc, a.evals)
