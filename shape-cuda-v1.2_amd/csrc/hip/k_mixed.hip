@@ -91,16 +91,17 @@ __device__ inline void raster_facet(ViewRec &V, const GeomSlot &G, const PosDev 
     bool hit = false;
     Edge32 E;
     edge32_init(P, i1, j1, E);
+    const float amb = facet_amb(P);
     float di = 0.0f, dj = 0.0f;
     for (int c = 0, i = i1, j = j1; c < nc; c++) {
       float s, t;
       float m = edge32(E, di, dj, s, t);
       long long pix = (long long)(i + n) * w + (j + n);
-      if (m >= AMB_TOL || (src && m >= -AMB_TOL)) {
+      if (m >= amb || (src && m >= -amb)) {
         hit = true;
         atomicMax(&kb[pix], (unsigned long long)pack_key(depth32(P, s, t), f));
       }
-      bool marginal = !src && m >= -AMB_TOL && m < AMB_TOL;
+      bool marginal = !src && m >= -amb && m < amb;
       hit = hit || marginal;
       mark_marginal(V, pd, marginal, (int)pix, i, j);
       if (++j > j2) { j = j1; i++; dj = 0.0f; di += 1.0f; } else { dj += 1.0f; }
@@ -122,11 +123,12 @@ __device__ inline void raster_facet(ViewRec &V, const GeomSlot &G, const PosDev 
       int nc = (i2 - i1 + 1) * (j2 - j1 + 1);
       Edge32 E;
       edge32_init(P, i1, j1, E);
+      const float amb = facet_amb(P);
       float di = 0.0f, dj = 0.0f;
       for (int c = 0, i = i1, j = j1; c < nc; c++) {
         // candidates: a superset of the facets covering the pixel in FP64
         float s, t;
-        if (edge32(E, di, dj, s, t) >= -AMB_TOL) {
+        if (edge32(E, di, dj, s, t) >= -amb) {
           int si = pd.stale_idx[V.pos_off + (long long)(i + n) * w + (j + n)];
           if (si >= 0) {
             int q = atomicAdd(&pd.stale[si].ncand, 1);

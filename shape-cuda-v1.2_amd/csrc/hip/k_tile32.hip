@@ -359,7 +359,7 @@ __global__ __launch_bounds__(256, 6) void k_tile32(const ViewRec *__restrict__ v
         const int q = q0 + lane;
         Proj32 P;
         Edge32 E;
-        float nz = 0.0f;
+        float nz = 0.0f, amb = AMB_TOL;
         int f = 0, ti1 = 0, ti2 = -1, tj1 = 0, tj2 = -1, wi1 = 0, wi2 = 0, wj1 = 0, wj2 = 0;
         if (q < nl) {
           f = flist[q];
@@ -374,6 +374,7 @@ __global__ __launch_bounds__(256, 6) void k_tile32(const ViewRec *__restrict__ v
             wi1 = max(imin, -n); wi2 = min(imax, n); wj1 = max(jmin, -n); wj2 = min(jmax, n);
             ti1 = max(wi1, i0); ti2 = min(wi2, i0 + BT - 1); tj1 = max(wj1, j0); tj2 = min(wj2, j0 + BT - 1);
             edge32_init(P, wi1, wj1, E);  // anchored as in the global raster
+            amb = facet_amb(P);
           }
         }
         const bool front = nz > 0.0f;
@@ -397,8 +398,8 @@ __global__ __launch_bounds__(256, 6) void k_tile32(const ViewRec *__restrict__ v
           for (int lp = lp0; lp < lp_end;) {
             float s, t;
             const float mg = edge32(E, di, dj, s, t);
-            const bool cov = mg >= -AMB_TOL;
-            if (front && mg >= AMB_TOL) {
+            const bool cov = mg >= -amb;
+            if (front && mg >= amb) {
               atomicMax(&key[lp], (unsigned long long)pack_key(depth32(P, s, t), f));
               hit = true;
             } else if (front && cov) {

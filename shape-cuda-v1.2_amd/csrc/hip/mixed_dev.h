@@ -160,9 +160,22 @@ __device__ inline bool block_rect(const ViewRec &V, const GeomSlot &G, int b, in
 // center within that distance of an edge may be covered by a different facet
 // than in FP64, or by none.  Such a center is "marginal": no facet claims it
 // in the FP32 z-buffer; it is recorded and resolved in FP64, in facet order,
-// exactly as the CPU does.  AMB_TOL bounds the band in barycentric units:
-// >= 5x the FP32 error for facets of >= 2 px.
+// exactly as the CPU does.  The band is facet_amb(P) in barycentric units:
+// AMB_TOL, >= 5x the FP32 error for well-shaped facets of >= 2 px, widened
+// for a facet whose barycentric coordinates amplify that error (a sliver, or
+// a facet seen nearly edge-on: its projected area is small).
 constexpr float AMB_TOL = 2.0e-4f;
+// The FP32 error of a barycentric coordinate is about the vertex error
+// (a few ulps of the largest projected coordinate M) times |grad s| <=
+// den x longest edge; the band takes 4x that, and never less than AMB_TOL.
+__device__ inline float facet_amb(const Proj32 &P) {
+  const float M = fmaxf(fmaxf(fmaxf(fabsf(P.x0), fabsf(P.y0)), fmaxf(fabsf(P.x1), fabsf(P.y1))),
+                        fmaxf(fabsf(P.x2), fabsf(P.y2)));
+  const float e01 = fabsf(P.x1 - P.x0) + fabsf(P.y1 - P.y0), e12 = fabsf(P.x2 - P.x1) + fabsf(P.y2 - P.y1),
+              e20 = fabsf(P.x0 - P.x2) + fabsf(P.y0 - P.y2);
+  const float dx = 3.0f * 5.9604645e-8f * (M + 1.0f);  // 3 ulps of the largest coordinate (px)
+  return fmaxf(AMB_TOL, 4.0f * dx * fabsf(P.den) * fmaxf(e01, fmaxf(e12, e20)));
+}
 __device__ inline float bary32(const Proj32 &P, int i, int j, float &s, float &t) {
 #pragma clang fp contract(off)
   float xi = (float)i, yj = (float)j;
