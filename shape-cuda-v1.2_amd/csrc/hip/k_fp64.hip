@@ -307,6 +307,7 @@ __global__ void k_realize_vnormals(GeomSlot *slots, int nslot) {
 __global__ void k_raster64(ViewRec *views, const GeomSlot *slots, PosDev pd, int src, int pass) {
   ViewRec &V = views[blockIdx.y];
   if (V.fp64_skip) return;
+  if (pass == 3 && !V.nstale) return;  // no replay record in this view: no candidates to collect
   const GeomSlot &G = slots[V.slot];
   int f = blockIdx.x * blockDim.x + threadIdx.x;
   int n = V.n, w = 2 * n + 1;
@@ -436,6 +437,7 @@ __global__ void k_resolve64(ViewRec *views, const GeomSlot *slots, PosDev pd, in
     }
   } else {
     // cos(scattering) is inherited from an earlier claimant: record for replay
+    V.nstale = 1;  // (pass 3 collects candidates only in views with records)
     int si = atomicAdd(pd.nstale, 1);
     if (si < pd.stale_cap) {
       StaleRec &S = pd.stale[si];

@@ -60,6 +60,7 @@ __global__ __launch_bounds__(256) void k_shade_radar32(ViewRec *__restrict__ vie
       }
       key[m] = k;
       if (k) {
+#pragma clang fp contract(off)
         double dop64 = V.ax * (double)i + V.ay * (double)j + V.dopshift;
         lo_p = min(lo_p, (int)floor(dop64 - V.dopdiff_max + 1 - width / 2.0) - V.P0);
         hi_p = max(hi_p, (int)floor(dop64 + V.dopdiff_max + width / 2.0) - V.P0);

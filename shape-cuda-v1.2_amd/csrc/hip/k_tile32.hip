@@ -467,6 +467,7 @@ __global__ __launch_bounds__(256, 6) void k_tile32(const ViewRec *__restrict__ v
         tile_stale(V, pd, ev_ok, lp, i, j, flag, spix, &nts);
       }
       if (k) {
+#pragma clang fp contract(off)
         double dop64 = V.ax * (double)i + V.ay * (double)j + V.dopshift;
         lo_p = min(lo_p, (int)floor(dop64 - V.dopdiff_max + 1 - width / 2.0) - V.P0);
         hi_p = max(hi_p, (int)floor(dop64 + V.dopdiff_max + width / 2.0) - V.P0);

@@ -407,6 +407,8 @@ __device__ inline float del_w(const ViewRec &V, const RadarPix &R, int a) {
 }
 
 __device__ inline void radar_pixel(const ViewRec &V, const double *pl, int i, int j, double z, float cose, RadarPix &R) {
+  // (FP64 bin coordinates in the CPU's order of operations: no contraction)
+#pragma clang fp contract(off)
   const bool dd = V.type == VT_DELAY;
   int width = V.sinc2width, ns = V.nsinc2;
   R.wide = false;
