@@ -20,10 +20,9 @@
  * Pins (tests/test_oracle.py): the reference ships no tests or golden
  * vectors, so each function is checked against closed forms, conservation
  * laws, brute force or an independent library.  Parity unpinned (restated
- * from the reference with no independent check): the radar laws other than
- * the cosine law, posmask's branch for partially blank bilinear stencils
- * (posmask.c:129-166), the accumulation order of vertex-normal sums, and
- * libm sin/pow/exp at the ulp level.
+ * from the reference with no independent check): the tabular and
+ * inhomogeneous radar laws (radlaw.c:56-66,118-135), the accumulation order
+ * of vertex-normal sums, and libm sin/pow/exp at the ulp level.
  */
 #ifndef SHAPE_ORACLE_H
 #define SHAPE_ORACLE_H

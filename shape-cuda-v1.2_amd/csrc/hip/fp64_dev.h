@@ -106,4 +106,36 @@ __device__ inline void shade64(const ViewRec &V, const GeomSlot &G, int f, int s
   if (!src && V.bistatic) cosb = nrm[0] * V.usrc[0] + nrm[1] * V.usrc[1] + nrm[2] * V.usrc[2];
 }
 
+// radar scattering laws in FP64 (radlaw.c:51-117), in the reference's order
+// of operations; pl: [type, R, C, cos_cutoff, qsR, qsC, qs_cutoff]
+__device__ inline double radlaw64(const double *pl, int type, double c) {
+#pragma clang fp contract(off)
+  double d = 0.0;
+  switch (type) {
+  case COSINELAW_DIFF: d = pl[1] * (pl[2] + 1) * pow(c, 2 * pl[2] - 1); break;
+  case GAUSSIANLAW:
+    if (c >= pl[3]) { double t2 = 1 / (c * c) - 1; d = pl[1] * pl[2] * exp(-pl[2] * t2) / (c * c * c * c * c); }
+    break;
+  case HAGFORSLAW:
+    if (c >= pl[3]) { double s2 = 1 - c * c, a = c * c * c * c + pl[2] * s2; d = 0.5 * pl[1] * pl[2] * pow(a, -1.5) / c; }
+    break;
+  case COSINELAW_QS:
+    if (c >= pl[3]) d = pl[1] * (pl[2] + 1) * pow(c, 2 * pl[2] - 1);
+    break;
+  case GAUSSIAN_COSINE:
+    d = pl[1] * (pl[2] + 1) * pow(c, 2 * pl[2] - 1);
+    if (c >= pl[6]) { double t2 = 1 / (c * c) - 1; d += pl[4] * pl[5] * exp(-pl[5] * t2) / (c * c * c * c * c); }
+    break;
+  case HAGFORS_COSINE:
+    d = pl[1] * (pl[2] + 1) * pow(c, 2 * pl[2] - 1);
+    if (c >= pl[6]) { double s2 = 1 - c * c, a = c * c * c * c + pl[5] * s2; d += 0.5 * pl[4] * pl[5] * pow(a, -1.5) / c; }
+    break;
+  case COSINE_COSINE:
+    d = pl[1] * (pl[2] + 1) * pow(c, 2 * pl[2] - 1);
+    if (c >= pl[6]) d += pl[4] * (pl[5] + 1) * pow(c, 2 * pl[5] - 1);
+    break;
+  }
+  return d;
+}
+
 }  // namespace shp

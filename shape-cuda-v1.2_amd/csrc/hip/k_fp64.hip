@@ -710,36 +710,6 @@ __device__ inline double delay_weight64(const ViewRec &V, double delPOS, int ide
   return tri2d((delPOS - idel) / (V.spb / V.stride));
 }
 
-__device__ inline double radlaw64(const double *pl, int type, double c) {
-  // pl: [type, R, C, cos_cutoff, qsR, qsC, qs_cutoff]
-  double d = 0.0;
-  switch (type) {
-  case COSINELAW_DIFF: d = pl[1] * (pl[2] + 1) * pow(c, 2 * pl[2] - 1); break;
-  case GAUSSIANLAW:
-    if (c >= pl[3]) { double t2 = 1 / (c * c) - 1; d = pl[1] * pl[2] * exp(-pl[2] * t2) / (c * c * c * c * c); }
-    break;
-  case HAGFORSLAW:
-    if (c >= pl[3]) { double s2 = 1 - c * c, a = c * c * c * c + pl[2] * s2; d = 0.5 * pl[1] * pl[2] * pow(a, -1.5) / c; }
-    break;
-  case COSINELAW_QS:
-    if (c >= pl[3]) d = pl[1] * (pl[2] + 1) * pow(c, 2 * pl[2] - 1);
-    break;
-  case GAUSSIAN_COSINE:
-    d = pl[1] * (pl[2] + 1) * pow(c, 2 * pl[2] - 1);
-    if (c >= pl[6]) { double t2 = 1 / (c * c) - 1; d += pl[4] * pl[5] * exp(-pl[5] * t2) / (c * c * c * c * c); }
-    break;
-  case HAGFORS_COSINE:
-    d = pl[1] * (pl[2] + 1) * pow(c, 2 * pl[2] - 1);
-    if (c >= pl[6]) { double s2 = 1 - c * c, a = c * c * c * c + pl[5] * s2; d += 0.5 * pl[4] * pl[5] * pow(a, -1.5) / c; }
-    break;
-  case COSINE_COSINE:
-    d = pl[1] * (pl[2] + 1) * pow(c, 2 * pl[2] - 1);
-    if (c >= pl[6]) d += pl[4] * (pl[5] + 1) * pow(c, 2 * pl[5] - 1);
-    break;
-  }
-  return d;
-}
-
 // one radar scattering-law value (radlaw.c:44-148)
 __global__ void k_radlaw_one(const double *pl, int type, double c, double *out) {
   if (threadIdx.x == 0 && blockIdx.x == 0) *out = radlaw64(pl, type, c);

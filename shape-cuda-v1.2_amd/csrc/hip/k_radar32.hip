@@ -110,14 +110,17 @@ __global__ __launch_bounds__(256) void k_shade_radar32(ViewRec *__restrict__ vie
     bary32(P, i, j, s, t);  // the raster's coordinates, to FP32 rounding
     const float nz = normal32_los(G, f, V.oa32, V.smooth, s, t);
     // cos(scattering) of an owner facing away is inherited from an earlier
-    // claimant: FP64 replay (records reserved once per wave)
-    int si = stale_slot(pd, !(nz > 0.0f));
-    if (!(nz > 0.0f)) {
+    // claimant, or (NaN law) too close to a quasispecular law's cutoff for
+    // FP32: FP64 replay (records reserved once per wave)
+    const float law = nz > 0.0f ? radlaw32(V, G, pl, f, s, t, nz) : 0.0f;
+    const bool rep = !(nz > 0.0f) || law != law;
+    int si = stale_slot(pd, rep);
+    if (rep) {
       if (si >= 0) stale_init(V, pd, si, blockIdx.y, (i + n) * w + (j + n), i, j);
       continue;
     }
     RadarPix R;
-    radar_pixel(V, pl, i, j, (double)key_z(k), nz, R);
+    radar_pixel(V, i, j, (double)key_z(k), law, R);
     if (!R.inb) anyo = 1;
     if (R.wide) { atomicAdd(&counters[3], 1); continue; }
     if (use_lds && R.inb) {

@@ -325,36 +325,76 @@ __device__ inline float delay_w32(const ViewRec &V, float drel) {
 // each; the per-pixel FP32 contributions carry ~1e-7 relative anyway)
 __device__ inline float powc(float c, float e) { return __builtin_amdgcn_exp2f(e * __builtin_amdgcn_logf(c)); }
 
-// radar scattering laws (radlaw.c:51-121); pl = [type, R, C, cutoff, qsR, qsC, qs_cutoff].
-// The laws other than the diffuse cosine law are out of line (cold code).
-__device__ __attribute__((noinline)) float radlaw32_other(const double *pl, int type, float c) {
-  float R = (float)pl[1], Cc = (float)pl[2];
-  float d = 0.0f;
-  switch (type) {
-  case GAUSSIANLAW:
-    if (c >= (float)pl[3]) { float t2 = 1.0f / (c * c) - 1.0f; d = R * Cc * __expf(-Cc * t2) / (c * c * c * c * c); }
-    break;
-  case HAGFORSLAW:
-    if (c >= (float)pl[3]) { float a = c * c * c * c + Cc * (1.0f - c * c); d = 0.5f * R * Cc * rsqrtf(a * a * a) / c; }
-    break;
-  case COSINELAW_QS:
-    if (c >= (float)pl[3]) d = R * (Cc + 1.0f) * powc(c, 2.0f * Cc - 1.0f);
-    break;
-  default: {
-    d = R * (Cc + 1.0f) * powc(c, 2.0f * Cc - 1.0f);
-    float qR = (float)pl[4], qC = (float)pl[5];
-    if (c >= (float)pl[6]) {
-      if (type == GAUSSIAN_COSINE) { float t2 = 1.0f / (c * c) - 1.0f; d += qR * qC * __expf(-qC * t2) / (c * c * c * c * c); }
-      else if (type == HAGFORS_COSINE) { float a = c * c * c * c + qC * (1.0f - c * c); d += 0.5f * qR * qC * rsqrtf(a * a * a) / c; }
-      else if (type == COSINE_COSINE) d += qR * (qC + 1.0f) * powc(c, 2.0f * qC - 1.0f);
-    }
+// The diffuse cosine law R(C+1) cos^(2C-1) (radlaw.c:51-54); pl = [type, R, C, ...].
+__device__ inline float cosine_law32(const double *pl, float c) {
+  return (float)pl[1] * ((float)pl[2] + 1.0f) * powc(c, 2.0f * (float)pl[2] - 1.0f);
+}
+
+// Half-width, in cos(scattering), of the band around a quasispecular law's
+// cutoff inside which a pixel's FP32 cosine cannot settle the cutoff test:
+// well above the FP32 error of the LOS normal.
+constexpr float CUT_BAND = 2e-5f;
+
+// The other homogeneous laws (radlaw.c:64-117) for a lit pixel of facet f
+// with FP32 cos(scattering) c at barycentric (s, t); pl = [type, R, C,
+// cutoff, qsR, qsC, qs_cutoff].  Their quasispecular terms peak steeply at
+// c = 1, where 1 - c^2 cancels: the Hagfors argument c^4 + C sin^2, the
+// Gaussian tan^2 = sin^2 / c^2 and cos^(2C-1) = exp((C - 1/2) log1p(-sin^2))
+// take sin^2 from the LOS-frame normal's own x and y components instead.
+// Returns NaN for a pixel within CUT_BAND of the cutoff: the caller hands it
+// to the FP64 replay, which evaluates the law from the CPU's cosine.
+// Out of line: no configuration of the hot path uses these laws.
+__device__ __attribute__((noinline)) float radlaw32_steep(const ViewRec &V, const GeomSlot &G, const double *pl, int f,
+                                                           float s, float t, float c) {
+#pragma clang fp contract(off)
+  const int type = V.law_type;
+  const bool hybrid = type >= GAUSSIAN_COSINE;
+  const float cut = (float)pl[hybrid ? 6 : 3];
+  if (fabsf(c - cut) <= CUT_BAND) return __builtin_nanf("");
+  float d = 0.0f, R = (float)pl[1], Cc = (float)pl[2];
+  if (hybrid) {
+    d = R * (Cc + 1.0f) * powc(c, 2.0f * Cc - 1.0f);  // the diffuse component
+    R = (float)pl[4];
+    Cc = (float)pl[5];
   }
+  if (!(c >= cut)) return d;
+  float bx, by, bz;  // body-frame normal: the facet's, or its vertex normals' at (s, t)
+  if (V.smooth) {
+    const int *fv = G.fv + 3 * f;
+    const float4 na = G.vn32[fv[0]], nb = G.vn32[fv[1]], nc = G.vn32[fv[2]];
+    bx = fmaf(t, nc.x - nb.x, fmaf(s, nb.x - na.x, na.x));
+    by = fmaf(t, nc.y - nb.y, fmaf(s, nb.y - na.y, na.y));
+    bz = fmaf(t, nc.z - nb.z, fmaf(s, nb.z - na.z, na.z));
+  } else {
+    const float4 n = G.fn32[f];
+    bx = n.x; by = n.y; bz = n.z;
+  }
+  const float *m = V.oa32;
+  const float x = fmaf(m[2], bz, fmaf(m[1], by, m[0] * bx));
+  const float y = fmaf(m[5], bz, fmaf(m[4], by, m[3] * bx));
+  const float s2 = fmaf(x, x, y * y) / fmaf(bz, bz, fmaf(by, by, bx * bx));
+  const float c2 = c * c;
+  switch (type) {
+  case GAUSSIANLAW: case GAUSSIAN_COSINE:
+    d += R * Cc * expf(-Cc * (s2 / c2)) / (c2 * c2 * c);
+    break;
+  case HAGFORSLAW: case HAGFORS_COSINE: {
+    const float q = c2 * c2 + Cc * s2;
+    d += 0.5f * R * Cc / (q * sqrtf(q) * c);
+    break;
+  }
+  default:  // COSINELAW_QS, COSINE_COSINE
+    d += R * (Cc + 1.0f) * expf((Cc - 0.5f) * log1pf(-s2));
   }
   return d;
 }
-__device__ inline float radlaw32(const double *pl, int type, float c) {
-  if (type == COSINELAW_DIFF) return (float)pl[1] * ((float)pl[2] + 1.0f) * powc(c, 2.0f * (float)pl[2] - 1.0f);
-  return radlaw32_other(pl, type, c);
+
+// A lit pixel's radar law value from its FP32 cos(scattering); NaN asks for
+// the FP64 replay (radlaw32_steep).
+__device__ inline float radlaw32(const ViewRec &V, const GeomSlot &G, const double *pl, int f, float s, float t,
+                                 float c) {
+  if (V.law_type == COSINELAW_DIFF) return cosine_law32(pl, c);
+  return radlaw32_steep(V, G, pl, f, s, t, c);
 }
 
 // One lit POS pixel's delay-Doppler footprint (pos2deldop.c:320-482 /
@@ -406,7 +446,7 @@ __device__ inline float del_w(const ViewRec &V, const RadarPix &R, int a) {
   return delay_w32(V, R.dfr + (float)(R.dm0 - a));
 }
 
-__device__ inline void radar_pixel(const ViewRec &V, const double *pl, int i, int j, double z, float cose, RadarPix &R) {
+__device__ inline void radar_pixel(const ViewRec &V, int i, int j, double z, float law, RadarPix &R) {
   // (FP64 bin coordinates in the CPU's order of operations: no contraction)
 #pragma clang fp contract(off)
   const bool dd = V.type == VT_DELAY;
@@ -473,7 +513,7 @@ __device__ inline void radar_pixel(const ViewRec &V, const double *pl, int i, in
     R.inb = (V.P0 + R.ip0 >= 1) && (V.P0 + R.ip1 <= V.ndop);
   }
   R.amp = R.wide ? 0.0f
-                  : radlaw32(pl, V.law_type, cose) * V.kmpp32 * V.kmpp32 * codef * __builtin_amdgcn_rcpf(sumd * sump);
+                  : law * V.kmpp32 * V.kmpp32 * codef * __builtin_amdgcn_rcpf(sumd * sump);
 }
 
 // scatter one pixel straight into the HBM fit image (FP64 atomics) or, for

@@ -52,13 +52,22 @@ __device__ inline void lower_limit_d(unsigned long long *a, double v) {
   if (o < *(volatile unsigned long long *)a) atomicMin(a, o);
 }
 
+// radlaw64 for the laws other than the diffuse cosine law, out of line so
+// that it costs the replay kernel no registers where those laws are not used
+__device__ __attribute__((noinline)) float radlaw64_other(const double *pl, int type, double c) {
+  return (float)radlaw64(pl, type, c);
+}
+
 // Scatter a replayed lit pixel straight into the HBM fit image (or the
 // overflow spectrum) and grow the view's bin limits by it: a replayed pixel
 // is lit like any other.
 __device__ inline void replay_emit(ViewRec &V, const double *pl, int i, int j, double zz, double cosa, double *img,
                                    double *ov) {
   RadarPix R;
-  radar_pixel(V, pl, i, j, zz, (float)cosa, R);
+  // the other laws in FP64 from the CPU's cosine: their cutoff test and
+  // steep quasispecular terms need it
+  const float law = V.law_type == COSINELAW_DIFF ? cosine_law32(pl, (float)cosa) : radlaw64_other(pl, V.law_type, cosa);
+  radar_pixel(V, i, j, zz, law, R);
   radar_scatter_global(V, R, img, ov);
   const bool dd = V.type == VT_DELAY;
   if (dd) {

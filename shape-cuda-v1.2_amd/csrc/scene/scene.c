@@ -988,6 +988,42 @@ void shp_pos_array(struct pos_t *pos, int which, double *out)
     for (int j = -n; j <= n; j++) *out++ = d ? d[i][j] : (double)iv[i][j];
 }
 
+/* overwrite POS array `which` (numbering of shp_pos_array) from in[(2n+1)^2] */
+void shp_pos_set_array(struct pos_t *pos, int which, const double *in)
+{
+  int n = pos->n;
+  double **d = NULL;
+  int **iv = NULL;
+  switch (which) {
+  case 0: d = pos->b; break;
+  case 1: d = pos->cose; break;
+  case 2: d = pos->cosi; break;
+  case 3: d = pos->z; break;
+  case 4: d = pos->cosill; break;
+  case 5: d = pos->zill; break;
+  case 10: iv = pos->f; break;
+  case 11: iv = pos->body; break;
+  case 12: iv = pos->comp; break;
+  case 13: iv = pos->fill; break;
+  default: return;
+  }
+  for (int i = -n; i <= n; i++)
+    for (int j = -n; j <= n; j++) {
+      if (d) d[i][j] = *in++;
+      else iv[i][j] = (int)*in++;
+    }
+}
+
+/* set the POS's observer and source frames (row-major 3x3) */
+void shp_pos_set_frames(struct pos_t *pos, const double *oe, const double *se)
+{
+  for (int r = 0; r < 3; r++)
+    for (int c = 0; c < 3; c++) {
+      pos->oe[r][c] = oe[3 * r + c];
+      pos->se[r][c] = se[3 * r + c];
+    }
+}
+
 void shp_pos_copy(struct pos_t *dst, const struct pos_t *src)
 {
   int n = src->n;
@@ -1139,6 +1175,34 @@ void shp_radar_law(struct mod_t *mod, struct dat_t *dat, int s, double out[2])
   int l = set->type == DELAY ? set->desc.deldop.iradlaw : set->desc.doppler.iradlaw;
   out[0] = mod->photo.radar[l].RC.R.val;
   out[1] = mod->photo.radar[l].RC.C.val;
+}
+
+/* Replace radar law l by one of the homogeneous laws (types 1-7), keeping
+   the addresses of its fittable R and C.  p: cosine law [R, C]; quasispecular
+   laws [R, C, cos_cutoff]; hybrids [diffuse R, C, quasispecular R, C,
+   cos_cutoff].  Returns 0, or -1 for another type. */
+int shp_set_radar_law(struct mod_t *mod, int l, int type, const double *p)
+{
+  if (l < 0 || l >= mod->photo.nradlaws) return -1;
+  union radscat_t *r = &mod->photo.radar[l];
+  switch (type) {
+  case COSINELAW_DIFF:
+    r->RC.R.val = p[0]; r->RC.C.val = p[1]; r->RC.R_save = p[0];
+    break;
+  case GAUSSIANLAW: case HAGFORSLAW: case COSINELAW_QS:
+    r->quasispec.R.val = p[0]; r->quasispec.C.val = p[1]; r->quasispec.cos_cutoff = p[2];
+    r->quasispec.R_save = p[0];
+    break;
+  case GAUSSIAN_COSINE: case HAGFORS_COSINE: case COSINE_COSINE:
+    r->hybrid.diff.R.val = p[0]; r->hybrid.diff.C.val = p[1]; r->hybrid.diff.R_save = p[0];
+    r->hybrid.qs.R.val = p[2]; r->hybrid.qs.C.val = p[3]; r->hybrid.qs.cos_cutoff = p[4];
+    r->hybrid.qs.R_save = p[2];
+    break;
+  default:
+    return -1;
+  }
+  mod->photo.radtype[l] = (unsigned char)type;
+  return 0;
 }
 
 /* pointer to the model's photo_t (for radlaw / apply_photo calls) */

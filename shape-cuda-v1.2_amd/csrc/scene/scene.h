@@ -133,6 +133,9 @@ void shp_par_set(struct par_t *par, int which, double v);
 struct photo_t *shp_mod_photo(struct mod_t *mod);
 int shp_vertex_facets(struct mod_t *mod, int v, int *out, int cap);
 void shp_radar_law(struct mod_t *mod, struct dat_t *dat, int s, double out[2]);
+void shp_pos_set_array(struct pos_t *pos, int which, const double *in);
+void shp_pos_set_frames(struct pos_t *pos, const double *oe, const double *se);
+int shp_set_radar_law(struct mod_t *mod, int l, int type, const double *p);
 double *shp_param_ptr(shp_scene *sc, int i);
 int shp_param_count(shp_scene *sc);
 int shp_param_type(shp_scene *sc, int i);

@@ -24,6 +24,9 @@ LIBDIR = os.path.join(HERE, "lib")
 GLOBAL, LOCAL = 1, 2
 SHORT, LONG_ORIG, LONG_MOD = 0, 1, 2
 DELAY, DOPPLER, POS, LGHTCRV = 0, 1, 2, 3
+# radar scattering law types (shape2.h's numbering)
+(COSINELAW_DIFF, GAUSSIANLAW, HAGFORSLAW, COSINELAW_QS, GAUSSIAN_COSINE, HAGFORS_COSINE,
+ COSINE_COSINE) = 1, 2, 3, 4, 5, 6, 7
 SHAPE_ELLIPSOID_TESS, SHAPE_ICOSPHERE = 0, 1
 
 
@@ -94,6 +97,9 @@ def load_scene():
             "shp_frame_consts": (None, [vp, C.c_int, C.c_int, dp]),
             "shp_par_set": (None, [vp, C.c_int, C.c_double]),
             "shp_mod_photo": (vp, [vp]),
+            "shp_set_radar_law": (C.c_int, [vp, C.c_int, C.c_int, dp]),
+            "shp_pos_set_array": (None, [vp, C.c_int, dp]),
+            "shp_pos_set_frames": (None, [vp, dp, dp]),
             "shp_param_ptr": (dp, [vp, C.c_int]),
             "shp_param_count": (C.c_int, [vp]),
             "shp_param_type": (C.c_int, [vp, C.c_int]),
@@ -358,6 +364,16 @@ class Scene:
         self.lib.shp_pos_array(pos, which, out.ctypes.data_as(C.POINTER(C.c_double)))
         return out
 
+    def pos_set_array(self, pos, which, a):
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        self.lib.shp_pos_set_array(pos, which, a.ctypes.data_as(C.POINTER(C.c_double)))
+
+    def pos_set_frames(self, pos, oe, se):
+        oe = np.ascontiguousarray(oe, dtype=np.float64)
+        se = np.ascontiguousarray(se, dtype=np.float64)
+        self.lib.shp_pos_set_frames(pos, oe.ctypes.data_as(C.POINTER(C.c_double)),
+                                    se.ctypes.data_as(C.POINTER(C.c_double)))
+
     def pos_copy(self, dst, src):
         self.lib.shp_pos_copy(dst, src)
 
@@ -402,6 +418,15 @@ class Scene:
     @property
     def photo(self):
         return self.lib.shp_mod_photo(self.mod)
+
+    def set_radar_law(self, l, law_type, p):
+        """Make radar law l a homogeneous law of type 1-7 (radlaw.c:51-117):
+        p = [R, C] (cosine), [R, C, cos_cutoff] (quasispecular) or
+        [diffuse R, C, quasispecular R, C, cos_cutoff] (hybrids)."""
+        a = np.zeros(6)
+        a[:len(p)] = p
+        if self.lib.shp_set_radar_law(self.mod, l, law_type, a.ctypes.data_as(C.POINTER(C.c_double))):
+            raise ValueError("radar law %d: type %d not settable" % (l, law_type))
 
     def par_flags(self):
         out = np.zeros(6)

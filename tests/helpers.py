@@ -102,6 +102,50 @@ def scene_pair(config, **over):
     return a, b
 
 
+def set_radar_laws(sc, law_type, p):
+    """Give every radar law of the scene (one per radar set) type law_type."""
+    for l in range(sc.nsets):
+        try:
+            sc.set_radar_law(l, law_type, p)
+        except ValueError:
+            break
+
+
+def posmask_plane_case(sc, pos, tol, seed, theta=0.37, a=0.37, b=-0.21, c0=3.0):
+    """A posmask input whose answer a plane fixes.  The source's z-buffer is
+    the plane zill = a i + b j + c0, blank on a lattice of isolated pixels
+    (i and j multiples of 3), so every valid pixel keeps a valid neighbour on
+    each axis; the source frame is the observer's turned by theta about the
+    line of sight, so each POS pixel lands at a fractional source position
+    (i0, j0).  Bilinear interpolation and its one-sided fallback beside blank
+    pixels (posmask.c:112-166) both reproduce a plane exactly, so the pixel
+    is shadowed iff plane(i0, j0) - z > tol.  z is set 1e-7 above or below
+    plane - tol at random.  Returns (expected cose, eligible, stencils with
+    a blank corner among the eligible pixels)."""
+    n = sc.spec.pos_n
+    I, J = np.meshgrid(np.arange(-n, n + 1), np.arange(-n, n + 1), indexing="ij")
+    blank = (I % 3 == 0) & (J % 3 == 0)
+    zill = np.where(blank, -1.0e20, a * I + b * J + c0)
+    c, s = np.cos(theta), np.sin(theta)
+    sc.pos_set_frames(pos, np.eye(3), np.array([[c, s, 0.0], [-s, c, 0.0], [0.0, 0.0, 1.0]]))
+    i0, j0 = c * I + s * J, -s * I + c * J
+    sign = np.random.default_rng(seed).choice([-1.0, 1.0], size=I.shape)
+    z = a * i0 + b * j0 + c0 - tol - sign * 1e-7
+    cose = np.full(I.shape, 0.5)
+    for which, arr in ((1, cose), (3, z), (5, zill), (10, np.ones_like(z)), (13, np.full(I.shape, 2.0)),
+                       (11, np.zeros_like(z)), (12, np.zeros_like(z))):
+        sc.pos_set_array(pos, which, arr)
+    inside = (np.abs(i0) < n) & (np.abs(j0) < n)
+    im = np.clip(np.floor(i0 + 0.5).astype(int), -n, n)
+    jm = np.clip(np.floor(j0 + 0.5).astype(int), -n, n)
+    eligible = inside & ~blank[im + n, jm + n]
+    i1 = np.clip(np.floor(i0).astype(int), -n, n - 1)
+    j1 = np.clip(np.floor(j0).astype(int), -n, n - 1)
+    kludge = eligible & (blank[i1 + n, j1 + n] | blank[i1 + 1 + n, j1 + n] | blank[i1 + n, j1 + 1 + n]
+                         | blank[i1 + 1 + n, j1 + 1 + n])
+    return np.where(eligible & (sign > 0), 0.0, cose), eligible, int(np.sum(kludge))
+
+
 def rel_err(a, b):
     a, b = np.asarray(a, dtype=float), np.asarray(b, dtype=float)
     den = max(np.max(np.abs(b)), 1e-300)

@@ -45,9 +45,11 @@ def _trials(sc):
     return ntr
 
 
-def _check(config, method, over, tol_chi, tol_img, flags=sh.BATCH_KEEP_FIT):
+def _check(config, method, over, tol_chi, tol_img, flags=sh.BATCH_KEEP_FIT, law=None):
     orc = H.load_oracle()
     sc = H.make_scene_with_obs(config, **over)
+    if law is not None:
+        H.set_radar_laws(sc, *law)
     ntr = _trials(sc)
     B = sh.Batch([sc], [(0, t) for t in ntr], method=method, flags=flags)
     obj, chi = B.run()
@@ -89,6 +91,19 @@ def test_fp64_batch_matches_oracle(config, over):
                                                   lc_ncalc=10))])
 def test_mixed_batch_matches_oracle(config, over):
     _check(config, sh.METHOD_MIXED, over, 1e-6, 1e-4)
+
+
+OTHER_LAWS = [(sh.HAGFORS_COSINE, [0.1, 1.2, 0.25, 40.0, 0.3]), (sh.GAUSSIAN_COSINE, [0.1, 1.2, 0.3, 6.0, 0.4]),
+              (sh.COSINELAW_QS, [0.2, 3.0, 0.5])]
+
+
+@pytest.mark.parametrize("method,tol_chi,tol_img", [(sh.METHOD_FP64, 1e-10, 1e-12), (sh.METHOD_MIXED, 1e-6, 1e-4)])
+@pytest.mark.parametrize("law", range(len(OTHER_LAWS)))
+def test_batch_other_radar_laws(method, tol_chi, tol_img, law):
+    """Delay-Doppler and Doppler frames under quasispecular and hybrid laws
+    (cutoffs inside the disk; the trials move the laws' R and C)."""
+    _check(1, method, dict(n_deldop=2, n_doppler=2, nbin_min=30, nbin_max=40), tol_chi, tol_img,
+           law=OTHER_LAWS[law])
 
 
 @pytest.mark.parametrize("method", [sh.METHOD_FP64, sh.METHOD_MIXED])

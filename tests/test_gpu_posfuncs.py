@@ -130,14 +130,45 @@ def test_bistatic_posvis_posmask_apply_photo():
         assert np.array_equal(a.pos_array(pa, 0), b.pos_array(pb, 0))
 
 
+def test_posmask_plane_case_matches():
+    """posmask on a constructed input whose stencils often hold blank sky
+    (the one-sided interpolation): the GPU's mask equals the oracle's and
+    the plane's answer."""
+    orc, hip = H.load_oracle(), H.load_hip()
+    a, b = H.sh.Scene(H.sh.spec_for(1)), H.sh.Scene(H.sh.spec_for(1))
+    pa, pb = a.pos(0, 0), b.pos(0, 0)
+    for tol, seed in ((0.0, 3), (0.05, 4)):
+        want, _, nk = H.posmask_plane_case(a, pa, tol, seed)
+        H.posmask_plane_case(b, pb, tol, seed)
+        orc.oracle_posmask(pa, tol)
+        hip.posmask(pb, tol)
+        assert hip.shape_hip_last_error() == 0
+        assert np.array_equal(a.pos_array(pa, 1), want)
+        assert np.array_equal(b.pos_array(pb, 1), want)
+
+
+RADAR_LAWS = [(H.sh.GAUSSIANLAW, [0.3, 6.0, 0.4]), (H.sh.HAGFORSLAW, [0.25, 40.0, 0.3]),
+              (H.sh.COSINELAW_QS, [0.2, 3.0, 0.5]), (H.sh.GAUSSIAN_COSINE, [0.1, 1.2, 0.3, 6.0, 0.4]),
+              (H.sh.HAGFORS_COSINE, [0.1, 1.2, 0.25, 40.0, 0.3]), (H.sh.COSINE_COSINE, [0.1, 1.2, 0.2, 3.0, 0.5])]
+
+
 def test_radlaw_matches():
     orc, hip, a, b = pair(2, n_deldop=4)
-    for ilaw in range(4):
-        for c in (1e-3, 0.2, 0.5, 0.9, 1.0):
+    cs = (1e-3, 0.2, 0.3, 0.4, 0.5, 0.9, 1.0)
+    for ilaw in range(4):   # the scenes' cosine laws
+        for c in cs:
             va = orc.oracle_radlaw(a.photo, ilaw, c, 0, 0)
             vb = hip.radlaw(b.photo, ilaw, c, 0, 0)
             assert hip.shape_hip_last_error() == 0
             assert abs(va - vb) <= 1e-14 * abs(va), (ilaw, c, va, vb)
+    for t, p in RADAR_LAWS:   # the other homogeneous laws, cutoffs included
+        a.set_radar_law(0, t, p)
+        b.set_radar_law(0, t, p)
+        for c in cs:
+            va = orc.oracle_radlaw(a.photo, 0, c, 0, 0)
+            vb = hip.radlaw(b.photo, 0, c, 0, 0)
+            assert hip.shape_hip_last_error() == 0
+            assert abs(va - vb) <= 1e-14 * abs(va), (t, c, va, vb)
 
 
 SCALARS = ["overflow_o2", "overflow_m2", "badradar_logfactor", "idellim0", "idellim1", "idoplim0", "idoplim1",
@@ -169,6 +200,25 @@ def radar_case(config, over, kind, offsets):
 def test_pos2deldop_matches():
     radar_case(2, dict(n_deldop=4), H.sh.DELAY, (0.4, -0.3, 0.25))
     radar_case(1, dict(codemethod=0), H.sh.DELAY, (0.0, 0.0, 0.0))
+
+
+@pytest.mark.parametrize("law", range(len(RADAR_LAWS)))
+def test_pos2deldop_matches_other_laws(law):
+    """pos2deldop with each quasispecular and hybrid law (cutoffs inside the
+    disk) against the oracle."""
+    t, p = RADAR_LAWS[law]
+    orc, hip, a, b = pair(1, n_deldop=2)
+    s = next(q for q in range(a.nsets) if a.set_type(q) == H.sh.DELAY)
+    for sc in (a, b):
+        H.set_radar_laws(sc, t, p)
+        sc.set_image(s, 0, 0, np.zeros_like(sc.image(s, 0, 0)))
+    ra = orc.oracle_pos2deldop(a.par, a.photo, 0.1, -0.2, 0.3, a.set_desc(s), 0, s, 0, 0)
+    rb = hip.pos2deldop(b.par, b.photo, 0.1, -0.2, 0.3, b.set_desc(s), 0, s, 0, 0)
+    assert hip.shape_hip_last_error() == 0
+    assert ra == rb
+    fa, fb = a.image(s, 0, 0), b.image(s, 0, 0)
+    assert np.max(fa) > 0.0
+    assert np.max(np.abs(fa - fb)) <= 1e-12 * np.max(np.abs(fa))
 
 
 def test_pos2doppler_matches_with_overflow():

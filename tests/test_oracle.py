@@ -286,6 +286,64 @@ def test_radlaw_cosine_closed_form(orc):
         assert got == pytest.approx(R * (Cc + 1) * c ** (2 * Cc - 1), rel=1e-15)
 
 
+def _law(orc, sc, law_type, p):
+    sc.set_radar_law(0, law_type, p)
+    return lambda mu: orc.oracle_radlaw(sc.photo, 0, mu, 0, 0)
+
+
+def _quad(g, a, b):
+    from scipy.integrate import quad
+    return quad(g, a, b, epsabs=0.0, epsrel=1e-13, limit=200)[0]
+
+
+def test_radlaw_weighted_integrals(orc):
+    """Each law against an exact weighted integral over mu = cos(incidence),
+    from the cutoff mu_c to 1, evaluated by quadrature of the oracle's values:
+      cosine R(C+1)mu^(2C-1):     int 2mu law      = 2R(C+1)(1 - mu_c^(2C+1))/(2C+1)
+                                  (a sphere's cross-section over pi a^2 at mu_c = 0)
+      Gaussian RC e^(-C tan^2)/mu^5:  int 2mu^2 law = R(1 - e^(-C tan^2(theta_c)))
+                                  (2 mu^2 law = R d/dmu e^(-C(1/mu^2 - 1)))
+      Hagfors RC/2 q^(-3/2)/mu, q = mu^4 + C(1-mu^2):
+                                  int 2mu(C - mu^4) law = RC(1 - mu_c/sqrt(q(mu_c)))
+                                  (d/dmu [mu/sqrt(q)] = (C - mu^4) q^(-3/2))
+    A wrong exponent, factor or term in any law moves its integral."""
+    sc = sh.Scene(sh.spec_for(1))
+    R, mc = 0.23, 0.35
+    for Cc in (0.8, 1.7, 3.0):
+        law = _law(orc, sc, sh.COSINELAW_DIFF, [R, Cc])
+        assert _quad(lambda m: 2 * m * law(m), 0.0, 1.0) == pytest.approx(2 * R * (Cc + 1) / (2 * Cc + 1), rel=1e-10)
+        law = _law(orc, sc, sh.COSINELAW_QS, [R, Cc, mc])
+        want = 2 * R * (Cc + 1) * (1 - mc ** (2 * Cc + 1)) / (2 * Cc + 1)
+        assert _quad(lambda m: 2 * m * law(m), mc, 1.0) == pytest.approx(want, rel=1e-10)
+    for Cc in (0.5, 8.0, 40.0):
+        law = _law(orc, sc, sh.GAUSSIANLAW, [R, Cc, mc])
+        want = R * (1 - math.exp(-Cc * (1 / mc ** 2 - 1)))
+        assert _quad(lambda m: 2 * m * m * law(m), mc, 1.0) == pytest.approx(want, rel=1e-10)
+        law = _law(orc, sc, sh.HAGFORSLAW, [R, Cc, mc])
+        want = R * Cc * (1 - mc / math.sqrt(mc ** 4 + Cc * (1 - mc * mc)))
+        assert _quad(lambda m: 2 * m * (Cc - m ** 4) * law(m), mc, 1.0) == pytest.approx(want, rel=1e-10)
+
+
+def test_radlaw_cutoffs_and_hybrids(orc):
+    """The quasispecular laws switch on at cos >= cutoff (inclusive), and each
+    hybrid is its diffuse cosine law plus its quasispecular law, each with its
+    own parameters (so a swapped parameter set fails)."""
+    sc = sh.Scene(sh.spec_for(1))
+    mc = 0.6
+    for t in (sh.GAUSSIANLAW, sh.HAGFORSLAW, sh.COSINELAW_QS):
+        law = _law(orc, sc, t, [0.3, 2.5, mc])
+        assert law(mc) > 0.0
+        assert law(math.nextafter(mc, 0.0)) == 0.0
+    diff, qs = [0.11, 1.3], [0.27, 6.0, mc]
+    for hyb, part in ((sh.GAUSSIAN_COSINE, sh.GAUSSIANLAW), (sh.HAGFORS_COSINE, sh.HAGFORSLAW),
+                      (sh.COSINE_COSINE, sh.COSINELAW_QS)):
+        for mu in (0.05, 0.3, math.nextafter(mc, 0.0), mc, 0.8, 0.999, 1.0):
+            h = _law(orc, sc, hyb, diff + qs)(mu)
+            d = _law(orc, sc, sh.COSINELAW_DIFF, diff)(mu)
+            q = _law(orc, sc, part, qs)(mu)
+            assert h == pytest.approx(d + q, rel=1e-15, abs=0.0), (hyb, mu)
+
+
 def test_deldop_power_conservation(orc):
     """sum(fit) = kmpp^2 * sum over lit pixels of radlaw(cos) when every footprint
     lies inside the frame (long code: codefactor 1)."""
@@ -450,6 +508,22 @@ def test_posmask_shadows_at_large_phase(orc):
     pos = sc.pos(0, 7)
     cose, cosi, fac = sc.pos_array(pos, 1), sc.pos_array(pos, 2), sc.pos_array(pos, 10)
     assert np.sum((fac >= 0) & (cosi > 0.05) & (cose == 0)) > 0
+
+
+def test_posmask_interpolation_exact_on_a_plane(orc):
+    """Shadowing decided by the interpolated source depth, one-sided beside
+    blank sky included: on a planar source z-buffer both interpolations are
+    exact, so the mask is known pixel by pixel (a nearest-pixel depth, a
+    transposed neighbour or a wrong sign on the fallback side flips pixels
+    whose margin is 1e-7)."""
+    sc = sh.Scene(sh.spec_for(1))
+    pos = sc.pos(0, 0)
+    for tol, seed in ((0.0, 1), (0.05, 2)):
+        want, eligible, nk = H.posmask_plane_case(sc, pos, tol, seed)
+        assert nk > 100 and eligible.sum() > 1000
+        orc.oracle_posmask(pos, tol)
+        got = sc.pos_array(pos, 1)
+        assert np.array_equal(got, want), np.argwhere(got != want)[:5]
 
 
 # ------------------------------------------------------------------ a11 spline
