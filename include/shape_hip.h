@@ -143,6 +143,28 @@ int shape_batch_counters(shape_batch *b, int *out4);
 int shape_batch_sizes(shape_batch *b, long long *out8);
 void shape_batch_destroy(shape_batch *b);
 
+/* ---------------- Part 3: batched line search (replaces mnbrak + brent_abs) --
+   The fit's search over one parameter (bestfit.c:446-468: nr/mnbrak.c:10
+   brackets a minimum from x0 and x0 + step, nr/brent_abs.c:13 refines it to
+   tol * |x| + max(abstol, 1e-10)) with k trial values per round instead of
+   one: the first round spreads x0 + step * {0, +1, -1, +g, -g, ...} (g the
+   golden ratio), rounds continue past an end point that is lowest, and with
+   a bracket (a, x, b) a round evaluates the parabola's vertex u, u -+ tol1
+   and an even grid over (a, b).  Stops on brent_abs's test, so the returned
+   minimum carries the same tolerance.  Returns 0 (converged), 1 (max_rounds
+   reached: *xmin is the lowest point so far), -1 (bad arguments), -2 (an
+   evaluation failed), -3 (shape_batch_line_search: the batch's evaluations
+   do not all set the same single parameter). */
+#define SHAPE_SEARCH_MAX_K 64
+/* evaluate the objective at x[0..k); return 0, or nonzero on failure */
+typedef int (*shape_eval_fn)(void *ctx, int k, const double *x, double *f);
+int shape_line_search(shape_eval_fn eval, void *ctx, int k, double x0, double step, double tol, double abstol,
+                      int max_rounds, double *xmin, double *fmin, int *rounds);
+/* the same, each round one replay of b: b's k evaluations each set the same
+   single parameter (its trial values); its objectives are the f values */
+int shape_batch_line_search(shape_batch *b, double x0, double step, double tol, double abstol, int max_rounds,
+                            double *xmin, double *fmin, int *rounds);
+
 /* measurement: the batch's hipStream_t; per-kernel HIP-event timing of eager
    replays (batches created without SHAPE_BATCH_GRAPH); and the last replay's
    footprint: model-rectangle pixels, POS pixels, occupied image cells, image

@@ -43,6 +43,13 @@ void oracle_realize_spin(struct par_t *par, struct mod_t *mod, struct dat_t *dat
 /* COM offsets: realize_delcor.c:50-151, deldopoffs.c:17-40, dopoffs.c:19-39 */
 void oracle_realize_delcor(struct dat_t *dat, double delta_delcor0, int delcor0_mode);
 
+/* the fit's one-parameter line search: nr/mnbrak.c:10, nr/brent_abs.c:13,
+   bestfit.c:446-468 (mnbrak from (x0, x0 + step), then brent_abs) */
+typedef double (*oracle_fn)(double);
+void oracle_mnbrak(double *a, double *b, double *c, double *fa, double *fb, double *fc, oracle_fn f);
+double oracle_brent_abs(double ax, double bx, double cx, oracle_fn f, double tol, double abstol, double *xmin);
+double oracle_line_search(oracle_fn f, double x0, double step, double tol, double abstol, double *xmin);
+
 /* plane-of-sky rendering: posclr.c:24, posvis.c:84-364, posmask.c:58-174 */
 void oracle_posclr(struct pos_t *pos);
 int oracle_posvis(struct vertices_t *verts, double orbit_offset[3], struct pos_t *pos,

@@ -110,6 +110,27 @@ extern "C" int shape_batch_run(shape_batch *B, double *objective, double *chi2v)
   return rc ? rc : shape_batch_wait(B, objective, chi2v);
 }
 
+// One round of shape_batch_line_search: the k trial values are the batch's values.
+static int batch_search_eval(void *ctx, int k, const double *x, double *f) {
+  shape_batch *B = (shape_batch *)ctx;
+  if (k != B->E.nevals) return 1;
+  int rc = shape_batch_set_values(B, x);
+  return rc ? rc : shape_batch_run(B, f, nullptr);
+}
+
+extern "C" int shape_batch_line_search(shape_batch *B, double x0, double step, double tol, double abstol,
+                                       int max_rounds, double *xmin, double *fmin, int *rounds) {
+  if (!B || B->E.nevals < 1) return -1;
+  const auto &t0 = B->E.eval_targets[0];
+  for (int e = 0; e < B->E.nevals; e++) {
+    const auto &t = B->E.eval_targets[e];
+    if (B->ntarg[e] != 1 || t.size() != 1 || t[0].kind != t0[0].kind || t[0].model != t0[0].model ||
+        t[0].offset != t0[0].offset)
+      return -3;
+  }
+  return shape_line_search(batch_search_eval, B, B->E.nevals, x0, step, tol, abstol, max_rounds, xmin, fmin, rounds);
+}
+
 extern "C" float shape_batch_last_ms(shape_batch *B) { return B->last_ms; }
 
 extern "C" void *shape_batch_stream(shape_batch *B) { return (void *)B->E.stream; }

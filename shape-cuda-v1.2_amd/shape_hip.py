@@ -112,6 +112,9 @@ def load_scene():
     return _scene_lib
 
 
+# shape_eval_fn: evaluate the objective at x[0..k) into f; 0 on success
+SEARCH_FN = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double))
+
 _hip_lib = None
 
 HIP_EXPORTS = ["realize_mod", "realize_spin", "realize_delcor", "calc_fits", "chi2",
@@ -156,6 +159,12 @@ def load_hip():
             "shape_batch_launch": (C.c_int, [vp]),
             "shape_batch_wait": (C.c_int, [vp, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
             "shape_batch_run": (C.c_int, [vp, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
+            "shape_line_search": (C.c_int, [SEARCH_FN, vp, C.c_int, C.c_double, C.c_double, C.c_double,
+                                            C.c_double, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double),
+                                            C.POINTER(C.c_int)]),
+            "shape_batch_line_search": (C.c_int, [vp, C.c_double, C.c_double, C.c_double, C.c_double, C.c_int,
+                                                  C.POINTER(C.c_double), C.POINTER(C.c_double),
+                                                  C.POINTER(C.c_int)]),
             "shape_batch_last_ms": (C.c_float, [vp]),
             "shape_batch_nframes": (C.c_int, [vp, C.c_int]),
             "shape_batch_frame_results": (C.c_int, [vp, C.c_int, C.POINTER(C.c_double),
@@ -467,6 +476,27 @@ METHOD_FP64, METHOD_MIXED = 0, 1
 BATCH_GRAPH, BATCH_KEEP_FIT, BATCH_GLOBAL_ZBUF, BATCH_FACET_MAP = 1, 2, 4, 8
 
 
+def line_search(f, k, x0, step, tol, abstol, max_rounds=40):
+    """shape_line_search over a Python objective f(x) (k trial values per
+    round, evaluated here one by one); returns (status, xmin, fmin, rounds,
+    the x values of every round)."""
+    lib = load_hip()
+    seen = []
+
+    def cb(ctx, kk, x, out):
+        xs = [x[q] for q in range(kk)]
+        seen.append(xs)
+        for q in range(kk):
+            out[q] = f(xs[q])
+        return 0
+
+    fn = SEARCH_FN(cb)
+    xmin, fmin, rounds = C.c_double(), C.c_double(), C.c_int()
+    rc = lib.shape_line_search(fn, None, k, x0, step, tol, abstol, max_rounds, C.byref(xmin), C.byref(fmin),
+                               C.byref(rounds))
+    return rc, xmin.value, fmin.value, rounds.value, seen
+
+
 class Batch:
     """Many objective evaluations in one device-resident replay.
 
@@ -510,6 +540,16 @@ class Batch:
         if rc:
             raise RuntimeError("shape_batch_run failed (%d)" % rc)
         return obj, chi
+
+    def line_search(self, x0, step, tol, abstol, max_rounds=40):
+        """Minimize the objective over the one parameter every evaluation sets
+        (shape_batch_line_search); returns (status, xmin, fmin, rounds)."""
+        xmin, fmin, rounds = C.c_double(), C.c_double(), C.c_int()
+        rc = self.lib.shape_batch_line_search(self.h, x0, step, tol, abstol, max_rounds, C.byref(xmin),
+                                              C.byref(fmin), C.byref(rounds))
+        if rc < 0:
+            raise RuntimeError("shape_batch_line_search failed (%d)" % rc)
+        return rc, xmin.value, fmin.value, rounds.value
 
     def launch(self):
         rc = self.lib.shape_batch_launch(self.h)

@@ -21,6 +21,10 @@ _oracle = None
 _hip = None
 
 
+DP = C.POINTER(C.c_double)
+ORACLE_FN = C.CFUNCTYPE(C.c_double, C.c_double)
+
+
 def load_oracle():
     global _oracle
     if _oracle is None:
@@ -50,6 +54,10 @@ def load_oracle():
             "oracle_calc_fits_threads": (None, [VP, VP, VP, C.c_int]),
             "oracle_chi2": (C.c_double, [VP, VP, C.c_int]),
             "oracle_objective_tail": (C.c_double, [VP, VP]),
+            "oracle_mnbrak": (None, [DP, DP, DP, DP, DP, DP, ORACLE_FN]),
+            "oracle_brent_abs": (C.c_double, [C.c_double, C.c_double, C.c_double, ORACLE_FN, C.c_double,
+                                              C.c_double, DP]),
+            "oracle_line_search": (C.c_double, [ORACLE_FN, C.c_double, C.c_double, C.c_double, C.c_double, DP]),
             "oracle_last_error": (C.c_char_p, []),
             "oracle_clear_error": (None, []),
         }

@@ -33,6 +33,7 @@ def declared_functions(path):
     text = open(path).read()
     text = re.sub(r"/\*.*?\*/", " ", text, flags=re.S)
     text = re.sub(r"^\s*#.*$", " ", text, flags=re.M)
+    text = re.sub(r"\btypedef\b[^;]*;", " ", text)  # (function-pointer types are not functions)
     names = re.findall(r"\b(\w+)\s*\([^;{}]*\)\s*;", text)
     return sorted(set(n for n in names if n not in ("if", "while", "sizeof")))
 

@@ -106,6 +106,38 @@ def test_batch_other_radar_laws(method, tol_chi, tol_img, law):
            law=OTHER_LAWS[law])
 
 
+@pytest.mark.parametrize("method,tol_f", [(sh.METHOD_FP64, 1e-10), (sh.METHOD_MIXED, 1e-6)])
+def test_batch_line_search_matches_oracle(method, tol_f):
+    """shape_batch_line_search over the size scale, 8 trial values per
+    replay, against the oracle's mnbrak + brent_abs on the oracle objective
+    (bestfit.c:446-468): the same minimum to brent_abs's tolerance, and the
+    batch's objective there equal to the oracle's to the method's contract."""
+    import ctypes as C
+    orc = H.load_oracle()
+    sc = H.make_scene_with_obs(1)
+    x0, step, tol, abstol = 1.02, 0.01, 1e-4, 1e-9
+    B = sh.Batch([sc], [(0, [(0, x0)]) for _ in range(8)], method=method)
+    rc, xb, fb, rounds = B.line_search(x0, step, tol, abstol)
+    assert rc == 0
+    saved, calls = sc.get_param(0), [0]
+
+    def f(x):
+        calls[0] += 1
+        sc.set_param(0, x)
+        H.oracle_forward(orc, sc)
+        return orc.oracle_objective_tail(sc.par, sc.dat)
+    xo = C.c_double()
+    fo = orc.oracle_line_search(H.ORACLE_FN(f), x0, step, tol, abstol, C.byref(xo))
+    fo_at_xb = f(xb)
+    sc.set_param(0, saved)
+    print("method", method, "batched: x", xb, "f", fb, "rounds", rounds, "| oracle: x", xo.value, "f", fo,
+          "evaluations", calls[0] - 1)
+    t1 = tol * abs(xo.value) + abstol
+    assert abs(xb - xo.value) <= 4 * t1
+    assert abs(fb - fo_at_xb) <= tol_f * abs(fo_at_xb)
+    assert rounds < calls[0] - 1
+
+
 @pytest.mark.parametrize("method", [sh.METHOD_FP64, sh.METHOD_MIXED])
 def test_graph_replays_match_eager(method):
     """Replays of the captured graph, with new trial values between them and
