@@ -63,6 +63,9 @@ def parse():
                          "mixed and FP64 rates, latency, accuracy, facet-map mismatch, CPU baselines")
     ap.add_argument("--mixed-only", action="store_true",
                     help="time only the mixed replays (for profiler runs)")
+    ap.add_argument("--line-search", type=int, default=0, metavar="K",
+                    help="time the batched line search (K trial values per replay, mixed method) over the "
+                         "first scale factor of the large-model workload, and replays of K evaluations")
     return ap.parse_args()
 
 
@@ -402,6 +405,27 @@ def config_report(config, a, orc):
     return out
 
 
+def line_search_report(a):
+    """Wall time of shape_batch_line_search (bestfit.c:446-468's search for
+    one parameter) on the large-model workload: k evaluations of the first
+    scale factor per replay, from 1.02 x its value, step 0.01, tol 1e-4."""
+    k = a.line_search
+    sc = make_scene(a.frames)
+    p0 = sc.get_param(0)
+    B = sh.Batch([sc], [(0, [(0, p0)]) for _ in range(k)], method=sh.METHOD_MIXED, flags=sh.BATCH_GRAPH)
+    wall, lat = time_batch(B, a.steps, a.warmup)
+    out = {"metric": "batched line search wall time", "unit": "ms", "k": k, "frames": a.frames,
+           "replay_ms_p50": round(float(np.median(lat)), 3), "replays_timed": a.steps}
+    searches = []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        rc, xmin, fmin, rounds = B.line_search(1.02 * p0, 0.01 * p0, 1e-4, 1e-9)
+        searches.append((time.perf_counter() - t0) * 1e3)
+    out.update({"status": rc, "xmin": xmin, "fmin": fmin, "rounds": rounds,
+                "search_ms": round(float(np.median(searches)), 2), "start": 1.02 * p0, "truth": p0})
+    return out
+
+
 def main():
     global KSTEPS
     a = parse()
@@ -417,6 +441,9 @@ def main():
     hip = sh.load_hip()
     if hip.shape_hip_device_count() < 1:
         raise RuntimeError("no HIP device")
+    if a.line_search:
+        print(json.dumps(line_search_report(a)), flush=True)
+        return
     if a.configs:
         orc = load_oracle()
         for c in [int(x) for x in a.configs.split(",")]:
