@@ -29,6 +29,24 @@ struct shape_batch {
 extern "C" shape_batch *shape_batch_create(struct par_t *par, struct mod_t **mods, struct dat_t **dats, int nmodels,
                                            int nevals, const int *eval_model, const int *n_targets,
                                            double *const *targets, const double *values, int method, int flags) {
+  // arguments checked before any device call
+  const char *bad = nullptr;
+  long long ntot = 0;
+  if (!par || !mods || !dats || nmodels < 1) bad = "no models";
+  else if (nevals < 1 || !eval_model || !n_targets) bad = "no evaluations";
+  else if (method != SHAPE_METHOD_FP64 && method != SHAPE_METHOD_MIXED) bad = "unknown method";
+  for (int e = 0; !bad && e < nevals; e++) {
+    if (eval_model[e] < 0 || eval_model[e] >= nmodels) bad = "evaluation's model index out of range";
+    else if (n_targets[e] < 0) bad = "negative target count";
+    else ntot += n_targets[e];
+  }
+  if (!bad && ntot > 0 && (!targets || !values)) bad = "targets or values missing";
+  for (long long q = 0; !bad && q < ntot; q++)
+    if (!targets[q]) bad = "null target address";
+  if (bad) {
+    fprintf(stderr, "shape_batch_create: %s\n", bad);
+    return nullptr;
+  }
   shape_batch *B = new shape_batch();
   std::vector<EvalSpec> ev(nevals);
   int q = 0;
@@ -41,7 +59,7 @@ extern "C" shape_batch *shape_batch_create(struct par_t *par, struct mod_t **mod
     }
   }
   RunOptions o;
-  o.method = method == 1 ? METHOD_MIXED : METHOD_FP64;
+  o.method = method == SHAPE_METHOD_MIXED ? METHOD_MIXED : METHOD_FP64;
   o.keep_fit = (flags & SHAPE_BATCH_KEEP_FIT) ? 1 : 0;
   o.global_zbuf = (flags & SHAPE_BATCH_GLOBAL_ZBUF) ? 1 : 0;
   o.facet_map = (flags & SHAPE_BATCH_FACET_MAP) ? 1 : 0;

@@ -17,6 +17,8 @@ import pytest
 import abi_parse
 import helpers as H
 
+sh = H.sh
+
 ROOT = H.ROOT
 HDR = os.path.join(ROOT, "include", "shape_hip.h")
 ABI = os.path.join(ROOT, "include", "shape_abi.h")
@@ -58,6 +60,28 @@ def test_no_compute_without_device_is_loud():
 
 
 @pytest.mark.skipif(not os.path.exists(REF_SHAPE2), reason="reference tree not present")
+def test_batch_create_rejects_bad_arguments():
+    """Bad batch arguments are refused before any device call (NULL)."""
+    lib = sh.load_hip()
+    sc = sh.Scene(sh.spec_for(1))
+    mods = (C.c_void_p * 1)(sc.mod)
+    dats = (C.c_void_p * 1)(sc.dat)
+    addr = C.cast(sc.lib.shp_param_ptr(sc.h, 0), C.c_void_p).value
+    tg = (C.c_void_p * 1)(addr)
+    vals = (C.c_double * 1)(1.0)
+
+    def create(nmodels, nevals, model, ntarg, targets=tg, method=sh.METHOD_FP64):
+        em = (C.c_int * max(1, nevals))(*([model] * max(1, nevals)))
+        nt = (C.c_int * max(1, nevals))(*([ntarg] * max(1, nevals)))
+        return lib.shape_batch_create(sc.par, mods, dats, nmodels, nevals, em, nt, targets, vals, method, 0)
+    assert not create(1, 0, 0, 1)                    # no evaluations
+    assert not create(0, 1, 0, 1)                    # no models
+    assert not create(1, 1, 1, 1)                    # model index out of range
+    assert not create(1, 1, 0, -1)                   # negative target count
+    assert not create(1, 1, 0, 1, method=7)          # unknown method
+    assert not create(1, 1, 0, 1, targets=(C.c_void_p * 1)(None))  # null target
+
+
 def test_struct_members_match_reference():
     ours = abi_parse.parse_structs(open(ABI).read())
     # const.h supplies the array-size macros shape2.h uses (MAXOVERFLOW, ...)
