@@ -524,9 +524,12 @@ int Engine::build(struct par_t *par, struct mod_t **mods, struct dat_t **dats, i
       V.binned = 0; V.bin_ntx = 0; V.bin_off = 0;
       if (opt.method == METHOD_MIXED && V.type != VT_LGHTCRV) {
         // binned tile kernel unless the POS is wider than its tile grid, the
-        // facet index does not fit an event, or the global z-buffer is asked for
+        // facet index does not fit an event, the global z-buffer is asked for,
+        // or the view's radar law is not the diffuse cosine law (the tile
+        // kernel evaluates that law only; the global path has the others)
         int ntx = (2 * V.n + 1 + BT - 1) / BT;
-        if (!opt.global_zbuf && ntx <= BIN_MAX_NTX && P.nf <= (1 << EV_FACET_BITS)) {
+        if (!opt.global_zbuf && ntx <= BIN_MAX_NTX && P.nf <= (1 << EV_FACET_BITS) &&
+            V.law_type == COSINELAW_DIFF) {
           V.binned = 1; V.bin_ntx = ntx; V.bin_off = bin_cursor;
           bin_cursor += (long long)ntx * ntx;
           max_bin_ntx = std::max(max_bin_ntx, ntx);

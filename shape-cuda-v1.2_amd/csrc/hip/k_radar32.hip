@@ -112,15 +112,15 @@ __global__ __launch_bounds__(256) void k_shade_radar32(ViewRec *__restrict__ vie
     // cos(scattering) of an owner facing away is inherited from an earlier
     // claimant, or (NaN law) too close to a quasispecular law's cutoff for
     // FP32: FP64 replay (records reserved once per wave)
-    const float law = nz > 0.0f ? radlaw32(V, G, pl, f, s, t, nz) : 0.0f;
-    const bool rep = !(nz > 0.0f) || law != law;
+    const float cl = nz > 0.0f ? radar_cl(V, G, pl, f, s, t, nz) : 0.0f;
+    const bool rep = !(nz > 0.0f) || cl != cl;
     int si = stale_slot(pd, rep);
     if (rep) {
       if (si >= 0) stale_init(V, pd, si, blockIdx.y, (i + n) * w + (j + n), i, j);
       continue;
     }
     RadarPix R;
-    radar_pixel(V, i, j, (double)key_z(k), law, R);
+    radar_pixel(V, pl, i, j, (double)key_z(k), cl, R);
     if (!R.inb) anyo = 1;
     if (R.wide) { atomicAdd(&counters[3], 1); continue; }
     if (use_lds && R.inb) {

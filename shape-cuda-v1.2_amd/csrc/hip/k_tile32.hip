@@ -536,16 +536,14 @@ __global__ __launch_bounds__(256, 6) void k_tile32(const ViewRec *__restrict__ v
     float s, t;
     bary32(P, i, j, s, t);  // the raster's coordinates, to FP32 rounding
     const float nz = normal32f_los(G, f, V.oa32, V.smooth, s, t);
-    // (NaN: cos(scattering) too close to a quasispecular law's cutoff)
-    const float law = nz > 0.0f ? radlaw32(V, G, pl, f, s, t, nz) : 0.0f;
-    if (!(nz > 0.0f) || law != law) {
+    if (!(nz > 0.0f)) {
       // cos(scattering) of an owner facing away is inherited from an earlier
-      // claimant, or the cutoff test needs FP64: FP64 replay
+      // claimant: FP64 replay
       tile_stale(V, pd, ev_ok, a * BT + c, i, j, flag, spix, &nts);
       continue;
     }
     RadarPix R;
-    radar_pixel(V, i, j, (double)key_z(k), law, R);
+    radar_pixel(V, pl, i, j, (double)key_z(k), nz, R);  // (binned views: diffuse cosine law)
     if (!R.inb) anyo = 1;
     if (R.wide) { atomicAdd(&counters[3], 1); continue; }
     if (use_lds && R.inb) {

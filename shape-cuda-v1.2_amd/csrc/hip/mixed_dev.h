@@ -389,11 +389,11 @@ __device__ __attribute__((noinline)) float radlaw32_steep(const ViewRec &V, cons
   return d;
 }
 
-// A lit pixel's radar law value from its FP32 cos(scattering); NaN asks for
-// the FP64 replay (radlaw32_steep).
-__device__ inline float radlaw32(const ViewRec &V, const GeomSlot &G, const double *pl, int f, float s, float t,
+// radar_pixel's cl for a lit pixel with FP32 cos(scattering) c: c itself
+// under the diffuse cosine law, else the law's value (NaN: FP64 replay).
+__device__ inline float radar_cl(const ViewRec &V, const GeomSlot &G, const double *pl, int f, float s, float t,
                                  float c) {
-  if (V.law_type == COSINELAW_DIFF) return cosine_law32(pl, c);
+  if (V.law_type == COSINELAW_DIFF) return c;
   return radlaw32_steep(V, G, pl, f, s, t, c);
 }
 
@@ -446,7 +446,10 @@ __device__ inline float del_w(const ViewRec &V, const RadarPix &R, int a) {
   return delay_w32(V, R.dfr + (float)(R.dm0 - a));
 }
 
-__device__ inline void radar_pixel(const ViewRec &V, int i, int j, double z, float law, RadarPix &R) {
+// cl: the pixel's FP32 cos(scattering) in a view with the diffuse cosine law
+// (the law is evaluated here, last), or the law's value itself otherwise.
+__device__ inline void radar_pixel(const ViewRec &V, const double *pl, int i, int j, double z, float cl,
+                                   RadarPix &R) {
   // (FP64 bin coordinates in the CPU's order of operations: no contraction)
 #pragma clang fp contract(off)
   const bool dd = V.type == VT_DELAY;
@@ -513,7 +516,8 @@ __device__ inline void radar_pixel(const ViewRec &V, int i, int j, double z, flo
     R.inb = (V.P0 + R.ip0 >= 1) && (V.P0 + R.ip1 <= V.ndop);
   }
   R.amp = R.wide ? 0.0f
-                  : law * V.kmpp32 * V.kmpp32 * codef * __builtin_amdgcn_rcpf(sumd * sump);
+                  : (V.law_type == COSINELAW_DIFF ? cosine_law32(pl, cl) : cl) * V.kmpp32 * V.kmpp32 * codef *
+                        __builtin_amdgcn_rcpf(sumd * sump);
 }
 
 // scatter one pixel straight into the HBM fit image (FP64 atomics) or, for

@@ -66,8 +66,8 @@ __device__ inline void replay_emit(ViewRec &V, const double *pl, int i, int j, d
   RadarPix R;
   // the other laws in FP64 from the CPU's cosine: their cutoff test and
   // steep quasispecular terms need it
-  const float law = V.law_type == COSINELAW_DIFF ? cosine_law32(pl, (float)cosa) : radlaw64_other(pl, V.law_type, cosa);
-  radar_pixel(V, i, j, zz, law, R);
+  const float cl = V.law_type == COSINELAW_DIFF ? (float)cosa : radlaw64_other(pl, V.law_type, cosa);
+  radar_pixel(V, pl, i, j, zz, cl, R);
   radar_scatter_global(V, R, img, ov);
   const bool dd = V.type == VT_DELAY;
   if (dd) {
