@@ -1047,7 +1047,9 @@ int Engine::writeback(int e, struct par_t *par, struct mod_t *mod, struct dat_t 
   if (write_pos && pd.z) {
     for (int v = 0; v < nviews; v++) {
       const ViewRec &V = vw[v];
-      if (V.eval != e) continue;
+      // a smeared frame's POS holds its view v0, which the CPU's view loop
+      // (v0 + 1, ..., v0 + nviews mod nviews) renders last
+      if (V.eval != e || !V.is_v0) continue;
       struct set_t *set = &dat->set[V.set];
       struct pos_t *pos = V.type == VT_DELAY ? &set->desc.deldop.frame[V.frm].pos
                           : V.type == VT_DOPPLER ? &set->desc.doppler.frame[V.frm].pos

@@ -1032,7 +1032,8 @@ __global__ void k_chi2_frames64(FrameRec *frames, int nframes, double *fit, cons
   double m2s = 0, m2c = 0, oms = 0, omc = 0;
   for (long long q = threadIdx.x; q < N; q += blockDim.x) {
     double fv = ft[q];
-    if (F.nviews > 1 && F.type != VT_LGHTCRV) { fv = fv * inv_nv; if (!clear_fit) ft[q] = fv; }
+    // (the buffer keeps the sum over a smeared frame's views: readers divide)
+    if (F.nviews > 1 && F.type != VT_LGHTCRV) fv = fv * inv_nv;
     if (clear_fit) ft[q] = 0.0;
     neumaier(m2s, m2c, fv * fv * w[q]);
     neumaier(oms, omc, fv * ob[q] * w[q]);

@@ -337,6 +337,8 @@ void shp_spec_defaults(shp_spec *s)
   s->delcor_a1 = 0.0;
   s->obs_lat_min_deg = 10.0;
   s->obs_lat_max_deg = 10.0;
+  s->radar_nviews = 1;
+  s->view_interval_days = 0.0;
 }
 
 /* Euler-angle matrix with the reference's convention (util/euler.c). */
@@ -588,8 +590,9 @@ static void build_data(shp_scene *sc)
     dd->dopscale.state = 'c';
     dd->dopscale.val = 1.0;
     dd->dopscale_save = 1.0;
-    dd->nviews = 1;
-    dd->v0 = 0;
+    dd->nviews = sp->radar_nviews > 1 ? sp->radar_nviews : 1;
+    dd->v0 = dd->nviews / 2;
+    dd->view_interval = sp->view_interval_days;
     dd->nframes = nfr;
     dd->frame = (struct deldopfrm_t *)tcalloc(sc, nfr, sizeof(struct deldopfrm_t));
     double set_dof = 0;
@@ -625,11 +628,13 @@ static void build_data(shp_scene *sc)
       set_dof += fr->dof;
       snprintf(fr->name, sizeof(fr->name), "dd_%03d_%03d", s, f);
       frame_pos(sc, &fr->pos, 0);
-      fr->view = (struct deldopview_t *)tcalloc(sc, 1, sizeof(struct deldopview_t));
-      fr->view[0].t = t;
-      double dir[3];
-      los_dir(sp, sp->epoch_span_days > 0 ? (t - sp->t0) / sp->epoch_span_days : 0.0, dir);
-      frame_from_dir(fr->view[0].oe, dir);
+      fr->view = (struct deldopview_t *)tcalloc(sc, dd->nviews, sizeof(struct deldopview_t));
+      for (int v = 0; v < dd->nviews; v++) {  /* smeared views around the frame epoch */
+        double tv = t + (v - dd->v0) * dd->view_interval, dir[3];
+        fr->view[v].t = tv;
+        los_dir(sp, sp->epoch_span_days > 0 ? (tv - sp->t0) / sp->epoch_span_days : 0.0, dir);
+        frame_from_dir(fr->view[v].oe, dir);
+      }
     }
     dd->dof = set_dof;
     set->dof = set_dof;
@@ -654,8 +659,9 @@ static void build_data(shp_scene *sc)
     dp->dopscale.state = 'c';
     dp->dopscale.val = 1.0;
     dp->dopscale_save = 1.0;
-    dp->nviews = 1;
-    dp->v0 = 0;
+    dp->nviews = sp->radar_nviews > 1 ? sp->radar_nviews : 1;
+    dp->v0 = dp->nviews / 2;
+    dp->view_interval = sp->view_interval_days;
     dp->nframes = nfr;
     dp->frame = (struct dopfrm_t *)tcalloc(sc, nfr, sizeof(struct dopfrm_t));
     double set_dof = 0;
@@ -678,11 +684,13 @@ static void build_data(shp_scene *sc)
       set_dof += fr->dof;
       snprintf(fr->name, sizeof(fr->name), "dop_%03d_%03d", s, f);
       frame_pos(sc, &fr->pos, 0);
-      fr->view = (struct dopview_t *)tcalloc(sc, 1, sizeof(struct dopview_t));
-      fr->view[0].t = t;
-      double dir[3];
-      los_dir(sp, sp->epoch_span_days > 0 ? (t - sp->t0) / sp->epoch_span_days : 0.0, dir);
-      frame_from_dir(fr->view[0].oe, dir);
+      fr->view = (struct dopview_t *)tcalloc(sc, dp->nviews, sizeof(struct dopview_t));
+      for (int v = 0; v < dp->nviews; v++) {  /* smeared views around the frame epoch */
+        double tv = t + (v - dp->v0) * dp->view_interval, dir[3];
+        fr->view[v].t = tv;
+        los_dir(sp, sp->epoch_span_days > 0 ? (tv - sp->t0) / sp->epoch_span_days : 0.0, dir);
+        frame_from_dir(fr->view[v].oe, dir);
+      }
     }
     dp->dof = set_dof;
     set->dof = set_dof;

@@ -66,6 +66,9 @@ typedef struct shp_spec {
   double delcor_a0, delcor_a1;
   /* viewing geometry drift: sub-observer latitude range (deg) */
   double obs_lat_min_deg, obs_lat_max_deg;
+  /* smearing: views per radar frame, centred on the frame epoch (v0 = nviews / 2) */
+  int radar_nviews;
+  double view_interval_days;
 } shp_spec;
 
 typedef struct shp_scene shp_scene;

@@ -45,7 +45,7 @@ def _trials(sc):
     return ntr
 
 
-def _check(config, method, over, tol_chi, tol_img, flags=sh.BATCH_KEEP_FIT, law=None):
+def _check(config, method, over, tol_chi, tol_img, flags=sh.BATCH_KEEP_FIT, law=None, tol_frame=None):
     orc = H.load_oracle()
     sc = H.make_scene_with_obs(config, **over)
     if law is not None:
@@ -65,7 +65,7 @@ def _check(config, method, over, tol_chi, tol_img, flags=sh.BATCH_KEEP_FIT, law=
         fchi, _ = B.frame_results(e)
         for q, (c, img) in enumerate(frames):
             if c is not None:
-                assert abs(fchi[q] - c) <= tol_chi * abs(c), (e, q, fchi[q], c)
+                assert abs(fchi[q] - c) <= (tol_frame or tol_chi) * abs(c), (e, q, fchi[q], c)
             if e == 0:
                 got = B.fit(e, q, img.shape)
                 err = np.max(np.abs(got - img)) / max(np.max(np.abs(img)), 1e-300)
@@ -74,10 +74,14 @@ def _check(config, method, over, tol_chi, tol_img, flags=sh.BATCH_KEEP_FIT, law=
     assert worst_img <= tol_img
 
 
+SMEAR = dict(radar_nviews=3, view_interval_days=0.02, n_deldop=2, n_doppler=2, nbin_min=30, nbin_max=40)
+
+
 @pytest.mark.parametrize("config,over", [(1, {}), (2, dict(n_deldop=16)),
                                          (1, dict(n_deldop=0, n_doppler=2, nbin_min=30, nbin_max=40)),
                                          (3, dict(n_deldop=3, n_doppler=3, n_lightcurves=2, lc_points=30,
-                                                  lc_ncalc=10))])
+                                                  lc_ncalc=10)),
+                                         (1, SMEAR)])
 def test_fp64_batch_matches_oracle(config, over):
     _check(config, sh.METHOD_FP64, over, 1e-10, 1e-12)
 
@@ -88,9 +92,15 @@ def test_fp64_batch_matches_oracle(config, over):
                                          (1, dict(pos_n=20, n_deldop=2)),
                                          (5, dict(n_deldop=6)),
                                          (3, dict(n_deldop=3, n_doppler=3, n_lightcurves=2, lc_points=30,
-                                                  lc_ncalc=10))])
+                                                  lc_ncalc=10)),
+                                         (1, SMEAR)])
 def test_mixed_batch_matches_oracle(config, over):
-    _check(config, sh.METHOD_MIXED, over, 1e-6, 1e-4)
+    # Per frame, a chi2 near its noise floor amplifies the mixed method's
+    # relative fit error eps (~1e-7) to ~2 eps sqrt(o2 / chi2): ~4e-6 at SNR
+    # 20.  Smeared Doppler frames at the true model reach 1.7e-6 - 2.5e-6 (as
+    # one unsmeared Doppler frame of SMEAR's scene does), so their frames are
+    # held to that bound; every evaluation's chi2 and objective keep 1e-6.
+    _check(config, sh.METHOD_MIXED, over, 1e-6, 1e-4, tol_frame=1e-5 if over is SMEAR else None)
 
 
 OTHER_LAWS = [(sh.HAGFORS_COSINE, [0.1, 1.2, 0.25, 40.0, 0.3]), (sh.GAUSSIAN_COSINE, [0.1, 1.2, 0.3, 6.0, 0.4]),

@@ -344,6 +344,25 @@ def test_radlaw_cutoffs_and_hybrids(orc):
             assert h == pytest.approx(d + q, rel=1e-15, abs=0.0), (hyb, mu)
 
 
+def test_smearing_identical_views_equal_one_view(orc):
+    """calc_fits.c's smearing averages the fits of a frame's views: three views
+    at the same epoch give the one-view fit (to rounding), and views spread
+    over time give a different one."""
+    base = dict(n_deldop=2, n_doppler=2, nbin_min=30, nbin_max=40)
+    one = sh.Scene(sh.spec_for(1, **base))
+    same = sh.Scene(sh.spec_for(1, radar_nviews=3, view_interval_days=0.0, **base))
+    spread = sh.Scene(sh.spec_for(1, radar_nviews=3, view_interval_days=0.02, **base))
+    for sc in (one, same, spread):
+        H.oracle_forward(orc, sc)
+        assert orc.oracle_last_error() in (None, b"")
+    for s, f in one.all_frames():
+        if f < 0:
+            continue
+        a, b, c = one.image(s, f, 0), same.image(s, f, 0), spread.image(s, f, 0)
+        assert np.max(np.abs(b - a)) <= 1e-14 * np.max(np.abs(a)), (s, f)
+        assert np.max(np.abs(c - a)) > 1e-6 * np.max(np.abs(a)), (s, f)
+
+
 def test_deldop_power_conservation(orc):
     """sum(fit) = kmpp^2 * sum over lit pixels of radlaw(cos) when every footprint
     lies inside the frame (long code: codefactor 1)."""
