@@ -339,6 +339,7 @@ void shp_spec_defaults(shp_spec *s)
   s->obs_lat_max_deg = 10.0;
   s->radar_nviews = 1;
   s->view_interval_days = 0.0;
+  s->lc_nviews = 1;
 }
 
 /* Euler-angle matrix with the reference's convention (util/euler.c). */
@@ -706,8 +707,9 @@ static void build_data(shp_scene *sc)
     for (int k = 0; k < 3; k++) { set->angleoff[k].state = 'c'; set->omegaoff[k].state = 'c'; }
     struct lghtcrv_t *lc = &set->desc.lghtcrv;
     lc->ioptlaw = l;
-    lc->nviews = 1;
-    lc->v0 = 0;
+    lc->nviews = sp->lc_nviews > 1 ? sp->lc_nviews : 1;
+    lc->v0 = lc->nviews / 2;
+    lc->view_interval = sp->view_interval_days;
     lc->n = sp->lc_points;
     lc->ncalc = sp->lc_ncalc;
     lc->weight = 1.0;
@@ -715,7 +717,7 @@ static void build_data(shp_scene *sc)
     lc->cal.val = 1.0;
     int nn = lc->n, nc = lc->ncalc;
     lc->t0 = t_vector(sc, 1, nn);
-    lc->t = t_matrix(sc, 1, nn, 0, 0);
+    lc->t = t_matrix(sc, 1, nn, 0, lc->nviews - 1);
     lc->obs = t_vector(sc, 1, nn);
     lc->oneovervar = t_vector(sc, 1, nn);
     lc->fit = t_vector(sc, 1, nn);
@@ -749,7 +751,7 @@ static void build_data(shp_scene *sc)
     for (int i = 1; i <= nn; i++) {
       double t = tstart + len * (0.02 + 0.96 * (i - 0.5) / nn);
       lc->t0[i] = t;
-      lc->t[i][0] = t;
+      for (int v = 0; v < lc->nviews; v++) lc->t[i][v] = t + (v - lc->v0) * lc->view_interval;
       lc->oneovervar[i] = 1.0;
     }
     lc->dof = lc->weight * nn;

@@ -69,6 +69,7 @@ typedef struct shp_spec {
   /* smearing: views per radar frame, centred on the frame epoch (v0 = nviews / 2) */
   int radar_nviews;
   double view_interval_days;
+  int lc_nviews;           /* smearing of lightcurve points (same interval) */
 } shp_spec;
 
 typedef struct shp_scene shp_scene;

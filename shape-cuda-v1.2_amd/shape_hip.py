@@ -54,7 +54,7 @@ class Spec(C.Structure):
         ("phase_min_deg", C.c_double), ("phase_max_deg", C.c_double), ("kaas_wt", C.c_double),
         ("delcor_a0", C.c_double), ("delcor_a1", C.c_double),
         ("obs_lat_min_deg", C.c_double), ("obs_lat_max_deg", C.c_double),
-        ("radar_nviews", C.c_int), ("view_interval_days", C.c_double),
+        ("radar_nviews", C.c_int), ("view_interval_days", C.c_double), ("lc_nviews", C.c_int),
     ]
 
 

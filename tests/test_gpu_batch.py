@@ -75,13 +75,15 @@ def _check(config, method, over, tol_chi, tol_img, flags=sh.BATCH_KEEP_FIT, law=
 
 
 SMEAR = dict(radar_nviews=3, view_interval_days=0.02, n_deldop=2, n_doppler=2, nbin_min=30, nbin_max=40)
+SMEAR_LC = dict(n_deldop=3, n_doppler=3, n_lightcurves=2, lc_points=30, lc_ncalc=10, lc_nviews=3,
+                view_interval_days=0.05)
 
 
 @pytest.mark.parametrize("config,over", [(1, {}), (2, dict(n_deldop=16)),
                                          (1, dict(n_deldop=0, n_doppler=2, nbin_min=30, nbin_max=40)),
                                          (3, dict(n_deldop=3, n_doppler=3, n_lightcurves=2, lc_points=30,
                                                   lc_ncalc=10)),
-                                         (1, SMEAR)])
+                                         (1, SMEAR), (3, SMEAR_LC)])
 def test_fp64_batch_matches_oracle(config, over):
     _check(config, sh.METHOD_FP64, over, 1e-10, 1e-12)
 
@@ -93,7 +95,7 @@ def test_fp64_batch_matches_oracle(config, over):
                                          (5, dict(n_deldop=6)),
                                          (3, dict(n_deldop=3, n_doppler=3, n_lightcurves=2, lc_points=30,
                                                   lc_ncalc=10)),
-                                         (1, SMEAR)])
+                                         (1, SMEAR), (3, SMEAR_LC)])
 def test_mixed_batch_matches_oracle(config, over):
     # Per frame, a chi2 near its noise floor amplifies the mixed method's
     # relative fit error eps (~1e-7) to ~2 eps sqrt(o2 / chi2): ~4e-6 at SNR

@@ -58,7 +58,8 @@ SMALL3 = dict(n_deldop=4, n_doppler=4, n_lightcurves=3, lc_points=40, lc_ncalc=1
 CASES = [(1, {}), (2, {}), (3, SMALL3), (1, dict(codemethod=0, delcor_a0=0.4, delcor_a1=-0.2)),
          (1, dict(n_deldop=0, n_doppler=2, nbin_min=30, nbin_max=40)),
          # smearing: three views per radar frame (calc_fits.c's view loop)
-         (1, dict(radar_nviews=3, view_interval_days=0.02, n_deldop=2, n_doppler=2, nbin_min=30, nbin_max=40))]
+         (1, dict(radar_nviews=3, view_interval_days=0.02, n_deldop=2, n_doppler=2, nbin_min=30, nbin_max=40)),
+         (3, dict(SMALL3, lc_nviews=3, view_interval_days=0.05))]
 
 
 def _trial(sc):

@@ -363,6 +363,23 @@ def test_smearing_identical_views_equal_one_view(orc):
         assert np.max(np.abs(c - a)) > 1e-6 * np.max(np.abs(a)), (s, f)
 
 
+def test_lightcurve_smearing_averages_views(orc):
+    """A lightcurve point smeared over views averages the spline at the views'
+    epochs (calc_fits.c's lightcurve branch): views at one epoch give the
+    one-view fit, views spread in time another."""
+    base = dict(n_deldop=1, n_doppler=0, n_lightcurves=1, lc_points=20, lc_ncalc=8)
+    one = sh.Scene(sh.spec_for(3, **base))
+    same = sh.Scene(sh.spec_for(3, lc_nviews=3, view_interval_days=0.0, **base))
+    spread = sh.Scene(sh.spec_for(3, lc_nviews=3, view_interval_days=0.05, **base))
+    for sc in (one, same, spread):
+        H.oracle_forward(orc, sc)
+        assert orc.oracle_last_error() in (None, b"")
+    s = next(q for q in range(one.nsets) if one.set_type(q) == sh.LGHTCRV)
+    a, b, c = one.image(s, 0, 0), same.image(s, 0, 0), spread.image(s, 0, 0)
+    assert np.max(np.abs(b - a)) <= 1e-14 * np.max(np.abs(a))
+    assert np.max(np.abs(c - a)) > 1e-6 * np.max(np.abs(a))
+
+
 def test_deldop_power_conservation(orc):
     """sum(fit) = kmpp^2 * sum over lit pixels of radlaw(cos) when every footprint
     lies inside the frame (long code: codefactor 1)."""
