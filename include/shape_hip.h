@@ -44,6 +44,21 @@ void realize_delcor(struct dat_t *dat, double delta_delcor0, int delcor0_mode);
    POS arrays are written back only after shape_hip_set_strict(1). */
 void calc_fits(struct par_t *par, struct mod_t *mod, struct dat_t *dat);
 
+/* calc_fits.c:440 / :830 / :1509  the fits of dataset s alone (a
+   delay-Doppler, Doppler or lightcurve set), as the HostMultiThreading
+   driver calls them (calc_fits_hmt.c:146-155): raises par's posbnd /
+   badradar flags and adds the un-normalized log factors, or-s in the
+   facets' seen flags.  Disjoint datasets may run from several host threads
+   at once. */
+void calc_deldop(struct par_t *par, struct mod_t *mod, struct deldop_t *deldop, int s);
+void calc_doppler(struct par_t *par, struct mod_t *mod, struct doppler_t *doppler, int s);
+void calc_lghtcrv(struct par_t *par, struct mod_t *mod, struct lghtcrv_t *lghtcrv, int s);
+
+/* realize_spin.c:286 / :317  the '=' states of the datasets' angle and spin
+   offsets (write_dat.c:123-124 calls them) */
+void realize_angleoff(struct dat_t *dat);
+void realize_omegaoff(struct dat_t *dat);
+
 /* chi2.c:240  chi-square of the current fits; floats calibration factors,
    writes frame->chi2, cal.val, set chi2 and dat->chi2, returns dat->chi2. */
 double chi2(struct par_t *par, struct dat_t *dat, int list_breakdown);
@@ -93,6 +108,11 @@ int shape_hip_device_count(void);
 void shape_hip_realize_delcor(struct par_t *par, struct mod_t *mod, struct dat_t *dat,
                               double delta_delcor0, int delcor0_mode);
 
+/* calc_fits and the one-dataset entries keep each call site's device arenas
+   (model, dataset) across calls and refresh them from the structs; this
+   frees every cached arena not in use (no reference counterpart) */
+void shape_hip_release_cache(void);
+
 /* number of POS pixels whose cos(scattering) the last calc_fits inherited
    from an earlier-claiming facet (diagnostic) */
 int shape_hip_last_replayed(void);
@@ -136,8 +156,8 @@ int shape_batch_facet_map(shape_batch *b, int e, int frame, int v, int *out);
 int shape_batch_frame_terms(shape_batch *b, int e, double *out);
 /* [0] pixels replayed in FP64, [1] error flags, [2] mixed: replayed pixels
    whose candidate facets a second raster pass collected (the rest come from
-   their tile's own list), [3] mixed: footprints too wide for the register
-   window */
+   their tile's own list), [3] reserved (0; footprints of every width are
+   synthesized) */
 int shape_batch_counters(shape_batch *b, int *out4);
 /* nevals, nviews, nframes, geometry slots, POS pixels, fit cells, max facets, max POS pixels */
 int shape_batch_sizes(shape_batch *b, long long *out8);

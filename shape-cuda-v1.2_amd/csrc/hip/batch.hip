@@ -216,7 +216,8 @@ extern "C" int shape_batch_facet_map(shape_batch *B, int e, int q, int v, int *o
 
 // counters of the last replay: [0] replayed pixels (FP64 method: owner faces
 // away; mixed: owner faces away or pixel on a facet edge), [1] error flags,
-// [2] unused, [3] mixed: footprints too wide
+// [2] mixed: replayed pixels whose candidates the second raster pass
+// collected, [3] reserved (0: every footprint width is synthesized)
 extern "C" int shape_batch_counters(shape_batch *B, int *out4) {
   return hipMemcpy(out4, B->E.pd.nstale, sizeof(int) * 4, hipMemcpyDeviceToHost) == hipSuccess ? 0 : -4;
 }

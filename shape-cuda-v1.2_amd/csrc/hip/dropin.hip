@@ -11,7 +11,10 @@
 #include <stdio.h>
 #include <string.h>
 
+#include <atomic>
+#include <list>
 #include <memory>
+#include <mutex>
 #include <vector>
 
 #include "common.h"
@@ -20,7 +23,7 @@
 
 using namespace shp;
 
-static int g_err = 0;
+static std::atomic<int> g_err{0};
 static int g_strict_pos = 0;
 static int g_last_nstale = 0;
 
@@ -46,27 +49,212 @@ extern "C" int shape_hip_device_count(void) {
 }
 
 // ---------------------------------------------------------------------------
-// calc_fits (calc_fits.c:340): delay-Doppler, Doppler and lightcurve fits for
-// every dataset from the already-realized model and spin state
+// Device arenas kept across calls.  A fit calls calc_fits (or, under the
+// HostMultiThreading driver, calc_deldop / calc_doppler / calc_lghtcrv per
+// dataset) on the same structs again and again with new parameter values.
+// Each call site -- (entry, model, dataset) -- keeps its engine: built on
+// the first call, refreshed from the structs on later ones (views, spin
+// state, frame scalars, parameters, realized geometry), rebuilt when the
+// structs' shape or the model's topology changed.  Engines not in use are
+// freed, least recently used first, when the cache holds more than its
+// device-memory budget (a quarter of the device); shape_hip_release_cache
+// frees them all.
 // ---------------------------------------------------------------------------
-extern "C" void calc_fits(struct par_t *par, struct mod_t *mod, struct dat_t *dat) {
-  Engine E;
-  RunOptions o;
-  o.method = METHOD_FP64;
-  o.host_spin = 1;
-  o.host_geometry = 1;
-  o.keep_fit = 1;
-  o.with_body = 0;
-  std::vector<EvalSpec> ev(1);
-  ev[0].model = 0;
-  int rc = E.build(par, &mod, &dat, 1, ev, o);
-  if (!rc) rc = E.upload_params();
+namespace {
+enum { DK_FITS = 0, DK_DELDOP = 1, DK_DOPPLER = 2, DK_LGHTCRV = 3 };
+
+struct CacheSlot {
+  int kind = 0;
+  const void *mod = nullptr, *key = nullptr;
+  unsigned long long topo = 0;
+  std::unique_ptr<Engine> E;
+  unsigned long long tick = 0;
+  bool busy = false;
+};
+std::mutex g_cache_mu;
+std::list<CacheSlot> g_cache;
+unsigned long long g_tick = 0;
+size_t g_cache_budget = 0;
+
+size_t cache_budget() {
+  if (!g_cache_budget) {
+    size_t fr = 0, tot = 0;
+    g_cache_budget = hipMemGetInfo(&fr, &tot) == hipSuccess && tot ? tot / 4 : (size_t)32 << 30;
+  }
+  return g_cache_budget;
+}
+
+// FNV-1a over the model's facet corners and vertex -> facet lists
+unsigned long long topo_hash(const struct mod_t *mod) {
+  unsigned long long h = 1469598103934665603ULL;
+  auto mix = [&h](long long v) { h = (h ^ (unsigned long long)v) * 1099511628211ULL; };
+  mix(mod->shape.ncomp);
+  for (int c = 0; c < mod->shape.ncomp; c++) {
+    const struct vertices_t *rv = &mod->shape.comp[c].real;
+    mix(rv->nv);
+    mix(rv->nf);
+    for (int f = 0; f < rv->nf; f++) { mix(rv->f[f].v[0]); mix(rv->f[f].v[1]); mix(rv->f[f].v[2]); }
+    for (int i = 0; i < rv->nv; i++) {
+      mix(rv->v[i].naf);
+      for (int q = 0; q < rv->v[i].naf; q++) mix(rv->v[i].af[q]);
+    }
+  }
+  return h;
+}
+
+// free idle engines, least recently used first, down to `keep` bytes
+// (called with g_cache_mu held)
+void cache_trim(size_t keep) {
+  for (;;) {
+    size_t held = 0;
+    for (auto &c : g_cache)
+      if (c.E) held += c.E->dev_bytes;
+    if (held <= keep) return;
+    auto victim = g_cache.end();
+    for (auto it = g_cache.begin(); it != g_cache.end(); ++it)
+      if (!it->busy && it->E && (victim == g_cache.end() || it->tick < victim->tick)) victim = it;
+    if (victim == g_cache.end()) return;
+    g_cache.erase(victim);
+  }
+}
+
+// The engine of this call site, built or refreshed; nullptr and *rc on failure.
+CacheSlot *acquire(int kind, struct par_t *par, struct mod_t *mod, struct dat_t *dat, const void *key, int only_set,
+                   int *rc) {
+  const unsigned long long topo = topo_hash(mod);
+  CacheSlot *slot = nullptr;
+  {
+    std::lock_guard<std::mutex> lk(g_cache_mu);
+    for (auto &c : g_cache)
+      if (c.kind == kind && c.mod == mod && c.key == key && !c.busy) { slot = &c; break; }
+    if (!slot) {
+      g_cache.emplace_back();
+      slot = &g_cache.back();
+      slot->kind = kind; slot->mod = mod; slot->key = key;
+    }
+    slot->busy = true;
+  }
+  *rc = 0;
+  bool built = false;
+  if (slot->E && slot->topo == topo) {
+    const int r = slot->E->refresh(par, mod, dat);
+    if (r < 0) *rc = r;
+    built = r == 0;
+  }
+  if (!built && !*rc) {
+    slot->E.reset();  // its device memory first
+    {
+      std::lock_guard<std::mutex> lk(g_cache_mu);
+      cache_trim(cache_budget());
+    }
+    slot->E.reset(new Engine());
+    RunOptions o;
+    o.method = METHOD_FP64;
+    o.host_spin = 1;
+    o.host_geometry = 1;
+    o.keep_fit = 1;
+    o.only_set = only_set;
+    std::vector<EvalSpec> ev(1);
+    ev[0].model = 0;
+    *rc = slot->E->build(par, &mod, &dat, 1, ev, o);
+    slot->topo = topo;
+  }
+  if (*rc) {
+    std::lock_guard<std::mutex> lk(g_cache_mu);
+    for (auto it = g_cache.begin(); it != g_cache.end(); ++it)
+      if (&*it == slot) { g_cache.erase(it); break; }
+    return nullptr;
+  }
+  return slot;
+}
+
+void release(CacheSlot *slot) {
+  std::lock_guard<std::mutex> lk(g_cache_mu);
+  slot->busy = false;
+  slot->tick = ++g_tick;
+  cache_trim(cache_budget());
+}
+
+// One evaluation on a drop-in engine: fits and seen flags start from zero
+// (calc_fits.c:358-361, :538, :919), as the reference's clrmat and reset do.
+int run_fits(Engine &E, struct par_t *par, struct mod_t *mod, struct dat_t *dat, int accumulate) {
+  SHP_CHECK(hipMemsetAsync(E.d_fit, 0, sizeof(double) * std::max(1LL, E.fit_len), E.stream));
+  const GeomSlot &G = E.h_slots[E.h_eval_slot[0]];
+  SHP_CHECK(hipMemsetAsync(G.seen, 0, std::max(1, G.nf), E.stream));
+  int rc = E.upload_params();
   if (!rc) rc = E.enqueue();
   if (!rc) rc = E.sync();
   if (!rc && E.error_flags()) rc = -30;
   g_last_nstale = E.last_nstale;
-  if (!rc) rc = E.writeback(0, par, mod, dat, g_strict_pos);
+  if (!rc) rc = E.writeback(0, par, mod, dat, g_strict_pos, accumulate);
+  return rc;
+}
+
+// A dat_t holding one caller dataset at index s (earlier entries are
+// placeholders the engine skips), for the one-dataset entries: the engine
+// packs it as dataset s, so exclude_seen and the parameter layout keep the
+// caller's numbering.  The dataset's description is copied shallowly:
+// every array the fits are written to stays the caller's.
+struct OneSet {
+  std::vector<struct set_t> sets;
+  struct dat_t dat;
+  OneSet(int s, int type, const void *desc, size_t desc_size) : sets((size_t)s + 1) {
+    memset((void *)sets.data(), 0, sizeof(struct set_t) * sets.size());
+    for (int k = 0; k < s; k++) sets[k].type = 255;
+    sets[s].type = (unsigned char)type;
+    memcpy((void *)&sets[s].desc, desc, desc_size);
+    memset((void *)&dat, 0, sizeof(dat));
+    dat.nsets = s + 1;
+    dat.set = sets.data();
+    dat.dof = dat.dof_deldop = dat.dof_doppler = dat.dof_poset = dat.dof_lghtcrv = 1.0;
+  }
+};
+
+void one_set_fits(int kind, int type, const char *name, struct par_t *par, struct mod_t *mod, const void *desc,
+                  size_t desc_size, int s) {
+  if (s < 0) { set_err(-15, name); return; }
+  OneSet one(s, type, desc, desc_size);
+  int rc = 0;
+  CacheSlot *slot = acquire(kind, par, mod, &one.dat, desc, s, &rc);
+  if (slot) {
+    rc = run_fits(*slot->E, par, mod, &one.dat, 1);
+    release(slot);
+  }
+  set_err(rc, name);
+}
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// calc_fits (calc_fits.c:340): delay-Doppler, Doppler and lightcurve fits for
+// every dataset from the already-realized model and spin state
+// ---------------------------------------------------------------------------
+extern "C" void calc_fits(struct par_t *par, struct mod_t *mod, struct dat_t *dat) {
+  int rc = 0;
+  CacheSlot *slot = acquire(DK_FITS, par, mod, dat, dat, -1, &rc);
+  if (slot) {
+    rc = run_fits(*slot->E, par, mod, dat, 0);
+    release(slot);
+  }
   set_err(rc, "calc_fits");
+}
+
+// calc_fits.c:440 / :830 / :1509: the fits of dataset s alone; the flags in
+// par are raised and the un-normalized log factors added to, the facets'
+// seen flags or-ed in (the caller resets them, as calc_fits does).
+// Disjoint datasets may be computed from several host threads at once.
+extern "C" void calc_deldop(struct par_t *par, struct mod_t *mod, struct deldop_t *deldop, int s) {
+  one_set_fits(DK_DELDOP, DELAY, "calc_deldop", par, mod, deldop, sizeof(*deldop), s);
+}
+extern "C" void calc_doppler(struct par_t *par, struct mod_t *mod, struct doppler_t *doppler, int s) {
+  one_set_fits(DK_DOPPLER, DOPPLER, "calc_doppler", par, mod, doppler, sizeof(*doppler), s);
+}
+extern "C" void calc_lghtcrv(struct par_t *par, struct mod_t *mod, struct lghtcrv_t *lghtcrv, int s) {
+  one_set_fits(DK_LGHTCRV, LGHTCRV, "calc_lghtcrv", par, mod, lghtcrv, sizeof(*lghtcrv), s);
+}
+
+extern "C" void shape_hip_release_cache(void) {
+  std::lock_guard<std::mutex> lk(g_cache_mu);
+  cache_trim(0);
 }
 
 // number of POS pixels whose cos(scattering) was inherited from an earlier
@@ -74,13 +262,52 @@ extern "C" void calc_fits(struct par_t *par, struct mod_t *mod, struct dat_t *da
 extern "C" int shape_hip_last_replayed(void) { return g_last_nstale; }
 
 // ---------------------------------------------------------------------------
+// realize_angleoff / realize_omegaoff (realize_spin.c:286, :317): the '='
+// state of each component of the datasets' angle and spin offsets takes the
+// value of the last earlier dataset whose state is not '='.  Bookkeeping of
+// the caller's structs, no arithmetic; the reference's bailout for an '='
+// on the first dataset becomes an error code.
+// ---------------------------------------------------------------------------
+static int resolve_offsets(struct dat_t *dat, int which) {
+  for (int j = 0; j <= 2; j++) {
+    int src = -1;
+    for (int s = 0; s < dat->nsets; s++) {
+      struct param_t *p = which ? &dat->set[s].omegaoff[j] : &dat->set[s].angleoff[j];
+      if (p->state != '=') src = s;
+      else if (src < 0) return -17;
+      else p->val = (which ? dat->set[src].omegaoff[j] : dat->set[src].angleoff[j]).val;
+    }
+  }
+  return 0;
+}
+extern "C" void realize_angleoff(struct dat_t *dat) { set_err(resolve_offsets(dat, 0), "realize_angleoff"); }
+extern "C" void realize_omegaoff(struct dat_t *dat) { set_err(resolve_offsets(dat, 1), "realize_omegaoff"); }
+
+// Device buffers of the chi2 and realize_mod drop-ins, grown on demand and
+// kept across calls.
+namespace {
+struct GrowBuf {
+  void *p = nullptr;
+  size_t cap = 0;
+  ~GrowBuf() { if (p) (void)hipFree(p); }
+  int get(size_t bytes) {
+    if (bytes == 0) bytes = 8;
+    if (bytes <= cap) return 0;
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    cap = 0;
+    if (hipMalloc(&p, bytes) != hipSuccess) return -2;
+    cap = bytes;
+    return 0;
+  }
+};
+std::mutex g_chi2_mu;
+GrowBuf g_c_fr, g_c_fit, g_c_obs, g_c_oov, g_c_set, g_c_tot;
+}  // namespace
+
+// ---------------------------------------------------------------------------
 // chi2 (chi2.c:240): chi-square of the fits currently held by the caller
 // ---------------------------------------------------------------------------
-struct DevBuf {
-  void *p = nullptr;
-  ~DevBuf() { if (p) (void)hipFree(p); }
-  int get(size_t bytes) { return hipMalloc(&p, bytes ? bytes : 8) == hipSuccess ? 0 : -2; }
-};
 
 extern "C" double chi2(struct par_t *par, struct dat_t *dat, int list_breakdown) {
   (void)par;
@@ -159,7 +386,8 @@ extern "C" double chi2(struct par_t *par, struct dat_t *dat, int list_breakdown)
     }
   });
   int nf = (int)fr.size(), nsets = dat->nsets;
-  DevBuf dfr, dfit, dobs, doov, dset, dtot;
+  std::lock_guard<std::mutex> lk(g_chi2_mu);
+  GrowBuf &dfr = g_c_fr, &dfit = g_c_fit, &dobs = g_c_obs, &doov = g_c_oov, &dset = g_c_set, &dtot = g_c_tot;
   const size_t nb = sizeof(double) * (size_t)ncell;
   int rc = dfr.get(sizeof(FrameRec) * nf) | dfit.get(nb) | dobs.get(nb) | doov.get(nb) | dset.get(sizeof(double) * nsets) |
            dtot.get(sizeof(double));
@@ -197,20 +425,29 @@ extern "C" double chi2(struct par_t *par, struct dat_t *dat, int list_breakdown)
 // realize_mod (realize_mod.c:153): vertex coordinates, facet normals, areas,
 // centroids and vertex normals of a single-component VERTEX model
 // ---------------------------------------------------------------------------
+namespace {
+// realize_mod's device arrays, kept across calls; the topology (facet
+// corners, vertex -> facet lists) is uploaded again only when the model or
+// its topology changes
+struct RealizeBufs {
+  const void *mod = nullptr;
+  unsigned long long topo = 0;
+  GrowBuf fv, off, idx, u, a, r, pc, x, vn, fn, fx, ar, x32, vn32, fn32, slot, flag;
+};
+std::mutex g_realize_mu;
+RealizeBufs g_rb;
+}  // namespace
+
 extern "C" void realize_mod(struct par_t *par, struct mod_t *mod) {
   if (mod->shape.ncomp != 1 || mod->shape.comp[0].type != VERTEX) { set_err(-10, "realize_mod"); return; }
   struct comp_t *comp = &mod->shape.comp[0];
   struct vertices_t *rv = &comp->real;
-  int nv = rv->nv, nf = rv->nf;
-  std::vector<int> fv(3 * (size_t)nf), af_off(nv + 1, 0), af_idx;
-  std::vector<double> u(3 * (size_t)nv), a(3 * (size_t)nv), r(nv), pc(9);
-  for (int f = 0; f < nf; f++)
-    for (int k = 0; k < 3; k++) fv[3 * f + k] = rv->f[f].v[k];
+  const int nv = rv->nv, nf = rv->nf;
+  std::vector<double> in(7 * (size_t)nv + 9);  // u, a (3 nv each), r (nv), scale/rot/off (9)
+  double *u = in.data(), *a = u + 3 * (size_t)nv, *r = a + 3 * (size_t)nv, *pc = r + nv;
   for (int i = 0; i < nv; i++) {
     struct vertex_t *v = &rv->v[i];
     if (v->r.state == '=') v->r.val = rv->v[v->v_mirror].r.val;  // realize_mod.c:450-454
-    af_off[i + 1] = af_off[i] + v->naf;
-    for (int q = 0; q < v->naf; q++) af_idx.push_back(v->af[q]);
     for (int k = 0; k < 3; k++) { u[3 * i + k] = v->u[k]; a[3 * i + k] = v->a[k]; }
     r[i] = v->r.val;
   }
@@ -221,46 +458,82 @@ extern "C" void realize_mod(struct par_t *par, struct mod_t *mod) {
     pc[3 + j] = comp->rot[j].val;
     pc[6 + j] = comp->off[j].val;
   }
-  DevBuf dfv, doff, didx, du, da, dr, dpc, dx, dvn, dfn, dfx, dar, dx32, dvn32, dfn32, dslot, dflag, dlogf;
-  int rc = dfv.get(sizeof(int) * fv.size()) | doff.get(sizeof(int) * af_off.size()) |
-           didx.get(sizeof(int) * af_idx.size()) | du.get(sizeof(double) * u.size()) | da.get(sizeof(double) * a.size()) |
-           dr.get(sizeof(double) * nv) | dpc.get(sizeof(double) * 9) | dx.get(sizeof(double) * 3 * nv) |
-           dvn.get(sizeof(double) * 3 * nv) | dfn.get(sizeof(double) * 3 * nf) | dfx.get(sizeof(double) * 3 * nf) |
-           dar.get(sizeof(double) * nf) | dx32.get(sizeof(float4) * nv) | dvn32.get(sizeof(float4) * nv) |
-           dfn32.get(sizeof(float4) * nf) | dslot.get(sizeof(GeomSlot)) | dflag.get(sizeof(int)) | dlogf.get(sizeof(double));
-  if (rc) { set_err(rc, "realize_mod allocation"); return; }
-  GeomSlot G;
-  memset(&G, 0, sizeof(G));
-  G.nv = nv; G.nf = nf; G.comp_pb = 0; G.eval = -1;
-  G.fv = (int *)dfv.p; G.af_off = (int *)doff.p; G.af_idx = (int *)didx.p; G.u = (double *)du.p; G.a = (double *)da.p;
-  G.r = (double *)dr.p; G.x = (double *)dx.p; G.vn = (double *)dvn.p; G.fn = (double *)dfn.p; G.fx = (double *)dfx.p;
-  G.area = (double *)dar.p; G.x32 = (float4 *)dx32.p; G.vn32 = (float4 *)dvn32.p; G.fn32 = (float4 *)dfn32.p;
-  bool ok = hipMemcpy(dfv.p, fv.data(), sizeof(int) * fv.size(), hipMemcpyHostToDevice) == hipSuccess &&
-            hipMemcpy(doff.p, af_off.data(), sizeof(int) * af_off.size(), hipMemcpyHostToDevice) == hipSuccess &&
-            (af_idx.empty() || hipMemcpy(didx.p, af_idx.data(), sizeof(int) * af_idx.size(), hipMemcpyHostToDevice) == hipSuccess) &&
-            hipMemcpy(du.p, u.data(), sizeof(double) * u.size(), hipMemcpyHostToDevice) == hipSuccess &&
-            hipMemcpy(da.p, a.data(), sizeof(double) * a.size(), hipMemcpyHostToDevice) == hipSuccess &&
-            hipMemcpy(dr.p, r.data(), sizeof(double) * nv, hipMemcpyHostToDevice) == hipSuccess &&
-            hipMemcpy(dpc.p, pc.data(), sizeof(double) * 9, hipMemcpyHostToDevice) == hipSuccess &&
-            hipMemcpy(dslot.p, &G, sizeof(G), hipMemcpyHostToDevice) == hipSuccess;
-  if (!ok) { set_err(-3, "realize_mod upload"); return; }
-  GeomSlot *ds = (GeomSlot *)dslot.p;
-  hipLaunchKernelGGL(k_realize_vertices, dim3((nv + 255) / 256, 1), dim3(256), 0, 0, ds, 1, (const double *)dpc.p,
-                     (int *)dflag.p, (double *)dlogf.p);
+  const unsigned long long topo = topo_hash(mod);
+  std::lock_guard<std::mutex> lk(g_realize_mu);
+  RealizeBufs &B = g_rb;
+  const bool new_topo = B.mod != mod || B.topo != topo;
+  int rc = B.u.get(sizeof(double) * 3 * nv) | B.a.get(sizeof(double) * 3 * nv) | B.r.get(sizeof(double) * nv) |
+           B.pc.get(sizeof(double) * 9) | B.x.get(sizeof(double) * 3 * nv) | B.vn.get(sizeof(double) * 3 * nv) |
+           B.fn.get(sizeof(double) * 3 * nf) | B.fx.get(sizeof(double) * 3 * nf) | B.ar.get(sizeof(double) * nf) |
+           B.x32.get(sizeof(float4) * nv) | B.vn32.get(sizeof(float4) * nv) | B.fn32.get(sizeof(float4) * nf) |
+           B.slot.get(sizeof(GeomSlot)) | B.flag.get(2 * sizeof(double));  // baddiam flag, log factor at +8
+  if (rc) { B.mod = nullptr; set_err(rc, "realize_mod allocation"); return; }
+  bool ok = true;
+  if (new_topo) {
+    std::vector<int> fv(3 * (size_t)nf), af_off(nv + 1, 0), af_idx;
+    for (int f = 0; f < nf; f++)
+      for (int k = 0; k < 3; k++) fv[3 * f + k] = rv->f[f].v[k];
+    for (int i = 0; i < nv; i++) {
+      af_off[i + 1] = af_off[i] + rv->v[i].naf;
+      for (int q = 0; q < rv->v[i].naf; q++) af_idx.push_back(rv->v[i].af[q]);
+    }
+    B.mod = nullptr;
+    ok = B.fv.get(sizeof(int) * fv.size()) == 0 && B.off.get(sizeof(int) * af_off.size()) == 0 &&
+         B.idx.get(sizeof(int) * af_idx.size()) == 0 &&
+         hipMemcpy(B.fv.p, fv.data(), sizeof(int) * fv.size(), hipMemcpyHostToDevice) == hipSuccess &&
+         hipMemcpy(B.off.p, af_off.data(), sizeof(int) * af_off.size(), hipMemcpyHostToDevice) == hipSuccess &&
+         (af_idx.empty() ||
+          hipMemcpy(B.idx.p, af_idx.data(), sizeof(int) * af_idx.size(), hipMemcpyHostToDevice) == hipSuccess);
+    GeomSlot G;
+    memset(&G, 0, sizeof(G));
+    G.nv = nv; G.nf = nf; G.comp_pb = 0; G.eval = -1;
+    G.fv = (int *)B.fv.p; G.af_off = (int *)B.off.p; G.af_idx = (int *)B.idx.p; G.u = (double *)B.u.p;
+    G.a = (double *)B.a.p; G.r = (double *)B.r.p; G.x = (double *)B.x.p; G.vn = (double *)B.vn.p;
+    G.fn = (double *)B.fn.p; G.fx = (double *)B.fx.p; G.area = (double *)B.ar.p; G.x32 = (float4 *)B.x32.p;
+    G.vn32 = (float4 *)B.vn32.p; G.fn32 = (float4 *)B.fn32.p;
+    ok = ok && hipMemcpy(B.slot.p, &G, sizeof(G), hipMemcpyHostToDevice) == hipSuccess;
+    if (ok) { B.mod = mod; B.topo = topo; }
+  } else {
+    // the inputs' buffers may have moved when they grew: the slot names them
+    GeomSlot G;
+    ok = hipMemcpy(&G, B.slot.p, sizeof(G), hipMemcpyDeviceToHost) == hipSuccess;
+    if (ok && (G.u != B.u.p || G.a != B.a.p || G.r != B.r.p || G.x != B.x.p || G.vn != B.vn.p || G.fn != B.fn.p ||
+               G.fx != B.fx.p || G.area != B.ar.p || G.x32 != B.x32.p || G.vn32 != B.vn32.p || G.fn32 != B.fn32.p)) {
+      G.u = (double *)B.u.p; G.a = (double *)B.a.p; G.r = (double *)B.r.p; G.x = (double *)B.x.p;
+      G.vn = (double *)B.vn.p; G.fn = (double *)B.fn.p; G.fx = (double *)B.fx.p; G.area = (double *)B.ar.p;
+      G.x32 = (float4 *)B.x32.p; G.vn32 = (float4 *)B.vn32.p; G.fn32 = (float4 *)B.fn32.p;
+      ok = hipMemcpy(B.slot.p, &G, sizeof(G), hipMemcpyHostToDevice) == hipSuccess;
+    }
+  }
+  ok = ok && hipMemcpy(B.u.p, u, sizeof(double) * 3 * nv, hipMemcpyHostToDevice) == hipSuccess &&
+       hipMemcpy(B.a.p, a, sizeof(double) * 3 * nv, hipMemcpyHostToDevice) == hipSuccess &&
+       hipMemcpy(B.r.p, r, sizeof(double) * nv, hipMemcpyHostToDevice) == hipSuccess &&
+       hipMemcpy(B.pc.p, pc, sizeof(double) * 9, hipMemcpyHostToDevice) == hipSuccess;
+  if (!ok) { (void)hipGetLastError(); B.mod = nullptr; set_err(-3, "realize_mod upload"); return; }
+  GeomSlot *ds = (GeomSlot *)B.slot.p;
+  int *dflag = (int *)B.flag.p;
+  double *dlogf = (double *)((char *)B.flag.p + sizeof(double));
+  hipLaunchKernelGGL(k_realize_vertices, dim3((nv + 255) / 256, 1), dim3(256), 0, 0, ds, 1, (const double *)B.pc.p,
+                     dflag, dlogf);
   hipLaunchKernelGGL(k_realize_facets, dim3((nf + 255) / 256, 1), dim3(256), 0, 0, ds, 1);
   hipLaunchKernelGGL(k_realize_vnormals, dim3((nv + 255) / 256, 1), dim3(256), 0, 0, ds, 1);
-  std::vector<double> x(3 * (size_t)nv), vn(3 * (size_t)nv), fn(3 * (size_t)nf), fx(3 * (size_t)nf), ar(nf);
+  // results through the thread's pinned stage (plain host memory if it cannot be pinned)
+  const size_t nout = 6 * (size_t)nv + 7 * (size_t)nf;
+  std::vector<double> plain;
+  double *x = host_stage(nout);
+  if (!x) { plain.resize(nout); x = plain.data(); }
+  double *vn = x + 3 * (size_t)nv, *fn = vn + 3 * (size_t)nv, *fx = fn + 3 * (size_t)nf, *ar = fx + 3 * (size_t)nf;
   int flag = 0;
   double logf = 0.0;
   ok = hipDeviceSynchronize() == hipSuccess &&
-       hipMemcpy(x.data(), dx.p, sizeof(double) * x.size(), hipMemcpyDeviceToHost) == hipSuccess &&
-       hipMemcpy(vn.data(), dvn.p, sizeof(double) * vn.size(), hipMemcpyDeviceToHost) == hipSuccess &&
-       hipMemcpy(fn.data(), dfn.p, sizeof(double) * fn.size(), hipMemcpyDeviceToHost) == hipSuccess &&
-       hipMemcpy(fx.data(), dfx.p, sizeof(double) * fx.size(), hipMemcpyDeviceToHost) == hipSuccess &&
-       hipMemcpy(ar.data(), dar.p, sizeof(double) * nf, hipMemcpyDeviceToHost) == hipSuccess &&
-       hipMemcpy(&flag, dflag.p, sizeof(int), hipMemcpyDeviceToHost) == hipSuccess &&
-       hipMemcpy(&logf, dlogf.p, sizeof(double), hipMemcpyDeviceToHost) == hipSuccess;
-  if (!ok) { set_err(-4, "realize_mod kernels"); return; }
+       hipMemcpy(x, B.x.p, sizeof(double) * 3 * nv, hipMemcpyDeviceToHost) == hipSuccess &&
+       hipMemcpy(vn, B.vn.p, sizeof(double) * 3 * nv, hipMemcpyDeviceToHost) == hipSuccess &&
+       hipMemcpy(fn, B.fn.p, sizeof(double) * 3 * nf, hipMemcpyDeviceToHost) == hipSuccess &&
+       hipMemcpy(fx, B.fx.p, sizeof(double) * 3 * nf, hipMemcpyDeviceToHost) == hipSuccess &&
+       hipMemcpy(ar, B.ar.p, sizeof(double) * nf, hipMemcpyDeviceToHost) == hipSuccess &&
+       hipMemcpy(&flag, dflag, sizeof(int), hipMemcpyDeviceToHost) == hipSuccess &&
+       hipMemcpy(&logf, dlogf, sizeof(double), hipMemcpyDeviceToHost) == hipSuccess;
+  if (!ok) { (void)hipGetLastError(); set_err(-4, "realize_mod kernels"); return; }
   for (int i = 0; i < nv; i++)
     for (int k = 0; k < 3; k++) { rv->v[i].x[k] = x[3 * i + k]; rv->v[i].n[k] = vn[3 * i + k]; }
   for (int f = 0; f < nf; f++) {
@@ -268,9 +541,9 @@ extern "C" void realize_mod(struct par_t *par, struct mod_t *mod) {
     rv->f[f].area = ar[f];
   }
   for (int j = 0; j < 3; j++) {
-    double s = pc[j];
-    if (s <= 0.01) s = 0.01 / (1 + 0.01 - s);
-    rv->scalefactor[j].val = s;
+    double sf = pc[j];
+    if (sf <= 0.01) sf = 0.01 / (1 + 0.01 - sf);
+    rv->scalefactor[j].val = sf;
   }
   par->baddiam = (unsigned char)flag;
   par->baddiam_logfactor = logf;

@@ -26,6 +26,7 @@ int Engine::alloc(void **p, size_t bytes) {
   if (bytes == 0) bytes = 8;
   SHP_CHECK(hipMalloc(p, bytes));
   allocs.push_back(*p);
+  dev_bytes += bytes;
   return 0;
 }
 
@@ -37,6 +38,7 @@ static int upload(Engine *E, T **dptr, const std::vector<T> &h) {
   if (bytes == 0) bytes = sizeof(T);
   if (hipMalloc(&p, bytes) != hipSuccess) return -2;
   E->allocs.push_back(p);
+  E->dev_bytes += bytes;
   if (!h.empty() && hipMemcpy(p, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice) != hipSuccess) rc = -3;
   *dptr = (T *)p;
   return rc;
@@ -77,7 +79,9 @@ struct ModelPack {
   double dof = 0, dof_dd = 0, dof_dop = 0, dof_lc = 0;
 };
 
-static std::vector<ModelPack> g_packs;  // scratch used during build
+// Packing scratch, per host thread: the one-dataset drop-ins (calc_deldop
+// and co.) are called from several host threads at once.
+static thread_local std::vector<ModelPack> g_packs;
 
 namespace {
 struct HostStage {
@@ -108,7 +112,9 @@ static void set_param_target(std::map<const double *, ParamTarget> &m, const dou
   m[addr] = ParamTarget{0, model, off};
 }
 
-int Engine::pack_model(int mi, struct par_t *par, struct mod_t *mod, struct dat_t *dat) {
+// repack: only what changes between calls of a drop-in (parameters, views,
+// frame scalars, realized geometry); the topology stays as built.
+int Engine::pack_model(int mi, struct par_t *par, struct mod_t *mod, struct dat_t *dat, bool repack) {
   ModelPack &P = g_packs[mi];
   auto &tmap = model_params[mi];
   if (mod->shape.ncomp != 1) return -10;  // the engine handles single-component models
@@ -234,6 +240,7 @@ int Engine::pack_model(int mi, struct par_t *par, struct mod_t *mod, struct dat_
   P.nf = rv->nf;
   P.verts = rv->v;
   if (opt.views_only) P.nv = P.nf = 0;
+  if (!repack) {
   P.fv.resize(3 * (size_t)P.nf);
   for (int f = 0; f < P.nf; f++)
     for (int k = 0; k < 3; k++) P.fv[3 * f + k] = rv->f[f].v[k];
@@ -249,6 +256,7 @@ int Engine::pack_model(int mi, struct par_t *par, struct mod_t *mod, struct dat_
     for (int k = 0; k < 3; k++) { P.u[3 * i + k] = v->u[k]; P.a[3 * i + k] = v->a[k]; }
     int src = v->r.state == '=' ? v->v_mirror : i;
     P.r[i] = rv->v[src].r.val;
+  }
   }
   if (opt.host_geometry) {
     P.hx.resize(3 * (size_t)P.nv); P.hvn.resize(3 * (size_t)P.nv); P.hfn.resize(3 * (size_t)P.nf);
@@ -269,6 +277,7 @@ int Engine::pack_model(int mi, struct par_t *par, struct mod_t *mod, struct dat_
   };
   auto copy9 = [](double *d, double m[3][3]) { for (int i = 0; i < 3; i++) for (int j = 0; j < 3; j++) d[3 * i + j] = m[i][j]; };
   for (int s = 0; s < nsets; s++) {
+    if (opt.only_set >= 0 && s != opt.only_set) continue;
     struct set_t *set = &dat->set[s];
     int pb_set = off_set + PB_SET_STRIDE * s;
     int pb_dc = dc_src[s] >= 0 ? off_set + PB_SET_STRIDE * dc_src[s] : pb_set;
@@ -595,18 +604,7 @@ int Engine::build(struct par_t *par, struct mod_t **mods, struct dat_t **dats, i
       if ((rc = alloc((void **)&G.fvn, sizeof(float4) * 3 * (size_t)G.nf))) return rc;
     }
     SHP_CHECK(hipMemset(G.seen, 0, G.nf));
-    if (opt.host_geometry) {
-      SHP_CHECK(hipMemcpy(G.x, P.hx.data(), sizeof(double) * 3 * G.nv, hipMemcpyHostToDevice));
-      SHP_CHECK(hipMemcpy(G.vn, P.hvn.data(), sizeof(double) * 3 * G.nv, hipMemcpyHostToDevice));
-      SHP_CHECK(hipMemcpy(G.fn, P.hfn.data(), sizeof(double) * 3 * G.nf, hipMemcpyHostToDevice));
-      std::vector<float4> t4(std::max(G.nv, G.nf));
-      for (int i = 0; i < G.nv; i++) t4[i] = make_float4((float)P.hx[3 * i], (float)P.hx[3 * i + 1], (float)P.hx[3 * i + 2], 0.f);
-      SHP_CHECK(hipMemcpy(G.x32, t4.data(), sizeof(float4) * G.nv, hipMemcpyHostToDevice));
-      for (int i = 0; i < G.nv; i++) t4[i] = make_float4((float)P.hvn[3 * i], (float)P.hvn[3 * i + 1], (float)P.hvn[3 * i + 2], 0.f);
-      SHP_CHECK(hipMemcpy(G.vn32, t4.data(), sizeof(float4) * G.nv, hipMemcpyHostToDevice));
-      for (int i = 0; i < G.nf; i++) t4[i] = make_float4((float)P.hfn[3 * i], (float)P.hfn[3 * i + 1], (float)P.hfn[3 * i + 2], 0.f);
-      SHP_CHECK(hipMemcpy(G.fn32, t4.data(), sizeof(float4) * G.nf, hipMemcpyHostToDevice));
-    }
+    if (opt.host_geometry && (rc = upload_host_geometry(s))) return rc;
   }
   if ((rc = upload(this, &d_slots, h_slots))) return rc;
   if ((rc = upload(this, &d_views, h_views))) return rc;
@@ -663,6 +661,8 @@ int Engine::build(struct par_t *par, struct mod_t **mods, struct dat_t **dats, i
   }
   if ((rc = upload(this, &d_lc_x, x_all))) return rc;
   if ((rc = upload(this, &d_lc_t, t_all))) return rc;
+  lcx_len = (long long)x_all.size();
+  lct_len = (long long)t_all.size();
   if ((rc = alloc((void **)&d_lc_y, sizeof(double) * std::max(1LL, y_cursor)))) return rc;
   if ((rc = alloc((void **)&d_lc_y2, sizeof(double) * std::max(1LL, y2_cursor)))) return rc;
   if ((rc = alloc((void **)&d_fit, sizeof(double) * std::max(1LL, fit_len)))) return rc;
@@ -744,7 +744,7 @@ int Engine::build(struct par_t *par, struct mod_t **mods, struct dat_t **dats, i
   pd.stale_cap = (int)std::min<long long>(std::max<long long>((long long)P / 32, 4096), 64LL << 20);
   if ((rc = alloc((void **)&pd.stale, sizeof(StaleRec) * (size_t)pd.stale_cap))) return rc;
   // counters: [0] replay records, [1] error flags, [2] records to collect,
-  // [3] wide footprints, [4] blocks reaching outside their POS window
+  // [3] unused (0), [4] blocks reaching outside their POS window
   if ((rc = alloc((void **)&pd.nstale, sizeof(int) * 8))) return rc;
   pd.error = pd.nstale + 1;
   pd.ncollect = pd.nstale + 2;
@@ -755,8 +755,99 @@ int Engine::build(struct par_t *par, struct mod_t **mods, struct dat_t **dats, i
   return 0;
 }
 
+// the realized geometry the caller's structs hold (drop-in mode), FP64 and
+// the FP32 copies, into slot s (g_packs holds the packed model)
+int Engine::upload_host_geometry(int s) {
+  GeomSlot &G = h_slots[s];
+  const ModelPack &P = g_packs[G.model];
+  SHP_CHECK(hipMemcpy(G.x, P.hx.data(), sizeof(double) * 3 * G.nv, hipMemcpyHostToDevice));
+  SHP_CHECK(hipMemcpy(G.vn, P.hvn.data(), sizeof(double) * 3 * G.nv, hipMemcpyHostToDevice));
+  SHP_CHECK(hipMemcpy(G.fn, P.hfn.data(), sizeof(double) * 3 * G.nf, hipMemcpyHostToDevice));
+  std::vector<float4> t4(std::max(G.nv, G.nf));
+  for (int i = 0; i < G.nv; i++) t4[i] = make_float4((float)P.hx[3 * i], (float)P.hx[3 * i + 1], (float)P.hx[3 * i + 2], 0.f);
+  SHP_CHECK(hipMemcpy(G.x32, t4.data(), sizeof(float4) * G.nv, hipMemcpyHostToDevice));
+  for (int i = 0; i < G.nv; i++) t4[i] = make_float4((float)P.hvn[3 * i], (float)P.hvn[3 * i + 1], (float)P.hvn[3 * i + 2], 0.f);
+  SHP_CHECK(hipMemcpy(G.vn32, t4.data(), sizeof(float4) * G.nv, hipMemcpyHostToDevice));
+  for (int i = 0; i < G.nf; i++) t4[i] = make_float4((float)P.hfn[3 * i], (float)P.hfn[3 * i + 1], (float)P.hfn[3 * i + 2], 0.f);
+  SHP_CHECK(hipMemcpy(G.fn32, t4.data(), sizeof(float4) * G.nf, hipMemcpyHostToDevice));
+  return 0;
+}
+
+int Engine::refresh(struct par_t *par, struct mod_t *mod, struct dat_t *dat) {
+  if (nevals != 1 || h_evals.size() != 1 || opt.views_only) return 1;
+  g_packs.assign(1, ModelPack());
+  if (model_params.empty()) model_params.assign(1, {});
+  int rc = pack_model(0, par, mod, dat, true);
+  if (rc) {
+    g_packs.clear();
+    return rc;
+  }
+  ModelPack &P = g_packs[0];
+  const GeomSlot &G0 = h_slots[h_eval_slot[0]];
+  bool same = (int)P.views.size() == nviews && (int)P.frames.size() == nframes && (int)P.base.size() <= pb_stride &&
+              P.nv == G0.nv && P.nf == G0.nf && (long long)P.lc_x.size() == lcx_len &&
+              (long long)P.lc_t.size() == lct_len;
+  for (int q = 0; same && q < nviews; q++) {
+    const ViewRec &A = h_views[q], &B = P.views[q];
+    same = A.type == B.type && A.n == B.n && A.ndel == B.ndel && A.ndop == B.ndop && A.nviews == B.nviews &&
+           A.law_type == B.law_type && A.set == B.set && A.frm == B.frm && A.v == B.v;
+  }
+  for (int q = 0; same && q < nframes; q++) {
+    const FrameRec &A = h_frames[q], &B = P.frames[q];
+    same = A.type == B.type && A.ndel == B.ndel && A.ndop == B.ndop && A.nviews == B.nviews && A.lc_n == B.lc_n &&
+           A.lc_ncalc == B.lc_ncalc && A.lc_nviews == B.lc_nviews;
+  }
+  if (!same) {
+    g_packs.clear();
+    return 1;
+  }
+  // views: the caller's description, the engine's offsets
+  for (int q = 0; q < nviews; q++) {
+    ViewRec V = P.views[q];
+    const ViewRec &O = h_views[q];
+    V.eval = O.eval; V.slot = O.slot; V.pb = O.pb; V.frame_id = O.frame_id; V.pos_off = O.pos_off;
+    V.tile_off = O.tile_off; V.bin_off = O.bin_off; V.bin_ntx = O.bin_ntx; V.binned = O.binned;
+    V.fp64_skip = O.fp64_skip;
+    h_views[q] = V;
+  }
+  for (int q = 0; q < nframes; q++) {
+    FrameRec F = P.frames[q];
+    const FrameRec &O = h_frames[q];
+    F.eval = O.eval; F.first_view = O.first_view; F.off = O.off; F.obs_off = O.obs_off;
+    F.lc_x_off = O.lc_x_off; F.lc_t_off = O.lc_t_off; F.lc_y_off = O.lc_y_off; F.lc_y2_off = O.lc_y2_off;
+    h_frames[q] = F;
+  }
+  EvalRec &E = h_evals[0];
+  E.dof = P.dof; E.dof_deldop = P.dof_dd; E.dof_doppler = P.dof_dop; E.dof_lghtcrv = P.dof_lc;
+  E.bad_objfactor = par->bad_objfactor;
+  std::copy(P.base.begin(), P.base.end(), h_params.begin());
+  std::copy(P.base.begin(), P.base.end(), h_params.begin() + (size_t)pb_stride);
+  if (h_pin) memcpy(h_pin, h_params.data(), sizeof(double) * h_params.size());
+  nsinc2 = par->nsinc2;
+  sinc2width = par->sinc2width;
+  mask_tol = par->mask_tol;
+  mincosine_seen = par->mincosine_seen;
+  exclude_seen = par->exclude_seen;
+  SHP_CHECK(hipMemcpy(d_views, h_views.data(), sizeof(ViewRec) * nviews, hipMemcpyHostToDevice));
+  SHP_CHECK(hipMemcpy(d_frames, h_frames.data(), sizeof(FrameRec) * nframes, hipMemcpyHostToDevice));
+  SHP_CHECK(hipMemcpy(d_evals, h_evals.data(), sizeof(EvalRec), hipMemcpyHostToDevice));
+  if (!P.lc_x.empty()) SHP_CHECK(hipMemcpy(d_lc_x, P.lc_x.data(), sizeof(double) * P.lc_x.size(), hipMemcpyHostToDevice));
+  if (!P.lc_t.empty()) SHP_CHECK(hipMemcpy(d_lc_t, P.lc_t.data(), sizeof(double) * P.lc_t.size(), hipMemcpyHostToDevice));
+  if (opt.host_geometry && (rc = upload_host_geometry(h_eval_slot[0]))) {
+    g_packs.clear();
+    return rc;
+  }
+  // o2 of every frame from the observations uploaded at build
+  hipLaunchKernelGGL(k_frame_o2, dim3(std::max(1, nframes)), dim3(256), 0, stream, d_frames, nframes, d_obs, d_oov);
+  SHP_CHECK(hipGetLastError());
+  g_packs.clear();
+  return 0;
+}
+
 int Engine::set_values(int e, const std::vector<double> &values) {
   if (e < 0 || e >= nevals || values.size() != eval_targets[e].size()) return -20;
+  // a replay in flight reads the pinned copy in its first step
+  SHP_CHECK(hipStreamSynchronize(stream));
   for (size_t q = 0; q < values.size(); q++) {
     const ParamTarget &T = eval_targets[e][q];
     if (T.kind == 0) {
@@ -781,6 +872,8 @@ static inline unsigned cdiv(long long a, long long b) { return (unsigned)((a + b
 
 int Engine::enqueue() {
   const int T = 256;
+  // an error left by an earlier, already reported call must not fail this one
+  (void)hipGetLastError();
   dim3 gv(cdiv(nviews, 64)), bv(64);
   SHP_LAUNCH(*this, "k_zero_counters", k_zero_counters, dim3(1), dim3(64), 0, stream, pd.nstale, 8);
   if (opt.facet_map) {  // owners of this replay only
@@ -933,7 +1026,7 @@ static int pos_to_host(hipStream_t st, T **dst, const T *src, long long off, int
   return 0;
 }
 
-int Engine::writeback(int e, struct par_t *par, struct mod_t *mod, struct dat_t *dat, int write_pos) {
+int Engine::writeback(int e, struct par_t *par, struct mod_t *mod, struct dat_t *dat, int write_pos, int accumulate) {
   std::vector<FrameRec> fr;
   std::vector<EvalRec> ev;
   std::vector<double> vo;
@@ -1030,19 +1123,49 @@ int Engine::writeback(int e, struct par_t *par, struct mod_t *mod, struct dat_t 
       }
     }
   }
-  // par flags (calc_fits.c:349-354, 407-409)
-  par->posbnd = (unsigned char)E.posbnd;
-  par->badradar = (unsigned char)E.badradar;
-  par->badposet = 0;
-  par->posbnd_logfactor = E.posbnd_logfactor;
-  par->badradar_logfactor = E.badradar_logfactor;
-  par->badposet_logfactor = 0.0 / dat->dof_poset;
-  // seen flags
+  if (!accumulate) {
+    // par flags (calc_fits.c:349-354, 407-409)
+    par->posbnd = (unsigned char)E.posbnd;
+    par->badradar = (unsigned char)E.badradar;
+    par->badposet = 0;
+    par->posbnd_logfactor = E.posbnd_logfactor;
+    par->badradar_logfactor = E.badradar_logfactor;
+    par->badposet_logfactor = 0.0 / dat->dof_poset;
+  } else {
+    // one dataset: flags raised and un-normalized log factors added
+    // (calc_fits.c:509-513, :539-550, :1650-1660, :1947)
+    for (int q = 0; q < E.nframes; q++) {
+      const FrameRec &F = fr[E.first_frame + q];
+      if (F.type == VT_LGHTCRV) {
+        double lf = 0.0;
+        for (int i = 0; i < F.lc_ncalc; i++) {
+          const double *o = vo.data() + (size_t)(F.first_view + i) * VIEW_OUT;
+          if (o[OUT_OUTBND] != 0.0) { par->posbnd = 1; lf += 0.5 * o[OUT_POSBND_LOG]; }
+          if (o[OUT_OUTBND + 1] != 0.0) { par->posbnd = 1; lf += 0.5 * o[OUT_POSBND_LOG + 1]; }
+        }
+        par->posbnd_logfactor += F.dof * (lf / F.lc_ncalc);
+        continue;
+      }
+      for (int v = 0; v < F.nviews; v++) {
+        const ViewRec &V = vw[F.first_view + v];
+        const double *o = vo.data() + (size_t)(F.first_view + v) * VIEW_OUT;
+        if (V.is_v0 && o[OUT_OUTBND] != 0.0) {
+          par->posbnd = 1;
+          par->posbnd_logfactor += F.dof * o[OUT_POSBND_LOG];
+        }
+        if (o[OUT_BADRADAR] != 0.0) {
+          par->badradar = 1;
+          par->badradar_logfactor += F.dof * o[OUT_BADRADAR_LOG] / F.nviews;
+        }
+      }
+    }
+  }
+  // seen flags (calc_fits resets them first; one dataset only raises them)
   const GeomSlot &G = h_slots[h_eval_slot[e]];
   std::vector<unsigned char> seen(G.nf);
   SHP_CHECK(hipMemcpy(seen.data(), G.seen, G.nf, hipMemcpyDeviceToHost));
   struct vertices_t *rv = &mod->shape.comp[0].real;
-  for (int f = 0; f < G.nf && f < rv->nf; f++) rv->f[f].seen = seen[f];
+  for (int f = 0; f < G.nf && f < rv->nf; f++) rv->f[f].seen = accumulate ? (rv->f[f].seen | seen[f]) : seen[f];
   // POS arrays of every view (with pos_scope GLOBAL the last view's survive)
   if (write_pos && pd.z) {
     for (int v = 0; v < nviews; v++) {
@@ -1064,9 +1187,25 @@ int Engine::writeback(int e, struct par_t *par, struct mod_t *mod, struct dat_t 
         if ((rc = pos_to_host(stream, pos->cosill, pd.cosill, V.pos_off, V.n))) return rc;
         if ((rc = pos_to_host(stream, pos->fill, pd.fill, V.pos_off, V.n))) return rc;
       }
-      // body / component maps of a single-body, single-component model (posvis.c:284-286)
+      // Pixels outside the rectangles are blank sky, as posclr leaves them
+      // (the device arrays may hold an earlier call's pixels there); body /
+      // component maps of a single-body, single-component model
+      // (posvis.c:284-286).
       for (int i = -V.n; i <= V.n; i++)
         for (int j = -V.n; j <= V.n; j++) {
+          if (i < V.xlim[0] || i > V.xlim[1] || j < V.ylim[0] || j > V.ylim[1]) {
+            pos->z[i][j] = -1.0e20;
+            pos->cose[i][j] = 0.0;
+            pos->f[i][j] = -1;
+            if (V.bistatic) { pos->cosi[i][j] = 0.0; pos->b[i][j] = 0.0; }
+          }
+          // brightness only where lit and illuminated (apply_photo.c:86-93)
+          if (V.bistatic && !(pos->cose[i][j] > 0.0 && pos->cosi[i][j] > 0.0)) pos->b[i][j] = 0.0;
+          if (V.bistatic && (i < V.xlim2[0] || i > V.xlim2[1] || j < V.ylim2[0] || j > V.ylim2[1])) {
+            pos->zill[i][j] = -1.0e20;
+            pos->cosill[i][j] = 0.0;
+            pos->fill[i][j] = -1;
+          }
           int own = pos->f[i][j] >= 0 ? 0 : -1;
           pos->body[i][j] = pos->comp[i][j] = own;
           if (V.bistatic) pos->bodyill[i][j] = pos->compill[i][j] = pos->fill[i][j] >= 0 ? 0 : -1;

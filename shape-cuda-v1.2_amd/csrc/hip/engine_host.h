@@ -64,6 +64,7 @@ struct RunOptions {
   int views_only = 0;       // pack parameters and views only (spin / offset drop-ins)
   int global_zbuf = 0;      // mixed method: rasterize every view through the global z-buffer
   int facet_map = 0;        // keep every POS pixel's owner facet for inspection
+  int only_set = -1;        // >= 0: views and frames of this dataset only (calc_deldop & co.)
 };
 
 class Engine {
@@ -72,7 +73,15 @@ class Engine {
   // Pack one scene per model; evaluations reference models.  Returns 0 or an error code.
   int build(struct par_t *par, struct mod_t **mods, struct dat_t **dats, int nmodels,
             const std::vector<EvalSpec> &evals, const RunOptions &opt);
+  // Re-pack what a single-model, single-evaluation engine reads from the
+  // caller's structs on every call -- views (epochs, spin state, frames of
+  // reference), frame scalars, parameter values, realized geometry when
+  // opt.host_geometry -- into the existing device arenas.  Returns 1 when
+  // the structs no longer have the shape the engine was built for (the
+  // caller then builds a new engine), 0 on success, < 0 on a device error.
+  int refresh(struct par_t *par, struct mod_t *mod, struct dat_t *dat);
   // Rewrite the trial values of evaluation e (same targets as at build time).
+  // Waits for a replay in flight first: the replay's first step reads them.
   int set_values(int e, const std::vector<double> &values);
   int upload_params();
   // Enqueue one full evaluation of every evaluation on the stream.
@@ -85,8 +94,12 @@ class Engine {
   int download_frames(std::vector<FrameRec> &out);
   int download_view_out(std::vector<double> &out);
   int download_views(std::vector<ViewRec> &out);
-  // write fits, frame scalars, view data and flags of evaluation e into the host structs
-  int writeback(int e, struct par_t *par, struct mod_t *mod, struct dat_t *dat, int write_pos);
+  // Write fits, frame scalars, view data and flags of evaluation e into the
+  // host structs.  accumulate = 0 (calc_fits.c:340-409): seen flags and the
+  // par flags and log factors are set; accumulate = 1 (one dataset,
+  // calc_deldop / calc_doppler / calc_lghtcrv): seen flags are or-ed in and
+  // the flags and un-normalized log factors added, as those routines do.
+  int writeback(int e, struct par_t *par, struct mod_t *mod, struct dat_t *dat, int write_pos, int accumulate = 0);
   int error_flags();
 
   hipStream_t stream = nullptr;
@@ -98,6 +111,7 @@ class Engine {
   int exclude_seen = -1;
   int last_nstale = 0;      // pixels replayed in the last evaluation (diagnostic)
   long long total_pix = 0, fit_len = 0, obs_len = 0, total_tiles = 0;
+  long long lcx_len = 0, lct_len = 0;  // lightcurve epochs: calculated points, observed points x views
   bool any_lc = false, any_dd = false, any_dop = false;
   // mixed method: views taking the binned tile kernel / the global z-buffer
   bool any_binned = false, any_global = false;
@@ -128,6 +142,7 @@ class Engine {
   int2 *d_out_list = nullptr;               // (view, block) of blocks reaching outside the POS window
   PosDev pd{};
   std::vector<void *> allocs;
+  size_t dev_bytes = 0;     // device memory held by allocs
   hipGraph_t graph = nullptr;
   hipGraphExec_t graph_exec = nullptr;
   // Optional timing of eager launches: HIP events around every kernel on
@@ -145,7 +160,8 @@ class Engine {
   std::vector<const char *> t_names;
   int t_used = 0;
   int alloc(void **p, size_t bytes);
-  int pack_model(int m, struct par_t *par, struct mod_t *mod, struct dat_t *dat);
+  int pack_model(int m, struct par_t *par, struct mod_t *mod, struct dat_t *dat, bool repack = false);
+  int upload_host_geometry(int slot);
 };
 
 // record a failure of a reference entry point for shape_hip_last_error()

@@ -122,7 +122,6 @@ __global__ __launch_bounds__(256) void k_shade_radar32(ViewRec *__restrict__ vie
     RadarPix R;
     radar_pixel(V, pl, i, j, (double)key_z(k), cl, R);
     if (!R.inb) anyo = 1;
-    if (R.wide) { atomicAdd(&counters[3], 1); continue; }
     if (use_lds && R.inb) {
       radar_scatter_lds(V, R, win + (dd ? (R.id0 - lo_d) * wp : 0) + (R.ip0 - lo_p), wp);
     } else {

@@ -545,7 +545,6 @@ __global__ __launch_bounds__(256, 6) void k_tile32(const ViewRec *__restrict__ v
     RadarPix R;
     radar_pixel(V, pl, i, j, (double)key_z(k), nz, R);  // (binned views: diffuse cosine law)
     if (!R.inb) anyo = 1;
-    if (R.wide) { atomicAdd(&counters[3], 1); continue; }
     if (use_lds && R.inb) {
       radar_scatter_lds(V, R, win + (dd ? (R.id0 - lo_d) * wp : 0) + (R.ip0 - lo_p), wp);
     } else {

@@ -10,7 +10,10 @@ namespace shp {
 constexpr int TILE = 16;         // resolve tiles: 16x16 pixels per workgroup of 256 threads
 constexpr int TPP = TILE * TILE / 256;  // pixels per thread
 constexpr int WIN_MAX = 2048;    // LDS delay-Doppler window (FP64 bins, 16 KiB)
-constexpr int DW = 4, PW = 16;   // register footprint limits (delay, Doppler bins)
+// Footprints of at most DW delay rows x PW Doppler bins take the unrolled
+// register scatter; wider ones (several samples per baud, subpixel sinc^2
+// means over pixels spanning many bins) take a general loop.
+constexpr int DW = 4, PW = 16;
 
 // Workgroups are dealt to the chip's 8 XCDs round-robin by linear block id.
 // With the grid's x size nb a multiple of 8 (pad_xcd), remap so that XCD k
@@ -403,7 +406,8 @@ __device__ inline float radar_cl(const ViewRec &V, const GeomSlot &G, const doub
 // the point of use, so no weight array stays live) and the amplitude of
 // cross section per unit weight.
 struct RadarPix {
-  bool inb, wide;
+  bool inb;
+  bool wide;         // more than DW rows or PW bins: the general scatter loop
   bool sinc1;        // single-sample sinc^2 Doppler response
   float amp, delrel, doprel;
   int id0, id1, ip0, ip1;
@@ -482,14 +486,11 @@ __device__ inline void radar_pixel(const ViewRec &V, const double *pl, int i, in
     R.s2 = sp * sp * (1.0f / (3.14159265358979f * 3.14159265358979f));
   }
   float sump = 0.0f;
-  if (np >= 1 && np <= PW) {
-    if (R.sinc1)
-      for (int b = 0; b < np; b++) sump += dop_weight(R.pfr, R.pm0, R.s2, b);
-    else
-      for (int b = 0; b < np; b++) sump += dop_w_sub(V, R.pfr, R.pm0, b);
-  } else {
-    R.wide = true;
-  }
+  if (R.sinc1)
+    for (int b = 0; b < np; b++) sump += dop_weight(R.pfr, R.pm0, R.s2, b);
+  else
+    for (int b = 0; b < np; b++) sump += dop_w_sub(V, R.pfr, R.pm0, b);
+  R.wide = np > PW;
   float sumd = 1.0f;
   R.delrel = 0.0f;
   R.id0 = 0;
@@ -506,18 +507,16 @@ __device__ inline void radar_pixel(const ViewRec &V, const double *pl, int i, in
     R.dm0 = (int)dfl - (V.D0 + R.id0);
     const int nd = R.id1 - R.id0 + 1;
     sumd = 0.0f;
-    if (nd <= DW) {
-      for (int a = 0; a < nd; a++) sumd += del_w(V, R, a);
-    } else {
-      R.wide = true;
-    }
+    for (int a = 0; a < nd; a++) sumd += del_w(V, R, a);
+    R.wide = R.wide || nd > DW;
     R.inb = (V.D0 + R.id0 >= 1) && (V.D0 + R.id1 <= V.ndel) && (V.P0 + R.ip0 >= 1) && (V.P0 + R.ip1 <= V.ndop);
   } else {
     R.inb = (V.P0 + R.ip0 >= 1) && (V.P0 + R.ip1 <= V.ndop);
   }
-  R.amp = R.wide ? 0.0f
-                  : (V.law_type == COSINELAW_DIFF ? cosine_law32(pl, cl) : cl) * V.kmpp32 * V.kmpp32 * codef *
-                        __builtin_amdgcn_rcpf(sumd * sump);
+  // (an empty Doppler range, sinc2width 0, adds nothing, as on the CPU)
+  R.amp = np < 1 ? 0.0f
+                 : (V.law_type == COSINELAW_DIFF ? cosine_law32(pl, cl) : cl) * V.kmpp32 * V.kmpp32 * codef *
+                       __builtin_amdgcn_rcpf(sumd * sump);
 }
 
 // scatter one pixel straight into the HBM fit image (FP64 atomics) or, for
@@ -525,7 +524,6 @@ __device__ inline void radar_pixel(const ViewRec &V, const double *pl, int i, in
 __device__ inline void radar_scatter_global(const ViewRec &V, const RadarPix &R, double *img, double *ov) {
   const bool dd = V.type == VT_DELAY;
   int np = R.ip1 - R.ip0 + 1, nd = dd ? R.id1 - R.id0 + 1 : 1, ndop = V.ndop;
-  if (R.wide) return;
   if (R.inb) {
     for (int a = 0; a < nd; a++) {
       const float ad = dd ? R.amp * del_w(V, R, a) : R.amp;
@@ -550,26 +548,31 @@ __device__ inline void radar_scatter_global(const ViewRec &V, const RadarPix &R,
 __device__ inline void radar_scatter_lds(const ViewRec &V, const RadarPix &R, double *origin, int wp) {
   const bool dd = V.type == VT_DELAY;
   const int np = R.ip1 - R.ip0 + 1, nd = dd ? R.id1 - R.id0 + 1 : 1;
-  float ad[DW];
+  int b0 = threadIdx.x & 15;
+  while (b0 >= np) b0 -= np;
+  // rows in groups of DW (one group unless the footprint is wide)
+  for (int a0 = 0; a0 < nd; a0 += DW) {
+    float ad[DW];
 #pragma unroll
-  for (int a = 0; a < DW; a++) ad[a] = (a < nd) ? (dd ? R.amp * del_w(V, R, a) : R.amp) : 0.0f;
-  int b = threadIdx.x & 15;
-  while (b >= np) b -= np;
-  if (R.sinc1) {
-    for (int st = 0; st < np; st++) {
-      const float pwb = dop_weight(R.pfr, R.pm0, R.s2, b);
+    for (int a = 0; a < DW; a++) ad[a] = (a0 + a < nd) ? (dd ? R.amp * del_w(V, R, a0 + a) : R.amp) : 0.0f;
+    double *org = origin + a0 * wp;
+    int b = b0;
+    if (R.sinc1) {
+      for (int st = 0; st < np; st++) {
+        const float pwb = dop_weight(R.pfr, R.pm0, R.s2, b);
 #pragma unroll
-      for (int a = 0; a < DW; a++)
-        if (a < nd) atomicAdd(&origin[a * wp + b], (double)(ad[a] * pwb));
-      if (++b == np) b = 0;
-    }
-  } else {
-    for (int st = 0; st < np; st++) {
-      const float pwb = dop_w_sub(V, R.pfr, R.pm0, b);
+        for (int a = 0; a < DW; a++)
+          if (a0 + a < nd) atomicAdd(&org[a * wp + b], (double)(ad[a] * pwb));
+        if (++b == np) b = 0;
+      }
+    } else {
+      for (int st = 0; st < np; st++) {
+        const float pwb = dop_w_sub(V, R.pfr, R.pm0, b);
 #pragma unroll
-      for (int a = 0; a < DW; a++)
-        if (a < nd) atomicAdd(&origin[a * wp + b], (double)(ad[a] * pwb));
-      if (++b == np) b = 0;
+        for (int a = 0; a < DW; a++)
+          if (a0 + a < nd) atomicAdd(&org[a * wp + b], (double)(ad[a] * pwb));
+        if (++b == np) b = 0;
+      }
     }
   }
 }

@@ -340,6 +340,8 @@ void shp_spec_defaults(shp_spec *s)
   s->radar_nviews = 1;
   s->view_interval_days = 0.0;
   s->lc_nviews = 1;
+  s->nsinc2 = 1;
+  s->sinc2width = 6;
 }
 
 /* Euler-angle matrix with the reference's convention (util/euler.c). */
@@ -795,8 +797,8 @@ static void build_par(shp_scene *sc)
   p->pos_width = sp->km_per_pixel * 2 * sp->pos_n;
   p->dd_gamma = 1.0;
   p->mask_tol = p->pos_width / (p->pos_pixels - 1);
-  p->nsinc2 = 1;
-  p->sinc2width = 6;
+  p->nsinc2 = sp->nsinc2 > 0 ? sp->nsinc2 : 1;
+  p->sinc2width = sp->sinc2width > 0 ? sp->sinc2width : 6;
   p->exclude_seen = -1;
   p->mincosine_seen = 0.0;
   p->int_abstol = 1.0e-7;
@@ -1126,6 +1128,44 @@ void shp_geometry(struct mod_t *mod, double *vx, double *vn, double *fn, double 
     for (int k = 0; k < 3; k++) if (fn) fn[3 * f + k] = rv->f[f].n[k];
     if (area) area[f] = rv->f[f].area;
   }
+}
+
+/* angle (which = 0) or spin (which = 1) offset component j of dataset s:
+   set its state and value (state 0 leaves the state), read them back */
+void shp_set_offset(struct dat_t *dat, int s, int which, int j, int state, double val)
+{
+  struct param_t *p = which ? &dat->set[s].omegaoff[j] : &dat->set[s].angleoff[j];
+  if (state) p->state = (char)state;
+  p->val = val;
+}
+double shp_get_offset(struct dat_t *dat, int s, int which, int j)
+{
+  return which ? dat->set[s].omegaoff[j].val : dat->set[s].angleoff[j].val;
+}
+
+/* what calc_fits does around its per-dataset calls (calc_fits.c:349-361,
+   :407-409): reset the flags, log factors and seen flags (start = 1), or
+   normalize the summed log factors by the degrees of freedom (start = 0) */
+void shp_fits_bracket(struct par_t *par, struct mod_t *mod, struct dat_t *dat, int start)
+{
+  if (start) {
+    par->posbnd = par->badposet = par->badradar = 0;
+    par->posbnd_logfactor = par->badposet_logfactor = par->badradar_logfactor = 0.0;
+    for (int c = 0; c < mod->shape.ncomp; c++)
+      for (int f = 0; f < mod->shape.comp[c].real.nf; f++) mod->shape.comp[c].real.f[f].seen = 0;
+  } else {
+    par->posbnd_logfactor /= dat->dof;
+    par->badposet_logfactor /= dat->dof_poset;
+    par->badradar_logfactor /= (dat->dof_deldop + dat->dof_doppler);
+  }
+}
+
+/* overwrite the realized vertex normals of component 0 (3*nv doubles) */
+void shp_set_vertex_normals(struct mod_t *mod, const double *vn)
+{
+  struct vertices_t *rv = &mod->shape.comp[0].real;
+  for (int i = 0; i < rv->nv; i++)
+    for (int k = 0; k < 3; k++) rv->v[i].n[k] = vn[3 * i + k];
 }
 
 void shp_facet_seen(struct mod_t *mod, unsigned char *out)

@@ -70,6 +70,9 @@ typedef struct shp_spec {
   int radar_nviews;
   double view_interval_days;
   int lc_nviews;           /* smearing of lightcurve points (same interval) */
+  /* Doppler response: sinc^2 evaluated at nsinc2 x nsinc2 points per POS
+     pixel over sinc2width bins (par_t nsinc2 / sinc2width) */
+  int nsinc2, sinc2width;
 } shp_spec;
 
 typedef struct shp_scene shp_scene;
@@ -130,6 +133,10 @@ void shp_set_view_data(struct dat_t *dat, int s, int f, int v, const double *in)
 /* realized geometry: x[3*nv], vertex n[3*nv], facet n[3*nf], area[nf] */
 void shp_geometry(struct mod_t *mod, double *vx, double *vn, double *fn, double *area);
 void shp_facet_seen(struct mod_t *mod, unsigned char *out);
+void shp_set_vertex_normals(struct mod_t *mod, const double *vn);
+void shp_fits_bracket(struct par_t *par, struct mod_t *mod, struct dat_t *dat, int start);
+void shp_set_offset(struct dat_t *dat, int s, int which, int j, int state, double val);
+double shp_get_offset(struct dat_t *dat, int s, int which, int j);
 /* par flags: posbnd, badradar, baddiam; logfactors posbnd, badradar, baddiam */
 void shp_par_flags(struct par_t *par, double out[6]);
 void shp_frame_consts(struct dat_t *dat, int s, int f, double out[9]);

@@ -55,6 +55,7 @@ class Spec(C.Structure):
         ("delcor_a0", C.c_double), ("delcor_a1", C.c_double),
         ("obs_lat_min_deg", C.c_double), ("obs_lat_max_deg", C.c_double),
         ("radar_nviews", C.c_int), ("view_interval_days", C.c_double), ("lc_nviews", C.c_int),
+        ("nsinc2", C.c_int), ("sinc2width", C.c_int),
     ]
 
 
@@ -94,6 +95,10 @@ def load_scene():
             "shp_set_view_data": (None, [vp, C.c_int, C.c_int, C.c_int, dp]),
             "shp_geometry": (None, [vp, dp, dp, dp, dp]),
             "shp_facet_seen": (None, [vp, C.POINTER(C.c_ubyte)]),
+            "shp_set_vertex_normals": (None, [vp, dp]),
+            "shp_fits_bracket": (None, [vp, vp, vp, C.c_int]),
+            "shp_set_offset": (None, [vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double]),
+            "shp_get_offset": (C.c_double, [vp, C.c_int, C.c_int, C.c_int]),
             "shp_par_flags": (None, [vp, dp]),
             "shp_frame_consts": (None, [vp, C.c_int, C.c_int, dp]),
             "shp_par_set": (None, [vp, C.c_int, C.c_double]),
@@ -146,6 +151,12 @@ def load_hip():
             "realize_spin": (None, [vp, vp, vp]),
             "realize_delcor": (None, [vp, C.c_double, C.c_int]),
             "calc_fits": (None, [vp, vp, vp]),
+            "calc_deldop": (None, [vp, vp, vp, C.c_int]),
+            "calc_doppler": (None, [vp, vp, vp, C.c_int]),
+            "calc_lghtcrv": (None, [vp, vp, vp, C.c_int]),
+            "realize_angleoff": (None, [vp]),
+            "realize_omegaoff": (None, [vp]),
+            "shape_hip_release_cache": (None, []),
             "chi2": (C.c_double, [vp, vp, C.c_int]),
             "shape_hip_last_error": (C.c_int, []),
             "shape_hip_clear_error": (None, []),
@@ -318,6 +329,10 @@ class Scene:
         self.lib.shp_geometry(self.mod, vx.ctypes.data_as(dp), vn.ctypes.data_as(dp),
                               fn.ctypes.data_as(dp), area.ctypes.data_as(dp))
         return vx.reshape(nv, 3), vn.reshape(nv, 3), fn.reshape(nf, 3), area
+
+    def set_vertex_normals(self, vn):
+        vn = np.ascontiguousarray(vn, dtype=np.float64).reshape(-1)
+        self.lib.shp_set_vertex_normals(self.mod, vn.ctypes.data_as(C.POINTER(C.c_double)))
 
     def facet_seen(self):
         out = np.zeros(self.nf, dtype=np.uint8)

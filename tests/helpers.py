@@ -158,3 +158,26 @@ def rel_err(a, b):
     a, b = np.asarray(a, dtype=float), np.asarray(b, dtype=float)
     den = max(np.max(np.abs(b)), 1e-300)
     return float(np.max(np.abs(a - b)) / den)
+
+
+# Delay and Doppler response cases beyond one sample per baud and one sinc^2
+# sample per pixel (pos2deldop.c:225-236, :371-456; pos2doppler.c), on a
+# one-frame delay-Doppler plus one-spectrum scene of config 1.  Rows per
+# pixel footprint: LONG_MOD / SHORT 2*ceil(const1) with const1 = (3 spb -
+# 1) / (2 stride), LONG_ORIG 2*spb/stride.  "wide" has pixels of 0.12 km
+# at 0.05 Hz per bin with nsinc2 = 3 and spb = 3: ~20 Doppler bins and 8
+# delay rows per pixel, beyond the mixed method's unrolled register window
+# (4 rows x 16 bins), so the general loop synthesizes them.
+_DD1 = dict(n_deldop=1, n_doppler=1, nbin_min=40, nbin_max=60)
+FOOTPRINTS = {
+    "spb2_longmod": dict(_DD1, spb=2, stride=1, codemethod=sh.LONG_MOD),
+    "spb2_longorig": dict(_DD1, spb=2, stride=1, codemethod=sh.LONG_ORIG),
+    "spb3_short": dict(_DD1, spb=3, stride=1, codemethod=sh.SHORT),
+    "spb4_stride2": dict(_DD1, spb=4, stride=2, codemethod=sh.LONG_MOD),
+    "spb3_longorig": dict(_DD1, spb=3, stride=1, codemethod=sh.LONG_ORIG),
+    "nsinc2_2": dict(_DD1, nsinc2=2),
+    "nsinc2_3_spb2": dict(_DD1, nsinc2=3, spb=2, stride=1),
+    "wide": dict(n_deldop=1, n_doppler=1, pos_n=12, km_per_pixel=0.12, dop_per_pixel=0.05, ndop_min=800,
+                 ndop_max=800, nbin_min=800, nbin_max=800, dop_per_bin_min=0.05, dop_per_bin_max=0.05,
+                 nsinc2=3, spb=3, stride=1),
+}

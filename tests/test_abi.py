@@ -59,7 +59,6 @@ def test_no_compute_without_device_is_loud():
     assert "oracle" not in src
 
 
-@pytest.mark.skipif(not os.path.exists(REF_SHAPE2), reason="reference tree not present")
 def test_batch_create_rejects_bad_arguments():
     """Bad batch arguments are refused before any device call (NULL)."""
     lib = sh.load_hip()
@@ -82,6 +81,7 @@ def test_batch_create_rejects_bad_arguments():
     assert not create(1, 1, 0, 1, targets=(C.c_void_p * 1)(None))  # null target
 
 
+@pytest.mark.skipif(not os.path.exists(REF_SHAPE2), reason="reference tree not present")
 def test_struct_members_match_reference():
     ours = abi_parse.parse_structs(open(ABI).read())
     # const.h supplies the array-size macros shape2.h uses (MAXOVERFLOW, ...)
@@ -109,3 +109,91 @@ def test_struct_sizes_agree_between_compilers():
     nb = scn.shp_abi_sizes(b, 32)
     assert na == nb and na > 10
     assert list(a)[:na] == list(b)[:nb]
+
+
+# Objects of the reference's link whose routines libshape_hip.so provides
+# (the link recipe drops them), and the routines of those objects the
+# recipe keeps instead: calc_fits.c also defines the plane-of-sky dataset
+# (calc_poset) and the write action's POS images (write_pos_*), which this
+# library does not compute; the recipe keeps them from a copy of
+# calc_fits.o whose other routines objcopy renames.
+REPLACED_OBJECTS = ["realize_mod", "realize_spin", "realize_delcor", "posclr", "posvis", "posmask", "radlaw",
+                    "pos2deldop", "pos2doppler", "apply_photo", "calc_fits", "chi2"]
+KEPT_BY_RECIPE = {"calc_poset", "write_pos_deldop", "write_pos_doppler", "write_pos_poset", "write_pos_lghtcrv"}
+REF_SRC = "/root/reference/src"
+
+
+def _strip_c(t):
+    t = re.sub(r"/\*.*?\*/", " ", t, flags=re.S)
+    return re.sub(r"//[^\n]*", " ", t)
+
+
+def _external_definitions(path):
+    """Names of the functions a C file defines without `static`."""
+    t = _strip_c(open(path, errors="replace").read())
+    out = set()
+    for m in re.finditer(r"^([A-Za-z_][\w \t\*]*?)\b(\w+)\s*\(([^;{)]*(?:\([^)]*\)[^;{)]*)*)\)\s*\{", t, flags=re.M):
+        if "static" not in m.group(1) and m.group(2) not in ("if", "while", "for", "switch"):
+            out.add(m.group(2))
+    return out
+
+
+@pytest.mark.skipif(not os.path.isdir(REF_SRC), reason="reference tree not present")
+def test_every_cross_file_callee_of_the_replaced_objects_resolves():
+    """Every routine that the dropped objects define and that any other source
+    file of the reference (C or CUDA, every directory) calls must be exported
+    by libshape_hip.so or kept by the recipe, so the documented link leaves no
+    dangling symbol (e.g. write_dat.c:123-124, calc_fits_hmt.c:146-155)."""
+    import glob
+    defs = {}
+    for obj in REPLACED_OBJECTS:
+        for name in _external_definitions(os.path.join(REF_SRC, "shape", obj + ".c")):
+            defs[name] = obj
+    replaced_paths = {os.path.join(REF_SRC, "shape", o + ".c") for o in REPLACED_OBJECTS}
+    called = {}
+    for path in glob.glob(REF_SRC + "/**/*.c", recursive=True) + glob.glob(REF_SRC + "/**/*.cu", recursive=True):
+        if path in replaced_paths:
+            continue
+        t = _strip_c(open(path, errors="replace").read())
+        for name in defs:
+            if re.search(r"(?<![\w.>])%s\s*\(" % name, t):
+                called.setdefault(name, path)
+    assert {"realize_angleoff", "calc_deldop", "calc_lghtcrv"} <= set(called)  # the scan sees them
+    lib = C.CDLL(os.path.join(H.PKG, "lib", "libshape_hip.so"))
+    missing = sorted(n for n in called if n not in KEPT_BY_RECIPE and not hasattr(lib, n))
+    assert not missing, [(n, called[n]) for n in missing]
+    recipe = open(os.path.join(ROOT, "INTEGRATION.md")).read()
+    for n in REPLACED_OBJECTS:
+        assert "./src/shape/%s.o" % n in recipe, n
+    # the recipe renames the library's routines in its copy of calc_fits.o
+    for n in ("calc_fits", "calc_deldop", "calc_doppler", "calc_lghtcrv"):
+        assert "--redefine-sym %s=" % n in recipe, n
+
+
+def test_offset_equal_states():
+    """realize_angleoff / realize_omegaoff (realize_spin.c:286-342): an '='
+    component takes the value of the last earlier dataset whose state is not
+    '='; '=' on the first dataset is an error.  Host bookkeeping: no device
+    call."""
+    lib = sh.load_hip()
+    scl = sh.load_scene()
+    sc = sh.Scene(sh.spec_for(2, n_deldop=4))
+    dat = sc.dat
+    for which, fn in ((0, lib.realize_angleoff), (1, lib.realize_omegaoff)):
+        for s in range(4):
+            for j in range(3):
+                scl.shp_set_offset(dat, s, which, j, ord("c"), 10.0 * s + j + 0.5 * which)
+        scl.shp_set_offset(dat, 1, which, 0, ord("="), -1.0)
+        scl.shp_set_offset(dat, 2, which, 0, ord("="), -2.0)
+        scl.shp_set_offset(dat, 3, which, 2, ord("="), -3.0)
+        lib.shape_hip_clear_error()
+        fn(dat)
+        assert lib.shape_hip_last_error() == 0
+        got = [[scl.shp_get_offset(dat, s, which, j) for j in range(3)] for s in range(4)]
+        b = 0.5 * which
+        assert got == [[0 + b, 1 + b, 2 + b], [0 + b, 11 + b, 12 + b], [0 + b, 21 + b, 22 + b],
+                       [30 + b, 31 + b, 22 + b]]
+        scl.shp_set_offset(dat, 0, which, 1, ord("="), 5.0)
+        fn(dat)
+        assert lib.shape_hip_last_error() != 0
+        lib.shape_hip_clear_error()

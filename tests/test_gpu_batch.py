@@ -105,6 +105,21 @@ def test_mixed_batch_matches_oracle(config, over):
     _check(config, sh.METHOD_MIXED, over, 1e-6, 1e-4, tol_frame=1e-5 if over is SMEAR else None)
 
 
+@pytest.mark.parametrize("method,tol_chi,tol_img", [(sh.METHOD_FP64, 1e-10, 1e-12), (sh.METHOD_MIXED, 1e-6, 1e-4)])
+@pytest.mark.parametrize("case", sorted(H.FOOTPRINTS))
+def test_batch_footprints(method, tol_chi, tol_img, case):
+    """Several samples per baud (spb 2-4, strides 1 and 2, the long-code
+    and short-code reductions) and sinc^2 means over nsinc2 x nsinc2
+    subpixels, in both methods against the oracle; "wide" footprints exceed
+    the mixed method's unrolled register window and take its general loop.
+    Each scene's Doppler spectrum has a chi2 of ~35 over 40-60 bins, at its
+    noise floor, where the mixed method's ~1e-7 relative fit error becomes
+    ~2e-6 of the frame's chi2 (test_mixed_batch_matches_oracle): its frames
+    are held to 1e-5, every evaluation's chi2 and objective to 1e-6."""
+    _check(1, method, H.FOOTPRINTS[case], tol_chi, tol_img,
+           tol_frame=1e-5 if method == sh.METHOD_MIXED else None)
+
+
 OTHER_LAWS = [(sh.HAGFORS_COSINE, [0.1, 1.2, 0.25, 40.0, 0.3]), (sh.GAUSSIAN_COSINE, [0.1, 1.2, 0.3, 6.0, 0.4]),
               (sh.COSINELAW_QS, [0.2, 3.0, 0.5])]
 

@@ -61,6 +61,40 @@ def test_realize_sphere_closed_form(orc):
     assert np.min(np.sum(vn * x / R, axis=1)) > 0.995
 
 
+def test_realize_vertex_normals_plain_sum(orc):
+    """Facet normals and areas from the corners' cross product, and each
+    vertex normal the normalized plain sum of its attached facets' unit
+    normals (realize_mod.c:532-563), all recomputed here from the realized
+    positions (facets oriented outward, away from the center of the
+    star-shaped mesh).  On this noisy mesh the facets around a vertex differ
+    in area, so an area-weighted sum would miss by > 1e-4."""
+    sc = sh.Scene(sh.spec_for(1))
+    orc.oracle_realize_mod(sc.par, sc.mod)
+    x, vn, fn, area = sc.geometry()
+    af = _attached(sc)
+    nf = len(fn)
+    tri = -np.ones((nf, 3), dtype=int)
+    cnt = np.zeros(nf, dtype=int)
+    for v, fs in enumerate(af):
+        for f in fs:
+            tri[f, cnt[f]] = v
+            cnt[f] += 1
+    assert np.all(cnt == 3)
+    a, b, c = x[tri[:, 0]], x[tri[:, 1]], x[tri[:, 2]]
+    cr = np.cross(b - a, c - a)
+    cr *= np.sign(np.sum(cr * (a + b + c), axis=1))[:, None]
+    ar = 0.5 * np.linalg.norm(cr, axis=1)
+    un = cr / (2 * ar[:, None])
+    assert np.max(np.abs(fn - un)) < 1e-13
+    assert np.max(np.abs(area - ar) / ar) < 1e-12
+    plain = np.array([un[fs].sum(axis=0) for fs in af])
+    plain /= np.linalg.norm(plain, axis=1)[:, None]
+    wtd = np.array([(ar[fs, None] * un[fs]).sum(axis=0) for fs in af])
+    wtd /= np.linalg.norm(wtd, axis=1)[:, None]
+    assert np.max(np.abs(vn - plain)) < 1e-13
+    assert np.max(np.abs(vn - wtd)) > 1e-4
+
+
 def test_realize_divergence_theorem(orc):
     """Closed outward-oriented surface: sum(area * n) = 0 and
     (1/3) sum(area * n . centroid) = volume of the polyhedron > 0."""
@@ -240,6 +274,32 @@ def test_posvis_against_analytic_ellipsoid(orc):
     assert len(errs) > 1000
     assert np.all(errs > -1e-9)
     assert np.max(errs) < 1.0
+
+
+def test_posvis_smoothed_cose_closed_form(orc):
+    """Smoothed cos(scattering) (posvis.c:243-260) interpolates the corners'
+    vertex normals linearly in the pixel center's barycentric coordinates,
+    rotates the result into the observer frame and normalizes it.  With
+    every vertex normal set to its own vertex position, linear interpolation
+    over a facet reproduces the surface point under the pixel center, so on
+    any mesh centered at the origin cose = z / |(x, y, z)| in closed form
+    (x, y the pixel center, z its depth).  An interpolation that weights the
+    corners otherwise (e.g. t (c - a) for t (c - b)) misses it by ~ the edge
+    length over the radius."""
+    sc = _realized(orc, 1)  # the 2%-deformed ellipsoid
+    orc.oracle_calc_fits(sc.par, sc.mod, sc.dat)  # POS frames of the view
+    sc.set_vertex_normals(sc.geometry()[0])
+    pos = sc.pos(0, 0)
+    orc.oracle_posclr(pos)
+    off = (C.c_double * 3)(0.0, 0.0, 0.0)
+    orc.oracle_posvis(sc.vertices, off, pos, 1, 0, 0, 0)
+    z, fac, cose = sc.pos_array(pos, 3), sc.pos_array(pos, 10), sc.pos_array(pos, 1)
+    n, kmpp = sc.spec.pos_n, sc.spec.km_per_pixel
+    ii, jj = np.nonzero((fac >= 0) & (z > 0))
+    px, py, pz = (ii - n) * kmpp, (jj - n) * kmpp, z[ii, jj]
+    expect = pz / np.sqrt(px * px + py * py + pz * pz)
+    assert len(ii) > 5000
+    assert np.max(np.abs(cose[ii, jj] - expect)) < 1e-13
 
 
 def test_posbnd_when_model_exceeds_window(orc):
@@ -426,6 +486,76 @@ def test_deldop_marginals(orc):
             cols[jd - 1] += p * wj / wj.sum()
         assert np.allclose(fit.sum(axis=1), rows, rtol=0, atol=1e-12 * rows.max())
         assert np.allclose(fit.sum(axis=0), cols, rtol=0, atol=1e-12 * cols.max())
+
+
+def _delay_rows(d, spb, stride, code, ndel):
+    """Response of every image row to a pixel at fractional row d, from its
+    definition (pos2deldop.c:371-424).  Long code, original reduction: each
+    sample per baud reduced on its own and the rows interleaved, a squared
+    unit triangle of half-width spb/stride rows.  Otherwise: spb unit
+    triangles of that half-width, the m-th centred m/stride - (spb-1)/(2
+    stride) rows after the row centre, summed, divided by spb and squared.
+    Every row of the frame is evaluated (zero outside the support)."""
+    hw = spb // stride
+    rows = np.arange(1, ndel + 1, dtype=float)
+
+    def tri(u):
+        return np.clip(1.0 - np.abs(u), 0.0, None)
+    if code == sh.LONG_ORIG:
+        return tri((d - rows) / hw) ** 2
+    c2 = (spb - 1) / (2.0 * stride)
+    acc = sum(tri((d - (rows - c2 + m / stride)) / hw) for m in range(spb))
+    return (acc / spb) ** 2
+
+
+def _sinc2(u):
+    a = np.pi * np.abs(u)
+    return np.where(a > 1e-4, (np.sin(a) / np.where(a > 0, a, 1.0)) ** 2, 1.0)
+
+
+@pytest.mark.parametrize("spb,stride,code,ns", [(2, 1, sh.LONG_MOD, 1), (2, 1, sh.LONG_ORIG, 1),
+                                                (3, 1, sh.LONG_MOD, 1), (4, 2, sh.LONG_MOD, 1),
+                                                (2, 2, sh.LONG_ORIG, 1), (1, 1, sh.LONG_MOD, 2),
+                                                (2, 1, sh.LONG_MOD, 3)])
+def test_deldop_marginals_samples_and_subpixels(orc, spb, stride, code, ns):
+    """Row and column sums of a delay-Doppler fit with several samples per
+    baud and nsinc2 x nsinc2 sinc^2 points per pixel.  Rows: each lit
+    pixel's power spread by the delay response written from its definition
+    (_delay_rows).  Columns: the mean of sinc^2 over the pixel's ns x ns
+    subpixel points, evenly spaced ((a + 1/2)/ns - 1/2 pixels from the
+    center in x and y, their Doppler from (w x r) at the subpoint), over the
+    sinc2width = 6 nearest bins widened by the subpoints' largest Doppler
+    offset.  Delay and Doppler bins from the range and the line-of-sight
+    velocity (_dd_coords)."""
+    sc = _realized(orc, 1, spb=spb, stride=stride, codemethod=code, nsinc2=ns)
+    orc.oracle_calc_fits(sc.par, sc.mod, sc.dat)
+    ii, jj, z, P = _pixel_power(sc, 0, 0)
+    delpos, dop = _dd_coords(sc, 0, 0, ii, jj, z)
+    # Doppler bins per pixel step in x and y, from two neighbouring centers
+    _, dx = _dd_coords(sc, 0, 0, ii + 1, jj, z)
+    _, dy = _dd_coords(sc, 0, 0, ii, jj + 1, z)
+    ax, ay = dx - dop, dy - dop
+    fit = sc.image(0, 0, 0)
+    ndel, ndop = fit.shape
+    rows, cols = np.zeros(ndel), np.zeros(ndop)
+    offs = [(a + 0.5) / ns - 0.5 for a in range(ns)]
+    cbins = np.arange(1, ndop + 1, dtype=float)
+    used = 0
+    for p, d, q, gx, gy in zip(P, delpos, dop, ax, ay):
+        wd = _delay_rows(d, spb, stride, code, ndel)
+        sub = np.array([q + u * gx + v * gy for u in offs for v in offs])
+        dmax = np.max(np.abs(sub - q))
+        j0, j1 = int(math.floor(q - dmax + 1 - 3)), int(math.floor(q + dmax + 3))
+        nz = np.nonzero(wd)[0]
+        if j0 < 1 or j1 > ndop or nz.min() == 0 or nz.max() == ndel - 1:
+            continue
+        wp = np.where((cbins >= j0) & (cbins <= j1), np.mean(_sinc2(sub[:, None] - cbins[None, :]), axis=0), 0.0)
+        rows += p * wd / wd.sum()
+        cols += p * wp / wp.sum()
+        used += 1
+    assert used > 1000
+    assert np.allclose(fit.sum(axis=1), rows, rtol=0, atol=1e-11 * rows.max())
+    assert np.allclose(fit.sum(axis=0), cols, rtol=0, atol=1e-11 * cols.max())
 
 
 def test_badradar_from_sinc_width(orc):
