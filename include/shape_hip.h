@@ -63,6 +63,20 @@ void realize_omegaoff(struct dat_t *dat);
    writes frame->chi2, cal.val, set chi2 and dat->chi2, returns dat->chi2. */
 double chi2(struct par_t *par, struct dat_t *dat, int list_breakdown);
 
+/* chi2 (above) on fits, observations and weights already in device memory,
+   allocating no device memory: no counterpart in the reference, for callers
+   that cannot allocate (code under hipGraph capture, tight serving loops).
+   d_fit / d_obs / d_oov hold every frame's cells in dataset order, each
+   frame row-major (delay-Doppler: ndel rows of ndop; Doppler: ndop;
+   lightcurve: n points), as chi2 flattens the structs.  Frame shapes,
+   calibration states, weights and overflow terms come from dat; results go
+   into dat as chi2 writes them, and the total is returned.  Its frame
+   records and sums live in a fixed device array per GPU (at most 8192
+   frames and 8192 datasets; more is an error): calls that overlap on one
+   GPU race. */
+double shape_chi2_noalloc(struct par_t *par, struct dat_t *dat, const double *d_fit, const double *d_obs,
+                          const double *d_oov);
+
 /* posclr.c:24  blank sky over the whole POS (z = -1e20, cos = b = 0,
    f/body/comp = -1, and the *ill arrays for a bistatic POS); empty rectangles. */
 void posclr(struct pos_t *pos);

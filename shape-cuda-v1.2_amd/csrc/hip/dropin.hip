@@ -309,15 +309,13 @@ GrowBuf g_c_fr, g_c_fit, g_c_obs, g_c_oov, g_c_set, g_c_tot;
 // chi2 (chi2.c:240): chi-square of the fits currently held by the caller
 // ---------------------------------------------------------------------------
 
-extern "C" double chi2(struct par_t *par, struct dat_t *dat, int list_breakdown) {
-  (void)par;
-  (void)list_breakdown;
-  // frame records first (offsets into flat arrays), then the fit,
-  // observation and weight arrays copied from the frames' row arrays in
-  // parallel
-  std::vector<FrameRec> fr;
-  std::vector<const void *> src;  // the frame (deldopfrm_t, dopfrm_t) or lightcurve (lghtcrv_t)
-  long long ncell = 0;
+// The chi-square frame records of every dataset in set order, with offsets
+// into flat fit / observation / weight arrays (each frame's cells row-major,
+// as the drop-ins flatten them); src gets each record's host frame
+// (deldopfrm_t, dopfrm_t) or lightcurve (lghtcrv_t).
+static int chi2_frames(struct dat_t *dat, std::vector<FrameRec> &fr, std::vector<const void *> &src,
+                       long long &ncell) {
+  ncell = 0;
   auto add = [&](FrameRec &F, int s, long long N, const void *h) {
     F.set = s;
     F.nviews = 1;
@@ -357,9 +355,48 @@ extern "C" double chi2(struct par_t *par, struct dat_t *dat, int list_breakdown)
       F.cal_floats = L->cal.state == 'f'; F.cal_fixed = L->cal.val; F.weight = L->weight;
       add(F, s, L->n, L);
     } else {
-      set_err(-15, "chi2 (plane-of-sky sets are outside this library)");
-      return dat->chi2;
+      return -15;  // plane-of-sky sets are outside this library
     }
+  }
+  return 0;
+}
+
+// o2 per frame, the frames' chi-square terms (overflow terms from the host
+// frames, chi2.c:430-431) and the set and total sums, on the null stream
+static void chi2_kernels(FrameRec *df, int nf, const double *fit, const double *obs, const double *oov,
+                         double *dset, int nsets, double *dtot) {
+  hipLaunchKernelGGL(k_frame_o2, dim3(std::max(1, nf)), dim3(256), 0, 0, df, nf, obs, oov);
+  hipLaunchKernelGGL(k_chi2_frames64, dim3(std::max(1, nf)), dim3(256), 0, 0, df, nf, (double *)fit, obs, oov,
+                     (const double *)nullptr, 0, 1, 0);
+  hipLaunchKernelGGL(k_chi2_sets, dim3(1), dim3(64), 0, 0, (const FrameRec *)df, nf, dset, nsets, dtot);
+}
+
+// results into the caller's structs (chi2.c:270-300, :430-460)
+static double chi2_store(struct dat_t *dat, const std::vector<FrameRec> &fr, const std::vector<double> &setc,
+                         double total) {
+  for (auto &F : fr) {
+    struct set_t *set = &dat->set[F.set];
+    if (F.type == VT_DELAY) { set->desc.deldop.frame[F.frm].chi2 = F.chi2; set->desc.deldop.frame[F.frm].cal.val = F.cal; }
+    else if (F.type == VT_DOPPLER) { set->desc.doppler.frame[F.frm].chi2 = F.chi2; set->desc.doppler.frame[F.frm].cal.val = F.cal; }
+    else set->desc.lghtcrv.cal.val = F.cal;
+  }
+  for (int s = 0; s < dat->nsets; s++) dat->set[s].chi2 = setc[s];
+  dat->chi2 = total;
+  return total;
+}
+
+extern "C" double chi2(struct par_t *par, struct dat_t *dat, int list_breakdown) {
+  (void)par;
+  (void)list_breakdown;
+  // frame records first (offsets into flat arrays), then the fit,
+  // observation and weight arrays copied from the frames' row arrays in
+  // parallel
+  std::vector<FrameRec> fr;
+  std::vector<const void *> src;
+  long long ncell = 0;
+  if (chi2_frames(dat, fr, src, ncell)) {
+    set_err(-15, "chi2 (plane-of-sky sets are outside this library)");
+    return dat->chi2;
   }
   // fit, obs and 1/var back to back in the thread's pinned stage (plain
   // host memory if it cannot be pinned)
@@ -397,12 +434,8 @@ extern "C" double chi2(struct par_t *par, struct dat_t *dat, int list_breakdown)
                             hipMemcpy(dobs.p, obs, nb, hipMemcpyHostToDevice) == hipSuccess &&
                             hipMemcpy(doov.p, oov, nb, hipMemcpyHostToDevice) == hipSuccess));
   if (!ok) { set_err(-3, "chi2 upload"); return dat->chi2; }
-  FrameRec *df = (FrameRec *)dfr.p;
-  hipLaunchKernelGGL(k_frame_o2, dim3(std::max(1, nf)), dim3(256), 0, 0, df, nf, (double *)dobs.p, (double *)doov.p);
-  hipLaunchKernelGGL(k_chi2_frames64, dim3(std::max(1, nf)), dim3(256), 0, 0, df, nf, (double *)dfit.p,
-                     (double *)dobs.p, (double *)doov.p, (const double *)nullptr, 0, 1, 0);
-  hipLaunchKernelGGL(k_chi2_sets, dim3(1), dim3(64), 0, 0, (const FrameRec *)df, nf, (double *)dset.p, nsets,
-                     (double *)dtot.p);
+  chi2_kernels((FrameRec *)dfr.p, nf, (const double *)dfit.p, (const double *)dobs.p, (const double *)doov.p,
+               (double *)dset.p, nsets, (double *)dtot.p);
   std::vector<double> setc(nsets);
   double total = 0.0;
   ok = hipDeviceSynchronize() == hipSuccess &&
@@ -410,15 +443,93 @@ extern "C" double chi2(struct par_t *par, struct dat_t *dat, int list_breakdown)
        hipMemcpy(setc.data(), dset.p, sizeof(double) * nsets, hipMemcpyDeviceToHost) == hipSuccess &&
        hipMemcpy(&total, dtot.p, sizeof(double), hipMemcpyDeviceToHost) == hipSuccess;
   if (!ok) { set_err(-4, "chi2 kernels"); return dat->chi2; }
-  for (auto &F : fr) {
-    struct set_t *set = &dat->set[F.set];
-    if (F.type == VT_DELAY) { set->desc.deldop.frame[F.frm].chi2 = F.chi2; set->desc.deldop.frame[F.frm].cal.val = F.cal; }
-    else if (F.type == VT_DOPPLER) { set->desc.doppler.frame[F.frm].chi2 = F.chi2; set->desc.doppler.frame[F.frm].cal.val = F.cal; }
-    else set->desc.lghtcrv.cal.val = F.cal;
+  return chi2_store(dat, fr, setc, total);
+}
+
+// ---------------------------------------------------------------------------
+// shape_chi2_noalloc: chi2 on fits, observations and weights the caller
+// already holds in device memory, for callers that cannot allocate device
+// memory (code under graph capture, serving loops).  The frame records and
+// the set and total sums live in `scratch`, one file-scope device array of
+// fixed size per GPU; the address of each GPU's copy is taken once, with
+// that GPU current, on the first call there.  Every call on a GPU shares its
+// array: overlapping calls on one GPU race.  Nothing in it needs clearing
+// between calls -- each call writes the frame records it reads, and
+// k_chi2_sets zeroes the set sums before adding.
+// ---------------------------------------------------------------------------
+namespace {
+constexpr int NOALLOC_MAX_FRAMES = 8192, NOALLOC_MAX_SETS = 8192;
+constexpr size_t NOALLOC_SETS_OFF = sizeof(FrameRec) * (size_t)NOALLOC_MAX_FRAMES;
+constexpr size_t NOALLOC_BYTES = NOALLOC_SETS_OFF + sizeof(double) * ((size_t)NOALLOC_MAX_SETS + 1);
+static_assert(NOALLOC_SETS_OFF % 256 == 0, "set sums aligned");
+__device__ __align__(256) unsigned char scratch[NOALLOC_BYTES];
+constexpr int NOALLOC_MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+void *g_scratch_dev[NOALLOC_MAX_DEVICES];
+
+// this GPU's copy of scratch (its device address, not the host-side name)
+int scratch_ptr(unsigned char **out) {
+  int dev = -1;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= NOALLOC_MAX_DEVICES) return -7;
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_dev[dev]) {
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+      fprintf(stderr, "shape_chi2_noalloc: hipGetSymbolAddress failed: %s\n", hipGetErrorString(e));
+      return -7;
+    }
+    g_scratch_dev[dev] = p;
   }
-  for (int s = 0; s < nsets; s++) dat->set[s].chi2 = setc[s];
-  dat->chi2 = total;
-  return total;
+  *out = (unsigned char *)g_scratch_dev[dev];
+  return 0;
+}
+}  // namespace
+
+extern "C" double shape_chi2_noalloc(struct par_t *par, struct dat_t *dat, const double *d_fit, const double *d_obs,
+                                     const double *d_oov) {
+  (void)par;
+  std::vector<FrameRec> fr;
+  std::vector<const void *> src;
+  long long ncell = 0;
+  if (chi2_frames(dat, fr, src, ncell)) {
+    set_err(-15, "shape_chi2_noalloc (plane-of-sky sets are outside this library)");
+    return dat->chi2;
+  }
+  const int nf = (int)fr.size(), nsets = dat->nsets;
+  if (nf > NOALLOC_MAX_FRAMES || nsets > NOALLOC_MAX_SETS) {
+    fprintf(stderr, "shape_chi2_noalloc: %d frames in %d datasets exceed its scratch (at most %d frames, %d datasets)\n",
+            nf, nsets, NOALLOC_MAX_FRAMES, NOALLOC_MAX_SETS);
+    set_err(-18, "shape_chi2_noalloc");
+    return dat->chi2;
+  }
+  if (ncell > 0 && (!d_fit || !d_obs || !d_oov)) {
+    set_err(-19, "shape_chi2_noalloc (null device array)");
+    return dat->chi2;
+  }
+  unsigned char *sp = nullptr;
+  int rc = scratch_ptr(&sp);
+  if (rc) { set_err(rc, "shape_chi2_noalloc"); return dat->chi2; }
+  FrameRec *df = (FrameRec *)sp;
+  double *dset = (double *)(sp + NOALLOC_SETS_OFF), *dtot = dset + NOALLOC_MAX_SETS;
+  if (nf > 0 && hipMemcpy(df, fr.data(), sizeof(FrameRec) * nf, hipMemcpyHostToDevice) != hipSuccess) {
+    (void)hipGetLastError();
+    set_err(-3, "shape_chi2_noalloc upload");
+    return dat->chi2;
+  }
+  chi2_kernels(df, nf, d_fit, d_obs, d_oov, dset, nsets, dtot);
+  std::vector<double> setc(nsets);
+  double total = 0.0;
+  const bool ok = hipStreamSynchronize(0) == hipSuccess &&
+                  (nf == 0 || hipMemcpy(fr.data(), df, sizeof(FrameRec) * nf, hipMemcpyDeviceToHost) == hipSuccess) &&
+                  (nsets == 0 || hipMemcpy(setc.data(), dset, sizeof(double) * nsets, hipMemcpyDeviceToHost) == hipSuccess) &&
+                  hipMemcpy(&total, dtot, sizeof(double), hipMemcpyDeviceToHost) == hipSuccess;
+  if (!ok) {
+    (void)hipGetLastError();
+    set_err(-4, "shape_chi2_noalloc kernels");
+    return dat->chi2;
+  }
+  return chi2_store(dat, fr, setc, total);
 }
 
 // ---------------------------------------------------------------------------

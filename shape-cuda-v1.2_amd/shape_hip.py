@@ -158,6 +158,7 @@ def load_hip():
             "realize_omegaoff": (None, [vp]),
             "shape_hip_release_cache": (None, []),
             "chi2": (C.c_double, [vp, vp, C.c_int]),
+            "shape_chi2_noalloc": (C.c_double, [vp, vp, vp, vp, vp]),
             "shape_hip_last_error": (C.c_int, []),
             "shape_hip_clear_error": (None, []),
             "shape_hip_set_strict": (None, [C.c_int]),
