@@ -113,6 +113,7 @@ struct ViewRec {
   float oa32[9], sa32[9], usrc32[3];
   int D0, P0;
   float dfr, pfr, delfact32, ax32, ay32, const1_32, kmpp32, inv_kmpp32, dcrel32;
+  float shade_err32;    // mixed method: FP32 shading error scale (shade_ill)
   // ---- accumulated by kernels ----
   int xlim[2], ylim[2], xlim2[2], ylim2[2];
   int outbnd, outbnd2, any_overflow, nstale, nclaimed;  // nclaimed: mixed z-buffer keys claimed

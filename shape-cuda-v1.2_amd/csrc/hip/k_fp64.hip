@@ -169,6 +169,9 @@ __global__ void k_setup_views(ViewRec *views, int nviews, const double *params) 
   for (int q = 0; q < 3; q++) V.usrc32[q] = (float)V.usrc[q];
   V.kmpp32 = (float)V.kmpp;
   V.inv_kmpp32 = (float)(1.0 / V.kmpp);
+  // FP32 projected coordinates of the window carry ~3 ulps of n + 1 pixels;
+  // times 2 (s and t), in pixels per km of facet edge
+  V.shade_err32 = (float)(2.0 * 3.0 * 5.9604645e-8 * (V.n + 1) / V.kmpp);
   if (V.type == VT_DELAY || V.type == VT_DOPPLER) {
     double d0 = floor(V.delshift), p0 = floor(V.dopshift);
     V.D0 = (int)d0; V.P0 = (int)p0;

@@ -291,9 +291,10 @@ int enqueue_mixed(Engine &E) {
   for (auto &V : E.h_views) wmax_ = std::max(wmax_, 2 * V.n + 1);
   int tiles = cdivm(wmax_, TILE);
   SHP_LAUNCH(E, "k_fill_u8", k_fill_u8, dim3(256), bt, 0, st, E.pd.stale_tiles, std::max(1LL, E.total_tiles));
+  // block and facet bounds, and the facet-major copies every shading kernel reads
+  SHP_LAUNCH(E, "k_block_bounds", k_block_bounds, dim3(cdivm(E.max_nblk, 4), E.nslots), bt, 0, st, E.d_slots,
+             E.nslots);
   if (E.any_binned) {
-    SHP_LAUNCH(E, "k_block_bounds", k_block_bounds, dim3(cdivm(E.max_nblk, 4), E.nslots), bt, 0, st, E.d_slots,
-               E.nslots);
     SHP_LAUNCH(E, "k_bin32", k_bin32, dim3(cdivm(E.max_nblk, 256), E.nviews), bt, 0, st, E.d_views, E.d_slots,
                E.d_bin_cnt, E.d_bin_list, E.d_out_list, E.pd.nstale + 4);
     SHP_LAUNCH(E, "k_outside32", k_outside32, dim3(512), bt, 0, st, E.d_views, E.d_slots, E.pd, E.d_out_list,

@@ -110,10 +110,12 @@ __global__ __launch_bounds__(256) void k_shade_radar32(ViewRec *__restrict__ vie
     bary32(P, i, j, s, t);  // the raster's coordinates, to FP32 rounding
     const float nz = normal32_los(G, f, V.oa32, V.smooth, s, t);
     // cos(scattering) of an owner facing away is inherited from an earlier
-    // claimant, or (NaN law) too close to a quasispecular law's cutoff for
-    // FP32: FP64 replay (records reserved once per wave)
-    const float cl = nz > 0.0f ? radar_cl(V, G, pl, f, s, t, nz) : 0.0f;
-    const bool rep = !(nz > 0.0f) || cl != cl;
+    // claimant, an ill-conditioned FP32 cosine needs FP64, and (NaN law) a
+    // pixel too close to a quasispecular law's cutoff is beyond FP32: FP64
+    // replay (records reserved once per wave)
+    const bool ill = !(nz > 0.0f) || shade_ill(V, G, f, P);
+    const float cl = ill ? 0.0f : radar_cl(V, G, pl, f, s, t, nz);
+    const bool rep = ill || cl != cl;
     int si = stale_slot(pd, rep);
     if (rep) {
       if (si >= 0) stale_init(V, pd, si, blockIdx.y, (i + n) * w + (j + n), i, j);

@@ -43,8 +43,18 @@ __global__ __launch_bounds__(256) void k_block_bounds(GeomSlot *slots, int nslot
     va = G.x32[fv[0]]; vb = G.x32[fv[1]]; vc = G.x32[fv[2]];
     fn = G.fn32[f];
     // facet-major copies for the tile kernel: one gather per facet, not two
-    G.fvx[3 * f] = va; G.fvx[3 * f + 1] = vb; G.fvx[3 * f + 2] = vc;
-    G.fvn[3 * f] = G.vn32[fv[0]]; G.fvn[3 * f + 1] = G.vn32[fv[1]]; G.fvn[3 * f + 2] = G.vn32[fv[2]];
+    // (.w: the facet's longest edge in km and the larger vertex-normal
+    // difference along its barycentric axes, b - a and c - b; shade_ill)
+    const float4 na = G.vn32[fv[0]], nb = G.vn32[fv[1]], nc = G.vn32[fv[2]];
+    const float e0 = sqrtf((vb.x - va.x) * (vb.x - va.x) + (vb.y - va.y) * (vb.y - va.y) + (vb.z - va.z) * (vb.z - va.z));
+    const float e1 = sqrtf((vc.x - vb.x) * (vc.x - vb.x) + (vc.y - vb.y) * (vc.y - vb.y) + (vc.z - vb.z) * (vc.z - vb.z));
+    const float e2 = sqrtf((va.x - vc.x) * (va.x - vc.x) + (va.y - vc.y) * (va.y - vc.y) + (va.z - vc.z) * (va.z - vc.z));
+    const float d0 = sqrtf((nb.x - na.x) * (nb.x - na.x) + (nb.y - na.y) * (nb.y - na.y) + (nb.z - na.z) * (nb.z - na.z));
+    const float d1 = sqrtf((nc.x - nb.x) * (nc.x - nb.x) + (nc.y - nb.y) * (nc.y - nb.y) + (nc.z - nb.z) * (nc.z - nb.z));
+    G.fvx[3 * f] = make_float4(va.x, va.y, va.z, fmaxf(e0, fmaxf(e1, e2)));
+    G.fvx[3 * f + 1] = vb; G.fvx[3 * f + 2] = vc;
+    G.fvn[3 * f] = make_float4(na.x, na.y, na.z, fmaxf(d0, d1));
+    G.fvn[3 * f + 1] = nb; G.fvn[3 * f + 2] = nc;
   }
   float lo[3], hi[3];
   {
@@ -532,13 +542,21 @@ __global__ __launch_bounds__(256, 6) void k_tile32(const ViewRec *__restrict__ v
     const int f = key_facet(k);
     if (pd.fmap) pd.fmap[V.pos_off + (long long)(i + n) * (2 * n + 1) + (j + n)] = f;  // (inspection)
     Proj32 P;
-    corners32f(V, G, f, V.oa32, P);
+    const float4 xa = G.fvx[3 * f];
+    corners32p(V, xa, G.fvx[3 * f + 1], G.fvx[3 * f + 2], V.oa32, P);
     float s, t;
     bary32(P, i, j, s, t);  // the raster's coordinates, to FP32 rounding
-    const float nz = normal32f_los(G, f, V.oa32, V.smooth, s, t);
-    if (!(nz > 0.0f)) {
+    float nz, spread = 0.0f;
+    if (V.smooth) {
+      const float4 na = G.fvn[3 * f];
+      nz = normal_los(V.oa32, na, G.fvn[3 * f + 1], G.fvn[3 * f + 2], s, t);
+      spread = na.w;
+    } else {
+      nz = facing32(V.oa32, G.fn32[f]);
+    }
+    if (!(nz > 0.0f) || shade_ill(V, P, xa.w, spread)) {
       // cos(scattering) of an owner facing away is inherited from an earlier
-      // claimant: FP64 replay
+      // claimant; an ill-conditioned FP32 cosine needs FP64: FP64 replay
       tile_stale(V, pd, ev_ok, a * BT + c, i, j, flag, spix, &nts);
       continue;
     }

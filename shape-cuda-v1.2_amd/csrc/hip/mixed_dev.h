@@ -264,6 +264,33 @@ __device__ inline void mark_marginal(const ViewRec &V, const PosDev &pd, bool ma
   else if (first) pd.stale_idx[p] = -1;  // list full: the error flag is set
 }
 
+// Whether a shaded pixel's FP32 cos(scattering) may be far off.  The
+// smoothed normal interpolates the facet's vertex normals at the pixel's
+// FP32 barycentric coordinates, whose error is the FP32 rounding of the
+// projected corners (V.shade_err32: 3 ulps of the window's largest
+// coordinate, twice, per km of edge) times the facet's longest edge over
+// twice its projected area (|P.den|); times the larger vertex-normal
+// difference along the barycentric axes, that bounds the cosine's error.
+// A facet seen nearly edge-on on a rough surface (vertex normals far apart)
+// makes it large: on the 3%-noise model one pixel's cosine was off by 7.6e-4
+// and the model image pixel it fed by 1.7e-4 of itself.  Pixels whose
+// bound exceeds SHADE_TOL are shaded by the FP64 replay instead.  The bound
+// runs ~250x above the actual errors, which it ranks well: emulating the
+// FP32 shading of every lit pixel of one frame, SHADE_TOL = 3e-4 sends
+// 0.09% (large-model workload, 0.2% noise) and 1.05% (3% noise) of them
+// to the replay and leaves the rest within 4e-5 and 1.3e-5 of the FP64
+// cosine.  The per-facet edge and normal spread sit in the .w lanes of the
+// facet-major copies (k_block_bounds).
+constexpr float SHADE_TOL = 3.0e-4f;
+// edge: the facet's longest edge (km); spread: its vertex-normal spread, or
+// 0 for unsmoothed shading (the facet normal does not depend on s and t)
+__device__ inline bool shade_ill(const ViewRec &V, const Proj32 &P, float edge, float spread) {
+  return V.shade_err32 * fabsf(P.den) * edge * spread > SHADE_TOL;
+}
+__device__ inline bool shade_ill(const ViewRec &V, const GeomSlot &G, int f, const Proj32 &P) {
+  return shade_ill(V, P, G.fvx[3 * f].w, V.smooth ? G.fvn[3 * f].w : 0.0f);
+}
+
 // Line-of-sight component of the normal at (s, t) of facet f in observer
 // coordinates (posvis.c:243-258): the radar's cos(scattering).  A smoothed
 // normal interpolates the facet's vertex normals na, nb, nc and is
@@ -298,30 +325,47 @@ __device__ inline float sinc2f(float x) {
 }
 __device__ inline float trif(float x) { float t = 1.0f - fabsf(x); return t > 0.0f ? t : 0.0f; }
 
-// delay response of one image row (pos2deldop.c:371-424); drel = delPOS - idel.
-// One sample per baud (the common case) inline, the rest out of line.
-__device__ __attribute__((noinline)) float delay_w32_other(const ViewRec &V, float drel) {
+// hw - |x| for x = hi + lo + c (hi + lo: an FP64 fraction in [0, 1) held as
+// an FP32 pair; c and hw small multiples of 1/4), clamped at zero: the
+// unit triangle of half-width hw is this over hw.  Near the triangle's feet,
+// where |x| ~ hw and the difference cancels, (hw -+ c) -+ hi is exact in
+// FP32, so the small weights of a footprint's edge rows keep FP32 relative
+// precision; formed from one rounded FP32 argument they would carry its
+// absolute error (~3e-8) instead.
+__device__ inline float tri_rem(float hi, float lo, float c, float hw) {
+#pragma clang fp contract(off)
+  const float r = (hi + c) + lo >= 0.0f ? ((hw - c) - hi) - lo : ((hw + c) + hi) + lo;
+  return r > 0.0f ? r : 0.0f;
+}
+
+// delay response of image row idel = delPOS - (hi + lo) - k
+// (pos2deldop.c:371-424; delPOS - idel = hi + lo + k).  One sample per baud
+// (the common case) inline, the rest out of line.
+__device__ __attribute__((noinline)) float delay_w32_other(const ViewRec &V, float hi, float lo, int k) {
+  const int sos = V.spb / V.stride;
+  const float hw = (float)sos, fk = (float)k;
   if (V.codemethod != 1) {
-    int sos = V.spb / V.stride;
-    float c2 = (float)V.const2;
-    if (V.spb == 2) {
-      float arg = (drel + c2) / sos;
-      float d = trif(arg) + trif(arg - 0.5f);
+    const float c2 = (float)V.const2;
+    if (V.spb == 2) {  // TRI(arg) + TRI(arg - 1/2), arg = (delPOS - idel + const2) / sos
+      const float d = (tri_rem(hi, lo, fk + c2, hw) + tri_rem(hi, lo, fk + c2 - 0.5f * hw, hw)) / hw;
       return d * (d / 4.0f);
     }
-    float d = 0.0f, osp = 1.0f / V.spb;
-    int m_min = max((int)floorf((drel - c2) * V.stride), 0);
-    int m_max = min((int)ceilf((drel + (float)V.const1) * V.stride), V.spb) - 1;
-    float arg = (drel + c2) / sos - m_min * osp;
-    for (int m = m_min; m <= m_max; m++) { d += trif(arg); arg -= osp; }
+    // sample m's triangle is centred m/stride - const2 rows after the row centre
+    const float drel = (hi + fk) + lo;
+    const int m_min = max((int)floorf((drel - c2) * V.stride), 0);
+    const int m_max = min((int)ceilf((drel + (float)V.const1) * V.stride), V.spb) - 1;
+    const float step = hw / V.spb;
+    float d = 0.0f;
+    for (int m = m_min; m <= m_max; m++) d += tri_rem(hi, lo, fk + c2 - m * step, hw);
+    d /= hw;
     return d * (d / (V.spb * V.spb));
   }
-  float t = trif(drel / (V.spb / V.stride));
+  const float t = tri_rem(hi, lo, fk, hw) / hw;  // long code, original reduction
   return t * t;
 }
-__device__ inline float delay_w32(const ViewRec &V, float drel) {
-  if (V.codemethod != 1 && V.spb == 1) { float t = trif(drel); return t * t; }
-  return delay_w32_other(V, drel);
+__device__ inline float delay_w32(const ViewRec &V, float hi, float lo, int k) {
+  if (V.codemethod != 1 && V.spb == 1) { const float t = tri_rem(hi, lo, (float)k, 1.0f); return t * t; }
+  return delay_w32_other(V, hi, lo, k);
 }
 
 // c^e for 0 < c <= 1 by the hardware base-2 log and exponential (~1 ulp
@@ -413,8 +457,8 @@ struct RadarPix {
   int id0, id1, ip0, ip1;
   float pfr, s2;     // fractional Doppler bin; sin^2(pi pfr) / pi^2 (sinc1)
   int pm0;           // Doppler coordinate - first bin = pfr + pm0
-  float dfr;         // fractional delay bin
-  int dm0;           // delay coordinate - first row = dfr + dm0
+  float dfr, dlo;    // fractional delay bin (FP64) as an FP32 pair dfr + dlo
+  int dm0;           // delay coordinate - first row = dfr + dlo + dm0
 };
 
 // Doppler weight of bin b of a single-sample sinc^2 footprint
@@ -447,7 +491,7 @@ __device__ inline float dop_w(const ViewRec &V, const RadarPix &R, int b) {
 
 // delay response of row id0 + a
 __device__ inline float del_w(const ViewRec &V, const RadarPix &R, int a) {
-  return delay_w32(V, R.dfr + (float)(R.dm0 - a));
+  return delay_w32(V, R.dfr, R.dlo, R.dm0 - a);
 }
 
 // cl: the pixel's FP32 cos(scattering) in a view with the diffuse cosine law
@@ -496,6 +540,7 @@ __device__ inline void radar_pixel(const ViewRec &V, const double *pl, int i, in
   R.id0 = 0;
   R.id1 = 0;
   R.dfr = 0.0f;
+  R.dlo = 0.0f;
   R.dm0 = 0;
   if (dd) {
     double del64 = (double)z * V.delfact + V.delshift;
@@ -503,7 +548,9 @@ __device__ inline void radar_pixel(const ViewRec &V, const double *pl, int i, in
     R.id1 = (int)ceil(del64 + V.const1) - 1 - V.D0;
     R.delrel = (float)(del64 - (double)V.D0);
     double dfl = floor(del64);
-    R.dfr = (float)(del64 - dfl);
+    const double dfrac = del64 - dfl;
+    R.dfr = (float)dfrac;
+    R.dlo = (float)(dfrac - (double)R.dfr);
     R.dm0 = (int)dfl - (V.D0 + R.id0);
     const int nd = R.id1 - R.id0 + 1;
     sumd = 0.0f;
