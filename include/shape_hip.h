@@ -199,6 +199,29 @@ int shape_line_search(shape_eval_fn eval, void *ctx, int k, double x0, double st
 int shape_batch_line_search(shape_batch *b, double x0, double step, double tol, double abstol, int max_rounds,
                             double *xmin, double *fmin, int *rounds);
 
+/* nsearch independent searches sharing every round (eval gets nsearch * k
+   values, search s at [s k, (s + 1) k)); a search that has stopped repeats
+   its best point.  status[s]: 0 converged, 1 max_rounds reached.  Returns
+   the worst status, or -1 / -2 as above. */
+int shape_line_searches(shape_eval_fn eval, void *ctx, int nsearch, int k, const double *x0, const double *step,
+                        double tol, double abstol, int max_rounds, double *xmin, double *fmin, int *status,
+                        int *rounds);
+/* the same over a batch whose evaluations form nsearch groups of k = nevals
+   / nsearch consecutive evaluations, each group setting one parameter (of
+   any of the batch's models): many models' searches in one replay per
+   round.  -3: the groups do not each set one common parameter. */
+int shape_batch_line_searches(shape_batch *b, int nsearch, const double *x0, const double *step, double tol,
+                              double abstol, int max_rounds, double *xmin, double *fmin, int *status, int *rounds);
+/* One pass of bestfit's loop over the fitted parameters (bestfit.c:392-470,
+   mnbrak + brent_abs per parameter): every evaluation of b sets the same P
+   parameters in the same order; for p = 0 .. P-1 parameter p is searched
+   from x[p] with step[p], tol[p], abstol[p] (bestfit's fparstep, fpartol,
+   fparabstol), nevals trial values per round, the others held at x, and
+   x[p] takes the minimum.  fmin: the objective at the pass's end; rounds:
+   replays taken.  Returns 0, 1 (a search reached max_rounds), -1, -2, -3. */
+int shape_batch_fit_pass(shape_batch *b, const double *step, const double *tol, const double *abstol, int max_rounds,
+                         double *x, double *fmin, int *rounds);
+
 /* measurement: the batch's hipStream_t; per-kernel HIP-event timing of eager
    replays (batches created without SHAPE_BATCH_GRAPH); and the last replay's
    footprint: model-rectangle pixels, POS pixels, occupied image cells, image

@@ -10,6 +10,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <cmath>
 #include <vector>
 
 #include "common.h"
@@ -147,6 +148,78 @@ extern "C" int shape_batch_line_search(shape_batch *B, double x0, double step, d
       return -3;
   }
   return shape_line_search(batch_search_eval, B, B->E.nevals, x0, step, tol, abstol, max_rounds, xmin, fmin, rounds);
+}
+
+// Many searches in one batch: evaluations [s k, (s + 1) k) form search s,
+// each setting the same single parameter (of any model).
+extern "C" int shape_batch_line_searches(shape_batch *B, int nsearch, const double *x0, const double *step,
+                                         double tol, double abstol, int max_rounds, double *xmin, double *fmin,
+                                         int *status, int *rounds) {
+  if (!B || nsearch < 1 || B->E.nevals % nsearch != 0) return -1;
+  const int k = B->E.nevals / nsearch;
+  for (int s = 0; s < nsearch; s++) {
+    const auto &t0 = B->E.eval_targets[(size_t)s * k];
+    for (int e = s * k; e < (s + 1) * k; e++) {
+      const auto &t = B->E.eval_targets[e];
+      if (B->ntarg[e] != 1 || t.size() != 1 || t0.size() != 1 || t[0].kind != t0[0].kind ||
+          t[0].model != t0[0].model || t[0].offset != t0[0].offset)
+        return -3;
+    }
+  }
+  return shape_line_searches(batch_search_eval, B, nsearch, k, x0, step, tol, abstol, max_rounds, xmin, fmin,
+                             status, rounds);
+}
+
+// One pass of bestfit's loop over the fitted parameters (bestfit.c:392-470):
+// every evaluation of the batch sets the same P parameters in the same
+// order; parameter p is searched (k = nevals trial values per round) with
+// the others held at x, then x[p] takes the minimum found.
+namespace {
+struct PassCtx {
+  shape_batch *B;
+  int P, p;
+  const double *x;
+  std::vector<double> vals;
+};
+int pass_eval(void *ctx, int k, const double *xs, double *f) {
+  PassCtx &C = *(PassCtx *)ctx;
+  if (k != C.B->E.nevals) return 1;
+  for (int e = 0; e < k; e++)
+    for (int q = 0; q < C.P; q++) C.vals[(size_t)e * C.P + q] = q == C.p ? xs[e] : C.x[q];
+  int rc = shape_batch_set_values(C.B, C.vals.data());
+  return rc ? rc : shape_batch_run(C.B, f, nullptr);
+}
+}  // namespace
+
+extern "C" int shape_batch_fit_pass(shape_batch *B, const double *step, const double *tol, const double *abstol,
+                                    int max_rounds, double *x, double *fmin, int *rounds) {
+  if (!B || !step || !tol || !abstol || !x || B->E.nevals < 3) return -1;
+  const int P = B->ntarg[0];
+  if (P < 1) return -1;
+  const auto &t0 = B->E.eval_targets[0];
+  for (int e = 1; e < B->E.nevals; e++) {
+    const auto &t = B->E.eval_targets[e];
+    if (B->ntarg[e] != P || t.size() != t0.size()) return -3;
+    for (int q = 0; q < P; q++)
+      if (t[q].kind != t0[q].kind || t[q].model != t0[q].model || t[q].offset != t0[q].offset) return -3;
+  }
+  PassCtx C{B, P, 0, x, std::vector<double>((size_t)B->E.nevals * P)};
+  int worst = 0, total = 0;
+  double fm = HUGE_VAL;
+  for (int p = 0; p < P; p++) {
+    C.p = p;
+    double xm = 0.0;
+    int r = 0;
+    const int rc = shape_line_search(pass_eval, &C, B->E.nevals, x[p], step[p], tol[p], abstol[p], max_rounds, &xm,
+                                     &fm, &r);
+    if (rc < 0) return rc;
+    worst = std::max(worst, rc);
+    total += r;
+    x[p] = xm;
+  }
+  if (fmin) *fmin = fm;
+  if (rounds) *rounds = total;
+  return worst;
 }
 
 extern "C" float shape_batch_last_ms(shape_batch *B) { return B->last_ms; }

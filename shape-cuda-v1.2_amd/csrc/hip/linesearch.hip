@@ -58,36 +58,51 @@ double nearest_gap(const std::vector<Pt> &pts, double x) {
   return g;
 }
 
-}  // namespace
-
-extern "C" int shape_line_search(shape_eval_fn eval, void *ctx, int k, double x0, double step, double tol,
-                                 double abstol, int max_rounds, double *xmin, double *fmin, int *rounds) {
-  if (!eval || !xmin || !fmin || k < 3 || k > SHAPE_SEARCH_MAX_K || !std::isfinite(x0) || !std::isfinite(step) ||
-      step == 0.0 || !(tol >= 0.0) || max_rounds < 1)
-    return -1;
-  const double atol = std::max(abstol, 1e-10);
-  std::vector<double> xs(k), fs(k);
+// One search's state between rounds: propose() writes the next round's k
+// trial values (the first nnew new, the rest repeating the first), accept()
+// takes their objective values.  A search that has stopped proposes its
+// best point k times (its results are dropped), so several searches can
+// share every round's replay.
+struct Search {
+  int k = 0, max_rounds = 0, r = 0, status = 1;
+  bool done = false, started = false;
+  double x0 = 0, step = 0, tol = 0, atol = 0;
   std::vector<Pt> pts;
-  int r = 0;
-  // evaluate xs[0, nnew) (the rest of the k slots repeat xs[0]; their results are dropped)
-  auto run = [&](int nnew) -> int {
-    for (int q = nnew; q < k; q++) xs[q] = xs[0];
-    if (eval(ctx, k, xs.data(), fs.data())) return -2;
-    for (int q = 0; q < nnew; q++) insert_point(pts, xs[q], fs[q]);
-    r++;
-    return 0;
-  };
-  for (int q = 0; q < k; q++) {
-    const double mag = q == 0 ? 0.0 : std::pow(kGold, (q - 1) / 2);
-    xs[q] = x0 + step * ((q & 1) ? mag : -mag);
-  }
-  if (run(k)) return -2;
-  int status = 1;
   std::vector<double> cand;
-  for (;;) {
+  int nnew = 0;
+
+  int init(int k_, double x0_, double step_, double tol_, double abstol, int max_rounds_) {
+    if (k_ < 3 || k_ > SHAPE_SEARCH_MAX_K || !std::isfinite(x0_) || !std::isfinite(step_) || step_ == 0.0 ||
+        !(tol_ >= 0.0) || max_rounds_ < 1)
+      return -1;
+    k = k_; x0 = x0_; step = step_; tol = tol_; atol = std::max(abstol, 1e-10); max_rounds = max_rounds_;
+    return 0;
+  }
+  double best_x() const { return pts.empty() ? x0 : pts[lowest(pts)].x; }
+  double best_f() const { return pts.empty() ? HUGE_VAL : pts[lowest(pts)].f; }
+  void fill(double *xs, int n) {
+    nnew = n;
+    for (int q = n; q < k; q++) xs[q] = xs[0];
+  }
+  // the next round's points; false when the search has stopped
+  bool propose(double *xs) {
+    if (done) {
+      for (int q = 0; q < k; q++) xs[q] = best_x();
+      nnew = 0;
+      return false;
+    }
+    if (!started) {
+      started = true;
+      for (int q = 0; q < k; q++) {
+        const double mag = q == 0 ? 0.0 : std::pow(kGold, (q - 1) / 2);
+        xs[q] = x0 + step * ((q & 1) ? mag : -mag);
+      }
+      fill(xs, k);
+      return true;
+    }
     const int m = lowest(pts), n = (int)pts.size();
     if (m == 0 || m == n - 1) {  // lowest at an end: continue past it
-      if (r >= max_rounds) break;
+      if (r >= max_rounds) return stop(1, xs);
       const int dir = m == 0 ? -1 : 1;
       const double edge = pts[m].x, span = std::fabs(edge - pts[m - dir].x);
       double d = span;
@@ -95,16 +110,13 @@ extern "C" int shape_line_search(shape_eval_fn eval, void *ctx, int k, double x0
         d *= kGold;
         xs[q] = edge + dir * d;
       }
-      if (run(k)) return -2;
-      continue;
+      fill(xs, k);
+      return true;
     }
     const double a = pts[m - 1].x, b = pts[m + 1].x, x = pts[m].x;
     const double tol1 = tol * std::fabs(x) + atol, xm = 0.5 * (a + b);
-    if (std::fabs(x - xm) <= 2.0 * tol1 - 0.5 * (b - a)) {
-      status = 0;
-      break;
-    }
-    if (r >= max_rounds) break;
+    if (std::fabs(x - xm) <= 2.0 * tol1 - 0.5 * (b - a)) return stop(0, xs);
+    if (r >= max_rounds) return stop(1, xs);
     cand.clear();
     // vertex of the parabola through (a, fa), (x, fx), (b, fb)
     const double fa = pts[m - 1].f, fx = pts[m].f, fb = pts[m + 1].f;
@@ -121,22 +133,75 @@ extern "C" int shape_line_search(shape_eval_fn eval, void *ctx, int k, double x0
     const int ng = k - (int)cand.size();
     for (int i = 1; i <= ng; i++) cand.push_back(a + (b - a) * i / (ng + 1));
     // new points inside (a, b), at least tol1 / 2 from the evaluated ones and each other
-    int nnew = 0;
+    int nn = 0;
     for (double c : cand) {
       if (!(c > a && c < b) || nearest_gap(pts, c) < 0.5 * tol1) continue;
       bool close = false;
-      for (int s = 0; s < nnew; s++) close |= std::fabs(xs[s] - c) < 0.5 * tol1;
-      if (!close) xs[nnew++] = c;
+      for (int s2 = 0; s2 < nn; s2++) close |= std::fabs(xs[s2] - c) < 0.5 * tol1;
+      if (!close) xs[nn++] = c;
     }
-    if (nnew == 0) {  // nothing left to resolve at this tolerance
-      status = 0;
-      break;
-    }
-    if (run(nnew)) return -2;
+    if (nn == 0) return stop(0, xs);  // nothing left to resolve at this tolerance
+    fill(xs, nn);
+    return true;
   }
-  const int m = lowest(pts);
-  *xmin = pts[m].x;
-  *fmin = pts[m].f;
+  bool stop(int st, double *xs) {
+    status = st;
+    done = true;
+    return propose(xs);
+  }
+  void accept(const double *xs, const double *fs) {
+    for (int q = 0; q < nnew; q++) insert_point(pts, xs[q], fs[q]);
+    if (nnew > 0) r++;
+  }
+};
+
+}  // namespace
+
+extern "C" int shape_line_search(shape_eval_fn eval, void *ctx, int k, double x0, double step, double tol,
+                                 double abstol, int max_rounds, double *xmin, double *fmin, int *rounds) {
+  if (!eval || !xmin || !fmin) return -1;
+  Search S;
+  if (S.init(k, x0, step, tol, abstol, max_rounds)) return -1;
+  std::vector<double> xs(k), fs(k);
+  while (S.propose(xs.data())) {
+    if (eval(ctx, k, xs.data(), fs.data())) return -2;
+    S.accept(xs.data(), fs.data());
+  }
+  *xmin = S.best_x();
+  *fmin = S.best_f();
+  if (rounds) *rounds = S.r;
+  return S.status;
+}
+
+// nsearch independent searches, k trial values each, sharing every round:
+// eval receives nsearch * k values (search s at [s k, (s + 1) k)).  A search
+// that has stopped repeats its best point until the others stop.  rounds:
+// the rounds (calls of eval) taken.  Returns 0 (every search converged), 1
+// (some reached max_rounds), -1 bad arguments, -2 an evaluation failed.
+extern "C" int shape_line_searches(shape_eval_fn eval, void *ctx, int nsearch, int k, const double *x0,
+                                   const double *step, double tol, double abstol, int max_rounds, double *xmin,
+                                   double *fmin, int *status, int *rounds) {
+  if (!eval || !x0 || !step || !xmin || !fmin || nsearch < 1) return -1;
+  std::vector<Search> S(nsearch);
+  for (int s = 0; s < nsearch; s++)
+    if (S[s].init(k, x0[s], step[s], tol, abstol, max_rounds)) return -1;
+  std::vector<double> xs((size_t)nsearch * k), fs((size_t)nsearch * k);
+  int r = 0;
+  for (;;) {
+    bool any = false;
+    for (int s = 0; s < nsearch; s++) any |= S[s].propose(xs.data() + (size_t)s * k);
+    if (!any) break;
+    if (eval(ctx, nsearch * k, xs.data(), fs.data())) return -2;
+    r++;
+    for (int s = 0; s < nsearch; s++) S[s].accept(xs.data() + (size_t)s * k, fs.data() + (size_t)s * k);
+  }
+  int worst = 0;
+  for (int s = 0; s < nsearch; s++) {
+    xmin[s] = S[s].best_x();
+    fmin[s] = S[s].best_f();
+    if (status) status[s] = S[s].status;
+    worst = std::max(worst, S[s].status);
+  }
   if (rounds) *rounds = r;
-  return status;
+  return worst;
 }

@@ -178,6 +178,17 @@ def load_hip():
             "shape_batch_line_search": (C.c_int, [vp, C.c_double, C.c_double, C.c_double, C.c_double, C.c_int,
                                                   C.POINTER(C.c_double), C.POINTER(C.c_double),
                                                   C.POINTER(C.c_int)]),
+            "shape_line_searches": (C.c_int, [SEARCH_FN, vp, C.c_int, C.c_int, C.POINTER(C.c_double),
+                                              C.POINTER(C.c_double), C.c_double, C.c_double, C.c_int,
+                                              C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_int),
+                                              C.POINTER(C.c_int)]),
+            "shape_batch_line_searches": (C.c_int, [vp, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double),
+                                                    C.c_double, C.c_double, C.c_int, C.POINTER(C.c_double),
+                                                    C.POINTER(C.c_double), C.POINTER(C.c_int),
+                                                    C.POINTER(C.c_int)]),
+            "shape_batch_fit_pass": (C.c_int, [vp, C.POINTER(C.c_double), C.POINTER(C.c_double),
+                                               C.POINTER(C.c_double), C.c_int, C.POINTER(C.c_double),
+                                               C.POINTER(C.c_double), C.POINTER(C.c_int)]),
             "shape_batch_last_ms": (C.c_float, [vp]),
             "shape_batch_nframes": (C.c_int, [vp, C.c_int]),
             "shape_batch_frame_results": (C.c_int, [vp, C.c_int, C.POINTER(C.c_double),
@@ -567,6 +578,35 @@ class Batch:
         if rc < 0:
             raise RuntimeError("shape_batch_line_search failed (%d)" % rc)
         return rc, xmin.value, fmin.value, rounds.value
+
+    def line_searches(self, x0, step, tol, abstol, max_rounds=40):
+        """One search per group of nevals / len(x0) consecutive evaluations
+        (shape_batch_line_searches); returns (status, xmin[], fmin[],
+        status[], rounds)."""
+        ns = len(x0)
+        dp = C.POINTER(C.c_double)
+        a0, st = np.ascontiguousarray(x0, dtype=float), np.ascontiguousarray(step, dtype=float)
+        xm, fm, ss = np.zeros(ns), np.zeros(ns), np.zeros(ns, dtype=np.int32)
+        r = C.c_int()
+        rc = self.lib.shape_batch_line_searches(self.h, ns, a0.ctypes.data_as(dp), st.ctypes.data_as(dp), tol, abstol,
+                                                max_rounds, xm.ctypes.data_as(dp), fm.ctypes.data_as(dp),
+                                                ss.ctypes.data_as(C.POINTER(C.c_int)), C.byref(r))
+        if rc < 0:
+            raise RuntimeError("shape_batch_line_searches failed (%d)" % rc)
+        return rc, xm, fm, ss, r.value
+
+    def fit_pass(self, x, step, tol, abstol, max_rounds=40):
+        """bestfit's pass over the parameters every evaluation sets
+        (shape_batch_fit_pass); returns (status, x after the pass, fmin, rounds)."""
+        dp = C.POINTER(C.c_double)
+        xx = np.array(x, dtype=float)
+        st, tl, ab = (np.ascontiguousarray(v, dtype=float) for v in (step, tol, abstol))
+        fm, r = C.c_double(), C.c_int()
+        rc = self.lib.shape_batch_fit_pass(self.h, st.ctypes.data_as(dp), tl.ctypes.data_as(dp), ab.ctypes.data_as(dp),
+                                           max_rounds, xx.ctypes.data_as(dp), C.byref(fm), C.byref(r))
+        if rc < 0:
+            raise RuntimeError("shape_batch_fit_pass failed (%d)" % rc)
+        return rc, xx, fm.value, r.value
 
     def launch(self):
         rc = self.lib.shape_batch_launch(self.h)

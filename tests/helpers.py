@@ -95,6 +95,43 @@ def make_scene_with_obs(config, snr=20.0, **over):
     return sc
 
 
+def oracle_threads():
+    """Host threads for the oracle: the process's CPU share, at most 16."""
+    return max(1, min(16, len(os.sched_getaffinity(0))))
+
+
+def oracle_forward_threads(orc, sc):
+    """The oracle's chain with frames dealt to host threads (scenes with one
+    POS per frame, pos_scope LOCAL); returns chi2."""
+    orc.oracle_realize_mod(sc.par, sc.mod)
+    orc.oracle_realize_spin(sc.par, sc.mod, sc.dat)
+    orc.oracle_realize_delcor(sc.dat, 0.0, 0)
+    orc.oracle_calc_fits_threads(sc.par, sc.mod, sc.dat, oracle_threads())
+    return orc.oracle_chi2(sc.par, sc.dat, 0)
+
+
+def truth_scene(config, **over):
+    """Scene with one POS per frame whose observations are the oracle's truth
+    fits plus noise, the fits computed on the host's threads."""
+    orc = load_oracle()
+    sc = sh.Scene(sh.spec_for(config, pos_scope=sh.LOCAL, **over))
+    oracle_forward_threads(orc, sc)
+    sc.synthesize_obs(snr=20.0)
+    return sc
+
+
+def oracle_objective(orc, sc, values):
+    """The oracle's objective with parameters {index: value} set, restored after."""
+    saved = {i: sc.get_param(i) for i in values}
+    for i, v in values.items():
+        sc.set_param(i, v)
+    oracle_forward_threads(orc, sc)
+    f = orc.oracle_objective_tail(sc.par, sc.dat)
+    for i, v in saved.items():
+        sc.set_param(i, v)
+    return f
+
+
 def copy_obs(src, dst):
     for s, f in src.all_frames():
         ff = max(f, 0)

@@ -29,8 +29,6 @@ of itself (2e-7 of the peak).  Dim image pixels fed by few POS pixels carry
 that; above 1% of the peak every pixel sums enough POS pixels to stay within
 1e-4.
 """
-import os
-
 import numpy as np
 import pytest
 
@@ -39,26 +37,15 @@ import helpers as H
 sh = H.sh
 pytestmark = pytest.mark.gpu
 
-NT = max(1, min(16, os.cpu_count() or 1))
 PIX_FLOOR = 1e-2  # per-pixel relative error is held above this fraction of the frame's peak
 
 
 def _oracle_truth_scene(config, **over):
-    """Scene (one POS per frame) whose observations are the oracle's truth
-    fits plus noise, the fits computed on NT host threads."""
-    orc = H.load_oracle()
-    sc = sh.Scene(sh.spec_for(config, pos_scope=sh.LOCAL, **over))
-    _oracle_forward(orc, sc)
-    sc.synthesize_obs(snr=20.0)
-    return sc
+    return H.truth_scene(config, **over)
 
 
 def _oracle_forward(orc, sc):
-    orc.oracle_realize_mod(sc.par, sc.mod)
-    orc.oracle_realize_spin(sc.par, sc.mod, sc.dat)
-    orc.oracle_realize_delcor(sc.dat, 0.0, 0)
-    orc.oracle_calc_fits_threads(sc.par, sc.mod, sc.dat, NT)
-    return orc.oracle_chi2(sc.par, sc.dat, 0)
+    return H.oracle_forward_threads(orc, sc)
 
 
 def _oracle_eval(orc, sc, trial, images=True):

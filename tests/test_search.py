@@ -118,3 +118,52 @@ def test_batched_search_limits_and_errors():
     assert lib.shape_line_search(cb, None, 8, 0.0, 0.0, 1e-6, 0.0, 10, C.byref(x), C.byref(fv), C.byref(r)) == -1
     bad = sh.SEARCH_FN(lambda ctx, k, xs, out: 1)
     assert lib.shape_line_search(bad, None, 8, 0.0, 1.0, 1e-6, 0.0, 10, C.byref(x), C.byref(fv), C.byref(r)) == -2
+
+
+def _run_searches(k, tol, abstol, max_rounds=40):
+    """shape_line_searches over every FUNCS objective at once (search s gets
+    x values [s k, (s + 1) k) of each round)."""
+    lib = sh.load_hip()
+    ns = len(FUNCS)
+    calls = [0]
+
+    def cb(ctx, kk, x, out):
+        assert kk == ns * k
+        calls[0] += 1
+        for q in range(kk):
+            out[q] = FUNCS[q // k][0](x[q])
+        return 0
+    fn = sh.SEARCH_FN(cb)
+    dp = C.POINTER(C.c_double)
+    x0 = np.array([f[2] for f in FUNCS])
+    st = np.array([f[3] for f in FUNCS])
+    xm, fm, ss = np.zeros(ns), np.zeros(ns), np.zeros(ns, dtype=np.int32)
+    r = C.c_int()
+    rc = lib.shape_line_searches(fn, None, ns, k, x0.ctypes.data_as(dp), st.ctypes.data_as(dp), tol, abstol,
+                                 max_rounds, xm.ctypes.data_as(dp), fm.ctypes.data_as(dp),
+                                 ss.ctypes.data_as(C.POINTER(C.c_int)), C.byref(r))
+    return rc, xm, fm, ss, r.value, calls[0]
+
+
+@pytest.mark.parametrize("k", [4, 8])
+def test_searches_sharing_rounds_equal_single_searches(k):
+    """Searches sharing every round (shape_line_searches) reach exactly the
+    minimum each reaches alone (shape_line_search): a search that stops early
+    only repeats its best point; the shared run takes as many rounds as the
+    slowest search."""
+    tol, abstol = 1e-6, 1e-10
+    rc, xm, fm, ss, rounds, calls = _run_searches(k, tol, abstol)
+    assert rc == 0 and list(ss) == [0] * len(FUNCS) and calls == rounds
+    most = 0
+    for s, (f, xstar, x0, step) in enumerate(FUNCS):
+        rc1, x1, f1, r1, _ = sh.line_search(f, k, x0, step, tol, abstol)
+        assert rc1 == 0
+        assert xm[s] == x1 and fm[s] == f1, (s, xm[s], x1)
+        assert abs(xm[s] - xstar) <= 2 * _tol1(xstar, tol, abstol)
+        most = max(most, r1)
+    assert rounds == most
+
+
+def test_searches_report_max_rounds():
+    rc, xm, fm, ss, rounds, calls = _run_searches(4, 1e-12, 0.0, max_rounds=2)
+    assert rc == 1 and rounds == 2 and 1 in list(ss)
