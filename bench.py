@@ -48,13 +48,16 @@ def log(msg):
 def parse():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--evals", type=int, default=4, help="objective evaluations per replay")
     ap.add_argument("--frames", type=int, default=NFRAMES_FULL, help="delay-Doppler frames (256: full workload)")
     ap.add_argument("--fp64-steps", type=int, default=3)
     ap.add_argument("--kernel-steps", type=int, default=5, help="eager replays timed per kernel")
     ap.add_argument("--cpu-frames", type=int, default=NFRAMES_FULL, help="frames in the CPU-baseline sample")
+    ap.add_argument("--cpu-evals", type=int, default=20, help="timed oracle evaluations on all threads")
+    ap.add_argument("--cpu1-frames", type=int, default=16, help="frames in the 1-thread CPU sample")
+    ap.add_argument("--cpu1-evals", type=int, default=4, help="timed 1-thread oracle evaluations")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only copy 1 GiB device to device (calibrates HBM byte counters), then exit")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
@@ -198,9 +201,26 @@ def kernel_roofline(sc, nev):
 
 
 # ---------------------------------------------------------------------------
-# CPU baseline: the oracle on the first --cpu-frames frames, all host threads
+# CPU baseline: the oracle (the FP64 CPU restatement of the reference's
+# path), timed on the host's cores beside the GPU, as the HostMultiThreading
+# variant partitions the work (frames dealt to threads, one POS per frame)
 # ---------------------------------------------------------------------------
-def cpu_baseline(nframes):
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(nframes, nevals, nframes1, nevals1):
+    """Oracle evaluations/s of the large-model workload: nevals timed
+    evaluations (after one warm-up) of the first nframes frames on every
+    thread of the process's CPU share (at most 16: a one-GPU box's share),
+    and nevals1 single-thread evaluations of the first nframes1 frames; both
+    scaled to one 256-frame evaluation."""
     path = os.path.join(ROOT, "oracle", "_build", "liboracle.so")
     if not os.path.exists(path):
         return None
@@ -212,31 +232,43 @@ def cpu_baseline(nframes):
                             ("oracle_chi2", C.c_double, [vp, vp, C.c_int])]:
         fn = getattr(orc, name)
         fn.restype, fn.argtypes = res, args
-    threads = max(1, min(16, len(os.sched_getaffinity(0))))
-    # one POS per frame (LOCAL scope): the threads render frames concurrently
-    sc = sh.Scene(sh.spec_for(4, n_deldop=nframes, pos_scope=sh.LOCAL))
-
-    def forward():
-        orc.oracle_realize_mod(sc.par, sc.mod)
-        orc.oracle_realize_spin(sc.par, sc.mod, sc.dat)
-        orc.oracle_realize_delcor(sc.dat, 0.0, 0)
-        orc.oracle_calc_fits_threads(sc.par, sc.mod, sc.dat, threads)
-        return orc.oracle_chi2(sc.par, sc.dat, 0)
-
     orc.oracle_last_error.restype = C.c_char_p
-    forward()
-    if orc.oracle_last_error():
-        raise RuntimeError("oracle: %s" % orc.oracle_last_error().decode())
-    sc.synthesize_obs(snr=20.0)
-    t0 = time.perf_counter()
-    forward()
-    dt = time.perf_counter() - t0
-    sc.close()
-    return {"value": round(nframes / (NFRAMES_FULL * dt), 4), "unit": "chi-square evals/s",
-            "cores": threads, "kind": "port",
-            "sample": "oracle (FP64 CPU restatement) realize+calc_fits+chi2 over the first %d of the "
-                      "%d frames, frames spread over %d threads, %.2f s wall; scaled to one "
-                      "%d-frame evaluation" % (nframes, NFRAMES_FULL, threads, dt, NFRAMES_FULL)}
+    share = len(os.sched_getaffinity(0))
+    threads = max(1, min(16, share))
+
+    def rate(frames, evals, nt):
+        # one POS per frame (LOCAL scope): the threads render frames concurrently
+        sc = sh.Scene(sh.spec_for(4, n_deldop=frames, pos_scope=sh.LOCAL))
+
+        def forward():
+            orc.oracle_realize_mod(sc.par, sc.mod)
+            orc.oracle_realize_spin(sc.par, sc.mod, sc.dat)
+            orc.oracle_realize_delcor(sc.dat, 0.0, 0)
+            orc.oracle_calc_fits_threads(sc.par, sc.mod, sc.dat, nt)
+            return orc.oracle_chi2(sc.par, sc.dat, 0)
+
+        forward()  # warm-up, and the truth fits the observations come from
+        if orc.oracle_last_error():
+            raise RuntimeError("oracle: %s" % orc.oracle_last_error().decode())
+        sc.synthesize_obs(snr=20.0)
+        t0 = time.perf_counter()
+        for _ in range(evals):
+            forward()
+        dt = time.perf_counter() - t0
+        sc.close()
+        return evals * frames / (NFRAMES_FULL * dt), dt
+
+    v, dt = rate(nframes, nevals, threads)
+    v1, dt1 = rate(nframes1, nevals1, 1)
+    return {"value": round(v, 4), "unit": "chi-square evals/s", "cores": threads, "kind": "port",
+            "sample": "oracle (FP64 CPU restatement of the reference's path) realize+calc_fits+chi2 of the "
+                      "large-model workload: %d timed evaluations of the first %d of its %d frames after one "
+                      "warm-up, frames dealt to %d threads, %.1f s wall; scaled to one %d-frame evaluation"
+                      % (nevals, nframes, NFRAMES_FULL, threads, dt, NFRAMES_FULL),
+            "one_thread": {"value": round(v1, 5), "unit": "chi-square evals/s", "cores": 1,
+                           "sample": "%d evaluations of the first %d frames on one thread, %.1f s wall; scaled "
+                                     "to one %d-frame evaluation" % (nevals1, nframes1, dt1, NFRAMES_FULL)},
+            "nproc": os.cpu_count(), "cpu_share": share, "cpu_model": cpu_model()}
 
 
 def copy_probe():
@@ -516,8 +548,9 @@ def main():
     sc.close()
 
     if not a.no_cpu:
-        log("CPU baseline (oracle, %d-frame sample)" % a.cpu_frames)
-        out["cpu_baseline"] = cpu_baseline(a.cpu_frames)
+        log("CPU baseline (oracle: %d x %d frames on all threads, %d x %d frames on one)"
+            % (a.cpu_evals, a.cpu_frames, a.cpu1_evals, a.cpu1_frames))
+        out["cpu_baseline"] = cpu_baseline(a.cpu_frames, a.cpu_evals, a.cpu1_frames, a.cpu1_evals)
     print(json.dumps(out), flush=True)
 
 

@@ -119,6 +119,11 @@ int shape_hip_last_error(void);       /* 0, or the code of the last failure */
 void shape_hip_clear_error(void);
 void shape_hip_set_strict(int write_pos); /* 1: calc_fits also writes POS arrays back */
 int shape_hip_device_count(void);
+/* Self-test of the device's wave-level primitives (no counterpart in the
+ * reference): over 256 input ints, per thread: the inclusive prefix sum, min,
+ * max and sum over its 64-lane wave, and the FP32 min and max (as bit
+ * patterns).  out holds 6 x 256 ints.  0 on success. */
+int shape_hip_wave_selftest(const int *in, int *out);
 void shape_hip_realize_delcor(struct par_t *par, struct mod_t *mod, struct dat_t *dat,
                               double delta_delcor0, int delcor0_mode);
 

@@ -226,7 +226,7 @@ __device__ inline void lds_append(bool need, unsigned int e, void *list, int *cn
   if (!mask) return;
   int lane = threadIdx.x & 63, leader = __ffsll((long long)mask) - 1, base = 0;
   if (lane == leader) base = atomicAdd(cnt, __popcll(mask));
-  base = __shfl(base, leader, 64);
+  base = __builtin_amdgcn_readlane(base, leader);
   if (need) {
     int q = base + __popcll(mask & ((1ULL << lane) - 1ULL));
     if (q < cap) static_cast<unsigned int *>(list)[q] = e;
@@ -430,15 +430,10 @@ __global__ __launch_bounds__(256, 6) void k_tile32(const ViewRec *__restrict__ v
         // one reservation for the whole wave's events: exclusive prefix sum
         // of the lanes' counts, one LDS atomic by the last lane
         const int ne = __popcll(covm);
-        int incl = ne;
-#pragma unroll
-        for (int o = 1; o < 64; o <<= 1) {
-          const int y = __shfl_up(incl, o, 64);
-          if (lane >= o) incl += y;
-        }
+        const int incl = wave_incl_scan(ne);
         int base = 0;
         if (lane == 63) base = atomicAdd(&nev, incl);
-        base = __shfl(base, 63, 64) + incl - ne;
+        base = __builtin_amdgcn_readlane(base, 63) + incl - ne;
         // row of cell c: (c + 0.5) / wj lies at least 0.5 / wj from an
         // integer, far beyond the FP32 error, so the float quotient is exact
         // for the boxes whose events count (nc <= 64)

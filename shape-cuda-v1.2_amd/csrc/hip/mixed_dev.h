@@ -10,10 +10,12 @@ namespace shp {
 constexpr int TILE = 16;         // resolve tiles: 16x16 pixels per workgroup of 256 threads
 constexpr int TPP = TILE * TILE / 256;  // pixels per thread
 constexpr int WIN_MAX = 2048;    // LDS delay-Doppler window (FP64 bins, 16 KiB)
-// Footprints of at most DW delay rows x PW Doppler bins take the unrolled
-// register scatter; wider ones (several samples per baud, subpixel sinc^2
-// means over pixels spanning many bins) take a general loop.
-constexpr int DW = 4, PW = 16;
+// The LDS scatter adds a footprint's delay rows DW at a time, unrolled (a
+// row past the footprint is a masked lane op, not a skipped one).  One
+// sample per baud puts a pixel's power into two delay rows (the triangle's
+// half-width is one row), so DW = 2 wastes no adds there; more samples per
+// baud take more groups.
+constexpr int DW = 2;
 
 // Workgroups are dealt to the chip's 8 XCDs round-robin by linear block id.
 // With the grid's x size nb a multiple of 8 (pad_xcd), remap so that XCD k
@@ -25,11 +27,58 @@ constexpr int NXCD = 8;
 __device__ inline int xcd_block(int b, int nb) { return (b % NXCD) * (nb / NXCD) + b / NXCD; }
 inline unsigned pad_xcd(unsigned n) { return (n + NXCD - 1) / NXCD * NXCD; }
 
-__device__ inline int wmin(int v) { for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o, 64)); return v; }
-__device__ inline int wmax(int v) { for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, 64)); return v; }
-__device__ inline int wsum(int v) { for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64); return v; }
-__device__ inline float wminf(float v) { for (int o = 32; o > 0; o >>= 1) v = fminf(v, __shfl_xor(v, o, 64)); return v; }
-__device__ inline float wmaxf(float v) { for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, 64)); return v; }
+// Wave-wide reductions (every lane of the wave active), returned to every
+// lane: DPP lane shifts within each 16-lane row leave the row's result in
+// its lane 15 (a lane whose DPP source falls outside its row takes the
+// identity); the four rows' results are read out and combined.  No LDS
+// round trips.
+template <int CTRL>
+__device__ inline int dpp_i(int v, int id) { return __builtin_amdgcn_update_dpp(id, v, CTRL, 0xf, 0xf, false); }
+// Five wait states between a write of EXEC (where diverged lanes rejoin)
+// and a DPP op that reads v: without them the DPP op may still run under the
+// old EXEC, leaving rejoining lanes out of the lane shifts.  The compiler
+// inserts them after vector writes of EXEC but not after scalar ones.
+__device__ inline int dpp_exec_guard(int v) {
+  asm volatile("s_nop 4" : "+v"(v));
+  return v;
+}
+template <typename Op>
+__device__ inline int wave_reduce_i(int v, int id, Op op) {
+  v = dpp_exec_guard(v);
+  v = op(v, dpp_i<0x111>(v, id));  // row_shr:1
+  v = op(v, dpp_i<0x112>(v, id));  // row_shr:2
+  v = op(v, dpp_i<0x114>(v, id));  // row_shr:4
+  v = op(v, dpp_i<0x118>(v, id));  // row_shr:8
+  return op(op(__builtin_amdgcn_readlane(v, 15), __builtin_amdgcn_readlane(v, 31)),
+            op(__builtin_amdgcn_readlane(v, 47), __builtin_amdgcn_readlane(v, 63)));
+}
+__device__ inline int wmin(int v) { return wave_reduce_i(v, 0x7fffffff, [](int a, int b) { return min(a, b); }); }
+__device__ inline int wmax(int v) { return wave_reduce_i(v, -0x7fffffff - 1, [](int a, int b) { return max(a, b); }); }
+__device__ inline int wsum(int v) { return wave_reduce_i(v, 0, [](int a, int b) { return a + b; }); }
+__device__ inline float wminf(float v) {
+  return __int_as_float(wave_reduce_i(__float_as_int(v), __float_as_int(HUGE_VALF),
+                                      [](int a, int b) { return __float_as_int(fminf(__int_as_float(a), __int_as_float(b))); }));
+}
+__device__ inline float wmaxf(float v) {
+  return __int_as_float(wave_reduce_i(__float_as_int(v), __float_as_int(-HUGE_VALF),
+                                      [](int a, int b) { return __float_as_int(fmaxf(__int_as_float(a), __int_as_float(b))); }));
+}
+
+// Inclusive prefix sum over the 64 lanes of a wave (every lane active): DPP
+// lane shifts within each 16-lane row, then the totals of the rows before a
+// lane's own, read out of lanes 15, 31 and 47.
+__device__ inline int wave_incl_scan(int v0) {
+  v0 = dpp_exec_guard(v0);
+  int v = v0 + __builtin_amdgcn_update_dpp(0, v0, 0x111, 0xf, 0xf, true);  // row_shr:1
+  v += __builtin_amdgcn_update_dpp(0, v0, 0x112, 0xf, 0xf, true);          // row_shr:2
+  v += __builtin_amdgcn_update_dpp(0, v0, 0x113, 0xf, 0xf, true);          // row_shr:3
+  v += __builtin_amdgcn_update_dpp(0, v, 0x114, 0xf, 0xe, true);           // row_shr:4, lanes 4-15 of a row
+  v += __builtin_amdgcn_update_dpp(0, v, 0x118, 0xf, 0xc, true);           // row_shr:8, lanes 8-15
+  const int t0 = __builtin_amdgcn_readlane(v, 15), t1 = __builtin_amdgcn_readlane(v, 31),
+            t2 = __builtin_amdgcn_readlane(v, 47);
+  const int row = (threadIdx.x & 63) >> 4;
+  return v + (row > 0 ? t0 : 0) + (row > 1 ? t1 : 0) + (row > 2 ? t2 : 0);
+}
 
 // block-wide minimum of four ints at once (one barrier pair)
 __device__ inline void blk_min4(int &a, int &b, int &c, int &d, int (*sh)[4]) {
@@ -233,7 +282,7 @@ __device__ inline int stale_slot(const PosDev &pd, bool need) {
     base = atomicAdd(pd.nstale, __popcll(mask));
     atomicAdd(pd.ncollect, __popcll(mask));  // candidates come from the global raster
   }
-  base = __shfl(base, leader, 64);
+  base = __builtin_amdgcn_readlane(base, leader);
   if (!need) return -1;
   int si = base + __popcll(mask & ((1ULL << lane) - 1ULL));
   if (si >= pd.stale_cap) {
@@ -451,7 +500,6 @@ __device__ inline float radar_cl(const ViewRec &V, const GeomSlot &G, const doub
 // cross section per unit weight.
 struct RadarPix {
   bool inb;
-  bool wide;         // more than DW rows or PW bins: the general scatter loop
   bool sinc1;        // single-sample sinc^2 Doppler response
   float amp, delrel, doprel;
   int id0, id1, ip0, ip1;
@@ -502,7 +550,6 @@ __device__ inline void radar_pixel(const ViewRec &V, const double *pl, int i, in
 #pragma clang fp contract(off)
   const bool dd = V.type == VT_DELAY;
   int width = V.sinc2width, ns = V.nsinc2;
-  R.wide = false;
   // Bin coordinates and the integer footprint decisions in FP64, as the CPU
   // forms them (pos2deldop.c:320-329): a footprint that straddles a frame edge
   // moves a pixel's whole power between the image and the overflow, so these
@@ -534,7 +581,6 @@ __device__ inline void radar_pixel(const ViewRec &V, const double *pl, int i, in
     for (int b = 0; b < np; b++) sump += dop_weight(R.pfr, R.pm0, R.s2, b);
   else
     for (int b = 0; b < np; b++) sump += dop_w_sub(V, R.pfr, R.pm0, b);
-  R.wide = np > PW;
   float sumd = 1.0f;
   R.delrel = 0.0f;
   R.id0 = 0;
@@ -555,7 +601,6 @@ __device__ inline void radar_pixel(const ViewRec &V, const double *pl, int i, in
     const int nd = R.id1 - R.id0 + 1;
     sumd = 0.0f;
     for (int a = 0; a < nd; a++) sumd += del_w(V, R, a);
-    R.wide = R.wide || nd > DW;
     R.inb = (V.D0 + R.id0 >= 1) && (V.D0 + R.id1 <= V.ndel) && (V.P0 + R.ip0 >= 1) && (V.P0 + R.ip1 <= V.ndop);
   } else {
     R.inb = (V.P0 + R.ip0 >= 1) && (V.P0 + R.ip1 <= V.ndop);
@@ -597,7 +642,7 @@ __device__ inline void radar_scatter_lds(const ViewRec &V, const RadarPix &R, do
   const int np = R.ip1 - R.ip0 + 1, nd = dd ? R.id1 - R.id0 + 1 : 1;
   int b0 = threadIdx.x & 15;
   while (b0 >= np) b0 -= np;
-  // rows in groups of DW (one group unless the footprint is wide)
+  // rows in groups of DW
   for (int a0 = 0; a0 < nd; a0 += DW) {
     float ad[DW];
 #pragma unroll

@@ -163,6 +163,7 @@ def load_hip():
             "shape_hip_clear_error": (None, []),
             "shape_hip_set_strict": (None, [C.c_int]),
             "shape_hip_device_count": (C.c_int, []),
+            "shape_hip_wave_selftest": (C.c_int, [C.POINTER(C.c_int), C.POINTER(C.c_int)]),
             "shape_hip_realize_delcor": (None, [vp, vp, vp, C.c_double, C.c_int]),
             "shape_hip_last_replayed": (C.c_int, []),
             "shape_batch_create": (vp, [vp, C.POINTER(vp), C.POINTER(vp), C.c_int, C.c_int,
