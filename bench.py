@@ -284,6 +284,27 @@ def copy_probe():
     print(json.dumps({"copy_probe_bytes": n}), flush=True)
 
 
+# VALU issue peak of the MI355X: 256 CUs x 4 SIMDs, one wave64 VALU
+# instruction per SIMD every 4 cycles (a 16-lane SIMD takes 4 passes) at
+# 2.4 GHz: 614.4e9 wave-instructions/s.
+VALU_PEAK = 256 * 4 * 2.4e9 / 4
+
+
+def measured_valu(kernel, workload):
+    """VALU wave-instructions per launch of `kernel` on `workload` (the
+    SQ_INSTS_VALU counter summarized in profiles/sq_counters.json); None when
+    no summary for this kernel and workload exists."""
+    path = os.path.join(ROOT, "profiles", "sq_counters.json")
+    try:
+        rec = json.load(open(path))
+    except (OSError, ValueError):
+        return None
+    k = rec.get("kernels", {}).get(kernel)
+    if not k or rec.get("workload") != workload:
+        return None
+    return k.get("SQ_INSTS_VALU")
+
+
 def measured_traffic(kernel, workload):
     """HBM bytes per launch of `kernel` on `workload`, from the counter runs
     summarized in profiles/hbm_traffic.json (FETCH_SIZE + WRITE_SIZE, scaled by
@@ -536,6 +557,16 @@ def main():
                        "frac": None if ach is None else round(ach / HBM_PEAK_GBS, 4),
                        "traffic": measured_traffic(dom, wl), "avg_ms": r["avg_ms"],
                        "algorithmic_bytes": r["bytes"]}
+    # The kernel's limiter is VALU issue, not HBM: the counted VALU
+    # instructions per launch over the live launch duration, against the
+    # issue peak.
+    vi = measured_valu(dom, wl)
+    if vi is not None and r["avg_ms"]:
+        va = vi / (r["avg_ms"] * 1e-3)
+        out["roofline_valu"] = {"kernel": dom, "achieved": round(va / 1e9, 1), "peak": round(VALU_PEAK / 1e9, 1),
+                                "unit": "G wave-instr/s", "frac": round(va / VALU_PEAK, 4),
+                                "valu_instr_per_launch": vi, "source": "profiles/sq_counters.json"}
+        out["limiter"] = "valu"
     out["kernels"] = rows
     out["footprint"] = fp
     pb = path_bytes(fp, a.evals, sc.nv, sc.nf)
