@@ -61,6 +61,8 @@ def parse():
     ap.add_argument("--copy-probe", action="store_true",
                     help="only copy 1 GiB device to device (calibrates HBM byte counters), then exit")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-tune", action="store_true",
+                    help="keep the default launch settings (no shape_batch_tune before the timed replays)")
     ap.add_argument("--configs", type=str, default=None,
                     help="comma-separated workload configs (1-5) to report one JSON line each for: "
                          "mixed and FP64 rates, latency, accuracy, facet-map mismatch, CPU baselines")
@@ -117,6 +119,16 @@ def time_batch(B, steps, warmup):
         lat.append(B.last_ms())
     t1 = time.perf_counter()
     return (t1 - t0), np.array(lat)
+
+
+def tune_batch(B, a):
+    """The launch settings the replays run with: shape_batch_tune's choice
+    (timed before, outside the measured region) unless --no-tune."""
+    if not a.no_tune:
+        log("tuning launch settings")
+        B.tune(replays=3)
+    t = B.get_tune()
+    return {"tile_grid": t[0], "replay_grid": t[1], "collect_grid": t[2], "tuned": not a.no_tune}
 
 
 def time_dropin(sc, steps):
@@ -178,9 +190,11 @@ def image_error(sc, evals, nframes, stride=1):
     return worst
 
 
-def kernel_roofline(sc, nev):
+def kernel_roofline(sc, nev, tuned):
     B = sh.Batch([sc], trial_evals(sc, nev), method=sh.METHOD_MIXED, flags=0)
     B.run()
+    if tuned:
+        B.tune(replays=3)  # the timed batch's settings (same shape: the tuner's cached choice)
     B.time_kernels(True)
     for _ in range(KSTEPS):
         B.launch()
@@ -421,6 +435,7 @@ def config_report(config, a, orc):
     steps = a.steps if config == 4 else max(a.steps, 100)
     Bm = sh.Batch(scenes, ev, method=sh.METHOD_MIXED, flags=sh.BATCH_GRAPH)
     Bm.run()
+    tune = tune_batch(Bm, a)
     t_m, lat = time_batch(Bm, steps, a.warmup)
     obj_m, chi_m = Bm.run()
     fp = Bm.footprint()
@@ -436,7 +451,7 @@ def config_report(config, a, orc):
     nf = sum(sc.nf for sc in scenes) // len(scenes)
     ms = 1e3 * t_m / steps
     pb = path_bytes(fp, ne, nv, nf)
-    out = {"config": config, "models": len(scenes), "evals_per_replay": ne, "vertices": nv, "facets": nf,
+    out = {"config": config, "launch": tune, "models": len(scenes), "evals_per_replay": ne, "vertices": nv, "facets": nf,
            "frames_per_model": len(scenes[0].all_frames()),
            "mixed_evals_per_s": round(ne * steps / t_m, 2), "replay_ms_p50": round(float(np.percentile(lat, 50)), 4),
            "replay_ms_p99": round(float(np.percentile(lat, 99)), 4), "replays": steps,
@@ -509,6 +524,7 @@ def main():
     log("mixed batch: %d evaluations per replay" % a.evals)
     Bm = sh.Batch([sc], evals, method=sh.METHOD_MIXED, flags=sh.BATCH_GRAPH)
     obj_m, chi_m = Bm.run()
+    tune = tune_batch(Bm, a)
     t_m, lat_m = time_batch(Bm, a.steps, a.warmup)
     obj_m, chi_m = Bm.run()
     cnt = Bm.counters()
@@ -524,7 +540,7 @@ def main():
            "data": "synthetic (seeded scene builder; observations = FP64 drop-in fits + noise)",
            "config": {"workload": "large model: 163842 V / 327680 F icosphere, POS 1023^2 at 2 m/px, "
                                   "%d delay-Doppler frames 128-512 x 64-256" % a.frames,
-                      "evals_per_replay": a.evals, "method": "mixed", "graph": True},
+                      "evals_per_replay": a.evals, "method": "mixed", "graph": True, "launch": tune},
            "replay_ms_p50": round(float(np.percentile(lat_m, 50)), 4),
            "replay_ms_p99": round(float(np.percentile(lat_m, 99)), 4),
            "replayed_pixels": cnt[0], "claimed_keys": fpm["claimed"]}
@@ -549,7 +565,7 @@ def main():
     out["speedup_vs_fp64_batch"] = round(evals_per_s / out["fp64_batch_evals_per_s"], 2)
 
     log("per-kernel timing (eager replays)")
-    dom, rows, fp, claimed = kernel_roofline(sc, a.evals)
+    dom, rows, fp, claimed = kernel_roofline(sc, a.evals, not a.no_tune)
     r = rows[dom]
     ach = r["GBps"]
     wl = "config4/frames=%d/evals=%d" % (a.frames, a.evals)

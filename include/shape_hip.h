@@ -227,6 +227,26 @@ int shape_batch_line_searches(shape_batch *b, int nsearch, const double *x0, con
 int shape_batch_fit_pass(shape_batch *b, const double *step, const double *tol, const double *abstol, int max_rounds,
                          double *x, double *fmin, int *rounds);
 
+/* ---------------- Part 4: launch-setting tuner (no counterpart in the reference) --
+   Settings that change how the mixed method spreads its work over the
+   device, never its results: [0] tile kernel form (0: one workgroup per
+   32x32 tile item; > 0, a multiple of 8: persistent, that many workgroups
+   taking tile items from per-XCD work counters), [1] FP32 replay workgroups
+   (at most), [2] candidate-collection workgroups.  shape_batch_tune times
+   `replays` replays (median, after one untimed) under each candidate, one
+   setting at a time -- tile form (per-tile grid; persistent with 2, 4 or 6
+   workgroups per compute unit), then replay grid, then collection grid --
+   and keeps the fastest; the choice is cached per problem shape for the
+   process (force = 1 tunes again) and, with SHAPE_HIP_TUNE_FILE set, read
+   from and appended to that file.  settings (3 ints) and best_ms may be
+   NULL.  A batch with SHAPE_BATCH_GRAPH is captured again under the chosen
+   settings.  The FP64 method has no such settings (returns its defaults). */
+int shape_batch_tune(shape_batch *b, int replays, int force, int *settings, double *best_ms);
+int shape_batch_get_tune(shape_batch *b, int *settings);
+/* -1: invalid settings (a persistent grid not a multiple of 8, or on a batch
+   without binned views) */
+int shape_batch_set_tune(shape_batch *b, const int *settings);
+
 /* measurement: the batch's hipStream_t; per-kernel HIP-event timing of eager
    replays (batches created without SHAPE_BATCH_GRAPH); and the last replay's
    footprint: model-rectangle pixels, POS pixels, occupied image cells, image

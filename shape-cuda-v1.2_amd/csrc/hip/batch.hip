@@ -7,10 +7,15 @@
 // synthesis, chi-square and the objective, all on the device.  With
 // use_graph the replay is a captured hipGraph; trial values are refreshed
 // from a host staging copy as the graph's first node.
+#include <stdio.h>
+#include <stdlib.h>
 #include <string.h>
 
 #include <algorithm>
 #include <cmath>
+#include <map>
+#include <mutex>
+#include <string>
 #include <vector>
 
 #include "common.h"
@@ -369,5 +374,178 @@ extern "C" int shape_batch_footprint(shape_batch *B, long long *out7) {
   for (const GeomSlot &G : B->E.h_slots) { nv += G.nv; nf += G.nf; }
   out6[0] = bbox; out6[1] = grid; out6[2] = win; out6[3] = cells; out6[4] = nv; out6[5] = nf;
   out6[6] = claimed;
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
+// Launch-setting tuner.  The mixed method's launch settings (Tune: the tile
+// kernel's form -- one workgroup per tile item, or persistent with 2, 4 or 6
+// workgroups per compute unit --, the FP32 replay's and the candidate
+// collection's grids) change how the work is spread over the device, never
+// the results.  shape_batch_tune times replays of the batch under each
+// candidate, one setting at a time (tile form, then replay grid, then
+// collection grid), keeps the fastest, and caches the choice under the
+// batch's problem shape: method, views, geometry slots, facets, tile grid,
+// POS pixels, frames, lightcurve and global-raster use, and the device's
+// compute units.  With SHAPE_HIP_TUNE_FILE set, choices are read from and
+// appended to that file, one line each: "key tile_grid replay_grid
+// collect_grid ms".
+// ---------------------------------------------------------------------------
+namespace {
+
+std::mutex g_tune_mu;
+std::map<std::string, std::pair<Tune, double>> g_tune_cache;
+bool g_tune_file_read = false;
+
+int device_cus() {
+  int dev = 0, cus = 0;
+  if (hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus < 1)
+    return 0;
+  return cus;
+}
+
+std::string shape_key(const Engine &E) {
+  char buf[256];
+  snprintf(buf, sizeof(buf), "m%d_v%d_s%d_f%d_t%d_p%lld_fr%d_lc%d_g%d_cu%d", E.opt.method, E.nviews, E.nslots, E.max_nf,
+           E.max_bin_ntx, E.total_pix, E.nframes, E.any_lc ? 1 : 0, E.any_global ? 1 : 0, device_cus());
+  return buf;
+}
+
+void read_tune_file() {
+  if (g_tune_file_read) return;
+  g_tune_file_read = true;
+  const char *path = getenv("SHAPE_HIP_TUNE_FILE");
+  if (!path) return;
+  FILE *f = fopen(path, "r");
+  if (!f) return;
+  char key[200];
+  Tune t;
+  double ms;
+  while (fscanf(f, "%199s %d %d %d %lf", key, &t.tile_grid, &t.replay_grid, &t.collect_grid, &ms) == 5)
+    g_tune_cache[key] = {t, ms};
+  fclose(f);
+}
+
+void append_tune_file(const std::string &key, const Tune &t, double ms) {
+  const char *path = getenv("SHAPE_HIP_TUNE_FILE");
+  if (!path) return;
+  FILE *f = fopen(path, "a");
+  if (!f) return;
+  fprintf(f, "%s %d %d %d %.6f\n", key.c_str(), t.tile_grid, t.replay_grid, t.collect_grid, ms);
+  fclose(f);
+}
+
+bool tune_valid(const Tune &t) {
+  return t.tile_grid >= 0 && t.tile_grid % NXCD == 0 && t.replay_grid >= 1 && t.collect_grid >= 1;
+}
+
+// settings into effect: a captured graph holds launch parameters, so it is
+// captured again
+int apply_tune(shape_batch *B, const Tune &t) {
+  B->E.tune = t;
+  return B->use_graph ? B->E.capture_graph() : 0;
+}
+
+// median replay time (ms) over `replays` replays after one untimed one
+int time_replays(shape_batch *B, int replays, double *ms) {
+  int rc = shape_batch_run(B, nullptr, nullptr);
+  if (rc) return rc;
+  std::vector<double> v;
+  for (int r = 0; r < replays; r++) {
+    if ((rc = shape_batch_run(B, nullptr, nullptr))) return rc;
+    v.push_back(B->last_ms);
+  }
+  std::sort(v.begin(), v.end());
+  *ms = v[v.size() / 2];
+  return 0;
+}
+
+}  // namespace
+
+static void tune_out(const Tune &t, int *settings) {
+  if (!settings) return;
+  settings[0] = t.tile_grid;
+  settings[1] = t.replay_grid;
+  settings[2] = t.collect_grid;
+}
+
+extern "C" int shape_batch_get_tune(shape_batch *B, int *settings) {
+  tune_out(B->E.tune, settings);
+  return 0;
+}
+
+extern "C" int shape_batch_set_tune(shape_batch *B, const int *settings) {
+  Tune t;
+  t.tile_grid = settings[0];
+  t.replay_grid = settings[1];
+  t.collect_grid = settings[2];
+  if (!tune_valid(t) || (t.tile_grid > 0 && !B->E.d_tile_work)) return -1;
+  return apply_tune(B, t);
+}
+
+extern "C" int shape_batch_tune(shape_batch *B, int replays, int force, int *settings, double *best_ms) {
+  if (replays < 1) return -1;
+  Engine &E = B->E;
+  const std::string key = shape_key(E);
+  {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    read_tune_file();
+    auto it = g_tune_cache.find(key);
+    if (!force && it != g_tune_cache.end()) {
+      int rc = apply_tune(B, it->second.first);
+      tune_out(E.tune, settings);
+      if (best_ms) *best_ms = it->second.second;
+      return rc;
+    }
+  }
+  Tune best = E.tune;
+  double best_t = 0.0;
+  int rc = apply_tune(B, best);
+  if (!rc) rc = time_replays(B, replays, &best_t);
+  if (rc) return rc;
+  auto consider = [&](Tune t) -> int {
+    if (!tune_valid(t)) return 0;
+    double ms = 0.0;
+    int r = apply_tune(B, t);
+    if (!r) r = time_replays(B, replays, &ms);
+    if (r) return r;
+    if (ms < best_t) { best_t = ms; best = t; }
+    return 0;
+  };
+  if (E.opt.method == METHOD_MIXED) {
+    const int cus = std::max(NXCD, device_cus());
+    if (E.any_binned && E.d_tile_work) {
+      for (int per_cu : {2, 4, TILE_WG_PER_CU}) {
+        Tune t = best;
+        t.tile_grid = std::max(NXCD, cus * per_cu / NXCD * NXCD);
+        if ((rc = consider(t))) break;
+      }
+      Tune t = best;
+      t.tile_grid = 0;
+      if (!rc) rc = consider(t);
+    }
+    for (int g : {512, 1024, 2048, 4096, 8192}) {
+      if (rc) break;
+      Tune t = best;
+      t.replay_grid = g;
+      rc = consider(t);
+    }
+    for (int g : {256, 1024, 2048, 4096}) {
+      if (rc) break;
+      Tune t = best;
+      t.collect_grid = g;
+      rc = consider(t);
+    }
+  }
+  if (!rc) rc = apply_tune(B, best);
+  if (rc) return rc;
+  {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    g_tune_cache[key] = {best, best_t};
+    append_tune_file(key, best, best_t);
+  }
+  tune_out(best, settings);
+  if (best_ms) *best_ms = best_t;
   return 0;
 }

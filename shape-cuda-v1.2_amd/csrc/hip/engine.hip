@@ -734,6 +734,9 @@ int Engine::build(struct par_t *par, struct mod_t **mods, struct dat_t **dats, i
     for (auto &V : h_views)
       if (V.binned) out_cap += (h_slots[V.slot].nf + BF - 1) / BF;
     if ((rc = alloc((void **)&d_out_list, sizeof(int2) * (size_t)std::max(1LL, out_cap)))) return rc;
+    // the persistent tile kernel's work counters start at zero and every launch leaves them there
+    if ((rc = alloc((void **)&d_tile_work, sizeof(int) * 2 * NXCD))) return rc;
+    SHP_CHECK(hipMemsetAsync(d_tile_work, 0, sizeof(int) * 2 * NXCD, stream));
   }
   if (opt.method == METHOD_MIXED) {
     if ((rc = alloc((void **)&pd.stale_tiles, (size_t)std::max(1LL, total_tiles)))) return rc;

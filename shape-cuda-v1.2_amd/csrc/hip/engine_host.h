@@ -55,6 +55,15 @@ struct EvalSpec {
   std::vector<double> values;
 };
 
+// Launch settings of the mixed method that do not change results, only how
+// the work is spread over the device; shape_batch_tune picks them by timing
+// replays.
+struct Tune {
+  int tile_grid = 0;       // tile kernel: 0 = one workgroup per tile item, > 0 = persistent, this many workgroups
+  int replay_grid = 2048;  // FP32 replay: workgroups at most (one thread per record, grid-stride)
+  int collect_grid = 2048; // global candidate collection: workgroups (grid-stride)
+};
+
 struct RunOptions {
   int method = METHOD_FP64;
   int host_spin = 0;        // 1: take ae/spin/deloff/dopoff from the host structs
@@ -140,6 +149,9 @@ class Engine {
   int *d_bin_cnt = nullptr;                 // per 32x32 tile of every binned view
   unsigned int *d_bin_list = nullptr;       // BIN_CAP entries per tile: block id << 4 | quarter mask
   int2 *d_out_list = nullptr;               // (view, block) of blocks reaching outside the POS window
+  int *d_tile_work = nullptr;               // the persistent tile kernel's work counters (2 NXCD ints)
+  // Launch settings a tuner may change between replays (Tune)
+  Tune tune;
   PosDev pd{};
   std::vector<void *> allocs;
   size_t dev_bytes = 0;     // device memory held by allocs

@@ -40,8 +40,11 @@ __global__ void k_bin32(const ViewRec *__restrict__ views, const GeomSlot *slots
                         unsigned int *bin_list, int2 *out_list, int *out_count);
 __global__ void k_tile32(const ViewRec *__restrict__ vrd, ViewRec *views, const GeomSlot *__restrict__ slots,
                          PosDev pd, int *bin_cnt, const unsigned int *__restrict__ bin_list, double *fit,
-                         const FrameRec *__restrict__ frames, double *overflow, const double *__restrict__ params,
-                         int *counters);
+                         const FrameRec *__restrict__ frames, double *overflow, const double *__restrict__ params);
+__global__ void k_tile32p(const ViewRec *__restrict__ vrd, ViewRec *views, const GeomSlot *__restrict__ slots,
+                          PosDev pd, int *bin_cnt, const unsigned int *__restrict__ bin_list, double *fit,
+                          const FrameRec *__restrict__ frames, double *overflow, const double *__restrict__ params,
+                          int *work, int ng, int nviews);
 
 // ---------------------------------------------------------------------------
 // raster (posvis.c:142-295), one facet per lane:
@@ -299,19 +302,26 @@ int enqueue_mixed(Engine &E) {
                E.d_bin_cnt, E.d_bin_list, E.d_out_list, E.pd.nstale + 4);
     SHP_LAUNCH(E, "k_outside32", k_outside32, dim3(512), bt, 0, st, E.d_views, E.d_slots, E.pd, E.d_out_list,
                E.pd.nstale + 4);
-    SHP_LAUNCH(E, "k_tile32", k_tile32, dim3(pad_xcd((unsigned)(E.max_bin_ntx * E.max_bin_ntx)), E.nviews), bt, 0, st,
-               E.d_views, E.d_views, E.d_slots, E.pd, E.d_bin_cnt, E.d_bin_list, E.d_fit, E.d_frames, E.d_overflow,
-               E.d_params, E.pd.nstale);
+    // one workgroup per tile item, or (tile_grid > 0) the persistent form
+    const int ng = (int)pad_xcd((unsigned)(E.max_bin_ntx * E.max_bin_ntx));
+    if (E.tune.tile_grid > 0)
+      SHP_LAUNCH(E, "k_tile32", k_tile32p, dim3(E.tune.tile_grid), bt, 0, st, E.d_views, E.d_views, E.d_slots, E.pd,
+                 E.d_bin_cnt, E.d_bin_list, E.d_fit, E.d_frames, E.d_overflow, E.d_params, E.d_tile_work, ng,
+                 E.nviews);
+    else
+      SHP_LAUNCH(E, "k_tile32", k_tile32, dim3(ng, E.nviews), bt, 0, st, E.d_views, E.d_views, E.d_slots, E.pd,
+                 E.d_bin_cnt, E.d_bin_list, E.d_fit, E.d_frames, E.d_overflow, E.d_params);
   }
   if (E.any_global) {
     SHP_LAUNCH(E, "k_raster32", k_raster32, gf, bt, 0, st, E.d_views, E.d_slots, E.pd, E.total_pix, 0);
     SHP_LAUNCH(E, "k_shade_radar32", k_shade_radar32, dim3(tiles * tiles, E.nviews), bt, 0, st, E.d_views, E.d_slots,
                E.pd, E.d_fit, E.d_frames, E.d_overflow, E.d_params, E.pd.nstale);
   }
-  SHP_LAUNCH(E, "k_collect32", k_collect32, dim3(2048), bt, 0, st, E.d_views, E.d_slots, E.pd, E.nviews,
-             (int)cdivm(E.max_nf, 256));
-  SHP_LAUNCH(E, "k_replay32", k_replay32, dim3(std::min(cdivm(E.pd.stale_cap, 256), 2048u)), bt, 0, st, E.d_views, E.d_slots, E.pd,
-                     E.d_fit, E.d_frames, E.d_overflow, E.d_params);
+  SHP_LAUNCH(E, "k_collect32", k_collect32, dim3(std::max(1, E.tune.collect_grid)), bt, 0, st, E.d_views, E.d_slots,
+             E.pd, E.nviews, (int)cdivm(E.max_nf, 256));
+  SHP_LAUNCH(E, "k_replay32", k_replay32,
+             dim3(std::max(1u, std::min(cdivm(E.pd.stale_cap, 256), (unsigned)E.tune.replay_grid))), bt, 0, st,
+             E.d_views, E.d_slots, E.pd, E.d_fit, E.d_frames, E.d_overflow, E.d_params);
   if (E.any_lc) {
     // Lightcurves: the FP64 kernels draw the observer and source views of
     // every lightcurve point (radar views carry fp64_skip), mask shadows and

@@ -236,7 +236,7 @@ __device__ inline void lds_append(bool need, unsigned int e, void *list, int *cn
 // Record pixel (i, j) (local index lp) for the FP64 replay: in the tile's LDS
 // list while its events are complete and the list has room, else in the
 // global replay records (k_collect32 collects its candidates).
-__device__ inline void tile_stale(const ViewRec &V, const PosDev &pd, bool ev_ok, int lp, int i, int j,
+__device__ inline void tile_stale(const ViewRec &V, int vi, const PosDev &pd, bool ev_ok, int lp, int i, int j,
                                   unsigned char *flag, short *spix, int *nts) {
   if (ev_ok) {
     int s = atomicAdd(nts, 1);
@@ -252,7 +252,7 @@ __device__ inline void tile_stale(const ViewRec &V, const PosDev &pd, bool ev_ok
     atomicOr(pd.error, 1);
     return;
   }
-  stale_init(V, pd, si, blockIdx.y, (i + V.n) * (2 * V.n + 1) + (j + V.n), i, j);
+  stale_init(V, pd, si, vi, (i + V.n) * (2 * V.n + 1) + (j + V.n), i, j);
 }
 
 // Pixel (a, c) of the tile taken by this thread in its m-th turn: each wave
@@ -266,28 +266,29 @@ __device__ inline void tile_pixel(int m, int &a, int &c) {
   c = (l & 7) * 4 + (g & 3);
 }
 
-// Six workgroups per CU (LDS 26 KiB each): the register cap of six waves per
-// SIMD costs a few spills but hides more of the gathers' latency (measured).
-// vrd and views are the same view records: every read goes through the
-// read-only, non-aliased vrd (the fields it reads stay fixed during the
-// launch), so the compiler keeps them in scalar loads instead of reloading
-// them per pixel after the kernel's global stores; the limits that workgroups
-// update are read and written through views only.
-__global__ __launch_bounds__(256, 6) void k_tile32(const ViewRec *__restrict__ vrd, ViewRec *views,
-                                                const GeomSlot *__restrict__ slots, PosDev pd, int *bin_cnt,
-                                                const unsigned int *__restrict__ bin_list, double *fit,
-                                                const FrameRec *__restrict__ frames, double *overflow,
-                                                const double *__restrict__ params, int *counters) {
-  const ViewRec &V = vrd[blockIdx.y];
-  ViewRec &Vw = views[blockIdx.y];
+// One tile item of view vi: item b of the view's ng items (ng a multiple of
+// NXCD; the workgroup runs on XCD b % NXCD).  vrd and views are the same
+// view records: every read goes through the read-only, non-aliased vrd (the
+// fields it reads stay fixed during the launch), so the compiler keeps them
+// in scalar loads instead of reloading them per pixel after the kernel's
+// global stores; the limits that workgroups update are read and written
+// through views only.  Every return is taken by the whole workgroup, except
+// the last, after which no thread reaches a barrier of this item.
+__device__ __forceinline__ void tile32_item(int vi, int b, int ng, const ViewRec *__restrict__ vrd, ViewRec *views,
+                                            const GeomSlot *__restrict__ slots, const PosDev &pd, int *bin_cnt,
+                                            const unsigned int *__restrict__ bin_list, double *fit,
+                                            const FrameRec *__restrict__ frames, double *overflow,
+                                            const double *__restrict__ params) {
+  const ViewRec &V = vrd[vi];
+  ViewRec &Vw = views[vi];
   if (!V.binned) return;
   const int n = V.n, ntx = V.bin_ntx;
   // each XCD takes a contiguous eighth of the view's tiles (neighbouring
   // tiles share blocks, whose vertices then stay in one L2); which eighth
   // rotates with the view, so that the edge bands, mostly empty sky, are
   // spread over all XCDs instead of idling two of them
-  const int per = gridDim.x / NXCD;
-  const int bx = (int)((blockIdx.x % NXCD + blockIdx.y) % NXCD) * per + (int)(blockIdx.x / NXCD);
+  const int per = ng / NXCD;
+  const int bx = (int)((b % NXCD + vi) % NXCD) * per + (int)(b / NXCD);
   if (bx >= ntx * ntx) return;
   const long long bin = V.bin_off + bx;
   const int cnt = bin_cnt[bin];
@@ -469,7 +470,7 @@ __global__ __launch_bounds__(256, 6) void k_tile32(const ViewRec *__restrict__ v
       if (k) ncl++;
       if (flag[lp] & 1) {
         k = 0ULL;  // not shaded: replayed in FP64
-        tile_stale(V, pd, ev_ok, lp, i, j, flag, spix, &nts);
+        tile_stale(V, vi, pd, ev_ok, lp, i, j, flag, spix, &nts);
       }
       if (k) {
 #pragma clang fp contract(off)
@@ -527,7 +528,7 @@ __global__ __launch_bounds__(256, 6) void k_tile32(const ViewRec *__restrict__ v
   // ---- B: shade, footprint, accumulate ----
   const double *pl = params + V.pb + V.pb_law;
   double *img = fit + frames[V.frame_id].off;
-  double *ov = overflow + (long long)blockIdx.y * 2000;
+  double *ov = overflow + (long long)vi * 2000;
   int anyo = 0;
   for (int q = lane; q < nsh; q += 64) {
     const unsigned long long e = shade[q];
@@ -552,7 +553,7 @@ __global__ __launch_bounds__(256, 6) void k_tile32(const ViewRec *__restrict__ v
     if (!(nz > 0.0f) || shade_ill(V, P, xa.w, spread)) {
       // cos(scattering) of an owner facing away is inherited from an earlier
       // claimant; an ill-conditioned FP32 cosine needs FP64: FP64 replay
-      tile_stale(V, pd, ev_ok, a * BT + c, i, j, flag, spix, &nts);
+      tile_stale(V, vi, pd, ev_ok, a * BT + c, i, j, flag, spix, &nts);
       continue;
     }
     RadarPix R;
@@ -619,7 +620,7 @@ __global__ __launch_bounds__(256, 6) void k_tile32(const ViewRec *__restrict__ v
           for (int q = lane; q < nc; q += 64) S.cand[q] = cand[s][q];
         if (lane == 0) {
           const int lp = spix[s], i = i0 + lp / BT, j = j0 + lp % BT;
-          S.view = blockIdx.y;
+          S.view = vi;
           S.pix = (i + n) * w + (j + n);
           S.oldf = -1; S.oldz = -1.0e20; S.oldcose = 0.0; S.oldcosi = 0.0;
           S.src = 0;
@@ -663,6 +664,56 @@ __global__ __launch_bounds__(256, 6) void k_tile32(const ViewRec *__restrict__ v
   }
   if (V.P0 + lim_p0 < Vw.idoplim[0]) atomicMin(&Vw.idoplim[0], V.P0 + lim_p0);
   if (V.P0 + lim_p1 > Vw.idoplim[1]) atomicMax(&Vw.idoplim[1], V.P0 + lim_p1);
+}
+
+// One workgroup per tile item: grid (ng, nviews).  Six workgroups per CU (LDS
+// 26 KiB each): the register cap of six waves per SIMD hides more of the
+// gathers' latency than a larger register budget would (measured).
+__global__ __launch_bounds__(256, TILE_WG_PER_CU) void k_tile32(const ViewRec *__restrict__ vrd, ViewRec *views,
+                                                             const GeomSlot *__restrict__ slots, PosDev pd,
+                                                             int *bin_cnt, const unsigned int *__restrict__ bin_list,
+                                                             double *fit, const FrameRec *__restrict__ frames,
+                                                             double *overflow, const double *__restrict__ params) {
+  tile32_item(blockIdx.y, blockIdx.x, gridDim.x, vrd, views, slots, pd, bin_cnt, bin_list, fit, frames, overflow,
+              params);
+}
+
+// Persistent form: gridDim.x workgroups (a multiple of NXCD; the hardware
+// deals workgroup g to XCD g % NXCD) take the nviews x ng tile items from one
+// work counter per XCD, so a workgroup on XCD x takes only items b with
+// b % NXCD == x and the tile-to-XCD mapping of the per-tile grid holds.  Each
+// workgroup fetches its next item while it works on the current one.
+// work[0..NXCD): the counters; work[NXCD..2 NXCD): workgroups done per XCD.
+// The last workgroup of an XCD to finish resets both, so they start every
+// launch at zero (they are zeroed once when allocated).
+__global__ __launch_bounds__(256, TILE_WG_PER_CU) void k_tile32p(const ViewRec *__restrict__ vrd, ViewRec *views,
+                                                              const GeomSlot *__restrict__ slots, PosDev pd,
+                                                              int *bin_cnt, const unsigned int *__restrict__ bin_list,
+                                                              double *fit, const FrameRec *__restrict__ frames,
+                                                              double *overflow, const double *__restrict__ params,
+                                                              int *work, int ng, int nviews) {
+  const int x = blockIdx.x % NXCD;
+  const long long nitems = (long long)(ng / NXCD) * nviews;  // items of this XCD
+  __shared__ int s_next;
+  if (threadIdx.x == 0) s_next = atomicAdd(&work[x], 1);
+  __syncthreads();
+  int k = s_next;
+  while (k < nitems) {
+    int nxt = 0;
+    if (threadIdx.x == 0) nxt = atomicAdd(&work[x], 1);
+    const long long it = (long long)k * NXCD + x;
+    tile32_item((int)(it / ng), (int)(it % ng), ng, vrd, views, slots, pd, bin_cnt, bin_list, fit, frames, overflow,
+                params);
+    __syncthreads();  // every thread has left the item and read s_next
+    if (threadIdx.x == 0) s_next = nxt;
+    __syncthreads();
+    k = s_next;
+  }
+  if (threadIdx.x == 0 && atomicAdd(&work[NXCD + x], 1) == (int)(gridDim.x / NXCD) - 1) {
+    // every workgroup of this XCD has taken its last item index
+    atomicExch(&work[x], 0);
+    atomicExch(&work[NXCD + x], 0);
+  }
 }
 
 }  // namespace shp

@@ -24,6 +24,10 @@ constexpr int TS_CAP = 64;       // replayed pixels per tile whose candidates co
 constexpr int TS_CAND = 32;      // candidates kept per such pixel (more: the global raster collects them)
 constexpr int WIN_T = 1024;      // FP64 LDS delay-Doppler window of a tile (8 KiB)
 constexpr int EV_FACET_BITS = 22;  // event = local pixel << 22 | facet (facets < 2^22)
+constexpr int TILE_WG_PER_CU = 6;  // the tile kernel's occupancy (registers and LDS)
+
+// Workgroups are dealt to the chip's 8 XCDs round-robin by linear block id.
+constexpr int NXCD = 8;
 
 // A pixel whose final owner has a back-facing smoothed normal; its
 // cos(scattering) is whatever an earlier claimant left (posvis.c:243-275).

@@ -17,13 +17,11 @@ constexpr int WIN_MAX = 2048;    // LDS delay-Doppler window (FP64 bins, 16 KiB)
 // baud take more groups.
 constexpr int DW = 2;
 
-// Workgroups are dealt to the chip's 8 XCDs round-robin by linear block id.
-// With the grid's x size nb a multiple of 8 (pad_xcd), remap so that XCD k
+// With the grid's x size nb a multiple of NXCD (pad_xcd), remap so that XCD k
 // runs the k-th contiguous eighth of [0, nb): in the raster, each XCD's
 // private L2 then keeps one eighth of the model's facets and vertices across
 // all views instead of streaming the whole model for every view (measured:
 // half the raster time on a 327,680-facet model).
-constexpr int NXCD = 8;
 __device__ inline int xcd_block(int b, int nb) { return (b % NXCD) * (nb / NXCD) + b / NXCD; }
 inline unsigned pad_xcd(unsigned n) { return (n + NXCD - 1) / NXCD * NXCD; }
 

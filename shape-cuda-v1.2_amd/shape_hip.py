@@ -190,6 +190,9 @@ def load_hip():
             "shape_batch_fit_pass": (C.c_int, [vp, C.POINTER(C.c_double), C.POINTER(C.c_double),
                                                C.POINTER(C.c_double), C.c_int, C.POINTER(C.c_double),
                                                C.POINTER(C.c_double), C.POINTER(C.c_int)]),
+            "shape_batch_tune": (C.c_int, [vp, C.c_int, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_double)]),
+            "shape_batch_get_tune": (C.c_int, [vp, C.POINTER(C.c_int)]),
+            "shape_batch_set_tune": (C.c_int, [vp, C.POINTER(C.c_int)]),
             "shape_batch_last_ms": (C.c_float, [vp]),
             "shape_batch_nframes": (C.c_int, [vp, C.c_int]),
             "shape_batch_frame_results": (C.c_int, [vp, C.c_int, C.POINTER(C.c_double),
@@ -608,6 +611,26 @@ class Batch:
         if rc < 0:
             raise RuntimeError("shape_batch_fit_pass failed (%d)" % rc)
         return rc, xx, fm.value, r.value
+
+    def tune(self, replays=3, force=False):
+        """Time the launch-setting candidates (shape_batch_tune) and keep the
+        fastest; returns ([tile_grid, replay_grid, collect_grid], best ms)."""
+        out, ms = (C.c_int * 3)(), C.c_double()
+        rc = self.lib.shape_batch_tune(self.h, replays, 1 if force else 0, out, C.byref(ms))
+        if rc:
+            raise RuntimeError("shape_batch_tune failed (%d)" % rc)
+        return list(out), ms.value
+
+    def get_tune(self):
+        out = (C.c_int * 3)()
+        self.lib.shape_batch_get_tune(self.h, out)
+        return list(out)
+
+    def set_tune(self, settings):
+        arr = (C.c_int * 3)(*settings)
+        rc = self.lib.shape_batch_set_tune(self.h, arr)
+        if rc:
+            raise RuntimeError("shape_batch_set_tune failed (%d)" % rc)
 
     def launch(self):
         rc = self.lib.shape_batch_launch(self.h)
