@@ -88,6 +88,7 @@ struct ViewRec {
   long long tile_off;    // offset of this view's stale-tile bitmap (mixed method)
   long long bin_off;     // binned mixed raster: first 32x32 tile of this view in the bin arrays
   int bin_ntx, binned;   // tiles per side; 1 when the view takes the binned tile kernel
+  int lc_rec_off, lc_rec_cap;  // mixed method, lightcurve views: this view's range of the replay records
   int pb;                // parameter block of the evaluation
   int pb_set, pb_law;    // offsets (within the block) of the set and scattering-law params
   int pb_dc;             // offset of the set block whose delay polynomial applies (= states)

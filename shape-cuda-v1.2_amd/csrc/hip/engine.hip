@@ -459,6 +459,7 @@ int Engine::build(struct par_t *par, struct mod_t **mods, struct dat_t **dats, i
   h_frames.clear();
   h_evals.clear();
   long long fit_cursor = 0, pos_cursor = 0, y_cursor = 0, y2_cursor = 0, tile_cursor = 0, bin_cursor = 0;
+  long long lc_rec_cursor = 0;
   std::vector<long long> model_obs_base(nmodels, 0), model_x_base(nmodels, 0), model_t_base(nmodels, 0);
   long long ob = 0, xb = 0, tb = 0;
   for (int m = 0; m < nmodels; m++) {
@@ -531,6 +532,15 @@ int Engine::build(struct par_t *par, struct mod_t **mods, struct dat_t **dats, i
       V.tile_off = tile_cursor;
       tile_cursor += (long long)((2 * V.n + STALE_TILE) / STALE_TILE) * ((2 * V.n + STALE_TILE) / STALE_TILE);
       V.binned = 0; V.bin_ntx = 0; V.bin_off = 0;
+      V.lc_rec_off = 0; V.lc_rec_cap = 0;
+      if (opt.method == METHOD_MIXED && V.type == VT_LGHTCRV) {
+        // a range of the replay records per lightcurve view: its records are
+        // reserved on a counter of its own (one shared counter serializes)
+        V.lc_rec_off = (int)lc_rec_cursor;
+        V.lc_rec_cap = (2 * V.n + 1) * (2 * V.n + 1) / 16 + 64;
+        lc_rec_cursor += V.lc_rec_cap;
+        max_lc_rec = std::max(max_lc_rec, V.lc_rec_cap);
+      }
       if (opt.method == METHOD_MIXED && V.type != VT_LGHTCRV) {
         // binned tile kernel unless the POS is wider than its tile grid, the
         // facet index does not fit an event, the global z-buffer is asked for,
@@ -576,6 +586,7 @@ int Engine::build(struct par_t *par, struct mod_t **mods, struct dat_t **dats, i
   total_pix = pos_cursor;
   total_tiles = tile_cursor;
   total_bins = bin_cursor;
+  lc_rec_total = lc_rec_cursor;
   fit_len = fit_cursor;
   // ---- device arenas ----
   max_nf = 0;
@@ -744,7 +755,12 @@ int Engine::build(struct par_t *par, struct mod_t **mods, struct dat_t **dats, i
   }
   // replay records: 1 per 32 POS pixels (a limb band and the facet-edge band
   // of every view take well under that), at least 4096, at most 64M
-  pd.stale_cap = (int)std::min<long long>(std::max<long long>((long long)P / 32, 4096), 64LL << 20);
+  pd.stale_cap = (int)std::min<long long>(std::max<long long>({(long long)P / 32, 4096LL, lc_rec_total}), 64LL << 20);
+  if (lc_rec_total > pd.stale_cap) return -31;  // lightcurve record ranges beyond the record list's bound
+  if (any_lc && opt.method == METHOD_MIXED) {
+    if ((rc = alloc((void **)&pd.lc_count, sizeof(int) * (size_t)std::max(1, nviews)))) return rc;
+    SHP_CHECK(hipMemsetAsync(pd.lc_count, 0, sizeof(int) * (size_t)std::max(1, nviews), stream));
+  }
   if ((rc = alloc((void **)&pd.stale, sizeof(StaleRec) * (size_t)pd.stale_cap))) return rc;
   // counters: [0] replay records, [1] error flags, [2] records to collect,
   // [3] unused (0), [4] blocks reaching outside their POS window

@@ -126,6 +126,8 @@ class Engine {
   bool any_binned = false, any_global = false;
   int max_bin_ntx = 0, max_nblk = 0;
   long long total_bins = 0;
+  long long lc_rec_total = 0;  // replay records in the lightcurve views' ranges
+  int max_lc_rec = 0;          // the largest such range
   std::vector<ViewRec> h_views;
   std::vector<FrameRec> h_frames;
   std::vector<EvalRec> h_evals;

@@ -71,6 +71,29 @@ __device__ inline void project_facet(const ViewRec &V, const GeomSlot &G, int f,
   P.den = 1 / ((P.v1[0] - P.v0[0]) * (P.v2[1] - P.v1[1]) - (P.v2[0] - P.v1[0]) * (P.v1[1] - P.v0[1]));
 }
 
+// project_facet without the bounding box (its four divisions): for the
+// per-pixel resolve and replay, which test given pixels only.  The facing,
+// vertices and den are the same expressions, so pixel_hit takes the same
+// bits.
+__device__ inline void project_facet_px(const ViewRec &V, const GeomSlot &G, int f, int src, FacetProj &P) {
+#pragma clang fp contract(off)
+  const double *oa = src ? V.sa : V.oa;
+  double nb[3] = {G.fn[3 * f], G.fn[3 * f + 1], G.fn[3 * f + 2]};
+  cotrans3(P.nrm, oa, nb);
+  P.front = P.nrm[2] > 0.0;
+  if (!P.front) return;
+  const int *fv = G.fv + 3 * f;
+  cotrans3(P.v0, oa, G.x + 3 * fv[0]);
+  cotrans3(P.v1, oa, G.x + 3 * fv[1]);
+  cotrans3(P.v2, oa, G.x + 3 * fv[2]);
+  for (int k = 0; k < 3; k++) {
+    P.v0[k] += V.orb[k];
+    P.v1[k] += V.orb[k];
+    P.v2[k] += V.orb[k];
+  }
+  P.den = 1 / ((P.v1[0] - P.v0[0]) * (P.v2[1] - P.v1[1]) - (P.v2[0] - P.v1[0]) * (P.v1[1] - P.v0[1]));
+}
+
 // barycentric test and depth at pixel (i, j); same expressions as posvis.c:217-228
 __device__ inline bool pixel_hit(const FacetProj &P, double kmpp, int i, int j, double &s, double &t, double &z) {
 #pragma clang fp contract(off)

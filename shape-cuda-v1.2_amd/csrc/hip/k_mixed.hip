@@ -36,6 +36,8 @@ __global__ void k_shade_radar32(ViewRec *__restrict__ views, const GeomSlot *slo
 __global__ void k_replay32(ViewRec *__restrict__ views, const GeomSlot *slots, PosDev pd, double *fit, const FrameRec *frames,
                            double *overflow, const double *params);
 __global__ void k_block_bounds(GeomSlot *slots, int nslots);
+__global__ void k_resolve_lc(ViewRec *views, const GeomSlot *slots, PosDev pd, int src);
+__global__ void k_replay_lc(ViewRec *views, const GeomSlot *slots, PosDev pd);
 __global__ void k_bin32(const ViewRec *__restrict__ views, const GeomSlot *slots, int *bin_cnt,
                         unsigned int *bin_list, int2 *out_list, int *out_count);
 __global__ void k_tile32(const ViewRec *__restrict__ vrd, ViewRec *views, const GeomSlot *__restrict__ slots,
@@ -64,6 +66,7 @@ struct RasterLim {
   int xl0, xl1, yl0, yl1, ob;
   float o0, o1, o2, o3;
 };
+constexpr int RASTER_CHUNKS = 4;  // chunks of 256 facets per k_raster32 workgroup
 
 __device__ inline void raster_facet(ViewRec &V, const GeomSlot &G, const PosDev &pd, long long sun_off, int src,
                                     int pass, int f, RasterLim &L) {
@@ -72,7 +75,7 @@ __device__ inline void raster_facet(ViewRec &V, const GeomSlot &G, const PosDev 
   L.o0 = 1e30f; L.o1 = -1e30f; L.o2 = 1e30f; L.o3 = -1e30f;
   Proj32 P;
   // pass 3 also admits nearly edge-on facets: the FP64 replay tests facing itself
-  if (!(f < G.nf && project32(V, G, f, src ? V.sa32 : V.oa32, pass == 3 ? -FRONT_TOL : 0.0f, P))) return;
+  if (!(f < G.nf && project32f(V, G, f, src ? V.sa32 : V.oa32, pass == 3 ? -FRONT_TOL : 0.0f, P))) return;
   float imin_d = floorf(fminf(P.x0, fminf(P.x1, P.x2)) + 0.5f);
   float imax_d = floorf(fmaxf(P.x0, fmaxf(P.x1, P.x2)) + 0.5f);
   float jmin_d = floorf(fminf(P.y0, fminf(P.y1, P.y2)) + 0.5f);
@@ -88,6 +91,11 @@ __device__ inline void raster_facet(ViewRec &V, const GeomSlot &G, const PosDev 
     L.yl0 = min(max(jmin, -n), n); L.yl1 = max(min(jmax, n), -n);
   } else if (pass == 1) {
     unsigned long long *kb = pd.zkey + V.pos_off + (src ? sun_off : 0);
+    // Marginal pixels (near an edge, where FP32 may differ from FP64) are
+    // left unclaimed and replayed in FP64: in the observer view, and in a
+    // lightcurve point's sun view, whose z-buffer the shadow test reads.
+    // (A bistatic radar view's second buffer would claim them instead.)
+    const bool mark = !src || V.type == VT_LGHTCRV;
     // one flat loop over the bounding box: a wave runs max(cells) iterations,
     // not max(rows) * max(columns)
     int nc = (i2 - i1 + 1) * (j2 - j1 + 1);
@@ -100,13 +108,13 @@ __device__ inline void raster_facet(ViewRec &V, const GeomSlot &G, const PosDev 
       float s, t;
       float m = edge32(E, di, dj, s, t);
       long long pix = (long long)(i + n) * w + (j + n);
-      if (m >= amb || (src && m >= -amb)) {
+      if (m >= amb || (!mark && m >= -amb)) {
         hit = true;
         atomicMax(&kb[pix], (unsigned long long)pack_key(depth32(P, s, t), f));
       }
-      bool marginal = !src && m >= -amb && m < amb;
+      bool marginal = mark && m >= -amb && m < amb;
       hit = hit || marginal;
-      mark_marginal(V, pd, marginal, (int)pix, i, j);
+      mark_marginal(V, pd, marginal, (int)pix, i, j, src);
       if (++j > j2) { j = j1; i++; dj = 0.0f; di += 1.0f; } else { dj += 1.0f; }
     }
     // The model rectangle grows by the box of every facet that covers a
@@ -168,17 +176,64 @@ __device__ inline void raster_limits(ViewRec &V, int src, RasterLim L) {
   }
 }
 
-// pass 1 over every facet of every view that is not binned: XCD-local
-// eighths of the model (each XCD's L2 then keeps one eighth of it)
+// The same for a whole workgroup (every thread calls it): one update of the
+// view's limits per workgroup.  Same-address atomics on the view's limits
+// serialize at the memory side; with one update per wave they cost more
+// than the raster of sub-pixel facets itself.
+__device__ inline void raster_limits_wg(ViewRec &V, int src, RasterLim L) {
+  __shared__ RasterLim sl[4];
+  L.xl0 = wmin(L.xl0); L.xl1 = wmax(L.xl1); L.yl0 = wmin(L.yl0); L.yl1 = wmax(L.yl1); L.ob = wmax(L.ob);
+  L.o0 = wminf(L.o0); L.o1 = wmaxf(L.o1); L.o2 = wminf(L.o2); L.o3 = wmaxf(L.o3);
+  const int wv = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) sl[wv] = L;
+  __syncthreads();
+  if (threadIdx.x != 0) return;
+  for (int q = 1; q < (int)(blockDim.x >> 6); q++) {
+    const RasterLim &M = sl[q];
+    L.xl0 = min(L.xl0, M.xl0); L.xl1 = max(L.xl1, M.xl1); L.yl0 = min(L.yl0, M.yl0); L.yl1 = max(L.yl1, M.yl1);
+    L.ob |= M.ob;
+    L.o0 = fminf(L.o0, M.o0); L.o1 = fmaxf(L.o1, M.o1); L.o2 = fminf(L.o2, M.o2); L.o3 = fmaxf(L.o3, M.o3);
+  }
+  const int n = V.n;
+  int *xl = src ? V.xlim2 : V.xlim, *yl = src ? V.ylim2 : V.ylim;
+  unsigned long long *ov = src ? V.ov2 : V.ov;
+  int *obp = src ? &V.outbnd2 : &V.outbnd;
+  if (L.xl0 < n && L.xl0 < xl[0]) atomicMin(&xl[0], L.xl0);
+  if (L.xl1 > -n && L.xl1 > xl[1]) atomicMax(&xl[1], L.xl1);
+  if (L.yl0 < n && L.yl0 < yl[0]) atomicMin(&yl[0], L.yl0);
+  if (L.yl1 > -n && L.yl1 > yl[1]) atomicMax(&yl[1], L.yl1);
+  if (L.ob && !*obp) atomicOr(obp, 1);
+  unsigned long long k;
+  if (L.o0 < 1e30f && (k = ord64((double)L.o0)) < ov[0]) atomicMin(&ov[0], k);
+  if (L.o1 > -1e30f && (k = ord64(-(double)L.o1)) < ov[1]) atomicMin(&ov[1], k);
+  if (L.o2 < 1e30f && (k = ord64((double)L.o2)) < ov[2]) atomicMin(&ov[2], k);
+  if (L.o3 > -1e30f && (k = ord64(-(double)L.o3)) < ov[3]) atomicMin(&ov[3], k);
+}
+
+// pass 1 over every facet of every view that is not binned, RASTER_CHUNKS
+// chunks of 256 facets per workgroup: XCD-local eighths of the model (each
+// XCD's L2 then keeps one eighth of it).  lc: 0 the radar views, 1 the
+// lightcurve points' views (observer, src 0, or sun, src 1).
 __global__ __launch_bounds__(256) void k_raster32(ViewRec *__restrict__ views, const GeomSlot *slots, PosDev pd,
-                                                  long long sun_off, int src) {
+                                                  long long sun_off, int src, int lc) {
   ViewRec &V = views[blockIdx.y];
-  if (V.type != VT_DELAY && V.type != VT_DOPPLER) return;  // lightcurve views take the FP64 kernels
+  if (lc ? V.type != VT_LGHTCRV : (V.type != VT_DELAY && V.type != VT_DOPPLER)) return;
   if ((src && !V.bistatic) || V.binned) return;
-  const int f = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+  const int c0 = xcd_block(blockIdx.x, gridDim.x) * RASTER_CHUNKS;
+  const GeomSlot &G = slots[V.slot];
   RasterLim L;
-  raster_facet(V, slots[V.slot], pd, sun_off, src, 1, f, L);
-  raster_limits(V, src, L);
+  for (int r = 0; r < RASTER_CHUNKS; r++) {
+    RasterLim L1;
+    raster_facet(V, G, pd, sun_off, src, 1, (c0 + r) * (int)blockDim.x + (int)threadIdx.x, L1);
+    if (r == 0) {
+      L = L1;
+    } else {
+      L.xl0 = min(L.xl0, L1.xl0); L.xl1 = max(L.xl1, L1.xl1); L.yl0 = min(L.yl0, L1.yl0);
+      L.yl1 = max(L.yl1, L1.yl1); L.ob |= L1.ob;
+      L.o0 = fminf(L.o0, L1.o0); L.o1 = fmaxf(L.o1, L1.o1); L.o2 = fminf(L.o2, L1.o2); L.o3 = fmaxf(L.o3, L1.o3);
+    }
+  }
+  raster_limits_wg(V, src, L);
 }
 
 // pass 0 over the bin blocks (view, block) that k_bin32 found reaching
@@ -198,13 +253,13 @@ __global__ __launch_bounds__(256) void k_outside32(ViewRec *__restrict__ views, 
 // pass 3, grid-stride over (view, 256 facets): a fixed small grid, so an
 // empty replay list costs one early exit per workgroup
 __global__ __launch_bounds__(256) void k_collect32(ViewRec *__restrict__ views, const GeomSlot *slots, PosDev pd,
-                                                   int nviews, int nchunk) {
+                                                   int nviews, int nchunk, int src, int lc) {
   if (*pd.ncollect == 0) return;
   for (long long w = blockIdx.x; w < (long long)nviews * nchunk; w += gridDim.x) {
     ViewRec &V = views[w / nchunk];
-    if (V.type != VT_DELAY && V.type != VT_DOPPLER) continue;
+    if (lc ? V.type != VT_LGHTCRV : (V.type != VT_DELAY && V.type != VT_DOPPLER)) continue;
     RasterLim L;
-    raster_facet(V, slots[V.slot], pd, 0, 0, 3, (int)(w % nchunk) * 256 + threadIdx.x, L);
+    raster_facet(V, slots[V.slot], pd, 0, src, 3, (int)(w % nchunk) * 256 + threadIdx.x, L);
   }
 }
 
@@ -289,7 +344,7 @@ static inline unsigned cdivm(long long a, long long b) { return (unsigned)((a + 
 
 int enqueue_mixed(Engine &E) {
   hipStream_t st = E.stream;
-  dim3 gf(pad_xcd(cdivm(E.max_nf, 256)), E.nviews), bt(256);
+  dim3 gf(pad_xcd(cdivm(E.max_nf, 256 * RASTER_CHUNKS)), E.nviews), bt(256);  // (k_raster32)
   int wmax_ = 0;
   for (auto &V : E.h_views) wmax_ = std::max(wmax_, 2 * V.n + 1);
   int tiles = cdivm(wmax_, TILE);
@@ -313,33 +368,35 @@ int enqueue_mixed(Engine &E) {
                  E.d_bin_cnt, E.d_bin_list, E.d_fit, E.d_frames, E.d_overflow, E.d_params);
   }
   if (E.any_global) {
-    SHP_LAUNCH(E, "k_raster32", k_raster32, gf, bt, 0, st, E.d_views, E.d_slots, E.pd, E.total_pix, 0);
+    SHP_LAUNCH(E, "k_raster32", k_raster32, gf, bt, 0, st, E.d_views, E.d_slots, E.pd, E.total_pix, 0, 0);
     SHP_LAUNCH(E, "k_shade_radar32", k_shade_radar32, dim3(tiles * tiles, E.nviews), bt, 0, st, E.d_views, E.d_slots,
                E.pd, E.d_fit, E.d_frames, E.d_overflow, E.d_params, E.pd.nstale);
   }
   SHP_LAUNCH(E, "k_collect32", k_collect32, dim3(std::max(1, E.tune.collect_grid)), bt, 0, st, E.d_views, E.d_slots,
-             E.pd, E.nviews, (int)cdivm(E.max_nf, 256));
+             E.pd, E.nviews, (int)cdivm(E.max_nf, 256), 0, 0);
   SHP_LAUNCH(E, "k_replay32", k_replay32,
              dim3(std::max(1u, std::min(cdivm(E.pd.stale_cap, 256), (unsigned)E.tune.replay_grid))), bt, 0, st,
              E.d_views, E.d_slots, E.pd, E.d_fit, E.d_frames, E.d_overflow, E.d_params);
   if (E.any_lc) {
-    // Lightcurves: the FP64 kernels draw the observer and source views of
-    // every lightcurve point (radar views carry fp64_skip), mask shadows and
-    // sum brightness, exactly as the FP64 method does.  Replay records are
-    // shared with the radar replay above, which has finished with them.
+    // Lightcurve points (k_lc32.hip): the observer view (src 0), then the sun
+    // view (src 1), each rasterized in FP32, resolved per pixel in FP64 and
+    // its edge-band and faces-away pixels replayed in FP64, through the same
+    // key buffer and replay records (the radar replay has finished with
+    // them); then the shadow test and the photometry in FP64.
     const int T = 256;
-    dim3 gfl(cdivm(E.max_nf, T), E.nviews), gpl(cdivm(E.max_pix, T), E.nviews);
-    SHP_LAUNCH(E, "k_zero_counters", k_zero_counters, dim3(1), dim3(64), 0, st, E.pd.nstale, 1);
-    SHP_LAUNCH(E, "k_raster64/1", k_raster64, gfl, bt, 0, st, E.d_views, E.d_slots, E.pd, 0, 1);
-    SHP_LAUNCH(E, "k_raster64/2", k_raster64, gfl, bt, 0, st, E.d_views, E.d_slots, E.pd, 0, 2);
-    SHP_LAUNCH(E, "k_resolve64", k_resolve64, gpl, bt, 0, st, E.d_views, E.d_slots, E.pd, 0, 0, 1, 0, 0);
-    SHP_LAUNCH(E, "k_raster64/3", k_raster64, gfl, bt, 0, st, E.d_views, E.d_slots, E.pd, 0, 3);
-    SHP_LAUNCH(E, "k_replay64", k_replay64, dim3(std::min(cdivm(E.pd.stale_cap, T), 2048u)), bt, 0, st, E.d_views,
-               E.d_slots, E.pd);
-    SHP_LAUNCH(E, "k_zero_counters", k_zero_counters, dim3(1), dim3(64), 0, st, E.pd.nstale, 1);
-    SHP_LAUNCH(E, "k_raster64/1/sun", k_raster64, gfl, bt, 0, st, E.d_views, E.d_slots, E.pd, 1, 1);
-    SHP_LAUNCH(E, "k_raster64/2/sun", k_raster64, gfl, bt, 0, st, E.d_views, E.d_slots, E.pd, 1, 2);
-    SHP_LAUNCH(E, "k_resolve64", k_resolve64, gpl, bt, 0, st, E.d_views, E.d_slots, E.pd, 1, 0, 1, 0, 0);
+    dim3 gpl(cdivm(E.max_pix, T), E.nviews);
+    const dim3 grl(std::max(1u, cdivm(E.max_lc_rec, T)), E.nviews);  // (k_replay_lc: per view)
+    for (int src = 0; src < 2; src++) {
+      SHP_LAUNCH(E, "k_fill_i32", k_fill_i32, dim3(cdivm(E.nviews, T)), bt, 0, st, E.pd.lc_count, (long long)E.nviews, 0);
+      SHP_LAUNCH(E, "k_zero_counters", k_zero_counters, dim3(1), dim3(64), 0, st, E.pd.ncollect, 1);
+      SHP_LAUNCH(E, "k_fill_u8", k_fill_u8, dim3(256), bt, 0, st, E.pd.stale_tiles, std::max(1LL, E.total_tiles));
+      SHP_LAUNCH(E, src ? "k_raster32/lc/sun" : "k_raster32/lc", k_raster32, gf, bt, 0, st, E.d_views, E.d_slots, E.pd,
+                 0LL, src, 1);
+      SHP_LAUNCH(E, "k_resolve_lc", k_resolve_lc, gpl, bt, 0, st, E.d_views, E.d_slots, E.pd, src);
+      SHP_LAUNCH(E, "k_collect32", k_collect32, dim3(std::max(1, E.tune.collect_grid)), bt, 0, st, E.d_views,
+                 E.d_slots, E.pd, E.nviews, (int)cdivm(E.max_nf, 256), src, 1);
+      SHP_LAUNCH(E, "k_replay_lc", k_replay_lc, grl, bt, 0, st, E.d_views, E.d_slots, E.pd);
+    }
     SHP_LAUNCH(E, "k_posmask64", k_posmask64, gpl, bt, 0, st, E.d_views, E.pd, E.mask_tol);
     SHP_LAUNCH(E, "k_photo64", k_photo64, gpl, bt, 0, st, E.d_views, E.pd, E.d_params);
   }

@@ -54,6 +54,7 @@ struct PosDev {
   int stale_cap;
   int *error;                // bit flags: 1 stale list full, 2 too many candidates
   int *fmap;                 // mixed path, when asked for: each POS pixel's owner facet (-1: none)
+  int *lc_count;             // mixed path: replay records in use per lightcurve view (ViewRec::lc_rec_off)
 };
 
 // per-view outputs written by k_finish_views (doubles)

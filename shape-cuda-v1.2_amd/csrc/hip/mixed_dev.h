@@ -159,6 +159,17 @@ __device__ inline bool project32nf(const ViewRec &V, const GeomSlot &G, int f, c
   corners32p(V, gload4(G.fvx + 3 * f), gload4(G.fvx + 3 * f + 1), gload4(G.fvx + 3 * f + 2), m, P);
   return true;
 }
+// as project32 from the facet-major copies (one coalesced gather level; the
+// same values, so the same bits)
+__device__ inline bool project32f(const ViewRec &V, const GeomSlot &G, int f, const float *m, float nz_min,
+                                  Proj32 &P) {
+#pragma clang fp contract(off)
+  float4 fn = G.fn32[f];
+  float nz = facing32(m, fn);
+  if (!(nz > nz_min)) return false;  // back-facing (posvis.c:152)
+  corners32f(V, G, f, m, P);
+  return true;
+}
 __device__ inline bool project32(const ViewRec &V, const GeomSlot &G, int f, const float *m, float nz_min, Proj32 &P) {
 #pragma clang fp contract(off)
   float4 fn = G.fn32[f];
@@ -290,12 +301,36 @@ __device__ inline int stale_slot(const PosDev &pd, bool need) {
   return si;
 }
 
+// As stale_slot for a lightcurve view (index vi): its records come from the
+// view's own range of the list (ViewRec::lc_rec_off, lc_rec_cap), reserved on
+// the view's own counter; ncollect is only set nonzero (k_collect32 skips an
+// empty pass).  Every active lane of the wave must call it.
+__device__ inline int stale_slot_lc(const ViewRec &V, const PosDev &pd, int vi, bool need) {
+  unsigned long long mask = __ballot(need);
+  if (!mask) return -1;
+  int lane = threadIdx.x & 63, leader = __ffsll((long long)mask) - 1, base = 0;
+  if (lane == leader) {
+    base = atomicAdd(&pd.lc_count[vi], __popcll(mask));
+    *pd.ncollect = 1;
+  }
+  base = __builtin_amdgcn_readlane(base, leader);
+  if (!need) return -1;
+  int off = base + __popcll(mask & ((1ULL << lane) - 1ULL));
+  if (off >= V.lc_rec_cap) {
+    atomicOr(pd.error, 1);
+    return -1;
+  }
+  return V.lc_rec_off + off;
+}
+
 // an empty replay record: the pixel's claims start from blank sky and are
 // replayed in facet order
-__device__ inline void stale_init(const ViewRec &V, const PosDev &pd, int si, int view, int pix, int i, int j) {
+// (src: 0 observer view, 1 a lightcurve point's sun view)
+__device__ inline void stale_init(const ViewRec &V, const PosDev &pd, int si, int view, int pix, int i, int j,
+                                  int src = 0) {
   StaleRec &S = pd.stale[si];
   S.view = view; S.pix = pix; S.oldf = -1; S.oldz = -1.0e20; S.oldcose = 0.0;
-  S.oldcosi = 0.0; S.ncand = 0; S.src = 0;
+  S.oldcosi = 0.0; S.ncand = 0; S.src = src;
   pd.stale_idx[V.pos_off + pix] = si;
   pd.stale_tiles[stale_tile(V, i, j)] = 1;
 }
@@ -303,11 +338,12 @@ __device__ inline void stale_init(const ViewRec &V, const PosDev &pd, int si, in
 // Record a marginal pixel for the FP64 replay, once: the first facet to find
 // it swaps its stale index from -1.  Every active lane of the wave calls this
 // (marginal or not) so that records are reserved once per wave.
-__device__ inline void mark_marginal(const ViewRec &V, const PosDev &pd, bool marginal, int pix, int i, int j) {
+__device__ inline void mark_marginal(const ViewRec &V, const PosDev &pd, bool marginal, int pix, int i, int j,
+                                     int src = 0) {
   long long p = V.pos_off + pix;
   bool first = marginal && atomicCAS(&pd.stale_idx[p], -1, -2) == -1;
-  int si = stale_slot(pd, first);
-  if (si >= 0) stale_init(V, pd, si, blockIdx.y, pix, i, j);
+  int si = V.type == VT_LGHTCRV ? stale_slot_lc(V, pd, blockIdx.y, first) : stale_slot(pd, first);
+  if (si >= 0) stale_init(V, pd, si, blockIdx.y, pix, i, j, src);
   else if (first) pd.stale_idx[p] = -1;  // list full: the error flag is set
 }
 

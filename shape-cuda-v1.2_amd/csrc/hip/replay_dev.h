@@ -29,7 +29,7 @@ __device__ inline int replay_owner64(const ViewRec &V, const GeomSlot &G, const 
   for (int q = 0; q < nc; q++) {
     if (q > 0 && c[q] == c[q - 1]) continue;
     FacetProj P;
-    project_facet(V, G, c[q], 0, P);
+    project_facet_px(V, G, c[q], 0, P);
     if (!P.front) continue;
     double s, t, z;
     if (!pixel_hit(P, V.kmpp, i, j, s, t, z)) continue;
