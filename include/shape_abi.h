@@ -73,6 +73,33 @@ extern "C" {
 #define KM2HZFACT 7.72139109254982e-5
 #define MAXBINS 100
 
+/* penalty types, par->pen.type[1..n] (const.h:102-126) */
+#define OPTALBDEL 1
+#define OPTALBVAR 2
+#define THETAVAR 3
+#define THETADEL 4
+#define RADALBDEL 5
+#define RADALBVAR 6
+#define RAD_C_DEL 7
+#define RAD_C_VAR 8
+#define NONSMOOTH 9
+#define CONCAVITY 10
+#define RDEV 11
+#define VOLUME 12
+#define COMDEV 13
+#define INERTIADEV 14
+#define INERTIADEV_UNI 15
+#define PA3TILT 16
+#define NONPA 17
+#define NONPA_UNI 18
+#define EULEROFFS 19
+#define FLATTENING 20
+#define MAXRDEV 21
+#define MAXELLIPDEV 22
+#define NONCOSINE 23
+#define BIFURCATION 24
+#define IMPULSE 25
+
 /* ---- par_t and its penalty block (shape2.h:415-716) ------------------ */
 struct pen_t {
   int n;

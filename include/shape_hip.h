@@ -26,9 +26,20 @@ extern "C" {
 /* ---------------- Part 1: reference entry points ---------------- */
 
 /* realize_mod.c:153  vertex coordinates, facet normals/areas/centroids and
-   vertex normals of the model's (single, VERTEX-type) component; sets
-   par->baddiam and par->baddiam_logfactor. */
+   vertex normals of the model's (single, VERTEX-type) component, its
+   active flags (check_surface, :572) and its moments (compute_moments,
+   :757: area, volume, centre of mass and inertia tensor of the component and
+   of the model); sets par->baddiam and par->baddiam_logfactor. */
 void realize_mod(struct par_t *par, struct mod_t *mod);
+
+/* penalties.c:120  sum over par's penalty list of |weight| * base, each base
+   the penalty (or its reciprocal for a negative weight) written into
+   par->pen.base[i]; reads the realized model (coordinates, facet normals,
+   side list, the moments realize_mod left), spin inertia and impulses, and
+   the datasets' angle and spin offsets.  Types NONSMOOTH..MAXELLIPDEV,
+   BIFURCATION and IMPULSE; others (the harmonic / inhomogeneous /
+   tabular-law penalties) record an error and return 0. */
+double penalties(struct par_t *par, struct mod_t *mod, struct dat_t *dat);
 
 /* realize_spin.c:41  Euler matrices ae, intrinsic and total spin vectors of
    every frame view and lightcurve point (principal-axis rotators). */
@@ -137,9 +148,10 @@ void shape_hip_release_cache(void);
 int shape_hip_last_replayed(void);
 
 /* ---------------- Part 2: batched evaluation (no reference counterpart) ----
-   Evaluates many trial parameter values of bestfit's objective (bestfit.c:718,
-   with zero penalties) at once: realization, rendering, synthesis, chi-square
-   and the bad-model factors for every evaluation, device-resident.  Each
+   Evaluates many trial parameter values of bestfit's objective (bestfit.c:718)
+   at once: realization, rendering, synthesis, chi-square, par's penalties
+   (penalties.c, the geometric, moment, spin and offset types) and the
+   bad-model factors for every evaluation, device-resident.  Each
    evaluation e uses model eval_model[e] (an index into mods/dats) and sets
    n_targets[e] parameters, given as addresses of `val` fields inside that
    model's structs (e.g. par->fpntr entries), to the given trial values.
@@ -173,6 +185,12 @@ int shape_batch_facet_map(shape_batch *b, int e, int frame, int v, int *out);
 /* per frame of evaluation e: o2, m2, om, overflow_o2, overflow_m2, cal, chi2,
    badradar_logfactor (8 doubles per frame, set/frame order) */
 int shape_batch_frame_terms(shape_batch *b, int e, double *out);
+/* penalties of evaluation e after the last replay: par->pen.base[1..n] into
+   base[0..n) and their weighted sum (the term added to chi2/dof) into *total;
+   returns n.  shape_batch_create fails when par lists a penalty type outside
+   NONSMOOTH..MAXELLIPDEV, BIFURCATION, IMPULSE, or one that reads the side
+   list of a model without one. */
+int shape_batch_penalties(shape_batch *b, int e, double *base, double *total);
 /* [0] pixels replayed in FP64, [1] error flags, [2] mixed: replayed pixels
    whose candidate facets a second raster pass collected (the rest come from
    their tile's own list), [3] reserved (0; footprints of every width are

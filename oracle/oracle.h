@@ -43,6 +43,14 @@ void oracle_realize_spin(struct par_t *par, struct mod_t *mod, struct dat_t *dat
 /* COM offsets: realize_delcor.c:50-151, deldopoffs.c:17-40, dopoffs.c:19-39 */
 void oracle_realize_delcor(struct dat_t *dat, double delta_delcor0, int delcor0_mode);
 
+/* moments and penalties: realize_mod.c:757-840 with facmom.c (oracle_realize_mod
+   calls it last), diag_inertia.c (Jacobi, nr/jacobi.c), penalties.c:120-1250
+   (the geometric, moment, spin and offset penalties; sets par->pen.base) */
+void oracle_compute_moments(struct mod_t *mod);
+void oracle_jacobi3(double a[3][3], double d[3], double v[3][3]);
+void oracle_diag_inertia(double inertia[3][3], double pmoment[3], double ap[3][3]);
+double oracle_penalties(struct par_t *par, struct mod_t *mod, struct dat_t *dat);
+
 /* the fit's one-parameter line search: nr/mnbrak.c:10, nr/brent_abs.c:13,
    bestfit.c:446-468 (mnbrak from (x0, x0 + step), then brent_abs) */
 typedef double (*oracle_fn)(double);
@@ -77,6 +85,9 @@ double oracle_chi2(struct par_t *par, struct dat_t *dat, int list_breakdown);
 /* objective() of bestfit.c:718-871 with no penalties: chi2/dof times the
    bad-model factors.  Realization is the caller's job, as in the reference. */
 double oracle_objective_tail(struct par_t *par, struct dat_t *dat);
+/* objective() of bestfit.c:718-871 in full: chi2/dof + penalties, times the
+   bad-model factors (the model realized by the caller) */
+double oracle_objective(struct par_t *par, struct mod_t *mod, struct dat_t *dat);
 
 /* HMT-style CPU baseline (HostMultiThreading/calc_fits_hmt.c): datasets are
    dealt round-robin to nthreads threads (inputnode = s mod T); requires

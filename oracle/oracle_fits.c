@@ -335,10 +335,8 @@ double oracle_chi2(struct par_t *par, struct dat_t *dat, int list_breakdown)
   return dat->chi2;
 }
 
-/* bestfit.c:789-866 with penalties = 0 */
-double oracle_objective_tail(struct par_t *par, struct dat_t *dat)
+static double o_bad_factors(struct par_t *par, double err)
 {
-  double err = oracle_chi2(par, dat, 0) / dat->dof;
   if (par->baddiam) err *= par->bad_objfactor * exp(par->baddiam_logfactor);
   if (par->badphoto) err *= par->bad_objfactor * exp(par->badphoto_logfactor);
   if (par->posbnd) err *= par->bad_objfactor * exp(par->posbnd_logfactor);
@@ -346,4 +344,19 @@ double oracle_objective_tail(struct par_t *par, struct dat_t *dat)
   if (par->badradar) err *= par->bad_objfactor * exp(par->badradar_logfactor);
   if (par->baddopscale) err *= par->bad_objfactor * exp(par->baddopscale_logfactor);
   return err;
+}
+
+/* bestfit.c:789-866 with penalties = 0 */
+double oracle_objective_tail(struct par_t *par, struct dat_t *dat)
+{
+  return o_bad_factors(par, oracle_chi2(par, dat, 0) / dat->dof);
+}
+
+/* bestfit.c:789-866: err = chi2/dof (:790-793), err += penalties (:807-811),
+   then the bad-model factors (:813-866) */
+double oracle_objective(struct par_t *par, struct mod_t *mod, struct dat_t *dat)
+{
+  double err = oracle_chi2(par, dat, 0) / dat->dof;
+  err += oracle_penalties(par, mod, dat);
+  return o_bad_factors(par, err);
 }

@@ -31,8 +31,8 @@ static double o_facnrm(struct vertices_t *vt, int fi)
 
 /* realize_mod.c:153-177 -> realize_coordinates :183, VERTEX branch :444-467,
    coordinates :475-479, component rotation/offset :484-512, facets :532-546,
-   vertex normals :552-563.  check_surface is the identity for one component;
-   compute_moments feeds only penalties and is not restated. */
+   vertex normals :552-563, check_surface for one component (:580-593: every
+   vertex, facet and side active), compute_moments :757 (oracle_pen.c). */
 void oracle_realize_mod(struct par_t *par, struct mod_t *mod)
 {
   par->baddiam = 0;
@@ -93,7 +93,9 @@ void oracle_realize_mod(struct par_t *par, struct mod_t *mod)
     }
     for (int i = 0; i < nv; i++) real->v[i].act = 1;
     for (int f = 0; f < nf; f++) real->f[f].act = 1;
+    for (int k = 0; k < real->ns; k++) real->s[k].act = 1;
   }
+  oracle_compute_moments(mod);
 }
 
 /* realize_spin.c:350-398: epochs and impulses met between t0 and t */

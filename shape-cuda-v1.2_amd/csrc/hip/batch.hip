@@ -69,6 +69,7 @@ extern "C" shape_batch *shape_batch_create(struct par_t *par, struct mod_t **mod
   o.keep_fit = (flags & SHAPE_BATCH_KEEP_FIT) ? 1 : 0;
   o.global_zbuf = (flags & SHAPE_BATCH_GLOBAL_ZBUF) ? 1 : 0;
   o.facet_map = (flags & SHAPE_BATCH_FACET_MAP) ? 1 : 0;
+  o.penalties = 1;
   B->use_graph = (flags & SHAPE_BATCH_GRAPH) ? 1 : 0;
   int rc = B->E.build(par, mods, dats, nmodels, ev, o);
   if (!rc && B->use_graph) rc = B->E.capture_graph();
@@ -262,6 +263,18 @@ extern "C" int shape_batch_frame_terms(shape_batch *B, int e, double *out) {
     o[5] = F.cal; o[6] = F.chi2; o[7] = F.badradar_logfactor;
   }
   return 0;
+}
+
+// Penalties of evaluation e after the last replay: par->pen.base[1..n] as
+// penalties.c leaves them into base[0..n), the weighted sum into *total;
+// returns n (0 when par lists none) or < 0.
+extern "C" int shape_batch_penalties(shape_batch *B, int e, double *base, double *total) {
+  if (e < 0 || e >= B->E.nevals) return -20;
+  std::vector<double> b;
+  int rc = B->E.download_penalties(e, b, total);
+  if (rc) return rc;
+  if (base) std::copy(b.begin(), b.end(), base);
+  return (int)b.size();
 }
 
 // FP64 copy of the fit image of frame q (0-based within evaluation e); needs SHAPE_BATCH_KEEP_FIT

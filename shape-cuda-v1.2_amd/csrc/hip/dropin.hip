@@ -543,8 +543,10 @@ namespace {
 struct RealizeBufs {
   const void *mod = nullptr;
   unsigned long long topo = 0;
-  GrowBuf fv, off, idx, u, a, r, pc, x, vn, fn, fx, ar, x32, vn32, fn32, slot, flag;
+  GrowBuf fv, off, idx, u, a, r, pc, x, vn, fn, fx, ar, x32, vn32, fn32, slot, flag, mom, part;
 };
+// workgroups per model for the moment reductions (as the engine's)
+int moment_blocks(int nf) { return std::max(1, std::min(64, (nf + 2047) / 2048)); }
 std::mutex g_realize_mu;
 RealizeBufs g_rb;
 }  // namespace
@@ -577,7 +579,8 @@ extern "C" void realize_mod(struct par_t *par, struct mod_t *mod) {
            B.pc.get(sizeof(double) * 9) | B.x.get(sizeof(double) * 3 * nv) | B.vn.get(sizeof(double) * 3 * nv) |
            B.fn.get(sizeof(double) * 3 * nf) | B.fx.get(sizeof(double) * 3 * nf) | B.ar.get(sizeof(double) * nf) |
            B.x32.get(sizeof(float4) * nv) | B.vn32.get(sizeof(float4) * nv) | B.fn32.get(sizeof(float4) * nf) |
-           B.slot.get(sizeof(GeomSlot)) | B.flag.get(2 * sizeof(double));  // baddiam flag, log factor at +8
+           B.slot.get(sizeof(GeomSlot)) | B.flag.get(2 * sizeof(double)) |  // baddiam flag, log factor at +8
+           B.mom.get(sizeof(double) * MOM_N) | B.part.get(sizeof(double) * MP1_N * moment_blocks(nf));
   if (rc) { B.mod = nullptr; set_err(rc, "realize_mod allocation"); return; }
   bool ok = true;
   if (new_topo) {
@@ -628,6 +631,12 @@ extern "C" void realize_mod(struct par_t *par, struct mod_t *mod) {
                      dflag, dlogf);
   hipLaunchKernelGGL(k_realize_facets, dim3((nf + 255) / 256, 1), dim3(256), 0, 0, ds, 1);
   hipLaunchKernelGGL(k_realize_vnormals, dim3((nv + 255) / 256, 1), dim3(256), 0, 0, ds, 1);
+  // compute_moments (realize_mod.c:757-840): area, volume, COM, inertia tensor
+  const int nb = moment_blocks(nf);
+  hipLaunchKernelGGL(k_moments_part, dim3(nb, 1), dim3(256), 0, 0, ds, (double *)B.part.p, nb);
+  hipLaunchKernelGGL(k_moments_fin, dim3(1), dim3(64), 0, 0, ds, (const double *)B.pc.p, (const double *)B.part.p, nb,
+                     (double *)B.mom.p, (const double *)nullptr, 0);
+  double mom[MOM_N];
   // results through the thread's pinned stage (plain host memory if it cannot be pinned)
   const size_t nout = 6 * (size_t)nv + 7 * (size_t)nf;
   std::vector<double> plain;
@@ -643,7 +652,8 @@ extern "C" void realize_mod(struct par_t *par, struct mod_t *mod) {
        hipMemcpy(fx, B.fx.p, sizeof(double) * 3 * nf, hipMemcpyDeviceToHost) == hipSuccess &&
        hipMemcpy(ar, B.ar.p, sizeof(double) * nf, hipMemcpyDeviceToHost) == hipSuccess &&
        hipMemcpy(&flag, dflag, sizeof(int), hipMemcpyDeviceToHost) == hipSuccess &&
-       hipMemcpy(&logf, dlogf, sizeof(double), hipMemcpyDeviceToHost) == hipSuccess;
+       hipMemcpy(&logf, dlogf, sizeof(double), hipMemcpyDeviceToHost) == hipSuccess &&
+       hipMemcpy(mom, B.mom.p, sizeof(double) * MOM_N, hipMemcpyDeviceToHost) == hipSuccess;
   if (!ok) { (void)hipGetLastError(); set_err(-4, "realize_mod kernels"); return; }
   for (int i = 0; i < nv; i++)
     for (int k = 0; k < 3; k++) { rv->v[i].x[k] = x[3 * i + k]; rv->v[i].n[k] = vn[3 * i + k]; }
@@ -658,6 +668,145 @@ extern "C" void realize_mod(struct par_t *par, struct mod_t *mod) {
   }
   par->baddiam = (unsigned char)flag;
   par->baddiam_logfactor = logf;
+  // check_surface for one component (realize_mod.c:580-593): everything active
+  for (int i = 0; i < nv; i++) rv->v[i].act = 1;
+  for (int f = 0; f < nf; f++) rv->f[f].act = 1;
+  if (rv->s)
+    for (int k = 0; k < rv->ns; k++) rv->s[k].act = 1;
+  // the component's moments equal the model's (one component, every facet active)
+  mod->shape.area = comp->area = mom[MOM_AREA];
+  mod->shape.volume = comp->volume = mom[MOM_VOL];
+  for (int j = 0; j < 3; j++) {
+    mod->shape.com[j] = comp->com[j] = mom[MOM_COM + j];
+    for (int k = 0; k < 3; k++) mod->shape.inertia[j][k] = comp->inertia[j][k] = mom[MOM_I + 3 * j + k];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// penalties (penalties.c:120): the penalty sum of par's list for the model as
+// realize_mod left it (vertex coordinates, facet normals, the moments in
+// mod->shape) and the datasets' angle / spin offsets; sets par->pen.base.
+// The sides, vertices and principal axes are reduced on the device by the
+// batched path's kernels, with the caller's moments in place of their own.
+// ---------------------------------------------------------------------------
+namespace {
+struct PenBufs {
+  GrowBuf fv, x, fn, ar, sides, u, r, slot, given, pb, type, weight, base, total, mom, part, ev, evslot;
+};
+std::mutex g_pen_mu;
+PenBufs g_pb;
+}  // namespace
+
+extern "C" double penalties(struct par_t *par, struct mod_t *mod, struct dat_t *dat) {
+  const int n = par->pen.n;
+  if (n <= 0) return 0.0;
+  if (mod->shape.ncomp != 1 || mod->shape.comp[0].type != VERTEX) { set_err(-10, "penalties"); return 0.0; }
+  struct comp_t *comp = &mod->shape.comp[0];
+  struct vertices_t *rv = &comp->real;
+  const int nv = rv->nv, nf = rv->nf, ns = rv->s ? rv->ns : 0;
+  std::vector<int> type(n);
+  std::vector<double> weight(n);
+  int need = 0;
+  for (int i = 0; i < n; i++) {
+    type[i] = par->pen.type[i + 1];
+    weight[i] = par->pen.weight[i + 1];
+    if (!pen_supported(type[i])) { set_err(-17, "penalties: type outside this path"); return 0.0; }
+    need |= pen_needs(type[i]);
+  }
+  if ((need & PEN_NEED_SIDES) && ns <= 0) { set_err(-18, "penalties: model has no side list"); return 0.0; }
+  // host inputs: corners, coordinates, facet normals, sides, directions, deviations
+  std::vector<int> fv(3 * (size_t)nf);
+  std::vector<double> x(3 * (size_t)nv), fn(3 * (size_t)nf), u(3 * (size_t)nv), r(nv), ar(nf);
+  std::vector<int4> sides(std::max(ns, 1));
+  for (int f = 0; f < nf; f++)
+    for (int k = 0; k < 3; k++) { fv[3 * f + k] = rv->f[f].v[k]; fn[3 * f + k] = rv->f[f].n[k]; }
+  for (int f = 0; f < nf; f++) ar[f] = rv->f[f].area;
+  for (int i = 0; i < nv; i++) {
+    for (int k = 0; k < 3; k++) { x[3 * i + k] = rv->v[i].x[k]; u[3 * i + k] = rv->v[i].u[k]; }
+    r[i] = rv->v[i].r.val;
+  }
+  for (int q = 0; q < ns; q++) sides[q] = make_int4(rv->s[q].v[0], rv->s[q].v[1], rv->s[q].f[0], rv->s[q].f[1]);
+  // the moments and realized scale factors realize_mod left
+  double given[17];
+  given[0] = mod->shape.area;
+  given[1] = mod->shape.volume;
+  for (int j = 0; j < 3; j++) given[2 + j] = mod->shape.com[j];
+  for (int j = 0; j < 3; j++)
+    for (int k = 0; k < 3; k++) given[5 + 3 * j + k] = mod->shape.inertia[j][k];
+  for (int j = 0; j < 3; j++) given[14 + j] = rv->scalefactor[j].val;
+  // a parameter block with the spin block and one offset block per dataset
+  const int nsets = dat->nsets;
+  std::vector<double> pb(PB_SPIN_END + (size_t)PB_SET_STRIDE * nsets, 0.0);
+  for (int j = 0; j < 3; j++) pb[PB_INERTIA + j] = mod->spin.inertia[j].val;
+  pb[PB_NIMP] = mod->spin.n_impulse;
+  for (int q = 0; q < mod->spin.n_impulse && q < MAXIMP; q++)
+    for (int j = 0; j < 3; j++) pb[PB_IMP + 3 * q + j] = mod->spin.impulse[q][j].val;
+  for (int k = 0; k < nsets; k++)
+    for (int j = 0; j < 3; j++) {
+      pb[PB_SPIN_END + PB_SET_STRIDE * k + PB_SET_ANGLEOFF + j] = dat->set[k].angleoff[j].val;
+      pb[PB_SPIN_END + PB_SET_STRIDE * k + PB_SET_OMEGAOFF + j] = dat->set[k].omegaoff[j].val;
+    }
+  EvalRec E;
+  memset(&E, 0, sizeof(E));
+  E.pb = 0;
+  E.pen_set_off = PB_SPIN_END;
+  E.pen_nsets = nsets;
+  const int nb = moment_blocks(nf), zero = 0;
+  std::lock_guard<std::mutex> lk(g_pen_mu);
+  PenBufs &B = g_pb;
+  int rc = B.fv.get(sizeof(int) * fv.size()) | B.x.get(sizeof(double) * x.size()) | B.fn.get(sizeof(double) * fn.size()) |
+           B.ar.get(sizeof(double) * ar.size()) | B.sides.get(sizeof(int4) * sides.size()) |
+           B.u.get(sizeof(double) * u.size()) | B.r.get(sizeof(double) * r.size()) | B.slot.get(sizeof(GeomSlot)) |
+           B.given.get(sizeof(given)) | B.pb.get(sizeof(double) * pb.size()) | B.type.get(sizeof(int) * n) |
+           B.weight.get(sizeof(double) * n) | B.base.get(sizeof(double) * n) | B.total.get(sizeof(double)) |
+           B.mom.get(sizeof(double) * MOM_N) | B.part.get(sizeof(double) * std::max(MP1_N, MP2_N) * nb) |
+           B.ev.get(sizeof(EvalRec)) | B.evslot.get(sizeof(int));
+  if (rc) { set_err(rc, "penalties allocation"); return 0.0; }
+  GeomSlot G;
+  memset(&G, 0, sizeof(G));
+  G.nv = nv; G.nf = nf; G.eval = -1;
+  G.fv = (int *)B.fv.p; G.x = (double *)B.x.p; G.fn = (double *)B.fn.p; G.area = (double *)B.ar.p;
+  G.u = (double *)B.u.p; G.r = (double *)B.r.p;
+  G.sides = ns > 0 ? (const int4 *)B.sides.p : nullptr;
+  G.ns = ns;
+  bool ok = hipMemcpy(B.fv.p, fv.data(), sizeof(int) * fv.size(), hipMemcpyHostToDevice) == hipSuccess &&
+            hipMemcpy(B.x.p, x.data(), sizeof(double) * x.size(), hipMemcpyHostToDevice) == hipSuccess &&
+            hipMemcpy(B.fn.p, fn.data(), sizeof(double) * fn.size(), hipMemcpyHostToDevice) == hipSuccess &&
+            hipMemcpy(B.ar.p, ar.data(), sizeof(double) * ar.size(), hipMemcpyHostToDevice) == hipSuccess &&
+            hipMemcpy(B.sides.p, sides.data(), sizeof(int4) * sides.size(), hipMemcpyHostToDevice) == hipSuccess &&
+            hipMemcpy(B.u.p, u.data(), sizeof(double) * u.size(), hipMemcpyHostToDevice) == hipSuccess &&
+            hipMemcpy(B.r.p, r.data(), sizeof(double) * r.size(), hipMemcpyHostToDevice) == hipSuccess &&
+            hipMemcpy(B.slot.p, &G, sizeof(G), hipMemcpyHostToDevice) == hipSuccess &&
+            hipMemcpy(B.given.p, given, sizeof(given), hipMemcpyHostToDevice) == hipSuccess &&
+            hipMemcpy(B.pb.p, pb.data(), sizeof(double) * pb.size(), hipMemcpyHostToDevice) == hipSuccess &&
+            hipMemcpy(B.type.p, type.data(), sizeof(int) * n, hipMemcpyHostToDevice) == hipSuccess &&
+            hipMemcpy(B.weight.p, weight.data(), sizeof(double) * n, hipMemcpyHostToDevice) == hipSuccess &&
+            hipMemcpy(B.ev.p, &E, sizeof(E), hipMemcpyHostToDevice) == hipSuccess &&
+            hipMemcpy(B.evslot.p, &zero, sizeof(int), hipMemcpyHostToDevice) == hipSuccess;
+  if (!ok) { (void)hipGetLastError(); set_err(-3, "penalties upload"); return 0.0; }
+  const GeomSlot *ds = (const GeomSlot *)B.slot.p;
+  double *part = (double *)B.part.p, *dmom = (double *)B.mom.p;
+  const int pa = (need & PEN_NEED_PA) ? 1 : 0;
+  hipLaunchKernelGGL(k_moments_part, dim3(nb, 1), dim3(256), 0, 0, ds, part, nb);
+  hipLaunchKernelGGL(k_moments_fin, dim3(1), dim3(64), 0, 0, ds, (const double *)B.pb.p, (const double *)part, nb, dmom,
+                     (const double *)B.given.p, pa);
+  if (need & PEN_NEED_VERTS) {
+    hipLaunchKernelGGL(k_pen_vert_part, dim3(nb, 1), dim3(256), 0, 0, ds, (const double *)dmom, part, nb, need);
+    hipLaunchKernelGGL(k_pen_vert_fin, dim3(1), dim3(64), 0, 0, ds, (const double *)part, nb, dmom, need);
+  }
+  PenDev P;
+  P.n = n; P.type = (const int *)B.type.p; P.weight = (const double *)B.weight.p; P.base = (double *)B.base.p;
+  P.total = (double *)B.total.p; P.mom = dmom;
+  hipLaunchKernelGGL(k_pen_eval, dim3(1), dim3(64), 0, 0, (const EvalRec *)B.ev.p, 1, (const int *)B.evslot.p,
+                     (const double *)B.pb.p, P);
+  std::vector<double> base(n);
+  double total = 0.0;
+  ok = hipDeviceSynchronize() == hipSuccess &&
+       hipMemcpy(base.data(), B.base.p, sizeof(double) * n, hipMemcpyDeviceToHost) == hipSuccess &&
+       hipMemcpy(&total, B.total.p, sizeof(double), hipMemcpyDeviceToHost) == hipSuccess;
+  if (!ok) { (void)hipGetLastError(); set_err(-4, "penalties kernels"); return 0.0; }
+  for (int i = 0; i < n; i++) par->pen.base[i + 1] = base[i];
+  return total;
 }
 
 // ---------------------------------------------------------------------------

@@ -35,7 +35,7 @@ enum Method { METHOD_FP64 = 0, METHOD_MIXED = 1 };
 // spin block
 constexpr int PB_ANGLE = 0, PB_OMEGA = 3, PB_OMEGADOT = 6, PB_T0 = 9, PB_LIBAMP = 10,
               PB_LIBFREQ = 11, PB_LIBPHASE = 12, PB_PA = 13, PB_NIMP = 14, PB_TIMP = 15,
-              PB_IMP = 25, PB_SPIN_END = 55;
+              PB_IMP = 25, PB_INERTIA = 55, PB_SPIN_END = 58;
 // per radar law (8): type, R, C, cos_cutoff, qsR, qsC, qs_cutoff, spare
 constexpr int PB_RAD_STRIDE = 8;
 // per optical law (6): type, R, wt, A0, D, k
@@ -65,6 +65,8 @@ struct GeomSlot {
   float4 *fb;            // mixed method: per facet, bounding sphere {center, radius}
   float4 *fvx, *fvn;     // mixed method: per facet, its 3 vertices' x32 / vn32 (one gather, not two)
   unsigned char *seen;   // nf
+  const int4 *sides;     // ns: the side (edge) list, {v0, v1, f0, f1} (penalties only)
+  int ns;
 };
 
 // view types (same numbers as set_t types)
@@ -147,6 +149,7 @@ struct FrameRec {
 
 struct EvalRec {
   int model, pb, first_frame, nframes;
+  int pen_set_off, pen_nsets;  // parameter-block offset and count of the datasets' offset blocks (euleroffs)
   double dof, dof_deldop, dof_doppler, dof_lghtcrv;
   double bad_objfactor;
   // outputs

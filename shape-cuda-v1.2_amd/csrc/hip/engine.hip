@@ -53,12 +53,15 @@ struct ModelPack {
   // topology / inputs (host)
   std::vector<int> fv, af_off, af_idx;
   std::vector<double> u, a, r;
+  std::vector<int4> sides;  // v0, v1, f0, f1 (penalties)
   int nv = 0, nf = 0;
+  int set_off = 0, nsets = 0;  // the datasets' blocks in the parameter block
   // realized geometry from the host (drop-in mode)
   std::vector<double> hx, hvn, hfn;
   // device topology
   int *d_fv = nullptr, *d_af_off = nullptr, *d_af_idx = nullptr;
   double *d_u = nullptr, *d_a = nullptr;
+  int4 *d_sides = nullptr;
   // view / frame templates
   std::vector<ViewRec> views;
   std::vector<FrameRec> frames;
@@ -128,6 +131,8 @@ int Engine::pack_model(int mi, struct par_t *par, struct mod_t *mod, struct dat_
   int size = off_comp + PB_COMP_STRIDE;
   P.base.assign(size, 0.0);
   P.comp_off = off_comp;
+  P.set_off = off_set;
+  P.nsets = nsets;
   double *b = P.base.data();
   struct spin_t *sp = &mod->spin;
   for (int j = 0; j < 3; j++) {
@@ -140,6 +145,10 @@ int Engine::pack_model(int mi, struct par_t *par, struct mod_t *mod, struct dat_
   b[PB_LIBFREQ] = sp->lib_freq.val; set_param_target(tmap, &sp->lib_freq.val, mi, PB_LIBFREQ);
   b[PB_LIBPHASE] = sp->lib_phase.val; set_param_target(tmap, &sp->lib_phase.val, mi, PB_LIBPHASE);
   b[PB_PA] = 1.0;
+  for (int j = 0; j < 3; j++) {
+    b[PB_INERTIA + j] = sp->inertia[j].val;
+    set_param_target(tmap, &sp->inertia[j].val, mi, PB_INERTIA + j);
+  }
   b[PB_NIMP] = sp->n_impulse;
   for (int q = 0; q < sp->n_impulse && q < MAXIMP; q++) {
     b[PB_TIMP + q] = sp->t_impulse[q];
@@ -257,6 +266,9 @@ int Engine::pack_model(int mi, struct par_t *par, struct mod_t *mod, struct dat_
     int src = v->r.state == '=' ? v->v_mirror : i;
     P.r[i] = rv->v[src].r.val;
   }
+  P.sides.clear();
+  if (opt.penalties && rv->s)
+    for (int q = 0; q < rv->ns; q++) P.sides.push_back(make_int4(rv->s[q].v[0], rv->s[q].v[1], rv->s[q].f[0], rv->s[q].f[1]));
   }
   if (opt.host_geometry) {
     P.hx.resize(3 * (size_t)P.nv); P.hvn.resize(3 * (size_t)P.nv); P.hfn.resize(3 * (size_t)P.nf);
@@ -441,6 +453,7 @@ int Engine::build(struct par_t *par, struct mod_t **mods, struct dat_t **dats, i
     if ((rc = upload(this, &P.d_af_idx, P.af_idx))) return rc;
     if ((rc = upload(this, &P.d_u, P.u))) return rc;
     if ((rc = upload(this, &P.d_a, P.a))) return rc;
+    if (!P.sides.empty() && (rc = upload(this, &P.d_sides, P.sides))) return rc;
   }
   // evaluations, their slots, views and frames
   eval_targets.assign(nevals, {});
@@ -518,6 +531,8 @@ int Engine::build(struct par_t *par, struct mod_t **mods, struct dat_t **dats, i
     E.nframes = (int)P.frames.size();
     E.dof = P.dof; E.dof_deldop = P.dof_dd; E.dof_doppler = P.dof_dop; E.dof_lghtcrv = P.dof_lc;
     E.bad_objfactor = par->bad_objfactor;
+    E.pen_set_off = P.set_off;
+    E.pen_nsets = P.nsets;
     h_evals.push_back(E);
     int view_base = (int)h_views.size(), frame_base = (int)h_frames.size();
     for (size_t q = 0; q < P.views.size(); q++) {
@@ -595,6 +610,7 @@ int Engine::build(struct par_t *par, struct mod_t **mods, struct dat_t **dats, i
     ModelPack &P = g_packs[G.model];
     max_nf = std::max(max_nf, G.nf);
     G.fv = P.d_fv; G.af_off = P.d_af_off; G.af_idx = P.d_af_idx; G.u = P.d_u; G.a = P.d_a;
+    G.sides = P.d_sides; G.ns = (int)P.sides.size();
     if ((rc = upload(this, &G.r, slot_r[s]))) return rc;
     if ((rc = alloc((void **)&G.x, sizeof(double) * 3 * G.nv))) return rc;
     if ((rc = alloc((void **)&G.vn, sizeof(double) * 3 * G.nv))) return rc;
@@ -617,6 +633,7 @@ int Engine::build(struct par_t *par, struct mod_t **mods, struct dat_t **dats, i
     SHP_CHECK(hipMemset(G.seen, 0, G.nf));
     if (opt.host_geometry && (rc = upload_host_geometry(s))) return rc;
   }
+  if ((rc = setup_penalties(par))) return rc;
   if ((rc = upload(this, &d_slots, h_slots))) return rc;
   if ((rc = upload(this, &d_views, h_views))) return rc;
   if ((rc = upload(this, &d_frames, h_frames))) return rc;
@@ -889,6 +906,60 @@ int Engine::upload_params() {
 
 static inline unsigned cdiv(long long a, long long b) { return (unsigned)((a + b - 1) / b); }
 
+// The penalty list of par (penalties.c): every type must be one the kernels
+// evaluate (-17 otherwise) and the side list must be present when a penalty
+// reads sides (-18).  Workgroups per slot for the moment reductions grow with
+// the facet count up to 64; the count is fixed for the engine's life, so the
+// sums repeat from replay to replay.
+int Engine::setup_penalties(struct par_t *par) {
+  pen = PenDev{};
+  pen_need = 0;
+  if (!opt.penalties || par->pen.n <= 0 || opt.views_only) return 0;
+  const int n = par->pen.n;
+  std::vector<int> type(n);
+  std::vector<double> weight(n);
+  for (int i = 0; i < n; i++) {
+    type[i] = par->pen.type[i + 1];
+    weight[i] = par->pen.weight[i + 1];
+    if (!pen_supported(type[i])) return -17;
+    pen_need |= pen_needs(type[i]);
+  }
+  if (pen_need & PEN_NEED_SIDES)
+    for (auto &G : h_slots)
+      if (G.ns <= 0 || !G.sides) return -18;
+  pen_nb = std::max(1, std::min(64, (int)cdiv(max_nf, 2048)));
+  int rc;
+  int *dt;
+  double *dw;
+  if ((rc = upload(this, &dt, type))) return rc;
+  if ((rc = upload(this, &dw, weight))) return rc;
+  pen.n = n;
+  pen.type = dt;
+  pen.weight = dw;
+  if ((rc = alloc((void **)&pen.base, sizeof(double) * (size_t)n * std::max(1, nevals)))) return rc;
+  if ((rc = alloc((void **)&pen.total, sizeof(double) * std::max(1, nevals)))) return rc;
+  if ((rc = alloc((void **)&d_mom, sizeof(double) * MOM_N * (size_t)nslots))) return rc;
+  if ((rc = alloc((void **)&d_mom_part, sizeof(double) * std::max(MP1_N, MP2_N) * (size_t)nslots * pen_nb))) return rc;
+  pen.mom = d_mom;
+  return 0;
+}
+
+void Engine::enqueue_penalties() {
+  if (!pen.n) return;
+  const bool pa = (pen_need & PEN_NEED_PA) != 0;
+  SHP_LAUNCH(*this, "k_moments_part", k_moments_part, dim3(pen_nb, nslots), dim3(256), 0, stream, d_slots, d_mom_part, pen_nb);
+  SHP_LAUNCH(*this, "k_moments_fin", k_moments_fin, dim3(nslots), dim3(64), 0, stream, d_slots, d_params, d_mom_part, pen_nb,
+             d_mom, nullptr, pa ? 1 : 0);
+  if (pen_need & PEN_NEED_VERTS) {
+    SHP_LAUNCH(*this, "k_pen_vert_part", k_pen_vert_part, dim3(pen_nb, nslots), dim3(256), 0, stream, d_slots, d_mom, d_mom_part,
+               pen_nb, pen_need);
+    SHP_LAUNCH(*this, "k_pen_vert_fin", k_pen_vert_fin, dim3(nslots), dim3(64), 0, stream, d_slots, d_mom_part, pen_nb, d_mom,
+               pen_need);
+  }
+  SHP_LAUNCH(*this, "k_pen_eval", k_pen_eval, dim3(std::max(1, nevals)), dim3(64), 0, stream, d_evals, nevals, d_eval_slot,
+             d_params, pen);
+}
+
 int Engine::enqueue() {
   const int T = 256;
   // an error left by an earlier, already reported call must not fail this one
@@ -908,6 +979,7 @@ int Engine::enqueue() {
     SHP_LAUNCH(*this, "k_realize_facets", k_realize_facets, dim3(cdiv(max_nf, T), nslots), dim3(T), 0, stream, d_slots, nslots);
     SHP_LAUNCH(*this, "k_realize_vnormals", k_realize_vnormals, dim3(cdiv(max_nv, T), nslots), dim3(T), 0, stream, d_slots, nslots);
   }
+  enqueue_penalties();
   if (opt.method == METHOD_MIXED) return enqueue_mixed(*this);
   dim3 gf(cdiv(max_nf, T), nviews), gp(cdiv(max_pix, T), nviews), bt(T);
   // observer view
@@ -937,7 +1009,7 @@ int Engine::enqueue() {
   SHP_LAUNCH(*this, "k_chi2_frames64", k_chi2_frames64, dim3(nframes), bt, 0, stream, d_frames, nframes, d_fit, d_obs, d_oov,
                      d_view_out, opt.keep_fit ? 0 : 1, 0, 0);
   SHP_LAUNCH(*this, "k_objective", k_objective, dim3(std::max(1, nevals)), dim3(256), 0, stream, d_evals, nevals, d_frames, d_views,
-                     d_view_out, d_slot_flags, d_slot_logf, d_eval_slot);
+                     d_view_out, d_slot_flags, d_slot_logf, d_eval_slot, pen.n ? pen.total : nullptr);
   SHP_CHECK(hipGetLastError());
   return 0;
 }
@@ -1006,6 +1078,14 @@ int Engine::collect_kernel_times() {
 int Engine::download_evals(std::vector<EvalRec> &out) {
   out.resize(nevals);
   SHP_CHECK(hipMemcpyAsync(out.data(), d_evals, sizeof(EvalRec) * nevals, hipMemcpyDeviceToHost, stream));
+  SHP_CHECK(hipStreamSynchronize(stream));
+  return 0;
+}
+int Engine::download_penalties(int e, std::vector<double> &base, double *total) {
+  base.assign(pen.n, 0.0);
+  if (!pen.n) { if (total) *total = 0.0; return 0; }
+  SHP_CHECK(hipMemcpyAsync(base.data(), pen.base + (size_t)e * pen.n, sizeof(double) * pen.n, hipMemcpyDeviceToHost, stream));
+  if (total) SHP_CHECK(hipMemcpyAsync(total, pen.total + e, sizeof(double), hipMemcpyDeviceToHost, stream));
   SHP_CHECK(hipStreamSynchronize(stream));
   return 0;
 }

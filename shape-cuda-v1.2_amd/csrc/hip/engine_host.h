@@ -74,6 +74,7 @@ struct RunOptions {
   int global_zbuf = 0;      // mixed method: rasterize every view through the global z-buffer
   int facet_map = 0;        // keep every POS pixel's owner facet for inspection
   int only_set = -1;        // >= 0: views and frames of this dataset only (calc_deldop & co.)
+  int penalties = 0;        // 1: add par's penalties to each objective (the batched interface)
 };
 
 class Engine {
@@ -154,6 +155,11 @@ class Engine {
   int *d_tile_work = nullptr;               // the persistent tile kernel's work counters (2 NXCD ints)
   // Launch settings a tuner may change between replays (Tune)
   Tune tune;
+  // penalties: the list, per-slot moments and terms, per-evaluation results
+  PenDev pen{};
+  int pen_need = 0, pen_nb = 1;
+  double *d_mom = nullptr, *d_mom_part = nullptr;
+  int download_penalties(int e, std::vector<double> &base, double *total);
   PosDev pd{};
   std::vector<void *> allocs;
   size_t dev_bytes = 0;     // device memory held by allocs
@@ -176,6 +182,8 @@ class Engine {
   int alloc(void **p, size_t bytes);
   int pack_model(int m, struct par_t *par, struct mod_t *mod, struct dat_t *dat, bool repack = false);
   int upload_host_geometry(int slot);
+  int setup_penalties(struct par_t *par);
+  void enqueue_penalties();
 };
 
 // record a failure of a reference entry point for shape_hip_last_error()

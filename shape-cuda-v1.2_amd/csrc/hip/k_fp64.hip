@@ -1088,12 +1088,13 @@ __global__ void k_chi2_frames64(FrameRec *frames, int nframes, double *fit, cons
   }
 }
 
-// objective per evaluation (bestfit.c:789-866 with zero penalties), in
+// objective per evaluation (bestfit.c:789-866; the penalty sum from k_pen_eval), in
 // fixed frame order; log factors follow calc_fits.c:407-409, :511-512,
 // :547-550, :1654-1677, :1947
 __global__ __launch_bounds__(256) void k_objective(EvalRec *evals, int nevals, const FrameRec *frames,
                                                    const ViewRec *views, const double *view_out, const int *slot_flags,
-                                                   const double *slot_logf, const int *eval_slot) {
+                                                   const double *slot_logf, const int *eval_slot,
+                                                   const double *pen_total) {
   // one workgroup per evaluation: the threads fetch 256 frames' terms at a
   // time into LDS, one thread adds them in frame order (the CPU's order)
   const int e = blockIdx.x;
@@ -1147,6 +1148,7 @@ __global__ __launch_bounds__(256) void k_objective(EvalRec *evals, int nevals, c
   E.baddiam = slot_flags[sl];
   E.baddiam_logfactor = slot_logf[sl];
   double err = chi2 / E.dof;
+  if (pen_total) err += pen_total[e];  // bestfit.c:807-811
   if (E.baddiam) err *= E.bad_objfactor * exp(E.baddiam_logfactor);
   if (E.posbnd) err *= E.bad_objfactor * exp(E.posbnd_logfactor);
   if (E.badradar) err *= E.bad_objfactor * exp(E.badradar_logfactor);

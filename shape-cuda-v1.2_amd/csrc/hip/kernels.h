@@ -62,6 +62,38 @@ constexpr int OUT_OUTBND = 0, OUT_POSBND_LOG = 2, OUT_DELLIM = 4, OUT_DOPLIM = 6
               OUT_BADRADAR = 12, OUT_BADRADAR_LOG = 13, OUT_IDELLIM = 14, OUT_IDOPLIM = 16,
               OUT_KM2HZ = 18, OUT_YSUM = 19, VIEW_OUT = 20;
 
+// ---- moments and penalties (k_pen.hip) ----
+// Per geometry slot, MOM_N doubles: the model's moments (realize_mod.c:757-840)
+// and what the penalties read from its geometry (penalties.c:597-1208).
+constexpr int MOM_AREA = 0, MOM_VOL = 1, MOM_COM = 2, MOM_I = 5, MOM_NONSMOOTH = 14, MOM_CONCAV = 15,
+              MOM_LO = 16, MOM_HI = 19, MOM_PM = 22, MOM_AP = 25, MOM_SF = 34, MOM_RDEV = 37, MOM_MAXRDEV = 38,
+              MOM_MAXELL = 39, MOM_BIF = 40, MOM_JACOBI_FAIL = 41, MOM_N = 48;
+// partial sums per (slot, workgroup): facets and sides / vertices
+constexpr int MP1_N = 19, MP2_N = 3 + 2 * 19;
+// what the vertex pass computes (flags)
+constexpr int PEN_NEED_SIDES = 1, PEN_NEED_VERTS = 2, PEN_NEED_PA = 4;
+
+// The penalty list of an objective (par->pen), shared by every evaluation.
+struct PenDev {
+  int n;                 // 0: no penalties
+  const int *type;       // n
+  const double *weight;  // n
+  double *base;          // nevals x n: par->pen.base of each evaluation
+  double *total;         // nevals: the penalty sum added to chi2/dof
+  const double *mom;     // nslots x MOM_N
+};
+
+// which penalty types the kernels evaluate, and what each needs
+int pen_supported(int type);
+int pen_needs(int type);
+
+__global__ void k_moments_part(const GeomSlot *slots, double *part, int nb);
+__global__ void k_moments_fin(const GeomSlot *slots, const double *params, const double *part, int nb, double *mom,
+                              const double *given, int need_pa);
+__global__ void k_pen_vert_part(const GeomSlot *slots, const double *mom, double *part, int nb, int need);
+__global__ void k_pen_vert_fin(const GeomSlot *slots, const double *part, int nb, double *mom, int need);
+__global__ void k_pen_eval(const EvalRec *evals, int nevals, const int *eval_slot, const double *params, PenDev pen);
+
 // ---- FP64 kernels (k_fp64.hip) ----
 __global__ void k_setup_views(ViewRec *views, int nviews, const double *params);
 __global__ void k_pos_setup(ViewRec *views);
@@ -88,7 +120,7 @@ __global__ void k_chi2_frames64(FrameRec *frames, int nframes, double *fit, cons
                                 const double *view_out, int clear_fit, int ovf_from_host, int lc_only);
 __global__ void k_objective(EvalRec *evals, int nevals, const FrameRec *frames, const ViewRec *views,
                             const double *view_out, const int *slot_flags, const double *slot_logf,
-                            const int *eval_slot);
+                            const int *eval_slot, const double *pen_total);
 
 }  // namespace shp
 

@@ -422,6 +422,62 @@ static void add_param(shp_scene *sc, double *p, int type)
   if (sc->nparams < 32) { sc->params[sc->nparams] = p; sc->ptype[sc->nparams] = type; sc->nparams++; }
 }
 
+/* The side (edge) list of a closed triangle mesh in read_mod.c's setupsides
+   order (:2386-2501): sides numbered by first appearance over the facets in
+   order, each facet's corners taken sorted (a < b < c) as (a,b), (a,c),
+   (b,c); s.v ascending; s.f[0] the lower-numbered facet, s.f[1] the other;
+   each facet's f.s[] in ascending side number.  A hash of the vertex pair
+   replaces the reference's linear scan of the list. */
+typedef struct { long long key; int s; } side_ent;
+
+static int side_find(side_ent *tab, size_t mask, long long key, int add, int *ns)
+{
+  size_t h = (size_t)((unsigned long long)key * 0x9E3779B97F4A7C15ULL >> 20) & mask;
+  while (tab[h].s >= 0 && tab[h].key != key) h = (h + 1) & mask;
+  if (tab[h].s < 0) {
+    if (!add) return -1;
+    tab[h].key = key;
+    tab[h].s = (*ns)++;
+  }
+  return tab[h].s;
+}
+
+static void build_sides(shp_scene *sc, struct vertices_t *vt)
+{
+  const int nf = vt->nf;
+  vt->ns = (3 * nf) / 2;
+  vt->s = (struct side_t *)tcalloc(sc, vt->ns, sizeof(struct side_t));
+  size_t cap = 1;
+  while (cap < 4 * (size_t)vt->ns) cap <<= 1;
+  side_ent *tab = (side_ent *)malloc(cap * sizeof(side_ent));
+  for (size_t i = 0; i < cap; i++) tab[i].s = -1;
+  int ns = 0;
+  for (int s = 0; s < vt->ns; s++) vt->s[s].f[0] = vt->s[s].f[1] = -1;
+  for (int f = 0; f < nf; f++) {
+    int v[3] = {vt->f[f].v[0], vt->f[f].v[1], vt->f[f].v[2]};
+    for (int i = 0; i < 2; i++)
+      for (int j = 0; j < 2 - i; j++)
+        if (v[j] > v[j + 1]) { int t = v[j]; v[j] = v[j + 1]; v[j + 1] = t; }
+    const int pa[3] = {v[0], v[0], v[1]}, pb[3] = {v[1], v[2], v[2]};
+    int got[3];
+    for (int k = 0; k < 3; k++) {
+      const int before = ns;
+      int s = side_find(tab, cap - 1, (long long)pa[k] * vt->nv + pb[k], 1, &ns);
+      if (s >= vt->ns) abort(); /* not a closed two-manifold mesh */
+      if (ns != before) { vt->s[s].v[0] = pa[k]; vt->s[s].v[1] = pb[k]; }
+      if (vt->s[s].f[0] < 0) vt->s[s].f[0] = f;
+      else vt->s[s].f[1] = f;
+      got[k] = s;
+    }
+    for (int i = 0; i < 2; i++)
+      for (int j = 0; j < 2 - i; j++)
+        if (got[j] > got[j + 1]) { int t = got[j]; got[j] = got[j + 1]; got[j + 1] = t; }
+    for (int k = 0; k < 3; k++) vt->f[f].s[k] = got[k];
+  }
+  free(tab);
+  for (int s = 0; s < vt->ns; s++) vt->s[s].act = 1;
+}
+
 static void build_model(shp_scene *sc)
 {
   const shp_spec *sp = &sc->spec;
@@ -472,6 +528,7 @@ static void build_model(shp_scene *sc)
       struct vertex_t *v = &vt->v[m.fv[3 * f + k]];
       v->af[v->naf++] = f;
     }
+  build_sides(sc, vt);
   c->real = *vt; /* setupreal: the VERTEX realization shares the description's arrays */
   free(m.u);
   free(m.fv);
@@ -1262,4 +1319,101 @@ void shp_par_flags(struct par_t *par, double out[6])
 {
   out[0] = par->posbnd; out[1] = par->badradar; out[2] = par->baddiam;
   out[3] = par->posbnd_logfactor; out[4] = par->badradar_logfactor; out[5] = par->baddiam_logfactor;
+}
+
+/* ---- penalties and the spin quantities they read ----------------------- */
+
+/* par->pen as read_par.c leaves it: n penalties, types and weights 1-based
+   (penalties.c:145-1237 reads pen.type[i], pen.weight[i], i = 1..n, and
+   writes pen.base[i]) */
+int shp_scene_set_penalties(shp_scene *sc, int n, const int *type, const double *weight)
+{
+  struct pen_t *pen = &sc->par.pen;
+  if (n < 0) return -1;
+  pen->n = n;
+  pen->type = (int *)tcalloc(sc, (size_t)n + 1, sizeof(int));
+  pen->weight = (double *)tcalloc(sc, (size_t)n + 1, sizeof(double));
+  pen->base = (double *)tcalloc(sc, (size_t)n + 1, sizeof(double));
+  for (int i = 1; i <= n; i++) { pen->type[i] = type[i - 1]; pen->weight[i] = weight[i - 1]; }
+  return 0;
+}
+
+void shp_pen_base(struct par_t *par, double *out)
+{
+  for (int i = 1; i <= par->pen.n; i++) out[i - 1] = par->pen.base[i];
+}
+
+void shp_set_spin_inertia(struct mod_t *mod, const double *in)
+{
+  for (int k = 0; k < 3; k++) mod->spin.inertia[k].val = in[k];
+}
+
+/* n spin impulses at epochs t[k] of (imp[3k], imp[3k+1], imp[3k+2]) rad/day */
+int shp_set_impulses(struct mod_t *mod, int n, const double *t, const double *imp)
+{
+  if (n < 0 || n > MAXIMP) return -1;
+  mod->spin.n_impulse = n;
+  for (int k = 0; k < n; k++) {
+    mod->spin.t_impulse[k] = t[k];
+    for (int j = 0; j < 3; j++) { mod->spin.impulse[k][j].val = imp[3 * k + j]; mod->spin.impulse[k][j].state = 'c'; }
+  }
+  return 0;
+}
+
+/* area, volume, com[3], inertia[9] of the model (mode 0) or component c (mode 1 + c) */
+void shp_moments(struct mod_t *mod, int mode, double out[14])
+{
+  const double *area, *volume, *com, *inertia;
+  if (mode == 0) {
+    area = &mod->shape.area; volume = &mod->shape.volume; com = mod->shape.com; inertia = &mod->shape.inertia[0][0];
+  } else {
+    struct comp_t *c = &mod->shape.comp[mode - 1];
+    area = &c->area; volume = &c->volume; com = c->com; inertia = &c->inertia[0][0];
+  }
+  out[0] = *area;
+  out[1] = *volume;
+  for (int k = 0; k < 3; k++) out[2 + k] = com[k];
+  for (int k = 0; k < 9; k++) out[5 + k] = inertia[k];
+}
+
+/* facet corners of the model's first component (3 per facet) */
+void shp_facets(struct mod_t *mod, int *out)
+{
+  struct vertices_t *r = &mod->shape.comp[0].real;
+  for (int f = 0; f < r->nf; f++)
+    for (int k = 0; k < 3; k++) out[3 * f + k] = r->f[f].v[k];
+}
+
+/* rotation (Euler angles, rad) and offset (km) of the model's first component */
+void shp_set_comp_transform(struct mod_t *mod, const double *rot, const double *off)
+{
+  struct comp_t *c = &mod->shape.comp[0];
+  for (int j = 0; j < 3; j++) { c->rot[j].val = rot[j]; c->off[j].val = off[j]; }
+}
+
+/* vertex deviations r.val of the model's first component */
+void shp_set_vertex_r(struct mod_t *mod, const double *r)
+{
+  struct vertices_t *rv = &mod->shape.comp[0].real;
+  for (int i = 0; i < rv->nv; i++) rv->v[i].r.val = r[i];
+}
+
+void shp_vertex_dirs(struct mod_t *mod, double *u, double *r)
+{
+  struct vertices_t *rv = &mod->shape.comp[0].real;
+  for (int i = 0; i < rv->nv; i++) {
+    for (int k = 0; k < 3; k++) u[3 * i + k] = rv->v[i].u[k];
+    r[i] = rv->v[i].r.val;
+  }
+}
+
+/* side list of the model's first component: per side v0, v1, f0, f1 */
+int shp_sides(struct mod_t *mod, int *out, int cap)
+{
+  struct vertices_t *r = &mod->shape.comp[0].real;
+  for (int s = 0; s < r->ns && s < cap; s++) {
+    out[4 * s] = r->s[s].v[0]; out[4 * s + 1] = r->s[s].v[1];
+    out[4 * s + 2] = r->s[s].f[0]; out[4 * s + 3] = r->s[s].f[1];
+  }
+  return r->ns;
 }

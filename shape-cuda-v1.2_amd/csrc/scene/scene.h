@@ -148,6 +148,17 @@ void shp_pos_set_array(struct pos_t *pos, int which, const double *in);
 void shp_pos_set_frames(struct pos_t *pos, const double *oe, const double *se);
 int shp_set_radar_law(struct mod_t *mod, int l, int type, const double *p);
 double *shp_param_ptr(shp_scene *sc, int i);
+/* penalties (par->pen, 1-based as penalties.c reads it) and spin quantities */
+int shp_scene_set_penalties(shp_scene *sc, int n, const int *type, const double *weight);
+void shp_pen_base(struct par_t *par, double *out);
+void shp_set_spin_inertia(struct mod_t *mod, const double *in);
+int shp_set_impulses(struct mod_t *mod, int n, const double *t, const double *imp);
+void shp_moments(struct mod_t *mod, int mode, double out[14]);
+int shp_sides(struct mod_t *mod, int *out, int cap);
+void shp_facets(struct mod_t *mod, int *out);
+void shp_set_vertex_r(struct mod_t *mod, const double *r);
+void shp_set_comp_transform(struct mod_t *mod, const double *rot, const double *off);
+void shp_vertex_dirs(struct mod_t *mod, double *u, double *r);
 int shp_param_count(shp_scene *sc);
 int shp_param_type(shp_scene *sc, int i);
 /* Deep copy of the parts of a scene the forward model writes (fit images,

@@ -28,6 +28,9 @@ DELAY, DOPPLER, POS, LGHTCRV = 0, 1, 2, 3
 (COSINELAW_DIFF, GAUSSIANLAW, HAGFORSLAW, COSINELAW_QS, GAUSSIAN_COSINE, HAGFORS_COSINE,
  COSINE_COSINE) = 1, 2, 3, 4, 5, 6, 7
 SHAPE_ELLIPSOID_TESS, SHAPE_ICOSPHERE = 0, 1
+# penalty types (const.h's numbering)
+(NONSMOOTH, CONCAVITY, RDEV, VOLUME, COMDEV, INERTIADEV, INERTIADEV_UNI, PA3TILT, NONPA, NONPA_UNI,
+ EULEROFFS, FLATTENING, MAXRDEV, MAXELLIPDEV, NONCOSINE, BIFURCATION, IMPULSE) = range(9, 26)
 
 
 class Spec(C.Structure):
@@ -109,6 +112,16 @@ def load_scene():
             "shp_param_ptr": (dp, [vp, C.c_int]),
             "shp_param_count": (C.c_int, [vp]),
             "shp_param_type": (C.c_int, [vp, C.c_int]),
+            "shp_scene_set_penalties": (C.c_int, [vp, C.c_int, ip, dp]),
+            "shp_pen_base": (None, [vp, dp]),
+            "shp_set_spin_inertia": (None, [vp, dp]),
+            "shp_set_impulses": (C.c_int, [vp, C.c_int, dp, dp]),
+            "shp_moments": (None, [vp, C.c_int, dp]),
+            "shp_sides": (C.c_int, [vp, ip, C.c_int]),
+            "shp_facets": (None, [vp, ip]),
+            "shp_set_vertex_r": (None, [vp, dp]),
+            "shp_set_comp_transform": (None, [vp, dp, dp]),
+            "shp_vertex_dirs": (None, [vp, dp, dp]),
         }
         for name, (res, args) in sigs.items():
             fn = getattr(lib, name)
@@ -123,7 +136,7 @@ SEARCH_FN = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_int, C.POINTER(C.c_double), C.P
 
 _hip_lib = None
 
-HIP_EXPORTS = ["realize_mod", "realize_spin", "realize_delcor", "calc_fits", "chi2",
+HIP_EXPORTS = ["realize_mod", "realize_spin", "realize_delcor", "calc_fits", "chi2", "penalties",
                "shape_hip_last_error", "shape_hip_clear_error", "shape_hip_set_strict",
                "shape_hip_device_count", "shape_hip_realize_delcor"]
 
@@ -158,6 +171,7 @@ def load_hip():
             "realize_omegaoff": (None, [vp]),
             "shape_hip_release_cache": (None, []),
             "chi2": (C.c_double, [vp, vp, C.c_int]),
+            "penalties": (C.c_double, [vp, vp, vp]),
             "shape_chi2_noalloc": (C.c_double, [vp, vp, vp, vp, vp]),
             "shape_hip_last_error": (C.c_int, []),
             "shape_hip_clear_error": (None, []),
@@ -200,6 +214,7 @@ def load_hip():
             "shape_batch_fit": (C.c_int, [vp, C.c_int, C.c_int, C.POINTER(C.c_double)]),
             "shape_batch_facet_map": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_int)]),
             "shape_batch_frame_terms": (C.c_int, [vp, C.c_int, C.POINTER(C.c_double)]),
+            "shape_batch_penalties": (C.c_int, [vp, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
             "shape_batch_counters": (C.c_int, [vp, C.POINTER(C.c_int)]),
             "shape_batch_sizes": (C.c_int, [vp, C.POINTER(C.c_longlong)]),
             "shape_batch_destroy": (None, [vp]),
@@ -488,6 +503,72 @@ class Scene:
     def param_type(self, i):
         return self.lib.shp_param_type(self.h, i)
 
+    # ---- penalties (penalties.c) and the quantities they read ----
+    def set_penalties(self, types, weights):
+        """par->pen: penalty types (NONSMOOTH ... IMPULSE) and weights."""
+        t = np.ascontiguousarray(types, dtype=np.int32)
+        w = np.ascontiguousarray(weights, dtype=np.float64)
+        if len(t) != len(w):
+            raise ValueError("one weight per penalty")
+        self.lib.shp_scene_set_penalties(self.h, len(t), t.ctypes.data_as(C.POINTER(C.c_int)),
+                                         w.ctypes.data_as(C.POINTER(C.c_double)))
+        self.npen = len(t)
+
+    def pen_base(self):
+        """par->pen.base[1..n] as penalties() left them."""
+        n = getattr(self, "npen", 0)
+        out = np.zeros(max(n, 1))
+        self.lib.shp_pen_base(self.par, out.ctypes.data_as(C.POINTER(C.c_double)))
+        return out[:n]
+
+    def set_spin_inertia(self, moments):
+        a = np.ascontiguousarray(moments, dtype=np.float64)
+        self.lib.shp_set_spin_inertia(self.mod, a.ctypes.data_as(C.POINTER(C.c_double)))
+
+    def set_impulses(self, t, imp):
+        t = np.ascontiguousarray(t, dtype=np.float64)
+        imp = np.ascontiguousarray(imp, dtype=np.float64).reshape(-1)
+        if self.lib.shp_set_impulses(self.mod, len(t), t.ctypes.data_as(C.POINTER(C.c_double)),
+                                     imp.ctypes.data_as(C.POINTER(C.c_double))):
+            raise ValueError("at most 10 impulses")
+
+    def moments(self, comp=None):
+        """(area, volume, com[3], inertia[3x3]) of the model, or of component comp."""
+        out = np.zeros(14)
+        self.lib.shp_moments(self.mod, 0 if comp is None else 1 + comp, out.ctypes.data_as(C.POINTER(C.c_double)))
+        return out[0], out[1], out[2:5].copy(), out[5:14].reshape(3, 3).copy()
+
+    def facets(self):
+        """facet corners, (nf, 3)"""
+        out = np.zeros((self.nf, 3), dtype=np.int32)
+        self.lib.shp_facets(self.mod, out.ctypes.data_as(C.POINTER(C.c_int)))
+        return out
+
+    def vertex_dirs(self):
+        """(u (nv, 3), r (nv,)): vertex directions and deviations"""
+        u, r = np.zeros((self.nv, 3)), np.zeros(self.nv)
+        self.lib.shp_vertex_dirs(self.mod, u.ctypes.data_as(C.POINTER(C.c_double)),
+                                 r.ctypes.data_as(C.POINTER(C.c_double)))
+        return u, r
+
+    def set_comp_transform(self, rot, off):
+        """Euler angles (rad) and offset (km) of the model's component"""
+        a = np.ascontiguousarray(rot, dtype=np.float64)
+        b = np.ascontiguousarray(off, dtype=np.float64)
+        self.lib.shp_set_comp_transform(self.mod, a.ctypes.data_as(C.POINTER(C.c_double)),
+                                        b.ctypes.data_as(C.POINTER(C.c_double)))
+
+    def set_vertex_r(self, r):
+        a = np.ascontiguousarray(r, dtype=np.float64)
+        self.lib.shp_set_vertex_r(self.mod, a.ctypes.data_as(C.POINTER(C.c_double)))
+
+    def sides(self):
+        """side list of the model: rows (v0, v1, f0, f1)."""
+        ns = self.lib.shp_sides(self.mod, None, 0)
+        out = np.zeros((ns, 4), dtype=np.int32)
+        self.lib.shp_sides(self.mod, out.ctypes.data_as(C.POINTER(C.c_int)), ns)
+        return out
+
     def synthesize_obs(self, snr=20.0, lc_rel_sigma=0.01, seed=1002):
         self.lib.shp_scene_synthesize_obs(self.h, snr, lc_rel_sigma, seed)
 
@@ -696,6 +777,16 @@ class Batch:
         if rc:
             raise RuntimeError("shape_batch_frame_terms failed (%d)" % rc)
         return out[:8 * nf].reshape(nf, 8)
+
+    def penalties(self, e):
+        """(par->pen.base of evaluation e, their weighted sum) after the last replay."""
+        n = self.lib.shape_batch_penalties(self.h, e, None, None)
+        if n < 0:
+            raise RuntimeError("shape_batch_penalties failed (%d)" % n)
+        base = np.zeros(max(n, 1))
+        total = C.c_double(0.0)
+        self.lib.shape_batch_penalties(self.h, e, base.ctypes.data_as(C.POINTER(C.c_double)), C.byref(total))
+        return base[:n], total.value
 
     def time_kernels(self, on=True):
         """Per-kernel HIP-event timing of eager replays (batches without BATCH_GRAPH)."""
