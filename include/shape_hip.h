@@ -41,6 +41,20 @@ void realize_mod(struct par_t *par, struct mod_t *mod);
    tabular-law penalties) record an error and return 0. */
 double penalties(struct par_t *par, struct mod_t *mod, struct dat_t *dat);
 
+/* vary_params.c:87  the means behind bestfit's joint adjustments, from the
+   first dataset as the reference loops: *deldop_zmax (distance toward Earth
+   of the sub-radar point, frames whose delay constant floats),
+   *rad_xsec (radar cross section, absolutely calibrated frames),
+   *opt_brightness (absolutely calibrated lightcurve at its observation
+   epochs), *cos_subradarlat (frames whose Doppler scale floats), each the
+   weighted sum over dat's weight sums.  The model and spin states must be
+   realized (realize_mod, realize_spin).  Radar frames and lightcurves of
+   one view each (an error otherwise); the fit / POS scratch the reference
+   leaves in those frames is not written.  action is unused, as in the
+   single-process reference. */
+void vary_params(struct par_t *par, struct mod_t *mod, struct dat_t *dat, int action, double *deldop_zmax,
+                 double *rad_xsec, double *opt_brightness, double *cos_subradarlat);
+
 /* realize_spin.c:41  Euler matrices ae, intrinsic and total spin vectors of
    every frame view and lightcurve point (principal-axis rotators). */
 void realize_spin(struct par_t *par, struct mod_t *mod, struct dat_t *dat);
@@ -189,7 +203,9 @@ int shape_batch_frame_terms(shape_batch *b, int e, double *out);
    base[0..n) and their weighted sum (the term added to chi2/dof) into *total;
    returns n.  shape_batch_create fails when par lists a penalty type outside
    NONSMOOTH..MAXELLIPDEV, BIFURCATION, IMPULSE, or one that reads the side
-   list of a model without one. */
+   list of a model without one, and when par turns on a vary_* joint
+   adjustment (evaluate those through the drop-ins: vary_params, realize_*,
+   calc_fits, chi2, penalties). */
 int shape_batch_penalties(shape_batch *b, int e, double *base, double *total);
 /* [0] pixels replayed in FP64, [1] error flags, [2] mixed: replayed pixels
    whose candidate facets a second raster pass collected (the rest come from

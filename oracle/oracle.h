@@ -51,6 +51,11 @@ void oracle_jacobi3(double a[3][3], double d[3], double v[3][3]);
 void oracle_diag_inertia(double inertia[3][3], double pmoment[3], double ap[3][3]);
 double oracle_penalties(struct par_t *par, struct mod_t *mod, struct dat_t *dat);
 
+/* vary_params.c:87-325: the four means of the joint adjustments (first
+   dataset only, as the reference loops) */
+void oracle_vary_params(struct par_t *par, struct mod_t *mod, struct dat_t *dat, double *deldop_zmax,
+                        double *rad_xsec, double *opt_brightness, double *cos_subradarlat);
+
 /* the fit's one-parameter line search: nr/mnbrak.c:10, nr/brent_abs.c:13,
    bestfit.c:446-468 (mnbrak from (x0, x0 + step), then brent_abs) */
 typedef double (*oracle_fn)(double);

@@ -252,6 +252,146 @@ extern "C" void calc_lghtcrv(struct par_t *par, struct mod_t *mod, struct lghtcr
   one_set_fits(DK_LGHTCRV, LGHTCRV, "calc_lghtcrv", par, mod, lghtcrv, sizeof(*lghtcrv), s);
 }
 
+// ---------------------------------------------------------------------------
+// vary_params (vary_params.c:87): the means behind bestfit's joint
+// adjustments, from the first dataset (the reference's loop runs s < 1): the
+// sub-radar point's distance toward Earth and the radar cross section of each
+// frame whose delay constant floats or whose calibration is absolute, the
+// absolutely calibrated lightcurve's brightness at the observation epochs,
+// and cos(sub-radar latitude) where the Doppler scale floats.  The rendering
+// and synthesis run on the cached one-dataset engine; one workgroup per frame
+// then reduces its POS and fit.  The reference also leaves these frames' fit
+// and POS arrays behind as scratch; they are not written back here.
+// ---------------------------------------------------------------------------
+namespace {
+// per frame: [0] max z over the POS rectangle's pixels with cos(scattering) > 0
+// (-1e20 if none), [1] the fit's sum plus the overflow cross section
+__global__ __launch_bounds__(256) void k_vary_frames(const FrameRec *frames, const ViewRec *views, PosDev pd,
+                                                     const double *fit, double *out) {
+  const FrameRec &F = frames[blockIdx.x];
+  __shared__ double sz[256], sf[256];
+  double zmax = -1.0e20, sum = 0.0;
+  const long long ncell = F.type == VT_DELAY ? (long long)F.ndel * F.ndop : F.type == VT_DOPPLER ? F.ndop : F.lc_n;
+  for (long long q = threadIdx.x; q < ncell; q += 256) sum += fit[F.off + q];
+  if (F.type != VT_LGHTCRV) {
+    const ViewRec &V = views[F.first_view];
+    const int w = 2 * V.n + 1, nx = V.xlim[1] - V.xlim[0] + 1, ny = V.ylim[1] - V.ylim[0] + 1;
+    if (nx > 0 && ny > 0)
+      for (int q = threadIdx.x; q < nx * ny; q += 256) {
+        const int i = V.xlim[0] + q / ny, j = V.ylim[0] + q % ny;
+        const long long p = V.pos_off + (long long)(i + V.n) * w + (j + V.n);
+        if (pd.cose[p] > 0.0 && pd.z[p] > zmax) zmax = pd.z[p];
+      }
+  }
+  sz[threadIdx.x] = zmax;
+  sf[threadIdx.x] = sum;
+  __syncthreads();
+  for (int k = 128; k > 0; k >>= 1) {
+    if ((int)threadIdx.x < k) {
+      sz[threadIdx.x] = sz[threadIdx.x] > sz[threadIdx.x + k] ? sz[threadIdx.x] : sz[threadIdx.x + k];
+      sf[threadIdx.x] += sf[threadIdx.x + k];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    out[2 * blockIdx.x] = sz[0];
+    out[2 * blockIdx.x + 1] = sf[0] + (F.type == VT_LGHTCRV ? 0.0 : F.overflow_xsec);
+  }
+}
+
+// cos(sub-radar latitude) of a view: the target-to-Earth vector in body
+// coordinates is the third row of oe ae^T (vary_params.c:177-184)
+double cos_subradar(double ae[3][3], double oe[3][3]) {
+  double t[3];
+  for (int j = 0; j < 3; j++) {
+    t[j] = 0.0;
+    for (int k = 0; k < 3; k++) t[j] += oe[2][k] * ae[j][k];
+  }
+  return sqrt(t[0] * t[0] + t[1] * t[1]);
+}
+
+// render and synthesize dataset 0 on its one-dataset engine; per frame
+// (zmax, fit sum + overflow) into out.  Returns 0 or an error code.
+int vary_set0(int kind, int type, struct par_t *par, struct mod_t *mod, const void *desc, size_t desc_size,
+              std::vector<double> &out) {
+  OneSet one(0, type, desc, desc_size);
+  int rc = 0;
+  CacheSlot *slot = acquire(kind, par, mod, &one.dat, desc, 0, &rc);
+  if (!slot) return rc ? rc : -2;
+  Engine &E = *slot->E;
+  double *d_out = nullptr;
+  rc = hipMemsetAsync(E.d_fit, 0, sizeof(double) * std::max(1LL, E.fit_len), E.stream) == hipSuccess ? 0 : -4;
+  if (!rc) rc = E.upload_params();
+  if (!rc) rc = E.enqueue();
+  if (!rc && hipMalloc(&d_out, sizeof(double) * 2 * std::max(1, E.nframes)) != hipSuccess) rc = -2;
+  if (!rc) {
+    hipLaunchKernelGGL(k_vary_frames, dim3(E.nframes), dim3(256), 0, E.stream, E.d_frames, E.d_views, E.pd, E.d_fit,
+                       d_out);
+    rc = E.sync();
+  }
+  if (!rc && E.error_flags()) rc = -30;
+  out.assign(2 * (size_t)E.nframes, 0.0);
+  if (!rc && hipMemcpy(out.data(), d_out, sizeof(double) * out.size(), hipMemcpyDeviceToHost) != hipSuccess) rc = -4;
+  if (d_out) (void)hipFree(d_out);
+  release(slot);
+  return rc;
+}
+}  // namespace
+
+extern "C" void vary_params(struct par_t *par, struct mod_t *mod, struct dat_t *dat, int action, double *deldop_zmax,
+                            double *rad_xsec, double *opt_brightness, double *cos_subradarlat) {
+  (void)action;
+  double s_zmax = 0.0, s_xsec = 0.0, s_bright = 0.0, s_cos = 0.0;
+  int rc = 0;
+  if (dat->nsets > 0) {
+    struct set_t *set = &dat->set[0];
+    std::vector<double> fr;
+    if (set->type == DELAY) {
+      struct deldop_t *dd = &set->desc.deldop;
+      bool any = false;
+      for (int f = 0; f < dd->nframes; f++)
+        any = any || (par->vary_delcor0 != 0 && dd->delcor.a[0].state != 'c') ||
+              (par->vary_radalb != 0 && dd->frame[f].cal.state == 'c');
+      if (any && dd->nviews != 1) rc = -19;  // one view per frame (view v0 = view 0)
+      if (any && !rc) rc = vary_set0(DK_DELDOP, DELAY, par, mod, dd, sizeof(*dd), fr);
+      for (int f = 0; f < dd->nframes && !rc; f++) {
+        if (par->vary_delcor0 != 0 && dd->delcor.a[0].state != 'c') s_zmax += fr[2 * f] * dd->frame[f].weight;
+        if (par->vary_radalb != 0 && dd->frame[f].cal.state == 'c')
+          s_xsec += fr[2 * f + 1] * dd->frame[f].cal.val * dd->frame[f].weight;
+        if (par->vary_dopscale != 0 && dd->dopscale.state != 'c')
+          s_cos += cos_subradar(dd->frame[f].view[dd->v0].ae, dd->frame[f].view[dd->v0].oe) * dd->frame[f].weight;
+      }
+    } else if (set->type == DOPPLER) {
+      struct doppler_t *dp = &set->desc.doppler;
+      bool any = false;
+      for (int f = 0; f < dp->nframes; f++) any = any || (par->vary_radalb != 0 && dp->frame[f].cal.state == 'c');
+      if (any && dp->nviews != 1) rc = -19;
+      if (any && !rc) rc = vary_set0(DK_DOPPLER, DOPPLER, par, mod, dp, sizeof(*dp), fr);
+      for (int f = 0; f < dp->nframes && !rc; f++) {
+        if (par->vary_radalb != 0 && dp->frame[f].cal.state == 'c')
+          s_xsec += fr[2 * f + 1] * dp->frame[f].cal.val * dp->frame[f].weight;
+        if (par->vary_dopscale != 0 && dp->dopscale.state != 'c')
+          s_cos += cos_subradar(dp->frame[f].view[dp->v0].ae, dp->frame[f].view[dp->v0].oe) * dp->frame[f].weight;
+      }
+    } else if (set->type == LGHTCRV) {
+      struct lghtcrv_t *lc = &set->desc.lghtcrv;
+      if (par->vary_optalb != 0 && lc->cal.state == 'c') {
+        if (lc->nviews != 1) rc = -19;
+        if (!rc) rc = vary_set0(DK_LGHTCRV, LGHTCRV, par, mod, lc, sizeof(*lc), fr);
+        if (!rc) s_bright += fr[1] * lc->weight;
+      }
+    }
+  }
+  if (rc) {
+    set_err(rc, "vary_params");
+    return;
+  }
+  *deldop_zmax = dat->sum_deldop_zmax_weights > 0.0 ? s_zmax / dat->sum_deldop_zmax_weights : 0.0;
+  *rad_xsec = dat->sum_rad_xsec_weights > 0.0 ? s_xsec / dat->sum_rad_xsec_weights : 0.0;
+  *opt_brightness = dat->sum_opt_brightness_weights > 0.0 ? s_bright / dat->sum_opt_brightness_weights : 0.0;
+  *cos_subradarlat = dat->sum_cos_subradarlat_weights > 0.0 ? s_cos / dat->sum_cos_subradarlat_weights : 0.0;
+}
+
 extern "C" void shape_hip_release_cache(void) {
   std::lock_guard<std::mutex> lk(g_cache_mu);
   cache_trim(0);

@@ -914,6 +914,8 @@ static inline unsigned cdiv(long long a, long long b) { return (unsigned)((a + b
 int Engine::setup_penalties(struct par_t *par) {
   pen = PenDev{};
   pen_need = 0;
+  // the joint adjustments of vary_params are not part of the batched objective
+  if (opt.penalties && (par->vary_delcor0 || par->vary_radalb || par->vary_optalb || par->vary_dopscale)) return -19;
   if (!opt.penalties || par->pen.n <= 0 || opt.views_only) return 0;
   const int n = par->pen.n;
   std::vector<int> type(n);

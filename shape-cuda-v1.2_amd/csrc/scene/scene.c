@@ -1384,6 +1384,85 @@ void shp_facets(struct mod_t *mod, int *out)
     for (int k = 0; k < 3; k++) out[3 * f + k] = r->f[f].v[k];
 }
 
+/* ---- joint adjustments (vary_params) ------------------------------------ */
+
+/* par's vary_delcor0, vary_radalb, vary_optalb, vary_dopscale (VARY_NONE 0 ...
+   VARY_ALL 3) */
+void shp_set_vary(struct par_t *par, int delcor0, int radalb, int optalb, int dopscale)
+{
+  par->vary_delcor0 = (unsigned char)delcor0;
+  par->vary_radalb = (unsigned char)radalb;
+  par->vary_optalb = (unsigned char)optalb;
+  par->vary_dopscale = (unsigned char)dopscale;
+}
+
+/* calibration state and value of frame f of dataset s (the lightcurve's own
+   for a lightcurve set, f ignored); which = 1: the Doppler scaling factor's
+   state and value of a radar set; which = 2: the state of the delay
+   polynomial's constant term; which = 3: the frame's (lightcurve's) weight */
+void shp_set_state(struct dat_t *dat, int s, int f, int which, int state, double val)
+{
+  struct set_t *set = &dat->set[s];
+  if (which == 0) {
+    struct param_t *cal = set->type == DELAY     ? &set->desc.deldop.frame[f].cal
+                          : set->type == DOPPLER ? &set->desc.doppler.frame[f].cal
+                                                 : &set->desc.lghtcrv.cal;
+    cal->state = (char)state;
+    cal->val = val;
+  } else if (which == 1) {
+    struct param_t *d = set->type == DELAY ? &set->desc.deldop.dopscale : &set->desc.doppler.dopscale;
+    d->state = (char)state;
+    d->val = val;
+    if (set->type == DELAY) set->desc.deldop.dopscale_save = val;
+    else set->desc.doppler.dopscale_save = val;
+  } else if (which == 2) {
+    struct delcor_t *dc = set->type == DELAY ? &set->desc.deldop.delcor : &set->desc.doppler.delcor;
+    dc->a[0].state = (char)state;
+  } else {
+    double *w = set->type == DELAY     ? &set->desc.deldop.frame[f].weight
+                : set->type == DOPPLER ? &set->desc.doppler.frame[f].weight
+                                       : &set->desc.lghtcrv.weight;
+    *w = val;
+  }
+}
+
+/* the weight sums read_dat.c forms while reading (:421-501, :686-692 and the
+   Doppler and lightcurve readers): frames whose delay constant floats,
+   absolutely calibrated frames and lightcurves, floating Doppler scales */
+void shp_dat_vary_weights(struct dat_t *dat)
+{
+  dat->sum_deldop_zmax_weights = dat->sum_rad_xsec_weights = 0.0;
+  dat->sum_opt_brightness_weights = dat->sum_cos_subradarlat_weights = 0.0;
+  for (int s = 0; s < dat->nsets; s++) {
+    struct set_t *set = &dat->set[s];
+    if (set->type == DELAY) {
+      struct deldop_t *dd = &set->desc.deldop;
+      dd->sum_deldop_zmax_weights = dd->sum_rad_xsec_weights = dd->sum_cos_subradarlat_weights = 0.0;
+      for (int f = 0; f < dd->nframes; f++) {
+        if (dd->delcor.a[0].state != 'c') dd->sum_deldop_zmax_weights += dd->frame[f].weight;
+        if (dd->frame[f].cal.state == 'c') dd->sum_rad_xsec_weights += dd->frame[f].weight;
+        if (dd->dopscale.state != 'c') dd->sum_cos_subradarlat_weights += dd->frame[f].weight;
+      }
+      dat->sum_deldop_zmax_weights += dd->sum_deldop_zmax_weights;
+      dat->sum_rad_xsec_weights += dd->sum_rad_xsec_weights;
+      dat->sum_cos_subradarlat_weights += dd->sum_cos_subradarlat_weights;
+    } else if (set->type == DOPPLER) {
+      struct doppler_t *dp = &set->desc.doppler;
+      dp->sum_rad_xsec_weights = dp->sum_cos_subradarlat_weights = 0.0;
+      for (int f = 0; f < dp->nframes; f++) {
+        if (dp->frame[f].cal.state == 'c') dp->sum_rad_xsec_weights += dp->frame[f].weight;
+        if (dp->dopscale.state != 'c') dp->sum_cos_subradarlat_weights += dp->frame[f].weight;
+      }
+      dat->sum_rad_xsec_weights += dp->sum_rad_xsec_weights;
+      dat->sum_cos_subradarlat_weights += dp->sum_cos_subradarlat_weights;
+    } else if (set->type == LGHTCRV) {
+      struct lghtcrv_t *lc = &set->desc.lghtcrv;
+      lc->sum_opt_brightness_weights = lc->cal.state == 'c' ? lc->weight * lc->n : 0.0;
+      dat->sum_opt_brightness_weights += lc->sum_opt_brightness_weights;
+    }
+  }
+}
+
 /* rotation (Euler angles, rad) and offset (km) of the model's first component */
 void shp_set_comp_transform(struct mod_t *mod, const double *rot, const double *off)
 {

@@ -158,6 +158,11 @@ int shp_sides(struct mod_t *mod, int *out, int cap);
 void shp_facets(struct mod_t *mod, int *out);
 void shp_set_vertex_r(struct mod_t *mod, const double *r);
 void shp_set_comp_transform(struct mod_t *mod, const double *rot, const double *off);
+/* vary_params: par's vary flags, calibration / Doppler-scale / delay-constant
+   states, and the weight sums read_dat forms */
+void shp_set_vary(struct par_t *par, int delcor0, int radalb, int optalb, int dopscale);
+void shp_set_state(struct dat_t *dat, int s, int f, int which, int state, double val);
+void shp_dat_vary_weights(struct dat_t *dat);
 void shp_vertex_dirs(struct mod_t *mod, double *u, double *r);
 int shp_param_count(shp_scene *sc);
 int shp_param_type(shp_scene *sc, int i);

@@ -121,6 +121,10 @@ def load_scene():
             "shp_facets": (None, [vp, ip]),
             "shp_set_vertex_r": (None, [vp, dp]),
             "shp_set_comp_transform": (None, [vp, dp, dp]),
+            "shp_set_vary": (None, [vp, C.c_int, C.c_int, C.c_int, C.c_int]),
+            "shp_radar_law": (None, [vp, vp, C.c_int, dp]),
+            "shp_set_state": (None, [vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double]),
+            "shp_dat_vary_weights": (None, [vp]),
             "shp_vertex_dirs": (None, [vp, dp, dp]),
         }
         for name, (res, args) in sigs.items():
@@ -136,7 +140,7 @@ SEARCH_FN = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_int, C.POINTER(C.c_double), C.P
 
 _hip_lib = None
 
-HIP_EXPORTS = ["realize_mod", "realize_spin", "realize_delcor", "calc_fits", "chi2", "penalties",
+HIP_EXPORTS = ["realize_mod", "realize_spin", "realize_delcor", "calc_fits", "chi2", "penalties", "vary_params",
                "shape_hip_last_error", "shape_hip_clear_error", "shape_hip_set_strict",
                "shape_hip_device_count", "shape_hip_realize_delcor"]
 
@@ -172,6 +176,8 @@ def load_hip():
             "shape_hip_release_cache": (None, []),
             "chi2": (C.c_double, [vp, vp, C.c_int]),
             "penalties": (C.c_double, [vp, vp, vp]),
+            "vary_params": (None, [vp, vp, vp, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double),
+                                   C.POINTER(C.c_double), C.POINTER(C.c_double)]),
             "shape_chi2_noalloc": (C.c_double, [vp, vp, vp, vp, vp]),
             "shape_hip_last_error": (C.c_int, []),
             "shape_hip_clear_error": (None, []),
@@ -550,6 +556,29 @@ class Scene:
         self.lib.shp_vertex_dirs(self.mod, u.ctypes.data_as(C.POINTER(C.c_double)),
                                  r.ctypes.data_as(C.POINTER(C.c_double)))
         return u, r
+
+    # ---- joint adjustments (vary_params.c) ----
+    def set_vary(self, delcor0=0, radalb=0, optalb=0, dopscale=0):
+        """par's vary_* flags (VARY_NONE 0, VARY_SIZE 1, VARY_SPIN 2, VARY_ALL 3)"""
+        self.lib.shp_set_vary(self.par, delcor0, radalb, optalb, dopscale)
+
+    def set_cal(self, s, f, state, val):
+        """calibration of frame f of set s (a lightcurve's own when set s is one)"""
+        self.lib.shp_set_state(self.dat, s, f, 0, ord(state), float(val))
+
+    def set_dopscale(self, s, state, val):
+        self.lib.shp_set_state(self.dat, s, 0, 1, ord(state), float(val))
+
+    def set_delcor0_state(self, s, state):
+        self.lib.shp_set_state(self.dat, s, 0, 2, ord(state), 0.0)
+
+    def set_weight(self, s, f, w):
+        """relative weight of frame f of set s (a lightcurve's own when set s is one)"""
+        self.lib.shp_set_state(self.dat, s, f, 3, 0, float(w))
+
+    def vary_weights(self):
+        """recompute dat's vary weight sums as read_dat does"""
+        self.lib.shp_dat_vary_weights(self.dat)
 
     def set_comp_transform(self, rot, off):
         """Euler angles (rad) and offset (km) of the model's component"""
