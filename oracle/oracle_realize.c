@@ -7,7 +7,12 @@
 
 #include "oracle_internal.h"
 
-#define O_SMALLRATIO 0.01 /* realize_mod.c:138 */
+#define O_SMALLRATIO 0.01 /* realize_mod.c:137-145 */
+#define O_HAIRWIDTH 1.0e-7
+#define O_SMALLOVOIDK1 0.01
+#define O_SMALLOVOIDK2 1.0e-6
+#define O_OVOIDTOL 1.0e-6
+#define O_SMALLCOEFF3 1.0e-5
 #define O_HALFSECOND 5.8e-6
 
 static char g_err[256];
@@ -29,7 +34,214 @@ static double o_facnrm(struct vertices_t *vt, int fi)
   return area;
 }
 
-/* realize_mod.c:153-177 -> realize_coordinates :183, VERTEX branch :444-467,
+/* scale factors of a VERTEX or HARMONIC component: '=' ties to the previous
+   factor, and factors at or below SMALLRATIO made positive with the baddiam
+   flag and log factor (realize_mod.c:425-441, :455-466) */
+static void o_scale_factors(struct par_t *par, struct param_t sf[3], struct vertices_t *real)
+{
+  for (int j = 0; j <= 2; j++) {
+    if (j > 0 && sf[j].state == '=') sf[j].val = sf[j - 1].val;
+    real->scalefactor[j].val = sf[j].val;
+    if (real->scalefactor[j].val <= O_SMALLRATIO) {
+      par->baddiam = 1;
+      par->baddiam_logfactor += log(1 + O_SMALLRATIO - real->scalefactor[j].val);
+      real->scalefactor[j].val = O_SMALLRATIO / (1 + O_SMALLRATIO - real->scalefactor[j].val);
+    }
+  }
+}
+
+/* ELLIPSE (realize_mod.c:217-284): inverse squared radii from the diameter and
+   the two axis ratios, each kept positive (HAIRWIDTH / SMALLRATIO) with the
+   baddiam flag; each vertex's distance along its direction cosines */
+static void o_realize_ellipse(struct par_t *par, struct comp_t *comp)
+{
+  struct vertices_t *real = &comp->real;
+  double a[3], diam, ratio;
+  diam = comp->desc.ell.two_a.val;
+  if (diam > O_HAIRWIDTH) {
+    a[0] = 2.0 / diam;
+  } else {
+    a[0] = (2.0 / O_HAIRWIDTH) * (1 + O_HAIRWIDTH - diam);
+    par->baddiam = 1;
+    par->baddiam_logfactor += log(1 + O_HAIRWIDTH - diam);
+  }
+  diam = (2.0 / a[0]);
+  ratio = comp->desc.ell.a_over_b.val;
+  if (ratio > O_SMALLRATIO) {
+    a[1] = 2.0 / (diam / ratio);
+  } else {
+    a[1] = (2.0 / (diam / O_SMALLRATIO)) / (1 + O_SMALLRATIO - ratio);
+    par->baddiam = 1;
+    par->baddiam_logfactor += log(1 + O_SMALLRATIO - ratio);
+  }
+  diam = (2.0 / a[1]);
+  ratio = comp->desc.ell.b_over_c.val;
+  if (ratio > O_SMALLRATIO) {
+    a[2] = 2.0 / (diam / ratio);
+  } else {
+    a[2] = (2.0 / (diam / O_SMALLRATIO)) / (1 + O_SMALLRATIO - ratio);
+    par->baddiam = 1;
+    par->baddiam_logfactor += log(1 + O_SMALLRATIO - ratio);
+  }
+  for (int i = 0; i <= 2; i++) a[i] *= a[i];
+  for (int i = 0; i < real->nv; i++) {
+    double den = 0.0;
+    for (int j = 0; j <= 2; j++) den += a[j] * (real->v[i].u[j] * real->v[i].u[j]);
+    real->v[i].r.val = 1 / sqrt(den);
+  }
+  for (int j = 0; j <= 2; j++) real->scalefactor[j].val = 1.0;
+}
+
+/* Real roots of coeff[3] x^3 + coeff[2] x^2 + coeff[1] x + coeff[0]
+   (realize_mod.c:852-941): the quadratic / linear cases below a small cubic
+   coefficient, otherwise the trigonometric (three roots) or Cardano (one root)
+   solution of Numerical Recipes in C, 2nd ed., section 5.6, with r^2 - q^3
+   written out in the coefficients.  Returns the number of roots, or -1 where
+   the reference bails out. */
+static int o_cubic_roots(const double coeff[4], double root[3])
+{
+  root[0] = root[1] = root[2] = -HUGENUMBER;
+  if (fabs(coeff[3]) < O_SMALLCOEFF3) {
+    const double a = coeff[2], b = coeff[1], c = coeff[0];
+    if (fabs(a) < SMALLVAL) {
+      if (fabs(b) < SMALLVAL) return -1;
+      root[0] = -c / b;
+      return 1;
+    }
+    const double disc = b * b - 4 * a * c;
+    if (disc < 0.0) return -1;
+    if (fabs(b) < SMALLVAL) {
+      root[0] = sqrt(disc) / (2 * a);
+      root[1] = -root[0];
+    } else {
+      const int bsign = (b < 0.0) ? -1 : 1;
+      const double q = -0.5 * (b + bsign * sqrt(disc));
+      root[0] = q / a;
+      root[1] = c / q;
+    }
+    return 2;
+  }
+  const double a = coeff[2] / coeff[3], b = coeff[1] / coeff[3], c = coeff[0] / coeff[3];
+  const double q = (a * a - 3 * b) / 9;
+  const double r = (2 * a * a * a - 9 * a * b + 27 * c) / 54;
+  const double d = (4 * a * a * a * c - a * a * b * b - 18 * a * b * c + 27 * c * c + 4 * b * b * b) / 108;
+  if (d >= 0.0) {
+    const double rsign = (r < 0.0) ? -1 : 1;
+    const double s = -rsign * pow(fabs(r) + sqrt(d), 1.0 / 3);
+    const double t = (fabs(s) >= SMALLVAL) ? q / s : 0.0;
+    root[0] = s + t - a / 3;
+    return 1;
+  }
+  const double qs = sqrt(q), theta = acos(r / (q * qs));
+  root[0] = -2 * qs * cos(theta / 3) - a / 3;
+  root[1] = -2 * qs * cos((theta + 2 * PIE) / 3) - a / 3;
+  root[2] = -2 * qs * cos((theta - 2 * PIE) / 3) - a / 3;
+  return 3;
+}
+
+/* OVOID (realize_mod.c:285-394): parameters kept in bounds with the baddiam
+   flag; x0 puts the centre of mass at the origin (series for small |k|);
+   each vertex's r / a is the largest admissible non-negative root of the
+   ovoid's cubic along its direction, or for tiny |k| the ellipsoid's value
+   with one Newton step.  Returns -1 where the reference bails out. */
+static int o_realize_ovoid(struct par_t *par, struct comp_t *comp)
+{
+  struct vertices_t *real = &comp->real;
+  double ar = comp->desc.ovoid.two_a.val / 2;
+  if (ar <= O_HAIRWIDTH / 2) {
+    par->baddiam = 1;
+    par->baddiam_logfactor += log(1 + O_HAIRWIDTH - 2 * ar);
+    ar = (O_HAIRWIDTH / 2) / (1 + O_HAIRWIDTH - 2 * ar);
+  }
+  double ab = comp->desc.ovoid.a_over_b.val;
+  if (ab <= O_SMALLRATIO) {
+    par->baddiam = 1;
+    par->baddiam_logfactor += log(1 + O_SMALLRATIO - ab);
+    ab = O_SMALLRATIO / (1 + O_SMALLRATIO - ab);
+  }
+  double bc = comp->desc.ovoid.b_over_c.val;
+  if (bc <= O_SMALLRATIO) {
+    par->baddiam = 1;
+    par->baddiam_logfactor += log(1 + O_SMALLRATIO - bc);
+    bc = O_SMALLRATIO / (1 + O_SMALLRATIO - bc);
+  }
+  double k = comp->desc.ovoid.k.val;
+  if (fabs(k) > 1 - SMALLVAL) {
+    par->baddiam = 1;
+    par->baddiam_logfactor += log(fabs(k) + SMALLVAL);
+    k = k > 0.0 ? 1 - SMALLVAL * (1 - SMALLVAL) / k : -1 - SMALLVAL * (1 - SMALLVAL) / k;
+  }
+  double x0;
+  if (fabs(k) > O_SMALLOVOIDK1) {
+    const double x0term = 3 * (1 - k * k) * log((1 + k) / (1 - k));
+    const double numer = 2.0 * k * (3 - 2 * k * k) - x0term;
+    const double denom = 2.0 * k * (3 - k * k) - x0term;
+    x0 = (ar / k) * (numer / denom);
+  } else {
+    x0 = 0.4 * k * ar;
+  }
+  for (int i = 0; i < real->nv; i++) {
+    const double ac = ab * bc;
+    const double h = ab * ab * real->v[i].u[1] * real->v[i].u[1] + ac * ac * real->v[i].u[2] * real->v[i].u[2];
+    const double alpha0 = x0 / ar, ux = real->v[i].u[0];
+    double coeff[4], root[3], good;
+    coeff[3] = (h - ux * ux) * k * ux;
+    coeff[2] = (1 + 3 * k * alpha0) * ux * ux + h * (1 - k * alpha0);
+    coeff[1] = (k - (2 + 3 * k * alpha0) * alpha0) * ux;
+    coeff[0] = -(1 - alpha0 * alpha0) * (1 + k * alpha0);
+    if (fabs(k) <= O_SMALLOVOIDK2) {
+      good = 1 / sqrt(ux * ux + h);
+      good -= (coeff[3] * good * good * good + coeff[1] * good) /
+              (3 * coeff[3] * good * good + 2 * coeff[2] * good + coeff[1]);
+    } else {
+      const int nr = o_cubic_roots(coeff, root);
+      if (nr < 0) {
+        oracle_set_error("oracle_realize_mod: ovoid cubic has no admissible form");
+        return -1;
+      }
+      good = -HUGENUMBER;
+      for (int q = 0; q < nr; q++)
+        if (root[q] >= 0.0 && fabs(root[q] * ux - alpha0) - 1 < O_OVOIDTOL) good = O_MAX(good, root[q]);
+    }
+    if (good < 0.0) {
+      oracle_set_error("oracle_realize_mod: no ovoid vertex displacement");
+      return -1;
+    }
+    real->v[i].r.val = good * ar;
+  }
+  for (int j = 0; j <= 2; j++) real->scalefactor[j].val = 1.0;
+  return 0;
+}
+
+/* HARMONIC (realize_mod.c:395-442): each vertex's distance is the
+   coefficients times the vertex's spherical-harmonic factors (afactor,
+   bfactor, set up with the vertex directions), kept above HAIRWIDTH / 2 with
+   the baddiam flag; then the scale factors */
+static void o_realize_harmonic(struct par_t *par, struct comp_t *comp)
+{
+  struct vertices_t *real = &comp->real;
+  const int L = comp->desc.har.nhar;
+  for (int i = 0; i < real->nv; i++) {
+    double r = 0.0;
+    for (int l = 0; l <= L; l++) {
+      r += comp->desc.har.a[l][0].val * real->v[i].afactor[l][0];
+      for (int m = 1; m <= l; m++)
+        r += comp->desc.har.a[l][m].val * real->v[i].afactor[l][m] +
+             comp->desc.har.b[l][m].val * real->v[i].bfactor[l][m];
+    }
+    if (r > O_HAIRWIDTH / 2) {
+      real->v[i].r.val = r;
+    } else {
+      par->baddiam = 1;
+      par->baddiam_logfactor += log(1 + O_HAIRWIDTH - 2 * r) / ((L + 1) * (L + 1));
+      real->v[i].r.val = (O_HAIRWIDTH / 2) / (1 + O_HAIRWIDTH - 2 * r);
+    }
+  }
+  o_scale_factors(par, comp->desc.har.scalefactor, real);
+}
+
+/* realize_mod.c:153-177 -> realize_coordinates :183, the ELLIPSE / OVOID /
+   HARMONIC branches :217-442 (above), VERTEX branch :444-467,
    coordinates :475-479, component rotation/offset :484-512, facets :532-546,
    vertex normals :552-563, check_surface for one component (:580-593: every
    vertex, facet and side active), compute_moments :757 (oracle_pen.c). */
@@ -41,21 +253,19 @@ void oracle_realize_mod(struct par_t *par, struct mod_t *mod)
     struct comp_t *comp = &mod->shape.comp[c];
     struct vertices_t *real = &comp->real;
     int nv = real->nv, nf = real->nf;
-    if (comp->type != VERTEX) {
-      oracle_set_error("oracle_realize_mod: only VERTEX components are restated");
+    if (comp->type == ELLIPSE) {
+      o_realize_ellipse(par, comp);
+    } else if (comp->type == OVOID) {
+      if (o_realize_ovoid(par, comp)) return;
+    } else if (comp->type == HARMONIC) {
+      o_realize_harmonic(par, comp);
+    } else if (comp->type == VERTEX) {
+      for (int i = 0; i < nv; i++)
+        if (real->v[i].r.state == '=') real->v[i].r.val = real->v[real->v[i].v_mirror].r.val;
+      o_scale_factors(par, comp->desc.ver.scalefactor, real);
+    } else {
+      oracle_set_error("oracle_realize_mod: unknown component type");
       return;
-    }
-    for (int i = 0; i < nv; i++)
-      if (real->v[i].r.state == '=') real->v[i].r.val = real->v[real->v[i].v_mirror].r.val;
-    for (int j = 0; j <= 2; j++) {
-      if (j > 0 && comp->desc.ver.scalefactor[j].state == '=')
-        comp->desc.ver.scalefactor[j].val = comp->desc.ver.scalefactor[j - 1].val;
-      real->scalefactor[j].val = comp->desc.ver.scalefactor[j].val;
-      if (real->scalefactor[j].val <= O_SMALLRATIO) {
-        par->baddiam = 1;
-        par->baddiam_logfactor += log(1 + O_SMALLRATIO - real->scalefactor[j].val);
-        real->scalefactor[j].val = O_SMALLRATIO / (1 + O_SMALLRATIO - real->scalefactor[j].val);
-      }
     }
     for (int i = 0; i < nv; i++)
       for (int j = 0; j <= 2; j++)

@@ -683,7 +683,7 @@ namespace {
 struct RealizeBufs {
   const void *mod = nullptr;
   unsigned long long topo = 0;
-  GrowBuf fv, off, idx, u, a, r, pc, x, vn, fn, fx, ar, x32, vn32, fn32, slot, flag, mom, part;
+  GrowBuf fv, off, idx, u, a, r, pc, x, vn, fn, fx, ar, x32, vn32, fn32, slot, flag, mom, part, hb;
 };
 // workgroups per model for the moment reductions (as the engine's)
 int moment_blocks(int nf) { return std::max(1, std::min(64, (nf + 2047) / 2048)); }
@@ -692,35 +692,55 @@ RealizeBufs g_rb;
 }  // namespace
 
 extern "C" void realize_mod(struct par_t *par, struct mod_t *mod) {
-  if (mod->shape.ncomp != 1 || mod->shape.comp[0].type != VERTEX) { set_err(-10, "realize_mod"); return; }
+  if (mod->shape.ncomp != 1) { set_err(-10, "realize_mod"); return; }
   struct comp_t *comp = &mod->shape.comp[0];
+  const int ctype = comp->type;
+  if (ctype != VERTEX && ctype != ELLIPSE && ctype != OVOID && ctype != HARMONIC) { set_err(-10, "realize_mod"); return; }
   struct vertices_t *rv = &comp->real;
   const int nv = rv->nv, nf = rv->nf;
-  std::vector<double> in(7 * (size_t)nv + 9);  // u, a (3 nv each), r (nv), scale/rot/off (9)
+  const int nhar = ctype == HARMONIC ? comp->desc.har.nhar : 0;
+  const int nbasis = ctype == HARMONIC ? (nhar + 1) * (nhar + 1) : 0;
+  const int npc = PB_COMP_STRIDE + nbasis;
+  std::vector<double> in(7 * (size_t)nv + npc);  // u, a (3 nv each), r (nv), the component block
   double *u = in.data(), *a = u + 3 * (size_t)nv, *r = a + 3 * (size_t)nv, *pc = r + nv;
   for (int i = 0; i < nv; i++) {
     struct vertex_t *v = &rv->v[i];
-    if (v->r.state == '=') v->r.val = rv->v[v->v_mirror].r.val;  // realize_mod.c:450-454
+    if (ctype == VERTEX && v->r.state == '=') v->r.val = rv->v[v->v_mirror].r.val;  // realize_mod.c:450-454
     for (int k = 0; k < 3; k++) { u[3 * i + k] = v->u[k]; a[3 * i + k] = v->a[k]; }
     r[i] = v->r.val;
   }
+  // the component block as the engine lays it out; '=' ties of scale factors
+  // are applied to the description here as realize_mod.c:455-457 / :425-428 do
+  int sf_tie = 0;
+  struct param_t *shp3 = ctype == ELLIPSE    ? &comp->desc.ell.two_a
+                         : ctype == OVOID    ? &comp->desc.ovoid.two_a
+                         : ctype == HARMONIC ? comp->desc.har.scalefactor
+                                             : comp->desc.ver.scalefactor;
   for (int j = 0; j < 3; j++) {
-    if (j > 0 && comp->desc.ver.scalefactor[j].state == '=')
-      comp->desc.ver.scalefactor[j].val = comp->desc.ver.scalefactor[j - 1].val;
-    pc[j] = comp->desc.ver.scalefactor[j].val;
+    if ((ctype == VERTEX || ctype == HARMONIC) && j > 0 && shp3[j].state == '=') {
+      shp3[j].val = shp3[j - 1].val;
+      sf_tie |= 1 << j;
+    }
+    pc[j] = shp3[j].val;
     pc[3 + j] = comp->rot[j].val;
     pc[6 + j] = comp->off[j].val;
+  }
+  pc[PB_COMP_OVOID_K] = ctype == OVOID ? comp->desc.ovoid.k.val : 0.0;
+  for (int l = 0, q = PB_COMP_HAR; l <= nhar && ctype == HARMONIC; l++) {
+    pc[q++] = comp->desc.har.a[l][0].val;
+    for (int m = 1; m <= l; m++) { pc[q++] = comp->desc.har.a[l][m].val; pc[q++] = comp->desc.har.b[l][m].val; }
   }
   const unsigned long long topo = topo_hash(mod);
   std::lock_guard<std::mutex> lk(g_realize_mu);
   RealizeBufs &B = g_rb;
   const bool new_topo = B.mod != mod || B.topo != topo;
   int rc = B.u.get(sizeof(double) * 3 * nv) | B.a.get(sizeof(double) * 3 * nv) | B.r.get(sizeof(double) * nv) |
-           B.pc.get(sizeof(double) * 9) | B.x.get(sizeof(double) * 3 * nv) | B.vn.get(sizeof(double) * 3 * nv) |
+           B.pc.get(sizeof(double) * npc) | B.x.get(sizeof(double) * 3 * nv) | B.vn.get(sizeof(double) * 3 * nv) |
            B.fn.get(sizeof(double) * 3 * nf) | B.fx.get(sizeof(double) * 3 * nf) | B.ar.get(sizeof(double) * nf) |
            B.x32.get(sizeof(float4) * nv) | B.vn32.get(sizeof(float4) * nv) | B.fn32.get(sizeof(float4) * nf) |
            B.slot.get(sizeof(GeomSlot)) | B.flag.get(2 * sizeof(double)) |  // baddiam flag, log factor at +8
-           B.mom.get(sizeof(double) * MOM_N) | B.part.get(sizeof(double) * MP1_N * moment_blocks(nf));
+           B.mom.get(sizeof(double) * MOM_N) | B.part.get(sizeof(double) * MP1_N * moment_blocks(nf)) |
+           B.hb.get(sizeof(double) * std::max((size_t)1, (size_t)nv * nbasis));
   if (rc) { B.mod = nullptr; set_err(rc, "realize_mod allocation"); return; }
   bool ok = true;
   if (new_topo) {
@@ -731,44 +751,47 @@ extern "C" void realize_mod(struct par_t *par, struct mod_t *mod) {
       af_off[i + 1] = af_off[i] + rv->v[i].naf;
       for (int q = 0; q < rv->v[i].naf; q++) af_idx.push_back(rv->v[i].af[q]);
     }
+    std::vector<double> hb((size_t)nv * nbasis);
+    for (int i = 0; i < nv && nbasis; i++) {
+      double *h = &hb[(size_t)i * nbasis];
+      for (int l = 0, q = 0; l <= nhar; l++) {
+        h[q++] = rv->v[i].afactor[l][0];
+        for (int m = 1; m <= l; m++) { h[q++] = rv->v[i].afactor[l][m]; h[q++] = rv->v[i].bfactor[l][m]; }
+      }
+    }
     B.mod = nullptr;
     ok = B.fv.get(sizeof(int) * fv.size()) == 0 && B.off.get(sizeof(int) * af_off.size()) == 0 &&
          B.idx.get(sizeof(int) * af_idx.size()) == 0 &&
          hipMemcpy(B.fv.p, fv.data(), sizeof(int) * fv.size(), hipMemcpyHostToDevice) == hipSuccess &&
          hipMemcpy(B.off.p, af_off.data(), sizeof(int) * af_off.size(), hipMemcpyHostToDevice) == hipSuccess &&
          (af_idx.empty() ||
-          hipMemcpy(B.idx.p, af_idx.data(), sizeof(int) * af_idx.size(), hipMemcpyHostToDevice) == hipSuccess);
-    GeomSlot G;
-    memset(&G, 0, sizeof(G));
-    G.nv = nv; G.nf = nf; G.comp_pb = 0; G.eval = -1;
-    G.fv = (int *)B.fv.p; G.af_off = (int *)B.off.p; G.af_idx = (int *)B.idx.p; G.u = (double *)B.u.p;
-    G.a = (double *)B.a.p; G.r = (double *)B.r.p; G.x = (double *)B.x.p; G.vn = (double *)B.vn.p;
-    G.fn = (double *)B.fn.p; G.fx = (double *)B.fx.p; G.area = (double *)B.ar.p; G.x32 = (float4 *)B.x32.p;
-    G.vn32 = (float4 *)B.vn32.p; G.fn32 = (float4 *)B.fn32.p;
-    ok = ok && hipMemcpy(B.slot.p, &G, sizeof(G), hipMemcpyHostToDevice) == hipSuccess;
+          hipMemcpy(B.idx.p, af_idx.data(), sizeof(int) * af_idx.size(), hipMemcpyHostToDevice) == hipSuccess) &&
+         (hb.empty() || hipMemcpy(B.hb.p, hb.data(), sizeof(double) * hb.size(), hipMemcpyHostToDevice) == hipSuccess);
     if (ok) { B.mod = mod; B.topo = topo; }
-  } else {
-    // the inputs' buffers may have moved when they grew: the slot names them
-    GeomSlot G;
-    ok = hipMemcpy(&G, B.slot.p, sizeof(G), hipMemcpyDeviceToHost) == hipSuccess;
-    if (ok && (G.u != B.u.p || G.a != B.a.p || G.r != B.r.p || G.x != B.x.p || G.vn != B.vn.p || G.fn != B.fn.p ||
-               G.fx != B.fx.p || G.area != B.ar.p || G.x32 != B.x32.p || G.vn32 != B.vn32.p || G.fn32 != B.fn32.p)) {
-      G.u = (double *)B.u.p; G.a = (double *)B.a.p; G.r = (double *)B.r.p; G.x = (double *)B.x.p;
-      G.vn = (double *)B.vn.p; G.fn = (double *)B.fn.p; G.fx = (double *)B.fx.p; G.area = (double *)B.ar.p;
-      G.x32 = (float4 *)B.x32.p; G.vn32 = (float4 *)B.vn32.p; G.fn32 = (float4 *)B.fn32.p;
-      ok = hipMemcpy(B.slot.p, &G, sizeof(G), hipMemcpyHostToDevice) == hipSuccess;
-    }
   }
-  ok = ok && hipMemcpy(B.u.p, u, sizeof(double) * 3 * nv, hipMemcpyHostToDevice) == hipSuccess &&
+  // the slot names every buffer (they may have moved when they grew) and the
+  // component's type, degree and ties
+  GeomSlot G;
+  memset(&G, 0, sizeof(G));
+  G.nv = nv; G.nf = nf; G.comp_pb = 0; G.eval = -1;
+  G.fv = (int *)B.fv.p; G.af_off = (int *)B.off.p; G.af_idx = (int *)B.idx.p; G.u = (double *)B.u.p;
+  G.a = (double *)B.a.p; G.r = (double *)B.r.p; G.x = (double *)B.x.p; G.vn = (double *)B.vn.p;
+  G.fn = (double *)B.fn.p; G.fx = (double *)B.fx.p; G.area = (double *)B.ar.p; G.x32 = (float4 *)B.x32.p;
+  G.vn32 = (float4 *)B.vn32.p; G.fn32 = (float4 *)B.fn32.p;
+  G.comp_type = ctype; G.nhar = nhar; G.hbasis = nbasis ? (const double *)B.hb.p : nullptr; G.sf_tie = sf_tie;
+  ok = ok && hipMemcpy(B.slot.p, &G, sizeof(G), hipMemcpyHostToDevice) == hipSuccess &&
+       hipMemcpy(B.u.p, u, sizeof(double) * 3 * nv, hipMemcpyHostToDevice) == hipSuccess &&
        hipMemcpy(B.a.p, a, sizeof(double) * 3 * nv, hipMemcpyHostToDevice) == hipSuccess &&
        hipMemcpy(B.r.p, r, sizeof(double) * nv, hipMemcpyHostToDevice) == hipSuccess &&
-       hipMemcpy(B.pc.p, pc, sizeof(double) * 9, hipMemcpyHostToDevice) == hipSuccess;
+       hipMemcpy(B.pc.p, pc, sizeof(double) * npc, hipMemcpyHostToDevice) == hipSuccess;
   if (!ok) { (void)hipGetLastError(); B.mod = nullptr; set_err(-3, "realize_mod upload"); return; }
   GeomSlot *ds = (GeomSlot *)B.slot.p;
   int *dflag = (int *)B.flag.p;
   double *dlogf = (double *)((char *)B.flag.p + sizeof(double));
   hipLaunchKernelGGL(k_realize_vertices, dim3((nv + 255) / 256, 1), dim3(256), 0, 0, ds, 1, (const double *)B.pc.p,
                      dflag, dlogf);
+  if (ctype == HARMONIC)
+    hipLaunchKernelGGL(k_realize_harmonic_bad, dim3(1), dim3(256), 0, 0, ds, (const double *)B.pc.p, dflag, dlogf);
   hipLaunchKernelGGL(k_realize_facets, dim3((nf + 255) / 256, 1), dim3(256), 0, 0, ds, 1);
   hipLaunchKernelGGL(k_realize_vnormals, dim3((nv + 255) / 256, 1), dim3(256), 0, 0, ds, 1);
   // compute_moments (realize_mod.c:757-840): area, volume, COM, inertia tensor
@@ -778,11 +801,12 @@ extern "C" void realize_mod(struct par_t *par, struct mod_t *mod) {
                      (double *)B.mom.p, (const double *)nullptr, 0);
   double mom[MOM_N];
   // results through the thread's pinned stage (plain host memory if it cannot be pinned)
-  const size_t nout = 6 * (size_t)nv + 7 * (size_t)nf;
+  const size_t nout = 7 * (size_t)nv + 7 * (size_t)nf;
   std::vector<double> plain;
   double *x = host_stage(nout);
   if (!x) { plain.resize(nout); x = plain.data(); }
   double *vn = x + 3 * (size_t)nv, *fn = vn + 3 * (size_t)nv, *fx = fn + 3 * (size_t)nf, *ar = fx + 3 * (size_t)nf;
+  double *rr = ar + nf;
   int flag = 0;
   double logf = 0.0;
   ok = hipDeviceSynchronize() == hipSuccess &&
@@ -791,19 +815,25 @@ extern "C" void realize_mod(struct par_t *par, struct mod_t *mod) {
        hipMemcpy(fn, B.fn.p, sizeof(double) * 3 * nf, hipMemcpyDeviceToHost) == hipSuccess &&
        hipMemcpy(fx, B.fx.p, sizeof(double) * 3 * nf, hipMemcpyDeviceToHost) == hipSuccess &&
        hipMemcpy(ar, B.ar.p, sizeof(double) * nf, hipMemcpyDeviceToHost) == hipSuccess &&
+       (ctype == VERTEX || hipMemcpy(rr, B.r.p, sizeof(double) * nv, hipMemcpyDeviceToHost) == hipSuccess) &&
        hipMemcpy(&flag, dflag, sizeof(int), hipMemcpyDeviceToHost) == hipSuccess &&
        hipMemcpy(&logf, dlogf, sizeof(double), hipMemcpyDeviceToHost) == hipSuccess &&
        hipMemcpy(mom, B.mom.p, sizeof(double) * MOM_N, hipMemcpyDeviceToHost) == hipSuccess;
   if (!ok) { (void)hipGetLastError(); set_err(-4, "realize_mod kernels"); return; }
-  for (int i = 0; i < nv; i++)
+  for (int i = 0; i < nv; i++) {
     for (int k = 0; k < 3; k++) { rv->v[i].x[k] = x[3 * i + k]; rv->v[i].n[k] = vn[3 * i + k]; }
+    if (ctype != VERTEX) rv->v[i].r.val = rr[i];  // the analytic component's realized distance
+  }
   for (int f = 0; f < nf; f++) {
     for (int k = 0; k < 3; k++) { rv->f[f].n[k] = fn[3 * f + k]; rv->f[f].x[k] = fx[3 * f + k]; }
     rv->f[f].area = ar[f];
   }
   for (int j = 0; j < 3; j++) {
-    double sf = pc[j];
-    if (sf <= 0.01) sf = 0.01 / (1 + 0.01 - sf);
+    double sf = 1.0;  // ELLIPSE, OVOID
+    if (ctype == VERTEX || ctype == HARMONIC) {
+      sf = pc[j];
+      if (sf <= 0.01) sf = 0.01 / (1 + 0.01 - sf);
+    }
     rv->scalefactor[j].val = sf;
   }
   par->baddiam = (unsigned char)flag;
@@ -840,7 +870,7 @@ PenBufs g_pb;
 extern "C" double penalties(struct par_t *par, struct mod_t *mod, struct dat_t *dat) {
   const int n = par->pen.n;
   if (n <= 0) return 0.0;
-  if (mod->shape.ncomp != 1 || mod->shape.comp[0].type != VERTEX) { set_err(-10, "penalties"); return 0.0; }
+  if (mod->shape.ncomp != 1) { set_err(-10, "penalties"); return 0.0; }
   struct comp_t *comp = &mod->shape.comp[0];
   struct vertices_t *rv = &comp->real;
   const int nv = rv->nv, nf = rv->nf, ns = rv->s ? rv->ns : 0;
@@ -850,7 +880,10 @@ extern "C" double penalties(struct par_t *par, struct mod_t *mod, struct dat_t *
   for (int i = 0; i < n; i++) {
     type[i] = par->pen.type[i + 1];
     weight[i] = par->pen.weight[i + 1];
-    if (!pen_supported(type[i])) { set_err(-17, "penalties: type outside this path"); return 0.0; }
+    if (!pen_supported(type[i]) || (type[i] == RDEV && comp->type != VERTEX)) {
+      set_err(-17, "penalties: type outside this path");
+      return 0.0;
+    }
     need |= pen_needs(type[i]);
   }
   if ((need & PEN_NEED_SIDES) && ns <= 0) { set_err(-18, "penalties: model has no side list"); return 0.0; }
@@ -904,7 +937,7 @@ extern "C" double penalties(struct par_t *par, struct mod_t *mod, struct dat_t *
   if (rc) { set_err(rc, "penalties allocation"); return 0.0; }
   GeomSlot G;
   memset(&G, 0, sizeof(G));
-  G.nv = nv; G.nf = nf; G.eval = -1;
+  G.nv = nv; G.nf = nf; G.eval = -1; G.comp_type = comp->type;
   G.fv = (int *)B.fv.p; G.x = (double *)B.x.p; G.fn = (double *)B.fn.p; G.area = (double *)B.ar.p;
   G.u = (double *)B.u.p; G.r = (double *)B.r.p;
   G.sides = ns > 0 ? (const int4 *)B.sides.p : nullptr;

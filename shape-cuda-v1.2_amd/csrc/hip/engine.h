@@ -43,8 +43,10 @@ constexpr int PB_OPT_STRIDE = 6;
 // per set (19): angleoff[3], omegaoff[3], dopscale, delcor t0, delcor n, a[0..9]
 constexpr int PB_SET_STRIDE = 19, PB_SET_ANGLEOFF = 0, PB_SET_OMEGAOFF = 3, PB_SET_DOPSCALE = 6,
               PB_SET_DCT0 = 7, PB_SET_DCN = 8, PB_SET_DCA = 9;
-// per model component (9): scalefactor[3], rot[3], off[3]
-constexpr int PB_COMP_STRIDE = 9;
+// per model component: [0..2] scalefactor (VERTEX, HARMONIC) or 2a, a/b, b/c
+// (ELLIPSE, OVOID); rot[3]; off[3]; [9] the ovoid's k; from [10] the
+// harmonic coefficients, per degree l: a_l0, then (a_lm, b_lm) for m = 1..l
+constexpr int PB_COMP_STRIDE = 10, PB_COMP_OVOID_K = 9, PB_COMP_HAR = 10;
 
 struct GeomSlot {
   int nv, nf;
@@ -67,6 +69,10 @@ struct GeomSlot {
   unsigned char *seen;   // nf
   const int4 *sides;     // ns: the side (edge) list, {v0, v1, f0, f1} (penalties only)
   int ns;
+  int comp_type;         // VERTEX, ELLIPSE, OVOID or HARMONIC
+  int nhar;              // HARMONIC: degree L; hbasis holds (L+1)^2 factors per vertex, in the
+  const double *hbasis;  //   coefficients' order (afactor[l][0], then afactor[l][m], bfactor[l][m])
+  int sf_tie;            // bit j: scale factor j takes factor j-1's value ('=' state)
 };
 
 // view types (same numbers as set_t types)

@@ -54,6 +54,8 @@ struct ModelPack {
   std::vector<int> fv, af_off, af_idx;
   std::vector<double> u, a, r;
   std::vector<int4> sides;  // v0, v1, f0, f1 (penalties)
+  std::vector<double> hbasis;  // harmonic factors per vertex, in the coefficients' order
+  int comp_type = VERTEX, nhar = 0, sf_tie = 0;
   int nv = 0, nf = 0;
   int set_off = 0, nsets = 0;  // the datasets' blocks in the parameter block
   // realized geometry from the host (drop-in mode)
@@ -62,6 +64,7 @@ struct ModelPack {
   int *d_fv = nullptr, *d_af_off = nullptr, *d_af_idx = nullptr;
   double *d_u = nullptr, *d_a = nullptr;
   int4 *d_sides = nullptr;
+  double *d_hbasis = nullptr;
   // view / frame templates
   std::vector<ViewRec> views;
   std::vector<FrameRec> frames;
@@ -128,7 +131,11 @@ int Engine::pack_model(int mi, struct par_t *par, struct mod_t *mod, struct dat_
   int nrad = mod->photo.nradlaws, nopt = mod->photo.noptlaws, nsets = dat->nsets;
   int off_rad = PB_SPIN_END, off_opt = off_rad + PB_RAD_STRIDE * nrad;
   int off_set = off_opt + PB_OPT_STRIDE * nopt, off_comp = off_set + PB_SET_STRIDE * nsets;
-  int size = off_comp + PB_COMP_STRIDE;
+  const int ctype = comp->type;
+  if (ctype != VERTEX && ctype != ELLIPSE && ctype != OVOID && ctype != HARMONIC) return -10;
+  const int nhar = ctype == HARMONIC ? comp->desc.har.nhar : 0;
+  const int nbasis = ctype == HARMONIC ? (nhar + 1) * (nhar + 1) : 0;
+  int size = off_comp + PB_COMP_STRIDE + nbasis;
   P.base.assign(size, 0.0);
   P.comp_off = off_comp;
   P.set_off = off_set;
@@ -235,14 +242,40 @@ int Engine::pack_model(int mi, struct par_t *par, struct mod_t *mod, struct dat_
       else dc_src[s] = last_dc >= 0 ? last_dc : s;
     }
   }
-  // component: scale factors ('=' tie applied at pack time), rotation, offset
+  // component: its shape parameters (scale factors, whose '=' ties the
+  // realization applies, or the ellipsoid / ovoid dimensions), rotation,
+  // offset, the ovoid's k, the harmonic coefficients
   double *pc = b + off_comp;
+  P.comp_type = ctype;
+  P.nhar = nhar;
+  P.sf_tie = 0;
+  struct param_t *shp3 = ctype == ELLIPSE  ? &comp->desc.ell.two_a
+                         : ctype == OVOID  ? &comp->desc.ovoid.two_a
+                         : ctype == HARMONIC ? comp->desc.har.scalefactor
+                                             : comp->desc.ver.scalefactor;
   for (int j = 0; j < 3; j++) {
-    pc[j] = comp->desc.ver.scalefactor[j].val;
-    if (j > 0 && comp->desc.ver.scalefactor[j].state == '=') pc[j] = pc[j - 1];
-    set_param_target(tmap, &comp->desc.ver.scalefactor[j].val, mi, off_comp + j);
+    // (ELLIPSE / OVOID: two_a, a_over_b, b_over_c are consecutive param_t members)
+    struct param_t *q = shp3 + j;
+    pc[j] = q->val;
+    set_param_target(tmap, &q->val, mi, off_comp + j);
+    if ((ctype == VERTEX || ctype == HARMONIC) && j > 0 && q->state == '=') P.sf_tie |= 1 << j;
     pc[3 + j] = comp->rot[j].val; set_param_target(tmap, &comp->rot[j].val, mi, off_comp + 3 + j);
     pc[6 + j] = comp->off[j].val; set_param_target(tmap, &comp->off[j].val, mi, off_comp + 6 + j);
+  }
+  if (ctype == OVOID) {
+    pc[PB_COMP_OVOID_K] = comp->desc.ovoid.k.val;
+    set_param_target(tmap, &comp->desc.ovoid.k.val, mi, off_comp + PB_COMP_OVOID_K);
+  }
+  for (int l = 0, q = PB_COMP_HAR; l <= nhar && ctype == HARMONIC; l++) {
+    pc[q] = comp->desc.har.a[l][0].val;
+    set_param_target(tmap, &comp->desc.har.a[l][0].val, mi, off_comp + q);
+    q++;
+    for (int m = 1; m <= l; m++, q += 2) {
+      pc[q] = comp->desc.har.a[l][m].val;
+      set_param_target(tmap, &comp->desc.har.a[l][m].val, mi, off_comp + q);
+      pc[q + 1] = comp->desc.har.b[l][m].val;
+      set_param_target(tmap, &comp->desc.har.b[l][m].val, mi, off_comp + q + 1);
+    }
   }
   // ---- topology and inputs (none for the views-only drop-ins) ----
   P.nv = rv->nv;
@@ -265,6 +298,17 @@ int Engine::pack_model(int mi, struct par_t *par, struct mod_t *mod, struct dat_
     for (int k = 0; k < 3; k++) { P.u[3 * i + k] = v->u[k]; P.a[3 * i + k] = v->a[k]; }
     int src = v->r.state == '=' ? v->v_mirror : i;
     P.r[i] = rv->v[src].r.val;
+  }
+  P.hbasis.clear();
+  if (ctype == HARMONIC) {
+    P.hbasis.resize((size_t)P.nv * nbasis);
+    for (int i = 0; i < P.nv; i++) {
+      double *h = &P.hbasis[(size_t)i * nbasis];
+      for (int l = 0, q = 0; l <= nhar; l++) {
+        h[q++] = rv->v[i].afactor[l][0];
+        for (int m = 1; m <= l; m++) { h[q++] = rv->v[i].afactor[l][m]; h[q++] = rv->v[i].bfactor[l][m]; }
+      }
+    }
   }
   P.sides.clear();
   if (opt.penalties && rv->s)
@@ -438,6 +482,8 @@ int Engine::build(struct par_t *par, struct mod_t **mods, struct dat_t **dats, i
   for (int m = 0; m < nmodels; m++)
     if ((rc = pack_model(m, par, mods[m], dats[m])) != 0) return rc;
   nevals = (int)evals.size();
+  any_harmonic = false;
+  for (auto &P : g_packs) any_harmonic = any_harmonic || P.comp_type == HARMONIC;
   pb_stride = 0;
   for (auto &P : g_packs) pb_stride = std::max(pb_stride, (int)P.base.size());
   pb_stride = (pb_stride + 7) & ~7;
@@ -454,6 +500,7 @@ int Engine::build(struct par_t *par, struct mod_t **mods, struct dat_t **dats, i
     if ((rc = upload(this, &P.d_u, P.u))) return rc;
     if ((rc = upload(this, &P.d_a, P.a))) return rc;
     if (!P.sides.empty() && (rc = upload(this, &P.d_sides, P.sides))) return rc;
+    if (!P.hbasis.empty() && (rc = upload(this, &P.d_hbasis, P.hbasis))) return rc;
   }
   // evaluations, their slots, views and frames
   eval_targets.assign(nevals, {});
@@ -611,6 +658,7 @@ int Engine::build(struct par_t *par, struct mod_t **mods, struct dat_t **dats, i
     max_nf = std::max(max_nf, G.nf);
     G.fv = P.d_fv; G.af_off = P.d_af_off; G.af_idx = P.d_af_idx; G.u = P.d_u; G.a = P.d_a;
     G.sides = P.d_sides; G.ns = (int)P.sides.size();
+    G.comp_type = P.comp_type; G.nhar = P.nhar; G.hbasis = P.d_hbasis; G.sf_tie = P.sf_tie;
     if ((rc = upload(this, &G.r, slot_r[s]))) return rc;
     if ((rc = alloc((void **)&G.x, sizeof(double) * 3 * G.nv))) return rc;
     if ((rc = alloc((void **)&G.vn, sizeof(double) * 3 * G.nv))) return rc;
@@ -924,6 +972,10 @@ int Engine::setup_penalties(struct par_t *par) {
     type[i] = par->pen.type[i + 1];
     weight[i] = par->pen.weight[i + 1];
     if (!pen_supported(type[i])) return -17;
+    // rdev needs a VERTEX component (penalties.c:725-726 bails out otherwise)
+    if (type[i] == RDEV)
+      for (auto &P : g_packs)
+        if (P.comp_type != VERTEX) return -17;
     pen_need |= pen_needs(type[i]);
   }
   if (pen_need & PEN_NEED_SIDES)
@@ -978,6 +1030,9 @@ int Engine::enqueue() {
     for (auto &G : h_slots) max_nv = std::max(max_nv, G.nv);
     SHP_LAUNCH(*this, "k_realize_vertices", k_realize_vertices, dim3(cdiv(max_nv, T), nslots), dim3(T), 0, stream, d_slots, nslots,
                        d_params, d_slot_flags, d_slot_logf);
+    if (any_harmonic)
+      SHP_LAUNCH(*this, "k_realize_harmonic_bad", k_realize_harmonic_bad, dim3(nslots), dim3(256), 0, stream, d_slots, d_params,
+                 d_slot_flags, d_slot_logf);
     SHP_LAUNCH(*this, "k_realize_facets", k_realize_facets, dim3(cdiv(max_nf, T), nslots), dim3(T), 0, stream, d_slots, nslots);
     SHP_LAUNCH(*this, "k_realize_vnormals", k_realize_vnormals, dim3(cdiv(max_nv, T), nslots), dim3(T), 0, stream, d_slots, nslots);
   }

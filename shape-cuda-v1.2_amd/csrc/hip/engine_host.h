@@ -123,6 +123,7 @@ class Engine {
   long long total_pix = 0, fit_len = 0, obs_len = 0, total_tiles = 0;
   long long lcx_len = 0, lct_len = 0;  // lightcurve epochs: calculated points, observed points x views
   bool any_lc = false, any_dd = false, any_dop = false;
+  bool any_harmonic = false;  // a model with a HARMONIC component (its per-vertex baddiam guard)
   // mixed method: views taking the binned tile kernel / the global z-buffer
   bool any_binned = false, any_global = false;
   int max_bin_ntx = 0, max_nblk = 0;

@@ -227,27 +227,180 @@ __global__ void k_pos_seed(ViewRec *views, PosDev pd, int src) {
 // ---------------------------------------------------------------------------
 // realization (realize_mod.c:444-563)
 // ---------------------------------------------------------------------------
+// Real roots of coeff[3] x^3 + coeff[2] x^2 + coeff[1] x + coeff[0]
+// (realize_mod.c:852-941): the quadratic or linear case below a small cubic
+// coefficient, otherwise the trigonometric (three roots) or Cardano (one
+// root) solution of Numerical Recipes in C, 2nd ed., section 5.6, with
+// r^2 - q^3 written out in the coefficients.  Returns the root count, or -1
+// where the reference bails out.
+__device__ int cubic_roots(const double coeff[4], double root[3]) {
+  root[0] = root[1] = root[2] = -1.0e20;
+  if (fabs(coeff[3]) < 1.0e-5) {
+    const double a = coeff[2], b = coeff[1], c = coeff[0];
+    if (fabs(a) < 1.0e-10) {
+      if (fabs(b) < 1.0e-10) return -1;
+      root[0] = -c / b;
+      return 1;
+    }
+    const double disc = b * b - 4 * a * c;
+    if (disc < 0.0) return -1;
+    if (fabs(b) < 1.0e-10) {
+      root[0] = sqrt(disc) / (2 * a);
+      root[1] = -root[0];
+    } else {
+      const int bsign = (b < 0.0) ? -1 : 1;
+      const double q = -0.5 * (b + bsign * sqrt(disc));
+      root[0] = q / a;
+      root[1] = c / q;
+    }
+    return 2;
+  }
+  const double a = coeff[2] / coeff[3], b = coeff[1] / coeff[3], c = coeff[0] / coeff[3];
+  const double q = (a * a - 3 * b) / 9;
+  const double r = (2 * a * a * a - 9 * a * b + 27 * c) / 54;
+  const double d = (4 * a * a * a * c - a * a * b * b - 18 * a * b * c + 27 * c * c + 4 * b * b * b) / 108;
+  if (d >= 0.0) {
+    const double rsign = (r < 0.0) ? -1 : 1;
+    const double s = -rsign * pow(fabs(r) + sqrt(d), 1.0 / 3);
+    const double t = (fabs(s) >= 1.0e-10) ? q / s : 0.0;
+    root[0] = s + t - a / 3;
+    return 1;
+  }
+  const double qs = sqrt(q), theta = acos(r / (q * qs));
+  root[0] = -2 * qs * cos(theta / 3) - a / 3;
+  root[1] = -2 * qs * cos((theta + 2 * 3.141592653589793) / 3) - a / 3;
+  root[2] = -2 * qs * cos((theta - 2 * 3.141592653589793) / 3) - a / 3;
+  return 3;
+}
+
+constexpr double HAIRWIDTH_D = 1.0e-7, SMALLRATIO_D = 0.01;
+
+// the harmonic radius of vertex i before the HAIRWIDTH guard
+__device__ inline double harmonic_r(const GeomSlot &G, const double *coef, int i) {
+  const int nb = (G.nhar + 1) * (G.nhar + 1);
+  const double *h = G.hbasis + (size_t)i * nb;
+  double r = 0.0;
+  for (int l = 0, q = 0; l <= G.nhar; l++) {
+    r += coef[q] * h[q];
+    q++;
+    for (int m = 1; m <= l; m++, q += 2) r += coef[q] * h[q] + coef[q + 1] * h[q + 1];
+  }
+  return r;
+}
+
+// Vertex coordinates x = s (u r + a), turned and shifted (realize_mod.c:444-512)
+// with r by component type (:217-442): VERTEX the slot's deviation; ELLIPSE
+// and OVOID from their parameters (kept in bounds with the baddiam flag,
+// scale factors 1); HARMONIC the coefficients times the vertex's factors.
+// Every thread derives the component's constants; thread 0 of the slot
+// writes the per-component baddiam flag and log factor (the harmonic
+// per-vertex guard adds to them in k_realize_harmonic_bad).  An ovoid vertex
+// with no admissible root (the reference bails out) gets a NaN position.
 __global__ void k_realize_vertices(GeomSlot *slots, int nslot, const double *params, int *slot_flags,
                                    double *slot_logf) {
   int s = blockIdx.y;
   GeomSlot &G = slots[s];
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   const double *pc = params + G.comp_pb;
-  double sf[3];
+  const int type = G.comp_type;
+  double sf[3] = {1.0, 1.0, 1.0};
   int bad = 0;
   double blog = 0.0;
-  for (int j = 0; j < 3; j++) {
-    sf[j] = pc[j];
-    if (sf[j] <= 0.01) {
-      bad = 1;
-      blog += log(1 + 0.01 - sf[j]);
-      sf[j] = 0.01 / (1 + 0.01 - sf[j]);
+  double ea[3] = {0.0, 0.0, 0.0};                        // ELLIPSE: inverse squared radii
+  double o_ar = 0.0, o_ab = 0.0, o_bc = 0.0, o_k = 0.0, o_x0 = 0.0;  // OVOID
+  if (type == ELLIPSE) {
+    double diam = pc[0], ratio;
+    if (diam > HAIRWIDTH_D) ea[0] = 2.0 / diam;
+    else { ea[0] = (2.0 / HAIRWIDTH_D) * (1 + HAIRWIDTH_D - diam); bad = 1; blog += log(1 + HAIRWIDTH_D - diam); }
+    diam = (2.0 / ea[0]);
+    ratio = pc[1];
+    if (ratio > SMALLRATIO_D) ea[1] = 2.0 / (diam / ratio);
+    else {
+      ea[1] = (2.0 / (diam / SMALLRATIO_D)) / (1 + SMALLRATIO_D - ratio);
+      bad = 1; blog += log(1 + SMALLRATIO_D - ratio);
+    }
+    diam = (2.0 / ea[1]);
+    ratio = pc[2];
+    if (ratio > SMALLRATIO_D) ea[2] = 2.0 / (diam / ratio);
+    else {
+      ea[2] = (2.0 / (diam / SMALLRATIO_D)) / (1 + SMALLRATIO_D - ratio);
+      bad = 1; blog += log(1 + SMALLRATIO_D - ratio);
+    }
+    for (int j = 0; j < 3; j++) ea[j] *= ea[j];
+  } else if (type == OVOID) {
+    o_ar = pc[0] / 2;
+    if (o_ar <= HAIRWIDTH_D / 2) {
+      bad = 1; blog += log(1 + HAIRWIDTH_D - 2 * o_ar);
+      o_ar = (HAIRWIDTH_D / 2) / (1 + HAIRWIDTH_D - 2 * o_ar);
+    }
+    o_ab = pc[1];
+    if (o_ab <= SMALLRATIO_D) { bad = 1; blog += log(1 + SMALLRATIO_D - o_ab); o_ab = SMALLRATIO_D / (1 + SMALLRATIO_D - o_ab); }
+    o_bc = pc[2];
+    if (o_bc <= SMALLRATIO_D) { bad = 1; blog += log(1 + SMALLRATIO_D - o_bc); o_bc = SMALLRATIO_D / (1 + SMALLRATIO_D - o_bc); }
+    o_k = pc[PB_COMP_OVOID_K];
+    if (fabs(o_k) > 1 - 1.0e-10) {
+      bad = 1; blog += log(fabs(o_k) + 1.0e-10);
+      o_k = o_k > 0.0 ? 1 - 1.0e-10 * (1 - 1.0e-10) / o_k : -1 - 1.0e-10 * (1 - 1.0e-10) / o_k;
+    }
+    if (fabs(o_k) > 0.01) {
+      const double x0term = 3 * (1 - o_k * o_k) * log((1 + o_k) / (1 - o_k));
+      const double numer = 2.0 * o_k * (3 - 2 * o_k * o_k) - x0term;
+      const double denom = 2.0 * o_k * (3 - o_k * o_k) - x0term;
+      o_x0 = (o_ar / o_k) * (numer / denom);
+    } else {
+      o_x0 = 0.4 * o_k * o_ar;
+    }
+  } else {
+    double raw[3];
+    for (int j = 0; j < 3; j++) {
+      raw[j] = (j > 0 && ((G.sf_tie >> j) & 1)) ? raw[j - 1] : pc[j];
+      sf[j] = raw[j];
+      if (sf[j] <= SMALLRATIO_D) {
+        bad = 1;
+        blog += log(1 + SMALLRATIO_D - sf[j]);
+        sf[j] = SMALLRATIO_D / (1 + SMALLRATIO_D - sf[j]);
+      }
     }
   }
   if (i == 0) { slot_flags[s] = bad; slot_logf[s] = blog; }
   if (i >= G.nv) return;
+  const double *u = G.u + 3 * i;
+  double r;
+  if (type == ELLIPSE) {
+    double den = 0.0;
+    for (int j = 0; j < 3; j++) den += ea[j] * (u[j] * u[j]);
+    r = 1 / sqrt(den);
+    G.r[i] = r;
+  } else if (type == OVOID) {
+    const double ac = o_ab * o_bc;
+    const double h = o_ab * o_ab * u[1] * u[1] + ac * ac * u[2] * u[2];
+    const double alpha0 = o_x0 / o_ar, ux = u[0];
+    double coeff[4], root[3], good;
+    coeff[3] = (h - ux * ux) * o_k * ux;
+    coeff[2] = (1 + 3 * o_k * alpha0) * ux * ux + h * (1 - o_k * alpha0);
+    coeff[1] = (o_k - (2 + 3 * o_k * alpha0) * alpha0) * ux;
+    coeff[0] = -(1 - alpha0 * alpha0) * (1 + o_k * alpha0);
+    if (fabs(o_k) <= 1.0e-6) {
+      good = 1 / sqrt(ux * ux + h);
+      good -= (coeff[3] * good * good * good + coeff[1] * good) /
+              (3 * coeff[3] * good * good + 2 * coeff[2] * good + coeff[1]);
+    } else {
+      const int nr = cubic_roots(coeff, root);
+      good = -1.0e20;
+      for (int q = 0; q < nr; q++)
+        if (root[q] >= 0.0 && fabs(root[q] * ux - alpha0) - 1 < 1.0e-6) good = fmax(good, root[q]);
+    }
+    r = good < 0.0 ? __builtin_nan("") : good * o_ar;
+    G.r[i] = r;
+  } else if (type == HARMONIC) {
+    r = harmonic_r(G, pc + PB_COMP_HAR, i);
+    if (!(r > HAIRWIDTH_D / 2)) r = (HAIRWIDTH_D / 2) / (1 + HAIRWIDTH_D - 2 * r);
+    G.r[i] = r;
+  } else {
+    r = G.r[i];
+  }
   double x[3];
-  for (int j = 0; j < 3; j++) x[j] = sf[j] * (G.u[3 * i + j] * G.r[i] + G.a[3 * i + j]);
+  for (int j = 0; j < 3; j++) x[j] = sf[j] * (u[j] * r + G.a[3 * i + j]);
   const double *rot = pc + 3, *off = pc + 6;
   if (!(rot[0] == 0 && rot[1] == 0 && rot[2] == 0)) {
     double m[9], y[3];
@@ -262,6 +415,38 @@ __global__ void k_realize_vertices(GeomSlot *slots, int nslot, const double *par
     for (int j = 0; j < 3; j++) x[j] += off[j];
   for (int j = 0; j < 3; j++) G.x[3 * i + j] = x[j];
   G.x32[i] = make_float4((float)x[0], (float)x[1], (float)x[2], 0.0f);
+}
+
+// The harmonic component's per-vertex guard (realize_mod.c:410-415): for
+// every vertex whose radius is not above HAIRWIDTH / 2, log(1 + HAIRWIDTH -
+// 2 r) / (L+1)^2 joins the slot's baddiam log factor.  One workgroup per
+// slot, a fixed tree: the sum repeats from replay to replay.
+__global__ __launch_bounds__(256) void k_realize_harmonic_bad(GeomSlot *slots, const double *params, int *slot_flags,
+                                                              double *slot_logf) {
+  const int s = blockIdx.x;
+  const GeomSlot &G = slots[s];
+  if (G.comp_type != HARMONIC) return;
+  __shared__ double sh[256];
+  __shared__ int any[256];
+  const double *coef = params + G.comp_pb + PB_COMP_HAR;
+  const double norm = (double)((G.nhar + 1) * (G.nhar + 1));
+  double acc = 0.0;
+  int a = 0;
+  for (int i = threadIdx.x; i < G.nv; i += 256) {
+    const double r = harmonic_r(G, coef, i);
+    if (!(r > HAIRWIDTH_D / 2)) { acc += log(1 + HAIRWIDTH_D - 2 * r) / norm; a = 1; }
+  }
+  sh[threadIdx.x] = acc;
+  any[threadIdx.x] = a;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) { sh[threadIdx.x] += sh[threadIdx.x + w]; any[threadIdx.x] |= any[threadIdx.x + w]; }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0 && any[0]) {
+    slot_flags[s] = 1;
+    slot_logf[s] = sh[0] + slot_logf[s];
+  }
 }
 
 __global__ void k_realize_facets(GeomSlot *slots, int nslot) {

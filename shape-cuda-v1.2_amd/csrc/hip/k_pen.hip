@@ -306,6 +306,7 @@ __global__ __launch_bounds__(256) void k_pen_vert_part(const GeomSlot *slots, co
     incr = (m[MOM_HI + jmax] - lo + SMALLVAL_D) / NBIF;
   }
   double rsum = 0.0, rmax = 0.0, emax = 0.0;
+  const bool vertex_comp = G.comp_type != ELLIPSE && G.comp_type != OVOID && G.comp_type != HARMONIC;
   double zs[NZONES], zc[NZONES];
 #pragma unroll
   for (int k = 0; k < NZONES; k++) zs[k] = zc[k] = 0.0;
@@ -318,9 +319,11 @@ __global__ __launch_bounds__(256) void k_pen_vert_part(const GeomSlot *slots, co
       scale += y * y;
     }
     scale = sqrt(scale);
-    const double dx = scale * G.r[v] / reff, d2 = dx * dx;
-    rsum += d2;
-    rmax = dmax(rmax, d2);
+    if (vertex_comp) {  // rdev / maxrdev visit VERTEX components only (penalties.c:701, :741)
+      const double dx = scale * G.r[v] / reff, d2 = dx * dx;
+      rsum += d2;
+      rmax = dmax(rmax, d2);
+    }
     if (want_pa) {
       // body -> principal-axis frame: w = ap^T x
       double w[3];

@@ -101,6 +101,7 @@ __global__ void k_pos_seed(ViewRec *views, PosDev pd, int src);
 __global__ void k_radlaw_one(const double *pl, int type, double c, double *out);
 __global__ void k_realize_vertices(GeomSlot *slots, int nslot, const double *params, int *slot_flags,
                                    double *slot_logf);
+__global__ void k_realize_harmonic_bad(GeomSlot *slots, const double *params, int *slot_flags, double *slot_logf);
 __global__ void k_realize_facets(GeomSlot *slots, int nslot);
 __global__ void k_realize_vnormals(GeomSlot *slots, int nslot);
 __global__ void k_raster64(ViewRec *views, const GeomSlot *slots, PosDev pd, int src, int pass);

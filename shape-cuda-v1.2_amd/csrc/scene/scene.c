@@ -300,6 +300,10 @@ void shp_spec_defaults(shp_spec *s)
   s->theta_steps = 30;
   s->semi_axes[0] = 1.0; s->semi_axes[1] = 0.8; s->semi_axes[2] = 0.6;
   s->rdev_sigma = 0.02;
+  s->comp_type = 0;
+  s->ovoid_k = 0.2;
+  s->nhar = 3;
+  s->har_sigma = 0.05;
   s->seed_geom = 1001;
   s->seed_frames = 1002;
   s->spin_period_h = 6.0;
@@ -478,6 +482,123 @@ static void build_sides(shp_scene *sc, struct vertices_t *vt)
   for (int s = 0; s < vt->ns; s++) vt->s[s].act = 1;
 }
 
+/* Normalized associated Legendre functions N_lm P_lm(x), 0 <= m <= l <= L,
+   N_lm = sqrt((2l+1) (l-m)!/(l+m)!), into p[l*(L+1)+m]: P_mm by the product
+   formula, then the upward recurrence in l (Condon-Shortley phase included,
+   as the Numerical Recipes routine the reference calls has it) */
+static void legendre_norm(int L, double x, double *p)
+{
+  const double s = sqrt((1.0 - x) * (1.0 + x));
+  for (int m = 0; m <= L; m++) {
+    double pmm = 1.0, fact = 1.0;
+    for (int i = 1; i <= m; i++) { pmm *= -fact * s; fact += 2.0; }
+    double prev = 0.0, cur = pmm;
+    for (int l = m; l <= L; l++) {
+      if (l == m + 1) { prev = cur; cur = x * (2 * m + 1) * pmm; }
+      else if (l > m + 1) { double nxt = (x * (2 * l - 1) * cur - (l + m - 1) * prev) / (l - m); prev = cur; cur = nxt; }
+      double ratio = 1.0; /* (l-m)!/(l+m)! */
+      for (int i = l - m + 1; i <= l + m; i++) ratio /= i;
+      p[l * (L + 1) + m] = sqrt((2 * l + 1) * ratio) * cur;
+    }
+  }
+}
+
+static void build_spin(shp_scene *sc)
+{
+  const shp_spec *sp = &sc->spec;
+  struct mod_t *mod = &sc->mod;
+  struct spin_t *s = &mod->spin;
+  s->pa = 1;
+  s->t0 = sp->t0;
+  s->angle[0].val = (sp->pole_lambda_deg + 90.0) * D2R;
+  s->angle[1].val = (90.0 - sp->pole_beta_deg) * D2R;
+  s->angle[2].val = sp->phase0_deg * D2R;
+  s->omega[2].val = TWOPI / (sp->spin_period_h / 24.0);
+  for (int k = 0; k < 3; k++) {
+    s->angle[k].state = 'f'; s->omega[k].state = k == 2 ? 'f' : 'c'; s->omegadot[k].state = 'c';
+    s->inertia[k].state = 'c'; s->inertia[k].val = 1.0;
+  }
+  s->lib_amp.state = s->lib_freq.state = s->lib_phase.state = 'c';
+}
+
+/* An analytic component on the ellipsoid tessellation's directions
+   (setuprealver's rings, read_mod.c:2179-2290): base points zero, r to be
+   realized; the harmonic factors afactor[l][m] = N_lm P_lm(cos theta)
+   cos(m phi), bfactor = ... sin(m phi); the parameters from the spec */
+static void build_analytic(shp_scene *sc, struct comp_t *c, const mesh_t *m, rng_t *rg)
+{
+  const shp_spec *sp = &sc->spec;
+  struct vertices_t *vt = &c->real;
+  vt->nv = m->nv;
+  vt->nf = m->nf;
+  vt->v = (struct vertex_t *)tcalloc(sc, m->nv, sizeof(struct vertex_t));
+  vt->f = (struct facet_t *)tcalloc(sc, m->nf, sizeof(struct facet_t));
+  for (int k = 0; k < 3; k++) { vt->scalefactor[k].state = 'f'; vt->scalefactor[k].val = 1.0; }
+  const int L = sp->nhar;
+  double *plm = (double *)malloc(sizeof(double) * (L + 1) * (L + 1));
+  for (int i = 0; i < m->nv; i++) {
+    struct vertex_t *v = &vt->v[i];
+    for (int k = 0; k < 3; k++) { v->u[k] = m->u[3 * i + k]; v->a[k] = 0.0; }
+    v->r.state = 'f';
+    v->act = 1;
+    v->v_mirror = i;
+    if (sp->comp_type == 3) {
+      const double phi = atan2(v->u[1], v->u[0]);
+      legendre_norm(L, v->u[2], plm);
+      v->afactor = t_matrix(sc, 0, L, 0, L);
+      if (L > 0) v->bfactor = t_matrix(sc, 1, L, 1, L);
+      for (int l = 0; l <= L; l++) {
+        v->afactor[l][0] = plm[l * (L + 1)];
+        for (int mm = 1; mm <= l; mm++) {
+          v->afactor[l][mm] = cos(mm * phi) * plm[l * (L + 1) + mm];
+          v->bfactor[l][mm] = sin(mm * phi) * plm[l * (L + 1) + mm];
+        }
+      }
+    }
+  }
+  free(plm);
+  for (int f = 0; f < m->nf; f++) {
+    for (int k = 0; k < 3; k++) vt->f[f].v[k] = m->fv[3 * f + k];
+    vt->f[f].act = 1;
+  }
+  const double *ax = sp->semi_axes;
+  if (sp->comp_type == 1) {
+    c->type = ELLIPSE;
+    c->desc.ell.ntheta = sp->theta_steps;
+    c->desc.ell.two_a.state = c->desc.ell.a_over_b.state = c->desc.ell.b_over_c.state = 'f';
+    c->desc.ell.two_a.val = 2 * ax[0];
+    c->desc.ell.a_over_b.val = ax[0] / ax[1];
+    c->desc.ell.b_over_c.val = ax[1] / ax[2];
+  } else if (sp->comp_type == 2) {
+    c->type = OVOID;
+    c->desc.ovoid.ntheta = sp->theta_steps;
+    c->desc.ovoid.two_a.state = c->desc.ovoid.a_over_b.state = c->desc.ovoid.b_over_c.state = 'f';
+    c->desc.ovoid.k.state = 'f';
+    c->desc.ovoid.two_a.val = 2 * ax[0];
+    c->desc.ovoid.a_over_b.val = ax[0] / ax[1];
+    c->desc.ovoid.b_over_c.val = ax[1] / ax[2];
+    c->desc.ovoid.k.val = sp->ovoid_k;
+  } else {
+    c->type = HARMONIC;
+    struct harmonic_t *h = &c->desc.har;
+    h->ntheta = sp->theta_steps;
+    h->nhar = L;
+    for (int k = 0; k < 3; k++) { h->scalefactor[k].state = 'f'; h->scalefactor[k].val = 1.0; }
+    h->a = (struct param_t **)tcalloc(sc, L + 1, sizeof(struct param_t *));
+    h->b = (struct param_t **)tcalloc(sc, L + 1, sizeof(struct param_t *));
+    const double r0 = cbrt(ax[0] * ax[1] * ax[2]);
+    for (int l = 0; l <= L; l++) {
+      h->a[l] = (struct param_t *)tcalloc(sc, l + 1, sizeof(struct param_t));
+      h->b[l] = (struct param_t *)tcalloc(sc, l + 1, sizeof(struct param_t));
+      for (int mm = 0; mm <= l; mm++) {
+        h->a[l][mm].state = h->b[l][mm].state = 'f';
+        h->a[l][mm].val = l == 0 ? r0 : sp->har_sigma * r0 * rng_normal(rg) / (l + 1);
+        h->b[l][mm].val = mm == 0 ? 0.0 : sp->har_sigma * r0 * rng_normal(rg) / (l + 1);
+      }
+    }
+  }
+}
+
 static void build_model(shp_scene *sc)
 {
   const shp_spec *sp = &sc->spec;
@@ -493,6 +614,44 @@ static void build_model(shp_scene *sc)
   struct comp_t *c = &mod->shape.comp[0];
   c->type = VERTEX;
   for (int k = 0; k < 3; k++) { c->off[k].state = 'c'; c->rot[k].state = 'c'; }
+  if (sp->comp_type != 0) {
+    build_analytic(sc, c, &m, &rg);
+    build_sides(sc, &c->real);
+    for (int f = 0; f < m.nf; f++)
+      for (int k = 0; k < 3; k++) c->real.v[m.fv[3 * f + k]].naf++;
+    for (int i = 0; i < m.nv; i++) {
+      c->real.v[i].af = (int *)tcalloc(sc, c->real.v[i].naf, sizeof(int));
+      c->real.v[i].naf = 0;
+    }
+    for (int f = 0; f < m.nf; f++)
+      for (int k = 0; k < 3; k++) {
+        struct vertex_t *v = &c->real.v[m.fv[3 * f + k]];
+        v->af[v->naf++] = f;
+      }
+    free(m.u);
+    free(m.fv);
+    build_spin(sc);
+    if (c->type == ELLIPSE) {
+      add_param(sc, &c->desc.ell.two_a.val, SHP_SIZEPAR);
+      add_param(sc, &c->desc.ell.a_over_b.val, SHP_SIZEPAR);
+      add_param(sc, &c->desc.ell.b_over_c.val, SHP_SIZEPAR);
+    } else if (c->type == OVOID) {
+      add_param(sc, &c->desc.ovoid.two_a.val, SHP_SIZEPAR);
+      add_param(sc, &c->desc.ovoid.a_over_b.val, SHP_SIZEPAR);
+      add_param(sc, &c->desc.ovoid.b_over_c.val, SHP_SIZEPAR);
+    } else {
+      for (int k = 0; k < 3; k++) add_param(sc, &c->desc.har.scalefactor[k].val, SHP_SIZEPAR);
+    }
+    add_param(sc, &sc->mod.spin.omega[2].val, SHP_SPINPAR);
+    add_param(sc, &sc->mod.spin.angle[0].val, SHP_SPINPAR);
+    add_param(sc, &sc->mod.spin.angle[1].val, SHP_SPINPAR);
+    if (c->type == OVOID) add_param(sc, &c->desc.ovoid.k.val, SHP_SHAPEPAR);
+    if (c->type == HARMONIC && c->desc.har.nhar >= 1) {
+      add_param(sc, &c->desc.har.a[0][0].val, SHP_SHAPEPAR);
+      add_param(sc, &c->desc.har.a[1][1].val, SHP_SHAPEPAR);
+    }
+    return;
+  }
   struct vertices_t *vt = &c->desc.ver;
   for (int k = 0; k < 3; k++) { vt->scalefactor[k].state = 'f'; vt->scalefactor[k].val = 1.0; }
   vt->nv = m.nv;
@@ -533,26 +692,14 @@ static void build_model(shp_scene *sc)
   free(m.u);
   free(m.fv);
 
-  /* spin */
-  struct spin_t *s = &mod->spin;
-  s->pa = 1;
-  s->t0 = sp->t0;
-  s->angle[0].val = (sp->pole_lambda_deg + 90.0) * D2R;
-  s->angle[1].val = (90.0 - sp->pole_beta_deg) * D2R;
-  s->angle[2].val = sp->phase0_deg * D2R;
-  s->omega[2].val = TWOPI / (sp->spin_period_h / 24.0);
-  for (int k = 0; k < 3; k++) {
-    s->angle[k].state = 'f'; s->omega[k].state = k == 2 ? 'f' : 'c'; s->omegadot[k].state = 'c';
-    s->inertia[k].state = 'c'; s->inertia[k].val = 1.0;
-  }
-  s->lib_amp.state = s->lib_freq.state = s->lib_phase.state = 'c';
+  build_spin(sc);
 
   add_param(sc, &c->desc.ver.scalefactor[0].val, SHP_SIZEPAR);
   add_param(sc, &c->desc.ver.scalefactor[1].val, SHP_SIZEPAR);
   add_param(sc, &c->desc.ver.scalefactor[2].val, SHP_SIZEPAR);
-  add_param(sc, &s->omega[2].val, SHP_SPINPAR);
-  add_param(sc, &s->angle[0].val, SHP_SPINPAR);
-  add_param(sc, &s->angle[1].val, SHP_SPINPAR);
+  add_param(sc, &mod->spin.omega[2].val, SHP_SPINPAR);
+  add_param(sc, &mod->spin.angle[0].val, SHP_SPINPAR);
+  add_param(sc, &mod->spin.angle[1].val, SHP_SPINPAR);
 }
 
 /* epoch of the k-th of n frames */
@@ -836,8 +983,8 @@ static void build_data(shp_scene *sc)
   }
   /* last, a vertex deviation: the radius of a vertex a third of the way
      through the list (a shape parameter that moves only its own facets) */
-  struct vertices_t *vt = &sc->mod.shape.comp[0].desc.ver;
-  if (vt->nv > 0) add_param(sc, &vt->v[vt->nv / 3].r.val, SHP_SHAPEPAR);
+  struct vertices_t *vt = &sc->mod.shape.comp[0].desc.ver;  /* (a union member: VERTEX only) */
+  if (sc->mod.shape.comp[0].type == VERTEX && vt->nv > 0) add_param(sc, &vt->v[vt->nv / 3].r.val, SHP_SHAPEPAR);
 }
 
 static void build_par(shp_scene *sc)
@@ -1374,6 +1521,27 @@ void shp_moments(struct mod_t *mod, int mode, double out[14])
   out[1] = *volume;
   for (int k = 0; k < 3; k++) out[2 + k] = com[k];
   for (int k = 0; k < 9; k++) out[5 + k] = inertia[k];
+}
+
+/* the harmonic component's coefficients, (l, m) with 0 <= m <= l <= L in
+   order, a and b (b[l][0] is 0): which = 0 reads them, 1 writes them;
+   returns L, or -1 for another component type */
+int shp_harmonic(struct mod_t *mod, int which, double *a, double *b)
+{
+  struct comp_t *c = &mod->shape.comp[0];
+  if (c->type != HARMONIC) return -1;
+  const int L = c->desc.har.nhar;
+  for (int l = 0, q = 0; l <= L; l++)
+    for (int m = 0; m <= l; m++, q++) {
+      if (which) {
+        c->desc.har.a[l][m].val = a[q];
+        if (m > 0) c->desc.har.b[l][m].val = b[q];
+      } else {
+        a[q] = c->desc.har.a[l][m].val;
+        b[q] = m > 0 ? c->desc.har.b[l][m].val : 0.0;
+      }
+    }
+  return L;
 }
 
 /* facet corners of the model's first component (3 per facet) */

@@ -73,6 +73,15 @@ typedef struct shp_spec {
   /* Doppler response: sinc^2 evaluated at nsinc2 x nsinc2 points per POS
      pixel over sinc2width bins (par_t nsinc2 / sinc2width) */
   int nsinc2, sinc2width;
+  /* component type (read_mod.c:289-420): VERTEX (0, the default), or the
+     analytic ELLIPSE (1), OVOID (2) and HARMONIC (3) components realized from
+     their parameters on the ellipsoid tessellation's directions
+     (theta_steps); ovoid asymmetry k; harmonic degree and the rms of the
+     random l >= 1 coefficients (relative to the mean radius) */
+  int comp_type;
+  double ovoid_k;
+  int nhar;
+  double har_sigma;
 } shp_spec;
 
 typedef struct shp_scene shp_scene;
@@ -156,6 +165,7 @@ int shp_set_impulses(struct mod_t *mod, int n, const double *t, const double *im
 void shp_moments(struct mod_t *mod, int mode, double out[14]);
 int shp_sides(struct mod_t *mod, int *out, int cap);
 void shp_facets(struct mod_t *mod, int *out);
+int shp_harmonic(struct mod_t *mod, int which, double *a, double *b);
 void shp_set_vertex_r(struct mod_t *mod, const double *r);
 void shp_set_comp_transform(struct mod_t *mod, const double *rot, const double *off);
 /* vary_params: par's vary flags, calibration / Doppler-scale / delay-constant

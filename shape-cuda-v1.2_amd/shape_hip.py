@@ -28,6 +28,8 @@ DELAY, DOPPLER, POS, LGHTCRV = 0, 1, 2, 3
 (COSINELAW_DIFF, GAUSSIANLAW, HAGFORSLAW, COSINELAW_QS, GAUSSIAN_COSINE, HAGFORS_COSINE,
  COSINE_COSINE) = 1, 2, 3, 4, 5, 6, 7
 SHAPE_ELLIPSOID_TESS, SHAPE_ICOSPHERE = 0, 1
+# scene component types (shp_spec.comp_type)
+COMP_VERTEX, COMP_ELLIPSE, COMP_OVOID, COMP_HARMONIC = 0, 1, 2, 3
 # penalty types (const.h's numbering)
 (NONSMOOTH, CONCAVITY, RDEV, VOLUME, COMDEV, INERTIADEV, INERTIADEV_UNI, PA3TILT, NONPA, NONPA_UNI,
  EULEROFFS, FLATTENING, MAXRDEV, MAXELLIPDEV, NONCOSINE, BIFURCATION, IMPULSE) = range(9, 26)
@@ -59,6 +61,7 @@ class Spec(C.Structure):
         ("obs_lat_min_deg", C.c_double), ("obs_lat_max_deg", C.c_double),
         ("radar_nviews", C.c_int), ("view_interval_days", C.c_double), ("lc_nviews", C.c_int),
         ("nsinc2", C.c_int), ("sinc2width", C.c_int),
+        ("comp_type", C.c_int), ("ovoid_k", C.c_double), ("nhar", C.c_int), ("har_sigma", C.c_double),
     ]
 
 
@@ -119,6 +122,7 @@ def load_scene():
             "shp_moments": (None, [vp, C.c_int, dp]),
             "shp_sides": (C.c_int, [vp, ip, C.c_int]),
             "shp_facets": (None, [vp, ip]),
+            "shp_harmonic": (C.c_int, [vp, C.c_int, dp, dp]),
             "shp_set_vertex_r": (None, [vp, dp]),
             "shp_set_comp_transform": (None, [vp, dp, dp]),
             "shp_set_vary": (None, [vp, C.c_int, C.c_int, C.c_int, C.c_int]),
@@ -543,6 +547,21 @@ class Scene:
         out = np.zeros(14)
         self.lib.shp_moments(self.mod, 0 if comp is None else 1 + comp, out.ctypes.data_as(C.POINTER(C.c_double)))
         return out[0], out[1], out[2:5].copy(), out[5:14].reshape(3, 3).copy()
+
+    def harmonic(self, a=None, b=None):
+        """the harmonic component's coefficients (a, b), (l, m) lower-triangular in
+        order; with a and b given, writes them first"""
+        out_a, out_b = np.zeros(1024), np.zeros(1024)
+        dp = C.POINTER(C.c_double)
+        if a is not None:
+            wa = np.ascontiguousarray(a, dtype=np.float64)
+            wb = np.ascontiguousarray(b, dtype=np.float64)
+            self.lib.shp_harmonic(self.mod, 1, wa.ctypes.data_as(dp), wb.ctypes.data_as(dp))
+        L = self.lib.shp_harmonic(self.mod, 0, out_a.ctypes.data_as(dp), out_b.ctypes.data_as(dp))
+        if L < 0:
+            raise ValueError("not a harmonic component")
+        n = (L + 1) * (L + 2) // 2
+        return L, out_a[:n].copy(), out_b[:n].copy()
 
     def facets(self):
         """facet corners, (nf, 3)"""
