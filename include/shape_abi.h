@@ -43,12 +43,45 @@ extern "C" {
 #define MAP 13
 #define VIEW 14
 #define AREA 15
+#define PHOTOFACETS 16
+#define PHOTOHARM 17
+#define SLICE 18
+#define ORBIT 19
+#define SPLIT 20
 
 #define NONE 0
 #define GLOBAL 1
 #define LOCAL 2
 #define BRUTE 1
 #define PERIODIC 2
+
+/* image resampling (dd_scaling, poset_scaling, poset_resample) */
+#define BLOCK 1
+#define BILINEAR 2
+#define BICUBIC 3
+#define CUBICCONV 4
+/* byte order of binary data files */
+#define BIG_ENDIAN_DATA 1
+#define LITTLE_ENDIAN_DATA 2
+/* scattering law of rendered views */
+#define OPTICALVIEW 1
+#define RADARVIEW 2
+#define LAMBERTVIEW 3
+/* joint variation of calibration-like parameters with size or spin */
+#define VARY_NONE 0
+#define VARY_SIZE 1
+#define VARY_SPIN 2
+#define VARY_ALL 3
+/* pa_ephfile formats and the scalefitobs options */
+#define HORIZONS 0
+#define DATATAKING 1
+#define SCALE_SEPARATELY 0
+#define SCALE_MAXFITOBS 1
+#define SCALE_FIT 2
+#define SCALE_OBS 3
+/* smearing: the frame epoch is the centre view or the first view */
+#define SMEAR_CENTER 0
+#define SMEAR_FIRST 1
 
 #define MAPMODE_DELDOP 0
 #define MAPMODE_POS 1
@@ -63,6 +96,11 @@ extern "C" {
 #define AU 1.4959787061e8
 #define SMALLVAL 1.0e-10
 #define TINYCALFACT 1.0e-40
+#define LN10 2.302585092994046
+#define HALFSECOND 5.8e-6
+#define DAYSPERAU 5.775518328e-3
+#define GRAVCONST 4.98232e-10
+#define SOLAR_VMAG -26.762
 
 #define SHORT 0
 #define LONG_ORIG 1

@@ -1664,3 +1664,82 @@ int shp_sides(struct mod_t *mod, int *out, int cap)
   }
   return r->ns;
 }
+
+/* ---- structs from other producers (the input-file readers) -------------- */
+
+/* sizes for callers that allocate the ABI structs themselves: 0 par_t,
+   1 mod_t, 2 dat_t */
+size_t shp_struct_size(int which)
+{
+  return which == 0 ? sizeof(struct par_t) : which == 1 ? sizeof(struct mod_t) : sizeof(struct dat_t);
+}
+
+/* the file name a reader takes from mod_t / dat_t */
+void shp_set_name(int which, void *p, const char *name)
+{
+  char *dst = which == 1 ? ((struct mod_t *)p)->name : ((struct dat_t *)p)->name;
+  const size_t cap = which == 1 ? sizeof(((struct mod_t *)0)->name) : sizeof(((struct dat_t *)0)->name);
+  snprintf(dst, cap, "%s", name);
+}
+
+/* realized vertex and facet counts of component 0 */
+int shp_mod_nv(struct mod_t *mod) { return mod->shape.comp[0].real.nv; }
+int shp_mod_nf(struct mod_t *mod) { return mod->shape.comp[0].real.nf; }
+
+/* each vertex's attached sides, flattened, with counts: returns the total */
+int shp_vertex_sides(struct mod_t *mod, int *count, int *out, int cap)
+{
+  struct vertices_t *r = &mod->shape.comp[0].real;
+  int n = 0;
+  for (int v = 0; v < r->nv; v++) {
+    count[v] = r->v[v].nas;
+    for (int q = 0; q < r->v[v].nas; q++, n++)
+      if (n < cap) out[n] = r->v[v].as[q];
+  }
+  return n;
+}
+
+/* facets' side triples */
+void shp_facet_sides(struct mod_t *mod, int *out)
+{
+  struct vertices_t *r = &mod->shape.comp[0].real;
+  for (int f = 0; f < r->nf; f++)
+    for (int k = 0; k < 3; k++) out[3 * f + k] = r->f[f].s[k];
+}
+
+/* a view's epoch and plane-of-sky spin contribution: t, orbspin[3] */
+void shp_view_epoch(struct dat_t *dat, int s, int f, int v, double out[4])
+{
+  struct set_t *set = &dat->set[s];
+  if (set->type == DELAY) {
+    struct deldopview_t *w = &set->desc.deldop.frame[f].view[v];
+    out[0] = w->t; out[1] = w->orbspin[0]; out[2] = w->orbspin[1]; out[3] = w->orbspin[2];
+  } else if (set->type == DOPPLER) {
+    struct dopview_t *w = &set->desc.doppler.frame[f].view[v];
+    out[0] = w->t; out[1] = w->orbspin[0]; out[2] = w->orbspin[1]; out[3] = w->orbspin[2];
+  } else {
+    struct crvrend_t *w = &set->desc.lghtcrv.rend[f];
+    out[0] = set->desc.lghtcrv.x[f]; out[1] = w->orbspin[0]; out[2] = w->orbspin[1]; out[3] = w->orbspin[2];
+  }
+}
+
+/* The free-parameter slots of a model read from a file, numbered as the
+   scene builder numbers its own: 0-2 component 0's size parameters (vertex
+   scale factors, or 2a, a/b, b/c of an ellipse or ovoid, or harmonic scale
+   factors), 3 the spin rate omega[2], 4-5 the first two Euler angles.
+   NULL past the end. */
+double *shp_mod_param_ptr(struct mod_t *mod, int i)
+{
+  struct comp_t *c = &mod->shape.comp[0];
+  if (i >= 0 && i < 3) {
+    switch (c->type) {
+    case VERTEX: return &c->desc.ver.scalefactor[i].val;
+    case HARMONIC: return &c->desc.har.scalefactor[i].val;
+    case ELLIPSE: return i == 0 ? &c->desc.ell.two_a.val : i == 1 ? &c->desc.ell.a_over_b.val : &c->desc.ell.b_over_c.val;
+    default: return i == 0 ? &c->desc.ovoid.two_a.val : i == 1 ? &c->desc.ovoid.a_over_b.val : &c->desc.ovoid.b_over_c.val;
+    }
+  }
+  if (i == 3) return &mod->spin.omega[2].val;
+  if (i == 4 || i == 5) return &mod->spin.angle[i - 4].val;
+  return NULL;
+}

@@ -11,6 +11,8 @@
 #ifndef SHAPE_SCENE_H
 #define SHAPE_SCENE_H
 
+#include <stddef.h>
+
 #include "shape_abi.h"
 
 #ifdef __cplusplus
@@ -176,6 +178,19 @@ void shp_dat_vary_weights(struct dat_t *dat);
 void shp_vertex_dirs(struct mod_t *mod, double *u, double *r);
 int shp_param_count(shp_scene *sc);
 int shp_param_type(shp_scene *sc, int i);
+/* structs allocated by the caller (the input-file readers fill them):
+   sizes (0 par_t, 1 mod_t, 2 dat_t), the mod_t / dat_t file name, component
+   0's realized counts, vertex side lists, facet side triples, and a view's
+   epoch with its plane-of-sky spin contribution (t, orbspin[3]; for a
+   lightcurve, calculated point f's light-time corrected epoch) */
+size_t shp_struct_size(int which);
+void shp_set_name(int which, void *p, const char *name);
+int shp_mod_nv(struct mod_t *mod);
+int shp_mod_nf(struct mod_t *mod);
+int shp_vertex_sides(struct mod_t *mod, int *count, int *out, int cap);
+void shp_facet_sides(struct mod_t *mod, int *out);
+void shp_view_epoch(struct dat_t *dat, int s, int f, int v, double out[4]);
+double *shp_mod_param_ptr(struct mod_t *mod, int i);
 /* Deep copy of the parts of a scene the forward model writes (fit images,
    frame scalars, view data) from src into dst; both built from one spec. */
 

@@ -121,6 +121,7 @@ def load_scene():
             "shp_set_impulses": (C.c_int, [vp, C.c_int, dp, dp]),
             "shp_moments": (None, [vp, C.c_int, dp]),
             "shp_sides": (C.c_int, [vp, ip, C.c_int]),
+            "shp_vertex_facets": (C.c_int, [vp, C.c_int, ip, C.c_int]),
             "shp_facets": (None, [vp, ip]),
             "shp_harmonic": (C.c_int, [vp, C.c_int, dp, dp]),
             "shp_set_vertex_r": (None, [vp, dp]),
@@ -130,6 +131,13 @@ def load_scene():
             "shp_set_state": (None, [vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double]),
             "shp_dat_vary_weights": (None, [vp]),
             "shp_vertex_dirs": (None, [vp, dp, dp]),
+            "shp_struct_size": (C.c_size_t, [C.c_int]),
+            "shp_set_name": (None, [C.c_int, vp, C.c_char_p]),
+            "shp_mod_nv": (C.c_int, [vp]), "shp_mod_nf": (C.c_int, [vp]),
+            "shp_vertex_sides": (C.c_int, [vp, ip, ip, C.c_int]),
+            "shp_facet_sides": (None, [vp, ip]),
+            "shp_view_epoch": (None, [vp, C.c_int, C.c_int, C.c_int, dp]),
+            "shp_mod_param_ptr": (dp, [vp, C.c_int]),
         }
         for name, (res, args) in sigs.items():
             fn = getattr(lib, name)
@@ -137,6 +145,45 @@ def load_scene():
             fn.argtypes = args
         _scene_lib = lib
     return _scene_lib
+
+
+_io_lib = None
+
+
+def load_io():
+    """libshape_io.so: the par / mod / obs file readers (host C)."""
+    global _io_lib
+    if _io_lib is None:
+        path = os.path.join(LIBDIR, "libshape_io.so")
+        if not os.path.exists(path):
+            raise RuntimeError("libshape_io.so is not built (run __graft_entry__.build())")
+        lib = C.CDLL(path)
+        vp, dp = C.c_void_p, C.POINTER(C.c_double)
+        sigs = {
+            "shape_read_par": (C.c_int, [C.c_char_p, vp]),
+            "shape_read_mod": (C.c_int, [vp, vp]),
+            "shape_read_dat": (C.c_int, [vp, vp, vp]),
+            "shape_io_error": (C.c_char_p, []),
+            "shape_io_free_par": (None, [vp]), "shape_io_free_mod": (None, [vp]),
+            "shape_io_free_dat": (None, [vp]),
+            "shape_io_cal2jd": (C.c_double, [C.c_int] * 6),
+            "shape_io_ephem2mat": (C.c_double, [vp, vp, C.c_double, dp, dp, dp, dp, dp, C.c_int]),
+            "setupsides": (None, [vp]), "setupvertices": (None, [vp]),
+        }
+        for name, (res, args) in sigs.items():
+            fn = getattr(lib, name)
+            fn.restype = res
+            fn.argtypes = args
+        _io_lib = lib
+    return _io_lib
+
+
+class ShapeIOError(RuntimeError):
+    """A reader refused its input; code is SHAPE_IO_E* (-1 file, -2 parse, -3 unsupported)."""
+
+    def __init__(self, code, msg):
+        super().__init__("%s (code %d)" % (msg, code))
+        self.code = code
 
 
 # shape_eval_fn: evaluate the objective at x[0..k) into f; 0 on success
@@ -504,6 +551,10 @@ class Scene:
     def nparams(self):
         return self.lib.shp_param_count(self.h)
 
+    def param_address(self, i):
+        """address of free parameter i's value (the batch's trial targets)"""
+        return C.cast(self.lib.shp_param_ptr(self.h, i), C.c_void_p).value
+
     def get_param(self, i):
         return self.lib.shp_param_ptr(self.h, i)[0]
 
@@ -677,7 +728,7 @@ class Batch:
         addrs, vals = [], []
         for m, tv in evals:
             for pi, v in tv:
-                addrs.append(C.cast(scenes[m].lib.shp_param_ptr(scenes[m].h, pi), C.c_void_p).value)
+                addrs.append(scenes[m].param_address(pi))
                 vals.append(v)
         self.nvals = len(vals)
         tarr = (C.c_void_p * max(1, len(addrs)))(*addrs)
@@ -878,3 +929,69 @@ class Batch:
             self.close()
         except Exception:
             pass
+
+
+class Job(Scene):
+    """par/mod/dat read from SHAPE's own input files (parameter, model and obs
+    files) by libshape_io.so; the same accessors as a synthetic Scene."""
+
+    def __init__(self, parfile, modfile, obsfile):
+        # mod_t and dat_t hold the file names in 80 and 40 characters
+        if len(modfile) > 79 or len(obsfile) > 39:
+            raise ValueError("model / obs file names longer than 79 / 39 characters: use a shorter "
+                             "(relative) path")
+        self.lib = load_scene()
+        self.io = load_io()
+        self.spec = None
+        self.h = None
+        self._bufs = [C.create_string_buffer(int(self.lib.shp_struct_size(w))) for w in range(3)]
+        self.par, self.mod, self.dat = (C.cast(b, C.c_void_p).value for b in self._bufs)
+        self._read = []
+        self._check(self.io.shape_read_par(parfile.encode(), self.par))
+        self._read.append(self.io.shape_io_free_par)
+        self.lib.shp_set_name(1, self.mod, modfile.encode())
+        self.nfpar_mod = self._check(self.io.shape_read_mod(self.par, self.mod))
+        self._read.append(self.io.shape_io_free_mod)
+        self.lib.shp_set_name(2, self.dat, obsfile.encode())
+        self.nfpar_dat = self._check(self.io.shape_read_dat(self.par, self.mod, self.dat))
+        self._read.append(self.io.shape_io_free_dat)
+
+    def _check(self, rc):
+        if rc < 0:
+            self.close()
+            raise ShapeIOError(rc, self.io.shape_io_error().decode())
+        return rc
+
+    def close(self):
+        roots = [self.par, self.mod, self.dat]
+        for free, root in zip(getattr(self, "_read", []), roots):
+            free(root)
+        self._read = []
+
+    @property
+    def nv(self):
+        return self.lib.shp_mod_nv(self.mod)
+
+    @property
+    def nf(self):
+        return self.lib.shp_mod_nf(self.mod)
+
+    def _param(self, i):
+        ptr = self.lib.shp_mod_param_ptr(self.mod, i)
+        if not ptr:
+            raise IndexError("no parameter slot %d" % i)
+        return ptr
+
+    def param_address(self, i):
+        return C.cast(self._param(i), C.c_void_p).value
+
+    def get_param(self, i):
+        return self._param(i)[0]
+
+    def set_param(self, i, val):
+        self._param(i)[0] = val
+
+    def view_epoch(self, s, f, v=0):
+        out = np.zeros(4)
+        self.lib.shp_view_epoch(self.dat, s, f, v, out.ctypes.data_as(C.POINTER(C.c_double)))
+        return out[0], out[1:]
