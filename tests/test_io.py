@@ -442,3 +442,32 @@ def test_library_exports_reference_reader_names():
         assert hasattr(lib, name)
     # internals stay local
     assert not hasattr(lib, "io_word")
+
+
+@pytest.mark.skipif(not os.path.isdir("/root/reference/src"), reason="reference tree not present")
+def test_every_host_caller_of_the_reader_objects_resolves():
+    """Routines that read_par.c, read_mod.c and read_dat.c define and that any
+    other C file of the reference calls (shape-cuda-v1.0.c, convex_hull.c,
+    merge_comps.c, split_mod.c) are exported by libshape_io.so, so those three
+    objects can leave a host link.  (read_dat_gpu.cu, the CUDA build's own
+    reader, calls read_dat.c's file-format helpers; it is not part of a build
+    for this device.)"""
+    import ctypes as C
+    import glob
+    import re
+    import test_abi as TA
+    lib = C.CDLL(os.path.join(sh.LIBDIR, "libshape_io.so"))
+    called = {}
+    for obj in ("read_par", "read_mod", "read_dat"):
+        src = os.path.join(TA.REF_SRC, "shape", obj + ".c")
+        defs = TA._external_definitions(src)
+        for path in glob.glob(TA.REF_SRC + "/**/*.c", recursive=True):
+            if path == src:
+                continue
+            t = TA._strip_c(open(path, errors="replace").read())
+            for name in defs:
+                if re.search(r"(?<![\w.>])%s\s*\(" % name, t):
+                    called.setdefault(name, path)
+    assert {"read_par", "read_mod", "read_dat", "setupsides", "setupvertices"} <= set(called)
+    missing = sorted(n for n in called if not hasattr(lib, n))
+    assert not missing, [(n, called[n]) for n in missing]
