@@ -40,6 +40,17 @@ void oracle_clear_error(void);
 void oracle_realize_mod(struct par_t *par, struct mod_t *mod);
 /* spin state per view: realize_spin.c:41-280 + inteuler.c:86-113 (PA rotators) */
 void oracle_realize_spin(struct par_t *par, struct mod_t *mod, struct dat_t *dat);
+/* inteuler.c:114-392 (oracle_npa.c): an NPA rotator's spin state.  The
+   integrator of one interval, its macro step count, the body-frame period of
+   w, K(k) from the complementary modulus, and the whole state from the spin
+   epoch through impulse epochs t[0..n-1] (method BRUTE or PERIODIC; 0 where
+   the period is not finite) */
+void oracle_npa_integrate(const double I[3], double t0, double t, double w[3], double m[3][3]);
+int oracle_npa_steps(const double I[3], const double w[3], double dt);
+double oracle_npa_period(const double inertia[3], const double w[3]);
+double oracle_ellip_k(double kc);
+int oracle_inteuler_npa(const struct spin_t *spin, const double t[], double imp[][3], int n, int method,
+                        double w[3], double m[3][3]);
 /* COM offsets: realize_delcor.c:50-151, deldopoffs.c:17-40, dopoffs.c:19-39 */
 void oracle_realize_delcor(struct dat_t *dat, double delta_delcor0, int delcor0_mode);
 

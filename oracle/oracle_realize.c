@@ -343,13 +343,17 @@ static void o_realize_impulse(const struct spin_t *spin, double t, double t_int[
   *n_int = k + 1;
 }
 
-/* inteuler.c:93-113: principal-axis rotator kinematics */
-static int o_inteuler_pa(const struct spin_t *spin, const double t[], double imp[][3], int n,
-                         double w[3], double m[3][3])
+/* inteuler.c:93-113: principal-axis rotator kinematics; NPA rotators by
+   Euler's equations (oracle_npa.c) */
+static int o_inteuler(const struct spin_t *spin, const double t[], double imp[][3], int n, int method,
+                      double w[3], double m[3][3])
 {
   if (!spin->pa) {
-    oracle_set_error("oracle_realize_spin: NPA rotators are not restated");
-    return 0;
+    if (!oracle_inteuler_npa(spin, t, imp, n, method, w, m)) {
+      oracle_set_error("oracle_realize_spin: the NPA spin state has no finite period");
+      return 0;
+    }
+    return 1;
   }
   double dt = t[n - 1] - t[0];
   double psi = spin->angle[2].val + spin->omega[2].val * dt + 0.5 * spin->omegadot[2].val * dt * dt +
@@ -367,12 +371,12 @@ static int o_inteuler_pa(const struct spin_t *spin, const double t[], double imp
 
 /* one view: impulses, kinematics, spin offsets, body->ecliptic, add orbital spin
    (realize_spin.c:87-128) */
-static void o_spin_view(struct mod_t *mod, struct set_t *set, double t, double t_int[],
+static void o_spin_view(struct par_t *par, struct mod_t *mod, struct set_t *set, double t, double t_int[],
                         double imp[][3], int *n_int, double intspin[3], double ae[3][3],
                         const double orbspin[3], double spin[3])
 {
   o_realize_impulse(&mod->spin, t, t_int, imp, n_int);
-  o_inteuler_pa(&mod->spin, t_int, imp, *n_int, intspin, ae);
+  o_inteuler(&mod->spin, t_int, imp, *n_int, par->int_method, intspin, ae);
   for (int j = 0; j <= 2; j++) intspin[j] += set->omegaoff[j].val;
   o_cotrans(intspin, ae, intspin, -1);
   for (int j = 0; j <= 2; j++) spin[j] = orbspin[j] + intspin[j];
@@ -381,7 +385,6 @@ static void o_spin_view(struct mod_t *mod, struct set_t *set, double t, double t
 /* realize_spin.c:41-280 (angle/omega offsets realized with '=' states, :283-342) */
 void oracle_realize_spin(struct par_t *par, struct mod_t *mod, struct dat_t *dat)
 {
-  (void)par;
   for (int j = 0; j <= 2; j++) {
     int sa = -1, so = -1;
     for (int s = 0; s < dat->nsets; s++) {
@@ -404,7 +407,7 @@ void oracle_realize_spin(struct par_t *par, struct mod_t *mod, struct dat_t *dat
         for (int k = 0; k < dd->nviews; k++) {
           struct deldopfrm_t *fr = &dd->frame[f];
           struct deldopview_t *v = &fr->view[k];
-          o_spin_view(mod, set, v->t, fr->t_integrate, fr->impulse, &fr->n_integrate, v->intspin,
+          o_spin_view(par, mod, set, v->t, fr->t_integrate, fr->impulse, &fr->n_integrate, v->intspin,
                       v->ae, v->orbspin, v->spin);
         }
     } else if (set->type == DOPPLER) {
@@ -413,14 +416,14 @@ void oracle_realize_spin(struct par_t *par, struct mod_t *mod, struct dat_t *dat
         for (int k = 0; k < dp->nviews; k++) {
           struct dopfrm_t *fr = &dp->frame[f];
           struct dopview_t *v = &fr->view[k];
-          o_spin_view(mod, set, v->t, fr->t_integrate, fr->impulse, &fr->n_integrate, v->intspin,
+          o_spin_view(par, mod, set, v->t, fr->t_integrate, fr->impulse, &fr->n_integrate, v->intspin,
                       v->ae, v->orbspin, v->spin);
         }
     } else if (set->type == LGHTCRV) {
       struct lghtcrv_t *lc = &set->desc.lghtcrv;
       for (int i = 1; i <= lc->ncalc; i++) {
         struct crvrend_t *r = &lc->rend[i];
-        o_spin_view(mod, set, lc->x[i], r->t_integrate, r->impulse, &r->n_integrate, r->intspin,
+        o_spin_view(par, mod, set, lc->x[i], r->t_integrate, r->impulse, &r->n_integrate, r->intspin,
                     r->ae, r->orbspin, r->spin);
       }
     } else {

@@ -126,7 +126,6 @@ int Engine::pack_model(int mi, struct par_t *par, struct mod_t *mod, struct dat_
   if (mod->shape.ncomp != 1) return -10;  // the engine handles single-component models
   struct comp_t *comp = &mod->shape.comp[0];
   struct vertices_t *rv = &comp->real;
-  if (!mod->spin.pa) return -11;          // principal-axis rotators only
   // ---- parameter block layout ----
   int nrad = mod->photo.nradlaws, nopt = mod->photo.noptlaws, nsets = dat->nsets;
   int off_rad = PB_SPIN_END, off_opt = off_rad + PB_RAD_STRIDE * nrad;
@@ -151,7 +150,8 @@ int Engine::pack_model(int mi, struct par_t *par, struct mod_t *mod, struct dat_
   b[PB_LIBAMP] = sp->lib_amp.val; set_param_target(tmap, &sp->lib_amp.val, mi, PB_LIBAMP);
   b[PB_LIBFREQ] = sp->lib_freq.val; set_param_target(tmap, &sp->lib_freq.val, mi, PB_LIBFREQ);
   b[PB_LIBPHASE] = sp->lib_phase.val; set_param_target(tmap, &sp->lib_phase.val, mi, PB_LIBPHASE);
-  b[PB_PA] = 1.0;
+  // 1: principal-axis kinematics; NPA by Euler's equations, 0 periodic / -1 brute
+  b[PB_PA] = sp->pa ? 1.0 : (par->int_method == BRUTE ? -1.0 : 0.0);
   for (int j = 0; j < 3; j++) {
     b[PB_INERTIA + j] = sp->inertia[j].val;
     set_param_target(tmap, &sp->inertia[j].val, mi, PB_INERTIA + j);

@@ -10,6 +10,7 @@
 #include "engine.h"
 #include "kernels.h"
 #include "fp64_dev.h"
+#include "npa_dev.h"
 
 namespace shp {
 
@@ -76,32 +77,48 @@ __global__ void k_setup_views(ViewRec *views, int nviews, const double *params) 
     for (int j = 0; j < 3; j++) ang[j] = pb[PB_ANGLE + j] + ps[PB_SET_ANGLEOFF + j];
     // impulse list from t0 to t (realize_spin.c:350-398)
     double t0 = pb[PB_T0], t = V.t;
-    double tint[MAXIMP + 2], imp[MAXIMP + 2];
+    double tint[MAXIMP + 2], imp3[MAXIMP + 2][3];
     int k = 0, nimp = (int)pb[PB_NIMP];
-    tint[0] = t0; imp[0] = 0.0;
+    tint[0] = t0; imp3[0][0] = imp3[0][1] = imp3[0][2] = 0.0;
     if (t >= t0) {
       for (int q = 0; q < nimp; q++) {
         double tq = pb[PB_TIMP + q];
-        if (tq > t0 && tq <= t) { k++; tint[k] = tq; imp[k] = pb[PB_IMP + 3 * q + 2]; }
+        if (tq > t0 && tq <= t) {
+          k++; tint[k] = tq;
+          for (int j = 0; j < 3; j++) imp3[k][j] = pb[PB_IMP + 3 * q + j];
+        }
       }
-      if (tint[k] < t) { k++; tint[k] = t; imp[k] = 0.0; }
+      if (tint[k] < t) { k++; tint[k] = t; imp3[k][0] = imp3[k][1] = imp3[k][2] = 0.0; }
     } else {
       for (int q = nimp - 1; q >= 0; q--) {
         double tq = pb[PB_TIMP + q];
-        if (tq < t0 && tq >= t) { k++; tint[k] = tq; imp[k] = -pb[PB_IMP + 3 * q + 2]; }
+        if (tq < t0 && tq >= t) {
+          k++; tint[k] = tq;
+          for (int j = 0; j < 3; j++) imp3[k][j] = -pb[PB_IMP + 3 * q + j];
+        }
       }
-      if (tint[k] > t) { k++; tint[k] = t; imp[k] = 0.0; }
+      if (tint[k] > t) { k++; tint[k] = t; imp3[k][0] = imp3[k][1] = imp3[k][2] = 0.0; }
     }
     int n = k + 1;
-    double dt = tint[n - 1] - tint[0];
-    double psi = ang[2] + pb[PB_OMEGA + 2] * dt + 0.5 * pb[PB_OMEGADOT + 2] * dt * dt +
-                 pb[PB_LIBAMP] * sin(pb[PB_LIBFREQ] * dt + pb[PB_LIBPHASE]);
     double w[3];
-    for (int i = 0; i <= 1; i++) w[i] = pb[PB_OMEGA + i] + pb[PB_OMEGADOT + i] * dt;
-    w[2] = pb[PB_OMEGA + 2] + pb[PB_OMEGADOT + 2] * dt +
-           pb[PB_LIBAMP] * pb[PB_LIBFREQ] * cos(pb[PB_LIBFREQ] * dt + pb[PB_LIBPHASE]);
-    for (int q = 0; q < n; q++) { psi += imp[q] * (tint[n - 1] - tint[q]); w[2] += imp[q]; }
-    euler2mat9(V.ae, ang[0], ang[1], psi);
+    if (pb[PB_PA] > 0.5) {
+      // principal-axis kinematics (inteuler.c:93-113)
+      double dt = tint[n - 1] - tint[0];
+      double psi = ang[2] + pb[PB_OMEGA + 2] * dt + 0.5 * pb[PB_OMEGADOT + 2] * dt * dt +
+                   pb[PB_LIBAMP] * sin(pb[PB_LIBFREQ] * dt + pb[PB_LIBPHASE]);
+      for (int i = 0; i <= 1; i++) w[i] = pb[PB_OMEGA + i] + pb[PB_OMEGADOT + i] * dt;
+      w[2] = pb[PB_OMEGA + 2] + pb[PB_OMEGADOT + 2] * dt +
+             pb[PB_LIBAMP] * pb[PB_LIBFREQ] * cos(pb[PB_LIBFREQ] * dt + pb[PB_LIBPHASE]);
+      for (int q = 0; q < n; q++) { psi += imp3[q][2] * (tint[n - 1] - tint[q]); w[2] += imp3[q][2]; }
+      euler2mat9(V.ae, ang[0], ang[1], psi);
+    } else {
+      // non-principal-axis rotation: Euler's equations (inteuler.c:114-392)
+      double om[3] = {pb[PB_OMEGA], pb[PB_OMEGA + 1], pb[PB_OMEGA + 2]};
+      double in[3] = {pb[PB_INERTIA], pb[PB_INERTIA + 1], pb[PB_INERTIA + 2]};
+      if (!npa_state(ang, om, in, tint, imp3, n, pb[PB_PA] < -0.5 ? -1 : 0, w, V.ae)) {
+        for (int q = 0; q < 9; q++) V.ae[q] = __longlong_as_double(0x7ff8000000000000LL);  // NaN: no finite period
+      }
+    }
     for (int j = 0; j < 3; j++) w[j] += ps[PB_SET_OMEGAOFF + j];
     double is[3];
     cotrans3T(is, V.ae, w);

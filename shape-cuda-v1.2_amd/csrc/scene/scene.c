@@ -1743,3 +1743,25 @@ double *shp_mod_param_ptr(struct mod_t *mod, int i)
   if (i == 4 || i == 5) return &mod->spin.angle[i - 4].val;
   return NULL;
 }
+
+/* A non-principal-axis rotator: spin vector w (rad/day, body frame, held
+   constant: state 'c'), principal moments I, and par's int_method (BRUTE or
+   PERIODIC); clears the PA flag. */
+void shp_set_npa(struct par_t *par, struct mod_t *mod, const double w[3], const double I[3], int method)
+{
+  for (int k = 0; k < 3; k++) {
+    mod->spin.omega[k].val = w[k];
+    mod->spin.inertia[k].val = I[k];
+    mod->spin.inertia[k].state = 'c';
+  }
+  mod->spin.omega[0].state = mod->spin.omega[1].state = 'c';
+  mod->spin.pa = 0;
+  par->int_method = (unsigned char)method;
+}
+
+/* spin epoch, Euler angles and spin vector: t0, angle[3], omega[3] */
+void shp_spin_state(struct mod_t *mod, double out[7])
+{
+  out[0] = mod->spin.t0;
+  for (int k = 0; k < 3; k++) { out[1 + k] = mod->spin.angle[k].val; out[4 + k] = mod->spin.omega[k].val; }
+}

@@ -191,6 +191,10 @@ int shp_vertex_sides(struct mod_t *mod, int *count, int *out, int cap);
 void shp_facet_sides(struct mod_t *mod, int *out);
 void shp_view_epoch(struct dat_t *dat, int s, int f, int v, double out[4]);
 double *shp_mod_param_ptr(struct mod_t *mod, int i);
+/* non-principal-axis spin (w rad/day, moments I, par's int_method), and the
+   spin epoch / angles / spin vector (7 doubles) */
+void shp_set_npa(struct par_t *par, struct mod_t *mod, const double w[3], const double I[3], int method);
+void shp_spin_state(struct mod_t *mod, double out[7]);
 /* Deep copy of the parts of a scene the forward model writes (fit images,
    frame scalars, view data) from src into dst; both built from one spec. */
 

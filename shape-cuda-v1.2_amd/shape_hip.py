@@ -22,6 +22,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIBDIR = os.path.join(HERE, "lib")
 
 GLOBAL, LOCAL = 1, 2
+BRUTE, PERIODIC = 1, 2  # par_t int_method
 SHORT, LONG_ORIG, LONG_MOD = 0, 1, 2
 DELAY, DOPPLER, POS, LGHTCRV = 0, 1, 2, 3
 # radar scattering law types (shape2.h's numbering)
@@ -138,6 +139,8 @@ def load_scene():
             "shp_facet_sides": (None, [vp, ip]),
             "shp_view_epoch": (None, [vp, C.c_int, C.c_int, C.c_int, dp]),
             "shp_mod_param_ptr": (dp, [vp, C.c_int]),
+            "shp_set_npa": (None, [vp, vp, dp, dp, C.c_int]),
+            "shp_spin_state": (None, [vp, dp]),
         }
         for name, (res, args) in sigs.items():
             fn = getattr(lib, name)
@@ -511,6 +514,26 @@ class Scene:
         out = np.zeros(29)
         self.lib.shp_view_data(self.dat, s, f, v, out.ctypes.data_as(C.POINTER(C.c_double)))
         return out
+
+    def view_epoch(self, s, f, v=0):
+        """a view's epoch and its plane-of-sky spin contribution (orbspin)"""
+        out = np.zeros(4)
+        self.lib.shp_view_epoch(self.dat, s, f, v, out.ctypes.data_as(C.POINTER(C.c_double)))
+        return out[0], out[1:]
+
+    def set_npa(self, omega, inertia, method="periodic"):
+        """make the model a non-principal-axis rotator: body-frame spin vector
+        omega (rad/day), principal moments, and the integration method"""
+        w = np.ascontiguousarray(omega, dtype=np.float64)
+        I = np.ascontiguousarray(inertia, dtype=np.float64)
+        dp = C.POINTER(C.c_double)
+        self.lib.shp_set_npa(self.par, self.mod, w.ctypes.data_as(dp), I.ctypes.data_as(dp),
+                             {"brute": BRUTE, "periodic": PERIODIC}[method])
+
+    def spin_state(self):
+        out = np.zeros(7)
+        self.lib.shp_spin_state(self.mod, out.ctypes.data_as(C.POINTER(C.c_double)))
+        return {"t0": out[0], "angle": out[1:4], "omega": out[4:7]}
 
     def set_view(self, s, f, v, arr):
         arr = np.ascontiguousarray(arr, dtype=np.float64)
@@ -991,7 +1014,3 @@ class Job(Scene):
     def set_param(self, i, val):
         self._param(i)[0] = val
 
-    def view_epoch(self, s, f, v=0):
-        out = np.zeros(4)
-        self.lib.shp_view_epoch(self.dat, s, f, v, out.ctypes.data_as(C.POINTER(C.c_double)))
-        return out[0], out[1:]
