@@ -90,6 +90,15 @@ int oracle_pos2doppler(struct par_t *par, struct photo_t *photo, double orbit_xo
                        int body, int set, int frm, int v);
 
 /* optical: apply_photo.c:78-309; natural cubic spline nr/spline.c, nr/splint.c */
+/* realize_photo.c:122-969 (albedo factors 1, mode 0): '=' states, harmonic
+   laws' local values at the facets' angular coordinates, bounds checks into
+   par->badphoto */
+void oracle_realize_photo(struct par_t *par, struct mod_t *mod);
+/* associated Legendre function P_l^m with the Condon-Shortley phase */
+double oracle_plm(int l, int m, double x);
+/* astro/hapke.c:41-176: Hapke's bidirectional reflectance */
+double oracle_hapke(double cosi, double cose, double phase, double w, double h, double B0, double g,
+                    double theta);
 double oracle_apply_photo(struct mod_t *mod, int ilaw, double phase, double intensityfactor,
                           struct pos_t *pos, int body, int s, int frm);
 void oracle_spline(double *x, double *y, int n, double yp1, double ypn, double *y2);

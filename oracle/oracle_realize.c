@@ -249,6 +249,13 @@ void oracle_realize_mod(struct par_t *par, struct mod_t *mod)
 {
   par->baddiam = 0;
   par->baddiam_logfactor = 0.0;
+  int harmonic_law = 0;
+  for (int l = 0; l < mod->photo.noptlaws; l++) {
+    const int t = mod->photo.opttype[l];
+    if (t == HARMLAMBERT || t == HARMLOMMEL || t == HARMHAPKE || t == HARMKAAS) harmonic_law = 1;
+  }
+  for (int l = 0; l < mod->photo.nradlaws; l++)
+    if (mod->photo.radtype[l] == HARMCOSINE_DIFF) harmonic_law = 1;
   for (int c = 0; c < mod->shape.ncomp; c++) {
     struct comp_t *comp = &mod->shape.comp[c];
     struct vertices_t *real = &comp->real;
@@ -291,6 +298,11 @@ void oracle_realize_mod(struct par_t *par, struct mod_t *mod)
       for (int j = 0; j <= 2; j++)
         real->f[f].x[j] = (real->v[real->f[f].v[0]].x[j] + real->v[real->f[f].v[1]].x[j] +
                            real->v[real->f[f].v[2]].x[j]) / 3;
+      if (harmonic_law) { /* realize_mod.c:517-545 */
+        const double *x = real->f[f].x;
+        real->f[f].theta = atan2(sqrt(x[0] * x[0] + x[1] * x[1]), x[2]);
+        real->f[f].phi = atan2(x[1], x[0]);
+      }
     }
     for (int i = 0; i < nv; i++) {
       double n[3] = {0.0, 0.0, 0.0};

@@ -751,23 +751,28 @@ extern "C" void realize_mod(struct par_t *par, struct mod_t *mod) {
       af_off[i + 1] = af_off[i] + rv->v[i].naf;
       for (int q = 0; q < rv->v[i].naf; q++) af_idx.push_back(rv->v[i].af[q]);
     }
-    std::vector<double> hb((size_t)nv * nbasis);
-    for (int i = 0; i < nv && nbasis; i++) {
-      double *h = &hb[(size_t)i * nbasis];
-      for (int l = 0, q = 0; l <= nhar; l++) {
-        h[q++] = rv->v[i].afactor[l][0];
-        for (int m = 1; m <= l; m++) { h[q++] = rv->v[i].afactor[l][m]; h[q++] = rv->v[i].bfactor[l][m]; }
-      }
-    }
     B.mod = nullptr;
     ok = B.fv.get(sizeof(int) * fv.size()) == 0 && B.off.get(sizeof(int) * af_off.size()) == 0 &&
          B.idx.get(sizeof(int) * af_idx.size()) == 0 &&
          hipMemcpy(B.fv.p, fv.data(), sizeof(int) * fv.size(), hipMemcpyHostToDevice) == hipSuccess &&
          hipMemcpy(B.off.p, af_off.data(), sizeof(int) * af_off.size(), hipMemcpyHostToDevice) == hipSuccess &&
          (af_idx.empty() ||
-          hipMemcpy(B.idx.p, af_idx.data(), sizeof(int) * af_idx.size(), hipMemcpyHostToDevice) == hipSuccess) &&
-         (hb.empty() || hipMemcpy(B.hb.p, hb.data(), sizeof(double) * hb.size(), hipMemcpyHostToDevice) == hipSuccess);
+          hipMemcpy(B.idx.p, af_idx.data(), sizeof(int) * af_idx.size(), hipMemcpyHostToDevice) == hipSuccess);
     if (ok) { B.mod = mod; B.topo = topo; }
+  }
+  if (nbasis) {
+    // the harmonic factors every call: the topology key does not cover the
+    // degree or the vertex directions they come from, and the buffer may
+    // have grown (moved) since the last upload
+    std::vector<double> hb((size_t)nv * nbasis);
+    for (int i = 0; i < nv; i++) {
+      double *h = &hb[(size_t)i * nbasis];
+      for (int l = 0, q = 0; l <= nhar; l++) {
+        h[q++] = rv->v[i].afactor[l][0];
+        for (int m = 1; m <= l; m++) { h[q++] = rv->v[i].afactor[l][m]; h[q++] = rv->v[i].bfactor[l][m]; }
+      }
+    }
+    ok = ok && hipMemcpy(B.hb.p, hb.data(), sizeof(double) * hb.size(), hipMemcpyHostToDevice) == hipSuccess;
   }
   // the slot names every buffer (they may have moved when they grew) and the
   // component's type, degree and ties
@@ -824,9 +829,22 @@ extern "C" void realize_mod(struct par_t *par, struct mod_t *mod) {
     for (int k = 0; k < 3; k++) { rv->v[i].x[k] = x[3 * i + k]; rv->v[i].n[k] = vn[3 * i + k]; }
     if (ctype != VERTEX) rv->v[i].r.val = rr[i];  // the analytic component's realized distance
   }
+  // a harmonic scattering law reads each facet's angular coordinates
+  // (realize_mod.c:517-545)
+  bool harm_law = false;
+  for (int l = 0; l < mod->photo.noptlaws; l++) {
+    const int t = mod->photo.opttype[l];
+    harm_law = harm_law || t == HARMLAMBERT || t == HARMLOMMEL || t == HARMHAPKE || t == HARMKAAS;
+  }
+  for (int l = 0; l < mod->photo.nradlaws; l++) harm_law = harm_law || mod->photo.radtype[l] == HARMCOSINE_DIFF;
   for (int f = 0; f < nf; f++) {
     for (int k = 0; k < 3; k++) { rv->f[f].n[k] = fn[3 * f + k]; rv->f[f].x[k] = fx[3 * f + k]; }
     rv->f[f].area = ar[f];
+    if (harm_law) {
+      const double *c = rv->f[f].x;
+      rv->f[f].theta = atan2(sqrt(c[0] * c[0] + c[1] * c[1]), c[2]);
+      rv->f[f].phi = atan2(c[1], c[0]);
+    }
   }
   for (int j = 0; j < 3; j++) {
     double sf = 1.0;  // ELLIPSE, OVOID

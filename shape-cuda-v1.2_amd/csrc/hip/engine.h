@@ -101,6 +101,10 @@ struct ViewRec {
   int pb_set, pb_law;    // offsets (within the block) of the set and scattering-law params
   int pb_dc;             // offset of the set block whose delay polynomial applies (= states)
   int law_type;
+  // per-evaluation resolved law values (realize_photo): a harmonic or
+  // inhomogeneous law's lawk values per facet, or a tabular law's rho table
+  const double *lawloc;
+  int lawk;
   double kmpp, t;
   double oe[9], se[9], orbspin[3];
   double ae[9], spin[3], intspin[3], deloff, dopoff;
@@ -162,6 +166,26 @@ struct EvalRec {
   double chi2, objective;
   int posbnd, badradar, baddiam;
   double posbnd_logfactor, badradar_logfactor, baddiam_logfactor;
+  int photo_job0, photo_njobs;  // this evaluation's realize_photo jobs
+  int badphoto;
+  double badphoto_logfactor;
+};
+
+// realize_photo on the device (k_realize_photo): one job per scattering law
+// per evaluation.  Kinds: 0 a homogeneous law (its parameters bounds-checked),
+// 1 harmonic (local values from the expansions at the facets' angular
+// coordinates), 2 inhomogeneous ('=' locals take the global values),
+// 3 tabular ('=' values take the previous one).
+constexpr int PHOTO_HOMOG = 0, PHOTO_HARM = 1, PHOTO_INHO = 2, PHOTO_TABLE = 3;
+struct PhotoJob {
+  int kind, type, k, n;  // k values per facet; n facets (kinds 1, 2) or table length (3)
+  int slot;              // geometry slot whose facet centers kind 1 reads
+  long long data;        // absolute parameter offset of the law's data region (kinds 1-3)
+  long long loc;         // offset of its resolved values in the local-value arena
+  int chk[5], nchk;      // kind 0: absolute parameter offsets checked, in the reference's order
+  int coef[5], nhar[5];  // kind 1: each expansion's offset (from data) and degree
+  double lo[5], hi[5];   // bounds and checkphotopar mode per value
+  int mode[5];
 };
 
 }  // namespace shp

@@ -59,7 +59,19 @@ struct ModelPack {
   int nv = 0, nf = 0;
   int set_off = 0, nsets = 0;  // the datasets' blocks in the parameter block
   // realized geometry from the host (drop-in mode)
-  std::vector<double> hx, hvn, hfn;
+  std::vector<double> hx, hvn, hfn, hfx;
+  // scattering laws as realize_photo sees them: the job template of each
+  // law, its parameter-block data and its share of the evaluation's
+  // resolved values (loc_len doubles per evaluation in all)
+  struct LawPack {
+    int pb_law;          // the law's header slots in the parameter block
+    int has_loc;         // the views of this law read resolved values
+    PhotoJob job;        // data / loc / chk relative to the block / the evaluation
+    int data_len, loc_len;
+  };
+  std::vector<LawPack> laws;
+  long long loc_len = 0;
+  int photo_off = 0;  // start of the laws' data regions (after the component block)
   // device topology
   int *d_fv = nullptr, *d_af_off = nullptr, *d_af_idx = nullptr;
   double *d_u = nullptr, *d_a = nullptr;
@@ -134,7 +146,114 @@ int Engine::pack_model(int mi, struct par_t *par, struct mod_t *mod, struct dat_
   if (ctype != VERTEX && ctype != ELLIPSE && ctype != OVOID && ctype != HARMONIC) return -10;
   const int nhar = ctype == HARMONIC ? comp->desc.har.nhar : 0;
   const int nbasis = ctype == HARMONIC ? (nhar + 1) * (nhar + 1) : 0;
-  int size = off_comp + PB_COMP_STRIDE + nbasis;
+  // ---- scattering laws (realize_photo): job templates and data regions ----
+  const int nf_law = rv->nf;
+  int off_photo = off_comp + PB_COMP_STRIDE + nbasis, photo_len = 0;
+  P.laws.clear();
+  P.loc_len = 0;
+  auto add_law = [&](int radar, int l) {
+    ModelPack::LawPack L;
+    memset(&L, 0, sizeof(L));
+    PhotoJob &J = L.job;
+    const int t = radar ? mod->photo.radtype[l] : mod->photo.opttype[l];
+    L.pb_law = radar ? off_rad + PB_RAD_STRIDE * l : off_opt + PB_OPT_STRIDE * l;
+    J.type = t;
+    auto bounds = [&](int j, double lo, double hi, int mode) { J.lo[j] = lo; J.hi[j] = hi; J.mode[j] = mode; };
+    auto chk = [&](int slot, double lo, double hi, int mode) {
+      bounds(J.nchk, lo, hi, mode);
+      J.chk[J.nchk++] = L.pb_law + slot;
+    };
+    if (t == NOLAW) return;
+    if (radar) {
+      J.k = 2;
+      bounds(0, par->rad_R_min, par->rad_R_max, 3);
+      bounds(1, par->rad_C_min, par->rad_C_max, 3);
+      switch (t) {
+      case COSINELAW_DIFF: case GAUSSIANLAW: case HAGFORSLAW: case COSINELAW_QS:
+        J.kind = PHOTO_HOMOG; J.nchk = 0;
+        chk(1, par->rad_R_min, par->rad_R_max, 3);
+        chk(2, par->rad_C_min, par->rad_C_max, 3);
+        break;
+      case GAUSSIAN_COSINE: case HAGFORS_COSINE: case COSINE_COSINE:  // qs R, qs C, diffuse R, C
+        J.kind = PHOTO_HOMOG; J.nchk = 0;
+        chk(4, par->rad_R_min, par->rad_R_max, 3);
+        chk(5, par->rad_C_min, par->rad_C_max, 3);
+        chk(1, par->rad_R_min, par->rad_R_max, 3);
+        chk(2, par->rad_C_min, par->rad_C_max, 3);
+        break;
+      case TABULARLAW:
+        J.kind = PHOTO_TABLE; J.k = 1; J.n = mod->photo.radar[l].tabular.n;
+        bounds(0, par->rad_rho_min, par->rad_rho_max, 3);
+        L.data_len = 2 * J.n; L.loc_len = J.n;
+        break;
+      case HARMCOSINE_DIFF: {
+        J.kind = PHOTO_HARM; J.n = nf_law;
+        struct harmonic_t *h = &mod->photo.radar[l].harmcosine.R;
+        for (int j = 0; j < 2; j++) {
+          J.nhar[j] = h[j].nhar; J.coef[j] = L.data_len;
+          L.data_len += (h[j].nhar + 1) * (h[j].nhar + 1);
+        }
+        L.loc_len = 2 * nf_law;
+        break;
+      }
+      case INHOCOSINE_DIFF:
+        J.kind = PHOTO_INHO; J.n = nf_law;
+        L.data_len = 2 + 4 * nf_law; L.loc_len = 2 * nf_law;
+        break;
+      default:
+        return;  // not evaluated by the engine; refused if a set uses it
+      }
+    } else {
+      const int hk = (t == HAPKE || t == HARMHAPKE || t == INHOHAPKE);
+      const int kk = (t == KAASALAINEN || t == HARMKAAS || t == INHOKAAS);
+      J.k = (hk || kk) ? 5 : 1;
+      if (hk) {
+        bounds(0, par->opt_w_min, par->opt_w_max, 3);
+        bounds(1, par->opt_h_min, par->opt_h_max, 3);
+        bounds(2, par->opt_B0_min, par->opt_B0_max, 2);
+        bounds(3, par->opt_g_min, par->opt_g_max, 3);
+        bounds(4, par->opt_theta_min, par->opt_theta_max, 2);
+      } else if (kk) {
+        bounds(0, par->opt_R_min, par->opt_R_max, 3);
+        bounds(1, par->opt_wt_min, par->opt_wt_max, 0);
+        bounds(2, par->opt_A0_min, par->opt_A0_max, 2);
+        bounds(3, par->opt_D_min, par->opt_D_max, 3);
+        bounds(4, par->opt_k_min, par->opt_k_max, 1);
+      } else {
+        bounds(0, par->opt_R_min, par->opt_R_max, 3);
+      }
+      if (t == GEOMETRICAL || t == LAMBERTLAW || t == LOMMEL || t == HAPKE || t == KAASALAINEN) {
+        J.kind = PHOTO_HOMOG;
+        const PhotoJob b = J;
+        J.nchk = 0;
+        for (int j = 0; j < b.k; j++) chk(1 + j, b.lo[j], b.hi[j], b.mode[j]);
+      } else if (t == HARMLAMBERT || t == HARMLOMMEL || t == HARMHAPKE || t == HARMKAAS) {
+        J.kind = PHOTO_HARM; J.n = nf_law;
+        union optscat_t *o = &mod->photo.optical[l];
+        struct harmonic_t *h = hk ? &o->harmhapke.w : kk ? &o->harmkaas.R : &o->harmR.R;
+        for (int j = 0; j < J.k; j++) {
+          J.nhar[j] = h[j].nhar; J.coef[j] = L.data_len;
+          L.data_len += (h[j].nhar + 1) * (h[j].nhar + 1);
+        }
+        L.loc_len = J.k * nf_law;
+      } else if (t == INHOLAMBERT || t == INHOLOMMEL || t == INHOHAPKE || t == INHOKAAS) {
+        J.kind = PHOTO_INHO; J.n = nf_law;
+        L.data_len = J.k + 2 * J.k * nf_law; L.loc_len = J.k * nf_law;
+      } else {
+        return;
+      }
+    }
+    L.has_loc = L.loc_len > 0;
+    J.data = off_photo + photo_len;  // relative to the block until the evaluation's is known
+    J.loc = P.loc_len;               // relative to the evaluation's resolved values
+    photo_len += L.data_len;
+    P.loc_len += L.loc_len;
+    P.laws.push_back(L);
+  };
+  for (int l = 0; l < nopt; l++) add_law(0, l);  // realize_photo's order: optical, then radar
+  for (int l = 0; l < nrad; l++) add_law(1, l);
+  int size = off_photo + photo_len;
+  P.photo_off = off_photo;
   P.base.assign(size, 0.0);
   P.comp_off = off_comp;
   P.set_off = off_set;
@@ -188,6 +307,11 @@ int Engine::pack_model(int mi, struct par_t *par, struct mod_t *mod, struct dat_
       set_param_target(tmap, &r->hybrid.qs.R.val, mi, off_rad + PB_RAD_STRIDE * l + 4);
       set_param_target(tmap, &r->hybrid.qs.C.val, mi, off_rad + PB_RAD_STRIDE * l + 5);
       break;
+    case TABULARLAW:
+      pl[2] = r->tabular.n;
+      break;
+    case HARMCOSINE_DIFF: case INHOCOSINE_DIFF:
+      break;
     default:
       pl[0] = -1;  // not evaluated by the engine; rejected below if a set uses it
     }
@@ -208,8 +332,78 @@ int Engine::pack_model(int mi, struct par_t *par, struct mod_t *mod, struct dat_
       set_param_target(tmap, &o->kaas.A0.val, mi, base + 3);
       set_param_target(tmap, &o->kaas.D.val, mi, base + 4);
       set_param_target(tmap, &o->kaas.k.val, mi, base + 5);
-    } else {
+    } else if (t == HAPKE) {
+      struct param_t *q = &o->hapke.w;
+      for (int j = 0; j < 5; j++) {
+        po[1 + j] = q[j].val;
+        set_param_target(tmap, &q[j].val, mi, base + 1 + j);
+      }
+    } else if (t < HARMLAMBERT || t > INHOKAAS) {
       po[0] = -1;
+    }
+  }
+  // the laws' data regions: expansion coefficients in the model file's order;
+  // an inhomogeneous law's k global values, then k per facet, then k tie flags
+  // per facet ('=' state); a tabular law's values, then its tie flags
+  for (auto &L : P.laws) {
+    const PhotoJob &J = L.job;
+    const int d0 = (int)J.data;
+    double *d = b + d0;
+    const bool radar = L.pb_law < off_opt;
+    const int l = radar ? (L.pb_law - off_rad) / PB_RAD_STRIDE : (L.pb_law - off_opt) / PB_OPT_STRIDE;
+    auto put = [&](int q, struct param_t *p) {
+      d[q] = p->val;
+      set_param_target(tmap, &p->val, mi, d0 + q);
+    };
+    if (J.kind == PHOTO_TABLE) {
+      struct tabular_t *tb = &mod->photo.radar[l].tabular;
+      for (int i = 0; i < J.n; i++) {
+        put(i, &tb->rho[i]);
+        d[J.n + i] = tb->rho[i].state == '=' ? 1.0 : 0.0;
+      }
+    } else if (J.kind == PHOTO_HARM) {
+      struct harmonic_t *h;
+      if (radar) h = &mod->photo.radar[l].harmcosine.R;
+      else {
+        union optscat_t *o = &mod->photo.optical[l];
+        const int t = J.type;
+        h = t == HARMHAPKE ? &o->harmhapke.w : t == HARMKAAS ? &o->harmkaas.R : &o->harmR.R;
+      }
+      for (int j = 0; j < J.k; j++) {
+        int q = J.coef[j];
+        for (int ll = 0; ll <= h[j].nhar; ll++) {
+          put(q++, &h[j].a[ll][0]);
+          for (int m = 1; m <= ll; m++) {
+            put(q++, &h[j].a[ll][m]);
+            put(q++, &h[j].b[ll][m]);
+          }
+        }
+      }
+    } else if (J.kind == PHOTO_INHO) {
+      struct param_t *g;
+      char *loc;
+      size_t elem;
+      if (radar) {
+        g = &mod->photo.radar[l].inhocosine.global.R;
+        loc = (char *)mod->photo.radar[l].inhocosine.local[0];
+        elem = sizeof(struct RC_t);
+      } else {
+        union optscat_t *o = &mod->photo.optical[l];
+        const int t = J.type;
+        g = t == INHOHAPKE ? &o->inhohapke.global.w : t == INHOKAAS ? &o->inhokaas.global.R : &o->inhoR.global.R;
+        loc = t == INHOHAPKE ? (char *)o->inhohapke.local[0] : t == INHOKAAS ? (char *)o->inhokaas.local[0]
+                                                                               : (char *)o->inhoR.local[0];
+        elem = t == INHOHAPKE ? sizeof(struct hapke_t) : t == INHOKAAS ? sizeof(struct kaas_t) : sizeof(struct R_t);
+      }
+      const int k = J.k;
+      for (int j = 0; j < k; j++) put(j, &g[j]);
+      for (int f = 0; f < J.n; f++) {
+        struct param_t *p = (struct param_t *)(loc + elem * f);
+        for (int j = 0; j < k; j++) {
+          put(k + k * f + j, &p[j]);
+          d[k + k * J.n + k * f + j] = p[j].state == '=' ? 1.0 : 0.0;
+        }
+      }
     }
   }
   // sets: offsets ('=' states of the delay polynomial resolved to the source set)
@@ -318,8 +512,9 @@ int Engine::pack_model(int mi, struct par_t *par, struct mod_t *mod, struct dat_
     P.hx.resize(3 * (size_t)P.nv); P.hvn.resize(3 * (size_t)P.nv); P.hfn.resize(3 * (size_t)P.nf);
     for (int i = 0; i < P.nv; i++)
       for (int k = 0; k < 3; k++) { P.hx[3 * i + k] = rv->v[i].x[k]; P.hvn[3 * i + k] = rv->v[i].n[k]; }
+    P.hfx.resize(3 * (size_t)P.nf);
     for (int f = 0; f < P.nf; f++)
-      for (int k = 0; k < 3; k++) P.hfn[3 * f + k] = rv->f[f].n[k];
+      for (int k = 0; k < 3; k++) { P.hfn[3 * f + k] = rv->f[f].n[k]; P.hfx[3 * f + k] = rv->f[f].x[k]; }
   }
   // ---- views and frames ----
   auto fill_view_common = [&](ViewRec &V, struct pos_t *pos) {
@@ -343,7 +538,7 @@ int Engine::pack_model(int mi, struct par_t *par, struct mod_t *mod, struct dat_
       struct doppler_t *Q = &set->desc.doppler;
       int nfr = dd ? D->nframes : Q->nframes, nv = dd ? D->nviews : Q->nviews, v0 = dd ? D->v0 : Q->v0;
       int ilaw = dd ? D->iradlaw : Q->iradlaw;
-      if (mod->photo.radtype[ilaw] < COSINELAW_DIFF || mod->photo.radtype[ilaw] > COSINE_COSINE) return -13;
+      if (mod->photo.radtype[ilaw] < COSINELAW_DIFF || mod->photo.radtype[ilaw] > TABULARLAW) return -13;
       for (int f = 0; f < nfr; f++) {
         FrameRec F;
         memset(&F, 0, sizeof(F));
@@ -418,7 +613,7 @@ int Engine::pack_model(int mi, struct par_t *par, struct mod_t *mod, struct dat_
     } else if (set->type == LGHTCRV) {
       struct lghtcrv_t *L = &set->desc.lghtcrv;
       int t = mod->photo.opttype[L->ioptlaw];
-      if (!(t == LAMBERTLAW || t == LOMMEL || t == GEOMETRICAL || t == KAASALAINEN)) return -14;
+      if (t < GEOMETRICAL || t > INHOKAAS) return -14;
       FrameRec F;
       memset(&F, 0, sizeof(F));
       F.type = VT_LGHTCRV;
@@ -519,6 +714,9 @@ int Engine::build(struct par_t *par, struct mod_t **mods, struct dat_t **dats, i
   h_frames.clear();
   h_evals.clear();
   long long fit_cursor = 0, pos_cursor = 0, y_cursor = 0, y2_cursor = 0, tile_cursor = 0, bin_cursor = 0;
+  h_photo_jobs.clear();
+  lawloc_len = 0;
+  std::vector<long long> view_loc;  // per view: offset of its law's resolved values, or -1
   long long lc_rec_cursor = 0;
   std::vector<long long> model_obs_base(nmodels, 0), model_x_base(nmodels, 0), model_t_base(nmodels, 0);
   long long ob = 0, xb = 0, tb = 0;
@@ -551,7 +749,7 @@ int Engine::build(struct par_t *par, struct mod_t **mods, struct dat_t **dats, i
       eval_targets[e].push_back(T);
       if (T.kind == 0) {
         h_params[pb + T.offset] = S.values[q];
-        if (T.offset >= P.comp_off) own_slot = true;
+        if (T.offset >= P.comp_off && T.offset < P.photo_off) own_slot = true;
       } else {
         if (!own_slot || r.empty()) r = P.r;
         own_slot = true;
@@ -580,6 +778,18 @@ int Engine::build(struct par_t *par, struct mod_t **mods, struct dat_t **dats, i
     E.bad_objfactor = par->bad_objfactor;
     E.pen_set_off = P.set_off;
     E.pen_nsets = P.nsets;
+    const long long loc_base = lawloc_len;
+    E.photo_job0 = (int)h_photo_jobs.size();
+    E.photo_njobs = (int)P.laws.size();
+    for (const auto &L : P.laws) {
+      PhotoJob J = L.job;
+      J.data += pb;
+      J.loc += loc_base;
+      J.slot = h_eval_slot[e];
+      for (int q = 0; q < J.nchk; q++) J.chk[q] += pb;
+      h_photo_jobs.push_back(J);
+    }
+    lawloc_len += P.loc_len;
     h_evals.push_back(E);
     int view_base = (int)h_views.size(), frame_base = (int)h_frames.size();
     for (size_t q = 0; q < P.views.size(); q++) {
@@ -595,6 +805,11 @@ int Engine::build(struct par_t *par, struct mod_t **mods, struct dat_t **dats, i
       tile_cursor += (long long)((2 * V.n + STALE_TILE) / STALE_TILE) * ((2 * V.n + STALE_TILE) / STALE_TILE);
       V.binned = 0; V.bin_ntx = 0; V.bin_off = 0;
       V.lc_rec_off = 0; V.lc_rec_cap = 0;
+      V.lawloc = nullptr; V.lawk = 0;
+      long long vloc = -1;
+      for (const auto &L : P.laws)
+        if (L.pb_law == V.pb_law && L.has_loc) { vloc = loc_base + L.job.loc; V.lawk = L.job.k; }
+      view_loc.push_back(vloc);
       if (opt.method == METHOD_MIXED && V.type == VT_LGHTCRV) {
         // a range of the replay records per lightcurve view: its records are
         // reserved on a counter of its own (one shared counter serializes)
@@ -682,6 +897,16 @@ int Engine::build(struct par_t *par, struct mod_t **mods, struct dat_t **dats, i
     if (opt.host_geometry && (rc = upload_host_geometry(s))) return rc;
   }
   if ((rc = setup_penalties(par))) return rc;
+  // realize_photo: resolved values, jobs and their outputs
+  if ((rc = alloc((void **)&d_lawloc, sizeof(double) * (size_t)std::max(1LL, lawloc_len)))) return rc;
+  SHP_CHECK(hipMemset(d_lawloc, 0, sizeof(double) * (size_t)std::max(1LL, lawloc_len)));
+  for (int q = 0; q < nviews; q++)
+    if (view_loc[q] >= 0) h_views[q].lawloc = d_lawloc + view_loc[q];
+  if (!h_photo_jobs.empty()) {
+    if ((rc = upload(this, &d_photo_jobs, h_photo_jobs))) return rc;
+    if ((rc = alloc((void **)&d_job_flag, sizeof(int) * h_photo_jobs.size()))) return rc;
+    if ((rc = alloc((void **)&d_job_log, sizeof(double) * h_photo_jobs.size()))) return rc;
+  }
   if ((rc = upload(this, &d_slots, h_slots))) return rc;
   if ((rc = upload(this, &d_views, h_views))) return rc;
   if ((rc = upload(this, &d_frames, h_frames))) return rc;
@@ -847,6 +1072,7 @@ int Engine::upload_host_geometry(int s) {
   SHP_CHECK(hipMemcpy(G.x, P.hx.data(), sizeof(double) * 3 * G.nv, hipMemcpyHostToDevice));
   SHP_CHECK(hipMemcpy(G.vn, P.hvn.data(), sizeof(double) * 3 * G.nv, hipMemcpyHostToDevice));
   SHP_CHECK(hipMemcpy(G.fn, P.hfn.data(), sizeof(double) * 3 * G.nf, hipMemcpyHostToDevice));
+  SHP_CHECK(hipMemcpy(G.fx, P.hfx.data(), sizeof(double) * 3 * G.nf, hipMemcpyHostToDevice));
   std::vector<float4> t4(std::max(G.nv, G.nf));
   for (int i = 0; i < G.nv; i++) t4[i] = make_float4((float)P.hx[3 * i], (float)P.hx[3 * i + 1], (float)P.hx[3 * i + 2], 0.f);
   SHP_CHECK(hipMemcpy(G.x32, t4.data(), sizeof(float4) * G.nv, hipMemcpyHostToDevice));
@@ -869,8 +1095,22 @@ int Engine::refresh(struct par_t *par, struct mod_t *mod, struct dat_t *dat) {
   ModelPack &P = g_packs[0];
   const GeomSlot &G0 = h_slots[h_eval_slot[0]];
   bool same = (int)P.views.size() == nviews && (int)P.frames.size() == nframes && (int)P.base.size() <= pb_stride &&
+              P.loc_len == lawloc_len && P.laws.size() == h_photo_jobs.size() &&
               P.nv == G0.nv && P.nf == G0.nf && (long long)P.lc_x.size() == lcx_len &&
               (long long)P.lc_t.size() == lct_len;
+  // the laws' jobs as a fresh build would make them (same kinds, offsets, bounds)
+  for (size_t q = 0; same && q < P.laws.size(); q++) {
+    PhotoJob J = P.laws[q].job;
+    const PhotoJob &O = h_photo_jobs[q];
+    J.data += h_evals[0].pb;
+    J.slot = O.slot;
+    for (int c = 0; c < J.nchk; c++) J.chk[c] += h_evals[0].pb;
+    same = J.kind == O.kind && J.type == O.type && J.k == O.k && J.n == O.n && J.data == O.data && J.loc == O.loc &&
+           J.nchk == O.nchk;
+    for (int c = 0; same && c < 5; c++)
+      same = J.chk[c] == O.chk[c] && J.coef[c] == O.coef[c] && J.nhar[c] == O.nhar[c] && J.lo[c] == O.lo[c] &&
+             J.hi[c] == O.hi[c] && J.mode[c] == O.mode[c];
+  }
   for (int q = 0; same && q < nviews; q++) {
     const ViewRec &A = h_views[q], &B = P.views[q];
     same = A.type == B.type && A.n == B.n && A.ndel == B.ndel && A.ndop == B.ndop && A.nviews == B.nviews &&
@@ -892,6 +1132,7 @@ int Engine::refresh(struct par_t *par, struct mod_t *mod, struct dat_t *dat) {
     V.eval = O.eval; V.slot = O.slot; V.pb = O.pb; V.frame_id = O.frame_id; V.pos_off = O.pos_off;
     V.tile_off = O.tile_off; V.bin_off = O.bin_off; V.bin_ntx = O.bin_ntx; V.binned = O.binned;
     V.fp64_skip = O.fp64_skip;
+    V.lawloc = O.lawloc; V.lawk = O.lawk;
     h_views[q] = V;
   }
   for (int q = 0; q < nframes; q++) {
@@ -1025,6 +1266,7 @@ int Engine::enqueue() {
     if (fm) SHP_LAUNCH(*this, "k_fill_i32", k_fill_i32, dim3(1024), dim3(256), 0, stream, fm, std::max(1LL, total_pix), -1);
   }
   SHP_LAUNCH(*this, "k_setup_views", k_setup_views, gv, bv, 0, stream, d_views, nviews, d_params);
+  const bool photo = !h_photo_jobs.empty() && !opt.views_only;
   if (!opt.host_geometry) {
     int max_nv = 0;
     for (auto &G : h_slots) max_nv = std::max(max_nv, G.nv);
@@ -1036,6 +1278,9 @@ int Engine::enqueue() {
     SHP_LAUNCH(*this, "k_realize_facets", k_realize_facets, dim3(cdiv(max_nf, T), nslots), dim3(T), 0, stream, d_slots, nslots);
     SHP_LAUNCH(*this, "k_realize_vnormals", k_realize_vnormals, dim3(cdiv(max_nv, T), nslots), dim3(T), 0, stream, d_slots, nslots);
   }
+  if (photo)
+    SHP_LAUNCH(*this, "k_realize_photo", k_realize_photo, dim3((unsigned)h_photo_jobs.size()), dim3(T), 0, stream,
+               d_photo_jobs, d_params, d_slots, d_lawloc, d_job_flag, d_job_log);
   enqueue_penalties();
   if (opt.method == METHOD_MIXED) return enqueue_mixed(*this);
   dim3 gf(cdiv(max_nf, T), nviews), gp(cdiv(max_pix, T), nviews), bt(T);
@@ -1066,7 +1311,8 @@ int Engine::enqueue() {
   SHP_LAUNCH(*this, "k_chi2_frames64", k_chi2_frames64, dim3(nframes), bt, 0, stream, d_frames, nframes, d_fit, d_obs, d_oov,
                      d_view_out, opt.keep_fit ? 0 : 1, 0, 0);
   SHP_LAUNCH(*this, "k_objective", k_objective, dim3(std::max(1, nevals)), dim3(256), 0, stream, d_evals, nevals, d_frames, d_views,
-                     d_view_out, d_slot_flags, d_slot_logf, d_eval_slot, pen.n ? pen.total : nullptr);
+                     d_view_out, d_slot_flags, d_slot_logf, d_eval_slot, pen.n ? pen.total : nullptr, d_job_flag,
+                     d_job_log);
   SHP_CHECK(hipGetLastError());
   return 0;
 }

@@ -146,6 +146,13 @@ class Engine {
   GeomSlot *d_slots = nullptr;
   int *d_eval_slot = nullptr, *d_slot_flags = nullptr;
   double *d_slot_logf = nullptr;
+  // realize_photo: jobs, their bad-photometry outputs, the resolved values
+  std::vector<PhotoJob> h_photo_jobs;
+  PhotoJob *d_photo_jobs = nullptr;
+  int *d_job_flag = nullptr;
+  double *d_job_log = nullptr;
+  double *d_lawloc = nullptr;
+  long long lawloc_len = 0;
   double *d_params = nullptr, *d_fit = nullptr, *d_obs = nullptr, *d_oov = nullptr;
   double *d_overflow = nullptr, *d_view_out = nullptr;
   double *d_lc_x = nullptr, *d_lc_t = nullptr, *d_lc_y = nullptr, *d_lc_y2 = nullptr;

@@ -161,4 +161,108 @@ __device__ inline double radlaw64(const double *pl, int type, double c) {
   return d;
 }
 
+// radlaw with the laws whose values are per facet or tabulated
+// (radlaw.c:56-66, :118-135): loc = the evaluation's resolved values (a
+// tabular law's rho[0..n-1]; harmcosine / inhocosine: R, C per facet), f the
+// pixel's facet.  The table index is clamped to n - 2: at cos(inc) below
+// ~1e-16 the reference would read rho[n].
+__device__ inline double radlaw64x(const double *pl, int type, double c, const double *loc, int f) {
+#pragma clang fp contract(off)
+  if (type == TABULARLAW) {
+    if (f < 0) return 0.0;
+    const int n = (int)pl[2];
+    const double inc = acos(c), angres = (3.141592653589793 / 2) / (n - 1), ratio = inc / angres;
+    int i = (int)floor(ratio);
+    if (i > n - 2) i = n - 2;
+    const double r1 = loc[i], r2 = loc[i + 1];
+    return (r1 + (ratio - i) * (r2 - r1)) / c;
+  }
+  if (type == HARMCOSINE_DIFF || type == INHOCOSINE_DIFF) {
+    if (f < 0) return 0.0;
+    const double R = loc[2 * f], C = loc[2 * f + 1];
+    return R * (C + 1) * pow(c, 2 * C - 1);
+  }
+  return radlaw64(pl, type, c);
+}
+
+// Hapke's bidirectional reflectance (astro/hapke.c:41-176; Hapke 1984 with
+// the Henyey-Greenstein phase function, the opposition surge, the
+// macroscopic roughness correction and Hapke 2002's H-function
+// approximation); 0 for parameters out of range, as the reference gives
+__device__ inline double hapke64(double cosi, double cose, double phase, double w, double h, double B0, double g,
+                                 double theta) {
+#pragma clang fp contract(off)
+  const double pi = 4 * atan(1.0);
+  if (w <= 0.0 || w >= 1.0 || h <= 0.0 || B0 < 0.0 || fabs(g) >= 1.0 || theta < 0.0 || theta >= pi / 2 ||
+      phase < 0.0 || phase > pi)
+    return 0.0;
+  const double mu0 = cosi, mu = cose;
+  const double cosphase = (g != 0.0 || theta != 0.0) ? cos(phase) : -999.99;
+  const double s1w = sqrt(1 - w), r0 = (1 - s1w) / (1 + s1w);
+  const double P = (g == 0.0) ? 1.0 : (1 - g * g) / pow(1 + 2 * g * cosphase + g * g, 1.5);
+  const double B = (B0 == 0.0) ? 0.0 : B0 / (1 + tan(phase / 2.0) / h);
+  double mu0p = mu0, mup = mu, S = 1.0;
+  if (theta != 0.0) {
+    const double i = acos(cosi), sini = sin(i), e = acos(cose), sine = sin(e);
+    double cospsi = 1.0;
+    if (sini > 0.0 && sine > 0.0) {
+      cospsi = (cosphase - cosi * cose) / (sini * sine);
+      if (fabs(cospsi) > 1.0) cospsi /= fabs(cospsi);
+    }
+    const double psi = acos(cospsi), hs = sin(psi / 2.0), s2p2 = hs * hs;
+    const double tt = tan(theta), ct = 1 / tt;
+    const double f = (psi < pi) ? exp(-2.0 * tan(psi / 2.0)) : 0.0;
+    const double C = 1 / sqrt(1 + pi * tt * tt);
+    double E1e = 0.0, E2e = 0.0, E1i = 0.0, E2i = 0.0;
+    if (e >= pi / 2) { E1e = 1.0; E2e = 1.0; }
+    else if (e > 0.0) { const double a = ct / tan(e); E1e = exp(-(2 / pi) * a); E2e = exp(-(1 / pi) * a * a); }
+    if (i >= pi / 2) { E1i = 1.0; E2i = 1.0; }
+    else if (i > 0.0) { const double a = ct / tan(i); E1i = exp(-(2 / pi) * a); E2i = exp(-(1 / pi) * a * a); }
+    const double mu0p0 = C * (mu0 + sini * tt * E2i / (2 - E1i));
+    const double mup0 = C * (mu + sine * tt * E2e / (2 - E1e));
+    if (i < e) {
+      const double den = 2 - E1e - (psi / pi) * E1i;
+      mu0p = C * (mu0 + sini * tt * (cospsi * E2e + s2p2 * E2i) / den);
+      mup = C * (mu + sine * tt * (E2e - s2p2 * E2i) / den);
+      S = ((mup * mu0) / (mup0 * mu0p0)) * C / (1 - f * (1 - C * mu0 / mu0p0));
+    } else {
+      const double den = 2 - E1i - (psi / pi) * E1e;
+      mu0p = C * (mu0 + sini * tt * (E2i - s2p2 * E2e) / den);
+      mup = C * (mu + sine * tt * (cospsi * E2i + s2p2 * E2e) / den);
+      S = ((mup * mu0) / (mup0 * mu0p0)) * C / (1 - f * (1 - C * mu / mup0));
+    }
+  }
+  const double H0 = mu0p > 1e-40 ? 1 / (1 - w * mu0p * (r0 + 0.5 * (1 - 2 * r0 * mu0p) * log(1 + 1 / mu0p))) : 1.0;
+  const double H1 = mup > 1e-40 ? 1 / (1 - w * mup * (r0 + 0.5 * (1 - 2 * r0 * mup) * log(1 + 1 / mup))) : 1.0;
+  const double r = ((w * mu0p) / (4 * pi * (mu0p + mup))) * ((1 + B) * P + H0 * H1 - 1.0) * S;
+  return r > 0.0 ? r : 0.0;
+}
+
+// one lit pixel's optical brightness (apply_photo.c:78-309), in the
+// reference's order of operations: pl the law's parameter slots, loc the
+// evaluation's per-facet values of a harmonic / inhomogeneous law (k per
+// facet), f the pixel's facet (< 0: none, no brightness for a per-facet law)
+__device__ inline double optlaw64(const double *pl, int type, double inf, double ci, double ce, double phase,
+                                  const double *loc, int k, int f) {
+#pragma clang fp contract(off)
+  const double pi = 3.141592653589793;
+  double q[5] = {pl[1], pl[2], pl[3], pl[4], pl[5]};
+  if (type >= HARMLAMBERT && type != HAPKE && type != KAASALAINEN) {  // per-facet laws
+    if (f < 0) return 0.0;
+    for (int j = 0; j < k; j++) q[j] = loc[(long long)k * f + j];
+  }
+  switch (type) {
+  case LAMBERTLAW: case HARMLAMBERT: case INHOLAMBERT: return inf * (q[0] / pi) * ci;
+  case LOMMEL: case HARMLOMMEL: case INHOLOMMEL: return inf * (q[0] / (4 * pi)) * ci / (ci + ce);
+  case GEOMETRICAL: return inf * q[0];
+  case HAPKE: case HARMHAPKE: case INHOHAPKE: return inf * hapke64(ci, ce, phase, q[0], q[1], q[2], q[3], q[4]);
+  default: {  // KAASALAINEN, HARMKAAS, INHOKAAS
+    const double pf = q[2] * exp(-phase / q[3]) + q[4] * phase + 1;
+    const double sls = (1 - q[1]) * pf * q[0] / (4 * pi);
+    const double sl = q[1] * pf * q[0] / pi;
+    return inf * ci * (sls / (ci + ce) + sl);
+  }
+  }
+}
+
 }  // namespace shp

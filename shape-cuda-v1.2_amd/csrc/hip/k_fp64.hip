@@ -781,8 +781,8 @@ __global__ void k_posmask64(ViewRec *views, PosDev pd, double tol) {
 }
 
 // ---------------------------------------------------------------------------
-// optical brightness (apply_photo.c:78-309, Lambert / Lommel-Seeliger /
-// geometric / Kaasalainen) summed per lightcurve point
+// optical brightness (apply_photo.c:78-309, every optical law) summed per
+// lightcurve point
 // ---------------------------------------------------------------------------
 __global__ void k_photo64(ViewRec *views, PosDev pd, const double *params) {
   ViewRec &V = views[blockIdx.y];
@@ -797,16 +797,7 @@ __global__ void k_photo64(ViewRec *views, PosDev pd, const double *params) {
       double ce = pd.cose[p], ci = pd.cosi[p];
       if (ce > 0.0 && ci > 0.0 && (pd.body ? pd.body[p] == V.body : true)) {
         const double *pl = params + V.pb + V.pb_law;
-        int type = V.law_type;
-        if (type == LAMBERTLAW) b = V.intensityfactor * (pl[1] / 3.141592653589793) * ci;
-        else if (type == LOMMEL) b = V.intensityfactor * (pl[1] / (4 * 3.141592653589793)) * ci / (ci + ce);
-        else if (type == GEOMETRICAL) b = V.intensityfactor * pl[1];
-        else if (type == KAASALAINEN) {
-          double pf = pl[3] * exp(-V.phase / pl[4]) + pl[5] * V.phase + 1;
-          double sls = (1 - pl[2]) * pf * pl[1] / (4 * 3.141592653589793);
-          double sl = pl[2] * pf * pl[1] / 3.141592653589793;
-          b = V.intensityfactor * ci * (sls / (ci + ce) + sl);
-        }
+        b = optlaw64(pl, V.law_type, V.intensityfactor, ci, ce, V.phase, V.lawloc, V.lawk, V.lawloc ? pd.f[p] : -1);
         if (pd.b) pd.b[p] = b;
       }
     }
@@ -916,8 +907,8 @@ __device__ inline double delay_weight64(const ViewRec &V, double delPOS, int ide
 }
 
 // one radar scattering-law value (radlaw.c:44-148)
-__global__ void k_radlaw_one(const double *pl, int type, double c, double *out) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) *out = radlaw64(pl, type, c);
+__global__ void k_radlaw_one(const double *pl, int type, double c, const double *loc, int f, double *out) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *out = radlaw64x(pl, type, c, loc, f);
 }
 
 __global__ __launch_bounds__(256) void k_deldop64(ViewRec *views, FrameRec *frames, PosDev pd, double *fit,
@@ -976,7 +967,7 @@ __global__ __launch_bounds__(256) void k_deldop64(ViewRec *views, FrameRec *fram
       for (int b = 0; b < DOPMAX; b++)
         if (a < nd && b < np) sumw += dc[a] * pc[b];
     const double *pl = params + V.pb + V.pb_law;
-    double amp = radlaw64(pl, V.law_type, pd.cose[p]) * V.kmpp * V.kmpp * codefactor / sumw;
+    double amp = radlaw64x(pl, V.law_type, pd.cose[p], V.lawloc, V.lawloc ? pd.f[p] : -1) * V.kmpp * V.kmpp * codefactor / sumw;
 #pragma unroll
     for (int a = 0; a < DELMAX; a++) {
       if (a < nd) {
@@ -997,7 +988,7 @@ __global__ __launch_bounds__(256) void k_deldop64(ViewRec *views, FrameRec *fram
       for (int b = idop_min; b <= idop_max; b++) sumw += d * sinc2_mean64(V, dopPOS, b);
     }
     const double *pl = params + V.pb + V.pb_law;
-    double amp = radlaw64(pl, V.law_type, pd.cose[p]) * V.kmpp * V.kmpp * codefactor / sumw;
+    double amp = radlaw64x(pl, V.law_type, pd.cose[p], V.lawloc, V.lawloc ? pd.f[p] : -1) * V.kmpp * V.kmpp * codefactor / sumw;
     for (int a = idel_min; a <= idel_max; a++) {
       double d = delay_weight64(V, delPOS, a);
       for (int b = idop_min; b <= idop_max; b++)
@@ -1067,7 +1058,7 @@ __global__ __launch_bounds__(256) void k_doppler64(ViewRec *views, FrameRec *fra
     }
     for (int b = DOPMAX; b < np; b++) sumw += sinc2_mean64(V, dopPOS, idop_min + b);
     const double *pl = params + V.pb + V.pb_law;
-    double amp = radlaw64(pl, V.law_type, pd.cose[p]) * V.kmpp * V.kmpp / sumw;
+    double amp = radlaw64x(pl, V.law_type, pd.cose[p], V.lawloc, V.lawloc ? pd.f[p] : -1) * V.kmpp * V.kmpp / sumw;
     if (small) {
       double *row = use_lds ? win + (idop_min - lo_p) : img + (idop_min - 1);
 #pragma unroll
@@ -1296,7 +1287,8 @@ __global__ void k_chi2_frames64(FrameRec *frames, int nframes, double *fit, cons
 __global__ __launch_bounds__(256) void k_objective(EvalRec *evals, int nevals, const FrameRec *frames,
                                                    const ViewRec *views, const double *view_out, const int *slot_flags,
                                                    const double *slot_logf, const int *eval_slot,
-                                                   const double *pen_total) {
+                                                   const double *pen_total, const int *job_flag,
+                                                   const double *job_log) {
   // one workgroup per evaluation: the threads fetch 256 frames' terms at a
   // time into LDS, one thread adds them in frame order (the CPU's order)
   const int e = blockIdx.x;
@@ -1349,9 +1341,16 @@ __global__ __launch_bounds__(256) void k_objective(EvalRec *evals, int nevals, c
   int sl = eval_slot[e];
   E.baddiam = slot_flags[sl];
   E.baddiam_logfactor = slot_logf[sl];
+  E.badphoto = 0;
+  E.badphoto_logfactor = 0.0;
+  for (int q = 0; q < E.photo_njobs; q++) {  // the jobs in law order
+    E.badphoto |= job_flag[E.photo_job0 + q];
+    E.badphoto_logfactor += job_log[E.photo_job0 + q];
+  }
   double err = chi2 / E.dof;
   if (pen_total) err += pen_total[e];  // bestfit.c:807-811
   if (E.baddiam) err *= E.bad_objfactor * exp(E.baddiam_logfactor);
+  if (E.badphoto) err *= E.bad_objfactor * exp(E.badphoto_logfactor);
   if (E.posbnd) err *= E.bad_objfactor * exp(E.posbnd_logfactor);
   if (E.badradar) err *= E.bad_objfactor * exp(E.badradar_logfactor);
   E.objective = err;
@@ -1360,6 +1359,123 @@ __global__ __launch_bounds__(256) void k_objective(EvalRec *evals, int nevals, c
 }  // namespace shp
 
 namespace shp {
+
+// ---------------------------------------------------------------------------
+// realize_photo (realize_photo.c:122-969) as the objective calls it (albedo
+// factors 1): one workgroup per job (scattering law x evaluation)
+// ---------------------------------------------------------------------------
+
+// checkphotopar (realize_photo.c:965-1004): the log factor a value adds;
+// mode bit 0 makes the lower bound exclusive (modes 1, 3), bit 1 the upper
+__device__ inline double photo_check(double v, double lo, double hi, int mode, int &bad) {
+  double add = 0.0;
+  if ((mode == 0 || mode == 2) ? v < lo : v <= lo) { bad = 1; add += log(1 + lo - v); }
+  if ((mode == 0 || mode == 1) ? v > hi : v >= hi) { bad = 1; add += log(1 + v - hi); }
+  return add;
+}
+
+// associated Legendre function P_l^m(x) with the Condon-Shortley phase, by
+// the three-term recurrence in l from P_m^m
+__device__ double photo_plm(int l, int m, double x) {
+  double pmm = 1.0;
+  if (m > 0) {
+    const double s = sqrt((1.0 - x) * (1.0 + x));
+    double fact = 1.0;
+    for (int i = 1; i <= m; i++) { pmm *= -fact * s; fact += 2.0; }
+  }
+  if (l == m) return pmm;
+  double pm1 = x * (2 * m + 1) * pmm;
+  if (l == m + 1) return pm1;
+  double pl = 0.0;
+  for (int ll = m + 2; ll <= l; ll++) {
+    pl = (x * (2 * ll - 1) * pm1 - (ll + m - 1) * pmm) / (ll - m);
+    pmm = pm1;
+    pm1 = pl;
+  }
+  return pl;
+}
+
+// sqrt((2l+1) (l-m)! / (l+m)!)
+__device__ inline double photo_nlm(int l, int m) {
+  double r = 1.0;
+  for (int k = l - m + 1; k <= l + m; k++) r /= k;
+  return sqrt((2 * l + 1) * r);
+}
+
+// an expansion's value at (cos theta, phi); c in the model file's order
+// a00, a10, a11, b11, a20, a21, b21, ...
+__device__ double photo_harm(const double *c, int L, double ct, double phi) {
+  double v = 0.0;
+  int q = 0;
+  for (int l = 0; l <= L; l++) {
+    v += c[q++] * (photo_nlm(l, 0) * photo_plm(l, 0, ct));
+    for (int m = 1; m <= l; m++) {
+      const double p = photo_nlm(l, m) * photo_plm(l, m, ct);
+      const double a = c[q], b = c[q + 1];
+      q += 2;
+      v += a * (cos(m * phi) * p) + b * (sin(m * phi) * p);
+    }
+  }
+  return v;
+}
+
+__global__ __launch_bounds__(256) void k_realize_photo(const PhotoJob *jobs, const double *params,
+                                                       const GeomSlot *slots, double *lawloc, int *job_flag,
+                                                       double *job_log) {
+  const PhotoJob &J = jobs[blockIdx.x];
+  const double *d = params + J.data;
+  double *out = lawloc + J.loc;
+  double lg = 0.0;
+  int bad = 0;
+  const int k = J.k;
+  if (J.kind == PHOTO_HOMOG) {
+    if (threadIdx.x == 0)
+      for (int q = 0; q < J.nchk; q++) lg += photo_check(params[J.chk[q]], J.lo[q], J.hi[q], J.mode[q], bad);
+  } else if (J.kind == PHOTO_TABLE) {
+    if (threadIdx.x == 0)
+      for (int i = 0; i < J.n; i++) {
+        const double v = (i > 0 && d[J.n + i] != 0.0) ? out[i - 1] : d[i];
+        out[i] = v;
+        lg += photo_check(v, J.lo[0], J.hi[0], J.mode[0], bad);
+      }
+  } else {
+    if (J.kind == PHOTO_INHO && threadIdx.x == 0)
+      for (int j = 0; j < k; j++) lg += photo_check(d[j], J.lo[j], J.hi[j], J.mode[j], bad);
+    const GeomSlot &G = slots[J.slot];
+    for (int f = threadIdx.x; f < J.n; f += blockDim.x) {
+      double v[5];
+      if (J.kind == PHOTO_HARM) {
+        const double *x = G.fx + 3 * (long long)f;
+        const double theta = atan2(sqrt(x[0] * x[0] + x[1] * x[1]), x[2]), phi = atan2(x[1], x[0]);
+        const double ct = cos(theta);
+        for (int j = 0; j < k; j++) v[j] = photo_harm(d + J.coef[j], J.nhar[j], ct, phi);
+      } else {
+        const double *loc = d + k + (long long)k * f, *tie = d + k + (long long)k * J.n + (long long)k * f;
+        for (int j = 0; j < k; j++) v[j] = tie[j] != 0.0 ? d[j] : loc[j];
+      }
+      for (int j = 0; j < k; j++) {
+        out[(long long)k * f + j] = v[j];
+        lg += photo_check(v[j], J.lo[j], J.hi[j], J.mode[j], bad);
+      }
+    }
+  }
+  __shared__ double s_lg[256];
+  __shared__ int s_bad[256];
+  s_lg[threadIdx.x] = lg;
+  s_bad[threadIdx.x] = bad;
+  __syncthreads();
+  for (int o = blockDim.x / 2; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) {
+      s_lg[threadIdx.x] += s_lg[threadIdx.x + o];
+      s_bad[threadIdx.x] |= s_bad[threadIdx.x + o];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    job_flag[blockIdx.x] = s_bad[0];
+    job_log[blockIdx.x] = s_lg[0];
+  }
+}
 
 // facets seen from Earth (calc_fits.c:522-532, :903-913, :1699-1710)
 __global__ void k_mark_seen(ViewRec *views, GeomSlot *slots, PosDev pd, double mincos) {

@@ -183,7 +183,7 @@ __device__ void replay32_one(ViewRec *views, const GeomSlot *slots, const PosDev
   if (pd.fmap) pd.fmap[V.pos_off + S.pix] = fac;  // (inspection)
   pd.stale_idx[V.pos_off + S.pix] = -1;
   if (!(cosa > 0.0)) return;
-  replay_emit(V, params + V.pb + V.pb_law, i, j, zz, cosa, fit + frames[V.frame_id].off,
+  replay_emit(V, params + V.pb + V.pb_law, i, j, zz, cosa, fac, fit + frames[V.frame_id].off,
               overflow + (long long)vi * 2000);
 }
 

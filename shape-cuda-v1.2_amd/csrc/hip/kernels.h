@@ -98,7 +98,7 @@ __global__ void k_pen_eval(const EvalRec *evals, int nevals, const int *eval_slo
 __global__ void k_setup_views(ViewRec *views, int nviews, const double *params);
 __global__ void k_pos_setup(ViewRec *views);
 __global__ void k_pos_seed(ViewRec *views, PosDev pd, int src);
-__global__ void k_radlaw_one(const double *pl, int type, double c, double *out);
+__global__ void k_radlaw_one(const double *pl, int type, double c, const double *loc, int f, double *out);
 __global__ void k_realize_vertices(GeomSlot *slots, int nslot, const double *params, int *slot_flags,
                                    double *slot_logf);
 __global__ void k_realize_harmonic_bad(GeomSlot *slots, const double *params, int *slot_flags, double *slot_logf);
@@ -121,7 +121,10 @@ __global__ void k_chi2_frames64(FrameRec *frames, int nframes, double *fit, cons
                                 const double *view_out, int clear_fit, int ovf_from_host, int lc_only);
 __global__ void k_objective(EvalRec *evals, int nevals, const FrameRec *frames, const ViewRec *views,
                             const double *view_out, const int *slot_flags, const double *slot_logf,
-                            const int *eval_slot, const double *pen_total);
+                            const int *eval_slot, const double *pen_total, const int *job_flag,
+                            const double *job_log);
+__global__ void k_realize_photo(const PhotoJob *jobs, const double *params, const GeomSlot *slots, double *lawloc,
+                                int *job_flag, double *job_log);
 
 }  // namespace shp
 

@@ -465,6 +465,53 @@ __device__ inline float cosine_law32(const double *pl, float c) {
 // well above the FP32 error of the LOS normal.
 constexpr float CUT_BAND = 2e-5f;
 
+// sin^2 of the scattering angle of facet f at barycentric (s, t) from the
+// LOS-frame normal's x and y components (accurate where 1 - c^2 cancels):
+// the body-frame normal is the facet's, or its vertex normals' at (s, t)
+__device__ inline float los_sin2(const ViewRec &V, const GeomSlot &G, int f, float s, float t) {
+  float bx, by, bz;
+  if (V.smooth) {
+    const int *fv = G.fv + 3 * f;
+    const float4 na = G.vn32[fv[0]], nb = G.vn32[fv[1]], nc = G.vn32[fv[2]];
+    bx = fmaf(t, nc.x - nb.x, fmaf(s, nb.x - na.x, na.x));
+    by = fmaf(t, nc.y - nb.y, fmaf(s, nb.y - na.y, na.y));
+    bz = fmaf(t, nc.z - nb.z, fmaf(s, nb.z - na.z, na.z));
+  } else {
+    const float4 n = G.fn32[f];
+    bx = n.x; by = n.y; bz = n.z;
+  }
+  const float *m = V.oa32;
+  const float x = fmaf(m[2], bz, fmaf(m[1], by, m[0] * bx));
+  const float y = fmaf(m[5], bz, fmaf(m[4], by, m[3] * bx));
+  return fmaf(x, x, y * y) / fmaf(bz, bz, fmaf(by, by, bx * bx));
+}
+
+// Below this cos(scattering) the tabular and per-facet laws go to the FP64
+// replay: rho / c (and c^(2C-1) for C < 1/2) grows without bound at the limb,
+// where the FP32 cosine's ~1e-7 absolute error becomes a large relative one.
+// A few pixels in a thousand on a convex body.
+constexpr float LIMB_REPLAY = 0.05f;
+
+// The tabular law and the per-facet cosine laws (radlaw.c:56-66, :118-135)
+// in FP32 from the evaluation's resolved values (V.lawloc): the tabular
+// law's incidence as atan2(sin, cos) with sin^2 from the normal (acos of an
+// FP32 cosine near 1 loses half its digits), its index clamped to n - 2.
+// NaN (FP64 replay) for a limb pixel.
+__device__ __attribute__((noinline)) float radlaw32_local(const ViewRec &V, const GeomSlot &G, const double *pl,
+                                                          int f, float s, float t, float c) {
+  if (c < LIMB_REPLAY) return __builtin_nanf("");
+  if (V.law_type == TABULARLAW) {
+    const int n = (int)pl[2];
+    const float inc = atan2f(sqrtf(los_sin2(V, G, f, s, t)), c);
+    const float ratio = inc * ((float)(n - 1) / 1.57079632679489662f);
+    const int i = min((int)floorf(ratio), n - 2);
+    const float r1 = (float)V.lawloc[i], r2 = (float)V.lawloc[i + 1];
+    return (r1 + (ratio - (float)i) * (r2 - r1)) / c;
+  }
+  const float R = (float)V.lawloc[2 * f], Cc = (float)V.lawloc[2 * f + 1];
+  return R * (Cc + 1.0f) * powc(c, 2.0f * Cc - 1.0f);
+}
+
 // The other homogeneous laws (radlaw.c:64-117) for a lit pixel of facet f
 // with FP32 cos(scattering) c at barycentric (s, t); pl = [type, R, C,
 // cutoff, qsR, qsC, qs_cutoff].  Their quasispecular terms peak steeply at
@@ -488,21 +535,7 @@ __device__ __attribute__((noinline)) float radlaw32_steep(const ViewRec &V, cons
     Cc = (float)pl[5];
   }
   if (!(c >= cut)) return d;
-  float bx, by, bz;  // body-frame normal: the facet's, or its vertex normals' at (s, t)
-  if (V.smooth) {
-    const int *fv = G.fv + 3 * f;
-    const float4 na = G.vn32[fv[0]], nb = G.vn32[fv[1]], nc = G.vn32[fv[2]];
-    bx = fmaf(t, nc.x - nb.x, fmaf(s, nb.x - na.x, na.x));
-    by = fmaf(t, nc.y - nb.y, fmaf(s, nb.y - na.y, na.y));
-    bz = fmaf(t, nc.z - nb.z, fmaf(s, nb.z - na.z, na.z));
-  } else {
-    const float4 n = G.fn32[f];
-    bx = n.x; by = n.y; bz = n.z;
-  }
-  const float *m = V.oa32;
-  const float x = fmaf(m[2], bz, fmaf(m[1], by, m[0] * bx));
-  const float y = fmaf(m[5], bz, fmaf(m[4], by, m[3] * bx));
-  const float s2 = fmaf(x, x, y * y) / fmaf(bz, bz, fmaf(by, by, bx * bx));
+  const float s2 = los_sin2(V, G, f, s, t);
   const float c2 = c * c;
   switch (type) {
   case GAUSSIANLAW: case GAUSSIAN_COSINE:
@@ -524,6 +557,7 @@ __device__ __attribute__((noinline)) float radlaw32_steep(const ViewRec &V, cons
 __device__ inline float radar_cl(const ViewRec &V, const GeomSlot &G, const double *pl, int f, float s, float t,
                                  float c) {
   if (V.law_type == COSINELAW_DIFF) return c;
+  if (V.lawloc) return radlaw32_local(V, G, pl, f, s, t, c);
   return radlaw32_steep(V, G, pl, f, s, t, c);
 }
 

@@ -95,9 +95,59 @@ int pack_radlaw(double pl[8], struct photo_t *photo, int ilaw) {
     pl[1] = r->hybrid.diff.R.val; pl[2] = r->hybrid.diff.C.val;
     pl[4] = r->hybrid.qs.R.val; pl[5] = r->hybrid.qs.C.val; pl[6] = r->hybrid.qs.cos_cutoff;
     return 0;
+  case TABULARLAW: pl[2] = r->tabular.n; return 0;
+  case HARMCOSINE_DIFF: case INHOCOSINE_DIFF: return 0;
   default:
-    return -13;  // tabular and inhomogeneous laws are outside this library
+    return -13;
   }
+}
+
+// The caller's resolved values of a tabular or per-facet law (its
+// realize_photo has run): a tabular law's rho table (k = 1), or k values per
+// facet of component 0 for facets 0 .. nf-1.  Empty for the other laws.
+void law_values(struct photo_t *photo, int radar, int ilaw, int nf, std::vector<double> &loc, int &k) {
+  loc.clear();
+  k = 0;
+  const int t = radar ? photo->radtype[ilaw] : photo->opttype[ilaw];
+  char *base = nullptr;
+  size_t elem = 0;
+  if (radar) {
+    union radscat_t *r = &photo->radar[ilaw];
+    if (t == TABULARLAW) {
+      k = 1;
+      for (int i = 0; i < r->tabular.n; i++) loc.push_back(r->tabular.rho[i].val);
+      return;
+    }
+    if (t == HARMCOSINE_DIFF) base = (char *)r->harmcosine.local[0];
+    else if (t == INHOCOSINE_DIFF) base = (char *)r->inhocosine.local[0];
+    else return;
+    k = 2;
+    elem = sizeof(struct RC_t);
+  } else {
+    union optscat_t *o = &photo->optical[ilaw];
+    switch (t) {
+    case HARMLAMBERT: case HARMLOMMEL: base = (char *)o->harmR.local[0]; k = 1; elem = sizeof(struct R_t); break;
+    case INHOLAMBERT: case INHOLOMMEL: base = (char *)o->inhoR.local[0]; k = 1; elem = sizeof(struct R_t); break;
+    case HARMHAPKE: base = (char *)o->harmhapke.local[0]; k = 5; elem = sizeof(struct hapke_t); break;
+    case INHOHAPKE: base = (char *)o->inhohapke.local[0]; k = 5; elem = sizeof(struct hapke_t); break;
+    case HARMKAAS: base = (char *)o->harmkaas.local[0]; k = 5; elem = sizeof(struct kaas_t); break;
+    case INHOKAAS: base = (char *)o->inhokaas.local[0]; k = 5; elem = sizeof(struct kaas_t); break;
+    default: return;
+    }
+  }
+  loc.resize((size_t)k * std::max(nf, 1), 0.0);
+  for (int f = 0; f < nf; f++) {
+    const struct param_t *p = (const struct param_t *)(base + elem * f);
+    for (int j = 0; j < k; j++) loc[(size_t)k * f + j] = p[j].val;
+  }
+}
+
+// facets a POS names: one more than the largest pos->f in its rectangle
+int pos_nf(struct pos_t *pos) {
+  int m = -1;
+  for (int i = pos->xlim[0]; i <= pos->xlim[1]; i++)
+    for (int j = pos->ylim[0]; j <= pos->ylim[1]; j++) m = std::max(m, pos->f[i][j]);
+  return m + 1;
 }
 
 inline unsigned cdiv(long long a, long long b) { return (unsigned)((a + b - 1) / b); }
@@ -283,19 +333,28 @@ extern "C" void posmask(struct pos_t *pos, double tol) {
 // radlaw (radlaw.c:44): one scattering-law value for cos(incidence) cosinc
 // ---------------------------------------------------------------------------
 extern "C" double radlaw(struct photo_t *photo, int ilaw, double cosinc, int c, int f) {
-  (void)c;
-  (void)f;  // only the inhomogeneous and tabular laws read them
+  (void)c;  // component 0: the library's models have one
   double pl[8];
   int rc = pack_radlaw(pl, photo, ilaw);
   if (rc) { record_error(rc, "radlaw (law type)"); return 0.0; }
+  // a per-facet law's values of facet f alone (passed as facet 0)
+  std::vector<double> loc;
+  int k = 0, fl = f;
+  law_values(photo, 1, ilaw, f + 1, loc, k);
+  if (k == 2 && f >= 0) {
+    loc.erase(loc.begin(), loc.begin() + 2 * f);
+    fl = 0;
+  }
   Scratch S;
-  double *dpl = S.get<double>(8), *dout = S.get<double>(1);
+  double *dpl = S.get<double>(8), *dout = S.get<double>(1), *dloc = loc.empty() ? nullptr : S.get<double>(loc.size());
   double v = 0.0;
-  if (!dpl || !dout || hipMemcpy(dpl, pl, sizeof(pl), hipMemcpyHostToDevice) != hipSuccess) {
+  if (!dpl || !dout || (!loc.empty() && !dloc) || hipMemcpy(dpl, pl, sizeof(pl), hipMemcpyHostToDevice) != hipSuccess ||
+      (dloc && hipMemcpy(dloc, loc.data(), sizeof(double) * loc.size(), hipMemcpyHostToDevice) != hipSuccess)) {
     record_error(-3, "radlaw upload");
     return 0.0;
   }
-  hipLaunchKernelGGL(k_radlaw_one, dim3(1), dim3(64), 0, 0, (const double *)dpl, (int)pl[0], cosinc, dout);
+  hipLaunchKernelGGL(k_radlaw_one, dim3(1), dim3(64), 0, 0, (const double *)dpl, (int)pl[0], cosinc,
+                     (const double *)dloc, fl, dout);
   if (hipMemcpy(&v, dout, sizeof(double), hipMemcpyDeviceToHost) != hipSuccess) {
     record_error(-4, "radlaw kernel");
     return 0.0;
@@ -368,10 +427,23 @@ int radar_view(struct par_t *par, struct photo_t *photo, double orbit_xoff, doub
   } else {
     memcpy(fit.data(), &Q->frame[frm].fit[1], sizeof(double) * N);
   }
+  std::vector<double> loc;
+  int lawk = 0;
+  law_values(photo, 1, ilaw, pos_nf(pos), loc, lawk);
   Scratch S;
   PosDev pd;
   memset(&pd, 0, sizeof(pd));
   pd.z = S.get<double>(P); pd.cose = S.get<double>(P); pd.body = S.get<int>(P);
+  double *dloc = nullptr;
+  if (lawk) {
+    pd.f = S.get<int>(P);
+    dloc = S.get<double>(loc.size());
+    if (!pd.f || !dloc || upload_mat(pd.f, pos->f, n) ||
+        hipMemcpy(dloc, loc.data(), sizeof(double) * loc.size(), hipMemcpyHostToDevice) != hipSuccess)
+      return -3;
+    V.lawloc = dloc;
+    V.lawk = lawk;
+  }
   ViewRec *dV = S.get<ViewRec>(1);
   FrameRec *dF = S.get<FrameRec>(1);
   double *dpb = S.get<double>(pb.size()), *dfit = S.get<double>(N), *dov = S.get<double>(2000),
@@ -452,7 +524,11 @@ extern "C" double apply_photo(struct mod_t *mod, int ilaw, double phase, double 
   double pl[PB_OPT_STRIDE] = {(double)t, 0, 0, 0, 0, 0};
   if (t == LAMBERTLAW || t == LOMMEL || t == GEOMETRICAL) pl[1] = o->R.R.val;
   else if (t == KAASALAINEN) { pl[1] = o->kaas.R.val; pl[2] = o->kaas.wt.val; pl[3] = o->kaas.A0.val; pl[4] = o->kaas.D.val; pl[5] = o->kaas.k.val; }
-  else { record_error(-14, "apply_photo (law type)"); return 1.0e-40; }
+  else if (t == HAPKE) { const struct param_t *q = &o->hapke.w; for (int j = 0; j < 5; j++) pl[1 + j] = q[j].val; }
+  else if (t < HARMLAMBERT || t > INHOKAAS) { record_error(-14, "apply_photo (law type)"); return 1.0e-40; }
+  std::vector<double> loc;
+  int lawk = 0;
+  law_values(&mod->photo, 0, ilaw, pos_nf(pos), loc, lawk);
   int n = pos->n;
   long long P = pos_pixels(pos);
   Scratch S;
@@ -465,6 +541,17 @@ extern "C" double apply_photo(struct mod_t *mod, int ilaw, double phase, double 
   V.pb = 0; V.pb_law = 0; V.law_type = t;
   V.phase = phase; V.intensityfactor = intensityfactor; V.body = body;
   V.ysum = 0.0;
+  if (lawk) {
+    pd.f = S.get<int>(P);
+    double *dloc = S.get<double>(loc.size());
+    if (!pd.f || !dloc || upload_mat(pd.f, pos->f, n) ||
+        hipMemcpy(dloc, loc.data(), sizeof(double) * loc.size(), hipMemcpyHostToDevice) != hipSuccess) {
+      record_error(-3, "apply_photo upload");
+      return 1.0e-40;
+    }
+    V.lawloc = dloc;
+    V.lawk = lawk;
+  }
   ViewRec *dV = S.get<ViewRec>(1);
   double *dpl = S.get<double>(PB_OPT_STRIDE);
   if (!pd.cose || !pd.cosi || !pd.b || !pd.body || !dV || !dpl) { record_error(-2, "apply_photo allocation"); return 1.0e-40; }

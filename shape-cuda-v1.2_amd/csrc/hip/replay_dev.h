@@ -54,19 +54,20 @@ __device__ inline void lower_limit_d(unsigned long long *a, double v) {
 
 // radlaw64 for the laws other than the diffuse cosine law, out of line so
 // that it costs the replay kernel no registers where those laws are not used
-__device__ __attribute__((noinline)) float radlaw64_other(const double *pl, int type, double c) {
-  return (float)radlaw64(pl, type, c);
+__device__ __attribute__((noinline)) float radlaw64_other(const double *pl, int type, double c, const double *loc,
+                                                          int f) {
+  return (float)radlaw64x(pl, type, c, loc, f);
 }
 
 // Scatter a replayed lit pixel straight into the HBM fit image (or the
 // overflow spectrum) and grow the view's bin limits by it: a replayed pixel
 // is lit like any other.
-__device__ inline void replay_emit(ViewRec &V, const double *pl, int i, int j, double zz, double cosa, double *img,
-                                   double *ov) {
+__device__ inline void replay_emit(ViewRec &V, const double *pl, int i, int j, double zz, double cosa, int fac,
+                                   double *img, double *ov) {
   RadarPix R;
   // the other laws in FP64 from the CPU's cosine: their cutoff test and
   // steep quasispecular terms need it
-  const float cl = V.law_type == COSINELAW_DIFF ? (float)cosa : radlaw64_other(pl, V.law_type, cosa);
+  const float cl = V.law_type == COSINELAW_DIFF ? (float)cosa : radlaw64_other(pl, V.law_type, cosa, V.lawloc, fac);
   radar_pixel(V, pl, i, j, zz, cl, R);
   radar_scatter_global(V, R, img, ov);
   const bool dd = V.type == VT_DELAY;

@@ -997,6 +997,15 @@ static void build_par(shp_scene *sc)
   p->pos_scope = (unsigned char)sp->pos_scope;
   p->int_method = PERIODIC;
   p->bad_objfactor = 2.0;
+  /* realize_photo's bounds, as read_par.c sets them by default */
+  p->rad_R_max = p->rad_C_max = p->rad_rho_max = HUGENUMBER;
+  p->opt_R_max = p->opt_A0_max = p->opt_D_max = p->opt_h_max = p->opt_B0_max = HUGENUMBER;
+  p->opt_wt_max = 1.0;
+  p->opt_k_min = -HUGENUMBER;
+  p->opt_w_max = 1.0;
+  p->opt_g_min = -1.0;
+  p->opt_g_max = 1.0;
+  p->opt_theta_max = PIE / 2;
   p->pos_pixels = 2 * sp->pos_n + 1;
   p->pos_width = sp->km_per_pixel * 2 * sp->pos_n;
   p->dd_gamma = 1.0;
@@ -1764,4 +1773,104 @@ void shp_spin_state(struct mod_t *mod, double out[7])
 {
   out[0] = mod->spin.t0;
   for (int k = 0; k < 3; k++) { out[1 + k] = mod->spin.angle[k].val; out[4 + k] = mod->spin.omega[k].val; }
+}
+
+/* a per-facet law's resolved values of facet f (component 0), k of them:
+   R (Lambert / Lommel), R C (harmcosine / inhocosine), w h B0 g theta
+   (Hapke), R wt A0 D k (Kaasalainen); 0 for the other laws */
+int shp_law_local(struct mod_t *mod, int radar, int ilaw, int f, double *out)
+{
+  const struct param_t *p = NULL;
+  int k = 0;
+  if (radar) {
+    union radscat_t *r = &mod->photo.radar[ilaw];
+    if (mod->photo.radtype[ilaw] == HARMCOSINE_DIFF) p = &r->harmcosine.local[0][f].R;
+    else if (mod->photo.radtype[ilaw] == INHOCOSINE_DIFF) p = &r->inhocosine.local[0][f].R;
+    k = 2;
+  } else {
+    union optscat_t *o = &mod->photo.optical[ilaw];
+    switch (mod->photo.opttype[ilaw]) {
+    case HARMLAMBERT: case HARMLOMMEL: p = &o->harmR.local[0][f].R; k = 1; break;
+    case INHOLAMBERT: case INHOLOMMEL: p = &o->inhoR.local[0][f].R; k = 1; break;
+    case HARMHAPKE: p = &o->harmhapke.local[0][f].w; k = 5; break;
+    case INHOHAPKE: p = &o->inhohapke.local[0][f].w; k = 5; break;
+    case HARMKAAS: p = &o->harmkaas.local[0][f].R; k = 5; break;
+    case INHOKAAS: p = &o->inhokaas.local[0][f].R; k = 5; break;
+    }
+  }
+  if (!p) return 0;
+  for (int j = 0; j < k; j++) out[j] = p[j].val;
+  return k;
+}
+
+/* a tabular law's values rho[0..n-1] (at most cap); returns n */
+int shp_law_table(struct mod_t *mod, int ilaw, double *out, int cap)
+{
+  if (mod->photo.radtype[ilaw] != TABULARLAW) return 0;
+  const struct tabular_t *t = &mod->photo.radar[ilaw].tabular;
+  for (int i = 0; i < t->n && i < cap; i++) out[i] = t->rho[i].val;
+  return t->n;
+}
+
+/* facet f of component 0: center x[3], theta, phi */
+void shp_facet_geom(struct mod_t *mod, int f, double out[5])
+{
+  const struct facet_t *fa = &mod->shape.comp[0].real.f[f];
+  for (int j = 0; j < 3; j++) out[j] = fa->x[j];
+  out[3] = fa->theta;
+  out[4] = fa->phi;
+}
+
+/* par's bad-photometry flag and log factor */
+void shp_badphoto(struct par_t *par, double out[2])
+{
+  out[0] = par->badphoto;
+  out[1] = par->badphoto_logfactor;
+}
+
+/* coefficient q of an expansion in the model file's order a00, a10, a11, b11, ... */
+static struct param_t *shp_coef(struct harmonic_t *h, int q)
+{
+  for (int l = 0; l <= h->nhar; l++) {
+    if (q == 0) return &h->a[l][0];
+    q--;
+    for (int m = 1; m <= l; m++) {
+      if (q == 0) return &h->a[l][m];
+      if (q == 1) return &h->b[l][m];
+      q -= 2;
+    }
+  }
+  return NULL;
+}
+
+/* address of a scattering law's parameter: a homogeneous law's which-th
+   parameter (R C; w h B0 g theta; R wt A0 D k; a tabular law's rho[which]),
+   a harmonic law's which-th coefficient of its first expansion, an
+   inhomogeneous law's which-th global value (f < 0) or facet f's */
+double *shp_law_param_ptr(struct mod_t *mod, int radar, int ilaw, int which, int f)
+{
+  struct param_t *p = NULL;
+  if (radar) {
+    union radscat_t *r = &mod->photo.radar[ilaw];
+    switch (mod->photo.radtype[ilaw]) {
+    case COSINELAW_DIFF: p = &r->RC.R + which; break;
+    case TABULARLAW: p = &r->tabular.rho[which]; break;
+    case HARMCOSINE_DIFF: p = shp_coef(&r->harmcosine.R, which); break;
+    case INHOCOSINE_DIFF: p = (f < 0 ? &r->inhocosine.global.R : &r->inhocosine.local[0][f].R) + which; break;
+    }
+  } else {
+    union optscat_t *o = &mod->photo.optical[ilaw];
+    switch (mod->photo.opttype[ilaw]) {
+    case GEOMETRICAL: case LAMBERTLAW: case LOMMEL: p = &o->R.R; break;
+    case HAPKE: p = &o->hapke.w + which; break;
+    case KAASALAINEN: p = &o->kaas.R + which; break;
+    case HARMLAMBERT: case HARMLOMMEL: p = shp_coef(&o->harmR.R, which); break;
+    case HARMHAPKE: p = shp_coef(&o->harmhapke.w, which); break;
+    case HARMKAAS: p = shp_coef(&o->harmkaas.R, which); break;
+    case INHOLAMBERT: case INHOLOMMEL: p = f < 0 ? &o->inhoR.global.R : &o->inhoR.local[0][f].R; break;
+    case INHOHAPKE: p = (f < 0 ? &o->inhohapke.global.w : &o->inhohapke.local[0][f].w) + which; break;
+    case INHOKAAS: p = (f < 0 ? &o->inhokaas.global.R : &o->inhokaas.local[0][f].R) + which; break;
+    }
+  }
+  return p ? &p->val : NULL;
 }

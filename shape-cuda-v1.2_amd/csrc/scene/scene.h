@@ -198,6 +198,12 @@ void shp_spin_state(struct mod_t *mod, double out[7]);
 /* Deep copy of the parts of a scene the forward model writes (fit images,
    frame scalars, view data) from src into dst; both built from one spec. */
 
+int shp_law_local(struct mod_t *mod, int radar, int ilaw, int f, double *out);
+int shp_law_table(struct mod_t *mod, int ilaw, double *out, int cap);
+void shp_facet_geom(struct mod_t *mod, int f, double out[5]);
+void shp_badphoto(struct par_t *par, double out[2]);
+double *shp_law_param_ptr(struct mod_t *mod, int radar, int ilaw, int which, int f);
+
 #ifdef __cplusplus
 }
 #endif

@@ -141,6 +141,11 @@ def load_scene():
             "shp_mod_param_ptr": (dp, [vp, C.c_int]),
             "shp_set_npa": (None, [vp, vp, dp, dp, C.c_int]),
             "shp_spin_state": (None, [vp, dp]),
+            "shp_law_local": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, dp]),
+            "shp_law_table": (C.c_int, [vp, C.c_int, dp, C.c_int]),
+            "shp_facet_geom": (None, [vp, C.c_int, dp]),
+            "shp_badphoto": (None, [vp, dp]),
+            "shp_law_param_ptr": (dp, [vp, C.c_int, C.c_int, C.c_int, C.c_int]),
         }
         for name, (res, args) in sigs.items():
             fn = getattr(lib, name)
@@ -475,7 +480,7 @@ class Scene:
         return self.lib.shp_frame_pos(self.dat, s, f)
 
     def pos_array(self, pos, which):
-        n = self.spec.pos_n
+        n = int(self.pos_scalars(pos)["n"])
         out = np.zeros((2 * n + 1, 2 * n + 1))
         self.lib.shp_pos_array(pos, which, out.ctypes.data_as(C.POINTER(C.c_double)))
         return out
@@ -534,6 +539,28 @@ class Scene:
         out = np.zeros(7)
         self.lib.shp_spin_state(self.mod, out.ctypes.data_as(C.POINTER(C.c_double)))
         return {"t0": out[0], "angle": out[1:4], "omega": out[4:7]}
+
+    def law_local(self, radar, ilaw, f):
+        """a per-facet law's resolved values at facet f (empty for other laws)"""
+        out = np.zeros(5)
+        k = self.lib.shp_law_local(self.mod, int(radar), ilaw, f, out.ctypes.data_as(C.POINTER(C.c_double)))
+        return out[:k]
+
+    def law_table(self, ilaw):
+        out = np.zeros(512)
+        n = self.lib.shp_law_table(self.mod, ilaw, out.ctypes.data_as(C.POINTER(C.c_double)), 512)
+        return out[:n]
+
+    def facet_geom(self, f):
+        """facet f's center, theta and phi"""
+        out = np.zeros(5)
+        self.lib.shp_facet_geom(self.mod, f, out.ctypes.data_as(C.POINTER(C.c_double)))
+        return out
+
+    def badphoto(self):
+        out = np.zeros(2)
+        self.lib.shp_badphoto(self.par, out.ctypes.data_as(C.POINTER(C.c_double)))
+        return int(out[0]), out[1]
 
     def set_view(self, s, f, v, arr):
         arr = np.ascontiguousarray(arr, dtype=np.float64)
@@ -1000,7 +1027,12 @@ class Job(Scene):
         return self.lib.shp_mod_nf(self.mod)
 
     def _param(self, i):
-        ptr = self.lib.shp_mod_param_ptr(self.mod, i)
+        """i: a slot of shp_mod_param_ptr, or ("law", radar, ilaw, which, f) for a
+        scattering law's parameter (shp_law_param_ptr)"""
+        if isinstance(i, tuple):
+            ptr = self.lib.shp_law_param_ptr(self.mod, *[int(x) for x in i[1:]])
+        else:
+            ptr = self.lib.shp_mod_param_ptr(self.mod, i)
         if not ptr:
             raise IndexError("no parameter slot %d" % i)
         return ptr

@@ -35,6 +35,9 @@ def load_oracle():
         sigs = {
             "oracle_realize_mod": (None, [VP, VP]),
             "oracle_realize_spin": (None, [VP, VP, VP]),
+            "oracle_realize_photo": (None, [VP, VP]),
+            "oracle_hapke": (C.c_double, [C.c_double] * 8),
+            "oracle_plm": (C.c_double, [C.c_int, C.c_int, C.c_double]),
             "oracle_realize_delcor": (None, [VP, C.c_double, C.c_int]),
             "oracle_posclr": (None, [VP]),
             "oracle_posvis": (C.c_int, [VP, C.POINTER(C.c_double), VP, C.c_int, C.c_int, C.c_int, C.c_int]),
@@ -81,6 +84,7 @@ def oracle_forward(orc, sc, realize=True):
     if realize:
         orc.oracle_realize_mod(sc.par, sc.mod)
         orc.oracle_realize_spin(sc.par, sc.mod, sc.dat)
+        orc.oracle_realize_photo(sc.par, sc.mod)
         orc.oracle_realize_delcor(sc.dat, 0.0, 0)
     orc.oracle_calc_fits(sc.par, sc.mod, sc.dat)
     return orc.oracle_chi2(sc.par, sc.dat, 0)
@@ -105,6 +109,7 @@ def oracle_forward_threads(orc, sc):
     POS per frame, pos_scope LOCAL); returns chi2."""
     orc.oracle_realize_mod(sc.par, sc.mod)
     orc.oracle_realize_spin(sc.par, sc.mod, sc.dat)
+    orc.oracle_realize_photo(sc.par, sc.mod)
     orc.oracle_realize_delcor(sc.dat, 0.0, 0)
     orc.oracle_calc_fits_threads(sc.par, sc.mod, sc.dat, oracle_threads())
     return orc.oracle_chi2(sc.par, sc.dat, 0)
