@@ -182,12 +182,151 @@ void oracle_diag_inertia(double inertia[3][3], double pmoment[3], double ap[3][3
 /* the model's effective radius: the radius of the sphere of its volume */
 static double o_reff(const struct mod_t *mod) { return pow(3 * mod->shape.volume / (4 * O_PIE), 1.0 / 3.0); }
 
+/* The per-facet value a law-reading penalty takes from law ilaw at facet f
+   of component c (penalties.c:150-522): kind 0 the optical albedo (R, or
+   Hapke's w), 1 Hapke's theta, 2 the radar R, 3 the radar C; NULL where the
+   law carries none. */
+static const double *o_law_value(struct mod_t *mod, int kind, int ilaw, int c, int f)
+{
+  if (kind >= 2) {
+    union radscat_t *r = &mod->photo.radar[ilaw];
+    const struct RC_t *q = mod->photo.radtype[ilaw] == HARMCOSINE_DIFF   ? &r->harmcosine.local[c][f]
+                           : mod->photo.radtype[ilaw] == INHOCOSINE_DIFF ? &r->inhocosine.local[c][f]
+                                                                         : NULL;
+    return q ? (kind == 2 ? &q->R.val : &q->C.val) : NULL;
+  }
+  union optscat_t *o = &mod->photo.optical[ilaw];
+  switch (mod->photo.opttype[ilaw]) {
+  case HARMLAMBERT: case HARMLOMMEL: return kind == 0 ? &o->harmR.local[c][f].R.val : NULL;
+  case INHOLAMBERT: case INHOLOMMEL: return kind == 0 ? &o->inhoR.local[c][f].R.val : NULL;
+  case HARMHAPKE: return kind == 0 ? &o->harmhapke.local[c][f].w.val : &o->harmhapke.local[c][f].theta.val;
+  case INHOHAPKE: return kind == 0 ? &o->inhohapke.local[c][f].w.val : &o->inhohapke.local[c][f].theta.val;
+  case HARMKAAS: return kind == 0 ? &o->harmkaas.local[c][f].R.val : NULL;
+  case INHOKAAS: return kind == 0 ? &o->inhokaas.local[c][f].R.val : NULL;
+  }
+  return NULL;
+}
+
+/* the "del" penalties: over sides whose facets are both active, the
+   length-weighted sum of |x - y| over that of |x + y| (penalties.c:150-217) */
+static double o_law_del(struct mod_t *mod, int kind)
+{
+  double a = 0.0, b = 0.0;
+  const int nlaws = kind >= 2 ? mod->photo.nradlaws : mod->photo.noptlaws;
+  for (int c = 0; c < mod->shape.ncomp; c++) {
+    struct vertices_t *r = &mod->shape.comp[c].real;
+    for (int s = 0; s < r->ns; s++) {
+      const int f1 = r->s[s].f[0], f2 = r->s[s].f[1];
+      if (!(r->f[f1].act && r->f[f2].act)) continue;
+      const double *x1 = r->v[r->s[s].v[0]].x, *x2 = r->v[r->s[s].v[1]].x;
+      const double len = sqrt((x1[0] - x2[0]) * (x1[0] - x2[0]) + (x1[1] - x2[1]) * (x1[1] - x2[1]) +
+                              (x1[2] - x2[2]) * (x1[2] - x2[2]));
+      for (int l = 0; l < nlaws; l++) {
+        const double *x = o_law_value(mod, kind, l, c, f1), *y = o_law_value(mod, kind, l, c, f2);
+        if (!x) continue;
+        a += len * fabs(*x + *y);
+        b += len * fabs(*x - *y);
+      }
+    }
+  }
+  if (a == 0.0) {
+    oracle_set_error("oracle_penalties: a 'del' penalty without a law it reads");
+    return 0.0;
+  }
+  return b / a;
+}
+
+/* the "var" penalties: the fractional variance over active facets,
+   ntot (sum x^2) / (sum x)^2 - 1, clamped at 0 (penalties.c:219-277).  The
+   reference's optalbvar does not count harmonic-Hapke facets in ntot; kept. */
+static double o_law_var(struct mod_t *mod, int kind, int albvar)
+{
+  double av = 0.0, av2 = 0.0;
+  int ntot = 0;
+  const int nlaws = kind >= 2 ? mod->photo.nradlaws : mod->photo.noptlaws;
+  for (int c = 0; c < mod->shape.ncomp; c++) {
+    struct vertices_t *r = &mod->shape.comp[c].real;
+    for (int f = 0; f < r->nf; f++) {
+      if (!r->f[f].act) continue;
+      for (int l = 0; l < nlaws; l++) {
+        const double *x = o_law_value(mod, kind, l, c, f);
+        if (!x) continue;
+        av += *x;
+        av2 += *x * *x;
+        if (!(albvar && mod->photo.opttype[l] == HARMHAPKE)) ++ntot;
+      }
+    }
+  }
+  if (ntot == 0) {
+    oracle_set_error("oracle_penalties: a 'var' penalty without a law it reads");
+    return 0.0;
+  }
+  const double pen = ntot * (av2 / (av * av)) - 1.0;
+  return pen < 0.0 ? 0.0 : pen;
+}
+
+/* log10 of a tabular rho, continued as a straight line below 1e-20 */
+static double o_logrho(double rho)
+{
+  return rho > 1.0e-20 ? log10(rho) : log10(1.0e-20) + (rho / 1.0e-20 - 1) / LN10;
+}
+
+/* noncosine (penalties.c:523-588): the mean squared residual of an
+   unweighted line fit of log10 rho against log10 cos(theta) over the table's
+   points below 90 deg, the 90 deg point adding its residual about the fit at
+   the point before it when it is the larger */
+static double o_noncosine(struct par_t *par, struct mod_t *mod)
+{
+  int ntot = 0;
+  double resid2sum = 0.0;
+  for (int l = 0; l < mod->photo.nradlaws; l++) {
+    if (mod->photo.radtype[l] != TABULARLAW) continue;
+    const struct tabular_t *t = &mod->photo.radar[l].tabular;
+    const int n = t->n;
+    ntot += n;
+    double ymean = 0.0;
+    for (int j = 0; j < n - 1; j++) {
+      par->pen.y_noncosine[j] = o_logrho(t->rho[j].val);
+      ymean += par->pen.y_noncosine[j];
+    }
+    ymean /= (n - 1);
+    double x2 = 0.0, y2 = 0.0, xy = 0.0;
+    for (int j = 0; j < n - 1; j++) {
+      const double x = par->pen.x_noncosine[j] - par->pen.xmean_noncosine, y = par->pen.y_noncosine[j] - ymean;
+      x2 += x * x;
+      y2 += y * y;
+      xy += x * y;
+    }
+    const double slope = xy / x2, intercept = ymean - slope * par->pen.xmean_noncosine;
+    resid2sum += y2 - xy * xy / x2;
+    if (t->rho[n - 1].val > t->rho[n - 2].val) {
+      const double y = o_logrho(t->rho[n - 1].val) - (slope * par->pen.x_noncosine[n - 2] + intercept);
+      resid2sum += y * y;
+    }
+  }
+  if (ntot == 0) {
+    oracle_set_error("oracle_penalties: noncosine without a tabular law");
+    return 0.0;
+  }
+  return O_MAX(resid2sum / ntot, 0.0);
+}
+
 /* one penalty's value (before its weight); sets the error for a type not restated */
-static double o_penalty(int type, struct mod_t *mod, struct dat_t *dat, double pm[3], double ap[3][3])
+static double o_penalty(int type, struct par_t *par, struct mod_t *mod, struct dat_t *dat, double pm[3],
+                        double ap[3][3])
 {
   double pen = 0.0;
   int ntot = 0;
   switch (type) {
+  case OPTALBDEL: return o_law_del(mod, 0);
+  case THETADEL: return o_law_del(mod, 1);
+  case RADALBDEL: return o_law_del(mod, 2);
+  case RAD_C_DEL: return o_law_del(mod, 3);
+  case OPTALBVAR: return o_law_var(mod, 0, 1);
+  case THETAVAR: return o_law_var(mod, 1, 0);
+  case RADALBVAR: return o_law_var(mod, 2, 0);
+  case RAD_C_VAR: return o_law_var(mod, 3, 0);
+  case NONCOSINE: return o_noncosine(par, mod);
   case NONSMOOTH:
     for (int c = 0; c < mod->shape.ncomp; c++) {
       struct vertices_t *r = &mod->shape.comp[c].real;
@@ -386,7 +525,7 @@ double oracle_penalties(struct par_t *par, struct mod_t *mod, struct dat_t *dat)
       oracle_diag_inertia(mod->shape.inertia, pm, ap);
       got_pa = 1;
     }
-    double pen = o_penalty(t, mod, dat, pm, ap);
+    double pen = o_penalty(t, par, mod, dat, pm, ap);
     par->pen.base[i] = par->pen.weight[i] >= 0.0 ? pen : 1.0 / O_MAX(pen, O_TINYPEN);
     sum += fabs(par->pen.weight[i]) * par->pen.base[i];
   }

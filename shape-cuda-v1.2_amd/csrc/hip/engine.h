@@ -179,6 +179,7 @@ struct EvalRec {
 constexpr int PHOTO_HOMOG = 0, PHOTO_HARM = 1, PHOTO_INHO = 2, PHOTO_TABLE = 3;
 struct PhotoJob {
   int kind, type, k, n;  // k values per facet; n facets (kinds 1, 2) or table length (3)
+  int radar;             // a radar law (else optical): the type numbers overlap
   int slot;              // geometry slot whose facet centers kind 1 reads
   long long data;        // absolute parameter offset of the law's data region (kinds 1-3)
   long long loc;         // offset of its resolved values in the local-value arena

@@ -158,6 +158,7 @@ int Engine::pack_model(int mi, struct par_t *par, struct mod_t *mod, struct dat_
     const int t = radar ? mod->photo.radtype[l] : mod->photo.opttype[l];
     L.pb_law = radar ? off_rad + PB_RAD_STRIDE * l : off_opt + PB_OPT_STRIDE * l;
     J.type = t;
+    J.radar = radar;
     auto bounds = [&](int j, double lo, double hi, int mode) { J.lo[j] = lo; J.hi[j] = hi; J.mode[j] = mode; };
     auto chk = [&](int slot, double lo, double hi, int mode) {
       bounds(J.nchk, lo, hi, mode);
@@ -1203,6 +1204,7 @@ static inline unsigned cdiv(long long a, long long b) { return (unsigned)((a + b
 int Engine::setup_penalties(struct par_t *par) {
   pen = PenDev{};
   pen_need = 0;
+  pen_law_need = 0;
   // the joint adjustments of vary_params are not part of the batched objective
   if (opt.penalties && (par->vary_delcor0 || par->vary_radalb || par->vary_optalb || par->vary_dopscale)) return -19;
   if (!opt.penalties || par->pen.n <= 0 || opt.views_only) return 0;
@@ -1212,7 +1214,8 @@ int Engine::setup_penalties(struct par_t *par) {
   for (int i = 0; i < n; i++) {
     type[i] = par->pen.type[i + 1];
     weight[i] = par->pen.weight[i + 1];
-    if (!pen_supported(type[i])) return -17;
+    if (!pen_supported(type[i]) && !pen_supported_laws(type[i])) return -17;
+    if (pen_supported_laws(type[i])) pen_law_need |= 1 << pen_law_slot(type[i]);
     // rdev needs a VERTEX component (penalties.c:725-726 bails out otherwise)
     if (type[i] == RDEV)
       for (auto &P : g_packs)
@@ -1236,6 +1239,38 @@ int Engine::setup_penalties(struct par_t *par) {
   if ((rc = alloc((void **)&d_mom, sizeof(double) * MOM_N * (size_t)nslots))) return rc;
   if ((rc = alloc((void **)&d_mom_part, sizeof(double) * std::max(MP1_N, MP2_N) * (size_t)nslots * pen_nb))) return rc;
   pen.mom = d_mom;
+  if (pen_law_need) {
+    // every model must carry a law the requested penalties read (the
+    // reference bails out otherwise); noncosine's abscissae from par, set up
+    // from the first tabular law (every tabular law must have its length)
+    noncos_n = 0;
+    for (const auto &J : h_photo_jobs)
+      if (J.radar && J.kind == PHOTO_TABLE && !noncos_n) noncos_n = J.n - 1;
+    for (int e = 0; e < nevals; e++) {
+      int have = 0;
+      for (int q = 0; q < h_evals[e].photo_njobs; q++) {
+        const PhotoJob &J = h_photo_jobs[h_evals[e].photo_job0 + q];
+        const bool loc = J.kind == PHOTO_HARM || J.kind == PHOTO_INHO;
+        if (loc && !J.radar) have |= 1 << 0;
+        if (loc && !J.radar && J.type != HARMHAPKE) have |= 1 << 1;  // optalbvar counts no harmonic-Hapke facet
+        if (loc && !J.radar && (J.type == HARMHAPKE || J.type == INHOHAPKE)) have |= (1 << 2) | (1 << 3);
+        if (loc && J.radar) have |= (1 << 4) | (1 << 5) | (1 << 6) | (1 << 7);
+        if (J.radar && J.kind == PHOTO_TABLE) {
+          have |= 1 << 8;
+          if (!par->pen.x_noncosine || J.n != noncos_n + 1) return -17;
+        }
+      }
+      if ((pen_law_need & have) != pen_law_need) return -17;
+    }
+    if (pen_law_need >> 8 & 1) {
+      std::vector<double> xc(noncos_n + 1);
+      for (int i = 0; i < noncos_n; i++) xc[i] = par->pen.x_noncosine[i];
+      xc[noncos_n] = par->pen.xmean_noncosine;
+      if ((rc = upload(this, &d_noncos, xc))) return rc;
+    }
+    if ((rc = alloc((void **)&d_pen_law, sizeof(double) * PEN_LAW_N * (size_t)std::max(1, nevals)))) return rc;
+    pen.law = d_pen_law;
+  }
   return 0;
 }
 
@@ -1251,6 +1286,9 @@ void Engine::enqueue_penalties() {
     SHP_LAUNCH(*this, "k_pen_vert_fin", k_pen_vert_fin, dim3(nslots), dim3(64), 0, stream, d_slots, d_mom_part, pen_nb, d_mom,
                pen_need);
   }
+  if (pen_law_need)
+    SHP_LAUNCH(*this, "k_pen_laws", k_pen_laws, dim3(std::max(1, nevals)), dim3(256), 0, stream, d_evals, nevals,
+               d_eval_slot, d_slots, d_photo_jobs, d_lawloc, d_noncos, pen_law_need, d_pen_law);
   SHP_LAUNCH(*this, "k_pen_eval", k_pen_eval, dim3(std::max(1, nevals)), dim3(64), 0, stream, d_evals, nevals, d_eval_slot,
              d_params, pen);
 }

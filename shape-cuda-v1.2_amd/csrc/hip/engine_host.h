@@ -153,6 +153,9 @@ class Engine {
   double *d_job_log = nullptr;
   double *d_lawloc = nullptr;
   long long lawloc_len = 0;
+  // the law-reading penalties: requested slots, noncosine's abscissae, values
+  int pen_law_need = 0, noncos_n = 0;
+  double *d_noncos = nullptr, *d_pen_law = nullptr;
   double *d_params = nullptr, *d_fit = nullptr, *d_obs = nullptr, *d_oov = nullptr;
   double *d_overflow = nullptr, *d_view_out = nullptr;
   double *d_lc_x = nullptr, *d_lc_t = nullptr, *d_lc_y = nullptr, *d_lc_y2 = nullptr;

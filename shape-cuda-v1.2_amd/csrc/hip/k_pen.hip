@@ -155,9 +155,12 @@ int pen_supported(int type) {
   return (type >= NONSMOOTH && type <= MAXELLIPDEV) || type == BIFURCATION || type == IMPULSE;
 }
 
+// the law-reading penalties, evaluated by the batch (k_pen_laws) only
+int pen_supported_laws(int type) { return (type >= OPTALBDEL && type <= RAD_C_VAR) || type == NONCOSINE; }
+
 int pen_needs(int type) {
   switch (type) {
-  case NONSMOOTH: case CONCAVITY: return PEN_NEED_SIDES;
+  case NONSMOOTH: case CONCAVITY: case OPTALBDEL: case THETADEL: case RADALBDEL: case RAD_C_DEL: return PEN_NEED_SIDES;
   case RDEV: case MAXRDEV: return PEN_NEED_VERTS;
   case MAXELLIPDEV: case BIFURCATION: return PEN_NEED_VERTS | PEN_NEED_PA;
   case INERTIADEV_UNI: case PA3TILT: case NONPA_UNI: case FLATTENING: return PEN_NEED_PA;
@@ -475,6 +478,10 @@ __global__ void k_pen_eval(const EvalRec *evals, int nevals, const int *eval_slo
       pen *= R2D_D * R2D_D / (3 * nimp);
       break;
     }
+    case OPTALBDEL: case OPTALBVAR: case THETAVAR: case THETADEL: case RADALBDEL: case RADALBVAR: case RAD_C_DEL:
+    case RAD_C_VAR: case NONCOSINE:
+      pen = P.law[(size_t)e * PEN_LAW_N + pen_law_slot(P.type[i])];
+      break;
     default: break;  // rejected when the penalty list is packed
     }
     const double base = P.weight[i] >= 0.0 ? pen : 1.0 / dmax(pen, TINYPEN_D);
@@ -482,6 +489,135 @@ __global__ void k_pen_eval(const EvalRec *evals, int nevals, const int *eval_slo
     sum += fabs(P.weight[i]) * base;
   }
   P.total[e] = sum;
+}
+
+// ---------------------------------------------------------------------------
+// The law-reading penalties (penalties.c:150-588) from the evaluation's
+// resolved per-facet values (k_realize_photo): one workgroup per evaluation.
+// kind 0 the optical albedo (R, or Hapke's w), 1 Hapke's theta, 2 the radar
+// R, 3 the radar C.
+// ---------------------------------------------------------------------------
+__device__ inline int law_value_index(const PhotoJob &J, int kind) {
+  if (J.kind != PHOTO_HARM && J.kind != PHOTO_INHO) return -1;
+  if (kind >= 2) return J.radar ? kind - 2 : -1;
+  if (J.radar) return -1;
+  const bool hapke = J.type == HARMHAPKE || J.type == INHOHAPKE;
+  if (kind == 1) return hapke ? 4 : -1;
+  return 0;  // R or w
+}
+
+__device__ inline void blk_sum2(double &a, double &b, double *sh) {
+  sh[threadIdx.x] = a;
+  sh[256 + threadIdx.x] = b;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) {
+      sh[threadIdx.x] += sh[threadIdx.x + o];
+      sh[256 + threadIdx.x] += sh[256 + threadIdx.x + o];
+    }
+    __syncthreads();
+  }
+  a = sh[0];
+  b = sh[256];
+  __syncthreads();
+}
+
+__device__ inline double logrho(double rho) {
+  return rho > 1.0e-20 ? log10(rho) : log10(1.0e-20) + (rho / 1.0e-20 - 1) / LN10;
+}
+
+// need: bit pen_law_slot(type) per requested type; noncos: the table angles'
+// log10 cos (n - 1 of them) then their mean
+__global__ __launch_bounds__(256) void k_pen_laws(const EvalRec *evals, int nevals, const int *eval_slot,
+                                                  const GeomSlot *slots, const PhotoJob *jobs, const double *lawloc,
+                                                  const double *noncos, int need, double *out) {
+  const int e = blockIdx.x;
+  if (e >= nevals) return;
+  const EvalRec &E = evals[e];
+  const GeomSlot &G = slots[eval_slot[e]];
+  __shared__ double sh[512];
+  __shared__ int shn[256];
+  for (int q = 0; q < 8; q++) {
+    if (!(need >> q & 1)) continue;
+    const int type = q + 1, kind = (type == OPTALBDEL || type == OPTALBVAR) ? 0
+                                   : (type == THETADEL || type == THETAVAR) ? 1
+                                   : (type == RADALBDEL || type == RADALBVAR) ? 2 : 3;
+    const bool del = type == OPTALBDEL || type == THETADEL || type == RADALBDEL || type == RAD_C_DEL;
+    double a = 0.0, b = 0.0;
+    int ntot = 0;
+    for (int jq = 0; jq < E.photo_njobs; jq++) {
+      const PhotoJob &J = jobs[E.photo_job0 + jq];
+      const int j = law_value_index(J, kind);
+      if (j < 0) continue;
+      const double *loc = lawloc + J.loc;
+      const int k = J.k;
+      if (del) {
+        for (int s = threadIdx.x; s < G.ns; s += blockDim.x) {
+          const int4 sd = G.sides[s];
+          const double *x1 = G.x + 3 * (long long)sd.x, *x2 = G.x + 3 * (long long)sd.y;
+          const double len = sqrt((x1[0] - x2[0]) * (x1[0] - x2[0]) + (x1[1] - x2[1]) * (x1[1] - x2[1]) +
+                                  (x1[2] - x2[2]) * (x1[2] - x2[2]));
+          const double x = loc[(long long)k * sd.z + j], y = loc[(long long)k * sd.w + j];
+          a += len * fabs(x + y);
+          b += len * fabs(x - y);
+        }
+      } else {
+        // the reference's optalbvar does not count harmonic-Hapke facets
+        const bool count = !(type == OPTALBVAR && J.type == HARMHAPKE);
+        for (int f = threadIdx.x; f < J.n; f += blockDim.x) {
+          const double x = loc[(long long)k * f + j];
+          a += x;
+          b += x * x;
+          if (count) ntot++;
+        }
+      }
+    }
+    blk_sum2(a, b, sh);
+    shn[threadIdx.x] = ntot;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+      if ((int)threadIdx.x < o) shn[threadIdx.x] += shn[threadIdx.x + o];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+      double pen;
+      if (del) pen = b / a;
+      else {
+        pen = shn[0] * (b / (a * a)) - 1.0;
+        if (pen < 0.0) pen = 0.0;
+      }
+      out[(size_t)e * PEN_LAW_N + q] = pen;
+    }
+    __syncthreads();
+  }
+  if ((need >> 8 & 1) && threadIdx.x == 0) {
+    int ntot = 0;
+    double resid2sum = 0.0;
+    for (int jq = 0; jq < E.photo_njobs; jq++) {
+      const PhotoJob &J = jobs[E.photo_job0 + jq];
+      if (!(J.radar && J.kind == PHOTO_TABLE)) continue;
+      const double *rho = lawloc + J.loc, xmean = noncos[J.n - 1];
+      const int n = J.n;
+      ntot += n;
+      double ymean = 0.0;
+      for (int i = 0; i < n - 1; i++) ymean += logrho(rho[i]);
+      ymean /= (n - 1);
+      double x2 = 0.0, y2 = 0.0, xy = 0.0;
+      for (int i = 0; i < n - 1; i++) {
+        const double x = noncos[i] - xmean, y = logrho(rho[i]) - ymean;
+        x2 += x * x;
+        y2 += y * y;
+        xy += x * y;
+      }
+      const double slope = xy / x2, intercept = ymean - slope * xmean;
+      resid2sum += y2 - xy * xy / x2;
+      if (rho[n - 1] > rho[n - 2]) {
+        const double y = logrho(rho[n - 1]) - (slope * noncos[n - 2] + intercept);
+        resid2sum += y * y;
+      }
+    }
+    out[(size_t)e * PEN_LAW_N + 8] = dmax(resid2sum / ntot, 0.0);
+  }
 }
 
 }  // namespace shp

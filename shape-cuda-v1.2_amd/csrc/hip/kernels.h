@@ -81,7 +81,12 @@ struct PenDev {
   double *base;          // nevals x n: par->pen.base of each evaluation
   double *total;         // nevals: the penalty sum added to chi2/dof
   const double *mom;     // nslots x MOM_N
+  const double *law;     // nevals x PEN_LAW_N: the law-reading penalties (k_pen_laws), or NULL
 };
+// the law-reading penalties' slots in PenDev::law: type - 1 for OPTALBDEL ..
+// RAD_C_VAR, then NONCOSINE
+constexpr int PEN_LAW_N = 9;
+__host__ __device__ inline int pen_law_slot(int type) { return type == NONCOSINE ? 8 : type - 1; }
 
 // which penalty types the kernels evaluate, and what each needs
 int pen_supported(int type);
@@ -93,6 +98,9 @@ __global__ void k_moments_fin(const GeomSlot *slots, const double *params, const
 __global__ void k_pen_vert_part(const GeomSlot *slots, const double *mom, double *part, int nb, int need);
 __global__ void k_pen_vert_fin(const GeomSlot *slots, const double *part, int nb, double *mom, int need);
 __global__ void k_pen_eval(const EvalRec *evals, int nevals, const int *eval_slot, const double *params, PenDev pen);
+int pen_supported_laws(int type);
+__global__ void k_pen_laws(const EvalRec *evals, int nevals, const int *eval_slot, const GeomSlot *slots,
+                           const PhotoJob *jobs, const double *lawloc, const double *noncos, int need, double *out);
 
 // ---- FP64 kernels (k_fp64.hip) ----
 __global__ void k_setup_views(ViewRec *views, int nviews, const double *params);

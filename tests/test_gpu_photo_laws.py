@@ -166,3 +166,47 @@ def test_batch_objective_matches_oracle(tmp_path, monkeypatch, case, method):
     batch.close()
     job.close()
     ref.close()
+
+
+LAW_PENALTIES = {
+    "inhocosine+harmlommel": [("optalbdel", 0.3), ("optalbvar", 0.2), ("radalbdel", 0.1), ("radalbvar", 0.15),
+                              ("rad_c_del", 0.05), ("rad_c_var", 0.07), ("nonsmooth", 0.5)],
+    "tabular+harmhapke": [("thetadel", 0.2), ("thetavar", 0.1), ("optalbdel", 0.1), ("noncosine", 0.3),
+                          ("volume", 0.25)],
+}
+
+
+@pytest.mark.parametrize("method", [sh.METHOD_FP64, sh.METHOD_MIXED])
+@pytest.mark.parametrize("case", sorted(LAW_PENALTIES))
+def test_batch_law_penalties_match_oracle(tmp_path, monkeypatch, case, method):
+    """the penalties that read the laws' per-facet values and tables, in the
+    batched objective, against the oracle's objective"""
+    monkeypatch.chdir(tmp_path)
+    if case == "tabular+harmhapke":
+        rad, opt = CASES["tabular+hapke"][0], _harm_hapke()
+        trials = [[], [(0, 1.02)], [(("law", 1, 0, 3, -1), 0.35)], [(("law", 0, 0, 1, -1), 0.08)]]
+    else:
+        rad, opt = CASES[case][0], CASES[case][1]
+        trials = [[], [(0, 1.02)], [(("law", 1, 0, 0, 101), 0.3)], [(("law", 0, 0, 1, -1), -0.04)]]
+    files = TI.write_job(".", radar_law=rad, optical_law=opt, penalties=LAW_PENALTIES[case])[0]
+    orc = _orc()
+    ref = sh.Job(*files)
+    want = []
+    for t in trials:
+        saved = [(i, ref.get_param(i)) for i, _ in t]
+        for i, v in t:
+            ref.set_param(i, v)
+        want.append(_oracle_chain(orc, ref, objective=True))
+        for i, v in saved:
+            ref.set_param(i, v)
+    assert not orc.oracle_last_error()
+    job = sh.Job(*files)
+    batch = sh.Batch([job], [(0, t) for t in trials], method=method)
+    obj, _ = batch.run()
+    print(case, method, "oracle", want, "device", list(obj))
+    tol = 1e-11 if method == sh.METHOD_FP64 else 1e-6
+    for w, g in zip(want, obj):
+        assert abs(g - w) <= tol * abs(w), (w, g)
+    batch.close()
+    job.close()
+    ref.close()

@@ -176,7 +176,8 @@ def config1_mesh():
 
 
 def write_job(d, data=None, fmts=("binary", "rdf", "ascii.gz"), ncalc=-1, mask_rows=(40, 61), mask_cols=(25, 56),
-              radar_law="cosine c 0.2 f 1.5", optical_law="kaasalainen f 0.3 c 0.1 c 0.0 c 0.1 c 0.0"):
+              radar_law="cosine c 0.2 f 1.5", optical_law="kaasalainen f 0.3 c 0.1 c 0.0 c 0.1 c 0.0",
+              penalties=(("nonsmooth", 0.5), ("volume", 0.25))):
     """A job with two delay-Doppler frames (the second masked), one Doppler
     set of two smeared views and one lightcurve, as files under d (a path
     relative to the working directory: the structs hold file names in fields
@@ -188,7 +189,7 @@ def write_job(d, data=None, fmts=("binary", "rdf", "ascii.gz"), ncalc=-1, mask_r
             "omega": [("c", 0.0), ("c", 0.0), ("f", 1440.0)]}
     SF.write_mod(os.path.join(d, "job.mod"), [SF.vertex_component(u, r, fac)],
                  [radar_law], [optical_law], spin)
-    SF.write_par(os.path.join(d, "job.par"), PAR, [("nonsmooth", 0.5), ("volume", 0.25)])
+    SF.write_par(os.path.join(d, "job.par"), PAR, list(penalties))
     eph = [(T0 - 1, RA, DEC, DIST), (T0 + 2, RA, DEC, DIST)]
     os.makedirs(os.path.join(d, "dd"), exist_ok=True)
     os.makedirs(os.path.join(d, "dop"), exist_ok=True)

@@ -264,3 +264,74 @@ def test_laws_reduce_to_their_homogeneous_forms(tmp_path, monkeypatch):
     # and a law that differs does move chi2
     moved = _chain(TI.write_job(".", radar_law=inho, optical_law=harm_text("harmlambert", [(1, [0.3, 0.05, 0, 0])]))[0])
     assert abs(moved - base) > 1e-6 * base
+
+
+# ---- the penalties that read the laws (penalties.c:150-588) -----------------
+def _pen(files, penalties):
+    import test_oracle_pen as P
+    orc = P._oracle()
+    _orc()
+    job = sh.Job(*files)
+    orc.oracle_realize_mod(job.par, job.mod)
+    orc.oracle_realize_photo(job.par, job.mod)
+    pen = orc.oracle_penalties(job.par, job.mod, job.dat)
+    err = orc.oracle_last_error()
+    geo = job.geometry()
+    fac = np.zeros(3 * NF, dtype=np.int32)
+    job.lib.shp_facets(job.mod, fac.ctypes.data_as(C.POINTER(C.c_int)))
+    job.close()
+    return pen, err, geo[0], fac.reshape(NF, 3)
+
+
+def _edges(x, fac):
+    """each edge once: (length, facet a, facet b), from the facet list alone"""
+    seen = {}
+    for f, (a, b, c) in enumerate(fac):
+        for u, v in ((a, b), (b, c), (c, a)):
+            seen.setdefault((min(u, v), max(u, v)), []).append(f)
+    return [(np.linalg.norm(x[u] - x[v]), fs[0], fs[1]) for (u, v), fs in seen.items()]
+
+
+def test_albedo_variance_and_contrast_penalties(tmp_path, monkeypatch):
+    monkeypatch.chdir(tmp_path)
+    two = [[("f", 1.0 if f % 2 else 3.0)] for f in range(NF)]
+    files = TI.write_job(".", optical_law=inho_text("inholambert", [("c", 2.0)], two), penalties=[("optalbvar", 1.0)])[0]
+    pen, err, _, _ = _pen(files, None)
+    assert not err
+    assert pen == pytest.approx(0.25, rel=1e-12)          # variance 1 over mean 2, squared
+    one = [[("f", 3.0 if f == 0 else 1.0)] for f in range(NF)]
+    files = TI.write_job(".", optical_law=inho_text("inholommel", [("c", 1.0)], one), penalties=[("optalbdel", 1.0)])[0]
+    pen, err, x, fac = _pen(files, None)
+    assert not err
+    val = np.where(np.arange(NF) == 0, 3.0, 1.0)
+    e = _edges(x, fac)
+    want = sum(l * abs(val[a] - val[b]) for l, a, b in e) / sum(l * abs(val[a] + val[b]) for l, a, b in e)
+    assert pen == pytest.approx(want, rel=1e-12) and pen > 0
+    flat = [[("=", 0.0), ("f", 1.5)] for f in range(NF)]
+    files = TI.write_job(".", radar_law=inho_text("inhocosine", [("c", 0.2), ("c", 1.5)], flat),
+                         penalties=[("radalbdel", 1.0), ("rad_c_var", 1.0), ("radalbvar", 1.0)])[0]
+    pen, err, _, _ = _pen(files, None)
+    assert not err and pen == pytest.approx(0.0, abs=1e-12)   # uniform R and C: no contrast, no variance
+    #  (to the rounding of ntot sum(x^2) / (sum x)^2 - 1)
+
+
+def test_noncosine_penalty(tmp_path, monkeypatch):
+    monkeypatch.chdir(tmp_path)
+    n, A, p = 7, 0.6, 2.4
+    th = np.arange(n) * (math.pi / 2) / (n - 1)
+    rho = A * np.cos(th[:-1]) ** p
+    def run(vals):
+        text = "tabular %d %s" % (n, " ".join("c %.17g" % v for v in vals))
+        files = TI.write_job(".", radar_law=text, penalties=[("noncosine", 1.0)])[0]
+        pen, err, _, _ = _pen(files, None)
+        assert not err
+        return pen
+    assert run(list(rho) + [0.0]) == pytest.approx(0.0, abs=1e-28)       # a cosine power law fits exactly
+    last = 0.5                                                            # the 90-degree point above its neighbour
+    y = math.log10(last) - (p * math.log10(math.cos(th[-2])) + math.log10(A))
+    assert run(list(rho) + [last]) == pytest.approx(y * y / n, rel=1e-10)
+    bent = rho.copy()
+    bent[2] *= 1.3
+    lx, ly = np.log10(np.cos(th[:-1])), np.log10(bent)
+    res = ly - np.polyval(np.polyfit(lx, ly, 1), lx)
+    assert run(list(bent) + [0.0]) == pytest.approx(np.sum(res ** 2) / n, rel=1e-9)
