@@ -879,7 +879,8 @@ extern "C" void realize_mod(struct par_t *par, struct mod_t *mod) {
 // ---------------------------------------------------------------------------
 namespace {
 struct PenBufs {
-  GrowBuf fv, x, fn, ar, sides, u, r, slot, given, pb, type, weight, base, total, mom, part, ev, evslot;
+  GrowBuf fv, x, fn, ar, sides, u, r, slot, given, pb, type, weight, base, total, mom, part, ev, evslot, jobs, loc,
+      noncos, law;
 };
 std::mutex g_pen_mu;
 PenBufs g_pb;
@@ -894,15 +895,68 @@ extern "C" double penalties(struct par_t *par, struct mod_t *mod, struct dat_t *
   const int nv = rv->nv, nf = rv->nf, ns = rv->s ? rv->ns : 0;
   std::vector<int> type(n);
   std::vector<double> weight(n);
-  int need = 0;
+  int need = 0, law_need = 0;
   for (int i = 0; i < n; i++) {
     type[i] = par->pen.type[i + 1];
     weight[i] = par->pen.weight[i + 1];
-    if (!pen_supported(type[i]) || (type[i] == RDEV && comp->type != VERTEX)) {
+    if ((!pen_supported(type[i]) && !pen_supported_laws(type[i])) || (type[i] == RDEV && comp->type != VERTEX)) {
       set_err(-17, "penalties: type outside this path");
       return 0.0;
     }
     need |= pen_needs(type[i]);
+    if (pen_supported_laws(type[i])) law_need |= 1 << pen_law_slot(type[i]);
+  }
+  // the law-reading penalties take the per-facet values and tables the
+  // caller's realize_photo left (component 0), one job per law
+  std::vector<PhotoJob> jobs;
+  std::vector<double> lawloc, noncos;
+  if (law_need) {
+    int have = 0, nc = 0;
+    auto add = [&](int radar, int t, int k, int kind, const struct param_t *first, size_t elem, int cnt) {
+      PhotoJob J;
+      memset(&J, 0, sizeof(J));
+      J.kind = kind; J.type = t; J.k = k; J.n = cnt; J.radar = radar; J.loc = (long long)lawloc.size();
+      for (int f = 0; f < cnt; f++) {
+        const struct param_t *q = (const struct param_t *)((const char *)first + elem * f);
+        for (int j = 0; j < k; j++) lawloc.push_back(q[j].val);
+      }
+      jobs.push_back(J);
+    };
+    for (int l = 0; l < mod->photo.noptlaws; l++) {
+      union optscat_t *o = &mod->photo.optical[l];
+      const int t = mod->photo.opttype[l];
+      const int kind = (t == HARMLAMBERT || t == HARMLOMMEL || t == HARMHAPKE || t == HARMKAAS) ? PHOTO_HARM : PHOTO_INHO;
+      switch (t) {
+      case HARMLAMBERT: case HARMLOMMEL: add(0, t, 1, kind, &o->harmR.local[0][0].R, sizeof(struct R_t), nf); break;
+      case INHOLAMBERT: case INHOLOMMEL: add(0, t, 1, kind, &o->inhoR.local[0][0].R, sizeof(struct R_t), nf); break;
+      case HARMHAPKE: add(0, t, 5, kind, &o->harmhapke.local[0][0].w, sizeof(struct hapke_t), nf); break;
+      case INHOHAPKE: add(0, t, 5, kind, &o->inhohapke.local[0][0].w, sizeof(struct hapke_t), nf); break;
+      case HARMKAAS: add(0, t, 5, kind, &o->harmkaas.local[0][0].R, sizeof(struct kaas_t), nf); break;
+      case INHOKAAS: add(0, t, 5, kind, &o->inhokaas.local[0][0].R, sizeof(struct kaas_t), nf); break;
+      default: continue;
+      }
+      have |= 1 << 0;
+      if (t != HARMHAPKE) have |= 1 << 1;  // optalbvar counts no harmonic-Hapke facet
+      if (t == HARMHAPKE || t == INHOHAPKE) have |= (1 << 2) | (1 << 3);
+    }
+    for (int l = 0; l < mod->photo.nradlaws; l++) {
+      union radscat_t *r = &mod->photo.radar[l];
+      const int t = mod->photo.radtype[l];
+      if (t == HARMCOSINE_DIFF) add(1, t, 2, PHOTO_HARM, &r->harmcosine.local[0][0].R, sizeof(struct RC_t), nf);
+      else if (t == INHOCOSINE_DIFF) add(1, t, 2, PHOTO_INHO, &r->inhocosine.local[0][0].R, sizeof(struct RC_t), nf);
+      else if (t == TABULARLAW) {
+        add(1, t, 1, PHOTO_TABLE, r->tabular.rho, sizeof(struct param_t), r->tabular.n);
+        if (!nc) nc = r->tabular.n - 1;
+        if (par->pen.x_noncosine && r->tabular.n == nc + 1) have |= 1 << 8;
+        continue;
+      } else continue;
+      have |= (1 << 4) | (1 << 5) | (1 << 6) | (1 << 7);
+    }
+    if ((law_need & have) != law_need) { set_err(-17, "penalties: no law the penalty reads"); return 0.0; }
+    if (law_need >> 8 & 1) {
+      for (int i = 0; i < nc; i++) noncos.push_back(par->pen.x_noncosine[i]);
+      noncos.push_back(par->pen.xmean_noncosine);
+    }
   }
   if ((need & PEN_NEED_SIDES) && ns <= 0) { set_err(-18, "penalties: model has no side list"); return 0.0; }
   // host inputs: corners, coordinates, facet normals, sides, directions, deviations
@@ -942,6 +996,8 @@ extern "C" double penalties(struct par_t *par, struct mod_t *mod, struct dat_t *
   E.pb = 0;
   E.pen_set_off = PB_SPIN_END;
   E.pen_nsets = nsets;
+  E.photo_job0 = 0;
+  E.photo_njobs = (int)jobs.size();
   const int nb = moment_blocks(nf), zero = 0;
   std::lock_guard<std::mutex> lk(g_pen_mu);
   PenBufs &B = g_pb;
@@ -951,8 +1007,21 @@ extern "C" double penalties(struct par_t *par, struct mod_t *mod, struct dat_t *
            B.given.get(sizeof(given)) | B.pb.get(sizeof(double) * pb.size()) | B.type.get(sizeof(int) * n) |
            B.weight.get(sizeof(double) * n) | B.base.get(sizeof(double) * n) | B.total.get(sizeof(double)) |
            B.mom.get(sizeof(double) * MOM_N) | B.part.get(sizeof(double) * std::max(MP1_N, MP2_N) * nb) |
-           B.ev.get(sizeof(EvalRec)) | B.evslot.get(sizeof(int));
+           B.ev.get(sizeof(EvalRec)) | B.evslot.get(sizeof(int)) |
+           B.jobs.get(sizeof(PhotoJob) * std::max((size_t)1, jobs.size())) |
+           B.loc.get(sizeof(double) * std::max((size_t)1, lawloc.size())) |
+           B.noncos.get(sizeof(double) * std::max((size_t)1, noncos.size())) | B.law.get(sizeof(double) * PEN_LAW_N);
   if (rc) { set_err(rc, "penalties allocation"); return 0.0; }
+  if (law_need &&
+      (hipMemcpy(B.jobs.p, jobs.data(), sizeof(PhotoJob) * jobs.size(), hipMemcpyHostToDevice) != hipSuccess ||
+       (!lawloc.empty() &&
+        hipMemcpy(B.loc.p, lawloc.data(), sizeof(double) * lawloc.size(), hipMemcpyHostToDevice) != hipSuccess) ||
+       (!noncos.empty() &&
+        hipMemcpy(B.noncos.p, noncos.data(), sizeof(double) * noncos.size(), hipMemcpyHostToDevice) != hipSuccess))) {
+    (void)hipGetLastError();
+    set_err(-3, "penalties upload");
+    return 0.0;
+  }
   GeomSlot G;
   memset(&G, 0, sizeof(G));
   G.nv = nv; G.nf = nf; G.eval = -1; G.comp_type = comp->type;
@@ -987,7 +1056,13 @@ extern "C" double penalties(struct par_t *par, struct mod_t *mod, struct dat_t *
   }
   PenDev P;
   P.n = n; P.type = (const int *)B.type.p; P.weight = (const double *)B.weight.p; P.base = (double *)B.base.p;
-  P.total = (double *)B.total.p; P.mom = dmom;
+  P.total = (double *)B.total.p; P.mom = dmom; P.law = nullptr;
+  if (law_need) {
+    hipLaunchKernelGGL(k_pen_laws, dim3(1), dim3(256), 0, 0, (const EvalRec *)B.ev.p, 1, (const int *)B.evslot.p, ds,
+                       (const PhotoJob *)B.jobs.p, (const double *)B.loc.p, (const double *)B.noncos.p, law_need,
+                       (double *)B.law.p);
+    P.law = (const double *)B.law.p;
+  }
   hipLaunchKernelGGL(k_pen_eval, dim3(1), dim3(64), 0, 0, (const EvalRec *)B.ev.p, 1, (const int *)B.evslot.p,
                      (const double *)B.pb.p, P);
   std::vector<double> base(n);

@@ -210,3 +210,26 @@ def test_batch_law_penalties_match_oracle(tmp_path, monkeypatch, case, method):
     batch.close()
     job.close()
     ref.close()
+
+
+@pytest.mark.parametrize("case", sorted(LAW_PENALTIES))
+def test_penalties_dropin_reads_the_laws(tmp_path, monkeypatch, case):
+    """the drop-in penalties() with the law-reading types, after the caller's
+    realize_mod and realize_photo, against the oracle's penalties"""
+    monkeypatch.chdir(tmp_path)
+    rad, opt = (CASES["tabular+hapke"][0], _harm_hapke()) if case == "tabular+harmhapke" else CASES[case][:2]
+    files = TI.write_job(".", radar_law=rad, optical_law=opt, penalties=LAW_PENALTIES[case])[0]
+    hip, orc = H.load_hip(), _orc()
+    a, b = sh.Job(*files), sh.Job(*files)
+    for job in (a, b):
+        orc.oracle_realize_mod(job.par, job.mod)
+        orc.oracle_realize_photo(job.par, job.mod)
+    pa = orc.oracle_penalties(a.par, a.mod, a.dat)
+    assert not orc.oracle_last_error()
+    hip.shape_hip_clear_error()
+    pb = hip.penalties(b.par, b.mod, b.dat)
+    assert hip.shape_hip_last_error() == 0
+    print(case, "penalties oracle", pa, "device", pb)
+    assert abs(pb - pa) <= 1e-12 * abs(pa), (pa, pb)
+    a.close()
+    b.close()
