@@ -127,6 +127,8 @@ struct ViewRec {
   int D0, P0;
   float dfr, pfr, delfact32, ax32, ay32, const1_32, kmpp32, inv_kmpp32, dcrel32;
   float shade_err32;    // mixed method: FP32 shading error scale (shade_ill)
+  float limb32;         // mixed method: cos(scattering) below which the diffuse cosine law
+                        // goes to the FP64 replay (C < 1/2: c^(2C-1) grows without bound)
   // ---- accumulated by kernels ----
   int xlim[2], ylim[2], xlim2[2], ylim2[2];
   int outbnd, outbnd2, any_overflow, nstale, nclaimed;  // nclaimed: mixed z-buffer keys claimed

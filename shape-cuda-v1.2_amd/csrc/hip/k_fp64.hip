@@ -189,6 +189,10 @@ __global__ void k_setup_views(ViewRec *views, int nviews, const double *params) 
   // FP32 projected coordinates of the window carry ~3 ulps of n + 1 pixels;
   // times 2 (s and t), in pixels per km of facet edge
   V.shade_err32 = (float)(2.0 * 3.0 * 5.9604645e-8 * (V.n + 1) / V.kmpp);
+  V.limb32 = 0.0f;
+  if ((V.type == VT_DELAY || V.type == VT_DOPPLER) && V.law_type == COSINELAW_DIFF &&
+      params[V.pb + V.pb_law + 2] < 0.5)
+    V.limb32 = 0.05f;  // the trial's own C
   if (V.type == VT_DELAY || V.type == VT_DOPPLER) {
     double d0 = floor(V.delshift), p0 = floor(V.dopshift);
     V.D0 = (int)d0; V.P0 = (int)p0;

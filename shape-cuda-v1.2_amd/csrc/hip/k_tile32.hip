@@ -550,9 +550,10 @@ __device__ __forceinline__ void tile32_item(int vi, int b, int ng, const ViewRec
     } else {
       nz = facing32(V.oa32, G.fn32[f]);
     }
-    if (!(nz > 0.0f) || shade_ill(V, P, xa.w, spread)) {
+    if (!(nz > V.limb32) || shade_ill(V, P, xa.w, spread)) {
       // cos(scattering) of an owner facing away is inherited from an earlier
-      // claimant; an ill-conditioned FP32 cosine needs FP64: FP64 replay
+      // claimant; an ill-conditioned FP32 cosine needs FP64 (and so does a
+      // limb pixel under a cosine law with C < 1/2): FP64 replay
       tile_stale(V, vi, pd, ev_ok, a * BT + c, i, j, flag, spix, &nts);
       continue;
     }

@@ -556,7 +556,7 @@ __device__ __attribute__((noinline)) float radlaw32_steep(const ViewRec &V, cons
 // under the diffuse cosine law, else the law's value (NaN: FP64 replay).
 __device__ inline float radar_cl(const ViewRec &V, const GeomSlot &G, const double *pl, int f, float s, float t,
                                  float c) {
-  if (V.law_type == COSINELAW_DIFF) return c;
+  if (V.law_type == COSINELAW_DIFF) return c < V.limb32 ? __builtin_nanf("") : c;
   if (V.lawloc) return radlaw32_local(V, G, pl, f, s, t, c);
   return radlaw32_steep(V, G, pl, f, s, t, c);
 }

@@ -233,3 +233,28 @@ def test_penalties_dropin_reads_the_laws(tmp_path, monkeypatch, case):
     assert abs(pb - pa) <= 1e-12 * abs(pa), (pa, pb)
     a.close()
     b.close()
+
+
+@pytest.mark.parametrize("flags", [sh.BATCH_GRAPH | sh.BATCH_KEEP_FIT,
+                                   sh.BATCH_GRAPH | sh.BATCH_KEEP_FIT | sh.BATCH_GLOBAL_ZBUF])
+def test_mixed_cosine_law_below_one_half_at_the_limb(tmp_path, monkeypatch, flags):
+    """a diffuse cosine law with C < 1/2 grows without bound at the limb
+    (c^(2C-1)); the mixed method replays limb pixels in FP64, so its images
+    stay within 1e-5 of the FP64 batch's peak (binned and global kernels)"""
+    monkeypatch.chdir(tmp_path)
+    files = TI.write_job(".", radar_law="cosine c 0.2 f 0.3")[0]
+    fits = {}
+    for m in (sh.METHOD_FP64, sh.METHOD_MIXED):
+        job = sh.Job(*files)
+        b = sh.Batch([job], [(0, []), (0, [(("law", 1, 0, 1, -1), 0.45)])], method=m, flags=flags)
+        obj, _ = b.run()
+        fits[m] = (obj, [[b.fit(e, 0, (100, 80)), b.fit(e, 1, (100, 80)), b.fit(e, 2, (100,))] for e in range(2)])
+        b.close()
+        job.close()
+    (o1, f1), (o2, f2) = fits[sh.METHOD_FP64], fits[sh.METHOD_MIXED]
+    for e in range(2):
+        assert abs(o2[e] - o1[e]) <= 1e-6 * abs(o1[e]), (e, o1[e], o2[e])
+        for a, b in zip(f1[e], f2[e]):
+            err = np.abs(a - b).max() / np.abs(a).max()
+            print("eval", e, "image error", err)
+            assert err < 1e-5, err
