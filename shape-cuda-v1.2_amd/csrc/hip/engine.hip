@@ -903,10 +903,13 @@ int Engine::build(struct par_t *par, struct mod_t **mods, struct dat_t **dats, i
   SHP_CHECK(hipMemset(d_lawloc, 0, sizeof(double) * (size_t)std::max(1LL, lawloc_len)));
   for (int q = 0; q < nviews; q++)
     if (view_loc[q] >= 0) h_views[q].lawloc = d_lawloc + view_loc[q];
+  photo_nblk = 1;
+  for (const auto &J : h_photo_jobs)
+    if (J.kind == PHOTO_HARM || J.kind == PHOTO_INHO) photo_nblk = std::max(photo_nblk, (J.n + 255) / 256);
   if (!h_photo_jobs.empty()) {
     if ((rc = upload(this, &d_photo_jobs, h_photo_jobs))) return rc;
-    if ((rc = alloc((void **)&d_job_flag, sizeof(int) * h_photo_jobs.size()))) return rc;
-    if ((rc = alloc((void **)&d_job_log, sizeof(double) * h_photo_jobs.size()))) return rc;
+    if ((rc = alloc((void **)&d_job_flag, sizeof(int) * h_photo_jobs.size() * photo_nblk))) return rc;
+    if ((rc = alloc((void **)&d_job_log, sizeof(double) * h_photo_jobs.size() * photo_nblk))) return rc;
   }
   if ((rc = upload(this, &d_slots, h_slots))) return rc;
   if ((rc = upload(this, &d_views, h_views))) return rc;
@@ -1317,7 +1320,7 @@ int Engine::enqueue() {
     SHP_LAUNCH(*this, "k_realize_vnormals", k_realize_vnormals, dim3(cdiv(max_nv, T), nslots), dim3(T), 0, stream, d_slots, nslots);
   }
   if (photo)
-    SHP_LAUNCH(*this, "k_realize_photo", k_realize_photo, dim3((unsigned)h_photo_jobs.size()), dim3(T), 0, stream,
+    SHP_LAUNCH(*this, "k_realize_photo", k_realize_photo, dim3(photo_nblk, (unsigned)h_photo_jobs.size()), dim3(T), 0, stream,
                d_photo_jobs, d_params, d_slots, d_lawloc, d_job_flag, d_job_log);
   enqueue_penalties();
   if (opt.method == METHOD_MIXED) return enqueue_mixed(*this);
@@ -1350,7 +1353,7 @@ int Engine::enqueue() {
                      d_view_out, opt.keep_fit ? 0 : 1, 0, 0);
   SHP_LAUNCH(*this, "k_objective", k_objective, dim3(std::max(1, nevals)), dim3(256), 0, stream, d_evals, nevals, d_frames, d_views,
                      d_view_out, d_slot_flags, d_slot_logf, d_eval_slot, pen.n ? pen.total : nullptr, d_job_flag,
-                     d_job_log);
+                     d_job_log, photo_nblk);
   SHP_CHECK(hipGetLastError());
   return 0;
 }

@@ -153,6 +153,7 @@ class Engine {
   double *d_job_log = nullptr;
   double *d_lawloc = nullptr;
   long long lawloc_len = 0;
+  int photo_nblk = 1;  // workgroups per realize_photo job (256 facets each)
   // the law-reading penalties: requested slots, noncosine's abscissae, values
   int pen_law_need = 0, noncos_n = 0;
   double *d_noncos = nullptr, *d_pen_law = nullptr;

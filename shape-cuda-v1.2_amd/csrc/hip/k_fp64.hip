@@ -1292,7 +1292,7 @@ __global__ __launch_bounds__(256) void k_objective(EvalRec *evals, int nevals, c
                                                    const ViewRec *views, const double *view_out, const int *slot_flags,
                                                    const double *slot_logf, const int *eval_slot,
                                                    const double *pen_total, const int *job_flag,
-                                                   const double *job_log) {
+                                                   const double *job_log, int photo_nblk) {
   // one workgroup per evaluation: the threads fetch 256 frames' terms at a
   // time into LDS, one thread adds them in frame order (the CPU's order)
   const int e = blockIdx.x;
@@ -1347,9 +1347,9 @@ __global__ __launch_bounds__(256) void k_objective(EvalRec *evals, int nevals, c
   E.baddiam_logfactor = slot_logf[sl];
   E.badphoto = 0;
   E.badphoto_logfactor = 0.0;
-  for (int q = 0; q < E.photo_njobs; q++) {  // the jobs in law order
-    E.badphoto |= job_flag[E.photo_job0 + q];
-    E.badphoto_logfactor += job_log[E.photo_job0 + q];
+  for (int q = 0; q < E.photo_njobs * photo_nblk; q++) {  // the jobs in law order, their blocks in order
+    E.badphoto |= job_flag[E.photo_job0 * photo_nblk + q];
+    E.badphoto_logfactor += job_log[E.photo_job0 * photo_nblk + q];
   }
   double err = chi2 / E.dof;
   if (pen_total) err += pen_total[e];  // bestfit.c:807-811
@@ -1366,7 +1366,9 @@ namespace shp {
 
 // ---------------------------------------------------------------------------
 // realize_photo (realize_photo.c:122-969) as the objective calls it (albedo
-// factors 1): one workgroup per job (scattering law x evaluation)
+// factors 1): grid (facet blocks, jobs), one job per scattering law per
+// evaluation; each workgroup leaves its flag and log factor at
+// job * gridDim.x + block
 // ---------------------------------------------------------------------------
 
 // checkphotopar (realize_photo.c:965-1004): the log factor a value adds;
@@ -1426,27 +1428,28 @@ __device__ double photo_harm(const double *c, int L, double ct, double phi) {
 __global__ __launch_bounds__(256) void k_realize_photo(const PhotoJob *jobs, const double *params,
                                                        const GeomSlot *slots, double *lawloc, int *job_flag,
                                                        double *job_log) {
-  const PhotoJob &J = jobs[blockIdx.x];
+  const PhotoJob &J = jobs[blockIdx.y];
   const double *d = params + J.data;
   double *out = lawloc + J.loc;
   double lg = 0.0;
   int bad = 0;
   const int k = J.k;
+  const bool first = blockIdx.x == 0;  // the global values and tables: the job's first workgroup
   if (J.kind == PHOTO_HOMOG) {
-    if (threadIdx.x == 0)
+    if (first && threadIdx.x == 0)
       for (int q = 0; q < J.nchk; q++) lg += photo_check(params[J.chk[q]], J.lo[q], J.hi[q], J.mode[q], bad);
   } else if (J.kind == PHOTO_TABLE) {
-    if (threadIdx.x == 0)
+    if (first && threadIdx.x == 0)
       for (int i = 0; i < J.n; i++) {
         const double v = (i > 0 && d[J.n + i] != 0.0) ? out[i - 1] : d[i];
         out[i] = v;
         lg += photo_check(v, J.lo[0], J.hi[0], J.mode[0], bad);
       }
   } else {
-    if (J.kind == PHOTO_INHO && threadIdx.x == 0)
+    if (J.kind == PHOTO_INHO && first && threadIdx.x == 0)
       for (int j = 0; j < k; j++) lg += photo_check(d[j], J.lo[j], J.hi[j], J.mode[j], bad);
     const GeomSlot &G = slots[J.slot];
-    for (int f = threadIdx.x; f < J.n; f += blockDim.x) {
+    for (int f = blockIdx.x * blockDim.x + threadIdx.x; f < J.n; f += gridDim.x * blockDim.x) {
       double v[5];
       if (J.kind == PHOTO_HARM) {
         const double *x = G.fx + 3 * (long long)f;
@@ -1476,8 +1479,9 @@ __global__ __launch_bounds__(256) void k_realize_photo(const PhotoJob *jobs, con
     __syncthreads();
   }
   if (threadIdx.x == 0) {
-    job_flag[blockIdx.x] = s_bad[0];
-    job_log[blockIdx.x] = s_lg[0];
+    const int q = blockIdx.y * gridDim.x + blockIdx.x;
+    job_flag[q] = s_bad[0];
+    job_log[q] = s_lg[0];
   }
 }
 

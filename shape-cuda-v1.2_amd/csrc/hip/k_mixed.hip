@@ -412,7 +412,7 @@ int enqueue_mixed(Engine &E) {
   }
   SHP_LAUNCH(E, "k_objective", k_objective, dim3(std::max(1, E.nevals)), dim3(256), 0, st, E.d_evals, E.nevals, E.d_frames,
                      E.d_views, E.d_view_out, E.d_slot_flags, E.d_slot_logf, E.d_eval_slot, E.pen.n ? E.pen.total : nullptr,
-                     E.d_job_flag, E.d_job_log);
+                     E.d_job_flag, E.d_job_log, E.photo_nblk);
   return hipGetLastError() == hipSuccess ? 0 : -42;
 }
 

@@ -130,7 +130,7 @@ __global__ void k_chi2_frames64(FrameRec *frames, int nframes, double *fit, cons
 __global__ void k_objective(EvalRec *evals, int nevals, const FrameRec *frames, const ViewRec *views,
                             const double *view_out, const int *slot_flags, const double *slot_logf,
                             const int *eval_slot, const double *pen_total, const int *job_flag,
-                            const double *job_log);
+                            const double *job_log, int photo_nblk);
 __global__ void k_realize_photo(const PhotoJob *jobs, const double *params, const GeomSlot *slots, double *lawloc,
                                 int *job_flag, double *job_log);
 
